@@ -1,0 +1,337 @@
+// C-ABI layer (include/fiesta_hip.h): context lifetime, host<->device copies
+// and the rk_func method sequence, on one HIP stream per context.
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "fst_common.hpp"
+#include "fst_fused.hpp"
+#include "fst_kernels.hpp"
+
+using namespace fst;
+
+struct fst_ctx {
+  fst_config cf;
+  Geo g;
+  Phys ph;
+  Bufs b;
+  FusedBufs fb;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  int fused = 1;
+  double last_ms = 0.0;
+  int device = 0;
+};
+
+namespace {
+
+int fail(fst_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(ctx, FST_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+int check_launch(fst_ctx *ctx, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ctx, FST_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return FST_OK;
+}
+
+int derive(const fst_config &c, fst_dims *d, std::string *why) {
+  if (c.ndim != 2 && c.ndim != 3) { *why = "ndim must be 2 or 3"; return FST_ERR_ARG; }
+  if (c.ns < 1 || c.ns > FST_MAX_SPECIES) { *why = "ns out of range"; return FST_ERR_ARG; }
+  if (c.ng < 3) { *why = "ng must be >= 3 (WENO5 reaches 3 cells)"; return FST_ERR_ARG; }
+  if (c.nci < 1 || c.ncj < 1 || (c.ndim == 3 && c.nck < 1)) { *why = "empty grid"; return FST_ERR_ARG; }
+  if ((c.xPer && c.nci < c.ng) || (c.yPer && c.ncj < c.ng) || (c.ndim == 3 && c.zPer && c.nck < c.ng)) {
+    *why = "a periodic direction needs at least ng interior cells";
+    return FST_ERR_ARG;
+  }
+  if (c.grid != 0) { *why = "only the cartesian grid is supported"; return FST_ERR_ARG; }
+  if (c.scheme < 1 || c.scheme > 3 || (c.ndim == 3 && c.scheme != FST_WENO5)) {
+    *why = "advection scheme: 3D supports weno5; 2D weno5, centered4, quick";
+    return FST_ERR_ARG;
+  }
+  const int bcs[6] = {c.bcL, c.bcR, c.bcB, c.bcT, c.bcH, c.bcF};
+  for (int q = 0; q < 6; ++q)
+    if (bcs[q] < 0 || bcs[q] > 3) { *why = "unknown boundary type"; return FST_ERR_ARG; }
+  d->ngi = c.nci + 2 * c.ng;
+  d->ngj = c.ncj + 2 * c.ng;
+  d->ngk = c.ndim == 3 ? c.nck + 2 * c.ng : 1;
+  d->nv = c.ndim + 1 + c.ns;  // src/input.cpp:354-364
+  d->nvt = d->nv + (c.ceq ? 5 : 0);
+  d->nvx = c.ndim + 3;
+  d->ncells_ghost = (int64_t)d->ngi * d->ngj * d->ngk;
+  d->ncells = (int64_t)c.nci * c.ncj * (c.ndim == 3 ? c.nck : 1);
+  return FST_OK;
+}
+
+double *array_ptr(fst_ctx *ctx, int a, size_t *count) {
+  const size_t N = (size_t)ctx->g.N;
+  double *p = nullptr;
+  size_t n = 0;
+  switch (a) {
+    case FST_VAR: p = ctx->b.var; n = N * ctx->g.nvt; break;
+    case FST_TMP1: p = ctx->b.tmp1; n = N * ctx->g.nvt; break;
+    case FST_DVAR: p = ctx->b.dvar; n = N * ctx->g.nvt; break;
+    case FST_VARX: p = ctx->b.varx; n = N * ctx->g.nvx; break;
+    case FST_P: p = ctx->b.p; n = N; break;
+    case FST_RHO: p = ctx->b.rho; n = N; break;
+    case FST_T: p = ctx->b.T; n = N; break;
+    case FST_VEL: p = ctx->b.vel; n = N * ctx->g.ndim; break;
+    case FST_GRADRHO: p = ctx->b.gradRho; n = ctx->cf.ceq ? N * (ctx->g.ndim == 3 ? 5 : 4) : 0; break;
+    default: return nullptr;
+  }
+  *count = n;
+  return p;
+}
+
+void free_all(fst_ctx *ctx) {
+  double **ps[] = {&ctx->b.var, &ctx->b.tmp1, &ctx->b.dvar, &ctx->b.varx, &ctx->b.p, &ctx->b.rho, &ctx->b.T,
+                   &ctx->b.vel, &ctx->b.fx, &ctx->b.fy, &ctx->b.fz, &ctx->b.stress, &ctx->b.gradRho,
+                   &ctx->b.mflux, &ctx->b.grid, &ctx->b.metrics};
+  for (auto pp : ps)
+    if (*pp) { (void)hipFree(*pp); *pp = nullptr; }
+  if (ctx->b.sc) { (void)hipFree(ctx->b.sc); ctx->b.sc = nullptr; }
+  fused_free(ctx->fb);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fst_abi_version(void) { return FST_ABI_VERSION; }
+
+// Defaults of executeConfiguration, src/input.cpp:130-443.
+void fst_config_init(fst_config *c) {
+  std::memset(c, 0, sizeof(*c));
+  c->ndim = 2;
+  c->grid = 0;
+  c->ns = 1;
+  c->ng = 3;
+  c->nci = c->ncj = c->nck = 1;
+  c->dx = c->dy = c->dz = 1.0;
+  c->R = 8.314462;
+  c->scheme = FST_WENO5;
+  c->kap = 1.23456;
+  c->gAccel = 9.81;
+  c->rhoRef = 0.0;
+  c->gamma[0] = 1.4;
+  c->M[0] = 0.02896;
+}
+
+int fst_config_dims(const fst_config *cfg, fst_dims *out) {
+  if (!cfg || !out) return FST_ERR_ARG;
+  std::string why;
+  return derive(*cfg, out, &why);
+}
+
+int fst_create(const fst_config *cfg, fst_ctx **out) {
+  if (!cfg || !out) return FST_ERR_ARG;
+  *out = nullptr;
+  fst_dims d;
+  std::string why;
+  int rc = derive(*cfg, &d, &why);
+  if (rc) {
+    fprintf(stderr, "fst_create: %s\n", why.c_str());
+    return rc;
+  }
+  fst_ctx *ctx = new fst_ctx();
+  ctx->cf = *cfg;
+  if (cfg->ndim == 2) {  // 2D defaults, src/input.cpp:354-362
+    ctx->cf.dz = cfg->dx;
+    ctx->cf.nck = 1;
+    ctx->cf.zPer = 0;
+  }
+  const fst_config &c = ctx->cf;
+  Geo &g = ctx->g;
+  g.ngi = d.ngi; g.ngj = d.ngj; g.ngk = d.ngk; g.ng = c.ng;
+  g.nci = c.nci; g.ncj = c.ncj; g.nck = c.ndim == 3 ? c.nck : 1; g.ndim = c.ndim;
+  g.nv = d.nv; g.nvt = d.nvt; g.ns = c.ns; g.nvx = d.nvx;
+  g.N = d.ncells_ghost;
+  Phys &ph = ctx->ph;
+  std::memset(&ph, 0, sizeof(ph));
+  // device config array, src/cart3d.cpp:152-171 / src/cart2d.cpp:92-110
+  ph.cd[0] = c.ns; ph.cd[1] = c.dx; ph.cd[2] = c.dy; ph.cd[3] = c.dz; ph.cd[4] = d.nv; ph.cd[5] = c.ng;
+  for (int s = 0; s < c.ns; ++s) {
+    double gammas = c.gamma[s], Rs = c.R / c.M[s];
+    ph.cd[6 + 3 * s] = gammas;
+    ph.cd[6 + 3 * s + 1] = Rs;
+    ph.cd[6 + 3 * s + 2] = c.mu[s];
+    ph.cpf[s] = gammas * Rs / (gammas - 1);
+    ph.cvf[s] = Rs / (gammas - 1);
+    ph.mus[s] = c.mu[s];
+  }
+  ph.dx = c.dx; ph.dy = c.dy; ph.dz = c.dz;
+  ph.dxmag2 = c.dx * c.dx + c.dy * c.dy + c.dz * c.dz;
+  ph.kap = c.kap; ph.eps = c.eps; ph.alpha = c.alpha; ph.g = c.gAccel; ph.rhoRef = c.rhoRef; ph.dt = c.dt;
+
+  auto alloc = [&](double **p, size_t planes) -> bool {
+    if (planes == 0) return true;
+    size_t bytes = planes * (size_t)g.N * sizeof(double);
+    if (hipMalloc((void **)p, bytes) != hipSuccess) return false;
+    return hipMemset(*p, 0, bytes) == hipSuccess;
+  };
+  bool ok = hipGetDevice(&ctx->device) == hipSuccess;
+  ok = ok && hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && hipEventCreate(&ctx->ev0) == hipSuccess && hipEventCreate(&ctx->ev1) == hipSuccess;
+  ok = ok && alloc(&ctx->b.var, g.nvt) && alloc(&ctx->b.tmp1, g.nvt) && alloc(&ctx->b.dvar, g.nvt);
+  ok = ok && alloc(&ctx->b.varx, g.nvx) && alloc(&ctx->b.p, 1) && alloc(&ctx->b.rho, 1) && alloc(&ctx->b.T, 1);
+  ok = ok && alloc(&ctx->b.vel, g.ndim) && alloc(&ctx->b.fx, 1) && alloc(&ctx->b.fy, 1);
+  ok = ok && alloc(&ctx->b.fz, g.ndim == 3 ? 1 : 0);
+  ok = ok && alloc(&ctx->b.stress, c.visc ? (g.ndim == 3 ? 6 : 8) : 0);
+  ok = ok && alloc(&ctx->b.gradRho, c.ceq ? (g.ndim == 3 ? 5 : 4) : 0);
+  ok = ok && alloc(&ctx->b.mflux, c.ceq ? (g.ndim == 3 ? 27 : 8) : 0);
+  ok = ok && hipMalloc((void **)&ctx->b.sc, sizeof(Scalars)) == hipSuccess &&
+       hipMemset(ctx->b.sc, 0, sizeof(Scalars)) == hipSuccess;
+  if (!ok) {
+    fprintf(stderr, "fst_create: device allocation failed\n");
+    free_all(ctx);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FST_ERR_NOMEM;
+  }
+  ctx->fused = fused_supported(c) ? 1 : 0;
+  *out = ctx;
+  return FST_OK;
+}
+
+int fst_destroy(fst_ctx *ctx) {
+  if (!ctx) return FST_ERR_ARG;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_all(ctx);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return FST_OK;
+}
+
+const char *fst_last_error(const fst_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int fst_get_dims(const fst_ctx *ctx, fst_dims *out) {
+  if (!ctx || !out) return FST_ERR_ARG;
+  std::string why;
+  return derive(ctx->cf, out, &why);
+}
+
+int fst_upload(fst_ctx *ctx, int array, const double *host, size_t count) {
+  if (!ctx || !host) return FST_ERR_ARG;
+  size_t n = 0;
+  double *p = array_ptr(ctx, array, &n);
+  if (!p || n == 0) return fail(ctx, FST_ERR_ARG, "fst_upload: array not available in this configuration");
+  if (count != n) return fail(ctx, FST_ERR_ARG, "fst_upload: count does not match the array size");
+  HIPCHK(ctx, hipMemcpyAsync(p, host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return FST_OK;
+}
+
+int fst_download(fst_ctx *ctx, int array, double *host, size_t count) {
+  if (!ctx || !host) return FST_ERR_ARG;
+  size_t n = 0;
+  double *p = array_ptr(ctx, array, &n);
+  if (!p || n == 0) return fail(ctx, FST_ERR_ARG, "fst_download: array not available in this configuration");
+  if (count != n) return fail(ctx, FST_ERR_ARG, "fst_download: count does not match the array size");
+  HIPCHK(ctx, hipMemcpyAsync(host, p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return FST_OK;
+}
+
+int fst_apply_bcs(fst_ctx *ctx) {
+  if (!ctx) return FST_ERR_ARG;
+  launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
+  return check_launch(ctx, "fst_apply_bcs");
+}
+
+int fst_pre_sim(fst_ctx *ctx) {
+  if (!ctx) return FST_ERR_ARG;
+  if (ctx->g.ndim == 2) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);  // src/cart2d.cpp:137
+  launch_secondary(ctx->g, ctx->ph, ctx->b, ctx->stream);
+  return check_launch(ctx, "fst_pre_sim");
+}
+
+int fst_pre_step(fst_ctx *ctx) {
+  if (!ctx) return FST_ERR_ARG;
+  if (ctx->g.ndim == 2) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);  // src/cart2d.cpp:147-149
+  return check_launch(ctx, "fst_pre_step");
+}
+
+int fst_compute(fst_ctx *ctx) {
+  if (!ctx) return FST_ERR_ARG;
+  if (ctx->g.ndim == 3) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);  // src/cart3d.cpp:221
+  launch_compute_ref(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->stream);
+  return check_launch(ctx, "fst_compute");
+}
+
+int fst_rk_stage(fst_ctx *ctx, int stage) {
+  if (!ctx || (stage != 1 && stage != 2)) return FST_ERR_ARG;
+  launch_rk_stage(ctx->g, stage, ctx->cf.dt, ctx->b.var, ctx->b.tmp1, ctx->b.dvar, ctx->stream);
+  return check_launch(ctx, "fst_rk_stage");
+}
+
+int fst_post_step(fst_ctx *ctx) {
+  if (!ctx) return FST_ERR_ARG;
+  launch_secondary(ctx->g, ctx->ph, ctx->b, ctx->stream);
+  return check_launch(ctx, "fst_post_step");
+}
+
+int fst_set_fused(fst_ctx *ctx, int fused) {
+  if (!ctx) return FST_ERR_ARG;
+  if (fused && !fused_supported(ctx->cf))
+    return fail(ctx, FST_ERR_ARG, "fused path supports 3D cartesian FP64 with ceq, visc and buoyancy flags only");
+  ctx->fused = fused ? 1 : 0;
+  return FST_OK;
+}
+
+int fst_step(fst_ctx *ctx, int nsteps) {
+  if (!ctx || nsteps < 0) return FST_ERR_ARG;
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (ctx->fused) {
+    int rc = fused_steps(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->fb, nsteps, ctx->stream, &ctx->err);
+    if (rc) return rc;
+  } else {
+    for (int t = 0; t < nsteps; ++t) {
+      if (ctx->g.ndim == 2) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
+      for (int stage = 1; stage <= 2; ++stage) {
+        if (ctx->g.ndim == 3) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
+        launch_compute_ref(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->stream);
+        launch_rk_stage(ctx->g, stage, ctx->cf.dt, ctx->b.var, ctx->b.tmp1, ctx->b.dvar, ctx->stream);
+      }
+    }
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  return check_launch(ctx, "fst_step");
+}
+
+int fst_last_step_ms(fst_ctx *ctx, double *ms) {
+  if (!ctx || !ms) return FST_ERR_ARG;
+  HIPCHK(ctx, hipEventSynchronize(ctx->ev1));
+  float f = 0.f;
+  HIPCHK(ctx, hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+  *ms = f;
+  return FST_OK;
+}
+
+int fst_last_maxima(fst_ctx *ctx, double *maxS, double *maxCh) {
+  if (!ctx || !maxS || !maxCh) return FST_ERR_ARG;
+  Scalars s;
+  HIPCHK(ctx, hipMemcpyAsync(&s, ctx->b.sc, sizeof(s), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *maxS = as_double(s.maxS);
+  *maxCh = as_double(s.maxCh);
+  return FST_OK;
+}
+
+int fst_sync(fst_ctx *ctx) {
+  if (!ctx) return FST_ERR_ARG;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return check_launch(ctx, "fst_sync");
+}
+
+}  // extern "C"

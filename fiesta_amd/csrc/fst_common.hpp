@@ -1,0 +1,55 @@
+// Shared device-side definitions of the FIESTA RHS/RK library (gfx950).
+//
+// Layout: every array is a stack of (ngi x ngj x ngk) planes, i fastest
+// (LayoutLeft, the reference GPU layout of src/kokkosTypes.hpp:40).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fiesta_hip.h"
+
+namespace fst {
+
+struct Geo {
+  int ngi, ngj, ngk, ng;
+  int nci, ncj, nck, ndim;
+  int nv, nvt, ns, nvx;
+  long long N;  // cells including ghosts
+  __host__ __device__ __forceinline__ long long id(int i, int j, int k) const {
+    return (long long)i + (long long)ngi * ((long long)j + (long long)ngj * (long long)k);
+  }
+};
+
+// Constants the kernels read.  cd[] is the reference device config array
+// [ns, dx, dy, dz, nv, ng, (gamma, R/M, mu) x ns] (src/cart3d.cpp:152-171);
+// cpf/cvf are gamma*Rs/(gamma-1) and Rs/(gamma-1), the per-species factors of
+// src/secondary.hpp:136-138 evaluated once on the host with the same operations.
+struct Phys {
+  double cd[6 + 3 * FST_MAX_SPECIES];
+  double cpf[FST_MAX_SPECIES], cvf[FST_MAX_SPECIES], mus[FST_MAX_SPECIES];
+  double dx, dy, dz;     // cf.dx, cf.dy, cf.dz
+  double dxmag2;         // dx*dx+dy*dy+dz*dz (src/cart3d.cpp:173)
+  double kap, eps, alpha, g, rhoRef, dt;
+};
+
+// Device-side scalars written by the reduction kernels (bit patterns of
+// non-negative doubles order like unsigned integers).
+struct Scalars {
+  unsigned long long maxS, maxC, maxCh, pad;
+};
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long *a, double v) {
+  atomicMax(a, (unsigned long long)__double_as_longlong(v));
+}
+
+__host__ __device__ __forceinline__ double as_double(unsigned long long b) {
+  return __builtin_bit_cast(double, b);
+}
+
+// Wave-wide max over 64 lanes.
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+}  // namespace fst

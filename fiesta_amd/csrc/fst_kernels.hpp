@@ -1,0 +1,77 @@
+// Device helpers shared by the reference-sequence and fused kernels, and the
+// launcher declarations used by the C-ABI layer.
+#pragma once
+#include "fst_common.hpp"
+
+namespace fst {
+
+// Device buffers of one context.  Arrays are LayoutLeft planes of N cells.
+struct Bufs {
+  double *var = nullptr, *tmp1 = nullptr, *dvar = nullptr, *varx = nullptr;
+  double *p = nullptr, *rho = nullptr, *T = nullptr, *vel = nullptr;
+  double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+  double *stress = nullptr;  // 3D: 6 planes; 2D: sx00 sx11 sx01 sy00 sy11 sy01 qx qy
+  double *gradRho = nullptr;
+  double *mflux = nullptr;   // 3D: 27 planes ((face*3+dir)*3+w); 2D: 8 planes
+  double *grid = nullptr, *metrics = nullptr;
+  Scalars *sc = nullptr;     // per-stage maxima (device)
+};
+
+// WENO5 of calculateFluxesG (src/flux.hpp:240-264): binary32 coefficients
+// promoted to double, the reference association kept term by term.
+__device__ __forceinline__ double weno5_f32c(double f1, double f2, double f3, double f4, double f5) {
+  const double eps = 0.000001;
+  double b1, b2, b3, w1, w2, w3, p1, p2, p3, a1, a2;
+  a1 = f1 - 2.0f * f2 + f3;
+  a2 = f1 - 4.0f * f2 + 3.0f * f3;
+  b1 = (13.0f / 12.0f) * a1 * a1 + (0.25f) * a2 * a2;
+  a1 = f2 - 2.0 * f3 + f4;
+  a2 = f2 - f4;
+  b2 = (13.0f / 12.0f) * a1 * a1 + (0.25f) * a2 * a2;
+  a1 = f3 - 2.0 * f4 + f5;
+  a2 = 3.0 * f3 - 4.0 * f4 + f5;
+  b3 = (13.0f / 12.0f) * a1 * a1 + (0.25f) * a2 * a2;
+  a1 = eps + b1;
+  w1 = (0.1f) / (a1 * a1);
+  a1 = eps + b2;
+  w2 = (0.6f) / (a1 * a1);
+  a1 = eps + b3;
+  w3 = (0.3f) / (a1 * a1);
+  p1 = (1.0f / 3.0f) * f1 + (-7.0f / 6.0f) * f2 + (11.0f / 6.0f) * f3;
+  p2 = (-1.0f / 6.0f) * f2 + (5.0f / 6.0f) * f3 + (1.0f / 3.0f) * f4;
+  p3 = (1.0f / 3.0f) * f3 + (5.0f / 6.0f) * f4 + (-1.0f / 6.0f) * f5;
+  return (w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3);
+}
+
+// WENO5 of computeFluxWeno2D (src/flux.hpp:38-62): double coefficients.
+__device__ __forceinline__ double weno5_f64c(double f1, double f2, double f3, double f4, double f5) {
+  const double eps = 1e-6;
+  double b1, b2, b3, w1, w2, w3, p1, p2, p3, a1, a2;
+  a1 = f1 - 2.0 * f2 + f3;
+  a2 = f1 - 4.0 * f2 + 3.0 * f3;
+  b1 = (13.0 / 12.0) * a1 * a1 + (0.25) * a2 * a2;
+  a1 = f2 - 2.0 * f3 + f4;
+  a2 = f2 - f4;
+  b2 = (13.0 / 12.0) * a1 * a1 + (0.25) * a2 * a2;
+  a1 = f3 - 2.0 * f4 + f5;
+  a2 = 3.0 * f3 - 4.0 * f4 + f5;
+  b3 = (13.0 / 12.0) * a1 * a1 + (0.25) * a2 * a2;
+  a1 = eps + b1;
+  w1 = (0.1) / (a1 * a1);
+  a1 = eps + b2;
+  w2 = (0.6) / (a1 * a1);
+  a1 = eps + b3;
+  w3 = (0.3) / (a1 * a1);
+  p1 = (1.0 / 3.0) * f1 + (-7.0 / 6.0) * f2 + (11.0 / 6.0) * f3;
+  p2 = (-1.0 / 6.0) * f2 + (5.0 / 6.0) * f3 + (1.0 / 3.0) * f4;
+  p3 = (1.0 / 3.0) * f3 + (5.0 / 6.0) * f4 + (-1.0 / 6.0) * f5;
+  return (w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3);
+}
+
+void launch_eos_vel(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
+void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);
+void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream_t s);
+void launch_rk_stage(const Geo &g, int stage, double dt, double *var, double *tmp, const double *dvar, hipStream_t s);
+void launch_secondary(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
+
+}  // namespace fst

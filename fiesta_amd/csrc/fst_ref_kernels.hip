@@ -1,0 +1,873 @@
+// Reference-sequence kernels: one HIP kernel per reference Kokkos kernel, same
+// ranges, same arithmetic order (compiled with -ffp-contract=off).  They back
+// fst_compute()/fst_rk_stage() (the rk_func::compute contract, dvar written
+// to HBM) and every 2D configuration.  The fused 3D stage kernel lives in
+// fst_fused3d.hip and must agree with these bit for bit.
+#include "fst_common.hpp"
+#include "fst_kernels.hpp"
+
+namespace fst {
+
+namespace {
+
+struct Range {
+  int i0, j0, k0, ni, nj, nk;
+};
+
+__device__ __forceinline__ bool decode(const Range &r, int &i, int &j, int &k) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long n = (long long)r.ni * r.nj * r.nk;
+  if (t >= n) return false;
+  i = r.i0 + (int)(t % r.ni);
+  t /= r.ni;
+  j = r.j0 + (int)(t % r.nj);
+  k = r.k0 + (int)(t / r.nj);
+  return true;
+}
+
+inline Range ghost_range(const Geo &g) { return {0, 0, 0, g.ngi, g.ngj, g.ngk}; }
+inline Range cell_range(const Geo &g) {
+  if (g.ndim == 2) return {g.ng, g.ng, 0, g.nci, g.ncj, 1};
+  return {g.ng, g.ng, g.ng, g.nci, g.ncj, g.nck};
+}
+// [ng-1, n-ng) in every active dimension (weno/face/tile range)
+inline Range face_range(const Geo &g) {
+  if (g.ndim == 2) return {g.ng - 1, g.ng - 1, 0, g.nci + 1, g.ncj + 1, 1};
+  return {g.ng - 1, g.ng - 1, g.ng - 1, g.nci + 1, g.ncj + 1, g.nck + 1};
+}
+
+inline dim3 grid_for(const Range &r, int bs = 256) {
+  long long n = (long long)r.ni * r.nj * r.nk;
+  return dim3((unsigned)((n + bs - 1) / bs));
+}
+
+// ------------------------------------------------------------- 3D
+
+// calculateRhoPT3D + computeVelocity3D (src/secondary.hpp:109-151,
+// src/velocity.hpp:68-83), fused: both are pointwise over the ghost range.
+__global__ void k_eos_vel3d(Geo g, Phys ph, Range r, const double *__restrict__ var, double *__restrict__ p,
+                            double *__restrict__ rho, double *__restrict__ T, double *__restrict__ vel) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long c = g.id(i, j, k), N = g.N;
+  double rr = 0.0;
+  for (int s = 0; s < g.ns; ++s) rr = rr + var[c + N * (4 + s)];
+  double Cp = 0, Cv = 0;
+  for (int s = 0; s < g.ns; ++s) {
+    double Y = var[c + N * (4 + s)] / rr;
+    Cp = Cp + Y * ph.cpf[s];
+    Cv = Cv + Y * ph.cvf[s];
+  }
+  double gamma = Cp / Cv;
+  double m0 = var[c], m1 = var[c + N], m2 = var[c + 2 * N];
+  double pp = (gamma - 1) * (var[c + 3 * N] - (0.5 / rr) * (m0 * m0 + m1 * m1 + m2 * m2));
+  rho[c] = rr;
+  p[c] = pp;
+  T[c] = pp / ((Cp - Cv) * rr);
+  vel[c] = m0 / rr;
+  vel[c + N] = m1 / rr;
+  vel[c + 2 * N] = m2 / rr;
+}
+
+// calculateFluxesG, src/flux.hpp:266-309 over the tile range
+__global__ void k_flux3d(Geo g, Phys ph, Range r, int v, const double *__restrict__ var, const double *__restrict__ p,
+                         const double *__restrict__ vel, double *__restrict__ fx, double *__restrict__ fy,
+                         double *__restrict__ fz) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const double *q = var + N * v;
+  const double isE = (double)(v == 3);
+  auto Q = [&](int a, int b, int cc) {
+    long long c = g.id(a, b, cc);
+    return q[c] + isE * p[c];
+  };
+  auto U = [&](int a, int b, int cc, int d) { return vel[g.id(a, b, cc) + N * d]; };
+  const long long c = g.id(i, j, k);
+  double f1, f2, f3, f4, f5, f6, ur;
+  f1 = Q(i + 3, j, k); f2 = Q(i + 2, j, k); f3 = Q(i + 1, j, k);
+  f4 = Q(i, j, k);     f5 = Q(i - 1, j, k); f6 = Q(i - 2, j, k);
+  ur = (-U(i + 2, j, k, 0) + 7.0f * U(i + 1, j, k, 0) + 7.0f * U(i, j, k, 0) - U(i - 1, j, k, 0)) / 12.0f;
+  fx[c] = (ur < 0.0) ? ur * weno5_f32c(f1, f2, f3, f4, f5) / ph.dx : ur * weno5_f32c(f6, f5, f4, f3, f2) / ph.dx;
+  f1 = Q(i, j + 3, k); f2 = Q(i, j + 2, k); f3 = Q(i, j + 1, k);
+  f4 = Q(i, j, k);     f5 = Q(i, j - 1, k); f6 = Q(i, j - 2, k);
+  ur = (-U(i, j + 2, k, 1) + 7.0f * U(i, j + 1, k, 1) + 7.0f * U(i, j, k, 1) - U(i, j - 1, k, 1)) / 12.0f;
+  fy[c] = (ur < 0.0) ? ur * weno5_f32c(f1, f2, f3, f4, f5) / ph.dy : ur * weno5_f32c(f6, f5, f4, f3, f2) / ph.dy;
+  f1 = Q(i, j, k + 3); f2 = Q(i, j, k + 2); f3 = Q(i, j, k + 1);
+  f4 = Q(i, j, k);     f5 = Q(i, j, k - 1); f6 = Q(i, j, k - 2);
+  ur = (-U(i, j, k + 2, 2) + 7.0f * U(i, j, k + 1, 2) + 7.0f * U(i, j, k, 2) - U(i, j, k - 1, 2)) / 12.0f;
+  fz[c] = (ur < 0.0) ? ur * weno5_f32c(f1, f2, f3, f4, f5) / ph.dz : ur * weno5_f32c(f6, f5, f4, f3, f2) / ph.dz;
+}
+
+// advect3D, src/advect.hpp:39-61
+__global__ void k_advect3d(Geo g, Range r, int v, const double *__restrict__ fx, const double *__restrict__ fy,
+                           const double *__restrict__ fz, double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long c = g.id(i, j, k);
+  double res = -((fx[c] - fx[g.id(i - 1, j, k)]) + (fy[c] - fy[g.id(i, j - 1, k)]) + (fz[c] - fz[g.id(i, j, k - 1)]));
+  dvar[c + g.N * v] = res;
+}
+
+// applyPressureGradient3D (dz := cd(2)), src/presgrad.hpp:44-71, then
+// computeBuoyancy3D, src/buoyancy.hpp:22-47 (both pointwise in dvar)
+__global__ void k_presgrad_buoy3d(Geo g, Phys ph, Range r, int buoyancy, const double *__restrict__ var,
+                                  const double *__restrict__ p, const double *__restrict__ rho,
+                                  double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long c = g.id(i, j, k), N = g.N;
+  const double dx = ph.cd[1], dy = ph.cd[2], dz = ph.cd[2];
+  auto P = [&](int a, int b, int cc) { return p[g.id(a, b, cc)]; };
+  double dxp = (P(i - 2, j, k) - 8.0 * P(i - 1, j, k) + 8.0 * P(i + 1, j, k) - P(i + 2, j, k)) / (12.0 * dx);
+  double dyp = (P(i, j - 2, k) - 8.0 * P(i, j - 1, k) + 8.0 * P(i, j + 1, k) - P(i, j + 2, k)) / (12.0 * dy);
+  double dzp = (P(i, j, k - 2) - 8.0 * P(i, j, k - 1) + 8.0 * P(i, j, k + 1) - P(i, j, k + 2)) / (12.0 * dz);
+  dvar[c] = dvar[c] - dxp;
+  dvar[c + N] = dvar[c + N] - dyp;
+  dvar[c + 2 * N] = dvar[c + 2 * N] - dzp;
+  if (buoyancy) {
+    double vv = var[c + N] / rho[c];
+    double rhop = rho[c] - ph.rhoRef;
+    double f = -ph.g * rhop;
+    dvar[c + N] += f;
+    dvar[c + 3 * N] += vv * f;
+  }
+}
+
+// calculateStressTensor{x,y,z}3dv, src/viscosity.hpp:416-624, weno range
+__global__ void k_stress3d(Geo g, Phys ph, Range r, int dir, const double *__restrict__ var,
+                           const double *__restrict__ rho, const double *__restrict__ vel,
+                           double *__restrict__ stress) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const double dx = ph.cd[1], dy = ph.cd[2], dz = ph.cd[3];
+  auto u = [&](int a, int b, int cc, int d) { return vel[g.id(a, b, cc) + N * d]; };
+  const int ip = dir == 0, jp = dir == 1, kp = dir == 2;
+  const long long c = g.id(i, j, k), cn = g.id(i + ip, j + jp, k + kp);
+  double mu = 0.0;
+  for (int s = 0; s < g.ns; ++s)
+    mu += ((var[cn + N * (4 + s)] / rho[cn] + var[c + N * (4 + s)] / rho[c]) / 2.0) * ph.mus[s];
+  double dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz;
+  if (dir == 0) {
+    dudx = (u(i + 1, j, k, 0) - u(i, j, k, 0)) / dx;
+    dvdx = (u(i + 1, j, k, 1) - u(i, j, k, 1)) / dx;
+    dwdx = (u(i + 1, j, k, 2) - u(i, j, k, 2)) / dx;
+    dudy = ((u(i + 1, j + 1, k, 0) + u(i, j + 1, k, 0)) - (u(i + 1, j - 1, k, 0) + u(i, j - 1, k, 0))) / (4 * dy);
+    dvdy = ((u(i + 1, j + 1, k, 1) + u(i, j + 1, k, 1)) - (u(i + 1, j - 1, k, 1) + u(i, j - 1, k, 1))) / (4 * dy);
+    dwdy = ((u(i + 1, j + 1, k, 2) + u(i, j + 1, k, 2)) - (u(i + 1, j - 1, k, 2) + u(i, j - 1, k, 2))) / (4 * dy);
+    dudz = ((u(i + 1, j, k + 1, 0) + u(i, j, k + 1, 0)) - (u(i + 1, j, k - 1, 0) + u(i, j, k - 1, 0))) / (4 * dz);
+    dvdz = ((u(i + 1, j, k + 1, 1) + u(i, j, k + 1, 1)) - (u(i + 1, j, k - 1, 1) + u(i, j, k - 1, 1))) / (4 * dz);
+    dwdz = ((u(i + 1, j, k + 1, 2) + u(i, j, k + 1, 2)) - (u(i + 1, j, k - 1, 2) + u(i, j, k - 1, 2))) / (4 * dz);
+  } else if (dir == 1) {
+    dudx = ((u(i + 1, j + 1, k, 0) + u(i + 1, j, k, 0)) - (u(i - 1, j + 1, k, 0) + u(i - 1, j, k, 0))) / (4 * dx);
+    dvdx = ((u(i + 1, j + 1, k, 1) + u(i + 1, j, k, 1)) - (u(i - 1, j + 1, k, 1) + u(i - 1, j, k, 1))) / (4 * dx);
+    dwdx = ((u(i + 1, j + 1, k, 2) + u(i + 1, j, k, 2)) - (u(i - 1, j + 1, k, 2) + u(i - 1, j, k, 2))) / (4 * dx);
+    dudy = (u(i, j + 1, k, 0) - u(i, j, k, 0)) / dy;
+    dvdy = (u(i, j + 1, k, 1) - u(i, j, k, 1)) / dy;
+    dwdy = (u(i, j + 1, k, 2) - u(i, j, k, 2)) / dy;
+    dudz = ((u(i, j + 1, k + 1, 0) + u(i, j, k + 1, 0)) - (u(i, j + 1, k - 1, 0) + u(i, j, k - 1, 0))) / (4 * dz);
+    dvdz = ((u(i, j + 1, k + 1, 1) + u(i, j, k + 1, 1)) - (u(i, j + 1, k - 1, 1) + u(i, j, k - 1, 1))) / (4 * dz);
+    dwdz = ((u(i, j + 1, k + 1, 2) + u(i, j, k + 1, 2)) - (u(i, j + 1, k - 1, 2) + u(i, j, k - 1, 2))) / (4 * dz);
+  } else {
+    dudx = ((u(i + 1, j, k + 1, 0) + u(i + 1, j, k, 0)) - (u(i - 1, j, k + 1, 0) + u(i - 1, j, k, 0))) / (4 * dx);
+    dvdx = ((u(i + 1, j, k + 1, 1) + u(i + 1, j, k, 1)) - (u(i - 1, j, k + 1, 1) + u(i - 1, j, k, 1))) / (4 * dx);
+    dwdx = ((u(i + 1, j, k + 1, 2) + u(i + 1, j, k, 2)) - (u(i - 1, j, k + 1, 2) + u(i - 1, j, k, 2))) / (4 * dx);
+    dudy = ((u(i, j + 1, k + 1, 0) + u(i, j + 1, k, 0)) - (u(i, j - 1, k + 1, 0) + u(i, j - 1, k, 0))) / (4 * dy);
+    dvdy = ((u(i, j + 1, k + 1, 1) + u(i, j + 1, k, 1)) - (u(i, j - 1, k + 1, 1) + u(i, j - 1, k, 1))) / (4 * dy);
+    dwdy = ((u(i, j + 1, k + 1, 2) + u(i, j + 1, k, 2)) - (u(i, j - 1, k + 1, 2) + u(i, j - 1, k, 2))) / (4 * dy);
+    dudz = (u(i, j, k + 1, 0) - u(i, j, k, 0)) / dz;
+    dvdz = (u(i, j, k + 1, 1) - u(i, j, k, 1)) / dz;
+    dwdz = (u(i, j, k + 1, 2) - u(i, j, k, 2)) / dz;
+  }
+  stress[c + N * 0] = (2.0 / 3.0) * mu * (2.0 * dudx - dvdy - dwdz);
+  stress[c + N * 1] = (2.0 / 3.0) * mu * (2.0 * dvdy - dudx - dwdz);
+  stress[c + N * 2] = (2.0 / 3.0) * mu * (2.0 * dwdz - dudx - dvdy);
+  stress[c + N * 3] = mu * (dudy + dvdx);
+  stress[c + N * 4] = mu * (dudz + dwdx);
+  stress[c + N * 5] = mu * (dvdz + dwdy);
+}
+
+// applyViscousTerm{x,y,z}3dv, src/viscosity.hpp:460-663
+__global__ void k_applyvisc3d(Geo g, Phys ph, Range r, int dir, const double *__restrict__ vel,
+                              const double *__restrict__ stress, double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const int s0 = dir == 0 ? 0 : (dir == 1 ? 3 : 4);
+  const int s1 = dir == 0 ? 3 : (dir == 1 ? 1 : 5);
+  const int s2 = dir == 0 ? 4 : (dir == 1 ? 5 : 2);
+  const double d = ph.cd[1 + dir];
+  const int ih = dir == 0, jh = dir == 1, kh = dir == 2;
+  const long long c = g.id(i, j, k), cp = g.id(i + ih, j + jh, k + kh), cm = g.id(i - ih, j - jh, k - kh);
+  double ur = (vel[cp] + vel[c]) / 2.0;
+  double ul = (vel[cm] + vel[c]) / 2.0;
+  double vr = (vel[cp + N] + vel[c + N]) / 2.0;
+  double vl = (vel[cm + N] + vel[c + N]) / 2.0;
+  double wr = (vel[cp + 2 * N] + vel[c + 2 * N]) / 2.0;
+  double wl = (vel[cm + 2 * N] + vel[c + 2 * N]) / 2.0;
+  auto S = [&](long long q, int s) { return stress[q + N * s]; };
+  double a = (S(c, s0) - S(cm, s0)) / d;
+  double b = (S(c, s1) - S(cm, s1)) / d;
+  double cc = (S(c, s2) - S(cm, s2)) / d;
+  double dd = (ur * S(c, s0) - ul * S(cm, s0)) / d + (vr * S(c, s1) - vl * S(cm, s1)) / d +
+              (wr * S(c, s2) - wl * S(cm, s2)) / d;
+  dvar[c] += a;
+  dvar[c + N] += b;
+  dvar[c + 2 * N] += cc;
+  dvar[c + 3 * N] += dd;
+}
+
+// calculateRhoGrad (src/ceq3d.hpp:95-187) fused with the two per-stage maxima
+// maxWaveSpeed (:25-63) and maxGradFunctor(var, nv+1) (:82-93); all cell range
+__global__ void k_rhograd_max3d(Geo g, Phys ph, Range r, const double *__restrict__ var,
+                                const double *__restrict__ p, const double *__restrict__ rho,
+                                const double *__restrict__ vel, double *__restrict__ gradRho, Scalars *sc) {
+  int i, j, k;
+  double ms = 0.0, mc = 0.0;
+  if (decode(r, i, j, k)) {
+    const long long N = g.N;
+    const double dx = ph.dx, dy = ph.dy, dz = ph.dz;
+    auto R = [&](int a, int b, int cc) { return rho[g.id(a, b, cc)]; };
+    auto V = [&](int a, int b, int cc, int w) { return vel[g.id(a, b, cc) + N * w]; };
+    auto E = [&](int a, int b, int cc) {
+      long long q = g.id(a, b, cc);
+      return var[q + 3 * N] / rho[q] -
+             0.5 * rho[q] * (vel[q] * vel[q] + vel[q + N] * vel[q + N] + vel[q + 2 * N] * vel[q + 2 * N]);
+    };
+    double dxr = (R(i - 2, j, k) - 8.0 * R(i - 1, j, k) + 8.0 * R(i + 1, j, k) - R(i + 2, j, k)) / (12.0 * dx);
+    double dyr = (R(i, j - 2, k) - 8.0 * R(i, j - 1, k) + 8.0 * R(i, j + 1, k) - R(i, j + 2, k)) / (12.0 * dy);
+    double dzr = (R(i, j, k - 2) - 8.0 * R(i, j, k - 1) + 8.0 * R(i, j, k + 1) - R(i, j, k + 2)) / (12.0 * dz);
+    double dxe = (E(i - 2, j, k) - 8.0 * E(i - 1, j, k) + 8.0 * E(i + 1, j, k) - E(i + 2, j, k)) / (12.0 * dx);
+    double dye = (E(i, j - 2, k) - 8.0 * E(i, j - 1, k) + 8.0 * E(i, j + 1, k) - E(i, j + 2, k)) / (12.0 * dy);
+    double dze = (E(i, j, k - 2) - 8.0 * E(i, j, k - 1) + 8.0 * E(i, j, k + 1) - E(i, j, k + 2)) / (12.0 * dz);
+    double dxu = (V(i - 2, j, k, 0) - 8.0 * V(i - 1, j, k, 0) + 8.0 * V(i + 1, j, k, 0) - V(i + 2, j, k, 0)) / (12.0 * dx);
+    double dyv = (V(i, j - 2, k, 1) - 8.0 * V(i, j - 1, k, 1) + 8.0 * V(i, j + 1, k, 1) - V(i, j + 2, k, 1)) / (12.0 * dy);
+    double dzw = (V(i, j, k - 2, 2) - 8.0 * V(i, j, k - 1, 2) + 8.0 * V(i, j, k + 1, 2) - V(i, j, k + 2, 2)) / (12.0 * dz);
+    double rgrad = sqrt(dxr * dxr + dyr * dyr + dzr * dzr);
+    double divu = dxu + dyv + dzw;
+    double dnednr = (dxr * dxe + dyr * dye + dzr * dze) * (dxr * dxr + dyr * dyr + dzr * dzr);
+    int ind = dnednr < 0.0 ? 1 : 0;
+    const long long c = g.id(i, j, k);
+    gradRho[c] = (divu < 0) ? (1 - ind) * rgrad : 0;
+    gradRho[c + N] = ind * rgrad;
+    gradRho[c + 2 * N] = dxr;
+    gradRho[c + 3 * N] = dyr;
+    gradRho[c + 4 * N] = dzr;
+    // maxWaveSpeed
+    double Cp = 0, Cv = 0;
+    for (int s = 0; s < g.ns; ++s) {
+      double Y = var[c + N * (4 + s)] / rho[c];
+      Cp = Cp + Y * ph.cpf[s];
+      Cv = Cv + Y * ph.cvf[s];
+    }
+    double gamma = Cp / Cv;
+    double a = sqrt(gamma * p[c] / rho[c]);
+    ms = a + sqrt(var[c] * var[c] + var[c + N] * var[c + N] + var[c + 2 * N] * var[c + 2 * N]) / rho[c];
+    mc = fabs(var[c + N * (g.nv + 1)]);
+  }
+  ms = wave_max(ms);
+  mc = wave_max(mc);
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_nonneg(&sc->maxS, ms);
+    atomic_max_nonneg(&sc->maxCh, mc);
+  }
+}
+
+// updateCeq, src/ceq3d.hpp:189-224 (maxS from the device reduction)
+__global__ void k_updateceq3d(Geo g, Phys ph, Range r, const double *__restrict__ var,
+                              const double *__restrict__ gradRho, const Scalars *sc, double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const double dx = ph.cd[1], dy = ph.cd[2], dz = ph.cd[3];
+  const int nc = g.nv;
+  const double maxS = as_double(sc->maxS);
+  const double dxmag = sqrt(dx * dx + dy * dy + dz * dz);
+  const long long c = g.id(i, j, k);
+  for (int n = 0; n < 5; ++n) {
+    const double *C = var + N * (nc + n);
+    double lap = (C[g.id(i - 1, j, k)] - 2 * C[c] + C[g.id(i + 1, j, k)]) / dx +
+                 (C[g.id(i, j - 1, k)] - 2 * C[c] + C[g.id(i, j + 1, k)]) / dy +
+                 (C[g.id(i, j, k - 1)] - 2 * C[c] + C[g.id(i, j, k + 1)]) / dz;
+    dvar[c + N * (nc + n)] = (maxS / (ph.eps * dxmag)) * (gradRho[c + N * n] - C[c]) + ph.kap * maxS * dxmag * lap;
+  }
+}
+
+// calculateCeqFaces (src/ceq3d.hpp:226-296) and calculateCeqGrads (:298-332),
+// fused per face (the product X*deriv is formed exactly as the two passes do)
+__global__ void k_ceqfaces3d(Geo g, Phys ph, Range r, const double *__restrict__ var, const double *__restrict__ rho,
+                             const double *__restrict__ vel, const Scalars *sc, double *__restrict__ mflux) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const int nv = g.nv;
+  const double alpha = (ph.dxmag2 / (as_double(sc->maxCh) + 1.0e-6)) * ph.alpha;
+  const double dx = ph.dx, dy = ph.dy, dz = ph.dz;
+  auto vv = [&](int a, int b, int cc, int w) { return vel[g.id(a, b, cc) + N * w]; };
+  const long long c = g.id(i, j, k);
+  double X[3][3];
+  for (int face = 0; face < 3; ++face) {
+    const long long q = g.id(i + (face == 0), j + (face == 1), k + (face == 2));
+    double rr = (rho[c] + rho[q]) / 2.0;
+    double chat = (var[c + N * (nv + 1)] + var[q + N * (nv + 1)]) / 2.0;
+    double cn[3];
+    cn[0] = (var[c + N * (nv + 2)] + var[q + N * (nv + 2)]) / 2.0;
+    cn[1] = (var[c + N * (nv + 3)] + var[q + N * (nv + 3)]) / 2.0;
+    cn[2] = (var[c + N * (nv + 4)] + var[q + N * (nv + 4)]) / 2.0;
+    double cmag = 1.0e-6;
+    cmag += cn[0] * cn[0];
+    cmag += cn[1] * cn[1];
+    cmag += cn[2] * cn[2];
+    for (int dir = 0; dir < 3; ++dir) {
+      double dirac = (face == dir) ? 1.0 : 0.0;
+      double M = dirac - (cn[face] * cn[dir]) / cmag;
+      X[face][dir] = alpha * rr * chat * M;
+    }
+  }
+  // mflux layout: ((face*3+dir)*3+w) planes of N
+  auto MF = [&](int f, int d, int w) -> double & { return mflux[c + N * ((f * 3 + d) * 3 + w)]; };
+  for (int w = 0; w < 3; ++w) {
+    MF(0, 0, w) = X[0][0] * ((vv(i + 1, j, k, w) - vv(i, j, k, w)) / dx);
+    MF(0, 1, w) = X[0][1] * (((vv(i, j + 1, k, w) + vv(i + 1, j + 1, k, w)) - (vv(i, j - 1, k, w) + vv(i + 1, j - 1, k, w))) / (4 * dy));
+    MF(0, 2, w) = X[0][2] * (((vv(i, j, k + 1, w) + vv(i + 1, j, k + 1, w)) - (vv(i, j, k - 1, w) + vv(i + 1, j, k - 1, w))) / (4 * dz));
+    MF(1, 0, w) = X[1][0] * (((vv(i + 1, j + 1, k, w) + vv(i + 1, j, k, w)) - (vv(i - 1, j + 1, k, w) + vv(i - 1, j, k, w))) / (4 * dx));
+    MF(1, 1, w) = X[1][1] * ((vv(i, j + 1, k, w) - vv(i, j, k, w)) / dy);
+    MF(1, 2, w) = X[1][2] * (((vv(i, j + 1, k + 1, w) + vv(i, j, k + 1, w)) - (vv(i, j + 1, k - 1, w) + vv(i, j, k - 1, w))) / (4 * dz));
+    MF(2, 0, w) = X[2][0] * (((vv(i + 1, j, k + 1, w) + vv(i + 1, j, k, w)) - (vv(i - 1, j, k + 1, w) + vv(i - 1, j, k, w))) / (4 * dx));
+    MF(2, 1, w) = X[2][1] * (((vv(i, j + 1, k + 1, w) + vv(i, j + 1, k, w)) - (vv(i, j - 1, k + 1, w) + vv(i, j - 1, k, w))) / (4 * dy));
+    MF(2, 2, w) = X[2][2] * ((vv(i, j, k + 1, w) - vv(i, j, k, w)) / dz);
+  }
+}
+
+// applyCeq, src/ceq3d.hpp:334-366
+__global__ void k_applyceq3d(Geo g, Phys ph, Range r, const double *__restrict__ mflux, double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const double d[3] = {ph.dx, ph.dy, ph.dz};
+  const long long c = g.id(i, j, k);
+  for (int w = 0; w < 3; ++w) {
+    double diffu = 0.0;
+    for (int face = 0; face < 3; ++face) {
+      const long long cm = g.id(i - (face == 0), j - (face == 1), k - (face == 2));
+      for (int dir = 0; dir < 3; ++dir) {
+        const long long pl = N * ((face * 3 + dir) * 3 + w);
+        diffu += (mflux[c + pl] - mflux[cm + pl]) / d[face];
+      }
+    }
+    dvar[c + N * w] += diffu;
+  }
+}
+
+// ------------------------------------------------------------- 2D
+
+// calculateRhoPT2D + computeVelocity2D, src/secondary.hpp:60-107, src/velocity.hpp:85-99
+__global__ void k_eos_vel2d(Geo g, Phys ph, Range r, const double *__restrict__ var, double *__restrict__ p,
+                            double *__restrict__ rho, double *__restrict__ T, double *__restrict__ vel) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long c = g.id(i, j, 0), N = g.N;
+  double rr = 0.0;
+  for (int s = 0; s < g.ns; ++s) rr = rr + var[c + N * (3 + s)];
+  double Cp = 0, Cv = 0;
+  for (int s = 0; s < g.ns; ++s) {
+    double Y = var[c + N * (3 + s)] / rr;
+    Cp = Cp + Y * ph.cpf[s];
+    Cv = Cv + Y * ph.cvf[s];
+  }
+  double gamma = Cp / Cv;
+  double m0 = var[c], m1 = var[c + N];
+  double pp = (gamma - 1) * (var[c + 2 * N] - (0.5 / rr) * (m0 * m0 + m1 * m1));
+  rho[c] = rr;
+  p[c] = pp;
+  T[c] = pp / ((Cp - Cv) * rr);
+  vel[c] = m0 / rr;
+  vel[c + N] = m1 / rr;
+}
+
+// computeFluxWeno2D / computeFluxCentered2D / computeFluxQuick2D
+// (src/flux.hpp:23-221), face range
+__global__ void k_flux2d(Geo g, Phys ph, Range r, int v, int scheme, const double *__restrict__ var,
+                         const double *__restrict__ p, const double *__restrict__ rho,
+                         const double *__restrict__ vel, double *__restrict__ fx, double *__restrict__ fy) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const double isE = (double)(v == 2);
+  auto VAR = [&](int a, int b, int w) { return var[g.id(a, b, 0) + N * w]; };
+  auto P = [&](int a, int b) { return p[g.id(a, b, 0)]; };
+  auto R = [&](int a, int b) { return rho[g.id(a, b, 0)]; };
+  auto U = [&](int a, int b, int w) { return vel[g.id(a, b, 0) + N * w]; };
+  auto Q = [&](int a, int b) { return VAR(a, b, v) + isE * P(a, b); };
+  const long long c = g.id(i, j, 0);
+  if (scheme == FST_CENTERED4) {
+    const double dx = ph.cd[1], dy = ph.cd[2];
+    double ur = (0.0 - VAR(i + 2, j, 0) / R(i + 2, j) + 7.0 * VAR(i + 1, j, 0) / R(i + 1, j) +
+                 7.0 * VAR(i, j, 0) / R(i, j) - VAR(i - 1, j, 0) / R(i - 1, j)) / 12.0;
+    double vr = (0.0 - VAR(i, j + 2, 1) / R(i, j + 2) + 7.0 * VAR(i, j + 1, 1) / R(i, j + 1) +
+                 7.0 * VAR(i, j, 1) / R(i, j) - VAR(i, j - 1, 1) / R(i, j - 1)) / 12.0;
+    double px = (0.0 - P(i + 2, j) + 7.0 * P(i + 1, j) + 7.0 * P(i, j) - P(i - 1, j)) / 12.0;
+    double py = (0.0 - P(i, j + 2) + 7.0 * P(i, j + 1) + 7.0 * P(i, j) - P(i, j - 1)) / 12.0;
+    double x1 = (0.0 - VAR(i + 2, j, v) + 7.0 * VAR(i + 1, j, v) + 7.0 * VAR(i, j, v) - VAR(i - 1, j, v)) / 12.0;
+    double y1 = (0.0 - VAR(i, j + 2, v) + 7.0 * VAR(i, j + 1, v) + 7.0 * VAR(i, j, v) - VAR(i, j - 1, v)) / 12.0;
+    if (v == 2) {
+      x1 = x1 + px;
+      y1 = y1 + py;
+    }
+    fx[c] = ur * x1 / dx;
+    fy[c] = vr * y1 / dy;
+  } else if (scheme == FST_QUICK) {
+    const double dx = ph.cd[1], dy = ph.cd[2];
+    double f1, f2, f3;
+    double ur = (-U(i + 2, j, 0) + 7.0 * U(i + 1, j, 0) + 7.0 * U(i, j, 0) - U(i - 1, j, 0)) / 12.0;
+    if (ur < 0.0) { f1 = Q(i + 2, j); f2 = Q(i + 1, j); f3 = Q(i, j); }
+    else          { f1 = Q(i - 1, j); f2 = Q(i, j);     f3 = Q(i + 1, j); }
+    fx[c] = ur * (0.75 * f2 + 0.375 * f3 - 0.125 * f1) / dx;
+    double vr = (-U(i, j + 2, 1) + 7.0 * U(i, j + 1, 1) + 7.0 * U(i, j, 1) - U(i, j - 1, 1)) / 12.0;
+    if (vr < 0.0) { f1 = Q(i, j + 2); f2 = Q(i, j + 1); f3 = Q(i, j); }
+    else          { f1 = Q(i, j - 1); f2 = Q(i, j);     f3 = Q(i, j + 1); }
+    fy[c] = vr * (0.75 * f2 + 0.375 * f3 - 0.125 * f1) / dy;
+  } else {
+    double f1, f2, f3, f4, f5;
+    double ur = (-U(i + 2, j, 0) + 7.0 * U(i + 1, j, 0) + 7.0 * U(i, j, 0) - U(i - 1, j, 0)) / 12.0;
+    double vr = (-U(i, j + 2, 1) + 7.0 * U(i, j + 1, 1) + 7.0 * U(i, j, 1) - U(i, j - 1, 1)) / 12.0;
+    if (ur < 0.0) { f1 = Q(i + 3, j); f2 = Q(i + 2, j); f3 = Q(i + 1, j); f4 = Q(i, j); f5 = Q(i - 1, j); }
+    else          { f1 = Q(i - 2, j); f2 = Q(i - 1, j); f3 = Q(i, j); f4 = Q(i + 1, j); f5 = Q(i + 2, j); }
+    fx[c] = ur * weno5_f64c(f1, f2, f3, f4, f5) / ph.dx;
+    if (vr < 0.0) { f1 = Q(i, j + 3); f2 = Q(i, j + 2); f3 = Q(i, j + 1); f4 = Q(i, j); f5 = Q(i, j - 1); }
+    else          { f1 = Q(i, j - 2); f2 = Q(i, j - 1); f3 = Q(i, j); f4 = Q(i, j + 1); f5 = Q(i, j + 2); }
+    fy[c] = vr * weno5_f64c(f1, f2, f3, f4, f5) / ph.dy;
+  }
+}
+
+// advect2D, src/advect.hpp:23-37
+__global__ void k_advect2d(Geo g, Range r, int v, const double *__restrict__ fx, const double *__restrict__ fy,
+                           double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long c = g.id(i, j, 0);
+  dvar[c + g.N * v] = -((fx[c] - fx[g.id(i - 1, j, 0)]) + (fy[c] - fy[g.id(i, j - 1, 0)]));
+}
+
+// applyPressureGradient2D (src/presgrad.hpp:22-42) then computeBuoyancy
+// (src/buoyancy.hpp:49-73)
+__global__ void k_presgrad_buoy2d(Geo g, Phys ph, Range r, int buoyancy, const double *__restrict__ var,
+                                  const double *__restrict__ p, const double *__restrict__ rho,
+                                  double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long c = g.id(i, j, 0), N = g.N;
+  const double dx = ph.cd[1], dy = ph.cd[2];
+  auto P = [&](int a, int b) { return p[g.id(a, b, 0)]; };
+  double dxp = (P(i - 2, j) - 8.0 * P(i - 1, j) + 8.0 * P(i + 1, j) - P(i + 2, j)) / (12.0 * dx);
+  double dyp = (P(i, j - 2) - 8.0 * P(i, j - 1) + 8.0 * P(i, j + 1) - P(i, j + 2)) / (12.0 * dy);
+  dvar[c] -= dxp;
+  dvar[c + N] -= dyp;
+  if (buoyancy) {
+    double vv = var[c + N] / var[c + 3 * N];
+    double rhop = rho[c] - ph.rhoRef;
+    double eps = 1e-6;
+    if (rhop >= eps || rhop <= -eps) {
+      double f = -ph.g * rhop;
+      dvar[c + N] += f;
+      dvar[c + 2 * N] += fabs(vv) * f;
+    }
+  }
+}
+
+// calculateStressTensor2dv + calculateHeatFlux2dv, src/viscosity.hpp:25-136,
+// face range; stress planes: sx00, sx11, sx01, sy00, sy11, sy01, qx, qy
+__global__ void k_stress2d(Geo g, Phys ph, Range r, const double *__restrict__ vel, const double *__restrict__ T,
+                           double *__restrict__ st) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const double dx = ph.cd[1], dy = ph.cd[2];
+  auto U = [&](int a, int b, int w) { return vel[g.id(a, b, 0) + N * w]; };
+  const double mur = 2.928e-5, mut = 2.928e-5;
+  const long long c = g.id(i, j, 0);
+  double dudx = (U(i + 1, j, 0) - U(i, j, 0)) / dx;
+  double dvdx = (U(i + 1, j, 1) - U(i, j, 1)) / dx;
+  double dudy = ((U(i, j + 1, 0) + U(i + 1, j + 1, 0)) - (U(i, j - 1, 1) + U(i + 1, j - 1, 0))) / (4.0 * dy);
+  double dvdy = ((U(i, j + 1, 1) + U(i + 1, j + 1, 1)) - (U(i, j - 1, 1) + U(i + 1, j - 1, 1))) / (4.0 * dy);
+  st[c + N * 0] = (2.0 / 3.0) * mur * (2.0 * dudx - dvdy);
+  st[c + N * 1] = (2.0 / 3.0) * mur * (2.0 * dvdy - dudx);
+  st[c + N * 2] = mur * (dudy + dvdx);
+  dudx = ((U(i + 1, j, 0) + U(i + 1, j + 1, 0)) - (U(i - 1, j, 0) + U(i - 1, j + 1, 0))) / (4.0 * dx);
+  dvdx = ((U(i + 1, j, 1) + U(i + 1, j + 1, 1)) - (U(i - 1, j, 1) + U(i - 1, j + 1, 1))) / (4.0 * dx);
+  dudy = (U(i, j + 1, 0) - U(i, j, 0)) / dy;
+  dvdy = (U(i, j + 1, 1) - U(i, j, 1)) / dy;
+  st[c + N * 3] = (2.0 / 3.0) * mut * (2.0 * dudx - dvdy);
+  st[c + N * 4] = (2.0 / 3.0) * mut * (2.0 * dvdy - dudx);
+  st[c + N * 5] = mut * (dudy + dvdx);
+  const double kk = 0.026;
+  st[c + N * 6] = -kk * (T[g.id(i + 1, j, 0)] - T[c]) / dx;
+  st[c + N * 7] = -kk * (T[g.id(i, j + 1, 0)] - T[c]) / dy;
+}
+
+// applyViscousTerm2dv, src/viscosity.hpp:138-199
+__global__ void k_applyvisc2d(Geo g, Phys ph, Range r, const double *__restrict__ vel, const double *__restrict__ st,
+                              double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const double dx = ph.cd[1], dy = ph.cd[2];
+  auto U = [&](int a, int b, int w) { return vel[g.id(a, b, 0) + N * w]; };
+  const long long c = g.id(i, j, 0), cl = g.id(i - 1, j, 0), cb = g.id(i, j - 1, 0);
+  double ur = (U(i + 1, j, 0) + U(i, j, 0)) / 2.0;
+  double ul = (U(i - 1, j, 0) + U(i, j, 0)) / 2.0;
+  double vr = (U(i + 1, j, 1) + U(i, j, 1)) / 2.0;
+  double vl = (U(i - 1, j, 1) + U(i, j, 1)) / 2.0;
+  double ut = (U(i, j + 1, 0) + U(i, j, 0)) / 2.0;
+  double ub = (U(i, j - 1, 0) + U(i, j, 0)) / 2.0;
+  double vt = (U(i, j + 1, 1) + U(i, j, 1)) / 2.0;
+  double vb = (U(i, j - 1, 1) + U(i, j, 1)) / 2.0;
+  auto S = [&](long long q, int s) { return st[q + N * s]; };
+  // sx00=0 sx11=1 sx01=sx10=2 sy00=3 sy11=4 sy01=sy10=5 qx=6 qy=7
+  double a = (S(c, 0) - S(cl, 0)) / dx + (S(c, 5) - S(cb, 5)) / dy;
+  double b = (S(c, 2) - S(cl, 2)) / dx + (S(c, 4) - S(cb, 4)) / dy;
+  double c1 = (ur * S(c, 0) - ul * S(cl, 0)) / dx + (vr * S(c, 2) - vl * S(cl, 2)) / dx +
+              (ut * S(c, 5) - ub * S(cb, 5)) / dy + (vt * S(c, 4) - vb * S(cb, 4)) / dy;
+  double c2 = (S(c, 6) - S(cl, 6)) / dx + (S(c, 7) - S(cb, 7)) / dy;
+  dvar[c] = dvar[c] + a;
+  dvar[c + N] = dvar[c + N] + b;
+  dvar[c + 2 * N] = dvar[c + 2 * N] + c1 - c2;
+}
+
+// calculateRhoGrad2D (src/ceq.hpp:80-151) with maxWaveSpeed2D (:43-78, the
+// reference's stride-2 species lookup kept) and maxCvar2D for C and C_hat
+__global__ void k_rhograd_max2d(Geo g, Phys ph, Range r, const double *__restrict__ var,
+                                const double *__restrict__ p, const double *__restrict__ rho,
+                                const double *__restrict__ vel, double *__restrict__ gradRho, Scalars *sc) {
+  int i, j, k;
+  double ms = 0.0, mc = 0.0, mch = 0.0;
+  if (decode(r, i, j, k)) {
+    const long long N = g.N;
+    const double dx = ph.dx, dy = ph.dy;
+    auto R = [&](int a, int b) { return rho[g.id(a, b, 0)]; };
+    auto U = [&](int a, int b, int w) { return vel[g.id(a, b, 0) + N * w]; };
+    auto E = [&](int a, int b) {
+      long long q = g.id(a, b, 0);
+      return var[q + 2 * N] / rho[q] - 0.5 * rho[q] * (vel[q] * vel[q] + vel[q + N] * vel[q + N]);
+    };
+    double dxr = (R(i - 2, j) - 8.0 * R(i - 1, j) + 8.0 * R(i + 1, j) - R(i + 2, j)) / (12.0 * dx);
+    double dyr = (R(i, j - 2) - 8.0 * R(i, j - 1) + 8.0 * R(i, j + 1) - R(i, j + 2)) / (12.0 * dy);
+    double dxe = (E(i - 2, j) - 8.0 * E(i - 1, j) + 8.0 * E(i + 1, j) - E(i + 2, j)) / (12.0 * dx);
+    double dye = (E(i, j - 2) - 8.0 * E(i, j - 1) + 8.0 * E(i, j + 1) - E(i, j + 2)) / (12.0 * dy);
+    double dxu = (U(i - 2, j, 0) - 8.0 * U(i - 1, j, 0) + 8.0 * U(i + 1, j, 0) - U(i + 2, j, 0)) / (12.0 * dx);
+    double dyv = (U(i, j - 2, 1) - 8.0 * U(i, j - 1, 1) + 8.0 * U(i, j + 1, 1) - U(i, j + 2, 1)) / (12.0 * dy);
+    double rgrad = sqrt(dxr * dxr + dyr * dyr);
+    double divu = dxu + dyv;
+    double dnednr = (dxr * dxe + dyr * dye) * (dxr * dxr + dyr * dyr);
+    double i1 = dnednr < 0.0 ? 1.0 : 0.0;
+    double i2 = divu < 0.0 ? 1.0 : 0.0;
+    const long long c = g.id(i, j, 0);
+    gradRho[c] = (1 - i1) * i2 * rgrad;
+    gradRho[c + N] = i1 * rgrad;
+    gradRho[c + 2 * N] = -dyr;
+    gradRho[c + 3 * N] = dxr;
+    double Cp = 0, Cv = 0;
+    for (int s = 0; s < g.ns; ++s) {
+      double gammas = ph.cd[6 + 2 * s], Rs = ph.cd[6 + 2 * s + 1];
+      Cp = Cp + (var[c + N * (3 + s)] / rho[c]) * (gammas * Rs / (gammas - 1));
+      Cv = Cv + (var[c + N * (3 + s)] / rho[c]) * (Rs / (gammas - 1));
+    }
+    double gamma = Cp / Cv;
+    double a = sqrt(gamma * p[c] / rho[c]);
+    ms = a + sqrt(var[c] * var[c] + var[c + N] * var[c + N]);
+    mc = fabs(var[c + N * g.nv]);
+    mch = fabs(var[c + N * (g.nv + 1)]);
+  }
+  ms = wave_max(ms);
+  mc = wave_max(mc);
+  mch = wave_max(mch);
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_nonneg(&sc->maxS, ms);
+    atomic_max_nonneg(&sc->maxC, mc);
+    atomic_max_nonneg(&sc->maxCh, mch);
+  }
+}
+
+// updateCeq2D, src/ceq.hpp:153-195
+__global__ void k_updateceq2d(Geo g, Phys ph, Range r, const double *__restrict__ var,
+                              const double *__restrict__ gradRho, const Scalars *sc, double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const double dx = ph.cd[1], dy = ph.cd[2];
+  const int nc = g.nv;
+  const double maxS = as_double(sc->maxS);
+  const double dxmag = sqrt(dx * dx + dy * dy);
+  const long long c = g.id(i, j, 0);
+  for (int n = 0; n < 4; ++n) {
+    const double *C = var + N * (nc + n);
+    double dx_right = (C[g.id(i + 1, j, 0)] - C[c]) / dx;
+    double dx_left = (C[c] - C[g.id(i - 1, j, 0)]) / dx;
+    double dy_top = (C[g.id(i, j + 1, 0)] - C[c]) / dy;
+    double dy_bot = (C[c] - C[g.id(i, j - 1, 0)]) / dy;
+    double lap = dx_right - dx_left + dy_top - dy_bot;
+    dvar[c + N * (nc + n)] = (maxS / (ph.eps * dxmag)) * (gradRho[c + N * n] - C[c]) + ph.kap * maxS * dxmag * lap;
+  }
+}
+
+// computeCeqFlux2D (src/ceq.hpp:197-245) and computeCeqFaces2D (:247-267)
+// fused per face; m planes ((f*2+d)*2+w)
+__global__ void k_ceqfaces2d(Geo g, Phys ph, Range r, const double *__restrict__ var, const double *__restrict__ rho,
+                             const double *__restrict__ vel, const Scalars *sc, double *__restrict__ m) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const int nv = g.nv;
+  const double alpha = (ph.dx * ph.dx + ph.dy * ph.dy) / (as_double(sc->maxCh) + 1.0e-6) * ph.alpha;
+  const double dx = ph.cd[1], dy = ph.cd[2];
+  auto U = [&](int a, int b, int w) { return vel[g.id(a, b, 0) + N * w]; };
+  const long long c = g.id(i, j, 0);
+  double X[2][2];
+  for (int f = 0; f < 2; ++f) {
+    const long long q = g.id(i + (f == 0), j + (f == 1), 0);
+    double rr = (rho[c] + rho[q]) / 2.0;
+    double chat = (var[c + N * (nv + 1)] + var[q + N * (nv + 1)]) / 2.0;
+    double ct[2];
+    ct[0] = (var[c + N * (nv + 2)] + var[q + N * (nv + 2)]) / 2.0;
+    ct[1] = (var[c + N * (nv + 3)] + var[q + N * (nv + 3)]) / 2.0;
+    double ctmag = (ct[0] * ct[0] + ct[1] * ct[1]) + 1.0e-6;
+    for (int d = 0; d < 2; ++d) X[f][d] = alpha * rr * chat * (ct[f] * ct[d]) / ctmag;
+  }
+  for (int w = 0; w < 2; ++w) {
+    m[c + N * ((0 * 2 + 0) * 2 + w)] = X[0][0] * ((U(i + 1, j, w) - U(i, j, w)) / dx);
+    m[c + N * ((0 * 2 + 1) * 2 + w)] = X[0][1] * (((U(i, j + 1, w) + U(i + 1, j + 1, w)) - (U(i, j - 1, w) + U(i + 1, j - 1, w))) / (4.0 * dy));
+    m[c + N * ((1 * 2 + 0) * 2 + w)] = X[1][0] * (((U(i + 1, j, w) + U(i + 1, j + 1, w)) - (U(i - 1, j, w) + U(i - 1, j + 1, w))) / (4.0 * dx));
+    m[c + N * ((1 * 2 + 1) * 2 + w)] = X[1][1] * ((U(i, j + 1, w) - U(i, j, w)) / dy);
+  }
+}
+
+// applyCeq2D, src/ceq.hpp:269-293
+__global__ void k_applyceq2d(Geo g, Phys ph, Range r, const double *__restrict__ m, double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const long long c = g.id(i, j, 0), cl = g.id(i - 1, j, 0), cb = g.id(i, j - 1, 0);
+  for (int w = 0; w < 2; ++w) {
+    auto M = [&](long long q, int f, int d) { return m[q + N * ((f * 2 + d) * 2 + w)]; };
+    double diffu = 0;
+    diffu += (M(c, 0, 0) - M(cl, 0, 0)) / ph.dx;
+    diffu += (M(c, 0, 1) - M(cl, 0, 1)) / ph.dx;
+    diffu += (M(c, 1, 0) - M(cb, 1, 0)) / ph.dy;
+    diffu += (M(c, 1, 1) - M(cb, 1, 1)) / ph.dy;
+    dvar[c + N * w] += diffu;
+  }
+}
+
+// ------------------------------------------------------------- ghosts / RK
+
+// bc_xPer / bc_yPer / bc_zPer, src/bc.cpp:133-176: one thread per (a, b, v)
+__global__ void k_bc_periodic(Geo g, int dir, double *__restrict__ var) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int na = dir == 0 ? g.ngj : g.ngi;
+  int nb = dir == 2 ? g.ngj : g.ngk;
+  long long n = (long long)na * nb * g.nvt;
+  if (t >= n) return;
+  int a = (int)(t % na);
+  t /= na;
+  int b = (int)(t % nb);
+  int v = (int)(t / nb);
+  double *u = var + g.N * v;
+  const int ng = g.ng;
+  for (int gg = 0; gg < ng; ++gg) {
+    if (dir == 0) {
+      u[g.id(gg, a, b)] = u[g.id(g.nci + gg, a, b)];
+      u[g.id(g.nci + ng + gg, a, b)] = u[g.id(ng + gg, a, b)];
+    } else if (dir == 1) {
+      u[g.id(a, gg, b)] = u[g.id(a, g.ncj + gg, b)];
+      u[g.id(a, g.ncj + ng + gg, b)] = u[g.id(a, ng + gg, b)];
+    } else {
+      u[g.id(a, b, gg)] = u[g.id(a, b, g.nck + gg)];
+      u[g.id(a, b, g.nck + ng + gg)] = u[g.id(a, b, ng + gg)];
+    }
+  }
+}
+
+// bc_gen, src/bc.cpp:33-131, on one boundary plane (i0 / j0 / k0 fixed)
+__global__ void k_bc_gen(Geo g, int type, int ih, int jh, int kh, int plane, double *__restrict__ var) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (ih != 0) {
+    if (t >= (long long)g.ngj * g.ngk) return;
+    i = plane; j = (int)(t % g.ngj); k = (int)(t / g.ngj);
+  } else if (jh != 0) {
+    if (t >= (long long)g.ngi * g.ngk) return;
+    j = plane; i = (int)(t % g.ngi); k = (int)(t / g.ngi);
+  } else {
+    if (t >= (long long)g.ngi * g.ngj) return;
+    k = plane; i = (int)(t % g.ngi); j = (int)(t / g.ngi);
+  }
+  const int ng = g.ng, nv = g.nvt;
+  auto U = [&](int a, int b, int c, int v) -> double & { return var[g.id(a, b, c) + g.N * v]; };
+  if (type == FST_BC_OUTFLOW) {
+    for (int v = 0; v < nv; ++v)
+      for (int n = 0; n < ng; ++n)
+        U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) = U(i - ih * n, j - jh * n, k - kh * n, v);
+  } else if (type == FST_BC_NOSLIP) {
+    for (int v = 0; v < nv; ++v) {
+      double reflect = (v == 0 || v == 1 || v == 2) ? -1.0 : 1.0;
+      for (int n = 0; n < ng; ++n)
+        U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) = reflect * U(i - ih * n, j - jh * n, k - kh * n, v);
+    }
+  } else {  // reflective, and the reflection pass of hydrostatic
+    for (int v = 0; v < nv; ++v) {
+      double reflect = 1.0;
+      if (ih != 0 && v == 0) reflect = -1.0;
+      if (jh != 0 && v == 1) reflect = -1.0;
+      if (kh != 0 && v == 2) reflect = -1.0;
+      for (int n = 0; n < ng; ++n)
+        U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) = reflect * U(i - ih * n, j - jh * n, k - kh * n, v);
+    }
+    if (type == FST_BC_HYDROSTATIC) {
+      const int edx = g.ndim, rdx = g.ndim + 1;
+      for (int n = 0; n < ng; ++n) {
+        int ig = i + ih * (n + 1), jg = j + jh * (n + 1), kg = k + kh * (n + 1);
+        int i1 = ig - ih, j1 = jg - jh, k1 = kg - kh;
+        int i2 = ig - 2 * ih, j2 = jg - 2 * jh, k2 = kg - 2 * kh;
+        double ke = 0, ke1 = 0, ke2 = 0;
+        for (int d = 0; d < g.ndim; ++d) {
+          ke += 0.5 * (U(ig, jg, kg, d) * U(ig, jg, kg, d));
+          ke1 += 0.5 * (U(i1, j1, k1, d) * U(i1, j1, k1, d));
+          ke2 += 0.5 * (U(i2, j2, k2, d) * U(i2, j2, k2, d));
+        }
+        ke = ke / U(ig, jg, kg, rdx);
+        ke1 = ke1 / U(i1, j1, k1, rdx);
+        ke2 = ke2 / U(i2, j2, k2, rdx);
+        U(ig, jg, kg, edx) = 2 * (U(i1, j1, k1, edx) - ke1) - (U(i2, j2, k2, edx) - ke2) + ke;
+      }
+    }
+  }
+}
+
+// clears the per-stage maxima (a kernel, not a memset node, so the sequence
+// stays capturable in a HIP graph)
+__global__ void k_zero_scalars(Scalars *sc) {
+  unsigned long long *w = reinterpret_cast<unsigned long long *>(sc);
+  if (threadIdx.x < 4) w[threadIdx.x] = 0ull;  // per-lane (vector) stores
+}
+
+// rkAdvance Loop1 / Loop2, src/rk.cpp:42-50 and :63-69, over every cell
+__global__ void k_rk_stage(long long n, int stage, double dt, double *__restrict__ var, double *__restrict__ tmp,
+                           const double *__restrict__ dvar) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  if (stage == 1) {
+    double u = var[t];
+    tmp[t] = u;
+    var[t] = u + 0.5f * dt * dvar[t];
+  } else {
+    var[t] = tmp[t] + dt * dvar[t];
+  }
+}
+
+// copyExtraVars2D/3D (src/secondary.hpp:25-59): varx = (vel..., p, T, rho)
+__global__ void k_copy_extra(long long N, int ndim, const double *__restrict__ vel, const double *__restrict__ p,
+                             const double *__restrict__ T, const double *__restrict__ rho, double *__restrict__ varx) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  for (int d = 0; d < ndim; ++d) varx[c + N * d] = vel[c + N * d];
+  varx[c + N * ndim] = p[c];
+  varx[c + N * (ndim + 1)] = T[c];
+  varx[c + N * (ndim + 2)] = rho[c];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+
+void launch_eos_vel(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s) {
+  Range r = ghost_range(g);
+  if (g.ndim == 3) k_eos_vel3d<<<grid_for(r), 256, 0, s>>>(g, ph, r, b.var, b.p, b.rho, b.T, b.vel);
+  else k_eos_vel2d<<<grid_for(r), 256, 0, s>>>(g, ph, r, b.var, b.p, b.rho, b.T, b.vel);
+}
+
+void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s) {
+  launch_eos_vel(g, ph, b, s);
+  Range cr = cell_range(g), fr = face_range(g);
+  if (g.ndim == 3) {
+    for (int v = 0; v < g.nv; ++v) {
+      k_flux3d<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, v, b.var, b.p, b.vel, b.fx, b.fy, b.fz);
+      k_advect3d<<<grid_for(cr), 256, 0, s>>>(g, cr, v, b.fx, b.fy, b.fz, b.dvar);
+    }
+    k_presgrad_buoy3d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, cf.buoyancy, b.var, b.p, b.rho, b.dvar);
+    if (cf.visc) {
+      for (int d = 0; d < 3; ++d) {
+        k_stress3d<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, d, b.var, b.rho, b.vel, b.stress);
+        k_applyvisc3d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, d, b.vel, b.stress, b.dvar);
+      }
+    }
+    if (cf.ceq) {
+      k_zero_scalars<<<1, 64, 0, s>>>(b.sc);
+      k_rhograd_max3d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.var, b.p, b.rho, b.vel, b.gradRho, b.sc);
+      k_updateceq3d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.var, b.gradRho, b.sc, b.dvar);
+      k_ceqfaces3d<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, b.var, b.rho, b.vel, b.sc, b.mflux);
+      k_applyceq3d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.mflux, b.dvar);
+    }
+  } else {
+    for (int v = 0; v < g.nv; ++v) {
+      k_flux2d<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, v, cf.scheme, b.var, b.p, b.rho, b.vel, b.fx, b.fy);
+      k_advect2d<<<grid_for(cr), 256, 0, s>>>(g, cr, v, b.fx, b.fy, b.dvar);
+    }
+    k_presgrad_buoy2d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, cf.buoyancy, b.var, b.p, b.rho, b.dvar);
+    if (cf.visc) {
+      k_stress2d<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, b.vel, b.T, b.stress);
+      k_applyvisc2d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.vel, b.stress, b.dvar);
+    }
+    if (cf.ceq) {
+      k_zero_scalars<<<1, 64, 0, s>>>(b.sc);
+      k_rhograd_max2d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.var, b.p, b.rho, b.vel, b.gradRho, b.sc);
+      k_updateceq2d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.var, b.gradRho, b.sc, b.dvar);
+      k_ceqfaces2d<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, b.var, b.rho, b.vel, b.sc, b.mflux);
+      k_applyceq2d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.mflux, b.dvar);
+    }
+  }
+}
+
+void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream_t s) {
+  const int bs = 256;
+  if (cf.xPer) {
+    long long n = (long long)g.ngj * g.ngk * g.nvt;
+    k_bc_periodic<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, 0, var);
+  }
+  if (cf.yPer) {
+    long long n = (long long)g.ngi * g.ngk * g.nvt;
+    k_bc_periodic<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, 1, var);
+  }
+  if (g.ndim == 3 && cf.zPer) {
+    long long n = (long long)g.ngi * g.ngj * g.nvt;
+    k_bc_periodic<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, 2, var);
+  }
+  const int ng = g.ng;
+  if (!cf.xPer) {
+    long long n = (long long)g.ngj * g.ngk;
+    k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcL, -1, 0, 0, ng, var);
+    k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcR, 1, 0, 0, ng + g.nci - 1, var);
+  }
+  if (!cf.yPer) {
+    long long n = (long long)g.ngi * g.ngk;
+    k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcB, 0, -1, 0, ng, var);
+    k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcT, 0, 1, 0, ng + g.ncj - 1, var);
+  }
+  if (g.ndim == 3 && !cf.zPer) {
+    long long n = (long long)g.ngi * g.ngj;
+    k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcH, 0, 0, -1, ng, var);
+    k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcF, 0, 0, 1, ng + g.nck - 1, var);
+  }
+}
+
+void launch_rk_stage(const Geo &g, int stage, double dt, double *var, double *tmp, const double *dvar, hipStream_t s) {
+  long long n = g.N * g.nvt;
+  k_rk_stage<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, stage, dt, var, tmp, dvar);
+}
+
+void launch_secondary(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s) {
+  launch_eos_vel(g, ph, b, s);
+  k_copy_extra<<<(unsigned)((g.N + 255) / 256), 256, 0, s>>>(g.N, g.ndim, b.vel, b.p, b.T, b.rho, b.varx);
+}
+
+}  // namespace fst

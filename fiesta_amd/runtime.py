@@ -1,0 +1,157 @@
+"""ctypes binding of the C-ABI in include/fiesta_hip.h (libfiesta_hip.so).
+
+`Module` mirrors FIESTA's `rk_func` (src/rkfunction.hpp:36-69): compute(),
+pre_step(), post_step(), pre_sim(), post_sim(), plus the array accessors the
+reference's writers use (var, varx, dvar).  There is no CPU fallback: without
+the built library every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .config import (Dims, FST_VAR, FST_TMP1, FST_DVAR, FST_VARX, FST_P, FST_RHO, FST_T, FST_VEL,
+                     FST_GRADRHO)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfiesta_hip.so")
+_LIB = None
+
+_EXPORTS = {
+    "fst_abi_version": ([], C.c_int),
+    "fst_config_init": ([C.c_void_p], None),
+    "fst_config_dims": ([C.c_void_p, C.c_void_p], C.c_int),
+    "fst_create": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
+    "fst_destroy": ([C.c_void_p], C.c_int),
+    "fst_last_error": ([C.c_void_p], C.c_char_p),
+    "fst_get_dims": ([C.c_void_p, C.c_void_p], C.c_int),
+    "fst_upload": ([C.c_void_p, C.c_int, C.c_void_p, C.c_size_t], C.c_int),
+    "fst_download": ([C.c_void_p, C.c_int, C.c_void_p, C.c_size_t], C.c_int),
+    "fst_pre_sim": ([C.c_void_p], C.c_int),
+    "fst_pre_step": ([C.c_void_p], C.c_int),
+    "fst_apply_bcs": ([C.c_void_p], C.c_int),
+    "fst_compute": ([C.c_void_p], C.c_int),
+    "fst_rk_stage": ([C.c_void_p, C.c_int], C.c_int),
+    "fst_post_step": ([C.c_void_p], C.c_int),
+    "fst_step": ([C.c_void_p, C.c_int], C.c_int),
+    "fst_last_maxima": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+    "fst_sync": ([C.c_void_p], C.c_int),
+    "fst_set_fused": ([C.c_void_p, C.c_int], C.c_int),
+    "fst_last_step_ms": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+}
+
+
+class FiestaError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libfiesta_hip.so; raise if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise FiestaError("libfiesta_hip.so is not built: run __graft_entry__.build() "
+                              "or `python -m fiesta_amd.build` (there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _EXPORTS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    return list(_EXPORTS)
+
+
+class Module:
+    """A cart2d/cart3d module instance on the GPU (one HIP stream)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self._cfg_keep = cfg
+        self.dims = Dims(cfg)
+        h = C.c_void_p()
+        rc = lib().fst_create(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise FiestaError("fst_create failed (code %d)" % rc)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fst_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = lib().fst_last_error(self.h)
+            raise FiestaError("%s failed (code %d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def _nvar(self, array):
+        d = self.dims
+        return {FST_VAR: d.nvt, FST_TMP1: d.nvt, FST_DVAR: d.nvt, FST_VARX: d.nvx, FST_P: 1, FST_RHO: 1,
+                FST_T: 1, FST_VEL: d.ndim, FST_GRADRHO: 5 if d.ndim == 3 else 4}[array]
+
+    def upload(self, array, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._chk(lib().fst_upload(self.h, array, a.ctypes.data, a.size), "fst_upload")
+
+    def download(self, array):
+        out = np.empty(self.dims.shape(self._nvar(array)), dtype=np.float64)
+        self._chk(lib().fst_download(self.h, array, out.ctypes.data, out.size), "fst_download")
+        return out
+
+    def pre_sim(self):
+        self._chk(lib().fst_pre_sim(self.h), "fst_pre_sim")
+
+    def pre_step(self):
+        self._chk(lib().fst_pre_step(self.h), "fst_pre_step")
+
+    def apply_bcs(self):
+        self._chk(lib().fst_apply_bcs(self.h), "fst_apply_bcs")
+
+    def compute(self):
+        self._chk(lib().fst_compute(self.h), "fst_compute")
+
+    def rk_stage(self, stage):
+        self._chk(lib().fst_rk_stage(self.h, stage), "fst_rk_stage")
+
+    def post_step(self):
+        self._chk(lib().fst_post_step(self.h), "fst_post_step")
+
+    def post_sim(self):
+        pass
+
+    def step(self, n=1):
+        self._chk(lib().fst_step(self.h, n), "fst_step")
+
+    def set_fused(self, on):
+        self._chk(lib().fst_set_fused(self.h, int(on)), "fst_set_fused")
+
+    def sync(self):
+        self._chk(lib().fst_sync(self.h), "fst_sync")
+
+    def last_step_ms(self):
+        v = C.c_double()
+        self._chk(lib().fst_last_step_ms(self.h, C.byref(v)), "fst_last_step_ms")
+        return v.value
+
+    def maxima(self):
+        a, b = C.c_double(), C.c_double()
+        self._chk(lib().fst_last_maxima(self.h, C.byref(a), C.byref(b)), "fst_last_maxima")
+        return a.value, b.value
+
+
+def rk_advance(f):
+    """rkAdvance (src/rk.cpp:28-72): two compute() calls, two stage updates."""
+    f.compute()
+    f.rk_stage(1)
+    f.compute()
+    f.rk_stage(2)

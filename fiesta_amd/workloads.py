@@ -1,0 +1,191 @@
+"""Synthetic initial states for the RHS/RK path.
+
+Initial conditions fill the interior only; ghosts stay zero until the first
+ghost fill (src/input.cpp:455-511).  States are numpy arrays of shape
+(nvt, ngk, ngj, ngi): the LayoutLeft (i,j,k,v) order of the device arrays.
+Seeds use numpy's PCG64 so a workload is reproducible from its seed alone.
+"""
+import math
+
+import numpy as np
+
+from .config import Dims, make_config
+
+AIR = dict(gamma=1.4, M=0.02896, mu=1.8e-5)
+HOT = dict(gamma=1.67, M=0.004, mu=1.8e-5)
+SF6 = dict(gamma=1.092, M=0.14606, mu=1.8e-5)
+RU = 8.314462
+
+
+def _mix(ns, specs):
+    specs = list(specs)[:ns]
+    return dict(gamma=tuple(s["gamma"] for s in specs), M=tuple(s["M"] for s in specs),
+                mu=tuple(s["mu"] for s in specs))
+
+
+def _cells(d):
+    """Cell-centre coordinates (interior) on the unit box, shapes broadcastable to (k,j,i)."""
+    x = (np.arange(d.nci) + 0.5) / d.nci
+    y = (np.arange(d.ncj) + 0.5) / d.ncj
+    z = (np.arange(d.nck) + 0.5) / d.nck
+    return x[None, None, :], y[None, :, None], z[:, None, None]
+
+
+def fill_conserved(cfg, rho_s, u, p, cvars=None):
+    """Pack primitive fields (interior, shape (k,j,i)) into a conserved state array.
+
+    rho_s: list of species densities; u: list of ndim velocity components;
+    p: pressure.  Energy uses the mixture gamma of src/secondary.hpp:128-141.
+    """
+    d = Dims(cfg)
+    var = np.zeros(d.shape(d.nvt), dtype=np.float64)
+    rho = sum(rho_s)
+    Cp = 0.0
+    Cv = 0.0
+    for s in range(cfg.ns):
+        g = cfg.gamma[s]
+        Rs = cfg.R / cfg.M[s]
+        Cp = Cp + (rho_s[s] / rho) * (g * Rs / (g - 1))
+        Cv = Cv + (rho_s[s] / rho) * (Rs / (g - 1))
+    gam = Cp / Cv
+    ke = 0.5 * rho * sum(ui * ui for ui in u)
+    E = p / (gam - 1) + ke
+    sl = d.interior()[1:]
+    for a in range(cfg.ndim):
+        var[(a,) + sl] = rho * u[a]
+    var[(cfg.ndim,) + sl] = E
+    for s in range(cfg.ns):
+        var[(cfg.ndim + 1 + s,) + sl] = rho_s[s]
+    if cvars is not None and d.nvt > d.nv:
+        for n, cv in enumerate(cvars):
+            var[(d.nv + n,) + sl] = cv
+    return var
+
+
+def random_state(cfg, seed=0, amp=0.2):
+    """Physically valid random state with per-cell noise (exercises every branch)."""
+    rng = np.random.default_rng(seed)
+    d = Dims(cfg)
+    sh = (d.nck, d.ncj, d.nci)
+    base_rho = [1.2, 0.16, 6.0][: cfg.ns] + [1.0] * max(0, cfg.ns - 3)
+    rho_s = [b * rng.uniform(0.3, 1.0, sh) for b in base_rho[: cfg.ns]]
+    p = 1.0e5 * rng.uniform(1 - amp, 1 + amp, sh)
+    u = [rng.normal(0.0, 60.0, sh) for _ in range(cfg.ndim)]
+    cv = None
+    if cfg.ceq:
+        cv = [np.abs(rng.normal(0, 5.0, sh)), np.abs(rng.normal(0, 5.0, sh))] + \
+             [rng.normal(0, 5.0, sh) for _ in range(3)]
+    return fill_conserved(cfg, rho_s, u, p, cv)
+
+
+def sod_config(nci=1000, ncj=4, dt=2e-4, ceq=True):
+    """Sod shock tube, 2D Cartesian 1000x4 on [0,1]x[0,0.004], one ideal gas
+    gamma=1.4, outflow x / periodic y, WENO5 + C-equation (kappa 10, eps 1,
+    alpha 2 as the reference's 2dSod deck)."""
+    return make_config(ndim=2, ns=1, nci=nci, ncj=ncj, dx=1.0 / nci, dy=1.0 / nci, R=RU,
+                       gamma=(1.4,), M=(0.02897,), mu=(2.928e-5,), ceq=ceq, kap=10.0, eps=1.0,
+                       alpha=2.0, yper=True, bc=("outflow", "outflow", "outflow", "outflow", "outflow", "outflow"),
+                       dt=dt)
+
+
+def sod_state(cfg):
+    d = Dims(cfg)
+    x, y, z = _cells(d)
+    sh = (1, d.ncj, d.nci)
+    left = np.broadcast_to(x < 0.5, sh)
+    rho = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.1)
+    u = [np.zeros(sh), np.zeros(sh)]
+    return fill_conserved(cfg, [rho], u, p, [np.zeros(sh)] * 5 if cfg.ceq else None)
+
+
+def expansion2d_config(n=1024, ceq=True, visc=True, ns=2):
+    """2D ideal expansion: 1 m box, n^2 cells, air + hot gas, Gaussian
+    high-pressure blob (50 atm, radius 0.05) in 1 atm, outflow on all sides,
+    dt from CFL 0.5 at t=0."""
+    mix = _mix(ns, [AIR, HOT, SF6])
+    cfg = make_config(ndim=2, ns=ns, nci=n, ncj=n, dx=1.0 / n, dy=1.0 / n, R=RU, visc=visc, ceq=ceq,
+                      kap=10.0, eps=1.0, alpha=2.0, bc=("outflow",) * 6, dt=1.0, **mix)
+    cfg.dt = cfl_dt(cfg, expansion2d_state(cfg), 0.5)
+    return cfg
+
+
+def expansion2d_state(cfg, T0=300.0, r0=0.05):
+    d = Dims(cfg)
+    x, y, _ = _cells(d)
+    r2 = (x - 0.5) ** 2 + (y - 0.5) ** 2
+    yh = np.exp(-r2 / r0 ** 2)[None, :, :] * np.ones((1, 1, 1))
+    p = 101325.0 * (1.0 + 49.0 * yh)
+    Rm = (1 - yh) * cfg.R / cfg.M[0] + yh * cfg.R / cfg.M[1]
+    rho = p / (Rm * T0)
+    rho_s = [rho * (1 - yh), rho * yh]
+    sh = rho.shape
+    return fill_conserved(cfg, rho_s, [np.zeros(sh), np.zeros(sh)], p,
+                          [np.zeros(sh)] * 5 if cfg.ceq else None)
+
+
+def rt3d_config(n=256, ceq=True, visc=True, seed=3, steps_dt=None):
+    """3D Rayleigh-Taylor: 1 m cube, n^3 cells, heavy air (M=28.96 g/mol)
+    over light gas (M=4 g/mol), gravity 9.81 along -y, x/z periodic, y
+    hydrostatic, viscous (mu 1.8e-5), C-equation (kappa 10, eps 5, alpha 20
+    as the reference's rt3d deck), dt from CFL 0.5 at t=0."""
+    mix = _mix(2, [AIR, HOT])
+    cfg = make_config(ndim=3, ns=2, nci=n, ncj=n, nck=n, dx=1.0 / n, dy=1.0 / n, dz=1.0 / n, R=RU,
+                      visc=visc, ceq=ceq, buoyancy=True, g=9.81, rhoRef=0.0, kap=10.0, eps=5.0, alpha=20.0,
+                      xper=True, zper=True, bc=("outflow", "outflow", "hydrostatic", "hydrostatic",
+                                                "outflow", "outflow"), dt=1.0, **mix)
+    # c_max = sqrt(gamma_light * p / rho_light) at p ~ 1e5 Pa, T0 = 300 K
+    c_light = math.sqrt(1.67 * RU / 0.004 * 300.0)
+    cfg.dt = 0.5 * cfg.dx / c_light if steps_dt is None else steps_dt
+    return cfg
+
+
+def rt3d_state(cfg, seed=3, T0=300.0, p_top=1.0e5, nmodes=16):
+    """Interface y = 0.5 + eta(x,z), eta = sum of 16 Fourier modes with
+    amplitudes N(0, 1e-3), distinct wavenumbers in {1..8}^2, phases U[0,2pi)."""
+    d = Dims(cfg)
+    rng = np.random.default_rng(seed)
+    x, y, z = _cells(d)
+    ks = rng.permutation(64)[:nmodes]
+    eta = np.zeros((d.nck, 1, d.nci))
+    for kk in ks:
+        kx, kz = 1 + kk % 8, 1 + kk // 8
+        a = rng.normal(0.0, 1.0e-3)
+        ph = rng.uniform(0.0, 2 * math.pi)
+        eta = eta + a * np.cos(2 * math.pi * (kx * x + kz * z) + ph)
+    h = 0.5 + eta
+    g = cfg.gAccel
+    rho_h = p_top / (cfg.R / cfg.M[0] * T0)
+    rho_l = p_top / (cfg.R / cfg.M[1] * T0)
+    heavy = y > h
+    p = np.where(heavy, p_top + rho_h * g * (1.0 - y), p_top + rho_h * g * (1.0 - h) + rho_l * g * (h - y))
+    sh = p.shape
+    rh = np.where(heavy, rho_h, 0.0) * np.ones(sh)
+    rl = np.where(heavy, 0.0, rho_l) * np.ones(sh)
+    # trace amounts keep both species present everywhere (mass fractions in (0,1))
+    rh = rh + 1e-6 * rho_h
+    rl = rl + 1e-6 * rho_l
+    zero = np.zeros(sh)
+    return fill_conserved(cfg, [rh, rl], [zero, zero, zero], p, [zero] * 5 if cfg.ceq else None)
+
+
+def cfl_dt(cfg, var, cfl):
+    """dt = cfl * min(d) / max(|u| + c) over the interior of `var`."""
+    d = Dims(cfg)
+    sl = d.interior()
+    v = var[sl]
+    rho_s = v[cfg.ndim + 1: cfg.ndim + 1 + cfg.ns]
+    rho = rho_s.sum(axis=0)
+    Cp = 0.0
+    Cv = 0.0
+    for s in range(cfg.ns):
+        g = cfg.gamma[s]
+        Rs = cfg.R / cfg.M[s]
+        Cp = Cp + rho_s[s] / rho * (g * Rs / (g - 1))
+        Cv = Cv + rho_s[s] / rho * (Rs / (g - 1))
+    gam = Cp / Cv
+    m2 = sum(v[a] ** 2 for a in range(cfg.ndim))
+    p = (gam - 1) * (v[cfg.ndim] - 0.5 / rho * m2)
+    c = np.sqrt(gam * p / rho) + np.sqrt(m2) / rho
+    dmin = min(cfg.dx, cfg.dy, cfg.dz if cfg.ndim == 3 else cfg.dy)
+    return float(cfl * dmin / c.max())

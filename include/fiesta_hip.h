@@ -1,0 +1,176 @@
+/*
+ * fiesta_hip.h -- C-ABI of the MI355X-native FIESTA right-hand-side and
+ * Runge-Kutta path (gfx950, hand-written HIP kernels).
+ *
+ * The entry points replace the device side of FIESTA's `rk_func` interface:
+ * a host `rk_func` subclass (cart2d / cart3d) keeps its compute() / preStep()
+ * / postStep() / preSim() / postSim() methods and forwards each one to the
+ * function named below.  Plain pointers and sizes only; every function
+ * returns 0 on success and a nonzero FST_ERR_* code otherwise, with a message
+ * available from fst_last_error().  No C++ exception crosses this boundary.
+ *
+ * Reference interfaces (paths relative to the FIESTA source tree):
+ *   class rk_func                      src/rkfunction.hpp:36-69
+ *   cart3d_func / cart2d_func          src/cart3d.cpp:44-378, src/cart2d.cpp:30-311
+ *   rkAdvance(cf, f)                   src/rk.hpp:26, src/rk.cpp:28-72
+ *   applyBCs(cf, f)                    src/bc.hpp:33, src/bc.cpp:179-249
+ *   struct inputConfig (fields used)   src/input.hpp:59-148
+ *
+ * Array layout (device and host): Kokkos LayoutLeft of the reference GPU
+ * build (src/kokkosTypes.hpp:40): element (i,j,k,v) of an array with
+ * extents (ngi,ngj,ngk,nv) sits at  i + ngi*(j + ngj*(k + ngk*v)),
+ * i.e. one contiguous plane per variable, i fastest, ghosts included.
+ */
+#ifndef FIESTA_HIP_H
+#define FIESTA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FST_MAX_SPECIES 8
+#define FST_ABI_VERSION 1
+
+/* Boundary types, in the order of `enum class BCType` (src/bc.hpp:31). */
+enum fst_bc_type {
+  FST_BC_OUTFLOW = 0,
+  FST_BC_REFLECTIVE = 1,
+  FST_BC_NOSLIP = 2,
+  FST_BC_HYDROSTATIC = 3
+};
+
+/* Advection schemes, numbered as cf.scheme (src/input.cpp:317-323). */
+enum fst_scheme { FST_WENO5 = 1, FST_CENTERED4 = 2, FST_QUICK = 3 };
+
+/* Arithmetic forms of the path. */
+enum fst_precision {
+  FST_FP64 = 0,      /* reference arithmetic, bitwise parity target */
+  FST_FP32_BULK = 1  /* FP32 stencils/fluxes, FP64 EOS and RK accumulation */
+};
+
+/* Arrays reachable through fst_upload / fst_download.  Extents in brackets. */
+enum fst_array {
+  FST_VAR = 0,     /* conserved state var      [ngi,ngj,ngk,nvt]        */
+  FST_TMP1 = 1,    /* RK stage copy tmp1       [ngi,ngj,ngk,nvt]        */
+  FST_DVAR = 2,    /* right-hand side dvar     [ngi,ngj,ngk,nvt]        */
+  FST_VARX = 3,    /* secondary vars varx      [ngi,ngj,ngk,nvx]        */
+  FST_P = 4,       /* pressure                 [ngi,ngj,ngk]            */
+  FST_RHO = 5,     /* total density            [ngi,ngj,ngk]            */
+  FST_T = 6,       /* temperature              [ngi,ngj,ngk]            */
+  FST_VEL = 7,     /* velocity                 [ngi,ngj,ngk,ndim]       */
+  FST_GRADRHO = 8, /* C-eq source terms        [ngi,ngj,ngk,5 | 4 in 2D] */
+  FST_GRID = 9,    /* grid nodes (generalized) [ni,nj,nk,ndim]          */
+  FST_METRICS = 10 /* inverse Jacobian         [ngi,ngj,ngk,ndim,ndim]  */
+};
+
+enum fst_err {
+  FST_OK = 0,
+  FST_ERR_ARG = 1,     /* bad argument / unsupported configuration */
+  FST_ERR_HIP = 2,     /* a HIP runtime call failed                */
+  FST_ERR_STATE = 3,   /* call out of order (e.g. no state loaded) */
+  FST_ERR_NOMEM = 4    /* device allocation failed                 */
+};
+
+/*
+ * Run configuration: the subset of `struct inputConfig` (src/input.hpp:59-148)
+ * that the RHS/RK path reads, with the defaults of executeConfiguration
+ * (src/input.cpp:130-443).  fst_config_init() fills those defaults.
+ */
+typedef struct fst_config {
+  int32_t ndim;          /* 2 or 3                                  (grid.ndim) */
+  int32_t grid;          /* 0 cartesian, 1 generalized               (cf.grid)  */
+  int32_t ns;            /* species count, 1..FST_MAX_SPECIES        (cf.ns)    */
+  int32_t ng;            /* ghost width, default 3                   (cf.ng)    */
+  int32_t nci, ncj, nck; /* interior cells; nck = 1 in 2D                       */
+  double dx, dy, dz;     /* cell sizes (1.0 on generalized grids)               */
+  double R;              /* universal gas constant, default 8.314462            */
+  double gamma[FST_MAX_SPECIES]; /* species ratio of specific heats              */
+  double M[FST_MAX_SPECIES];     /* species molar mass                           */
+  double mu[FST_MAX_SPECIES];    /* species viscosity                            */
+  int32_t visc;          /* viscosity.enabled                                   */
+  int32_t ceq;           /* ceq.enabled                                         */
+  int32_t buoyancy;      /* buoyancy.enabled                                    */
+  int32_t scheme;        /* fst_scheme, default FST_WENO5                       */
+  double kap, eps, alpha;/* ceq.kappa (1.23456), ceq.epsilon, ceq.alpha         */
+  double gAccel, rhoRef; /* buoyancy.acceleration (9.81), rho_reference (0)     */
+  int32_t xPer, yPer, zPer;           /* bc.{x,y,z}periodic                     */
+  int32_t bcL, bcR, bcB, bcT, bcH, bcF; /* bc.xmin,xmax,ymin,ymax,zmin,zmax     */
+  double dt;             /* time.dt                                             */
+  int32_t precision;     /* fst_precision                                       */
+  int32_t learned_c;     /* 1: learned C-indicator replaces the C-equation      */
+  int32_t reserved[8];
+} fst_config;
+
+typedef struct fst_ctx fst_ctx;
+
+/* Sizes derived from a config exactly as src/input.cpp:354-392 derives them. */
+typedef struct fst_dims {
+  int32_t ngi, ngj, ngk; /* extents with ghosts (ngk = 1 in 2D) */
+  int32_t nv;            /* ndim + 1 + ns                       */
+  int32_t nvt;           /* nv + 5 when the C-equation is on    */
+  int32_t nvx;           /* secondary variables: ndim + 3       */
+  int64_t ncells_ghost;  /* ngi*ngj*ngk                         */
+  int64_t ncells;        /* nci*ncj*nck                         */
+} fst_dims;
+
+int fst_abi_version(void);
+void fst_config_init(fst_config *cfg);
+int fst_config_dims(const fst_config *cfg, fst_dims *out);
+
+/* Allocate the device arrays of a module (cart3d_func ctor, src/cart3d.cpp:44-174;
+ * cart2d_func ctor, src/cart2d.cpp:30-134).  One HIP stream per context. */
+int fst_create(const fst_config *cfg, fst_ctx **out);
+int fst_destroy(fst_ctx *ctx);
+const char *fst_last_error(const fst_ctx *ctx);
+int fst_get_dims(const fst_ctx *ctx, fst_dims *out);
+
+/* Host <-> device copies of a whole array (layout above).  `count` is the
+ * number of doubles the host buffer holds; it must equal the array size.
+ * Both synchronize the context stream. */
+int fst_upload(fst_ctx *ctx, int array, const double *host, size_t count);
+int fst_download(fst_ctx *ctx, int array, double *host, size_t count);
+
+/* rk_func::preSim (src/cart3d.cpp:176-185, src/cart2d.cpp:136-145). */
+int fst_pre_sim(fst_ctx *ctx);
+/* rk_func::preStep: 2D applies the boundary conditions (src/cart2d.cpp:147-149),
+ * 3D does nothing (src/cart3d.cpp:187-188). */
+int fst_pre_step(fst_ctx *ctx);
+/* applyBCs (src/bc.cpp:179-249), no-MPI branch: periodic x,y,z then
+ * xmin,xmax,ymin,ymax,zmin,zmax planes. */
+int fst_apply_bcs(fst_ctx *ctx);
+/* rk_func::compute(): dvar = R(var), reference kernel sequence
+ * (src/cart3d.cpp:220-298, src/cart2d.cpp:151-237). */
+int fst_compute(fst_ctx *ctx);
+/* One Runge-Kutta stage update on the current dvar (src/rk.cpp:42-50 for
+ * stage 1, src/rk.cpp:63-69 for stage 2). */
+int fst_rk_stage(fst_ctx *ctx, int stage);
+/* rk_func::postStep: secondary variables varx (src/cart3d.cpp:300-317). */
+int fst_post_step(fst_ctx *ctx);
+
+/* `nsteps` full time steps: Fiesta::step (src/fiesta.cpp:333-340) =
+ * preStep + rkAdvance, on the fused fast path (one RHS+RK kernel per stage).
+ * Results equal the compute()/rk_stage() sequence. */
+int fst_step(fst_ctx *ctx, int nsteps);
+
+/* Per-stage global maxima of the C-equation (max wave speed, max |C_hat|)
+ * of the last computed stage (fsMax, src/cart3d.cpp:190-198, 281-283). */
+int fst_last_maxima(fst_ctx *ctx, double *maxS, double *maxCh);
+
+/* Block until all work queued on the context stream has finished. */
+int fst_sync(fst_ctx *ctx);
+
+/* Which implementation fst_step uses: 1 = fused kernels (default),
+ * 0 = the reference kernel sequence through fst_compute/fst_rk_stage. */
+int fst_set_fused(fst_ctx *ctx, int fused);
+
+/* Elapsed device milliseconds of the last fst_step call, measured with
+ * hipEvents on the context stream (Timer::fiestaTimer, src/timer.hpp:52-119). */
+int fst_last_step_ms(fst_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FIESTA_HIP_H */

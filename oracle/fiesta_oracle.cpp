@@ -1,0 +1,1197 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- the oracle.
+ *
+ * CPU restatement of FIESTA's right-hand-side evaluation, Runge-Kutta update
+ * and ghost-cell fill, written from the reference's kernels in the same
+ * arithmetic order (same association, same constants, no FMA contraction:
+ * build with -ffp-contract=off).  Only tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py may load it; the product never does.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the FIESTA source tree).  Loops are OpenMP-parallel over the slowest index;
+ * every value is a per-cell function and the only reductions are maxima, so
+ * the result is independent of the thread count.
+ *
+ * Pinning (what checks each function independently of this restatement):
+ *   apply_bcs       the reference's own known-answer tests
+ *                   (tests/bc_outflow.cpp, bc_reflective.cpp, bc_hydrostatic.cpp,
+ *                   tests/test.cpp:327-456) restated in tests/test_oracle_bc.py
+ *   eos / velocity  closed forms (ideal-gas single species, mass-fraction mixture)
+ *   weno3d/weno2d   constant data (binary32 coefficient sums), linear data,
+ *                   brute-force high-precision evaluation
+ *   advect+presgrad telescoping conservation on periodic domains
+ *   rk              amplification factor 1+z+z^2/2 on dU/dt = lambda*U
+ *   viscous         linear shear (uniform stress: zero divergence), quadratic
+ *                   profile with analytic divergence
+ *   C-equation      uniform state (zero sources), planar-shock switch logic,
+ *                   relaxation of a uniform C field
+ *   full 2D step    Sod shock tube against the exact Riemann solution
+ *   Parity unpinned (no independent check beyond the above): the exact
+ *   bit-level composition of the C-equation artificial-viscosity faces
+ *   (calculateCeqFaces/Grads/applyCeq) and the 2D Q7/Q6 quirk values.
+ */
+#include "fiesta_hip.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <cfloat>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct Orc {
+  fst_config c;
+  int ndim, ns, ng, nci, ncj, nck, ngi, ngj, ngk, nv, nvt, nvx;
+  size_t N;
+  std::vector<double> cd;  // device config array: src/cart3d.cpp:152-171
+  std::vector<double> var, tmp1, dvar, varx, p, rho, T, vel;
+  std::vector<double> fluxx, fluxy, fluxz;
+  std::vector<double> stress;            // 3D (ngi,ngj,ngk,6)
+  std::vector<double> stressx, stressy;  // 2D (ngi,ngj,2,2)
+  std::vector<double> qx, qy;            // 2D heat flux
+  std::vector<double> gradRho;           // (.,5) 3D, (.,4) 2D
+  std::vector<double> mFlux;             // 3D (3,3,ngi,ngj,ngk,3), 2D (2,2,ngi,ngj,2)
+  double dxmag;                          // src/cart3d.cpp:173 (no sqrt)
+  double maxS = 0, maxC = 0, maxCh = 0, alpha = 0;
+
+  size_t id(int i, int j, int k) const {
+    return (size_t)i + (size_t)ngi * ((size_t)j + (size_t)ngj * (size_t)k);
+  }
+  double &V(std::vector<double> &a, int i, int j, int k, int v) {
+    return a[id(i, j, k) + N * (size_t)v];
+  }
+  // mFlux(face,dir,i,j,k,w): LayoutLeft over (3,3,ngi,ngj,ngk,3)
+  double &MF(int f, int d, int i, int j, int k, int w) {
+    return mFlux[(size_t)f + 3 * ((size_t)d + 3 * (id(i, j, k) + N * (size_t)w))];
+  }
+  // 2D m(face,dir,i,j,w): (2,2,ngi,ngj,2)
+  double &M2(int f, int d, int i, int j, int w) {
+    return mFlux[(size_t)f + 2 * ((size_t)d + 2 * (id(i, j, 0) + N * (size_t)w))];
+  }
+};
+
+// ---------------------------------------------------------------- 3D kernels
+
+// calculateRhoPT3D, src/secondary.hpp:109-151
+void eos3d(Orc &o) {
+  const int ns = o.ns;
+#pragma omp parallel for
+  for (int k = 0; k < o.ngk; ++k)
+    for (int j = 0; j < o.ngj; ++j)
+      for (int i = 0; i < o.ngi; ++i) {
+        size_t c = o.id(i, j, k);
+        double r = 0.0;
+        for (int s = 0; s < ns; ++s) r = r + o.var[c + o.N * (4 + s)];
+        double Cp = 0, Cv = 0;
+        for (int s = 0; s < ns; ++s) {
+          double gammas = o.cd[6 + 3 * s];
+          double Rs = o.cd[6 + 3 * s + 1];
+          Cp = Cp + (o.var[c + o.N * (4 + s)] / r) * (gammas * Rs / (gammas - 1));
+          Cv = Cv + (o.var[c + o.N * (4 + s)] / r) * (Rs / (gammas - 1));
+        }
+        double gamma = Cp / Cv;
+        double m0 = o.var[c], m1 = o.var[c + o.N], m2 = o.var[c + 2 * o.N];
+        double pp = (gamma - 1) * (o.var[c + 3 * o.N] - (0.5 / r) * (m0 * m0 + m1 * m1 + m2 * m2));
+        o.rho[c] = r;
+        o.p[c] = pp;
+        o.T[c] = pp / ((Cp - Cv) * r);
+      }
+}
+
+// computeVelocity3D, src/velocity.hpp:68-83
+void vel3d(Orc &o) {
+#pragma omp parallel for
+  for (int k = 0; k < o.ngk; ++k)
+    for (int j = 0; j < o.ngj; ++j)
+      for (int i = 0; i < o.ngi; ++i) {
+        size_t c = o.id(i, j, k);
+        for (int d = 0; d < 3; ++d) o.vel[c + o.N * d] = o.var[c + o.N * d] / o.rho[c];
+      }
+}
+
+// calculateFluxesG::weno, src/flux.hpp:240-264 (binary32 constants)
+double weno3d(double f1, double f2, double f3, double f4, double f5) {
+  const double eps = 0.000001;  // src/flux.hpp:233
+  double b1, b2, b3, w1, w2, w3, p1, p2, p3, a1, a2;
+  a1 = f1 - 2.0f * f2 + f3;
+  a2 = f1 - 4.0f * f2 + 3.0f * f3;
+  b1 = (13.0f / 12.0f) * a1 * a1 + (0.25f) * a2 * a2;
+  a1 = f2 - 2.0 * f3 + f4;
+  a2 = f2 - f4;
+  b2 = (13.0f / 12.0f) * a1 * a1 + (0.25f) * a2 * a2;
+  a1 = f3 - 2.0 * f4 + f5;
+  a2 = 3.0 * f3 - 4.0 * f4 + f5;
+  b3 = (13.0f / 12.0f) * a1 * a1 + (0.25f) * a2 * a2;
+  a1 = eps + b1;
+  w1 = (0.1f) / (a1 * a1);
+  a1 = eps + b2;
+  w2 = (0.6f) / (a1 * a1);
+  a1 = eps + b3;
+  w3 = (0.3f) / (a1 * a1);
+  p1 = (1.0f / 3.0f) * f1 + (-7.0f / 6.0f) * f2 + (11.0f / 6.0f) * f3;
+  p2 = (-1.0f / 6.0f) * f2 + (5.0f / 6.0f) * f3 + (1.0f / 3.0f) * f4;
+  p3 = (1.0f / 3.0f) * f3 + (5.0f / 6.0f) * f4 + (-1.0f / 6.0f) * f5;
+  return (w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3);
+}
+
+// calculateFluxesG::operator(), src/flux.hpp:266-309, launched over the tile
+// range [ng-1, n-ng)^3 (src/cart3d.cpp:228-231, 245)
+void flux3d(Orc &o, int v) {
+  const int ng = o.ng;
+  const double dx = o.c.dx, dy = o.c.dy, dz = o.c.dz;
+  double *var = o.var.data() + o.N * v;
+  const double *p = o.p.data();
+  const double *vel = o.vel.data();
+  const size_t N = o.N;
+  const double isE = (double)(v == 3);
+#pragma omp parallel for
+  for (int k = ng - 1; k < o.ngk - ng; ++k)
+    for (int j = ng - 1; j < o.ngj - ng; ++j)
+      for (int i = ng - 1; i < o.ngi - ng; ++i) {
+        double ur, vr, wr, f1, f2, f3, f4, f5, f6;
+        auto q = [&](int ii, int jj, int kk) {
+          size_t c = o.id(ii, jj, kk);
+          return var[c] + isE * p[c];
+        };
+        auto u = [&](int ii, int jj, int kk, int d) { return vel[o.id(ii, jj, kk) + N * d]; };
+        size_t c = o.id(i, j, k);
+        f1 = q(i + 3, j, k); f2 = q(i + 2, j, k); f3 = q(i + 1, j, k);
+        f4 = q(i, j, k);     f5 = q(i - 1, j, k); f6 = q(i - 2, j, k);
+        ur = (-u(i + 2, j, k, 0) + 7.0f * u(i + 1, j, k, 0) + 7.0f * u(i, j, k, 0) - u(i - 1, j, k, 0)) / 12.0f;
+        if (ur < 0.0) o.fluxx[c] = ur * weno3d(f1, f2, f3, f4, f5) / dx;
+        else          o.fluxx[c] = ur * weno3d(f6, f5, f4, f3, f2) / dx;
+
+        f1 = q(i, j + 3, k); f2 = q(i, j + 2, k); f3 = q(i, j + 1, k);
+        f4 = q(i, j, k);     f5 = q(i, j - 1, k); f6 = q(i, j - 2, k);
+        vr = (-u(i, j + 2, k, 1) + 7.0f * u(i, j + 1, k, 1) + 7.0f * u(i, j, k, 1) - u(i, j - 1, k, 1)) / 12.0f;
+        if (vr < 0.0) o.fluxy[c] = vr * weno3d(f1, f2, f3, f4, f5) / dy;
+        else          o.fluxy[c] = vr * weno3d(f6, f5, f4, f3, f2) / dy;
+
+        f1 = q(i, j, k + 3); f2 = q(i, j, k + 2); f3 = q(i, j, k + 1);
+        f4 = q(i, j, k);     f5 = q(i, j, k - 1); f6 = q(i, j, k - 2);
+        wr = (-u(i, j, k + 2, 2) + 7.0f * u(i, j, k + 1, 2) + 7.0f * u(i, j, k, 2) - u(i, j, k - 1, 2)) / 12.0f;
+        if (wr < 0.0) o.fluxz[c] = wr * weno3d(f1, f2, f3, f4, f5) / dz;
+        else          o.fluxz[c] = wr * weno3d(f6, f5, f4, f3, f2) / dz;
+      }
+}
+
+// advect3D, src/advect.hpp:39-61 (assignment into dvar)
+void advect3d(Orc &o, int v) {
+  const int ng = o.ng;
+#pragma omp parallel for
+  for (int k = ng; k < o.ngk - ng; ++k)
+    for (int j = ng; j < o.ngj - ng; ++j)
+      for (int i = ng; i < o.ngi - ng; ++i) {
+        double res = -((o.fluxx[o.id(i, j, k)] - o.fluxx[o.id(i - 1, j, k)]) +
+                       (o.fluxy[o.id(i, j, k)] - o.fluxy[o.id(i, j - 1, k)]) +
+                       (o.fluxz[o.id(i, j, k)] - o.fluxz[o.id(i, j, k - 1)]));
+        o.V(o.dvar, i, j, k, v) = res;
+      }
+}
+
+// applyPressureGradient3D, src/presgrad.hpp:44-71 (dz := cd(2), :56)
+void presgrad3d(Orc &o) {
+  const int ng = o.ng;
+  const double dx = o.cd[1], dy = o.cd[2], dz = o.cd[2];
+#pragma omp parallel for
+  for (int k = ng; k < o.ngk - ng; ++k)
+    for (int j = ng; j < o.ngj - ng; ++j)
+      for (int i = ng; i < o.ngi - ng; ++i) {
+        auto P = [&](int a, int b, int cc) { return o.p[o.id(a, b, cc)]; };
+        double dxp = (P(i - 2, j, k) - 8.0 * P(i - 1, j, k) + 8.0 * P(i + 1, j, k) - P(i + 2, j, k)) / (12.0 * dx);
+        double dyp = (P(i, j - 2, k) - 8.0 * P(i, j - 1, k) + 8.0 * P(i, j + 1, k) - P(i, j + 2, k)) / (12.0 * dy);
+        double dzp = (P(i, j, k - 2) - 8.0 * P(i, j, k - 1) + 8.0 * P(i, j, k + 1) - P(i, j, k + 2)) / (12.0 * dz);
+        o.V(o.dvar, i, j, k, 0) = o.V(o.dvar, i, j, k, 0) - dxp;
+        o.V(o.dvar, i, j, k, 1) = o.V(o.dvar, i, j, k, 1) - dyp;
+        o.V(o.dvar, i, j, k, 2) = o.V(o.dvar, i, j, k, 2) - dzp;
+      }
+}
+
+// computeBuoyancy3D, src/buoyancy.hpp:22-47
+void buoyancy3d(Orc &o) {
+  const int ng = o.ng;
+  const double g = o.c.gAccel, rhoRef = o.c.rhoRef;
+#pragma omp parallel for
+  for (int k = ng; k < o.ngk - ng; ++k)
+    for (int j = ng; j < o.ngj - ng; ++j)
+      for (int i = ng; i < o.ngi - ng; ++i) {
+        size_t c = o.id(i, j, k);
+        double v = o.var[c + o.N] / o.rho[c];
+        double rhop = o.rho[c] - rhoRef;
+        double f = -g * rhop;
+        o.dvar[c + o.N] += f;
+        o.dvar[c + 3 * o.N] += v * f;
+      }
+}
+
+// calculateStressTensor{x,y,z}3dv, src/viscosity.hpp:416-459, 499-541, 581-624,
+// launched over the weno range (src/cart3d.cpp:265-270)
+void stress3d(Orc &o, int dir) {
+  const int ng = o.ng, ns = o.ns;
+  const double dx = o.cd[1], dy = o.cd[2], dz = o.cd[3];
+  const size_t N = o.N;
+#pragma omp parallel for
+  for (int k = ng - 1; k < o.ngk - ng; ++k)
+    for (int j = ng - 1; j < o.ngj - ng; ++j)
+      for (int i = ng - 1; i < o.ngi - ng; ++i) {
+        auto u = [&](int a, int b, int cc, int d) { return o.vel[o.id(a, b, cc) + N * d]; };
+        auto Y = [&](int a, int b, int cc, int s) {
+          size_t q = o.id(a, b, cc);
+          return o.var[q + N * (4 + s)] / o.rho[q];
+        };
+        double dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz, mu;
+        const int ip = dir == 0, jp = dir == 1, kp = dir == 2;
+        mu = 0.0;
+        for (int s = 0; s < ns; ++s)
+          mu += ((Y(i + ip, j + jp, k + kp, s) + Y(i, j, k, s)) / 2.0) * o.cd[6 + 3 * s + 2];
+        if (dir == 0) {
+          dudx = (u(i + 1, j, k, 0) - u(i, j, k, 0)) / dx;
+          dvdx = (u(i + 1, j, k, 1) - u(i, j, k, 1)) / dx;
+          dwdx = (u(i + 1, j, k, 2) - u(i, j, k, 2)) / dx;
+          dudy = ((u(i + 1, j + 1, k, 0) + u(i, j + 1, k, 0)) - (u(i + 1, j - 1, k, 0) + u(i, j - 1, k, 0))) / (4 * dy);
+          dvdy = ((u(i + 1, j + 1, k, 1) + u(i, j + 1, k, 1)) - (u(i + 1, j - 1, k, 1) + u(i, j - 1, k, 1))) / (4 * dy);
+          dwdy = ((u(i + 1, j + 1, k, 2) + u(i, j + 1, k, 2)) - (u(i + 1, j - 1, k, 2) + u(i, j - 1, k, 2))) / (4 * dy);
+          dudz = ((u(i + 1, j, k + 1, 0) + u(i, j, k + 1, 0)) - (u(i + 1, j, k - 1, 0) + u(i, j, k - 1, 0))) / (4 * dz);
+          dvdz = ((u(i + 1, j, k + 1, 1) + u(i, j, k + 1, 1)) - (u(i + 1, j, k - 1, 1) + u(i, j, k - 1, 1))) / (4 * dz);
+          dwdz = ((u(i + 1, j, k + 1, 2) + u(i, j, k + 1, 2)) - (u(i + 1, j, k - 1, 2) + u(i, j, k - 1, 2))) / (4 * dz);
+        } else if (dir == 1) {
+          dudx = ((u(i + 1, j + 1, k, 0) + u(i + 1, j, k, 0)) - (u(i - 1, j + 1, k, 0) + u(i - 1, j, k, 0))) / (4 * dx);
+          dvdx = ((u(i + 1, j + 1, k, 1) + u(i + 1, j, k, 1)) - (u(i - 1, j + 1, k, 1) + u(i - 1, j, k, 1))) / (4 * dx);
+          dwdx = ((u(i + 1, j + 1, k, 2) + u(i + 1, j, k, 2)) - (u(i - 1, j + 1, k, 2) + u(i - 1, j, k, 2))) / (4 * dx);
+          dudy = (u(i, j + 1, k, 0) - u(i, j, k, 0)) / dy;
+          dvdy = (u(i, j + 1, k, 1) - u(i, j, k, 1)) / dy;
+          dwdy = (u(i, j + 1, k, 2) - u(i, j, k, 2)) / dy;
+          dudz = ((u(i, j + 1, k + 1, 0) + u(i, j, k + 1, 0)) - (u(i, j + 1, k - 1, 0) + u(i, j, k - 1, 0))) / (4 * dz);
+          dvdz = ((u(i, j + 1, k + 1, 1) + u(i, j, k + 1, 1)) - (u(i, j + 1, k - 1, 1) + u(i, j, k - 1, 1))) / (4 * dz);
+          dwdz = ((u(i, j + 1, k + 1, 2) + u(i, j, k + 1, 2)) - (u(i, j + 1, k - 1, 2) + u(i, j, k - 1, 2))) / (4 * dz);
+        } else {
+          dudx = ((u(i + 1, j, k + 1, 0) + u(i + 1, j, k, 0)) - (u(i - 1, j, k + 1, 0) + u(i - 1, j, k, 0))) / (4 * dx);
+          dvdx = ((u(i + 1, j, k + 1, 1) + u(i + 1, j, k, 1)) - (u(i - 1, j, k + 1, 1) + u(i - 1, j, k, 1))) / (4 * dx);
+          dwdx = ((u(i + 1, j, k + 1, 2) + u(i + 1, j, k, 2)) - (u(i - 1, j, k + 1, 2) + u(i - 1, j, k, 2))) / (4 * dx);
+          dudy = ((u(i, j + 1, k + 1, 0) + u(i, j + 1, k, 0)) - (u(i, j - 1, k + 1, 0) + u(i, j - 1, k, 0))) / (4 * dy);
+          dvdy = ((u(i, j + 1, k + 1, 1) + u(i, j + 1, k, 1)) - (u(i, j - 1, k + 1, 1) + u(i, j - 1, k, 1))) / (4 * dy);
+          dwdy = ((u(i, j + 1, k + 1, 2) + u(i, j + 1, k, 2)) - (u(i, j - 1, k + 1, 2) + u(i, j - 1, k, 2))) / (4 * dy);
+          dudz = (u(i, j, k + 1, 0) - u(i, j, k, 0)) / dz;
+          dvdz = (u(i, j, k + 1, 1) - u(i, j, k, 1)) / dz;
+          dwdz = (u(i, j, k + 1, 2) - u(i, j, k, 2)) / dz;
+        }
+        size_t c = o.id(i, j, k);
+        o.stress[c + N * 0] = (2.0 / 3.0) * mu * (2.0 * dudx - dvdy - dwdz);
+        o.stress[c + N * 1] = (2.0 / 3.0) * mu * (2.0 * dvdy - dudx - dwdz);
+        o.stress[c + N * 2] = (2.0 / 3.0) * mu * (2.0 * dwdz - dudx - dvdy);
+        o.stress[c + N * 3] = mu * (dudy + dvdx);
+        o.stress[c + N * 4] = mu * (dudz + dwdx);
+        o.stress[c + N * 5] = mu * (dvdz + dwdy);
+      }
+}
+
+// applyViscousTerm{x,y,z}3dv, src/viscosity.hpp:460-498, 542-580, 625-663
+void applyvisc3d(Orc &o, int dir) {
+  const int ng = o.ng;
+  const size_t N = o.N;
+  // stress components entering a, b, c, d for each direction
+  static const int comp[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+  const double d = o.cd[1 + dir];
+  const int ih = dir == 0, jh = dir == 1, kh = dir == 2;
+#pragma omp parallel for
+  for (int k = ng; k < o.ngk - ng; ++k)
+    for (int j = ng; j < o.ngj - ng; ++j)
+      for (int i = ng; i < o.ngi - ng; ++i) {
+        size_t c = o.id(i, j, k), cp = o.id(i + ih, j + jh, k + kh), cm = o.id(i - ih, j - jh, k - kh);
+        auto u = [&](size_t q, int w) { return o.vel[q + N * w]; };
+        double ur = (u(cp, 0) + u(c, 0)) / 2.0;
+        double ul = (u(cm, 0) + u(c, 0)) / 2.0;
+        double vr = (u(cp, 1) + u(c, 1)) / 2.0;
+        double vl = (u(cm, 1) + u(c, 1)) / 2.0;
+        double wr = (u(cp, 2) + u(c, 2)) / 2.0;
+        double wl = (u(cm, 2) + u(c, 2)) / 2.0;
+        auto S = [&](size_t q, int s) { return o.stress[q + N * s]; };
+        const int s0 = comp[dir][0], s1 = comp[dir][1], s2 = comp[dir][2];
+        double a = (S(c, s0) - S(cm, s0)) / d;
+        double b = (S(c, s1) - S(cm, s1)) / d;
+        double cc = (S(c, s2) - S(cm, s2)) / d;
+        double dd = (ur * S(c, s0) - ul * S(cm, s0)) / d +
+                    (vr * S(c, s1) - vl * S(cm, s1)) / d +
+                    (wr * S(c, s2) - wl * S(cm, s2)) / d;
+        o.dvar[c + N * 0] += a;
+        o.dvar[c + N * 1] += b;
+        o.dvar[c + N * 2] += cc;
+        o.dvar[c + N * 3] += dd;
+      }
+}
+
+// calculateRhoGrad, src/ceq3d.hpp:95-187
+void rhograd3d(Orc &o) {
+  const int ng = o.ng;
+  const size_t N = o.N;
+  const double dx = o.c.dx, dy = o.c.dy, dz = o.c.dz;
+#pragma omp parallel for
+  for (int k = ng; k < o.ngk - ng; ++k)
+    for (int j = ng; j < o.ngj - ng; ++j)
+      for (int i = ng; i < o.ngi - ng; ++i) {
+        auto rho = [&](int a, int b, int cc) { return o.rho[o.id(a, b, cc)]; };
+        auto vel = [&](int a, int b, int cc, int w) { return o.vel[o.id(a, b, cc) + N * w]; };
+        auto nrg = [&](int a, int b, int cc) {
+          size_t q = o.id(a, b, cc);
+          return o.var[q + 3 * N] / o.rho[q] -
+                 0.5 * o.rho[q] * (o.vel[q] * o.vel[q] + o.vel[q + N] * o.vel[q + N] + o.vel[q + 2 * N] * o.vel[q + 2 * N]);
+        };
+        auto dRho = [&](int ih, int jh, int kh, double d) {
+          return (rho(i - 2 * ih, j - 2 * jh, k - 2 * kh) - 8.0 * rho(i - ih, j - jh, k - kh) +
+                  8.0 * rho(i + ih, j + jh, k + kh) - rho(i + 2 * ih, j + 2 * jh, k + 2 * kh)) / (12.0 * d);
+        };
+        auto dE = [&](int ih, int jh, int kh, double d) {
+          return (nrg(i - 2 * ih, j - 2 * jh, k - 2 * kh) - 8.0 * nrg(i - ih, j - jh, k - kh) +
+                  8.0 * nrg(i + ih, j + jh, k + kh) - nrg(i + 2 * ih, j + 2 * jh, k + 2 * kh)) / (12.0 * d);
+        };
+        auto dV = [&](int ih, int jh, int kh, int w, double d) {
+          return (vel(i - 2 * ih, j - 2 * jh, k - 2 * kh, w) - 8.0 * vel(i - ih, j - jh, k - kh, w) +
+                  8.0 * vel(i + ih, j + jh, k + kh, w) - vel(i + 2 * ih, j + 2 * jh, k + 2 * kh, w)) / (12.0 * d);
+        };
+        double dxr = dRho(1, 0, 0, dx), dyr = dRho(0, 1, 0, dy), dzr = dRho(0, 0, 1, dz);
+        double dxe = dE(1, 0, 0, dx), dye = dE(0, 1, 0, dy), dze = dE(0, 0, 1, dz);
+        double dxu = dV(1, 0, 0, 0, dx), dyv = dV(0, 1, 0, 1, dy), dzw = dV(0, 0, 1, 2, dz);
+        double n1 = dxr, n2 = dyr, n3 = dzr;
+        double rgrad = std::sqrt(dxr * dxr + dyr * dyr + dzr * dzr);
+        double divu = dxu + dyv + dzw;
+        double dnednr = (n1 * dxe + n2 * dye + n3 * dze) * (n1 * dxr + n2 * dyr + n3 * dzr);
+        int indicator = dnednr < 0.0 ? 1 : 0;
+        size_t c = o.id(i, j, k);
+        if (divu < 0) o.gradRho[c] = (1 - indicator) * rgrad;
+        else          o.gradRho[c] = 0;
+        o.gradRho[c + N * 1] = indicator * rgrad;
+        o.gradRho[c + N * 2] = dxr;
+        o.gradRho[c + N * 3] = dyr;
+        o.gradRho[c + N * 4] = dzr;
+      }
+}
+
+// maxWaveSpeed reduce (src/ceq3d.hpp:25-63) and maxGradFunctor on var(nv+1)
+// (src/ceq3d.hpp:82-93, called src/cart3d.cpp:281-283); fsMax returns the
+// local maximum (src/cart3d.cpp:190-198, the one-rank value)
+void maxima3d(Orc &o) {
+  const int ng = o.ng, ns = o.ns;
+  const size_t N = o.N;
+  double ms = -DBL_MAX, mc = -DBL_MAX;
+#pragma omp parallel for reduction(max : ms, mc)
+  for (int k = ng; k < o.ngk - ng; ++k)
+    for (int j = ng; j < o.ngj - ng; ++j)
+      for (int i = ng; i < o.ngi - ng; ++i) {
+        size_t c = o.id(i, j, k);
+        double Cp = 0, Cv = 0;
+        for (int s = 0; s < ns; ++s) {
+          double gammas = o.cd[6 + 3 * s], Rs = o.cd[6 + 3 * s + 1];
+          Cp = Cp + (o.var[c + N * (4 + s)] / o.rho[c]) * (gammas * Rs / (gammas - 1));
+          Cv = Cv + (o.var[c + N * (4 + s)] / o.rho[c]) * (Rs / (gammas - 1));
+        }
+        double gamma = Cp / Cv;
+        double a = std::sqrt(gamma * o.p[c] / o.rho[c]);
+        double s = a + std::sqrt(o.var[c] * o.var[c] + o.var[c + N] * o.var[c + N] + o.var[c + 2 * N] * o.var[c + 2 * N]) / o.rho[c];
+        if (s > ms) ms = s;
+        double ch = std::abs(o.var[c + N * (o.nv + 1)]);
+        if (ch > mc) mc = ch;
+      }
+  o.maxS = ms;
+  o.maxCh = mc;
+}
+
+// updateCeq, src/ceq3d.hpp:189-224
+void updateceq3d(Orc &o) {
+  const int ng = o.ng;
+  const size_t N = o.N;
+  const double dx = o.cd[1], dy = o.cd[2], dz = o.cd[3];
+  const int nc = (int)o.cd[0] + 4;
+  const double maxS = o.maxS, kap = o.c.kap, eps = o.c.eps;
+  const double dxmag = std::sqrt(dx * dx + dy * dy + dz * dz);
+#pragma omp parallel for
+  for (int k = ng; k < o.ngk - ng; ++k)
+    for (int j = ng; j < o.ngj - ng; ++j)
+      for (int i = ng; i < o.ngi - ng; ++i) {
+        for (int n = 0; n < 5; ++n) {
+          auto C = [&](int a, int b, int cc) { return o.var[o.id(a, b, cc) + N * (nc + n)]; };
+          double lap = (C(i - 1, j, k) - 2 * C(i, j, k) + C(i + 1, j, k)) / dx +
+                       (C(i, j - 1, k) - 2 * C(i, j, k) + C(i, j + 1, k)) / dy +
+                       (C(i, j, k - 1) - 2 * C(i, j, k) + C(i, j, k + 1)) / dz;
+          size_t c = o.id(i, j, k);
+          o.dvar[c + N * (nc + n)] =
+              (maxS / (eps * dxmag)) * (o.gradRho[c + N * n] - o.var[c + N * (nc + n)]) + kap * maxS * dxmag * lap;
+        }
+      }
+}
+
+// calculateCeqFaces (src/ceq3d.hpp:226-296) then calculateCeqGrads
+// (src/ceq3d.hpp:298-332), both over the weno range (src/cart3d.cpp:288-289)
+void ceqfaces3d(Orc &o) {
+  const int ng = o.ng, nv = o.nv;
+  const size_t N = o.N;
+  const double alpha = o.alpha;
+#pragma omp parallel for
+  for (int k = ng - 1; k < o.ngk - ng; ++k)
+    for (int j = ng - 1; j < o.ngj - ng; ++j)
+      for (int i = ng - 1; i < o.ngi - ng; ++i) {
+        double r, chat, cn[3], M, cmag;
+        for (int face = 0; face < 3; ++face) {
+          int ip = face == 0, jp = face == 1, kp = face == 2;
+          size_t c = o.id(i, j, k), q = o.id(i + ip, j + jp, k + kp);
+          r = (o.rho[c] + o.rho[q]) / 2.0;
+          chat = (o.var[c + N * (nv + 1)] + o.var[q + N * (nv + 1)]) / 2.0;
+          cn[0] = (o.var[c + N * (nv + 2)] + o.var[q + N * (nv + 2)]) / 2.0;
+          cn[1] = (o.var[c + N * (nv + 3)] + o.var[q + N * (nv + 3)]) / 2.0;
+          cn[2] = (o.var[c + N * (nv + 4)] + o.var[q + N * (nv + 4)]) / 2.0;
+          cmag = 1.0e-6;
+          cmag += cn[0] * cn[0];
+          cmag += cn[1] * cn[1];
+          cmag += cn[2] * cn[2];
+          for (int dir = 0; dir < 3; ++dir) {
+            double dirac = 0.0;
+            if (face == dir) dirac = 1.0;
+            M = dirac - (cn[face] * cn[dir]) / cmag;
+            for (int w = 0; w < 3; ++w) o.MF(face, dir, i, j, k, w) = alpha * r * chat * M;
+          }
+        }
+      }
+  const double dx = o.c.dx, dy = o.c.dy, dz = o.c.dz;
+#pragma omp parallel for
+  for (int k = ng - 1; k < o.ngk - ng; ++k)
+    for (int j = ng - 1; j < o.ngj - ng; ++j)
+      for (int i = ng - 1; i < o.ngi - ng; ++i) {
+        auto vel = [&](int a, int b, int cc, int w) { return o.vel[o.id(a, b, cc) + N * w]; };
+        for (int w = 0; w < 3; ++w) {
+          o.MF(0, 0, i, j, k, w) *= (vel(i + 1, j, k, w) - vel(i, j, k, w)) / dx;
+          o.MF(0, 1, i, j, k, w) *= ((vel(i, j + 1, k, w) + vel(i + 1, j + 1, k, w)) - (vel(i, j - 1, k, w) + vel(i + 1, j - 1, k, w))) / (4 * dy);
+          o.MF(0, 2, i, j, k, w) *= ((vel(i, j, k + 1, w) + vel(i + 1, j, k + 1, w)) - (vel(i, j, k - 1, w) + vel(i + 1, j, k - 1, w))) / (4 * dz);
+          o.MF(1, 0, i, j, k, w) *= ((vel(i + 1, j + 1, k, w) + vel(i + 1, j, k, w)) - (vel(i - 1, j + 1, k, w) + vel(i - 1, j, k, w))) / (4 * dx);
+          o.MF(1, 1, i, j, k, w) *= (vel(i, j + 1, k, w) - vel(i, j, k, w)) / dy;
+          o.MF(1, 2, i, j, k, w) *= ((vel(i, j + 1, k + 1, w) + vel(i, j, k + 1, w)) - (vel(i, j + 1, k - 1, w) + vel(i, j, k - 1, w))) / (4 * dz);
+          o.MF(2, 0, i, j, k, w) *= ((vel(i + 1, j, k + 1, w) + vel(i + 1, j, k, w)) - (vel(i - 1, j, k + 1, w) + vel(i - 1, j, k, w))) / (4 * dx);
+          o.MF(2, 1, i, j, k, w) *= ((vel(i, j + 1, k + 1, w) + vel(i, j + 1, k, w)) - (vel(i, j - 1, k + 1, w) + vel(i, j - 1, k, w))) / (4 * dy);
+          o.MF(2, 2, i, j, k, w) *= (vel(i, j, k + 1, w) - vel(i, j, k, w)) / dz;
+        }
+      }
+}
+
+// applyCeq, src/ceq3d.hpp:334-366
+void applyceq3d(Orc &o) {
+  const int ng = o.ng;
+  const size_t N = o.N;
+  const double d[3] = {o.c.dx, o.c.dy, o.c.dz};
+#pragma omp parallel for
+  for (int k = ng; k < o.ngk - ng; ++k)
+    for (int j = ng; j < o.ngj - ng; ++j)
+      for (int i = ng; i < o.ngi - ng; ++i) {
+        for (int w = 0; w < 3; ++w) {
+          double diffu = 0.0;
+          for (int face = 0; face < 3; ++face) {
+            int ih = face == 0, jh = face == 1, kh = face == 2;
+            for (int dir = 0; dir < 3; ++dir)
+              diffu += (o.MF(face, dir, i, j, k, w) - o.MF(face, dir, i - ih, j - jh, k - kh, w)) / d[face];
+          }
+          o.dvar[o.id(i, j, k) + N * w] += diffu;
+        }
+      }
+}
+
+// ---------------------------------------------------------------- 2D kernels
+
+// calculateRhoPT2D, src/secondary.hpp:60-107
+void eos2d(Orc &o) {
+  const int ns = o.ns;
+#pragma omp parallel for
+  for (int j = 0; j < o.ngj; ++j)
+    for (int i = 0; i < o.ngi; ++i) {
+      size_t c = o.id(i, j, 0);
+      double r = 0.0;
+      for (int s = 0; s < ns; ++s) r = r + o.var[c + o.N * (3 + s)];
+      double Cp = 0, Cv = 0;
+      for (int s = 0; s < ns; ++s) {
+        double gammas = o.cd[6 + 3 * s], Rs = o.cd[6 + 3 * s + 1];
+        Cp = Cp + (o.var[c + o.N * (3 + s)] / r) * (gammas * Rs / (gammas - 1));
+        Cv = Cv + (o.var[c + o.N * (3 + s)] / r) * (Rs / (gammas - 1));
+      }
+      double gamma = Cp / Cv;
+      double m0 = o.var[c], m1 = o.var[c + o.N];
+      double pp = (gamma - 1) * (o.var[c + 2 * o.N] - (0.5 / r) * (m0 * m0 + m1 * m1));
+      o.rho[c] = r;
+      o.p[c] = pp;
+      o.T[c] = pp / ((Cp - Cv) * r);
+    }
+}
+
+// computeVelocity2D, src/velocity.hpp:85-99
+void vel2d(Orc &o) {
+#pragma omp parallel for
+  for (int j = 0; j < o.ngj; ++j)
+    for (int i = 0; i < o.ngi; ++i) {
+      size_t c = o.id(i, j, 0);
+      o.vel[c] = o.var[c] / o.rho[c];
+      o.vel[c + o.N] = o.var[c + o.N] / o.rho[c];
+    }
+}
+
+// computeFluxWeno2D::weno, src/flux.hpp:38-62 (double constants)
+double weno2d(double f1, double f2, double f3, double f4, double f5) {
+  const double eps = 1e-6;  // src/flux.hpp:32
+  double b1, b2, b3, w1, w2, w3, p1, p2, p3, a1, a2;
+  a1 = f1 - 2.0 * f2 + f3;
+  a2 = f1 - 4.0 * f2 + 3.0 * f3;
+  b1 = (13.0 / 12.0) * a1 * a1 + (0.25) * a2 * a2;
+  a1 = f2 - 2.0 * f3 + f4;
+  a2 = f2 - f4;
+  b2 = (13.0 / 12.0) * a1 * a1 + (0.25) * a2 * a2;
+  a1 = f3 - 2.0 * f4 + f5;
+  a2 = 3.0 * f3 - 4.0 * f4 + f5;
+  b3 = (13.0 / 12.0) * a1 * a1 + (0.25) * a2 * a2;
+  a1 = eps + b1;
+  w1 = (0.1) / (a1 * a1);
+  a1 = eps + b2;
+  w2 = (0.6) / (a1 * a1);
+  a1 = eps + b3;
+  w3 = (0.3) / (a1 * a1);
+  p1 = (1.0 / 3.0) * f1 + (-7.0 / 6.0) * f2 + (11.0 / 6.0) * f3;
+  p2 = (-1.0 / 6.0) * f2 + (5.0 / 6.0) * f3 + (1.0 / 3.0) * f4;
+  p3 = (1.0 / 3.0) * f3 + (5.0 / 6.0) * f4 + (-1.0 / 6.0) * f5;
+  return (w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3);
+}
+
+// computeFluxWeno2D (src/flux.hpp:64-105), computeFluxCentered2D (:121-166),
+// computeFluxQuick2D (:184-220), all over the face range (src/cart2d.cpp:154,164-172)
+void flux2d(Orc &o, int v) {
+  const int ng = o.ng;
+  const size_t N = o.N;
+  const double isE = (double)(v == 2);
+  const int scheme = o.c.scheme;
+#pragma omp parallel for
+  for (int j = ng - 1; j < o.ngj - ng; ++j)
+    for (int i = ng - 1; i < o.ngi - ng; ++i) {
+      auto var = [&](int a, int b, int w) { return o.var[o.id(a, b, 0) + N * w]; };
+      auto P = [&](int a, int b) { return o.p[o.id(a, b, 0)]; };
+      auto R = [&](int a, int b) { return o.rho[o.id(a, b, 0)]; };
+      auto u = [&](int a, int b, int w) { return o.vel[o.id(a, b, 0) + N * w]; };
+      auto q = [&](int a, int b) { return var(a, b, v) + isE * P(a, b); };
+      size_t c = o.id(i, j, 0);
+      if (scheme == FST_CENTERED4) {
+        const double dx = o.cd[1], dy = o.cd[2];
+        double ur, vr, x1, y1, px, py;
+        ur = (0.0 - var(i + 2, j, 0) / R(i + 2, j) + 7.0 * var(i + 1, j, 0) / R(i + 1, j) +
+              7.0 * var(i, j, 0) / R(i, j) - var(i - 1, j, 0) / R(i - 1, j)) / 12.0;
+        vr = (0.0 - var(i, j + 2, 1) / R(i, j + 2) + 7.0 * var(i, j + 1, 1) / R(i, j + 1) +
+              7.0 * var(i, j, 1) / R(i, j) - var(i, j - 1, 1) / R(i, j - 1)) / 12.0;
+        px = (0.0 - P(i + 2, j) + 7.0 * P(i + 1, j) + 7.0 * P(i, j) - P(i - 1, j)) / 12.0;
+        py = (0.0 - P(i, j + 2) + 7.0 * P(i, j + 1) + 7.0 * P(i, j) - P(i, j - 1)) / 12.0;
+        x1 = (0.0 - var(i + 2, j, v) + 7.0 * var(i + 1, j, v) + 7.0 * var(i, j, v) - var(i - 1, j, v)) / 12.0;
+        y1 = (0.0 - var(i, j + 2, v) + 7.0 * var(i, j + 1, v) + 7.0 * var(i, j, v) - var(i, j - 1, v)) / 12.0;
+        if (v == 2) {
+          x1 = x1 + px;
+          y1 = y1 + py;
+        }
+        o.fluxx[c] = ur * x1 / dx;
+        o.fluxy[c] = vr * y1 / dy;
+      } else if (scheme == FST_QUICK) {
+        const double dx = o.cd[1], dy = o.cd[2];
+        auto quick = [](double f1, double f2, double f3) { return 0.75 * f2 + 0.375 * f3 - 0.125 * f1; };
+        double ur, vr, f1, f2, f3;
+        ur = (-u(i + 2, j, 0) + 7.0 * u(i + 1, j, 0) + 7.0 * u(i, j, 0) - u(i - 1, j, 0)) / 12.0;
+        if (ur < 0.0) { f1 = q(i + 2, j); f2 = q(i + 1, j); f3 = q(i, j); }
+        else          { f1 = q(i - 1, j); f2 = q(i, j);     f3 = q(i + 1, j); }
+        o.fluxx[c] = ur * quick(f1, f2, f3) / dx;
+        vr = (-u(i, j + 2, 1) + 7.0 * u(i, j + 1, 1) + 7.0 * u(i, j, 1) - u(i, j - 1, 1)) / 12.0;
+        if (vr < 0.0) { f1 = q(i, j + 2); f2 = q(i, j + 1); f3 = q(i, j); }
+        else          { f1 = q(i, j - 1); f2 = q(i, j);     f3 = q(i, j + 1); }
+        o.fluxy[c] = vr * quick(f1, f2, f3) / dy;
+      } else {
+        const double dx = o.c.dx, dy = o.c.dy;
+        double ur, vr, f1, f2, f3, f4, f5;
+        ur = (-u(i + 2, j, 0) + 7.0 * u(i + 1, j, 0) + 7.0 * u(i, j, 0) - u(i - 1, j, 0)) / 12.0;
+        vr = (-u(i, j + 2, 1) + 7.0 * u(i, j + 1, 1) + 7.0 * u(i, j, 1) - u(i, j - 1, 1)) / 12.0;
+        if (ur < 0.0) { f1 = q(i + 3, j); f2 = q(i + 2, j); f3 = q(i + 1, j); f4 = q(i, j); f5 = q(i - 1, j); }
+        else          { f1 = q(i - 2, j); f2 = q(i - 1, j); f3 = q(i, j); f4 = q(i + 1, j); f5 = q(i + 2, j); }
+        o.fluxx[c] = ur * weno2d(f1, f2, f3, f4, f5) / dx;
+        if (vr < 0.0) { f1 = q(i, j + 3); f2 = q(i, j + 2); f3 = q(i, j + 1); f4 = q(i, j); f5 = q(i, j - 1); }
+        else          { f1 = q(i, j - 2); f2 = q(i, j - 1); f3 = q(i, j); f4 = q(i, j + 1); f5 = q(i, j + 2); }
+        o.fluxy[c] = vr * weno2d(f1, f2, f3, f4, f5) / dy;
+      }
+    }
+}
+
+// advect2D, src/advect.hpp:23-37
+void advect2d(Orc &o, int v) {
+  const int ng = o.ng;
+#pragma omp parallel for
+  for (int j = ng; j < o.ngj - ng; ++j)
+    for (int i = ng; i < o.ngi - ng; ++i)
+      o.V(o.dvar, i, j, 0, v) = -((o.fluxx[o.id(i, j, 0)] - o.fluxx[o.id(i - 1, j, 0)]) +
+                                  (o.fluxy[o.id(i, j, 0)] - o.fluxy[o.id(i, j - 1, 0)]));
+}
+
+// applyPressureGradient2D, src/presgrad.hpp:22-42
+void presgrad2d(Orc &o) {
+  const int ng = o.ng;
+  const double dx = o.cd[1], dy = o.cd[2];
+#pragma omp parallel for
+  for (int j = ng; j < o.ngj - ng; ++j)
+    for (int i = ng; i < o.ngi - ng; ++i) {
+      auto P = [&](int a, int b) { return o.p[o.id(a, b, 0)]; };
+      double dxp = (P(i - 2, j) - 8.0 * P(i - 1, j) + 8.0 * P(i + 1, j) - P(i + 2, j)) / (12.0 * dx);
+      double dyp = (P(i, j - 2) - 8.0 * P(i, j - 1) + 8.0 * P(i, j + 1) - P(i, j + 2)) / (12.0 * dy);
+      o.V(o.dvar, i, j, 0, 0) -= dxp;
+      o.V(o.dvar, i, j, 0, 1) -= dyp;
+    }
+}
+
+// computeBuoyancy (2D), src/buoyancy.hpp:49-73
+void buoyancy2d(Orc &o) {
+  const int ng = o.ng;
+  const double g = o.c.gAccel, rhoRef = o.c.rhoRef;
+  const size_t N = o.N;
+#pragma omp parallel for
+  for (int j = ng; j < o.ngj - ng; ++j)
+    for (int i = ng; i < o.ngi - ng; ++i) {
+      size_t c = o.id(i, j, 0);
+      double v = o.var[c + N] / o.var[c + 3 * N];
+      double rhop = o.rho[c] - rhoRef;
+      double eps = 1e-6;
+      if (rhop >= eps || rhop <= -eps) {
+        double f = -g * rhop;
+        o.dvar[c + N] += f;
+        o.dvar[c + 2 * N] += std::abs(v) * f;
+      }
+    }
+}
+
+// 2D stress on x and y faces (2,2) layout helper
+inline double &S2(std::vector<double> &a, const Orc &o, size_t c, int r, int s) {
+  return a[c + o.N * (size_t)(r + 2 * s)];
+}
+
+// calculateStressTensor2dv (src/viscosity.hpp:25-109) and calculateHeatFlux2dv
+// (:111-136) over the face range, applyViscousTerm2dv (:138-199) over cells
+void visc2d(Orc &o) {
+  const int ng = o.ng;
+  const size_t N = o.N;
+  const double dx = o.cd[1], dy = o.cd[2];
+#pragma omp parallel for
+  for (int j = ng - 1; j < o.ngj - ng; ++j)
+    for (int i = ng - 1; i < o.ngi - ng; ++i) {
+      auto vel = [&](int a, int b, int w) { return o.vel[o.id(a, b, 0) + N * w]; };
+      double dudx, dvdy, dudy, dvdx;
+      double mur = 2.928e-5, mut = 2.928e-5;
+      size_t c = o.id(i, j, 0);
+      dudx = (vel(i + 1, j, 0) - vel(i, j, 0)) / dx;
+      dvdx = (vel(i + 1, j, 1) - vel(i, j, 1)) / dx;
+      dudy = ((vel(i, j + 1, 0) + vel(i + 1, j + 1, 0)) - (vel(i, j - 1, 1) + vel(i + 1, j - 1, 0))) / (4.0 * dy);
+      dvdy = ((vel(i, j + 1, 1) + vel(i + 1, j + 1, 1)) - (vel(i, j - 1, 1) + vel(i + 1, j - 1, 1))) / (4.0 * dy);
+      S2(o.stressx, o, c, 0, 0) = (2.0 / 3.0) * mur * (2.0 * dudx - dvdy);
+      S2(o.stressx, o, c, 1, 1) = (2.0 / 3.0) * mur * (2.0 * dvdy - dudx);
+      S2(o.stressx, o, c, 0, 1) = mur * (dudy + dvdx);
+      S2(o.stressx, o, c, 1, 0) = S2(o.stressx, o, c, 0, 1);
+      dudx = ((vel(i + 1, j, 0) + vel(i + 1, j + 1, 0)) - (vel(i - 1, j, 0) + vel(i - 1, j + 1, 0))) / (4.0 * dx);
+      dvdx = ((vel(i + 1, j, 1) + vel(i + 1, j + 1, 1)) - (vel(i - 1, j, 1) + vel(i - 1, j + 1, 1))) / (4.0 * dx);
+      dudy = (vel(i, j + 1, 0) - vel(i, j, 0)) / dy;
+      dvdy = (vel(i, j + 1, 1) - vel(i, j, 1)) / dy;
+      S2(o.stressy, o, c, 0, 0) = (2.0 / 3.0) * mut * (2.0 * dudx - dvdy);
+      S2(o.stressy, o, c, 1, 1) = (2.0 / 3.0) * mut * (2.0 * dvdy - dudx);
+      S2(o.stressy, o, c, 0, 1) = mut * (dudy + dvdx);
+      S2(o.stressy, o, c, 1, 0) = S2(o.stressy, o, c, 0, 1);
+    }
+#pragma omp parallel for
+  for (int j = ng - 1; j < o.ngj - ng; ++j)
+    for (int i = ng - 1; i < o.ngi - ng; ++i) {
+      const double kk = 0.026;
+      size_t c = o.id(i, j, 0);
+      o.qx[c] = -kk * (o.T[o.id(i + 1, j, 0)] - o.T[c]) / dx;
+      o.qy[c] = -kk * (o.T[o.id(i, j + 1, 0)] - o.T[c]) / dy;
+    }
+#pragma omp parallel for
+  for (int j = ng; j < o.ngj - ng; ++j)
+    for (int i = ng; i < o.ngi - ng; ++i) {
+      auto vel = [&](int a, int b, int w) { return o.vel[o.id(a, b, 0) + N * w]; };
+      size_t c = o.id(i, j, 0), cl = o.id(i - 1, j, 0), cb = o.id(i, j - 1, 0);
+      double ur = (vel(i + 1, j, 0) + vel(i, j, 0)) / 2.0;
+      double ul = (vel(i - 1, j, 0) + vel(i, j, 0)) / 2.0;
+      double vr = (vel(i + 1, j, 1) + vel(i, j, 1)) / 2.0;
+      double vl = (vel(i - 1, j, 1) + vel(i, j, 1)) / 2.0;
+      double ut = (vel(i, j + 1, 0) + vel(i, j, 0)) / 2.0;
+      double ub = (vel(i, j - 1, 0) + vel(i, j, 0)) / 2.0;
+      double vt = (vel(i, j + 1, 1) + vel(i, j, 1)) / 2.0;
+      double vb = (vel(i, j - 1, 1) + vel(i, j, 1)) / 2.0;
+      auto sx = [&](size_t q, int r, int s) { return S2(o.stressx, o, q, r, s); };
+      auto sy = [&](size_t q, int r, int s) { return S2(o.stressy, o, q, r, s); };
+      double a = (sx(c, 0, 0) - sx(cl, 0, 0)) / dx + (sy(c, 0, 1) - sy(cb, 0, 1)) / dy;
+      double b = (sx(c, 1, 0) - sx(cl, 1, 0)) / dx + (sy(c, 1, 1) - sy(cb, 1, 1)) / dy;
+      double c1 = (ur * sx(c, 0, 0) - ul * sx(cl, 0, 0)) / dx + (vr * sx(c, 0, 1) - vl * sx(cl, 0, 1)) / dx +
+                  (ut * sy(c, 1, 0) - ub * sy(cb, 1, 0)) / dy + (vt * sy(c, 1, 1) - vb * sy(cb, 1, 1)) / dy;
+      double c2 = (o.qx[c] - o.qx[cl]) / dx + (o.qy[c] - o.qy[cb]) / dy;
+      o.dvar[c] = o.dvar[c] + a;
+      o.dvar[c + N] = o.dvar[c + N] + b;
+      o.dvar[c + 2 * N] = o.dvar[c + 2 * N] + c1 - c2;
+    }
+}
+
+// 2D C-equation: calculateRhoGrad2D (src/ceq.hpp:80-151), maxWaveSpeed2D
+// (:43-78, stride-2 species lookup and |m| kept), maxCvar2D (:23-41),
+// updateCeq2D (:153-195), computeCeqFlux2D (:197-245), computeCeqFaces2D
+// (:247-267), applyCeq2D (:269-293); sequence src/cart2d.cpp:201-236
+void ceq2d(Orc &o) {
+  const int ng = o.ng, ns = o.ns;
+  const size_t N = o.N;
+  const double dx = o.c.dx, dy = o.c.dy;
+#pragma omp parallel for
+  for (int j = ng; j < o.ngj - ng; ++j)
+    for (int i = ng; i < o.ngi - ng; ++i) {
+      auto R = [&](int a, int b) { return o.rho[o.id(a, b, 0)]; };
+      auto U = [&](int a, int b, int w) { return o.vel[o.id(a, b, 0) + N * w]; };
+      auto nrg = [&](int a, int b) {
+        size_t q = o.id(a, b, 0);
+        return o.var[q + 2 * N] / o.rho[q] - 0.5 * o.rho[q] * (o.vel[q] * o.vel[q] + o.vel[q + N] * o.vel[q + N]);
+      };
+      auto dR = [&](int ih, int jh, double d) {
+        return (R(i - 2 * ih, j - 2 * jh) - 8.0 * R(i - ih, j - jh) + 8.0 * R(i + ih, j + jh) - R(i + 2 * ih, j + 2 * jh)) / (12.0 * d);
+      };
+      auto dE = [&](int ih, int jh, double d) {
+        return (nrg(i - 2 * ih, j - 2 * jh) - 8.0 * nrg(i - ih, j - jh) + 8.0 * nrg(i + ih, j + jh) - nrg(i + 2 * ih, j + 2 * jh)) / (12.0 * d);
+      };
+      auto dV = [&](int ih, int jh, int w, double d) {
+        return (U(i - 2 * ih, j - 2 * jh, w) - 8.0 * U(i - ih, j - jh, w) + 8.0 * U(i + ih, j + jh, w) - U(i + 2 * ih, j + 2 * jh, w)) / (12.0 * d);
+      };
+      double dxr = dR(1, 0, dx), dyr = dR(0, 1, dy);
+      double dxe = dE(1, 0, dx), dye = dE(0, 1, dy);
+      double dxu = dV(1, 0, 0, dx), dyv = dV(0, 1, 1, dy);
+      double n1 = dxr, n2 = dyr;
+      double rgrad = std::sqrt(dxr * dxr + dyr * dyr);
+      double divu = dxu + dyv;
+      double dnednr = (n1 * dxe + n2 * dye) * (n1 * dxr + n2 * dyr);
+      double i1 = 0;
+      if (dnednr < 0.0) i1 = 1.0; else i1 = 0.0;
+      double i2 = 0;
+      if (divu < 0.0) i2 = 1.0;
+      size_t c = o.id(i, j, 0);
+      o.gradRho[c] = (1 - i1) * i2 * rgrad;
+      o.gradRho[c + N] = i1 * rgrad;
+      o.gradRho[c + 2 * N] = -dyr;
+      o.gradRho[c + 3 * N] = dxr;
+    }
+  const int nv = o.nv;
+  double ms = -DBL_MAX, mc = -DBL_MAX, mch = -DBL_MAX;
+#pragma omp parallel for reduction(max : ms, mc, mch)
+  for (int j = ng; j < o.ngj - ng; ++j)
+    for (int i = ng; i < o.ngi - ng; ++i) {
+      size_t c = o.id(i, j, 0);
+      double Cp = 0, Cv = 0;
+      for (int s = 0; s < ns; ++s) {
+        double gammas = o.cd[6 + 2 * s], Rs = o.cd[6 + 2 * s + 1];
+        Cp = Cp + (o.var[c + N * (3 + s)] / o.rho[c]) * (gammas * Rs / (gammas - 1));
+        Cv = Cv + (o.var[c + N * (3 + s)] / o.rho[c]) * (Rs / (gammas - 1));
+      }
+      double gamma = Cp / Cv;
+      double a = std::sqrt(gamma * o.p[c] / o.rho[c]);
+      double s = a + std::sqrt(o.var[c] * o.var[c] + o.var[c + N] * o.var[c + N]);
+      if (s > ms) ms = s;
+      double c0 = std::abs(o.var[c + N * (nv + 0)]);
+      if (c0 > mc) mc = c0;
+      double c1 = std::abs(o.var[c + N * (nv + 1)]);
+      if (c1 > mch) mch = c1;
+    }
+  o.maxS = ms;
+  o.maxC = mc;
+  o.maxCh = mch;
+  o.alpha = (o.c.dx * o.c.dx + o.c.dy * o.c.dy) / (o.maxCh + 1.0e-6) * o.c.alpha;
+  {
+    const double cdx = o.cd[1], cdy = o.cd[2];
+    const int nc = (int)o.cd[0] + 3;
+    const double maxS = o.maxS, kap = o.c.kap, eps = o.c.eps;
+    const double dxmag = std::sqrt(cdx * cdx + cdy * cdy);
+#pragma omp parallel for
+    for (int j = ng; j < o.ngj - ng; ++j)
+      for (int i = ng; i < o.ngi - ng; ++i) {
+        for (int n = 0; n < 4; ++n) {
+          auto C = [&](int a, int b) { return o.var[o.id(a, b, 0) + N * (nc + n)]; };
+          double dx_right = (C(i + 1, j) - C(i, j)) / cdx;
+          double dx_left = (C(i, j) - C(i - 1, j)) / cdx;
+          double dy_top = (C(i, j + 1) - C(i, j)) / cdy;
+          double dy_bot = (C(i, j) - C(i, j - 1)) / cdy;
+          double lap = dx_right - dx_left + dy_top - dy_bot;
+          size_t c = o.id(i, j, 0);
+          o.dvar[c + N * (nc + n)] =
+              (maxS / (eps * dxmag)) * (o.gradRho[c + N * n] - C(i, j)) + kap * maxS * dxmag * lap;
+        }
+      }
+  }
+  const double alpha = o.alpha;
+#pragma omp parallel for
+  for (int j = ng - 1; j < o.ngj - ng; ++j)
+    for (int i = ng - 1; i < o.ngi - ng; ++i) {
+      for (int f = 0; f < 2; ++f) {
+        int ih = f == 0, jh = f == 1;
+        size_t c = o.id(i, j, 0), q = o.id(i + ih, j + jh, 0);
+        double r = (o.rho[c] + o.rho[q]) / 2.0;
+        double chat = (o.var[c + N * (nv + 1)] + o.var[q + N * (nv + 1)]) / 2.0;
+        double ct[2];
+        ct[0] = (o.var[c + N * (nv + 2)] + o.var[q + N * (nv + 2)]) / 2.0;
+        ct[1] = (o.var[c + N * (nv + 3)] + o.var[q + N * (nv + 3)]) / 2.0;
+        double ctmag = (ct[0] * ct[0] + ct[1] * ct[1]) + 1.0e-6;
+        for (int d = 0; d < 2; ++d)
+          for (int w = 0; w < 2; ++w) o.M2(f, d, i, j, w) = alpha * r * chat * (ct[f] * ct[d]) / ctmag;
+      }
+    }
+  {
+    const double cdx = o.cd[1], cdy = o.cd[2];
+#pragma omp parallel for
+    for (int j = ng - 1; j < o.ngj - ng; ++j)
+      for (int i = ng - 1; i < o.ngi - ng; ++i) {
+        auto v = [&](int a, int b, int w) { return o.vel[o.id(a, b, 0) + N * w]; };
+        for (int w = 0; w < 2; ++w) {
+          o.M2(0, 0, i, j, w) *= (v(i + 1, j, w) - v(i, j, w)) / cdx;
+          o.M2(0, 1, i, j, w) *= ((v(i, j + 1, w) + v(i + 1, j + 1, w)) - (v(i, j - 1, w) + v(i + 1, j - 1, w))) / (4.0 * cdy);
+          o.M2(1, 0, i, j, w) *= ((v(i + 1, j, w) + v(i + 1, j + 1, w)) - (v(i - 1, j, w) + v(i - 1, j + 1, w))) / (4.0 * cdx);
+          o.M2(1, 1, i, j, w) *= (v(i, j + 1, w) - v(i, j, w)) / cdy;
+        }
+      }
+  }
+#pragma omp parallel for
+  for (int j = ng; j < o.ngj - ng; ++j)
+    for (int i = ng; i < o.ngi - ng; ++i) {
+      for (int w = 0; w < 2; ++w) {
+        double diffu = 0;
+        diffu += (o.M2(0, 0, i, j, w) - o.M2(0, 0, i - 1, j, w)) / dx;
+        diffu += (o.M2(0, 1, i, j, w) - o.M2(0, 1, i - 1, j, w)) / dx;
+        diffu += (o.M2(1, 0, i, j, w) - o.M2(1, 0, i, j - 1, w)) / dy;
+        diffu += (o.M2(1, 1, i, j, w) - o.M2(1, 1, i, j - 1, w)) / dy;
+        o.dvar[o.id(i, j, 0) + N * w] += diffu;
+      }
+    }
+}
+
+// ---------------------------------------------------------------- ghosts
+
+// bc_gen, src/bc.cpp:33-131, launched on one plane (src/bc.cpp:217-247)
+void bc_gen(Orc &o, int type, int ih, int jh, int kh, int i0, int i1, int j0, int j1, int k0, int k1) {
+  const int ng = o.ng, nv = o.nvt, ndim = o.ndim;
+  for (int k = k0; k < k1; ++k)
+    for (int j = j0; j < j1; ++j)
+      for (int i = i0; i < i1; ++i) {
+        auto U = [&](int a, int b, int c, int v) -> double & { return o.var[o.id(a, b, c) + o.N * v]; };
+        if (type == FST_BC_OUTFLOW) {
+          for (int v = 0; v < nv; ++v)
+            for (int n = 0; n < ng; ++n)
+              U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) = U(i - ih * n, j - jh * n, k - kh * n, v);
+        }
+        if (type == FST_BC_REFLECTIVE || type == FST_BC_HYDROSTATIC) {
+          for (int v = 0; v < nv; ++v) {
+            double reflect = 1.0;
+            if (ih != 0 && v == 0) reflect = -1.0;
+            if (jh != 0 && v == 1) reflect = -1.0;
+            if (kh != 0 && v == 2) reflect = -1.0;
+            for (int n = 0; n < ng; ++n)
+              U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) = reflect * U(i - ih * n, j - jh * n, k - kh * n, v);
+          }
+        }
+        if (type == FST_BC_NOSLIP) {
+          for (int v = 0; v < nv; ++v) {
+            double reflect = 1.0;
+            if (ih != 0 && (v == 0 || v == 1 || v == 2)) reflect = -1.0;
+            if (jh != 0 && (v == 0 || v == 1 || v == 2)) reflect = -1.0;
+            if (kh != 0 && (v == 0 || v == 1 || v == 2)) reflect = -1.0;
+            for (int n = 0; n < ng; ++n)
+              U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) = reflect * U(i - ih * n, j - jh * n, k - kh * n, v);
+          }
+        }
+        if (type == FST_BC_HYDROSTATIC) {
+          const int edx = ndim, rdx = ndim + 1;
+          for (int n = 0; n < ng; ++n) {
+            int ig = i + ih * (n + 1), jg = j + jh * (n + 1), kg = k + kh * (n + 1);
+            int i1_ = ig - ih, j1_ = jg - jh, k1_ = kg - kh;
+            int i2_ = ig - 2 * ih, j2_ = jg - 2 * jh, k2_ = kg - 2 * kh;
+            double ke = 0, ke1 = 0, ke2 = 0;
+            for (int d = 0; d < ndim; ++d) {
+              ke += 0.5 * (U(ig, jg, kg, d) * U(ig, jg, kg, d));
+              ke1 += 0.5 * (U(i1_, j1_, k1_, d) * U(i1_, j1_, k1_, d));
+              ke2 += 0.5 * (U(i2_, j2_, k2_, d) * U(i2_, j2_, k2_, d));
+            }
+            ke = ke / U(ig, jg, kg, rdx);
+            ke1 = ke1 / U(i1_, j1_, k1_, rdx);
+            ke2 = ke2 / U(i2_, j2_, k2_, rdx);
+            U(ig, jg, kg, edx) = 2 * (U(i1_, j1_, k1_, edx) - ke1) - (U(i2_, j2_, k2_, edx) - ke2) + ke;
+          }
+        }
+      }
+}
+
+// applyBCs, src/bc.cpp:179-249 (no-MPI branch); bc_xPer/yPer/zPer :133-176
+void apply_bcs(Orc &o) {
+  const int ng = o.ng, nvt = o.nvt;
+  auto U = [&](int a, int b, int c, int v) -> double & { return o.var[o.id(a, b, c) + o.N * v]; };
+  if (o.c.xPer == 1)
+    for (int v = 0; v < nvt; ++v)
+      for (int k = 0; k < o.ngk; ++k)
+        for (int j = 0; j < o.ngj; ++j)
+          for (int g = 0; g < ng; ++g) {
+            U(g, j, k, v) = U(o.nci + g, j, k, v);
+            U(o.nci + ng + g, j, k, v) = U(ng + g, j, k, v);
+          }
+  if (o.c.yPer == 1)
+    for (int v = 0; v < nvt; ++v)
+      for (int k = 0; k < o.ngk; ++k)
+        for (int i = 0; i < o.ngi; ++i)
+          for (int g = 0; g < ng; ++g) {
+            U(i, g, k, v) = U(i, o.ncj + g, k, v);
+            U(i, o.ncj + ng + g, k, v) = U(i, ng + g, k, v);
+          }
+  if (o.ndim == 3 && o.c.zPer == 1)
+    for (int v = 0; v < nvt; ++v)
+      for (int j = 0; j < o.ngj; ++j)
+        for (int i = 0; i < o.ngi; ++i)
+          for (int g = 0; g < ng; ++g) {
+            U(i, j, g, v) = U(i, j, o.nck + g, v);
+            U(i, j, o.nck + ng + g, v) = U(i, j, ng + g, v);
+          }
+  // xMinus = -1 + xPer etc. (src/input.cpp:408-413)
+  if (!o.c.xPer) {
+    bc_gen(o, o.c.bcL, -1, 0, 0, ng, ng + 1, 0, o.ngj, 0, o.ngk);
+    bc_gen(o, o.c.bcR, 1, 0, 0, ng + o.nci - 1, ng + o.nci, 0, o.ngj, 0, o.ngk);
+  }
+  if (!o.c.yPer) {
+    bc_gen(o, o.c.bcB, 0, -1, 0, 0, o.ngi, ng, ng + 1, 0, o.ngk);
+    bc_gen(o, o.c.bcT, 0, 1, 0, 0, o.ngi, ng + o.ncj - 1, ng + o.ncj, 0, o.ngk);
+  }
+  if (o.ndim == 3 && !o.c.zPer) {
+    bc_gen(o, o.c.bcH, 0, 0, -1, 0, o.ngi, 0, o.ngj, ng, ng + 1);
+    bc_gen(o, o.c.bcF, 0, 0, 1, 0, o.ngi, 0, o.ngj, ng + o.nck - 1, ng + o.nck);
+  }
+}
+
+// ---------------------------------------------------------------- modules
+
+// cart3d_func::compute, src/cart3d.cpp:220-298
+void compute3d(Orc &o) {
+  apply_bcs(o);
+  eos3d(o);
+  vel3d(o);
+  for (int v = 0; v < o.nv; ++v) {
+    flux3d(o, v);
+    advect3d(o, v);
+  }
+  presgrad3d(o);
+  if (o.c.buoyancy) buoyancy3d(o);
+  if (o.c.visc) {
+    for (int d = 0; d < 3; ++d) {
+      stress3d(o, d);
+      applyvisc3d(o, d);
+    }
+  }
+  if (o.c.ceq) {
+    rhograd3d(o);
+    maxima3d(o);
+    o.alpha = (o.dxmag / (o.maxCh + 1.0e-6)) * o.c.alpha;
+    updateceq3d(o);
+    ceqfaces3d(o);
+    applyceq3d(o);
+  }
+}
+
+// cart2d_func::compute, src/cart2d.cpp:151-237
+void compute2d(Orc &o) {
+  eos2d(o);
+  vel2d(o);
+  for (int v = 0; v < o.nv; ++v) {
+    flux2d(o, v);
+    advect2d(o, v);
+  }
+  presgrad2d(o);
+  if (o.c.buoyancy) buoyancy2d(o);
+  if (o.c.visc) visc2d(o);
+  if (o.c.ceq) ceq2d(o);
+}
+
+void compute(Orc &o) {
+  if (o.ndim == 3) compute3d(o);
+  else compute2d(o);
+}
+
+// rkAdvance stage updates, src/rk.cpp:42-50 (stage 1) and :63-69 (stage 2)
+void rk_stage(Orc &o, int stage) {
+  const double dt = o.c.dt;
+  const size_t tot = o.N * o.nvt;
+  if (stage == 1) {
+#pragma omp parallel for
+    for (size_t q = 0; q < tot; ++q) {
+      o.tmp1[q] = o.var[q];
+      o.var[q] = o.var[q] + 0.5f * dt * o.dvar[q];
+    }
+  } else {
+#pragma omp parallel for
+    for (size_t q = 0; q < tot; ++q) o.var[q] = o.tmp1[q] + dt * o.dvar[q];
+  }
+}
+
+void pre_step(Orc &o) {
+  if (o.ndim == 2) apply_bcs(o);  // src/cart2d.cpp:147-149
+}
+
+// secondary variables: calculateRhoPT + computeVelocity + copyExtraVars
+// (src/cart3d.cpp:176-185, src/cart2d.cpp:136-145, src/secondary.hpp:25-59)
+void secondary(Orc &o) {
+  if (o.ndim == 3) { eos3d(o); vel3d(o); }
+  else { eos2d(o); vel2d(o); }
+  const size_t N = o.N;
+  for (size_t c = 0; c < N; ++c) {
+    for (int d = 0; d < o.ndim; ++d) o.varx[c + N * d] = o.vel[c + N * d];
+    o.varx[c + N * (o.ndim + 0)] = o.p[c];
+    o.varx[c + N * (o.ndim + 1)] = o.T[c];
+    o.varx[c + N * (o.ndim + 2)] = o.rho[c];
+  }
+}
+
+void pre_sim(Orc &o) {
+  if (o.ndim == 2) apply_bcs(o);  // src/cart2d.cpp:137
+  secondary(o);
+}
+
+std::vector<double> *array_of(Orc &o, int a, size_t *count) {
+  std::vector<double> *v = nullptr;
+  switch (a) {
+    case FST_VAR: v = &o.var; break;
+    case FST_TMP1: v = &o.tmp1; break;
+    case FST_DVAR: v = &o.dvar; break;
+    case FST_VARX: v = &o.varx; break;
+    case FST_P: v = &o.p; break;
+    case FST_RHO: v = &o.rho; break;
+    case FST_T: v = &o.T; break;
+    case FST_VEL: v = &o.vel; break;
+    case FST_GRADRHO: v = &o.gradRho; break;
+    default: return nullptr;
+  }
+  if (count) *count = v->size();
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+void *orc_create(const fst_config *cfg) {
+  Orc *o = new Orc();
+  o->c = *cfg;
+  o->ndim = cfg->ndim;
+  o->ns = cfg->ns;
+  o->ng = cfg->ng;
+  o->nci = cfg->nci;
+  o->ncj = cfg->ncj;
+  o->nck = cfg->ndim == 3 ? cfg->nck : 1;
+  o->ngi = o->nci + 2 * o->ng;
+  o->ngj = o->ncj + 2 * o->ng;
+  o->ngk = cfg->ndim == 3 ? o->nck + 2 * o->ng : 1;
+  o->nv = cfg->ndim == 3 ? 4 + o->ns : 3 + o->ns;
+  o->nvt = o->nv + (cfg->ceq ? 5 : 0);
+  o->nvx = o->ndim + 3;
+  if (cfg->ndim == 2) o->c.dz = cfg->dx;  // src/input.cpp:356
+  o->N = (size_t)o->ngi * o->ngj * o->ngk;
+  const size_t N = o->N;
+  o->cd.assign(6 + 3 * o->ns, 0.0);
+  o->cd[0] = o->ns;
+  o->cd[1] = o->c.dx;
+  o->cd[2] = o->c.dy;
+  o->cd[3] = o->c.dz;
+  o->cd[4] = o->nv;
+  o->cd[5] = o->ng;
+  for (int s = 0; s < o->ns; ++s) {
+    o->cd[6 + 3 * s] = cfg->gamma[s];
+    o->cd[6 + 3 * s + 1] = cfg->R / cfg->M[s];
+    o->cd[6 + 3 * s + 2] = cfg->mu[s];
+  }
+  o->dxmag = o->c.dx * o->c.dx + o->c.dy * o->c.dy + o->c.dz * o->c.dz;
+  o->var.assign(N * o->nvt, 0.0);
+  o->tmp1.assign(N * o->nvt, 0.0);
+  o->dvar.assign(N * o->nvt, 0.0);
+  o->varx.assign(N * o->nvx, 0.0);
+  o->p.assign(N, 0.0);
+  o->rho.assign(N, 0.0);
+  o->T.assign(N, 0.0);
+  o->vel.assign(N * o->ndim, 0.0);
+  o->fluxx.assign(N, 0.0);
+  o->fluxy.assign(N, 0.0);
+  o->fluxz.assign(o->ndim == 3 ? N : 0, 0.0);
+  if (cfg->visc) {
+    if (o->ndim == 3) o->stress.assign(N * 6, 0.0);
+    else {
+      o->stressx.assign(N * 4, 0.0);
+      o->stressy.assign(N * 4, 0.0);
+      o->qx.assign(N, 0.0);
+      o->qy.assign(N, 0.0);
+    }
+  }
+  if (cfg->ceq) {
+    o->gradRho.assign(N * (o->ndim == 3 ? 5 : 4), 0.0);
+    o->mFlux.assign(o->ndim == 3 ? 27 * N : 8 * N, 0.0);
+  }
+  return o;
+}
+
+void orc_destroy(void *h) { delete (Orc *)h; }
+
+long long orc_size(void *h, int array) {
+  size_t n = 0;
+  if (!array_of(*(Orc *)h, array, &n)) return -1;
+  return (long long)n;
+}
+
+int orc_set(void *h, int array, const double *src) {
+  size_t n;
+  auto *v = array_of(*(Orc *)h, array, &n);
+  if (!v) return 1;
+  std::memcpy(v->data(), src, n * sizeof(double));
+  return 0;
+}
+
+int orc_get(void *h, int array, double *dst) {
+  size_t n;
+  auto *v = array_of(*(Orc *)h, array, &n);
+  if (!v) return 1;
+  std::memcpy(dst, v->data(), n * sizeof(double));
+  return 0;
+}
+
+void orc_apply_bcs(void *h) { apply_bcs(*(Orc *)h); }
+void orc_compute(void *h) { compute(*(Orc *)h); }
+void orc_rk_stage(void *h, int stage) { rk_stage(*(Orc *)h, stage); }
+void orc_pre_sim(void *h) { pre_sim(*(Orc *)h); }
+void orc_pre_step(void *h) { pre_step(*(Orc *)h); }
+void orc_post_step(void *h) { secondary(*(Orc *)h); }
+
+// Fiesta::step (src/fiesta.cpp:333-340): preStep, rkAdvance, postStep
+void orc_step(void *h, int nsteps) {
+  Orc &o = *(Orc *)h;
+  for (int t = 0; t < nsteps; ++t) {
+    pre_step(o);
+    compute(o);
+    rk_stage(o, 1);
+    compute(o);
+    rk_stage(o, 2);
+  }
+}
+
+void orc_maxima(void *h, double *maxS, double *maxCh, double *alpha) {
+  Orc &o = *(Orc *)h;
+  *maxS = o.maxS;
+  *maxCh = o.maxCh;
+  *alpha = o.alpha;
+}
+
+double orc_weno3d(double f1, double f2, double f3, double f4, double f5) { return weno3d(f1, f2, f3, f4, f5); }
+double orc_weno2d(double f1, double f2, double f3, double f4, double f5) { return weno2d(f1, f2, f3, f4, f5); }
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+}  // extern "C"
