@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Benchmark of the FIESTA RHS + RK2 path on one MI355X.
+
+Workload: 3D Rayleigh-Taylor, 256^3 cells, 2 species (air over light gas),
+viscous, C-equation, gravity, RK2 midpoint (the reference integrator), FP64.
+A step is one full RK2 time step (two RHS evaluations + stage updates); the
+metric counts interior cells x 2 stages / wall second (Gcell-stage/s).
+
+Prints ONE JSON line.  `value` is the FP64 plain path (bitwise parity with
+the oracle).  The roofline block covers the dominant kernel, timed live with
+HIP events on the stream it runs on; algorithmic bytes are 20*nvt per
+cell-stage (stage 1 reads U, writes U*; stage 2 reads U*, U^n, writes U^{n+1}).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--path", choices=["fused", "reference"], default="fused")
+    ap.add_argument("--copy-probe", action="store_true", help="also time a plain copy kernel")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gpu_leg(args):
+    from fiesta_amd import workloads as W
+    from fiesta_amd.config import FST_VAR, Dims
+    from fiesta_amd.runtime import Module
+
+    cfg = W.rt3d_config(args.size)
+    d = Dims(cfg)
+    t0 = time.time()
+    var = W.rt3d_state(cfg)
+    log("state generated in %.1fs (%d^3, nvt=%d)" % (time.time() - t0, args.size, d.nvt))
+    m = Module(cfg)
+    m.upload(FST_VAR, var)
+    del var
+    m.pre_sim()
+    fused_ok = True
+    if args.path == "reference":
+        m.set_fused(False)
+    else:
+        try:
+            m.set_fused(True)
+        except Exception:
+            fused_ok = False
+            m.set_fused(False)
+    m.step(args.warmup)
+    m.sync()
+    log("warmup done")
+    m.profile(True)
+    t0 = time.perf_counter()
+    m.step(args.steps)
+    m.sync()
+    wall = time.perf_counter() - t0
+    launches, kms, kname = m.profile_read()
+    m.profile(False)
+    ev_ms = m.last_step_ms()
+    cell_stages = d.ncells * 2 * args.steps
+    value = cell_stages / wall / 1e9
+    res = dict(cfg=cfg, dims=d, wall=wall, ev_ms=ev_ms, value=value, launches=launches, kernel_ms=kms,
+               kernel=kname, fused=fused_ok and args.path == "fused")
+    if args.copy_probe:
+        nbytes = 2 << 30
+        ms = m.copy_probe(nbytes, 10)
+        res["copy_gbs"] = 2 * nbytes / (ms * 1e-3) / 1e9
+    m.close()
+    return res
+
+
+def cpu_leg(args, cfg_size):
+    import oracle
+    from fiesta_amd import workloads as W
+    from fiesta_amd.config import FST_VAR
+
+    threads = min(os.cpu_count() or 1, 16)
+    oracle.set_threads(threads)
+    cfg = W.rt3d_config(cfg_size)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, W.rt3d_state(cfg))
+    o.pre_sim()
+    o.step(1)
+    t0 = time.perf_counter()
+    o.step(args.cpu_steps)
+    wall = time.perf_counter() - t0
+    cs = cfg.nci * cfg.ncj * cfg.nck * 2 * args.cpu_steps
+    return dict(value=cs / wall / 1e9, unit="Gcell-stage/s", cores=threads, kind="port",
+                sample="%d RK2 steps (after 1 warm-up) of the %d^3 workload with the oracle "
+                       "(C++/OpenMP restatement of the reference kernels), %d threads" % (args.cpu_steps, cfg_size, threads))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (multi-GPU sharding is "
+              "a design note, not a measured path)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    r = gpu_leg(args)
+    d = r["dims"]
+    nvt = d.nvt
+    balg = 20 * nvt  # bytes per cell-stage (FP64 RK2 with tmp as a pointer swap)
+    roof = None
+    if r["launches"] > 0:
+        avg_ms = r["kernel_ms"] / r["launches"]
+        if r["fused"]:
+            per_launch = d.ncells * balg  # mean over alternating stage-1/stage-2 launches
+            unit_note = "fused stage kernel: %d B per cell-stage x %d cells" % (balg, d.ncells)
+        else:
+            # reference sequence: report the whole step against the same bytes
+            per_launch = d.ncells * balg * 2 * r["launches"] and None
+            unit_note = None
+        if per_launch:
+            ach = per_launch / (avg_ms * 1e-3) / 1e9
+            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": r["kernel"],
+                    "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes": unit_note}
+    if roof is None:
+        # whole-step rate against the algorithmic bytes of the step
+        ach = d.ncells * balg * 2 * (r["value"] * 1e9 / (d.ncells * 2)) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "reference kernel sequence (whole step)"}
+    out = {
+        "metric": "Gcell-stage/s, 3D 256^3 2-species: FP64 vs FP32+learned-C; % HBM roofline",
+        "value": round(r["value"], 4),
+        "unit": "Gcell-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["wall"] * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "3D Rayleigh-Taylor %d^3, 2 species, viscous, C-equation, gravity, RK2 (FP64 plain)"
+                               % args.size, "cells": d.ncells, "nvt": nvt,
+                   "path": "fused" if r["fused"] else "reference-sequence"},
+        "roofline": roof,
+    }
+    if "copy_gbs" in r:
+        out["copy_kernel_gbs"] = round(r["copy_gbs"], 1)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_leg(args, args.size)
+        out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 5)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

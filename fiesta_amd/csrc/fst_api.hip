@@ -22,6 +22,7 @@ struct fst_ctx {
   int fused = 1;
   double last_ms = 0.0;
   int device = 0;
+  Prof prof;
 };
 
 namespace {
@@ -198,6 +199,7 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
     return FST_ERR_NOMEM;
   }
   ctx->fused = fused_supported(c) ? 1 : 0;
+  ctx->b.prof = &ctx->prof;
   *out = ctx;
   return FST_OK;
 }
@@ -229,6 +231,7 @@ int fst_upload(fst_ctx *ctx, int array, const double *host, size_t count) {
   if (count != n) return fail(ctx, FST_ERR_ARG, "fst_upload: count does not match the array size");
   HIPCHK(ctx, hipMemcpyAsync(p, host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (array == FST_VAR) ctx->fb.valid = false;
   return FST_OK;
 }
 
@@ -245,12 +248,14 @@ int fst_download(fst_ctx *ctx, int array, double *host, size_t count) {
 
 int fst_apply_bcs(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
+  ctx->fb.valid = false;
   launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
   return check_launch(ctx, "fst_apply_bcs");
 }
 
 int fst_pre_sim(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
+  ctx->fb.valid = false;
   if (ctx->g.ndim == 2) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);  // src/cart2d.cpp:137
   launch_secondary(ctx->g, ctx->ph, ctx->b, ctx->stream);
   return check_launch(ctx, "fst_pre_sim");
@@ -258,12 +263,14 @@ int fst_pre_sim(fst_ctx *ctx) {
 
 int fst_pre_step(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
+  ctx->fb.valid = false;
   if (ctx->g.ndim == 2) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);  // src/cart2d.cpp:147-149
   return check_launch(ctx, "fst_pre_step");
 }
 
 int fst_compute(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
+  ctx->fb.valid = false;
   if (ctx->g.ndim == 3) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);  // src/cart3d.cpp:221
   launch_compute_ref(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->stream);
   return check_launch(ctx, "fst_compute");
@@ -271,6 +278,7 @@ int fst_compute(fst_ctx *ctx) {
 
 int fst_rk_stage(fst_ctx *ctx, int stage) {
   if (!ctx || (stage != 1 && stage != 2)) return FST_ERR_ARG;
+  ctx->fb.valid = false;
   launch_rk_stage(ctx->g, stage, ctx->cf.dt, ctx->b.var, ctx->b.tmp1, ctx->b.dvar, ctx->stream);
   return check_launch(ctx, "fst_rk_stage");
 }
@@ -296,6 +304,7 @@ int fst_step(fst_ctx *ctx, int nsteps) {
     int rc = fused_steps(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->fb, nsteps, ctx->stream, &ctx->err);
     if (rc) return rc;
   } else {
+    ctx->fb.valid = false;
     for (int t = 0; t < nsteps; ++t) {
       if (ctx->g.ndim == 2) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
       for (int stage = 1; stage <= 2; ++stage) {
@@ -325,6 +334,56 @@ int fst_last_maxima(fst_ctx *ctx, double *maxS, double *maxCh) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   *maxS = as_double(s.maxS);
   *maxCh = as_double(s.maxCh);
+  return FST_OK;
+}
+
+int fst_profile_enable(fst_ctx *ctx, int enable) {
+  if (!ctx) return FST_ERR_ARG;
+  ctx->prof.on = enable != 0;
+  ctx->prof.used = 0;
+  return FST_OK;
+}
+
+int fst_profile_read(fst_ctx *ctx, int64_t *launches, double *total_ms, char *name, size_t name_len) {
+  if (!ctx || !launches || !total_ms) return FST_ERR_ARG;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  double tot = 0.0;
+  for (size_t q = 0; q + 1 < ctx->prof.used; q += 2) {
+    float f = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&f, ctx->prof.ev[q], ctx->prof.ev[q + 1]));
+    tot += f;
+  }
+  *launches = (int64_t)(ctx->prof.used / 2);
+  *total_ms = tot;
+  if (name && name_len) {
+    std::strncpy(name, ctx->prof.name, name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  ctx->prof.used = 0;
+  return FST_OK;
+}
+
+int fst_copy_probe(fst_ctx *ctx, size_t bytes, int reps, double *ms_per_copy) {
+  if (!ctx || !ms_per_copy || bytes < 16 || reps < 1) return FST_ERR_ARG;
+  bytes &= ~(size_t)15;
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(ctx, hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(ctx, FST_ERR_NOMEM, "fst_copy_probe: allocation failed");
+  }
+  (void)hipMemsetAsync(a, 0, bytes, ctx->stream);
+  launch_copy(a, b, bytes, ctx->stream);  // warm-up
+  (void)hipEventRecord(ctx->ev0, ctx->stream);
+  for (int r = 0; r < reps; ++r) launch_copy(a, b, bytes, ctx->stream);
+  (void)hipEventRecord(ctx->ev1, ctx->stream);
+  hipError_t e = hipEventSynchronize(ctx->ev1);
+  float f = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&f, ctx->ev0, ctx->ev1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(ctx, FST_ERR_HIP, std::string("fst_copy_probe: ") + hipGetErrorString(e));
+  *ms_per_copy = (double)f / reps;
   return FST_OK;
 }
 

@@ -9,8 +9,10 @@ namespace fst {
 
 struct FusedBufs {
   double *stage = nullptr;  // U* (stage-1 output), nvt planes
-  double *prim = nullptr;   // primitive planes rho, p, u, v, w
-  size_t planes_stage = 0;
+  double *prim = nullptr;   // two sets of primitive planes rho, p, u, v, w, nrg, Y_s
+  Scalars *sc = nullptr;    // two sets of per-stage maxima
+  int kchunk = 0;           // k planes per block (0: default)
+  bool valid = false;       // prim[0] / sc[0] describe the current var (interior)
 };
 
 bool fused_supported(const fst_config &cf);

@@ -1,20 +1,709 @@
-// Fused 3D FP64 stage kernels (placeholder until the fused kernel lands).
+// Fused 3D FP64 stage path (cart3d, src/cart3d.cpp:220-298 + src/rk.cpp:28-72).
+//
+// One Runge-Kutta stage = ghost fill (src/bc.cpp:179-249) + ghost primitives +
+// two marching kernels:
+//   k_adv3d   advective fluxes on every face (WENO5), flux differences,
+//             pressure gradient and buoyancy -> partial RHS of the nv flow
+//             variables (dvar planes);
+//   k_vcr3d   viscous stresses and C-equation fluxes on every face, their
+//             divergences, the C-equation sources, the stage update, and the
+//             primitives + maxima of the new state for the next stage.
+// Every term keeps the reference association and constants, so the result
+// equals the reference kernel sequence bit for bit:
+//   * divisions by run constants (dx, 4*dy, 12*dx, 12) use q0 = a*r,
+//     e = fma(-b, q0, a), q = fma(e, r, q0) with r = RN(1/b) (Markstein's
+//     theorem: q is the correctly rounded a/b); other divisions are IEEE.
+//   * per-cell dvar is accumulated in the reference order: advection (x,y,z),
+//     pressure gradient, buoyancy, viscous x, y, z, C-equation.
+//
+// Work decomposition of both kernels (block = 9 waves of 64 lanes, tile =
+// 64 (i) x 8 (j) cells, marching k over a chunk):
+//   waves 0..7  row r: x-face right of its cell, y-face below its row, z-face
+//               above its cell, then the cell itself;
+//   wave 8      y-face above the tile, and the 8 x-faces left of the tile.
+// x and y face values are exchanged through LDS; the z-face of the previous
+// plane stays in registers.  Each face is computed exactly once per kernel.
+//
+// Primitive planes P (rho, p, u, v, w, nrg, Y_0..Y_{ns-1}) of the stage input
+// come from the previous stage's epilogue (interior) and a ghost pass; the
+// per-stage maxima (max wave speed, max |C_hat|) likewise.
+//
+// Addressing: element offsets are 32-bit (checked on the host: every array
+// stays below 2^29 elements) on uniform base pointers.
+#include <algorithm>
+#include <cmath>
+
 #include "fst_fused.hpp"
 
 namespace fst {
 
-bool fused_supported(const fst_config &cf) { (void)cf; return false; }
+namespace {
 
-int fused_steps(const Geo &, const Phys &, const fst_config &, Bufs &, FusedBufs &, int, hipStream_t,
-                std::string *err) {
-  *err = "fused path not available";
-  return FST_ERR_ARG;
+constexpr int TX = 64;
+constexpr int TY = 8;
+constexpr int NWAVE = TY + 1;
+constexpr int NTHREADS = 64 * NWAVE;
+constexpr int NFV = 12;  // 3 stresses + 9 C-equation fluxes per face
+
+struct FC {  // run constants
+  double d[3], rd[3];      // cell sizes and RN(1/d)
+  double d4[3], rd4[3];    // 4*d
+  double d12[3], rd12[3];  // 12*d (C-equation gradients)
+  double pg[3], rpg[3];    // pressure-gradient divisors 12*cd(1), 12*cd(2), 12*cd(2)
+  double r12;              // RN(1/12), face-velocity interpolation
+  double cpf[FST_MAX_SPECIES], cvf[FST_MAX_SPECIES], mus[FST_MAX_SPECIES];
+  double g, rhoRef, kap, eps, alpha_cf, dxmag2, dxmag;
+};
+
+struct Grid32 {
+  uint32_t N, s1, s2;  // plane size, j stride, k stride
+  int ngi, ngj, ng, nci, ncj, nck, nv, nvt;
+  __device__ __forceinline__ uint32_t id(int i, int j, int k) const {
+    return (uint32_t)i + (uint32_t)j * s1 + (uint32_t)k * s2;
+  }
+};
+
+__device__ __forceinline__ double ld(const double *__restrict__ base, uint32_t off) {
+  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + (off << 3));
+}
+__device__ __forceinline__ void st(double *__restrict__ base, uint32_t off, double v) {
+  *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + (off << 3)) = v;
+}
+
+__device__ __forceinline__ double cdiv(double a, double b, double rb) {
+  double q0 = a * rb;
+  double e = fma(-b, q0, a);
+  return fma(e, rb, q0);
+}
+
+template <int D>
+__device__ __forceinline__ uint32_t stride(const Grid32 &g) {
+  return D == 0 ? 1u : (D == 1 ? g.s1 : g.s2);
+}
+
+// Tile placement shared by both kernels.  Hardware block b runs on XCD b % 8;
+// the remap gives consecutive logical tiles (neighbours in i, then j) the same
+// XCD, so halo rows are found in that XCD's L2.
+struct Tile {
+  int w, l, i0, j0, kb, ke, i, jrow;
+  bool cellok, yfok, xbok;
+};
+
+__device__ __forceinline__ Tile place(const Grid32 &g, int kchunk, int nbx, int nby, int nbz) {
+  Tile t;
+  const int tid = threadIdx.x;
+  t.w = tid >> 6;
+  t.l = tid & 63;
+  const int nb = nbx * nby * nbz;
+  const int hb = blockIdx.x;
+  int lb = hb;
+  const int full = (nb / 8) * 8;
+  if (hb < full) lb = (hb % 8) * (full / 8) + hb / 8;
+  const int bx = lb % nbx, by = (lb / nbx) % nby, bz = lb / (nbx * nby);
+  t.i0 = g.ng + bx * TX;
+  t.j0 = g.ng + by * TY;
+  t.kb = g.ng + bz * kchunk;
+  t.ke = min(t.kb + kchunk, g.ng + g.nck);
+  const int iend = g.ng + g.nci, jend = g.ng + g.ncj;
+  t.i = t.i0 + t.l;
+  t.jrow = t.j0 + t.w;
+  t.cellok = (t.w < TY) && (t.jrow < jend) && (t.i < iend);
+  t.yfok = (t.i < iend) && (t.j0 + t.w - 1 < jend);
+  t.xbok = (t.w == TY) && (t.l < TY) && (t.j0 + t.l < jend);
+  return t;
+}
+
+// ---------------------------------------------------------------- faces
+
+// Advective fluxes of the nv flow variables on the face between c and c+e_D
+// (calculateFluxesG, src/flux.hpp:266-309).
+template <int D, int NS>
+__device__ __forceinline__ void face_adv(const FC &k, const Grid32 &g, const double *__restrict__ U,
+                                         const double *__restrict__ P, uint32_t c, double *out) {
+  const uint32_t N = g.N, e = stride<D>(g);
+  const uint32_t vb = N * (2 + D);
+  const double v_m1 = ld(P, c - e + vb), v_0 = ld(P, c + vb);
+  const double v_p1 = ld(P, c + e + vb), v_p2 = ld(P, c + 2 * e + vb);
+  const double ud = cdiv(-v_p2 + 7.0f * v_p1 + 7.0f * v_0 - v_m1, 12.0, k.r12);  // :278
+  const bool neg = ud < 0.0;
+#pragma unroll
+  for (int v = 0; v < 4 + NS; ++v) {
+    double q[6];  // q[t] at c + (t-2)*e: var(v) + (v==3)*p
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+      const uint32_t cc = c + (uint32_t)(t - 2) * e;
+      q[t] = (v == 3) ? ld(U, cc + N * v) + ld(P, cc + N) : ld(U, cc + N * v);
+    }
+    const double f1 = neg ? q[5] : q[0];
+    const double f2 = neg ? q[4] : q[1];
+    const double f3 = neg ? q[3] : q[2];
+    const double f4 = neg ? q[2] : q[3];
+    const double f5 = neg ? q[1] : q[4];
+    out[v] = cdiv(ud * weno5_f32c(f1, f2, f3, f4, f5), k.d[D], k.rd[D]);
+  }
+}
+
+// Viscous stresses and C-equation fluxes on the face between c and c+e_D.
+// out[0..2]: the stress components the divergence in direction D consumes
+// (calculateStressTensor{x,y,z}3dv, src/viscosity.hpp:416-624);
+// out[3+3*dir+w]: mFlux(D, dir, w) (calculateCeqFaces src/ceq3d.hpp:246-295
+// times calculateCeqGrads src/ceq3d.hpp:306-331).  Both use the same nine
+// face derivatives of the velocity.
+template <int D, int NS>
+__device__ __forceinline__ void face_vc(const FC &k, const Grid32 &g, const double *__restrict__ U,
+                                        const double *__restrict__ P, double alpha, int visc, int ceq,
+                                        uint32_t c, double *out) {
+  const uint32_t N = g.N, e = stride<D>(g);
+  constexpr int T1 = D == 0 ? 1 : 0;
+  constexpr int T2 = D == 2 ? 1 : 2;
+  const uint32_t t1 = T1 == 1 ? g.s1 : 1u;
+  const uint32_t t2 = T2 == 1 ? g.s1 : g.s2;
+  double der[3][3];
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    const uint32_t vb = N * (2 + w);
+    der[w][D] = cdiv(ld(P, c + e + vb) - ld(P, c + vb), k.d[D], k.rd[D]);
+    der[w][T1] = cdiv((ld(P, c + t1 + vb) + ld(P, c + e + t1 + vb)) - (ld(P, c - t1 + vb) + ld(P, c + e - t1 + vb)),
+                      k.d4[T1], k.rd4[T1]);
+    der[w][T2] = cdiv((ld(P, c + t2 + vb) + ld(P, c + e + t2 + vb)) - (ld(P, c - t2 + vb) + ld(P, c + e - t2 + vb)),
+                      k.d4[T2], k.rd4[T2]);
+  }
+  if (visc) {
+    double mu = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) mu += ((ld(P, c + e + N * (6 + s)) + ld(P, c + N * (6 + s))) / 2.0) * k.mus[s];
+    const double dudx = der[0][0], dudy = der[0][1], dudz = der[0][2];
+    const double dvdx = der[1][0], dvdy = der[1][1], dvdz = der[1][2];
+    const double dwdx = der[2][0], dwdy = der[2][1], dwdz = der[2][2];
+    if (D == 0) {
+      out[0] = (2.0 / 3.0) * mu * (2.0 * dudx - dvdy - dwdz);
+      out[1] = mu * (dudy + dvdx);
+      out[2] = mu * (dudz + dwdx);
+    } else if (D == 1) {
+      out[0] = mu * (dudy + dvdx);
+      out[1] = (2.0 / 3.0) * mu * (2.0 * dvdy - dudx - dwdz);
+      out[2] = mu * (dvdz + dwdy);
+    } else {
+      out[0] = mu * (dudz + dwdx);
+      out[1] = mu * (dvdz + dwdy);
+      out[2] = (2.0 / 3.0) * mu * (2.0 * dwdz - dudx - dvdy);
+    }
+  }
+  if (ceq) {
+    const int nv = g.nv;
+    const double r = (ld(P, c) + ld(P, c + e)) / 2.0;
+    const double chat = (ld(U, c + N * (nv + 1)) + ld(U, c + e + N * (nv + 1))) / 2.0;
+    double cn[3];
+    cn[0] = (ld(U, c + N * (nv + 2)) + ld(U, c + e + N * (nv + 2))) / 2.0;
+    cn[1] = (ld(U, c + N * (nv + 3)) + ld(U, c + e + N * (nv + 3))) / 2.0;
+    cn[2] = (ld(U, c + N * (nv + 4)) + ld(U, c + e + N * (nv + 4))) / 2.0;
+    double cmag = 1.0e-6;
+    cmag += cn[0] * cn[0];
+    cmag += cn[1] * cn[1];
+    cmag += cn[2] * cn[2];
+    const double arc = alpha * r * chat;
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+      const double dirac = (dir == D) ? 1.0 : 0.0;
+      const double M = dirac - (cn[D] * cn[dir]) / cmag;
+      const double X = arc * M;
+#pragma unroll
+      for (int w = 0; w < 3; ++w) out[3 + 3 * dir + w] = X * der[w][dir];
+    }
+  }
+}
+
+// Primitives of one cell's state (calculateRhoPT3D + computeVelocity3D,
+// src/secondary.hpp:109-151, src/velocity.hpp:68-83; nrg as
+// src/ceq3d.hpp:138-141) and its max-wave-speed term (maxWaveSpeed,
+// src/ceq3d.hpp:25-63).
+template <int NS>
+__device__ __forceinline__ void prims_of(const FC &k, const double *u, double *pr, double &ms) {
+  double rho = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) rho = rho + u[4 + s];
+  double Cp = 0, Cv = 0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double Y = u[4 + s] / rho;
+    pr[6 + s] = Y;
+    Cp = Cp + Y * k.cpf[s];
+    Cv = Cv + Y * k.cvf[s];
+  }
+  const double gamma = Cp / Cv;
+  const double p = (gamma - 1) * (u[3] - (0.5 / rho) * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+  const double vx = u[0] / rho, vy = u[1] / rho, vz = u[2] / rho;
+  pr[0] = rho;
+  pr[1] = p;
+  pr[2] = vx;
+  pr[3] = vy;
+  pr[4] = vz;
+  pr[5] = u[3] / rho - 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+  const double aa = sqrt(gamma * p / rho);
+  ms = aa + sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) / rho;
+}
+
+// ---------------------------------------------------------------- kernel 1
+
+// advect3D (src/advect.hpp:52-55), applyPressureGradient3D (dz := cd(2),
+// src/presgrad.hpp:53-67), computeBuoyancy3D (src/buoyancy.hpp:33-46):
+// dvar[v] for v < nv.
+template <int NS>
+__global__ void __launch_bounds__(NTHREADS, 2)
+    k_adv3d(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
+            double *__restrict__ dvar, int buoy, int kchunk, int nbx, int nby, int nbz) {
+  constexpr int NV = 4 + NS;
+  __shared__ double xsh[TY][NV][TX + 1];  // index 0: left of the tile; l+1: right of lane l
+  __shared__ double ysh[NWAVE][NV][TX];   // row w: face below tile row w
+  const Tile t = place(g, kchunk, nbx, nby, nbz);
+  const int w = t.w, l = t.l;
+  const uint32_t N = g.N;
+  double Z[NV], Zp[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) Z[q] = Zp[q] = 0.0;
+  if (t.cellok) face_adv<2, NS>(k, g, U, P, g.id(t.i, t.jrow, t.kb - 1), Zp);
+  for (int kk = t.kb; kk < t.ke; ++kk) {
+    uint32_t c = g.id(t.i, t.jrow, kk);
+    uint32_t cxb = g.id(t.i0 - 1, t.j0 + l, kk);
+    asm volatile("" : "+v"(c), "+v"(cxb));  // keep address arithmetic per plane
+    if (t.cellok) {
+      double X[NV];
+      face_adv<0, NS>(k, g, U, P, c, X);
+#pragma unroll
+      for (int q = 0; q < NV; ++q) xsh[w][q][l + 1] = X[q];
+    }
+    if (t.xbok) {
+      double X[NV];
+      face_adv<0, NS>(k, g, U, P, cxb, X);
+#pragma unroll
+      for (int q = 0; q < NV; ++q) xsh[l][q][0] = X[q];
+    }
+    if (t.yfok) {
+      double Y[NV];
+      face_adv<1, NS>(k, g, U, P, c - g.s1, Y);
+#pragma unroll
+      for (int q = 0; q < NV; ++q) ysh[w][q][l] = Y[q];
+    }
+    if (t.cellok) face_adv<2, NS>(k, g, U, P, c, Z);
+    __syncthreads();
+    if (t.cellok) {
+      double dv[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        dv[v] = -((xsh[w][v][l + 1] - xsh[w][v][l]) + (ysh[w + 1][v][l] - ysh[w][v][l]) + (Z[v] - Zp[v]));
+      const uint32_t pb = N;  // pressure plane
+      const uint32_t sx = 1, sy = g.s1, sz = g.s2;
+      const double dxp = cdiv(ld(P, c - 2 * sx + pb) - 8.0 * ld(P, c - sx + pb) + 8.0 * ld(P, c + sx + pb) -
+                                  ld(P, c + 2 * sx + pb), k.pg[0], k.rpg[0]);
+      const double dyp = cdiv(ld(P, c - 2 * sy + pb) - 8.0 * ld(P, c - sy + pb) + 8.0 * ld(P, c + sy + pb) -
+                                  ld(P, c + 2 * sy + pb), k.pg[1], k.rpg[1]);
+      const double dzp = cdiv(ld(P, c - 2 * sz + pb) - 8.0 * ld(P, c - sz + pb) + 8.0 * ld(P, c + sz + pb) -
+                                  ld(P, c + 2 * sz + pb), k.pg[2], k.rpg[2]);
+      dv[0] = dv[0] - dxp;
+      dv[1] = dv[1] - dyp;
+      dv[2] = dv[2] - dzp;
+      if (buoy) {
+        const double rho = ld(P, c);
+        const double vv = ld(U, c + N) / rho;
+        const double rhop = rho - k.rhoRef;
+        const double f = -k.g * rhop;
+        dv[1] += f;
+        dv[3] += vv * f;
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) st(dvar, c + N * v, dv[v]);
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) Zp[q] = Z[q];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- kernel 2
+
+// applyViscousTerm{x,y,z}3dv (src/viscosity.hpp:472-497, 554-579, 637-662),
+// calculateRhoGrad + updateCeq + applyCeq (src/ceq3d.hpp:143-186, 202-223,
+// 343-365), the Runge-Kutta stage update (src/rk.cpp:116-119 / :137-139),
+// then the primitives and the maxima of the new state.
+template <int NS>
+__global__ void __launch_bounds__(NTHREADS, 1)
+    k_vcr3d(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ Un,
+            double *__restrict__ Uo, const double *__restrict__ P, double *__restrict__ Po,
+            const double *__restrict__ dvar, const Scalars *scin, Scalars *scout, double h, int stage, int visc,
+            int ceq, int kchunk, int nbx, int nby, int nbz) {
+  constexpr int NV = 4 + NS;
+  constexpr int NVT = NV + 5;
+  __shared__ double xsh[TY][NFV][TX + 1];
+  __shared__ double ysh[NWAVE][NFV][TX];
+  __shared__ double red[2][NWAVE];
+  const Tile t = place(g, kchunk, nbx, nby, nbz);
+  const int w = t.w, l = t.l;
+  const uint32_t N = g.N;
+  const double maxS = as_double(scin->maxS);
+  const double alpha = (k.dxmag2 / (as_double(scin->maxCh) + 1.0e-6)) * k.alpha_cf;  // src/cart3d.cpp:285
+  const bool faces = visc || ceq;
+  double Z[NFV], Zp[NFV];
+#pragma unroll
+  for (int q = 0; q < NFV; ++q) Z[q] = Zp[q] = 0.0;
+  if (faces && t.cellok) face_vc<2, NS>(k, g, U, P, alpha, visc, ceq, g.id(t.i, t.jrow, t.kb - 1), Zp);
+  double mS = 0.0, mC = 0.0;
+  for (int kk = t.kb; kk < t.ke; ++kk) {
+    uint32_t c = g.id(t.i, t.jrow, kk);
+    uint32_t cxb = g.id(t.i0 - 1, t.j0 + l, kk);
+    asm volatile("" : "+v"(c), "+v"(cxb));
+    if (faces) {
+      if (t.cellok) {
+        double X[NFV];
+        face_vc<0, NS>(k, g, U, P, alpha, visc, ceq, c, X);
+#pragma unroll
+        for (int q = 0; q < NFV; ++q) xsh[w][q][l + 1] = X[q];
+      }
+      if (t.xbok) {
+        double X[NFV];
+        face_vc<0, NS>(k, g, U, P, alpha, visc, ceq, cxb, X);
+#pragma unroll
+        for (int q = 0; q < NFV; ++q) xsh[l][q][0] = X[q];
+      }
+      if (t.yfok) {
+        double Y[NFV];
+        face_vc<1, NS>(k, g, U, P, alpha, visc, ceq, c - g.s1, Y);
+#pragma unroll
+        for (int q = 0; q < NFV; ++q) ysh[w][q][l] = Y[q];
+      }
+      if (t.cellok) face_vc<2, NS>(k, g, U, P, alpha, visc, ceq, c, Z);
+      __syncthreads();
+    }
+    if (t.cellok) {
+      double dv[NVT];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) dv[v] = ld(dvar, c + N * v);
+      if (visc) {
+#pragma unroll
+        for (int D = 0; D < 3; ++D) {
+          const uint32_t e = D == 0 ? 1u : (D == 1 ? g.s1 : g.s2);
+          double T0c, T0m, T1c, T1m, T2c, T2m;
+          if (D == 0) {
+            T0c = xsh[w][0][l + 1]; T0m = xsh[w][0][l];
+            T1c = xsh[w][1][l + 1]; T1m = xsh[w][1][l];
+            T2c = xsh[w][2][l + 1]; T2m = xsh[w][2][l];
+          } else if (D == 1) {
+            T0c = ysh[w + 1][0][l]; T0m = ysh[w][0][l];
+            T1c = ysh[w + 1][1][l]; T1m = ysh[w][1][l];
+            T2c = ysh[w + 1][2][l]; T2m = ysh[w][2][l];
+          } else {
+            T0c = Z[0]; T0m = Zp[0];
+            T1c = Z[1]; T1m = Zp[1];
+            T2c = Z[2]; T2m = Zp[2];
+          }
+          const uint32_t b0 = 2 * N, b1 = 3 * N, b2 = 4 * N;
+          const double V0 = ld(P, c + b0), V1 = ld(P, c + b1), V2 = ld(P, c + b2);
+          const double ur = (ld(P, c + e + b0) + V0) / 2.0, ul = (ld(P, c - e + b0) + V0) / 2.0;
+          const double vr = (ld(P, c + e + b1) + V1) / 2.0, vl = (ld(P, c - e + b1) + V1) / 2.0;
+          const double wr = (ld(P, c + e + b2) + V2) / 2.0, wl = (ld(P, c - e + b2) + V2) / 2.0;
+          const double dd = k.d[D], rdd = k.rd[D];
+          const double ta = cdiv(T0c - T0m, dd, rdd);
+          const double tb = cdiv(T1c - T1m, dd, rdd);
+          const double tc = cdiv(T2c - T2m, dd, rdd);
+          const double td = cdiv(ur * T0c - ul * T0m, dd, rdd) + cdiv(vr * T1c - vl * T1m, dd, rdd) +
+                            cdiv(wr * T2c - wl * T2m, dd, rdd);
+          dv[0] += ta;
+          dv[1] += tb;
+          dv[2] += tc;
+          dv[3] += td;
+        }
+      }
+      if (ceq) {
+        // calculateRhoGrad, src/ceq3d.hpp:143-186 (rho, nrg, vel planes of P)
+        double G[5];
+        {
+          double dr[3], de[3], du[3];
+#pragma unroll
+          for (int D = 0; D < 3; ++D) {
+            const uint32_t e = D == 0 ? 1u : (D == 1 ? g.s1 : g.s2);
+            const uint32_t rb = 0, eb = 5 * N, vb = (2 + D) * N;
+            dr[D] = cdiv(ld(P, c - 2 * e + rb) - 8.0 * ld(P, c - e + rb) + 8.0 * ld(P, c + e + rb) -
+                             ld(P, c + 2 * e + rb), k.d12[D], k.rd12[D]);
+            de[D] = cdiv(ld(P, c - 2 * e + eb) - 8.0 * ld(P, c - e + eb) + 8.0 * ld(P, c + e + eb) -
+                             ld(P, c + 2 * e + eb), k.d12[D], k.rd12[D]);
+            du[D] = cdiv(ld(P, c - 2 * e + vb) - 8.0 * ld(P, c - e + vb) + 8.0 * ld(P, c + e + vb) -
+                             ld(P, c + 2 * e + vb), k.d12[D], k.rd12[D]);
+          }
+          const double rgrad = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+          const double divu = du[0] + du[1] + du[2];
+          const double dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) *
+                                (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+          const int ind = dnednr < 0.0 ? 1 : 0;
+          G[0] = (divu < 0) ? (1 - ind) * rgrad : 0;
+          G[1] = ind * rgrad;
+          G[2] = dr[0];
+          G[3] = dr[1];
+          G[4] = dr[2];
+        }
+        // updateCeq, src/ceq3d.hpp:202-223
+        const double c1 = maxS / (k.eps * k.dxmag);
+        const double c2 = k.kap * maxS * k.dxmag;
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+          const uint32_t cb = N * (NV + n);
+          const double cc = ld(U, c + cb);
+          const double lap = cdiv(ld(U, c - 1 + cb) - 2 * cc + ld(U, c + 1 + cb), k.d[0], k.rd[0]) +
+                             cdiv(ld(U, c - g.s1 + cb) - 2 * cc + ld(U, c + g.s1 + cb), k.d[1], k.rd[1]) +
+                             cdiv(ld(U, c - g.s2 + cb) - 2 * cc + ld(U, c + g.s2 + cb), k.d[2], k.rd[2]);
+          dv[NV + n] = c1 * (G[n] - cc) + c2 * lap;
+        }
+        // applyCeq, src/ceq3d.hpp:343-365
+#pragma unroll
+        for (int ww = 0; ww < 3; ++ww) {
+          double diffu = 0.0;
+#pragma unroll
+          for (int dir = 0; dir < 3; ++dir)
+            diffu += cdiv(xsh[w][3 + 3 * dir + ww][l + 1] - xsh[w][3 + 3 * dir + ww][l], k.d[0], k.rd[0]);
+#pragma unroll
+          for (int dir = 0; dir < 3; ++dir)
+            diffu += cdiv(ysh[w + 1][3 + 3 * dir + ww][l] - ysh[w][3 + 3 * dir + ww][l], k.d[1], k.rd[1]);
+#pragma unroll
+          for (int dir = 0; dir < 3; ++dir)
+            diffu += cdiv(Z[3 + 3 * dir + ww] - Zp[3 + 3 * dir + ww], k.d[2], k.rd[2]);
+          dv[ww] += diffu;
+        }
+      }
+      // Runge-Kutta stage update and primitives of the new state
+      const int nvt = g.nvt;
+      double un[NVT];
+#pragma unroll
+      for (int v = 0; v < NVT; ++v) {
+        if (v < nvt) {
+          const double base = (stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v);
+          un[v] = base + h * dv[v];
+          st(Uo, c + N * v, un[v]);
+        }
+      }
+      double pr[6 + NS];
+      double ms;
+      prims_of<NS>(k, un, pr, ms);
+#pragma unroll
+      for (int q = 0; q < 6 + NS; ++q) st(Po, c + N * q, pr[q]);
+      mS = fmax(mS, ms);
+      if (ceq) mC = fmax(mC, fabs(un[NV + 1]));
+    }
+#pragma unroll
+    for (int q = 0; q < NFV; ++q) Zp[q] = Z[q];
+    if (faces) __syncthreads();
+  }
+  // block maxima -> one atomic per block and value
+  mS = wave_max(mS);
+  mC = wave_max(mC);
+  if (l == 0) {
+    red[0][w] = mS;
+    red[1][w] = mC;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0, cm = 0.0;
+    for (int q = 0; q < NWAVE; ++q) {
+      s = fmax(s, red[0][q]);
+      cm = fmax(cm, red[1][q]);
+    }
+    atomic_max_nonneg(&scout->maxS, s);
+    atomic_max_nonneg(&scout->maxCh, cm);
+  }
+}
+
+// ---------------------------------------------------------------- primitives
+
+// Primitives of U over the whole ghost range (with the maxima of the
+// interior) or over the ghost cells only (after a ghost fill).  Threads 0..3
+// of block 0 clear `clear` when given.
+template <int NS>
+__global__ void k_prims3d(const Grid32 g, const FC k, const double *__restrict__ U, double *__restrict__ P,
+                          Scalars *sc, int ghost_only, Scalars *clear) {
+  if (clear && blockIdx.x == 0 && threadIdx.x < 4) reinterpret_cast<unsigned long long *>(clear)[threadIdx.x] = 0ull;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool inrange = t < g.N;
+  const int i = (int)(t % (uint32_t)g.ngi);
+  const int j = (int)((t / (uint32_t)g.ngi) % (uint32_t)g.ngj);
+  const int kk = (int)(t / g.s2);
+  const int ng = g.ng;
+  const bool interior = i >= ng && i < ng + g.nci && j >= ng && j < ng + g.ncj && kk >= ng && kk < ng + g.nck;
+  double ms = 0.0, mc = 0.0;
+  if (inrange && !(ghost_only && interior)) {
+    const uint32_t N = g.N;
+    double u[4 + NS];
+#pragma unroll
+    for (int v = 0; v < 4 + NS; ++v) u[v] = ld(U, t + N * v);
+    double pr[6 + NS];
+    prims_of<NS>(k, u, pr, ms);
+#pragma unroll
+    for (int q = 0; q < 6 + NS; ++q) st(P, t + N * q, pr[q]);
+    if (!interior) ms = 0.0;
+    else if (g.nvt > g.nv) mc = fabs(ld(U, t + N * (g.nv + 1)));
+  }
+  if (!ghost_only && sc) {
+    ms = wave_max(ms);
+    mc = wave_max(mc);
+    if ((threadIdx.x & 63) == 0) {
+      atomic_max_nonneg(&sc->maxS, ms);
+      atomic_max_nonneg(&sc->maxCh, mc);
+    }
+  }
+}
+
+__global__ void k_clear(Scalars *sc) {
+  if (threadIdx.x < 4) reinterpret_cast<unsigned long long *>(sc)[threadIdx.x] = 0ull;
+}
+
+FC make_fc(const Phys &ph, const fst_config &cf) {
+  FC k;
+  const double d[3] = {cf.dx, cf.dy, cf.dz};
+  for (int q = 0; q < 3; ++q) {
+    k.d[q] = d[q];
+    k.rd[q] = 1.0 / d[q];
+    k.d4[q] = 4 * d[q];
+    k.rd4[q] = 1.0 / k.d4[q];
+    k.d12[q] = 12.0 * d[q];
+    k.rd12[q] = 1.0 / k.d12[q];
+  }
+  k.pg[0] = 12.0 * ph.cd[1];
+  k.pg[1] = 12.0 * ph.cd[2];
+  k.pg[2] = 12.0 * ph.cd[2];
+  for (int q = 0; q < 3; ++q) k.rpg[q] = 1.0 / k.pg[q];
+  k.r12 = 1.0 / 12.0;
+  for (int s = 0; s < FST_MAX_SPECIES; ++s) {
+    k.cpf[s] = ph.cpf[s];
+    k.cvf[s] = ph.cvf[s];
+    k.mus[s] = ph.mus[s];
+  }
+  k.g = ph.g;
+  k.rhoRef = ph.rhoRef;
+  k.kap = ph.kap;
+  k.eps = ph.eps;
+  k.alpha_cf = ph.alpha;
+  k.dxmag2 = ph.dxmag2;
+  k.dxmag = std::sqrt(ph.cd[1] * ph.cd[1] + ph.cd[2] * ph.cd[2] + ph.cd[3] * ph.cd[3]);
+  return k;
+}
+
+Grid32 make_grid(const Geo &g) {
+  Grid32 q;
+  q.N = (uint32_t)g.N;
+  q.s1 = (uint32_t)g.ngi;
+  q.s2 = (uint32_t)g.ngi * (uint32_t)g.ngj;
+  q.ngi = g.ngi; q.ngj = g.ngj; q.ng = g.ng;
+  q.nci = g.nci; q.ncj = g.ncj; q.nck = g.nck;
+  q.nv = g.nv; q.nvt = g.nvt;
+  return q;
+}
+
+struct Launch {
+  int nbx, nby, nbz, nb;
+};
+
+Launch shape(const Geo &g, int kchunk) {
+  Launch L;
+  L.nbx = (g.nci + TX - 1) / TX;
+  L.nby = (g.ncj + TY - 1) / TY;
+  L.nbz = (g.nck + kchunk - 1) / kchunk;
+  L.nb = L.nbx * L.nby * L.nbz;
+  return L;
+}
+
+template <int NS>
+void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf, const double *U,
+               const double *Un, double *Uo, const double *P, double *Po, double *dvar, const Scalars *scin,
+               Scalars *scout, double h, int stage, int kchunk, Prof *prof, hipStream_t s) {
+  const Launch L = shape(g, kchunk);
+  if (prof) prof->begin(s);
+  k_adv3d<NS><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
+  if (prof) prof->end(s);
+  k_vcr3d<NS><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, scin, scout, h, stage, cf.visc, cf.ceq,
+                                        kchunk, L.nbx, L.nby, L.nbz);
+}
+
+template <int NS>
+void run_prims(const Grid32 &q, const FC &k, const double *U, double *P, Scalars *sc, int ghost_only,
+               Scalars *clear, hipStream_t s) {
+  k_prims3d<NS><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, U, P, sc, ghost_only, clear);
+}
+
+using StageFn = void (*)(const Grid32 &, const Geo &, const FC &, const fst_config &, const double *,
+                         const double *, double *, const double *, double *, double *, const Scalars *,
+                         Scalars *, double, int, int, Prof *, hipStream_t);
+using PrimFn = void (*)(const Grid32 &, const FC &, const double *, double *, Scalars *, int, Scalars *,
+                        hipStream_t);
+
+bool pick(int ns, StageFn *sf, PrimFn *pf) {
+  switch (ns) {
+    case 1: *sf = run_stage<1>; *pf = run_prims<1>; return true;
+    case 2: *sf = run_stage<2>; *pf = run_prims<2>; return true;
+    case 3: *sf = run_stage<3>; *pf = run_prims<3>; return true;
+    default: return false;
+  }
+}
+
+}  // namespace
+
+bool fused_supported(const fst_config &cf) {
+  if (!(cf.ndim == 3 && cf.grid == 0 && cf.precision == FST_FP64 && cf.scheme == FST_WENO5 && cf.ns >= 1 &&
+        cf.ns <= 3 && !cf.learned_c))
+    return false;
+  // 32-bit element offsets: the largest array (nvt planes) below 2^29 elements
+  const double N = (double)(cf.nci + 2 * cf.ng) * (cf.ncj + 2 * cf.ng) * (cf.nck + 2 * cf.ng);
+  const int nvt = 4 + cf.ns + (cf.ceq ? 5 : 0);
+  return N * std::max(nvt, 6 + cf.ns) < 536870912.0;
+}
+
+int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb, int nsteps,
+                hipStream_t s, std::string *err) {
+  StageFn sf;
+  PrimFn pf;
+  if (!pick(cf.ns, &sf, &pf) || !fused_supported(cf)) {
+    *err = "fused path: configuration not supported";
+    return FST_ERR_ARG;
+  }
+  const int np = 6 + cf.ns;
+  if (!fb.stage) {
+    const size_t bytes = (size_t)g.N * sizeof(double);
+    if (hipMalloc((void **)&fb.stage, bytes * g.nvt) != hipSuccess ||
+        hipMalloc((void **)&fb.prim, bytes * np * 2) != hipSuccess ||
+        hipMalloc((void **)&fb.sc, sizeof(Scalars) * 2) != hipSuccess) {
+      *err = "fused path: device allocation failed";
+      return FST_ERR_NOMEM;
+    }
+    (void)hipMemsetAsync(fb.stage, 0, bytes * g.nvt, s);
+    (void)hipMemsetAsync(fb.prim, 0, bytes * np * 2, s);
+  }
+  const FC k = make_fc(ph, cf);
+  const Grid32 q = make_grid(g);
+  const int kchunk = fb.kchunk > 0 ? fb.kchunk : 16;
+  double *Pa = fb.prim, *Pb = fb.prim + (size_t)g.N * np;
+  Scalars *S0 = fb.sc, *S1 = fb.sc + 1;
+  Prof *prof = b.prof && b.prof->on ? b.prof : nullptr;
+  if (prof) prof->name = "k_adv3d";
+  if (!fb.valid) {  // primitives and maxima of var, all cells
+    k_clear<<<1, 64, 0, s>>>(S0);
+    pf(q, k, b.var, Pa, S0, 0, nullptr, s);
+    fb.valid = true;
+  }
+  const double h1 = 0.5f * cf.dt, h2 = cf.dt;
+  for (int t = 0; t < nsteps; ++t) {
+    // stage 1: U^n (b.var) -> U* (fb.stage); Pa/S0 hold the primitives/maxima of U^n
+    launch_apply_bcs(g, cf, b.var, s);       // src/cart3d.cpp:221
+    pf(q, k, b.var, Pa, nullptr, 1, S1, s);  // ghost primitives; clears S1
+    sf(q, g, k, cf, b.var, nullptr, fb.stage, Pa, Pb, b.dvar, S0, S1, h1, 1, kchunk, prof, s);
+    // stage 2: U* -> U^{n+1}, written in place into b.var (read pointwise)
+    launch_apply_bcs(g, cf, fb.stage, s);
+    pf(q, k, fb.stage, Pb, nullptr, 1, S0, s);  // ghost primitives of U*; clears S0
+    sf(q, g, k, cf, fb.stage, b.var, b.var, Pb, Pa, b.dvar, S1, S0, h2, 2, kchunk, prof, s);
+  }
+  return FST_OK;
 }
 
 void fused_free(FusedBufs &fb) {
   if (fb.stage) (void)hipFree(fb.stage);
   if (fb.prim) (void)hipFree(fb.prim);
+  if (fb.sc) (void)hipFree(fb.sc);
   fb.stage = fb.prim = nullptr;
+  fb.sc = nullptr;
 }
 
 }  // namespace fst

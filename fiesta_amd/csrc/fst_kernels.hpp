@@ -3,7 +3,34 @@
 #pragma once
 #include "fst_common.hpp"
 
+#include <vector>
+
 namespace fst {
+
+// Event-bracketed timing of one kernel family on a stream.
+struct Prof {
+  bool on = false;
+  const char *name = "";
+  std::vector<hipEvent_t> ev;  // pairs: begin, end
+  size_t used = 0;
+  void begin(hipStream_t s) {
+    if (!on) return;
+    while (ev.size() < used + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
+      ev.push_back(e);
+    }
+    (void)hipEventRecord(ev[used], s);
+  }
+  void end(hipStream_t s) {
+    if (!on) return;
+    (void)hipEventRecord(ev[used + 1], s);
+    used += 2;
+  }
+  ~Prof() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
 
 // Device buffers of one context.  Arrays are LayoutLeft planes of N cells.
 struct Bufs {
@@ -15,6 +42,7 @@ struct Bufs {
   double *mflux = nullptr;   // 3D: 27 planes ((face*3+dir)*3+w); 2D: 8 planes
   double *grid = nullptr, *metrics = nullptr;
   Scalars *sc = nullptr;     // per-stage maxima (device)
+  Prof *prof = nullptr;      // timing of the dominant kernel (host side)
 };
 
 // WENO5 of calculateFluxesG (src/flux.hpp:240-264): binary32 coefficients
@@ -72,6 +100,7 @@ void launch_eos_vel(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
 void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);
 void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream_t s);
 void launch_rk_stage(const Geo &g, int stage, double dt, double *var, double *tmp, const double *dvar, hipStream_t s);
+void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 void launch_secondary(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
 
 }  // namespace fst

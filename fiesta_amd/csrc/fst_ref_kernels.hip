@@ -765,6 +765,13 @@ __global__ void k_rk_stage(long long n, int stage, double dt, double *__restrict
   }
 }
 
+// plain copy, 16 bytes per lane per iteration (bandwidth probe)
+__global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long long n) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; t < n; t += stride) dst[t] = src[t];
+}
+
 // copyExtraVars2D/3D (src/secondary.hpp:25-59): varx = (vel..., p, T, rho)
 __global__ void k_copy_extra(long long N, int ndim, const double *__restrict__ vel, const double *__restrict__ p,
                              const double *__restrict__ T, const double *__restrict__ rho, double *__restrict__ varx) {
@@ -791,7 +798,9 @@ void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, cons
   Range cr = cell_range(g), fr = face_range(g);
   if (g.ndim == 3) {
     for (int v = 0; v < g.nv; ++v) {
+      if (b.prof) { b.prof->name = "k_flux3d"; b.prof->begin(s); }
       k_flux3d<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, v, b.var, b.p, b.vel, b.fx, b.fy, b.fz);
+      if (b.prof) b.prof->end(s);
       k_advect3d<<<grid_for(cr), 256, 0, s>>>(g, cr, v, b.fx, b.fy, b.fz, b.dvar);
     }
     k_presgrad_buoy3d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, cf.buoyancy, b.var, b.p, b.rho, b.dvar);
@@ -863,6 +872,11 @@ void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream
 void launch_rk_stage(const Geo &g, int stage, double dt, double *var, double *tmp, const double *dvar, hipStream_t s) {
   long long n = g.N * g.nvt;
   k_rk_stage<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, stage, dt, var, tmp, dvar);
+}
+
+void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {
+  long long n = (long long)(bytes / 16);
+  k_copy<<<256 * 8 * 4, 256, 0, s>>>((const double2 *)src, (double2 *)dst, n);
 }
 
 void launch_secondary(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s) {

@@ -38,6 +38,10 @@ _EXPORTS = {
     "fst_sync": ([C.c_void_p], C.c_int),
     "fst_set_fused": ([C.c_void_p, C.c_int], C.c_int),
     "fst_last_step_ms": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "fst_profile_enable": ([C.c_void_p, C.c_int], C.c_int),
+    "fst_profile_read": ([C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_char_p, C.c_size_t],
+                         C.c_int),
+    "fst_copy_probe": ([C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
 }
 
 
@@ -147,6 +151,21 @@ class Module:
         a, b = C.c_double(), C.c_double()
         self._chk(lib().fst_last_maxima(self.h, C.byref(a), C.byref(b)), "fst_last_maxima")
         return a.value, b.value
+
+
+    def profile(self, on=True):
+        self._chk(lib().fst_profile_enable(self.h, int(on)), "fst_profile_enable")
+
+    def profile_read(self):
+        n, ms = C.c_int64(), C.c_double()
+        name = C.create_string_buffer(64)
+        self._chk(lib().fst_profile_read(self.h, C.byref(n), C.byref(ms), name, 64), "fst_profile_read")
+        return n.value, ms.value, name.value.decode()
+
+    def copy_probe(self, nbytes, reps=10):
+        ms = C.c_double()
+        self._chk(lib().fst_copy_probe(self.h, nbytes, reps, C.byref(ms)), "fst_copy_probe")
+        return ms.value
 
 
 def rk_advance(f):

@@ -170,6 +170,19 @@ int fst_set_fused(fst_ctx *ctx, int fused);
  * hipEvents on the context stream (Timer::fiestaTimer, src/timer.hpp:52-119). */
 int fst_last_step_ms(fst_ctx *ctx, double *ms);
 
+/* Live timing of the dominant kernel (the per-stage RHS kernel: the fused
+ * stage kernel, or the WENO flux kernel on the reference sequence).  While
+ * enabled, each launch of it is bracketed by hipEvents on the context stream.
+ * fst_profile_read synchronizes and returns the launch count, the summed
+ * duration in ms and the kernel's name; it then clears the counters. */
+int fst_profile_enable(fst_ctx *ctx, int enable);
+int fst_profile_read(fst_ctx *ctx, int64_t *launches, double *total_ms, char *name, size_t name_len);
+
+/* Device-to-device copy of `bytes` bytes with a plain vectorized HIP kernel,
+ * `reps` times, on the context stream; returns the mean ms per copy.  Used to
+ * measure the attainable HBM bandwidth and to scale rocprof byte counters. */
+int fst_copy_probe(fst_ctx *ctx, size_t bytes, int reps, double *ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

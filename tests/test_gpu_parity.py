@@ -122,3 +122,40 @@ def test_rk_stage_split_matches_step():
     a.compute(); a.rk_stage(1); a.compute(); a.rk_stage(2)
     b.set_fused(False); b.step(1)
     assert_bitwise(a.download(FST_VAR), b.download(FST_VAR), "split vs step")
+
+
+FUSED_CASES = dict(CASES_3D)
+FUSED_CASES["multi_tile"] = dict(nci=70, ncj=19, nck=37)
+FUSED_CASES["multi_tile_walls"] = dict(nci=130, ncj=9, nck=20, per=(False, False, False),
+                                       bc=("reflective", "outflow", "hydrostatic", "hydrostatic", "noslip", "outflow"))
+
+
+@pytest.mark.parametrize("name", sorted(FUSED_CASES))
+def test_steps_3d_fused(name):
+    """The fused stage kernels equal the oracle bit for bit over several steps."""
+    cfg = cfg3d(**FUSED_CASES[name])
+    _say("case 3d steps (fused)", name)
+    o, m = _pair(cfg, 31)
+    o.pre_sim(); m.pre_sim()
+    m.set_fused(True)
+    o.step(2); m.step(2)
+    assert_bitwise(m.download(FST_VAR), o.get(FST_VAR), "var after 2 steps")
+    o.step(1); m.step(1)
+    assert_bitwise(m.download(FST_VAR), o.get(FST_VAR), "var after 3 steps")
+    # maxima of the new state equal a fresh reduction of the oracle's state
+    m.compute(); o.compute()
+    if cfg.ceq:
+        assert m.maxima() == o.maxima()[:2]
+
+
+def test_fused_interleaved_with_reference_sequence():
+    """Switching between the fused path and compute()/rk_stage() keeps parity."""
+    cfg = cfg3d(nci=20, ncj=12, nck=10)
+    _say("case fused/reference interleave")
+    o, m = _pair(cfg, 41)
+    o.pre_sim(); m.pre_sim()
+    m.step(1)
+    m.compute(); m.rk_stage(1); m.compute(); m.rk_stage(2)
+    m.step(1)
+    o.step(3)
+    assert_bitwise(m.download(FST_VAR), o.get(FST_VAR), "var after mixed steps")
