@@ -16,11 +16,11 @@
 //   * per-cell dvar is accumulated in the reference order: advection (x,y,z),
 //     pressure gradient, buoyancy, viscous x, y, z, C-equation.
 //
-// Work decomposition of both kernels (block = 9 waves of 64 lanes, tile =
-// 64 (i) x 8 (j) cells, marching k over a chunk):
-//   waves 0..7  row r: x-face right of its cell, y-face below its row, z-face
+// Work decomposition of both kernels (block = 8 waves of 64 lanes, tile =
+// 64 (i) x 7 (j) cells, marching k over a chunk):
+//   waves 0..6  row r: x-face right of its cell, y-face below its row, z-face
 //               above its cell, then the cell itself;
-//   wave 8      y-face above the tile, and the 8 x-faces left of the tile.
+//   wave 7      y-face above the tile, and the 7 x-faces left of the tile.
 // x and y face values are exchanged through LDS; the z-face of the previous
 // plane stays in registers.  Each face is computed exactly once per kernel.
 //
@@ -32,6 +32,7 @@
 // stays below 2^29 elements) on uniform base pointers.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "fst_fused.hpp"
 
@@ -40,7 +41,7 @@ namespace fst {
 namespace {
 
 constexpr int TX = 64;
-constexpr int TY = 8;
+constexpr int TY = 7;
 constexpr int NWAVE = TY + 1;
 constexpr int NTHREADS = 64 * NWAVE;
 constexpr int NFV = 12;  // 3 stresses + 9 C-equation fluxes per face
@@ -511,32 +512,60 @@ __global__ void __launch_bounds__(NTHREADS, 1)
 
 // ---------------------------------------------------------------- primitives
 
-// Primitives of U over the whole ghost range (with the maxima of the
-// interior) or over the ghost cells only (after a ghost fill).  Threads 0..3
-// of block 0 clear `clear` when given.
+// Primitives of U over the whole ghost range, with the maxima of the interior
+// (ghost_only = 0), or over the ghost cells only (ghost_only = 1: threads
+// enumerate the k slabs, then the j slabs, then the i slabs of the ghost
+// shell).  Threads 0..3 of block 0 clear `clear` when given.
 template <int NS>
 __global__ void k_prims3d(const Grid32 g, const FC k, const double *__restrict__ U, double *__restrict__ P,
                           Scalars *sc, int ghost_only, Scalars *clear) {
   if (clear && blockIdx.x == 0 && threadIdx.x < 4) reinterpret_cast<unsigned long long *>(clear)[threadIdx.x] = 0ull;
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool inrange = t < g.N;
-  const int i = (int)(t % (uint32_t)g.ngi);
-  const int j = (int)((t / (uint32_t)g.ngi) % (uint32_t)g.ngj);
-  const int kk = (int)(t / g.s2);
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ng = g.ng;
-  const bool interior = i >= ng && i < ng + g.nci && j >= ng && j < ng + g.ncj && kk >= ng && kk < ng + g.nck;
+  bool inrange, interior = false;
+  uint32_t cell = 0;
+  if (ghost_only) {
+    const uint32_t nk = 2u * ng * g.s2;                          // k < ng or k >= ng+nck
+    const uint32_t nj = 2u * ng * (uint32_t)g.ngi * g.nck;       // interior k, j ghost
+    const uint32_t ni = 2u * ng * (uint32_t)g.ncj * g.nck;       // interior k and j, i ghost
+    inrange = t < nk + nj + ni;
+    if (t < nk) {
+      const uint32_t kq = t / g.s2, r = t % g.s2;
+      const uint32_t kk = kq < (uint32_t)ng ? kq : kq + g.nck;
+      cell = r + kk * g.s2;
+    } else if (t < nk + nj) {
+      t -= nk;
+      const uint32_t i = t % g.ngi, q = t / g.ngi;
+      const uint32_t jq = q % (2u * ng), kq = q / (2u * ng);
+      const uint32_t j = jq < (uint32_t)ng ? jq : jq + g.ncj;
+      cell = i + j * g.s1 + (kq + ng) * g.s2;
+    } else if (inrange) {
+      t -= nk + nj;
+      const uint32_t iq = t % (2u * ng), q = t / (2u * ng);
+      const uint32_t j = q % g.ncj, kq = q / g.ncj;
+      const uint32_t i = iq < (uint32_t)ng ? iq : iq + g.nci;
+      cell = i + (j + ng) * g.s1 + (kq + ng) * g.s2;
+    }
+  } else {
+    inrange = t < g.N;
+    cell = t;
+    const int i = (int)(t % (uint32_t)g.ngi);
+    const int j = (int)((t / (uint32_t)g.ngi) % (uint32_t)g.ngj);
+    const int kk = (int)(t / g.s2);
+    interior = i >= ng && i < ng + g.nci && j >= ng && j < ng + g.ncj && kk >= ng && kk < ng + g.nck;
+  }
   double ms = 0.0, mc = 0.0;
-  if (inrange && !(ghost_only && interior)) {
+  if (inrange) {
     const uint32_t N = g.N;
     double u[4 + NS];
 #pragma unroll
-    for (int v = 0; v < 4 + NS; ++v) u[v] = ld(U, t + N * v);
+    for (int v = 0; v < 4 + NS; ++v) u[v] = ld(U, cell + N * v);
     double pr[6 + NS];
     prims_of<NS>(k, u, pr, ms);
 #pragma unroll
-    for (int q = 0; q < 6 + NS; ++q) st(P, t + N * q, pr[q]);
+    for (int q = 0; q < 6 + NS; ++q) st(P, cell + N * q, pr[q]);
     if (!interior) ms = 0.0;
-    else if (g.nvt > g.nv) mc = fabs(ld(U, t + N * (g.nv + 1)));
+    else if (g.nvt > g.nv) mc = fabs(ld(U, cell + N * (g.nv + 1)));
   }
   if (!ghost_only && sc) {
     ms = wave_max(ms);
@@ -612,9 +641,7 @@ void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf,
                const double *Un, double *Uo, const double *P, double *Po, double *dvar, const Scalars *scin,
                Scalars *scout, double h, int stage, int kchunk, Prof *prof, hipStream_t s) {
   const Launch L = shape(g, kchunk);
-  if (prof) prof->begin(s);
   k_adv3d<NS><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
-  if (prof) prof->end(s);
   k_vcr3d<NS><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, scin, scout, h, stage, cf.visc, cf.ceq,
                                         kchunk, L.nbx, L.nby, L.nbz);
 }
@@ -622,7 +649,10 @@ void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf,
 template <int NS>
 void run_prims(const Grid32 &q, const FC &k, const double *U, double *P, Scalars *sc, int ghost_only,
                Scalars *clear, hipStream_t s) {
-  k_prims3d<NS><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, U, P, sc, ghost_only, clear);
+  uint32_t n = q.N;
+  if (ghost_only)
+    n = 2u * q.ng * q.s2 + 2u * q.ng * (uint32_t)q.ngi * q.nck + 2u * q.ng * (uint32_t)q.ncj * q.nck;
+  k_prims3d<NS><<<(n + 255) / 256, 256, 0, s>>>(q, k, U, P, sc, ghost_only, clear);
 }
 
 using StageFn = void (*)(const Grid32 &, const Geo &, const FC &, const fst_config &, const double *,
@@ -674,11 +704,12 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
   }
   const FC k = make_fc(ph, cf);
   const Grid32 q = make_grid(g);
-  const int kchunk = fb.kchunk > 0 ? fb.kchunk : 16;
+  int kchunk = fb.kchunk > 0 ? fb.kchunk : 8;
+  if (const char *ev = getenv("FST_KCHUNK")) kchunk = std::max(1, atoi(ev));  // tuning override
   double *Pa = fb.prim, *Pb = fb.prim + (size_t)g.N * np;
   Scalars *S0 = fb.sc, *S1 = fb.sc + 1;
   Prof *prof = b.prof && b.prof->on ? b.prof : nullptr;
-  if (prof) prof->name = "k_adv3d";
+  if (prof) prof->name = "stage: ghost fill + ghost primitives + k_adv3d + k_vcr3d";
   if (!fb.valid) {  // primitives and maxima of var, all cells
     k_clear<<<1, 64, 0, s>>>(S0);
     pf(q, k, b.var, Pa, S0, 0, nullptr, s);
@@ -687,13 +718,17 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
   const double h1 = 0.5f * cf.dt, h2 = cf.dt;
   for (int t = 0; t < nsteps; ++t) {
     // stage 1: U^n (b.var) -> U* (fb.stage); Pa/S0 hold the primitives/maxima of U^n
+    if (prof) prof->begin(s);
     launch_apply_bcs(g, cf, b.var, s);       // src/cart3d.cpp:221
     pf(q, k, b.var, Pa, nullptr, 1, S1, s);  // ghost primitives; clears S1
     sf(q, g, k, cf, b.var, nullptr, fb.stage, Pa, Pb, b.dvar, S0, S1, h1, 1, kchunk, prof, s);
+    if (prof) prof->end(s);
     // stage 2: U* -> U^{n+1}, written in place into b.var (read pointwise)
+    if (prof) prof->begin(s);
     launch_apply_bcs(g, cf, fb.stage, s);
     pf(q, k, fb.stage, Pb, nullptr, 1, S0, s);  // ghost primitives of U*; clears S0
     sf(q, g, k, cf, fb.stage, b.var, b.var, Pb, Pa, b.dvar, S1, S0, h2, 2, kchunk, prof, s);
+    if (prof) prof->end(s);
   }
   return FST_OK;
 }
