@@ -1,0 +1,80 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/fiesta_hip.h declares, and its host-only entry points (defaults,
+derived sizes, argument validation) behave as src/input.cpp:130-443 does."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from fiesta_amd import runtime
+from fiesta_amd.config import Dims, FstConfig, make_config
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = []
+    for f in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if f.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", f)).read()
+            names += re.findall(r"^\s*(?:int|void|const char \*)\s+(fst_\w+)\s*\(", src, re.M)
+    return sorted(set(names))
+
+
+class FstDims(C.Structure):
+    _fields_ = [("ngi", C.c_int32), ("ngj", C.c_int32), ("ngk", C.c_int32), ("nv", C.c_int32),
+                ("nvt", C.c_int32), ("nvx", C.c_int32), ("ncells_ghost", C.c_int64), ("ncells", C.c_int64)]
+
+
+def test_library_exports_every_declared_symbol():
+    lib = runtime.lib()
+    declared = _declared()
+    assert len(declared) >= 20
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(declared) <= set(runtime.exported_symbols()) | {"fst_config_init"}
+
+
+def test_abi_version_and_struct_size():
+    lib = runtime.lib()
+    assert lib.fst_abi_version() == 1
+    # fst_config layout: the ctypes mirror and the C struct must agree in size
+    c = FstConfig()
+    lib.fst_config_init(C.byref(c))
+    assert (c.ndim, c.ng, c.ns, c.scheme) == (2, 3, 1, 1)
+    assert c.R == 8.314462 and c.kap == 1.23456 and c.gAccel == 9.81
+
+
+@pytest.mark.parametrize("ndim,ns,ceq", [(2, 1, 0), (2, 3, 1), (3, 2, 1), (3, 1, 0)])
+def test_derived_sizes_match_input_cpp(ndim, ns, ceq):
+    cfg = make_config(ndim=ndim, ns=ns, nci=17, ncj=9, nck=5, gamma=(1.4,) * ns, M=(0.029,) * ns,
+                      mu=(1e-5,) * ns, ceq=bool(ceq))
+    d = FstDims()
+    assert runtime.lib().fst_config_dims(C.byref(cfg), C.byref(d)) == 0
+    py = Dims(cfg)
+    assert (d.ngi, d.ngj, d.ngk) == (py.ngi, py.ngj, py.ngk) == (23, 15, 11 if ndim == 3 else 1)
+    assert d.nv == ndim + 1 + ns and d.nvt == d.nv + 5 * ceq and d.nvx == ndim + 3
+    assert d.ncells == 17 * 9 * (5 if ndim == 3 else 1)
+
+
+@pytest.mark.parametrize("field,value", [("ndim", 4), ("ns", 0), ("ns", 9), ("ng", 2), ("scheme", 7),
+                                         ("bcL", 5), ("nci", 0)])
+def test_invalid_configurations_are_rejected(field, value):
+    cfg = make_config(ndim=3, ns=1, nci=8, ncj=8, nck=8)
+    setattr(cfg, field, value)
+    d = FstDims()
+    assert runtime.lib().fst_config_dims(C.byref(cfg), C.byref(d)) == 1  # FST_ERR_ARG
+
+
+def test_periodic_direction_needs_ng_cells():
+    cfg = make_config(ndim=3, ns=1, nci=2, ncj=8, nck=8, xper=True)
+    assert runtime.lib().fst_config_dims(C.byref(cfg), C.byref(FstDims())) == 1
+
+
+def test_null_arguments_are_errors_not_crashes():
+    lib = runtime.lib()
+    assert lib.fst_config_dims(None, None) == 1
+    assert lib.fst_create(None, None) == 1
+    assert lib.fst_destroy(None) == 1
+    assert lib.fst_step(None, 1) == 1

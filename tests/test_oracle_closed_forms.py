@@ -1,0 +1,382 @@
+"""Pins of the oracle's arithmetic against closed forms and invariants.
+
+The reference ships no numerical tests for its RHS kernels (its tests cover
+ghost fills only), so each oracle function is checked here against what the
+mathematics fixes, chosen so that a plausible slip (a dropped term, a wrong
+sign, index or species stride, a swapped stage coefficient) fails:
+
+  EOS (src/secondary.hpp:109-151)      ideal-gas round trip, mixture gamma
+  WENO5 (src/flux.hpp:38-62, 240-264)  constant data (exact binary32 sums),
+                                       linear data, exact-rational evaluation
+  advection + pressure gradient        uniform translation gives dvar == 0;
+                                       periodic telescoping conserves sums
+  viscous 3D (src/viscosity.hpp:416-663) linear shear: zero momentum
+                                       divergence, energy term mu*a^2
+  buoyancy + RK2 (src/buoyancy.hpp:22-47, src/rk.cpp:42-69)
+                                       uniform fluid at rest under gravity:
+                                       m_y = -g rho' dt, E += (g rho')^2 dt^2/(2 rho)
+  C-equation relaxation (src/ceq3d.hpp:189-224) uniform C field decays by
+                                       1 - z + z^2/2, z = maxS dt/(eps |dx|)
+  shock/contact switch (src/ceq3d.hpp:143-186) linear ramps
+  full 2D step (src/cart2d.cpp)        Sod shock tube vs the exact Riemann
+                                       solution at t = 0.2
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from fiesta_amd import workloads as W
+from fiesta_amd.config import FST_DVAR, FST_GRADRHO, FST_VAR, FST_VARX, Dims, make_config
+
+RU = 8.314462
+
+
+def _interior(d):
+    return d.interior()
+
+
+# ------------------------------------------------------------------ EOS
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_eos_ideal_gas_round_trip(ndim):
+    cfg = make_config(ndim=ndim, ns=1, nci=6, ncj=5, nck=4, gamma=(1.4,), M=(0.02896,), mu=(0.0,),
+                      xper=True, yper=True, zper=True)
+    d = Dims(cfg)
+    rng = np.random.default_rng(1)
+    sh = (d.nck, d.ncj, d.nci)
+    rho = rng.uniform(0.5, 2.0, sh)
+    p = rng.uniform(5e4, 2e5, sh)
+    u = [rng.normal(0, 50, sh) for _ in range(ndim)]
+    E = p / 0.4 + 0.5 * rho * sum(x * x for x in u)
+    var = np.zeros(d.shape(d.nvt))
+    sl = d.interior()[1:]
+    for a in range(ndim):
+        var[(a,) + sl] = rho * u[a]
+    var[(ndim,) + sl] = E
+    var[(ndim + 1,) + sl] = rho
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    x = o.get(FST_VARX)
+    np.testing.assert_allclose(x[(ndim,) + sl], p, rtol=1e-12)                       # pressure
+    np.testing.assert_allclose(x[(ndim + 1,) + sl], p / (rho * RU / 0.02896), rtol=1e-12)  # temperature
+    np.testing.assert_allclose(x[(ndim + 2,) + sl], rho, rtol=0)                      # density
+    for a in range(ndim):
+        np.testing.assert_allclose(x[(a,) + sl], u[a], rtol=1e-13, atol=1e-12)
+
+
+def test_eos_two_species_mixture():
+    """Mass-fraction weighted Cp, Cv (doc/equations.tex mixture rule)."""
+    g = (1.4, 1.67)
+    M = (0.02896, 0.004)
+    cfg = make_config(ndim=3, ns=2, nci=4, ncj=4, nck=4, gamma=g, M=M, mu=(0.0, 0.0), xper=True, yper=True, zper=True)
+    d = Dims(cfg)
+    sh = (d.nck, d.ncj, d.nci)
+    Y = np.linspace(0.05, 0.95, d.nci)[None, None, :] * np.ones(sh)
+    rho = 1.3 * np.ones(sh)
+    p = 1.0e5 * np.ones(sh)
+    R = [RU / m for m in M]
+    cp = Y * g[0] * R[0] / (g[0] - 1) + (1 - Y) * g[1] * R[1] / (g[1] - 1)
+    cv = Y * R[0] / (g[0] - 1) + (1 - Y) * R[1] / (g[1] - 1)
+    gam = cp / cv
+    var = np.zeros(d.shape(d.nvt))
+    sl = d.interior()[1:]
+    var[(3,) + sl] = p / (gam - 1)
+    var[(4,) + sl] = rho * Y
+    var[(5,) + sl] = rho * (1 - Y)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    x = o.get(FST_VARX)
+    np.testing.assert_allclose(x[(3,) + sl], p, rtol=1e-12)
+    np.testing.assert_allclose(x[(4,) + sl], p / (rho * (cp - cv)), rtol=1e-12)
+
+
+# ------------------------------------------------------------------ WENO5
+
+F32 = lambda x: Fraction(float(np.float32(x)))  # noqa: E731  (binary32-rounded constant)
+
+
+def _weno_exact(f, c):
+    """WENO5 evaluated in exact rational arithmetic with coefficient set c."""
+    f1, f2, f3, f4, f5 = [Fraction(x) for x in f]
+    eps = Fraction(1e-6)
+    b1 = c["c13"] * (f1 - 2 * f2 + f3) ** 2 + c["q"] * (f1 - 4 * f2 + 3 * f3) ** 2
+    b2 = c["c13"] * (f2 - 2 * f3 + f4) ** 2 + c["q"] * (f2 - f4) ** 2
+    b3 = c["c13"] * (f3 - 2 * f4 + f5) ** 2 + c["q"] * (3 * f3 - 4 * f4 + f5) ** 2
+    w1 = c["g1"] / (eps + b1) ** 2
+    w2 = c["g2"] / (eps + b2) ** 2
+    w3 = c["g3"] / (eps + b3) ** 2
+    p1 = c["a"] * f1 + c["b"] * f2 + c["cc"] * f3
+    p2 = c["d"] * f2 + c["e"] * f3 + c["a"] * f4
+    p3 = c["a"] * f3 + c["e"] * f4 + c["d"] * f5
+    return float((w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3))
+
+
+C64 = dict(c13=Fraction(13.0 / 12.0), q=Fraction(0.25), g1=Fraction(0.1), g2=Fraction(0.6), g3=Fraction(0.3),
+           a=Fraction(1.0 / 3.0), b=Fraction(-7.0 / 6.0), cc=Fraction(11.0 / 6.0), d=Fraction(-1.0 / 6.0),
+           e=Fraction(5.0 / 6.0))
+C32 = dict(c13=F32(13.0 / 12.0), q=F32(0.25), g1=F32(0.1), g2=F32(0.6), g3=F32(0.3), a=F32(1.0 / 3.0),
+           b=F32(-7.0 / 6.0), cc=F32(11.0 / 6.0), d=F32(-1.0 / 6.0), e=F32(5.0 / 6.0))
+
+
+def test_weno3d_constant_data_binary32_sums():
+    """Constant data: the 3D WENO5 returns c * sum of binary32-rounded weights
+    and coefficients (1.0000000894069672 for p1, 0.9999999850988388 for p2, p3)."""
+    assert float(F32(1 / 3) + F32(-7 / 6) + F32(11 / 6)) == 1.0000000894069672
+    assert float(F32(-1 / 6) + F32(5 / 6) + F32(1 / 3)) == 0.9999999850988388
+    v = oracle.weno3d(1, 1, 1, 1, 1)
+    assert v != 1.0
+    assert abs(v - _weno_exact([1] * 5, C32)) <= 2e-16
+    assert oracle.weno2d(1, 1, 1, 1, 1) == pytest.approx(1.0, abs=4e-16)
+
+
+def test_weno_linear_data_exact_reconstruction():
+    """WENO5 reproduces linear data at the face: f = i -> 2.5 (x_{i+1/2})."""
+    assert oracle.weno2d(0, 1, 2, 3, 4) == pytest.approx(2.5, abs=1e-14)
+    assert oracle.weno3d(0, 1, 2, 3, 4) == pytest.approx(_weno_exact([0, 1, 2, 3, 4], C32), rel=1e-15)
+    assert abs(oracle.weno3d(0, 1, 2, 3, 4) - 2.5) > 1e-9  # binary32 constants are visible
+
+
+def test_weno_random_data_against_exact_rationals():
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        f = rng.normal(0, 1, 5) * 10 ** rng.uniform(-3, 6)
+        assert oracle.weno2d(*f) == pytest.approx(_weno_exact(f, C64), rel=1e-12, abs=1e-300)
+        assert oracle.weno3d(*f) == pytest.approx(_weno_exact(f, C32), rel=1e-12, abs=1e-300)
+
+
+def test_weno_upwind_stencil_is_nonoscillatory_at_a_jump():
+    """A jump inside the stencil: the result stays between the data bounds."""
+    for f in ([0, 0, 0, 1, 1], [1, 1, 0, 0, 0], [0, 0, 1, 1, 1]):
+        for w in (oracle.weno2d, oracle.weno3d):
+            v = w(*f)
+            assert -1e-3 <= v <= 1 + 1e-3
+
+
+# ------------------------------------------------------------------ advection + pressure
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_uniform_translation_has_zero_rhs(ndim):
+    cfg = make_config(ndim=ndim, ns=2, nci=8, ncj=7, nck=6, gamma=(1.4, 1.67), M=(0.02896, 0.004),
+                      mu=(1.8e-5, 1.8e-5), visc=True, xper=True, yper=True, zper=True)
+    d = Dims(cfg)
+    sh = (d.nck, d.ncj, d.nci)
+    one = np.ones(sh)
+    var = W.fill_conserved(cfg, [0.8 * one, 0.1 * one], [30 * one, -20 * one, 10 * one][:ndim], 1e5 * one)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.compute() if ndim == 3 else (o.pre_step(), o.compute())
+    dv = o.get(FST_DVAR)[(slice(0, d.nv),) + d.interior()[1:]]
+    assert np.abs(dv).max() == 0.0
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_periodic_rhs_telescopes(ndim):
+    """On a periodic domain every flux-form term telescopes: the interior sum
+    of dvar over mass, momentum and energy vanishes to round-off."""
+    cfg = make_config(ndim=ndim, ns=2, nci=10, ncj=9, nck=8, dx=1e-2, dy=1e-2, dz=1e-2, gamma=(1.4, 1.67),
+                      M=(0.02896, 0.004), mu=(1.8e-5, 1.8e-5), visc=True, ceq=True, kap=10.0, eps=5.0,
+                      alpha=20.0, xper=True, yper=True, zper=True, dt=2e-7)
+    d = Dims(cfg)
+    var = W.random_state(cfg, 3)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.pre_step()
+    o.compute()
+    dv = o.get(FST_DVAR)[(slice(0, d.nv),) + d.interior()[1:]].reshape(d.nv, -1)
+    assert np.all(np.abs(dv.sum(axis=1)) <= 1e-12 * np.abs(dv).sum(axis=1)), dv.sum(axis=1)
+
+
+def test_periodic_3d_steps_conserve_mass_momentum_energy():
+    """3D refreshes ghosts every stage, so whole RK2 steps conserve too."""
+    cfg = make_config(ndim=3, ns=2, nci=10, ncj=9, nck=8, dx=1e-2, dy=1e-2, dz=1e-2, gamma=(1.4, 1.67),
+                      M=(0.02896, 0.004), mu=(1.8e-5, 1.8e-5), visc=True, ceq=True, kap=10.0, eps=5.0,
+                      alpha=20.0, xper=True, yper=True, zper=True, dt=2e-7)
+    d = Dims(cfg)
+    var = W.random_state(cfg, 3)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    sel = (slice(0, d.nv),) + d.interior()[1:]
+    before = var[sel].reshape(d.nv, -1).sum(axis=1)
+    o.step(3)
+    after = o.get(FST_VAR)[sel].reshape(d.nv, -1).sum(axis=1)
+    scale = np.abs(var[sel]).reshape(d.nv, -1).sum(axis=1)
+    assert np.all(np.abs(after - before) <= 1e-12 * scale), (after - before) / scale
+
+
+# ------------------------------------------------------------------ viscous (3D)
+
+def test_linear_shear_viscous_terms():
+    """u = a*y: uniform stress tau_xy = mu a -> zero momentum divergence,
+    energy source d/dy(u tau_xy) = mu a^2 (mu = mass-fraction mixture)."""
+    a = 50.0
+    mus = (1.8e-5, 3.0e-5)
+    cfg = make_config(ndim=3, ns=2, nci=6, ncj=12, nck=6, dx=1e-2, dy=1e-2, dz=1e-2, gamma=(1.4, 1.67),
+                      M=(0.02896, 0.004), mu=mus, visc=True, xper=True, zper=True,
+                      bc=("outflow",) * 6)
+    d = Dims(cfg)
+    sh = (d.nck, d.ncj, d.nci)
+    y = ((np.arange(d.ncj) + 0.5) * cfg.dy)[None, :, None] * np.ones(sh)
+    Y0 = 0.3
+    one = np.ones(sh)
+    var = W.fill_conserved(cfg, [Y0 * one, (1 - Y0) * one], [a * y, 0 * one, 0 * one], 1e5 * one)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.compute()
+    dv = o.get(FST_DVAR)
+    g = d.ng
+    core = (slice(None), slice(g, g + d.nck), slice(g + 3, g + d.ncj - 3), slice(g, g + d.nci))
+    mu = Y0 * mus[0] + (1 - Y0) * mus[1]
+    for v in (0, 1, 2):
+        assert np.abs(dv[v][core[1:]]).max() < 1e-9 * mu * a / cfg.dy
+    np.testing.assert_allclose(dv[3][core[1:]], mu * a * a, rtol=1e-8)
+
+
+# ------------------------------------------------------------------ buoyancy + RK2
+
+def test_fluid_at_rest_under_gravity_rk2_closed_form():
+    g0, rref, dt = 9.81, 0.4, 1e-3
+    cfg = make_config(ndim=3, ns=1, nci=5, ncj=5, nck=5, gamma=(1.4,), M=(0.02896,), mu=(0.0,), buoyancy=True,
+                      g=g0, rhoRef=rref, xper=True, yper=True, zper=True, dt=dt)
+    d = Dims(cfg)
+    one = np.ones((d.nck, d.ncj, d.nci))
+    rho = 1.2
+    var = W.fill_conserved(cfg, [rho * one], [0 * one] * 3, 1e5 * one)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.step(1)
+    u = o.get(FST_VAR)[(slice(None),) + d.interior()[1:]]
+    f = -g0 * (rho - rref)
+    np.testing.assert_allclose(u[1], f * dt, rtol=1e-10)          # stage 2 from U^n with the full dt
+    np.testing.assert_allclose(u[0], 0.0, atol=1e-12)  # (p - 8p) + 8p - p rounds
+    E0 = 1e5 / 0.4
+    np.testing.assert_allclose(u[3] - E0, 0.5 * f * f * dt * dt / rho, atol=4 * np.spacing(E0))
+
+
+# ------------------------------------------------------------------ C-equation
+
+def test_uniform_c_field_relaxes_with_rk2_factor():
+    eps, kap = 5.0, 10.0
+    dt = 2e-6
+    cfg = make_config(ndim=3, ns=1, nci=6, ncj=6, nck=6, dx=1e-2, dy=1e-2, dz=1e-2, gamma=(1.4,), M=(0.02896,),
+                      mu=(0.0,), ceq=True, kap=kap, eps=eps, alpha=20.0, xper=True, yper=True, zper=True, dt=dt)
+    d = Dims(cfg)
+    one = np.ones((d.nck, d.ncj, d.nci))
+    rho, u0 = 1.2, 30.0
+    C0 = [0.7 * one, 0.3 * one, 0.2 * one, -0.1 * one, 0.5 * one]
+    var = W.fill_conserved(cfg, [rho * one], [u0 * one, 0 * one, 0 * one], 1e5 * one, C0)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.step(1)
+    out = o.get(FST_VAR)[(slice(None),) + d.interior()[1:]]
+    c = np.sqrt(1.4 * 1e5 / rho)
+    maxS = c + u0
+    z = maxS / (eps * np.sqrt(3) * 1e-2) * dt
+    for n in range(5):
+        np.testing.assert_allclose(out[d.nv + n], C0[n] * (1 - z + z * z / 2), rtol=1e-12)
+    # uniform flow: no artificial viscosity, momentum unchanged
+    np.testing.assert_allclose(out[0], rho * u0, rtol=1e-14)
+
+
+@pytest.mark.parametrize("kind", ["shock", "contact"])
+def test_shock_and_contact_switch_on_linear_ramps(kind):
+    """rho = rho0 + a x.  Shock-like (e grows with rho, compression) feeds
+    gradRho[0]; contact-like (p uniform, e falls as rho grows) feeds gradRho[1]."""
+    a = 20.0
+    cfg = make_config(ndim=3, ns=1, nci=12, ncj=4, nck=4, dx=1e-2, dy=1e-2, dz=1e-2, gamma=(1.4,), M=(0.02896,),
+                      mu=(0.0,), ceq=True, kap=1.0, eps=1.0, alpha=1.0, yper=True, zper=True)
+    d = Dims(cfg)
+    sh = (d.nck, d.ncj, d.nci)
+    x = ((np.arange(d.nci) + 0.5) * cfg.dx)[None, None, :] * np.ones(sh)
+    rho = 1.0 + a * x
+    if kind == "shock":
+        p = 1e5 * rho ** 1.4                      # e = p/((g-1) rho) increases with rho
+        u = -100.0 * x                            # div u < 0
+    else:
+        p = 1e5 * np.ones(sh)                      # e decreases as rho increases
+        u = 0 * x
+    var = W.fill_conserved(cfg, [rho], [u, 0 * x, 0 * x], p, [0 * x] * 5)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.compute()
+    G = o.get(FST_GRADRHO)
+    g = d.ng
+    core = (slice(g, g + d.nck), slice(g, g + d.ncj), slice(g + 3, g + d.nci - 3))
+    np.testing.assert_allclose(G[2][core], a, rtol=1e-9)
+    assert np.abs(G[3][core]).max() < 1e-9 and np.abs(G[4][core]).max() < 1e-9
+    if kind == "shock":
+        np.testing.assert_allclose(G[0][core], a, rtol=1e-9)
+        assert np.all(G[1][core] == 0)
+    else:
+        np.testing.assert_allclose(G[1][core], a, rtol=1e-9)
+        assert np.all(G[0][core] == 0)
+
+
+# ------------------------------------------------------------------ Sod
+
+def _sod_exact(x, t, gam=1.4):
+    """Exact Riemann solution of Sod's problem (left 1,1,0 / right 0.125,0.1,0)."""
+    rl, pl, rr, pr = 1.0, 1.0, 0.125, 0.1
+    cl, cr = np.sqrt(gam * pl / rl), np.sqrt(gam * pr / rr)
+
+    def f(p, rk, pk, ck):
+        if p > pk:
+            A, B = 2 / ((gam + 1) * rk), (gam - 1) / (gam + 1) * pk
+            return (p - pk) * np.sqrt(A / (p + B))
+        return 2 * ck / (gam - 1) * ((p / pk) ** ((gam - 1) / (2 * gam)) - 1)
+
+    lo, hi = 1e-6, 5.0
+    for _ in range(200):
+        pm = 0.5 * (lo + hi)
+        if f(pm, rl, pl, cl) + f(pm, rr, pr, cr) > 0:
+            hi = pm
+        else:
+            lo = pm
+    ps = 0.5 * (lo + hi)
+    us = 0.5 * (f(ps, rr, pr, cr) - f(ps, rl, pl, cl))
+    rsl = rl * (ps / pl) ** (1 / gam)
+    rsr = rr * ((ps / pr + (gam - 1) / (gam + 1)) / ((gam - 1) / (gam + 1) * ps / pr + 1))
+    sh = cr * np.sqrt((gam + 1) / (2 * gam) * ps / pr + (gam - 1) / (2 * gam))  # right state at rest
+    csl = cl * (ps / pl) ** ((gam - 1) / (2 * gam))
+    xi = (x - 0.5) / t
+    rho = np.where(xi < -cl, rl, 0.0)
+    fan = (xi >= -cl) & (xi < us - csl)
+    rho = np.where(fan, rl * (2 / (gam + 1) + (gam - 1) / ((gam + 1) * cl) * (-xi)) ** (2 / (gam - 1)), rho)
+    rho = np.where((xi >= us - csl) & (xi < us), rsl, rho)
+    rho = np.where((xi >= us) & (xi < sh), rsr, rho)
+    rho = np.where(xi >= sh, rr, rho)
+    return rho, ps, us, sh
+
+
+def test_sod_shock_tube_against_exact_riemann_solution():
+    cfg = W.sod_config()
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, W.sod_state(cfg))
+    o.pre_sim()
+    o.step(1000)  # t = 0.2
+    rho = o.get(FST_VAR)[3, 0, 3, 3:-3]
+    x = (np.arange(1000) + 0.5) / 1000
+    ex, ps, us, sh = _sod_exact(x, 0.2)
+    assert ps == pytest.approx(0.30313, abs=1e-4) and us == pytest.approx(0.92745, abs=1e-4)
+    assert 0.5 + 0.2 * sh == pytest.approx(0.8504, abs=1e-3)
+    l1 = np.abs(rho - ex).mean()
+    assert l1 < 0.01, l1
+    xs = x[np.where((x > 0.7) & (rho < 0.5 * (0.26557 + 0.125)))[0][0]]
+    assert abs(xs - (0.5 + 0.2 * sh)) < 0.015, xs
+    # Mass is not asserted: the 2D module refreshes ghosts once per step
+    # (src/cart2d.cpp:147-149), so stage 2 reads step-start periodic ghosts in
+    # y and the reference itself drifts by ~0.5% over this run.
+    # rarefaction head, plateau and far field
+    assert np.all(np.abs(rho[x < 0.2] - 1.0) < 1e-3)
+    assert np.all(np.abs(rho[x > 0.95] - 0.125) < 1e-3)
+    assert np.abs(rho[(x > 0.5) & (x < 0.56)] - 0.42632).max() < 0.005  # left star state
