@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--path", choices=["fused", "reference"], default="fused")
     ap.add_argument("--copy-probe", action="store_true", help="also time a plain copy kernel")
+    ap.add_argument("--forms", default="fp64,fp32",
+                    help="comma list of arithmetic forms to time: fp64 (reported as value), fp32, fp32_learned")
     return ap.parse_args()
 
 
@@ -42,19 +44,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def gpu_leg(args):
+def gpu_leg(args, form="fp64", var=None, keep_state=False):
     from fiesta_amd import workloads as W
-    from fiesta_amd.config import FST_VAR, Dims
+    from fiesta_amd.config import FST_FP32_BULK, FST_VAR, Dims
     from fiesta_amd.runtime import Module
 
     cfg = W.rt3d_config(args.size)
+    if form.startswith("fp32"):
+        cfg.precision = FST_FP32_BULK
+    if form.endswith("learned"):
+        cfg.learned_c = 1
     d = Dims(cfg)
-    t0 = time.time()
-    var = W.rt3d_state(cfg)
-    log("state generated in %.1fs (%d^3, nvt=%d)" % (time.time() - t0, args.size, d.nvt))
     m = Module(cfg)
     m.upload(FST_VAR, var)
-    del var
     m.pre_sim()
     fused_ok = True
     if args.path == "reference":
@@ -80,6 +82,8 @@ def gpu_leg(args):
     value = cell_stages / wall / 1e9
     res = dict(cfg=cfg, dims=d, wall=wall, ev_ms=ev_ms, value=value, launches=launches, kernel_ms=kms,
                kernel=kname, fused=fused_ok and args.path == "fused")
+    if keep_state:
+        res["state"] = m.download(FST_VAR)
     if args.copy_probe:
         nbytes = 2 << 30
         ms = m.copy_probe(nbytes, 10)
@@ -109,13 +113,53 @@ def cpu_leg(args, cfg_size):
                        "(C++/OpenMP restatement of the reference kernels), %d threads" % (args.cpu_steps, cfg_size, threads))
 
 
+def form_errors(cfg, got, want, cfg64):
+    """Relative L2 / Linf of density, pressure and mass fractions (interior)
+    of a form's final state against the FP64 path's final state (the FP64
+    path equals the oracle bit for bit)."""
+    from fiesta_amd.config import Dims
+
+    RU = cfg.R
+
+    def fields(c, var):
+        d = Dims(c)
+        v = var[d.interior()]
+        rs = v[4:4 + c.ns]
+        rho = rs.sum(axis=0)
+        Y = rs / rho
+        cp = sum(Y[s] * c.gamma[s] * (RU / c.M[s]) / (c.gamma[s] - 1) for s in range(c.ns))
+        cv = sum(Y[s] * (RU / c.M[s]) / (c.gamma[s] - 1) for s in range(c.ns))
+        p = (cp / cv - 1) * (v[3] - 0.5 / rho * (v[0] ** 2 + v[1] ** 2 + v[2] ** 2))
+        return {"rho": rho, "p": p, "Y": Y}
+
+    a, b = fields(cfg, got), fields(cfg64, want)
+    out = {}
+    for k in ("rho", "p", "Y"):
+        dd = a[k] - b[k]
+        out[k] = {"rel_l2": float("%.3e" % np.sqrt((dd ** 2).sum() / (b[k] ** 2).sum())),
+                  "rel_linf": float("%.3e" % (np.abs(dd).max() / np.abs(b[k]).max()))}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (multi-GPU sharding is "
               "a design note, not a measured path)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    r = gpu_leg(args)
+    from fiesta_amd import workloads as W
+    from fiesta_amd.config import Dims
+
+    cfg0 = W.rt3d_config(args.size)
+    t0 = time.time()
+    var = W.rt3d_state(cfg0)
+    log("state generated in %.1fs (%d^3, nvt=%d)" % (time.time() - t0, args.size, Dims(cfg0).nvt))
+    forms = [f for f in args.forms.split(",") if f]
+    results = {}
+    for f in forms:
+        log("form %s" % f)
+        results[f] = gpu_leg(args, f, var, keep_state=len(forms) > 1)
+    r = results.get("fp64") or results[forms[0]]
     d = r["dims"]
     nvt = d.nvt
     balg = 20 * nvt  # bytes per cell-stage (FP64 RK2 with tmp as a pointer swap)
@@ -157,6 +201,16 @@ def main():
     }
     if "copy_gbs" in r:
         out["copy_kernel_gbs"] = round(r["copy_gbs"], 1)
+    ref = r.get("state")
+    fm = {}
+    for f, q in results.items():
+        e = {"value": round(q["value"], 4), "ms_per_step": round(q["wall"] * 1e3 / args.steps, 4),
+             "speedup_vs_fp64": round(q["value"] / r["value"], 3),
+             "stage_ms": round(q["kernel_ms"] / max(q["launches"], 1), 4)}
+        if f != "fp64" and ref is not None and "state" in q:
+            e["error_vs_fp64"] = form_errors(q["cfg"], q["state"], ref, r["cfg"])
+        fm[f] = e
+    out["forms"] = fm
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(args, args.size)
         out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 5)

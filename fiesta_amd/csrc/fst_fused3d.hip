@@ -114,11 +114,58 @@ __device__ __forceinline__ Tile place(const Grid32 &g, int kchunk, int nbx, int 
   return t;
 }
 
+// ---------------------------------------------------------------- arithmetic forms
+
+// FP64 form: the reference arithmetic (divisions by run constants through
+// cdiv, all others IEEE).  FP32 form (precision form): stencil, viscous and
+// C-equation arithmetic in binary32 with hardware reciprocals; loads, flux
+// assembly, pressure gradient, RK accumulation and EOS stay FP64.
+template <bool F32>
+struct Ar;
+template <>
+struct Ar<false> {
+  using R = double;
+  static __device__ __forceinline__ R cdv(R a, double b, double rb) { return cdiv(a, b, rb); }
+  static __device__ __forceinline__ R div(R a, R b) { return a / b; }
+  static __device__ __forceinline__ R sq(R a) { return sqrt(a); }
+};
+template <>
+struct Ar<true> {
+  using R = float;
+  static __device__ __forceinline__ R cdv(R a, double, double rb) { return a * (float)rb; }
+  static __device__ __forceinline__ R div(R a, R b) { return a * __builtin_amdgcn_rcpf(b); }
+  static __device__ __forceinline__ R sq(R a) { return __builtin_sqrtf(a); }
+};
+
+// WENO5 of the precision form: same stencils, weights and smoothness
+// indicators as calculateFluxesG (src/flux.hpp:240-264) in binary32, with
+// hardware reciprocals.  Callers pass deviations from the face-centre value
+// (WENO5 is translation covariant), which keeps FP32 cancellation small.
+__device__ __forceinline__ float weno5_fast(float f1, float f2, float f3, float f4, float f5) {
+#pragma clang fp contract(fast)
+  const float eps = 1.0e-6f;
+  float a1 = f1 - 2.0f * f2 + f3, a2 = f1 - 4.0f * f2 + 3.0f * f3;
+  const float b1 = (13.0f / 12.0f) * a1 * a1 + 0.25f * a2 * a2;
+  a1 = f2 - 2.0f * f3 + f4;
+  a2 = f2 - f4;
+  const float b2 = (13.0f / 12.0f) * a1 * a1 + 0.25f * a2 * a2;
+  a1 = f3 - 2.0f * f4 + f5;
+  a2 = 3.0f * f3 - 4.0f * f4 + f5;
+  const float b3 = (13.0f / 12.0f) * a1 * a1 + 0.25f * a2 * a2;
+  const float w1 = 0.1f * __builtin_amdgcn_rcpf((eps + b1) * (eps + b1));
+  const float w2 = 0.6f * __builtin_amdgcn_rcpf((eps + b2) * (eps + b2));
+  const float w3 = 0.3f * __builtin_amdgcn_rcpf((eps + b3) * (eps + b3));
+  const float p1 = (1.0f / 3.0f) * f1 + (-7.0f / 6.0f) * f2 + (11.0f / 6.0f) * f3;
+  const float p2 = (-1.0f / 6.0f) * f2 + (5.0f / 6.0f) * f3 + (1.0f / 3.0f) * f4;
+  const float p3 = (1.0f / 3.0f) * f3 + (5.0f / 6.0f) * f4 + (-1.0f / 6.0f) * f5;
+  return (w1 * p1 + w2 * p2 + w3 * p3) * __builtin_amdgcn_rcpf(w1 + w2 + w3);
+}
+
 // ---------------------------------------------------------------- faces
 
 // Advective fluxes of the nv flow variables on the face between c and c+e_D
-// (calculateFluxesG, src/flux.hpp:266-309).
-template <int D, int NS>
+// (calculateFluxesG, src/flux.hpp:266-309).  Fluxes are FP64 in both forms.
+template <int D, int NS, bool F32>
 __device__ __forceinline__ void face_adv(const FC &k, const Grid32 &g, const double *__restrict__ U,
                                          const double *__restrict__ P, uint32_t c, double *out) {
   const uint32_t N = g.N, e = stride<D>(g);
@@ -140,7 +187,14 @@ __device__ __forceinline__ void face_adv(const FC &k, const Grid32 &g, const dou
     const double f3 = neg ? q[3] : q[2];
     const double f4 = neg ? q[2] : q[3];
     const double f5 = neg ? q[1] : q[4];
-    out[v] = cdiv(ud * weno5_f32c(f1, f2, f3, f4, f5), k.d[D], k.rd[D]);
+    if (!F32) {
+      out[v] = cdiv(ud * weno5_f32c(f1, f2, f3, f4, f5), k.d[D], k.rd[D]);
+    } else {
+      const double c0 = q[2];
+      const float w = weno5_fast((float)(f1 - c0), (float)(f2 - c0), (float)(f3 - c0), (float)(f4 - c0),
+                                 (float)(f5 - c0));
+      out[v] = cdiv(ud * (c0 + (double)w), k.d[D], k.rd[D]);
+    }
   }
 }
 
@@ -150,64 +204,69 @@ __device__ __forceinline__ void face_adv(const FC &k, const Grid32 &g, const dou
 // out[3+3*dir+w]: mFlux(D, dir, w) (calculateCeqFaces src/ceq3d.hpp:246-295
 // times calculateCeqGrads src/ceq3d.hpp:306-331).  Both use the same nine
 // face derivatives of the velocity.
-template <int D, int NS>
+template <int D, int NS, bool F32>
 __device__ __forceinline__ void face_vc(const FC &k, const Grid32 &g, const double *__restrict__ U,
                                         const double *__restrict__ P, double alpha, int visc, int ceq,
-                                        uint32_t c, double *out) {
+                                        uint32_t c, typename Ar<F32>::R *out) {
+  using A = Ar<F32>;
+  using R = typename A::R;
   const uint32_t N = g.N, e = stride<D>(g);
   constexpr int T1 = D == 0 ? 1 : 0;
   constexpr int T2 = D == 2 ? 1 : 2;
   const uint32_t t1 = T1 == 1 ? g.s1 : 1u;
   const uint32_t t2 = T2 == 1 ? g.s1 : g.s2;
-  double der[3][3];
+  auto V = [&](uint32_t off) { return (R)ld(P, off); };
+  R der[3][3];
 #pragma unroll
   for (int w = 0; w < 3; ++w) {
     const uint32_t vb = N * (2 + w);
-    der[w][D] = cdiv(ld(P, c + e + vb) - ld(P, c + vb), k.d[D], k.rd[D]);
-    der[w][T1] = cdiv((ld(P, c + t1 + vb) + ld(P, c + e + t1 + vb)) - (ld(P, c - t1 + vb) + ld(P, c + e - t1 + vb)),
-                      k.d4[T1], k.rd4[T1]);
-    der[w][T2] = cdiv((ld(P, c + t2 + vb) + ld(P, c + e + t2 + vb)) - (ld(P, c - t2 + vb) + ld(P, c + e - t2 + vb)),
-                      k.d4[T2], k.rd4[T2]);
+    der[w][D] = A::cdv(V(c + e + vb) - V(c + vb), k.d[D], k.rd[D]);
+    der[w][T1] = A::cdv((V(c + t1 + vb) + V(c + e + t1 + vb)) - (V(c - t1 + vb) + V(c + e - t1 + vb)), k.d4[T1],
+                        k.rd4[T1]);
+    der[w][T2] = A::cdv((V(c + t2 + vb) + V(c + e + t2 + vb)) - (V(c - t2 + vb) + V(c + e - t2 + vb)), k.d4[T2],
+                        k.rd4[T2]);
   }
   if (visc) {
-    double mu = 0.0;
+    R mu = 0.0;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) mu += ((ld(P, c + e + N * (6 + s)) + ld(P, c + N * (6 + s))) / 2.0) * k.mus[s];
-    const double dudx = der[0][0], dudy = der[0][1], dudz = der[0][2];
-    const double dvdx = der[1][0], dvdy = der[1][1], dvdz = der[1][2];
-    const double dwdx = der[2][0], dwdy = der[2][1], dwdz = der[2][2];
+    for (int s = 0; s < NS; ++s) mu += ((V(c + e + N * (6 + s)) + V(c + N * (6 + s))) / (R)2.0) * (R)k.mus[s];
+    const R dudx = der[0][0], dudy = der[0][1], dudz = der[0][2];
+    const R dvdx = der[1][0], dvdy = der[1][1], dvdz = der[1][2];
+    const R dwdx = der[2][0], dwdy = der[2][1], dwdz = der[2][2];
+    const R c23 = (R)(2.0 / 3.0), two = (R)2.0;
     if (D == 0) {
-      out[0] = (2.0 / 3.0) * mu * (2.0 * dudx - dvdy - dwdz);
+      out[0] = c23 * mu * (two * dudx - dvdy - dwdz);
       out[1] = mu * (dudy + dvdx);
       out[2] = mu * (dudz + dwdx);
     } else if (D == 1) {
       out[0] = mu * (dudy + dvdx);
-      out[1] = (2.0 / 3.0) * mu * (2.0 * dvdy - dudx - dwdz);
+      out[1] = c23 * mu * (two * dvdy - dudx - dwdz);
       out[2] = mu * (dvdz + dwdy);
     } else {
       out[0] = mu * (dudz + dwdx);
       out[1] = mu * (dvdz + dwdy);
-      out[2] = (2.0 / 3.0) * mu * (2.0 * dwdz - dudx - dvdy);
+      out[2] = c23 * mu * (two * dwdz - dudx - dvdy);
     }
   }
   if (ceq) {
     const int nv = g.nv;
-    const double r = (ld(P, c) + ld(P, c + e)) / 2.0;
-    const double chat = (ld(U, c + N * (nv + 1)) + ld(U, c + e + N * (nv + 1))) / 2.0;
-    double cn[3];
-    cn[0] = (ld(U, c + N * (nv + 2)) + ld(U, c + e + N * (nv + 2))) / 2.0;
-    cn[1] = (ld(U, c + N * (nv + 3)) + ld(U, c + e + N * (nv + 3))) / 2.0;
-    cn[2] = (ld(U, c + N * (nv + 4)) + ld(U, c + e + N * (nv + 4))) / 2.0;
-    double cmag = 1.0e-6;
+    auto Uv = [&](uint32_t off) { return (R)ld(U, off); };
+    const R r = (V(c) + V(c + e)) / (R)2.0;
+    const R chat = (Uv(c + N * (nv + 1)) + Uv(c + e + N * (nv + 1))) / (R)2.0;
+    R cn[3];
+    cn[0] = (Uv(c + N * (nv + 2)) + Uv(c + e + N * (nv + 2))) / (R)2.0;
+    cn[1] = (Uv(c + N * (nv + 3)) + Uv(c + e + N * (nv + 3))) / (R)2.0;
+    cn[2] = (Uv(c + N * (nv + 4)) + Uv(c + e + N * (nv + 4))) / (R)2.0;
+    R cmag = (R)1.0e-6;
     cmag += cn[0] * cn[0];
     cmag += cn[1] * cn[1];
     cmag += cn[2] * cn[2];
-    const double arc = alpha * r * chat;
+    const R arc = (R)alpha * r * chat;
 #pragma unroll
     for (int dir = 0; dir < 3; ++dir) {
-      const double dirac = (dir == D) ? 1.0 : 0.0;
-      const double M = dirac - (cn[D] * cn[dir]) / cmag;
-      const double X = arc * M;
+      const R dirac = (dir == D) ? (R)1.0 : (R)0.0;
+      const R M = dirac - A::div(cn[D] * cn[dir], cmag);
+      const R X = arc * M;
 #pragma unroll
       for (int w = 0; w < 3; ++w) out[3 + 3 * dir + w] = X * der[w][dir];
     }
@@ -249,7 +308,7 @@ __device__ __forceinline__ void prims_of(const FC &k, const double *u, double *p
 // advect3D (src/advect.hpp:52-55), applyPressureGradient3D (dz := cd(2),
 // src/presgrad.hpp:53-67), computeBuoyancy3D (src/buoyancy.hpp:33-46):
 // dvar[v] for v < nv.
-template <int NS>
+template <int NS, bool F32>
 __global__ void __launch_bounds__(NTHREADS, 2)
     k_adv3d(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
             double *__restrict__ dvar, int buoy, int kchunk, int nbx, int nby, int nbz) {
@@ -262,30 +321,30 @@ __global__ void __launch_bounds__(NTHREADS, 2)
   double Z[NV], Zp[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) Z[q] = Zp[q] = 0.0;
-  if (t.cellok) face_adv<2, NS>(k, g, U, P, g.id(t.i, t.jrow, t.kb - 1), Zp);
+  if (t.cellok) face_adv<2, NS, F32>(k, g, U, P, g.id(t.i, t.jrow, t.kb - 1), Zp);
   for (int kk = t.kb; kk < t.ke; ++kk) {
     uint32_t c = g.id(t.i, t.jrow, kk);
     uint32_t cxb = g.id(t.i0 - 1, t.j0 + l, kk);
     asm volatile("" : "+v"(c), "+v"(cxb));  // keep address arithmetic per plane
     if (t.cellok) {
       double X[NV];
-      face_adv<0, NS>(k, g, U, P, c, X);
+      face_adv<0, NS, F32>(k, g, U, P, c, X);
 #pragma unroll
       for (int q = 0; q < NV; ++q) xsh[w][q][l + 1] = X[q];
     }
     if (t.xbok) {
       double X[NV];
-      face_adv<0, NS>(k, g, U, P, cxb, X);
+      face_adv<0, NS, F32>(k, g, U, P, cxb, X);
 #pragma unroll
       for (int q = 0; q < NV; ++q) xsh[l][q][0] = X[q];
     }
     if (t.yfok) {
       double Y[NV];
-      face_adv<1, NS>(k, g, U, P, c - g.s1, Y);
+      face_adv<1, NS, F32>(k, g, U, P, c - g.s1, Y);
 #pragma unroll
       for (int q = 0; q < NV; ++q) ysh[w][q][l] = Y[q];
     }
-    if (t.cellok) face_adv<2, NS>(k, g, U, P, c, Z);
+    if (t.cellok) face_adv<2, NS, F32>(k, g, U, P, c, Z);
     __syncthreads();
     if (t.cellok) {
       double dv[NV];
@@ -325,17 +384,20 @@ __global__ void __launch_bounds__(NTHREADS, 2)
 // applyViscousTerm{x,y,z}3dv (src/viscosity.hpp:472-497, 554-579, 637-662),
 // calculateRhoGrad + updateCeq + applyCeq (src/ceq3d.hpp:143-186, 202-223,
 // 343-365), the Runge-Kutta stage update (src/rk.cpp:116-119 / :137-139),
-// then the primitives and the maxima of the new state.
-template <int NS>
+// then the primitives and the maxima of the new state.  dvar accumulates in
+// FP64 in both forms.
+template <int NS, bool F32>
 __global__ void __launch_bounds__(NTHREADS, 1)
     k_vcr3d(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ Un,
             double *__restrict__ Uo, const double *__restrict__ P, double *__restrict__ Po,
             const double *__restrict__ dvar, const Scalars *scin, Scalars *scout, double h, int stage, int visc,
             int ceq, int kchunk, int nbx, int nby, int nbz) {
+  using A = Ar<F32>;
+  using R = typename A::R;
   constexpr int NV = 4 + NS;
   constexpr int NVT = NV + 5;
-  __shared__ double xsh[TY][NFV][TX + 1];
-  __shared__ double ysh[NWAVE][NFV][TX];
+  __shared__ R xsh[TY][NFV][TX + 1];
+  __shared__ R ysh[NWAVE][NFV][TX];
   __shared__ double red[2][NWAVE];
   const Tile t = place(g, kchunk, nbx, nby, nbz);
   const int w = t.w, l = t.l;
@@ -343,10 +405,10 @@ __global__ void __launch_bounds__(NTHREADS, 1)
   const double maxS = as_double(scin->maxS);
   const double alpha = (k.dxmag2 / (as_double(scin->maxCh) + 1.0e-6)) * k.alpha_cf;  // src/cart3d.cpp:285
   const bool faces = visc || ceq;
-  double Z[NFV], Zp[NFV];
+  R Z[NFV], Zp[NFV];
 #pragma unroll
-  for (int q = 0; q < NFV; ++q) Z[q] = Zp[q] = 0.0;
-  if (faces && t.cellok) face_vc<2, NS>(k, g, U, P, alpha, visc, ceq, g.id(t.i, t.jrow, t.kb - 1), Zp);
+  for (int q = 0; q < NFV; ++q) Z[q] = Zp[q] = (R)0.0;
+  if (faces && t.cellok) face_vc<2, NS, F32>(k, g, U, P, alpha, visc, ceq, g.id(t.i, t.jrow, t.kb - 1), Zp);
   double mS = 0.0, mC = 0.0;
   for (int kk = t.kb; kk < t.ke; ++kk) {
     uint32_t c = g.id(t.i, t.jrow, kk);
@@ -354,35 +416,36 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     asm volatile("" : "+v"(c), "+v"(cxb));
     if (faces) {
       if (t.cellok) {
-        double X[NFV];
-        face_vc<0, NS>(k, g, U, P, alpha, visc, ceq, c, X);
+        R X[NFV];
+        face_vc<0, NS, F32>(k, g, U, P, alpha, visc, ceq, c, X);
 #pragma unroll
         for (int q = 0; q < NFV; ++q) xsh[w][q][l + 1] = X[q];
       }
       if (t.xbok) {
-        double X[NFV];
-        face_vc<0, NS>(k, g, U, P, alpha, visc, ceq, cxb, X);
+        R X[NFV];
+        face_vc<0, NS, F32>(k, g, U, P, alpha, visc, ceq, cxb, X);
 #pragma unroll
         for (int q = 0; q < NFV; ++q) xsh[l][q][0] = X[q];
       }
       if (t.yfok) {
-        double Y[NFV];
-        face_vc<1, NS>(k, g, U, P, alpha, visc, ceq, c - g.s1, Y);
+        R Y[NFV];
+        face_vc<1, NS, F32>(k, g, U, P, alpha, visc, ceq, c - g.s1, Y);
 #pragma unroll
         for (int q = 0; q < NFV; ++q) ysh[w][q][l] = Y[q];
       }
-      if (t.cellok) face_vc<2, NS>(k, g, U, P, alpha, visc, ceq, c, Z);
+      if (t.cellok) face_vc<2, NS, F32>(k, g, U, P, alpha, visc, ceq, c, Z);
       __syncthreads();
     }
     if (t.cellok) {
       double dv[NVT];
 #pragma unroll
       for (int v = 0; v < NV; ++v) dv[v] = ld(dvar, c + N * v);
+      auto PV = [&](uint32_t off) { return (R)ld(P, off); };
       if (visc) {
 #pragma unroll
         for (int D = 0; D < 3; ++D) {
           const uint32_t e = D == 0 ? 1u : (D == 1 ? g.s1 : g.s2);
-          double T0c, T0m, T1c, T1m, T2c, T2m;
+          R T0c, T0m, T1c, T1m, T2c, T2m;
           if (D == 0) {
             T0c = xsh[w][0][l + 1]; T0m = xsh[w][0][l];
             T1c = xsh[w][1][l + 1]; T1m = xsh[w][1][l];
@@ -397,16 +460,16 @@ __global__ void __launch_bounds__(NTHREADS, 1)
             T2c = Z[2]; T2m = Zp[2];
           }
           const uint32_t b0 = 2 * N, b1 = 3 * N, b2 = 4 * N;
-          const double V0 = ld(P, c + b0), V1 = ld(P, c + b1), V2 = ld(P, c + b2);
-          const double ur = (ld(P, c + e + b0) + V0) / 2.0, ul = (ld(P, c - e + b0) + V0) / 2.0;
-          const double vr = (ld(P, c + e + b1) + V1) / 2.0, vl = (ld(P, c - e + b1) + V1) / 2.0;
-          const double wr = (ld(P, c + e + b2) + V2) / 2.0, wl = (ld(P, c - e + b2) + V2) / 2.0;
+          const R V0 = PV(c + b0), V1 = PV(c + b1), V2 = PV(c + b2);
+          const R ur = (PV(c + e + b0) + V0) / (R)2.0, ul = (PV(c - e + b0) + V0) / (R)2.0;
+          const R vr = (PV(c + e + b1) + V1) / (R)2.0, vl = (PV(c - e + b1) + V1) / (R)2.0;
+          const R wr = (PV(c + e + b2) + V2) / (R)2.0, wl = (PV(c - e + b2) + V2) / (R)2.0;
           const double dd = k.d[D], rdd = k.rd[D];
-          const double ta = cdiv(T0c - T0m, dd, rdd);
-          const double tb = cdiv(T1c - T1m, dd, rdd);
-          const double tc = cdiv(T2c - T2m, dd, rdd);
-          const double td = cdiv(ur * T0c - ul * T0m, dd, rdd) + cdiv(vr * T1c - vl * T1m, dd, rdd) +
-                            cdiv(wr * T2c - wl * T2m, dd, rdd);
+          const R ta = A::cdv(T0c - T0m, dd, rdd);
+          const R tb = A::cdv(T1c - T1m, dd, rdd);
+          const R tc = A::cdv(T2c - T2m, dd, rdd);
+          const R td = A::cdv(ur * T0c - ul * T0m, dd, rdd) + A::cdv(vr * T1c - vl * T1m, dd, rdd) +
+                       A::cdv(wr * T2c - wl * T2m, dd, rdd);
           dv[0] += ta;
           dv[1] += tb;
           dv[2] += tc;
@@ -415,60 +478,61 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       }
       if (ceq) {
         // calculateRhoGrad, src/ceq3d.hpp:143-186 (rho, nrg, vel planes of P)
-        double G[5];
+        R G[5];
         {
-          double dr[3], de[3], du[3];
+          R dr[3], de[3], du[3];
 #pragma unroll
           for (int D = 0; D < 3; ++D) {
             const uint32_t e = D == 0 ? 1u : (D == 1 ? g.s1 : g.s2);
             const uint32_t rb = 0, eb = 5 * N, vb = (2 + D) * N;
-            dr[D] = cdiv(ld(P, c - 2 * e + rb) - 8.0 * ld(P, c - e + rb) + 8.0 * ld(P, c + e + rb) -
-                             ld(P, c + 2 * e + rb), k.d12[D], k.rd12[D]);
-            de[D] = cdiv(ld(P, c - 2 * e + eb) - 8.0 * ld(P, c - e + eb) + 8.0 * ld(P, c + e + eb) -
-                             ld(P, c + 2 * e + eb), k.d12[D], k.rd12[D]);
-            du[D] = cdiv(ld(P, c - 2 * e + vb) - 8.0 * ld(P, c - e + vb) + 8.0 * ld(P, c + e + vb) -
-                             ld(P, c + 2 * e + vb), k.d12[D], k.rd12[D]);
+            dr[D] = A::cdv(PV(c - 2 * e + rb) - (R)8.0 * PV(c - e + rb) + (R)8.0 * PV(c + e + rb) -
+                               PV(c + 2 * e + rb), k.d12[D], k.rd12[D]);
+            de[D] = A::cdv(PV(c - 2 * e + eb) - (R)8.0 * PV(c - e + eb) + (R)8.0 * PV(c + e + eb) -
+                               PV(c + 2 * e + eb), k.d12[D], k.rd12[D]);
+            du[D] = A::cdv(PV(c - 2 * e + vb) - (R)8.0 * PV(c - e + vb) + (R)8.0 * PV(c + e + vb) -
+                               PV(c + 2 * e + vb), k.d12[D], k.rd12[D]);
           }
-          const double rgrad = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-          const double divu = du[0] + du[1] + du[2];
-          const double dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) *
-                                (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-          const int ind = dnednr < 0.0 ? 1 : 0;
-          G[0] = (divu < 0) ? (1 - ind) * rgrad : 0;
-          G[1] = ind * rgrad;
+          const R rgrad = A::sq(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+          const R divu = du[0] + du[1] + du[2];
+          const R dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) *
+                           (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+          const int ind = dnednr < (R)0.0 ? 1 : 0;
+          G[0] = (divu < 0) ? (R)(1 - ind) * rgrad : (R)0;
+          G[1] = (R)ind * rgrad;
           G[2] = dr[0];
           G[3] = dr[1];
           G[4] = dr[2];
         }
         // updateCeq, src/ceq3d.hpp:202-223
-        const double c1 = maxS / (k.eps * k.dxmag);
-        const double c2 = k.kap * maxS * k.dxmag;
+        const R c1 = (R)(maxS / (k.eps * k.dxmag));
+        const R c2 = (R)(k.kap * maxS * k.dxmag);
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
           const uint32_t cb = N * (NV + n);
-          const double cc = ld(U, c + cb);
-          const double lap = cdiv(ld(U, c - 1 + cb) - 2 * cc + ld(U, c + 1 + cb), k.d[0], k.rd[0]) +
-                             cdiv(ld(U, c - g.s1 + cb) - 2 * cc + ld(U, c + g.s1 + cb), k.d[1], k.rd[1]) +
-                             cdiv(ld(U, c - g.s2 + cb) - 2 * cc + ld(U, c + g.s2 + cb), k.d[2], k.rd[2]);
+          auto CU = [&](uint32_t off) { return (R)ld(U, off); };
+          const R cc = CU(c + cb);
+          const R lap = A::cdv(CU(c - 1 + cb) - (R)2 * cc + CU(c + 1 + cb), k.d[0], k.rd[0]) +
+                        A::cdv(CU(c - g.s1 + cb) - (R)2 * cc + CU(c + g.s1 + cb), k.d[1], k.rd[1]) +
+                        A::cdv(CU(c - g.s2 + cb) - (R)2 * cc + CU(c + g.s2 + cb), k.d[2], k.rd[2]);
           dv[NV + n] = c1 * (G[n] - cc) + c2 * lap;
         }
         // applyCeq, src/ceq3d.hpp:343-365
 #pragma unroll
         for (int ww = 0; ww < 3; ++ww) {
-          double diffu = 0.0;
+          R diffu = (R)0.0;
 #pragma unroll
           for (int dir = 0; dir < 3; ++dir)
-            diffu += cdiv(xsh[w][3 + 3 * dir + ww][l + 1] - xsh[w][3 + 3 * dir + ww][l], k.d[0], k.rd[0]);
+            diffu += A::cdv(xsh[w][3 + 3 * dir + ww][l + 1] - xsh[w][3 + 3 * dir + ww][l], k.d[0], k.rd[0]);
 #pragma unroll
           for (int dir = 0; dir < 3; ++dir)
-            diffu += cdiv(ysh[w + 1][3 + 3 * dir + ww][l] - ysh[w][3 + 3 * dir + ww][l], k.d[1], k.rd[1]);
+            diffu += A::cdv(ysh[w + 1][3 + 3 * dir + ww][l] - ysh[w][3 + 3 * dir + ww][l], k.d[1], k.rd[1]);
 #pragma unroll
           for (int dir = 0; dir < 3; ++dir)
-            diffu += cdiv(Z[3 + 3 * dir + ww] - Zp[3 + 3 * dir + ww], k.d[2], k.rd[2]);
+            diffu += A::cdv(Z[3 + 3 * dir + ww] - Zp[3 + 3 * dir + ww], k.d[2], k.rd[2]);
           dv[ww] += diffu;
         }
       }
-      // Runge-Kutta stage update and primitives of the new state
+      // Runge-Kutta stage update and primitives of the new state (FP64)
       const int nvt = g.nvt;
       double un[NVT];
 #pragma unroll
@@ -636,13 +700,13 @@ Launch shape(const Geo &g, int kchunk) {
   return L;
 }
 
-template <int NS>
+template <int NS, bool F32>
 void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf, const double *U,
                const double *Un, double *Uo, const double *P, double *Po, double *dvar, const Scalars *scin,
                Scalars *scout, double h, int stage, int kchunk, Prof *prof, hipStream_t s) {
   const Launch L = shape(g, kchunk);
-  k_adv3d<NS><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
-  k_vcr3d<NS><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, scin, scout, h, stage, cf.visc, cf.ceq,
+  k_adv3d<NS, F32><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
+  k_vcr3d<NS, F32><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, scin, scout, h, stage, cf.visc, cf.ceq,
                                         kchunk, L.nbx, L.nby, L.nbz);
 }
 
@@ -661,11 +725,11 @@ using StageFn = void (*)(const Grid32 &, const Geo &, const FC &, const fst_conf
 using PrimFn = void (*)(const Grid32 &, const FC &, const double *, double *, Scalars *, int, Scalars *,
                         hipStream_t);
 
-bool pick(int ns, StageFn *sf, PrimFn *pf) {
+bool pick(int ns, bool f32, StageFn *sf, PrimFn *pf) {
   switch (ns) {
-    case 1: *sf = run_stage<1>; *pf = run_prims<1>; return true;
-    case 2: *sf = run_stage<2>; *pf = run_prims<2>; return true;
-    case 3: *sf = run_stage<3>; *pf = run_prims<3>; return true;
+    case 1: *sf = f32 ? run_stage<1, true> : run_stage<1, false>; *pf = run_prims<1>; return true;
+    case 2: *sf = f32 ? run_stage<2, true> : run_stage<2, false>; *pf = run_prims<2>; return true;
+    case 3: *sf = f32 ? run_stage<3, true> : run_stage<3, false>; *pf = run_prims<3>; return true;
     default: return false;
   }
 }
@@ -673,8 +737,8 @@ bool pick(int ns, StageFn *sf, PrimFn *pf) {
 }  // namespace
 
 bool fused_supported(const fst_config &cf) {
-  if (!(cf.ndim == 3 && cf.grid == 0 && cf.precision == FST_FP64 && cf.scheme == FST_WENO5 && cf.ns >= 1 &&
-        cf.ns <= 3 && !cf.learned_c))
+  if (!(cf.ndim == 3 && cf.grid == 0 && (cf.precision == FST_FP64 || cf.precision == FST_FP32_BULK) &&
+        cf.scheme == FST_WENO5 && cf.ns >= 1 && cf.ns <= 3 && !cf.learned_c))
     return false;
   // 32-bit element offsets: the largest array (nvt planes) below 2^29 elements
   const double N = (double)(cf.nci + 2 * cf.ng) * (cf.ncj + 2 * cf.ng) * (cf.nck + 2 * cf.ng);
@@ -686,7 +750,7 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
                 hipStream_t s, std::string *err) {
   StageFn sf;
   PrimFn pf;
-  if (!pick(cf.ns, &sf, &pf) || !fused_supported(cf)) {
+  if (!pick(cf.ns, cf.precision == FST_FP32_BULK, &sf, &pf) || !fused_supported(cf)) {
     *err = "fused path: configuration not supported";
     return FST_ERR_ARG;
   }

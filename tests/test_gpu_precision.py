@@ -1,0 +1,62 @@
+"""Precision form (FP32 stencil/flux/viscous/C-equation arithmetic, FP64 EOS
+and RK accumulation) against the FP64 oracle: relative L2 and Linf errors of
+density, pressure and mass fractions after several steps.  Tolerance: 1e-4
+relative L2 on density (the requirement), and the same bar for pressure and
+mass fractions on these rough random states."""
+import numpy as np
+import pytest
+
+import oracle
+from fiesta_amd import workloads as W
+from fiesta_amd.config import FST_FP32_BULK, FST_VAR, Dims
+from fiesta_amd.runtime import Module
+from helpers import cfg3d
+
+pytestmark = pytest.mark.gpu
+
+RU = 8.314462
+
+
+def fields(cfg, var):
+    """Density, pressure and mass fractions of the interior (FP64, numpy)."""
+    d = Dims(cfg)
+    v = var[d.interior()]
+    rs = v[4:4 + cfg.ns]
+    rho = rs.sum(axis=0)
+    Y = rs / rho
+    cp = sum(Y[s] * cfg.gamma[s] * (RU / cfg.M[s]) / (cfg.gamma[s] - 1) for s in range(cfg.ns))
+    cv = sum(Y[s] * (RU / cfg.M[s]) / (cfg.gamma[s] - 1) for s in range(cfg.ns))
+    p = (cp / cv - 1) * (v[3] - 0.5 / rho * (v[0] ** 2 + v[1] ** 2 + v[2] ** 2))
+    return rho, p, Y
+
+
+def rel_errors(a, b):
+    l2 = np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())
+    linf = np.abs(a - b).max() / np.abs(b).max()
+    return l2, linf
+
+
+@pytest.mark.parametrize("name,kw", [("rt_like", dict()), ("walls_ns3", dict(ns=3, per=(False, False, False),
+                                     bc=("outflow", "reflective", "hydrostatic", "noslip", "reflective", "outflow")))])
+def test_fp32_form_error_against_fp64_oracle(name, kw):
+    cfg = cfg3d(nci=40, ncj=21, nck=18, **kw)
+    print("case fp32 form", name, flush=True)
+    var = W.random_state(cfg, 51)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.step(4)
+    want = o.get(FST_VAR)
+    cfg.precision = FST_FP32_BULK
+    m = Module(cfg)
+    m.upload(FST_VAR, var)
+    m.pre_sim()
+    m.set_fused(True)
+    m.step(4)
+    got = m.download(FST_VAR)
+    assert np.isfinite(got).all()
+    assert not np.array_equal(got, want)  # it really is a different arithmetic form
+    for (a, b), label in zip(zip(fields(cfg, got), fields(cfg, want)), ("rho", "p", "Y")):
+        l2, linf = rel_errors(a, b)
+        print("  %s: rel L2 %.3e  rel Linf %.3e" % (label, l2, linf), flush=True)
+        assert l2 < 1e-4, (label, l2)
