@@ -163,39 +163,52 @@ __device__ __forceinline__ float weno5_fast(float f1, float f2, float f3, float 
 
 // ---------------------------------------------------------------- faces
 
-// Advective fluxes of the nv flow variables on the face between c and c+e_D
-// (calculateFluxesG, src/flux.hpp:266-309).  Fluxes are FP64 in both forms.
-template <int D, int NS, bool F32>
-__device__ __forceinline__ void face_adv(const FC &k, const Grid32 &g, const double *__restrict__ U,
-                                         const double *__restrict__ P, uint32_t c, double *out) {
-  const uint32_t N = g.N, e = stride<D>(g);
-  const uint32_t vb = N * (2 + D);
-  const double v_m1 = ld(P, c - e + vb), v_0 = ld(P, c + vb);
-  const double v_p1 = ld(P, c + e + vb), v_p2 = ld(P, c + 2 * e + vb);
-  const double ud = cdiv(-v_p2 + 7.0f * v_p1 + 7.0f * v_0 - v_m1, 12.0, k.r12);  // :278
+// Advective fluxes of the nv flow variables on one face (calculateFluxesG,
+// src/flux.hpp:266-309) from its stencil: q[t][v] = var(v) + (v==3)*p at the
+// six cells c-2e .. c+3e, vel_D at c-e .. c+2e.  Fluxes are FP64 in both forms.
+template <int NS, bool F32>
+__device__ __forceinline__ void flux_from_stencil(const FC &k, int D, const double *vd, const double (*q)[4 + NS],
+                                                  double *out) {
+  const double ud = cdiv(-vd[3] + 7.0f * vd[2] + 7.0f * vd[1] - vd[0], 12.0, k.r12);  // :278
   const bool neg = ud < 0.0;
 #pragma unroll
   for (int v = 0; v < 4 + NS; ++v) {
-    double q[6];  // q[t] at c + (t-2)*e: var(v) + (v==3)*p
-#pragma unroll
-    for (int t = 0; t < 6; ++t) {
-      const uint32_t cc = c + (uint32_t)(t - 2) * e;
-      q[t] = (v == 3) ? ld(U, cc + N * v) + ld(P, cc + N) : ld(U, cc + N * v);
-    }
-    const double f1 = neg ? q[5] : q[0];
-    const double f2 = neg ? q[4] : q[1];
-    const double f3 = neg ? q[3] : q[2];
-    const double f4 = neg ? q[2] : q[3];
-    const double f5 = neg ? q[1] : q[4];
+    const double f1 = neg ? q[5][v] : q[0][v];
+    const double f2 = neg ? q[4][v] : q[1][v];
+    const double f3 = neg ? q[3][v] : q[2][v];
+    const double f4 = neg ? q[2][v] : q[3][v];
+    const double f5 = neg ? q[1][v] : q[4][v];
     if (!F32) {
       out[v] = cdiv(ud * weno5_f32c(f1, f2, f3, f4, f5), k.d[D], k.rd[D]);
     } else {
-      const double c0 = q[2];
+      const double c0 = q[2][v];
       const float w = weno5_fast((float)(f1 - c0), (float)(f2 - c0), (float)(f3 - c0), (float)(f4 - c0),
                                  (float)(f5 - c0));
       out[v] = cdiv(ud * (c0 + (double)w), k.d[D], k.rd[D]);
     }
   }
+}
+
+// q of one cell: var(v) + (v==3)*p for v < nv
+template <int NS>
+__device__ __forceinline__ void load_q(const Grid32 &g, const double *__restrict__ U, const double *__restrict__ P,
+                                       uint32_t cc, double *q) {
+  const uint32_t N = g.N;
+#pragma unroll
+  for (int v = 0; v < 4 + NS; ++v) q[v] = (v == 3) ? ld(U, cc + N * v) + ld(P, cc + N) : ld(U, cc + N * v);
+}
+
+// Advective fluxes on the face between c and c+e_D, stencil read from memory.
+template <int D, int NS, bool F32>
+__device__ __forceinline__ void face_adv(const FC &k, const Grid32 &g, const double *__restrict__ U,
+                                         const double *__restrict__ P, uint32_t c, double *out) {
+  const uint32_t N = g.N, e = stride<D>(g);
+  const uint32_t vb = N * (2 + D);
+  const double vd[4] = {ld(P, c - e + vb), ld(P, c + vb), ld(P, c + e + vb), ld(P, c + 2 * e + vb)};
+  double q[6][4 + NS];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) load_q<NS>(g, U, P, c + (uint32_t)(t - 2) * e, q[t]);
+  flux_from_stencil<NS, F32>(k, D, vd, q, out);
 }
 
 // Viscous stresses and C-equation fluxes on the face between c and c+e_D.
@@ -309,7 +322,7 @@ __device__ __forceinline__ void prims_of(const FC &k, const double *u, double *p
 // src/presgrad.hpp:53-67), computeBuoyancy3D (src/buoyancy.hpp:33-46):
 // dvar[v] for v < nv.
 template <int NS, bool F32>
-__global__ void __launch_bounds__(NTHREADS, 2)
+__global__ void __launch_bounds__(NTHREADS, 1)
     k_adv3d(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
             double *__restrict__ dvar, int buoy, int kchunk, int nbx, int nby, int nbz) {
   constexpr int NV = 4 + NS;
@@ -318,10 +331,20 @@ __global__ void __launch_bounds__(NTHREADS, 2)
   const Tile t = place(g, kchunk, nbx, nby, nbz);
   const int w = t.w, l = t.l;
   const uint32_t N = g.N;
+  // z-stencil of this column carried in registers: qz[t] holds q at plane
+  // kk-2+t for the face between kk and kk+1, vz[t] the z-velocity at kk-1+t.
+  double qz[6][NV], vz[4];
   double Z[NV], Zp[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) Z[q] = Zp[q] = 0.0;
-  if (t.cellok) face_adv<2, NS, F32>(k, g, U, P, g.id(t.i, t.jrow, t.kb - 1), Zp);
+  const uint32_t c00 = g.id(t.i, t.jrow, t.kb - 1);
+  if (t.cellok) {
+#pragma unroll
+    for (int s = 0; s < 6; ++s) load_q<NS>(g, U, P, c00 + (uint32_t)(s - 2) * g.s2, qz[s]);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) vz[s] = ld(P, c00 + (uint32_t)(s - 1) * g.s2 + 4 * N);
+    flux_from_stencil<NS, F32>(k, 2, vz, qz, Zp);
+  }
   for (int kk = t.kb; kk < t.ke; ++kk) {
     uint32_t c = g.id(t.i, t.jrow, kk);
     uint32_t cxb = g.id(t.i0 - 1, t.j0 + l, kk);
@@ -344,7 +367,17 @@ __global__ void __launch_bounds__(NTHREADS, 2)
 #pragma unroll
       for (int q = 0; q < NV; ++q) ysh[w][q][l] = Y[q];
     }
-    if (t.cellok) face_adv<2, NS, F32>(k, g, U, P, c, Z);
+    if (t.cellok) {
+#pragma unroll
+      for (int s = 0; s < 5; ++s)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qz[s][v] = qz[s + 1][v];
+      load_q<NS>(g, U, P, c + 3 * g.s2, qz[5]);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) vz[s] = vz[s + 1];
+      vz[3] = ld(P, c + 2 * g.s2 + 4 * N);
+      flux_from_stencil<NS, F32>(k, 2, vz, qz, Z);
+    }
     __syncthreads();
     if (t.cellok) {
       double dv[NV];
