@@ -10,6 +10,13 @@ Prints ONE JSON line.  `value` is the FP64 plain path (bitwise parity with
 the oracle).  The roofline block covers the dominant kernel, timed live with
 HIP events on the stream it runs on; algorithmic bytes are 20*nvt per
 cell-stage (stage 1 reads U, writes U*; stage 2 reads U*, U^n, writes U^{n+1}).
+
+Forms: fp64 (reported as `value`), fp32 (FP32 stencils, FP64 EOS/RK) and
+fp32_learned (fp32 + the learned C-indicator in place of the C-equation's
+relaxation solve: nvt = nv).  The learned network is trained first, outside
+the timed region, on rows harvested on the GPU from plain runs (a 64^3 RT
+run and four 48x48x4 shear layers); each form's final state is compared with
+the FP64 plain final state (density, pressure, mass fractions).
 """
 import argparse
 import json
@@ -35,7 +42,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--path", choices=["fused", "reference"], default="fused")
     ap.add_argument("--copy-probe", action="store_true", help="also time a plain copy kernel")
-    ap.add_argument("--forms", default="fp64,fp32",
+    ap.add_argument("--forms", default="fp64,fp32,fp32_learned",
                     help="comma list of arithmetic forms to time: fp64 (reported as value), fp32, fp32_learned")
     return ap.parse_args()
 
@@ -44,7 +51,46 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def gpu_leg(args, form="fp64", var=None, keep_state=False):
+def train_learned():
+    """Harvest (stencil, C_hat) rows from plain FP64 runs on the GPU and train
+    the network with the library's own kernels.  Returns weights and a summary."""
+    from fiesta_amd import workloads as W
+    from fiesta_amd.config import FST_VAR
+    from fiesta_amd.runtime import Module, Trainer
+
+    t0 = time.perf_counter()
+    tr = Trainer(seed=1)
+    runs = []
+    cfg = W.rt3d_config(64)
+    runs.append((cfg, W.rt3d_state(cfg), 200))
+    for s in (1, 2, 5, 6):
+        cfg = W.shear3d_config(48, nz=4)
+        runs.append((cfg, W.shear3d_state(cfg, s), 300))
+    for cfg, var, steps in runs:
+        m = Module(cfg)
+        m.upload(FST_VAR, var)
+        m.pre_sim()
+        for done in range(20, steps + 1, 20):
+            m.step(20)
+            if done >= 40:
+                tr.harvest(m, stride=16)
+        m.close()
+    rows = tr.rows()
+    t1 = time.perf_counter()
+    l1 = tr.train(epochs=40, batch=512, lr=1e-3, seed=40)
+    l2 = tr.train(epochs=20, batch=512, lr=3e-4, seed=20)
+    t2 = time.perf_counter()
+    w = tr.weights()
+    tr.close()
+    info = {"rows": int(rows), "harvest_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
+            "epochs": 60, "batch": 512, "loss_first_epoch": float("%.4g" % l1[0]),
+            "loss_last_epoch": float("%.4g" % l2[-1]),
+            "data": "plain FP64 runs: RT 64^3 (200 steps) + 4 shear layers 48x48x4 (300 steps), "
+                    "harvested every 20 steps from step 40"}
+    return w, info
+
+
+def gpu_leg(args, form="fp64", var=None, keep_state=False, weights=None):
     from fiesta_amd import workloads as W
     from fiesta_amd.config import FST_FP32_BULK, FST_VAR, Dims
     from fiesta_amd.runtime import Module
@@ -52,11 +98,14 @@ def gpu_leg(args, form="fp64", var=None, keep_state=False):
     cfg = W.rt3d_config(args.size)
     if form.startswith("fp32"):
         cfg.precision = FST_FP32_BULK
-    if form.endswith("learned"):
+    learned = form.endswith("learned")
+    if learned:
         cfg.learned_c = 1
     d = Dims(cfg)
     m = Module(cfg)
-    m.upload(FST_VAR, var)
+    m.upload(FST_VAR, var[:d.nvt])
+    if learned:
+        m.set_mlp_weights(weights)
     m.pre_sim()
     fused_ok = True
     if args.path == "reference":
@@ -82,6 +131,9 @@ def gpu_leg(args, form="fp64", var=None, keep_state=False):
     value = cell_stages / wall / 1e9
     res = dict(cfg=cfg, dims=d, wall=wall, ev_ms=ev_ms, value=value, launches=launches, kernel_ms=kms,
                kernel=kname, fused=fused_ok and args.path == "fused")
+    if learned:
+        a, nfb, mch = m.learned_stats()
+        res["learned"] = {"active_cells": int(a), "fallback_cells": int(nfb), "max_chat": float("%.5g" % mch)}
     if keep_state:
         res["state"] = m.download(FST_VAR)
     if args.copy_probe:
@@ -155,10 +207,15 @@ def main():
     var = W.rt3d_state(cfg0)
     log("state generated in %.1fs (%d^3, nvt=%d)" % (time.time() - t0, args.size, Dims(cfg0).nvt))
     forms = [f for f in args.forms.split(",") if f]
+    weights, train_info = None, None
+    if any(f.endswith("learned") for f in forms):
+        log("training the learned C-indicator")
+        weights, train_info = train_learned()
+        log("trained: %s" % train_info)
     results = {}
     for f in forms:
         log("form %s" % f)
-        results[f] = gpu_leg(args, f, var, keep_state=len(forms) > 1)
+        results[f] = gpu_leg(args, f, var, keep_state=len(forms) > 1, weights=weights)
     r = results.get("fp64") or results[forms[0]]
     d = r["dims"]
     nvt = d.nvt
@@ -204,13 +261,20 @@ def main():
     ref = r.get("state")
     fm = {}
     for f, q in results.items():
+        stage_ms = q["kernel_ms"] / max(q["launches"], 1)
+        b_form = 20 * q["dims"].nvt
         e = {"value": round(q["value"], 4), "ms_per_step": round(q["wall"] * 1e3 / args.steps, 4),
-             "speedup_vs_fp64": round(q["value"] / r["value"], 3),
-             "stage_ms": round(q["kernel_ms"] / max(q["launches"], 1), 4)}
+             "speedup_vs_fp64": round(q["value"] / r["value"], 3), "stage_ms": round(stage_ms, 4),
+             "algorithmic_bytes_per_cell_stage": b_form,
+             "hbm_frac": round(d.ncells * b_form / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         if f != "fp64" and ref is not None and "state" in q:
             e["error_vs_fp64"] = form_errors(q["cfg"], q["state"], ref, r["cfg"])
+        if "learned" in q:
+            e["learned"] = q["learned"]
         fm[f] = e
     out["forms"] = fm
+    if train_info:
+        out["learned_training"] = train_info
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(args, args.size)
         out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 5)

@@ -73,7 +73,8 @@ class Dims:
         self.ngj = c.ncj + 2 * c.ng
         self.ngk = self.nck + 2 * c.ng if c.ndim == 3 else 1
         self.nv = (4 if c.ndim == 3 else 3) + c.ns
-        self.nvt = self.nv + (5 if c.ceq else 0)
+        # learned C-indicator: no C-equation variables are stored
+        self.nvt = self.nv + (5 if (c.ceq and not c.learned_c) else 0)
         self.nvx = c.ndim + 3
         self.N = self.ngi * self.ngj * self.ngk
         self.ncells = self.nci * self.ncj * self.nck

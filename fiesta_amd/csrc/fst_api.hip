@@ -7,6 +7,7 @@
 #include "fst_common.hpp"
 #include "fst_fused.hpp"
 #include "fst_kernels.hpp"
+#include "fst_mlp.hpp"
 
 using namespace fst;
 
@@ -16,6 +17,8 @@ struct fst_ctx {
   Phys ph;
   Bufs b;
   FusedBufs fb;
+  LearnedBufs lb;         // learned C-indicator fields (cf.learned_c)
+  double *lprim = nullptr;  // primitive planes for fst_learned_compute
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
@@ -55,6 +58,10 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
     return FST_ERR_ARG;
   }
   if (c.grid != 0) { *why = "only the cartesian grid is supported"; return FST_ERR_ARG; }
+  if (c.learned_c && (c.ndim != 3 || !c.ceq)) {
+    *why = "the learned C-indicator replaces the 3D C-equation: needs ndim = 3 and ceq = 1";
+    return FST_ERR_ARG;
+  }
   if (c.scheme < 1 || c.scheme > 3 || (c.ndim == 3 && c.scheme != FST_WENO5)) {
     *why = "advection scheme: 3D supports weno5; 2D weno5, centered4, quick";
     return FST_ERR_ARG;
@@ -66,7 +73,7 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
   d->ngj = c.ncj + 2 * c.ng;
   d->ngk = c.ndim == 3 ? c.nck + 2 * c.ng : 1;
   d->nv = c.ndim + 1 + c.ns;  // src/input.cpp:354-364
-  d->nvt = d->nv + (c.ceq ? 5 : 0);
+  d->nvt = d->nv + (c.ceq && !c.learned_c ? 5 : 0);  // learned form: no C-equation variables
   d->nvx = c.ndim + 3;
   d->ncells_ghost = (int64_t)d->ngi * d->ngj * d->ngk;
   d->ncells = (int64_t)c.nci * c.ncj * (c.ndim == 3 ? c.nck : 1);
@@ -86,7 +93,7 @@ double *array_ptr(fst_ctx *ctx, int a, size_t *count) {
     case FST_RHO: p = ctx->b.rho; n = N; break;
     case FST_T: p = ctx->b.T; n = N; break;
     case FST_VEL: p = ctx->b.vel; n = N * ctx->g.ndim; break;
-    case FST_GRADRHO: p = ctx->b.gradRho; n = ctx->cf.ceq ? N * (ctx->g.ndim == 3 ? 5 : 4) : 0; break;
+    case FST_GRADRHO: p = ctx->b.gradRho; n = ctx->b.gradRho ? N * (ctx->g.ndim == 3 ? 5 : 4) : 0; break;
     default: return nullptr;
   }
   *count = n;
@@ -101,6 +108,12 @@ void free_all(fst_ctx *ctx) {
     if (*pp) { (void)hipFree(*pp); *pp = nullptr; }
   if (ctx->b.sc) { (void)hipFree(ctx->b.sc); ctx->b.sc = nullptr; }
   fused_free(ctx->fb);
+  learned_free(ctx->lb);
+  if (ctx->lprim) { (void)hipFree(ctx->lprim); ctx->lprim = nullptr; }
+}
+
+int learned_only(fst_ctx *ctx, const char *what) {
+  return fail(ctx, FST_ERR_ARG, std::string(what) + ": the learned C-indicator runs through fst_step only");
 }
 
 }  // namespace
@@ -187,8 +200,20 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
   ok = ok && alloc(&ctx->b.vel, g.ndim) && alloc(&ctx->b.fx, 1) && alloc(&ctx->b.fy, 1);
   ok = ok && alloc(&ctx->b.fz, g.ndim == 3 ? 1 : 0);
   ok = ok && alloc(&ctx->b.stress, c.visc ? (g.ndim == 3 ? 6 : 8) : 0);
-  ok = ok && alloc(&ctx->b.gradRho, c.ceq ? (g.ndim == 3 ? 5 : 4) : 0);
-  ok = ok && alloc(&ctx->b.mflux, c.ceq ? (g.ndim == 3 ? 27 : 8) : 0);
+  const bool ceq_vars = c.ceq && !c.learned_c;
+  ok = ok && alloc(&ctx->b.gradRho, ceq_vars ? (g.ndim == 3 ? 5 : 4) : 0);
+  ok = ok && alloc(&ctx->b.mflux, ceq_vars ? (g.ndim == 3 ? 27 : 8) : 0);
+  if (ok && c.learned_c) {
+    if (!fused_supported(c)) {
+      fprintf(stderr, "fst_create: the learned C-indicator needs the fused 3D path (ns <= 3, FP64 or FP32_BULK)\n");
+      free_all(ctx);
+      if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+      delete ctx;
+      return FST_ERR_ARG;
+    }
+    ok = learned_alloc(g, ctx->lb) == FST_OK;
+  }
+  ok = ok && hipDeviceSynchronize() == hipSuccess;
   ok = ok && hipMalloc((void **)&ctx->b.sc, sizeof(Scalars)) == hipSuccess &&
        hipMemset(ctx->b.sc, 0, sizeof(Scalars)) == hipSuccess;
   if (!ok) {
@@ -270,6 +295,7 @@ int fst_pre_step(fst_ctx *ctx) {
 
 int fst_compute(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
+  if (ctx->cf.learned_c) return learned_only(ctx, "fst_compute");
   ctx->fb.valid = false;
   if (ctx->g.ndim == 3) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);  // src/cart3d.cpp:221
   launch_compute_ref(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->stream);
@@ -293,6 +319,7 @@ int fst_set_fused(fst_ctx *ctx, int fused) {
   if (!ctx) return FST_ERR_ARG;
   if (fused && !fused_supported(ctx->cf))
     return fail(ctx, FST_ERR_ARG, "fused path supports 3D cartesian FP64 with ceq, visc and buoyancy flags only");
+  if (!fused && ctx->cf.learned_c) return learned_only(ctx, "fst_set_fused(0)");
   ctx->fused = fused ? 1 : 0;
   return FST_OK;
 }
@@ -301,7 +328,7 @@ int fst_step(fst_ctx *ctx, int nsteps) {
   if (!ctx || nsteps < 0) return FST_ERR_ARG;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (ctx->fused) {
-    int rc = fused_steps(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->fb, nsteps, ctx->stream, &ctx->err);
+    int rc = fused_steps(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->fb, &ctx->lb, nsteps, ctx->stream, &ctx->err);
     if (rc) return rc;
   } else {
     ctx->fb.valid = false;
@@ -391,6 +418,171 @@ int fst_sync(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return check_launch(ctx, "fst_sync");
+}
+
+// ---------------------------------------------------------------- learned C-indicator
+
+int fst_mlp_nparams(void) { return MLP_NPARAM; }
+
+int fst_set_mlp_weights(fst_ctx *ctx, const float *w, size_t n) {
+  if (!ctx || !w) return FST_ERR_ARG;
+  if (!ctx->cf.learned_c) return fail(ctx, FST_ERR_ARG, "fst_set_mlp_weights: context has learned_c = 0");
+  if (n != (size_t)MLP_NPARAM) return fail(ctx, FST_ERR_ARG, "fst_set_mlp_weights: wrong parameter count");
+  const int rc = learned_set_weights(ctx->lb, w, ctx->stream);
+  return rc ? fail(ctx, rc, "fst_set_mlp_weights: copy failed") : FST_OK;
+}
+
+int fst_learned_params(fst_ctx *ctx, float tau, float tol) {
+  if (!ctx || !(tau >= 0.f) || !(tol >= 0.f)) return FST_ERR_ARG;
+  if (!ctx->cf.learned_c) return fail(ctx, FST_ERR_ARG, "fst_learned_params: context has learned_c = 0");
+  ctx->lb.tau = tau;
+  ctx->lb.tol = tol;
+  return FST_OK;
+}
+
+int fst_learned_compute(fst_ctx *ctx) {
+  if (!ctx) return FST_ERR_ARG;
+  if (!ctx->cf.learned_c) return fail(ctx, FST_ERR_ARG, "fst_learned_compute: context has learned_c = 0");
+  const size_t np = 6 + (size_t)ctx->cf.ns;
+  if (!ctx->lprim) HIPCHK(ctx, hipMalloc((void **)&ctx->lprim, np * (size_t)ctx->g.N * sizeof(double)));
+  launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
+  int rc = fused_primitives(ctx->g, ctx->ph, ctx->cf, ctx->b.var, ctx->lprim, ctx->stream);
+  if (!rc) rc = learned_fields(ctx->g, ctx->cf, ctx->ph, ctx->lprim, ctx->lb, ctx->stream);
+  if (rc) return fail(ctx, rc, rc == FST_ERR_STATE ? "fst_learned_compute: no network weights set"
+                                                   : "fst_learned_compute: launch failed");
+  ctx->fb.valid = false;
+  return check_launch(ctx, "fst_learned_compute");
+}
+
+int fst_learned_stats(fst_ctx *ctx, int64_t *active, int64_t *fallback, double *maxCh) {
+  if (!ctx) return FST_ERR_ARG;
+  if (!ctx->cf.learned_c) return fail(ctx, FST_ERR_ARG, "fst_learned_stats: context has learned_c = 0");
+  unsigned c[4];
+  Scalars sc;
+  HIPCHK(ctx, hipMemcpyAsync(c, ctx->lb.cnt, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(&sc, ctx->lb.sc, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (active) *active = c[0];
+  if (fallback) *fallback = c[1];
+  if (maxCh) *maxCh = as_double(sc.maxCh);
+  return FST_OK;
+}
+
+int fst_learned_fields(fst_ctx *ctx, float *chat, float *dir, float *feat, float *g1, size_t plane) {
+  if (!ctx) return FST_ERR_ARG;
+  if (!ctx->cf.learned_c) return fail(ctx, FST_ERR_ARG, "fst_learned_fields: context has learned_c = 0");
+  const size_t N = (size_t)ctx->g.N;
+  if (plane != N) return fail(ctx, FST_ERR_ARG, "fst_learned_fields: plane size does not match");
+  if (chat) HIPCHK(ctx, hipMemcpyAsync(chat, ctx->lb.chat, N * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (dir) HIPCHK(ctx, hipMemcpyAsync(dir, ctx->lb.dir, 3 * N * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (feat) HIPCHK(ctx, hipMemcpyAsync(feat, ctx->lb.F, 3 * N * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (g1) HIPCHK(ctx, hipMemcpyAsync(g1, ctx->lb.G1, N * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return FST_OK;
+}
+
+}  // extern "C"
+
+struct fst_mlp {
+  Trainer t;
+  std::string err;
+};
+
+namespace {
+int mfail(fst_mlp *m, int code, const std::string &msg) {
+  if (m) m->err = msg;
+  return code;
+}
+}  // namespace
+
+extern "C" {
+
+int fst_mlp_create(uint64_t seed, fst_mlp **out) {
+  if (!out) return FST_ERR_ARG;
+  *out = nullptr;
+  fst_mlp *m = new fst_mlp();
+  const int rc = trainer_create(m->t, seed);
+  if (rc) {
+    trainer_free(m->t);
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return FST_OK;
+}
+
+int fst_mlp_destroy(fst_mlp *m) {
+  if (!m) return FST_ERR_ARG;
+  if (m->t.s) (void)hipStreamSynchronize(m->t.s);
+  trainer_free(m->t);
+  delete m;
+  return FST_OK;
+}
+
+const char *fst_mlp_last_error(const fst_mlp *m) { return m ? m->err.c_str() : "null trainer"; }
+
+int fst_mlp_set_weights(fst_mlp *m, const float *w, size_t n) {
+  if (!m || !w) return FST_ERR_ARG;
+  if (n != (size_t)MLP_NPARAM) return mfail(m, FST_ERR_ARG, "fst_mlp_set_weights: wrong parameter count");
+  const int rc = trainer_set_weights(m->t, w);
+  return rc ? mfail(m, rc, "fst_mlp_set_weights: copy failed") : FST_OK;
+}
+
+int fst_mlp_get_weights(fst_mlp *m, float *w, size_t n) {
+  if (!m || !w) return FST_ERR_ARG;
+  if (n != (size_t)MLP_NPARAM) return mfail(m, FST_ERR_ARG, "fst_mlp_get_weights: wrong parameter count");
+  const int rc = trainer_get_weights(m->t, w);
+  return rc ? mfail(m, rc, "fst_mlp_get_weights: copy failed") : FST_OK;
+}
+
+int fst_mlp_harvest(fst_mlp *m, fst_ctx *ctx, uint32_t stride, int64_t max_rows, int64_t *added) {
+  if (!m || !ctx) return FST_ERR_ARG;
+  const fst_config &c = ctx->cf;
+  if (c.ndim != 3 || !c.ceq || c.learned_c || c.ns > 3)
+    return mfail(m, FST_ERR_ARG, "fst_mlp_harvest: needs a 3D context with the C-equation (ns <= 3)");
+  const long long np = 6 + c.ns;
+  if (m->t.Pcells != ctx->g.N || m->t.Pplanes != np) {
+    if (m->t.P) (void)hipFree(m->t.P);
+    m->t.P = nullptr;
+    m->t.Pcells = 0;
+    if (hipMalloc((void **)&m->t.P, (size_t)np * ctx->g.N * sizeof(double)) != hipSuccess)
+      return mfail(m, FST_ERR_NOMEM, "fst_mlp_harvest: allocation failed");
+    m->t.Pcells = ctx->g.N;
+    m->t.Pplanes = (int)np;
+  }
+  // ghost cells of the state as compute() would see them, then primitives
+  launch_apply_bcs(ctx->g, c, ctx->b.var, ctx->stream);
+  int rc = fused_primitives(ctx->g, ctx->ph, c, ctx->b.var, m->t.P, ctx->stream);
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = FST_ERR_HIP;
+  long long a = 0;
+  if (!rc) rc = trainer_harvest(m->t, ctx->g, c, ctx->ph, ctx->b.var, m->t.P, stride, max_rows, &a);
+  if (added) *added = a;
+  return rc ? mfail(m, rc, "fst_mlp_harvest: failed") : FST_OK;
+}
+
+int fst_mlp_rows(fst_mlp *m, int64_t *rows) {
+  if (!m || !rows) return FST_ERR_ARG;
+  *rows = m->t.nrows;
+  return FST_OK;
+}
+
+int fst_mlp_read_rows(fst_mlp *m, int64_t r0, int64_t n, float *X, float *T) {
+  if (!m) return FST_ERR_ARG;
+  const int rc = trainer_read_rows(m->t, r0, n, X, T);
+  return rc ? mfail(m, rc, "fst_mlp_read_rows: bad range or copy failed") : FST_OK;
+}
+
+int fst_mlp_train(fst_mlp *m, int epochs, int batch, float lr, uint64_t seed, float *loss_per_epoch) {
+  if (!m || !(lr > 0.f)) return FST_ERR_ARG;
+  const int rc = trainer_train(m->t, epochs, batch, lr, seed, loss_per_epoch);
+  return rc ? mfail(m, rc, "fst_mlp_train: needs epochs >= 1, 16 <= batch <= rows") : FST_OK;
+}
+
+int fst_mlp_forward_backward(fst_mlp *m, const float *X, const float *T, int64_t rows, float *Y, float *grad,
+                             float *loss) {
+  if (!m || !X || !T || rows < 1 || rows > (1 << 24)) return FST_ERR_ARG;
+  const int rc = trainer_forward_backward(m->t, X, T, (int)rows, Y, grad, loss);
+  return rc ? mfail(m, rc, "fst_mlp_forward_backward: failed") : FST_OK;
 }
 
 }  // extern "C"

@@ -15,9 +15,15 @@ struct FusedBufs {
   bool valid = false;       // prim[0] / sc[0] describe the current var (interior)
 };
 
+struct LearnedBufs;
+
 bool fused_supported(const fst_config &cf);
-int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb, int nsteps,
-                hipStream_t s, std::string *err);
+// nsteps RK2 steps of the fused path; lb carries the learned C-indicator
+// fields when cf.learned_c is set (else may be null).
+int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb, LearnedBufs *lb,
+                int nsteps, hipStream_t s, std::string *err);
+// Primitive planes (rho, p, u, v, w, nrg, Y_s) of U over every cell (3D).
+int fused_primitives(const Geo &g, const Phys &ph, const fst_config &cf, const double *U, double *P, hipStream_t s);
 void fused_free(FusedBufs &fb);
 
 }  // namespace fst

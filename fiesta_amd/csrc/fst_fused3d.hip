@@ -35,6 +35,7 @@
 #include <cstdlib>
 
 #include "fst_fused.hpp"
+#include "fst_mlp.hpp"
 
 namespace fst {
 
@@ -216,10 +217,12 @@ __device__ __forceinline__ void face_adv(const FC &k, const Grid32 &g, const dou
 // (calculateStressTensor{x,y,z}3dv, src/viscosity.hpp:416-624);
 // out[3+3*dir+w]: mFlux(D, dir, w) (calculateCeqFaces src/ceq3d.hpp:246-295
 // times calculateCeqGrads src/ceq3d.hpp:306-331).  Both use the same nine
-// face derivatives of the velocity.
-template <int D, int NS, bool F32>
+// face derivatives of the velocity.  LC (learned C-indicator): C_hat and the
+// direction n come from the FP32 planes CH and DR instead of var(nv+1..nv+4).
+template <int D, int NS, bool F32, bool LC>
 __device__ __forceinline__ void face_vc(const FC &k, const Grid32 &g, const double *__restrict__ U,
-                                        const double *__restrict__ P, double alpha, int visc, int ceq,
+                                        const double *__restrict__ P, const float *__restrict__ CH,
+                                        const float *__restrict__ DR, double alpha, int visc, int ceq,
                                         uint32_t c, typename Ar<F32>::R *out) {
   using A = Ar<F32>;
   using R = typename A::R;
@@ -265,11 +268,17 @@ __device__ __forceinline__ void face_vc(const FC &k, const Grid32 &g, const doub
     const int nv = g.nv;
     auto Uv = [&](uint32_t off) { return (R)ld(U, off); };
     const R r = (V(c) + V(c + e)) / (R)2.0;
-    const R chat = (Uv(c + N * (nv + 1)) + Uv(c + e + N * (nv + 1))) / (R)2.0;
-    R cn[3];
-    cn[0] = (Uv(c + N * (nv + 2)) + Uv(c + e + N * (nv + 2))) / (R)2.0;
-    cn[1] = (Uv(c + N * (nv + 3)) + Uv(c + e + N * (nv + 3))) / (R)2.0;
-    cn[2] = (Uv(c + N * (nv + 4)) + Uv(c + e + N * (nv + 4))) / (R)2.0;
+    R chat, cn[3];
+    if (LC) {
+      chat = ((R)CH[c] + (R)CH[c + e]) / (R)2.0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) cn[q] = ((R)DR[c + N * q] + (R)DR[c + e + N * q]) / (R)2.0;
+    } else {
+      chat = (Uv(c + N * (nv + 1)) + Uv(c + e + N * (nv + 1))) / (R)2.0;
+      cn[0] = (Uv(c + N * (nv + 2)) + Uv(c + e + N * (nv + 2))) / (R)2.0;
+      cn[1] = (Uv(c + N * (nv + 3)) + Uv(c + e + N * (nv + 3))) / (R)2.0;
+      cn[2] = (Uv(c + N * (nv + 4)) + Uv(c + e + N * (nv + 4))) / (R)2.0;
+    }
     R cmag = (R)1.0e-6;
     cmag += cn[0] * cn[0];
     cmag += cn[1] * cn[1];
@@ -418,17 +427,19 @@ __global__ void __launch_bounds__(NTHREADS, 1)
 // calculateRhoGrad + updateCeq + applyCeq (src/ceq3d.hpp:143-186, 202-223,
 // 343-365), the Runge-Kutta stage update (src/rk.cpp:116-119 / :137-139),
 // then the primitives and the maxima of the new state.  dvar accumulates in
-// FP64 in both forms.
-template <int NS, bool F32>
+// FP64 in both forms.  LC: learned C-indicator (C_hat, n from CH, DR; no
+// C-equation variables, nvt = nv; maxCh comes from the inference pass).
+template <int NS, bool F32, bool LC>
 __global__ void __launch_bounds__(NTHREADS, 1)
     k_vcr3d(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ Un,
             double *__restrict__ Uo, const double *__restrict__ P, double *__restrict__ Po,
-            const double *__restrict__ dvar, const Scalars *scin, Scalars *scout, double h, int stage, int visc,
-            int ceq, int kchunk, int nbx, int nby, int nbz) {
+            const double *__restrict__ dvar, const float *__restrict__ CH, const float *__restrict__ DR,
+            const Scalars *scin, Scalars *scout, double h, int stage, int visc, int ceq, int kchunk, int nbx,
+            int nby, int nbz) {
   using A = Ar<F32>;
   using R = typename A::R;
   constexpr int NV = 4 + NS;
-  constexpr int NVT = NV + 5;
+  constexpr int NVT = LC ? NV : NV + 5;
   __shared__ R xsh[TY][NFV][TX + 1];
   __shared__ R ysh[NWAVE][NFV][TX];
   __shared__ double red[2][NWAVE];
@@ -441,7 +452,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
   R Z[NFV], Zp[NFV];
 #pragma unroll
   for (int q = 0; q < NFV; ++q) Z[q] = Zp[q] = (R)0.0;
-  if (faces && t.cellok) face_vc<2, NS, F32>(k, g, U, P, alpha, visc, ceq, g.id(t.i, t.jrow, t.kb - 1), Zp);
+  if (faces && t.cellok) face_vc<2, NS, F32, LC>(k, g, U, P, CH, DR, alpha, visc, ceq, g.id(t.i, t.jrow, t.kb - 1), Zp);
   double mS = 0.0, mC = 0.0;
   for (int kk = t.kb; kk < t.ke; ++kk) {
     uint32_t c = g.id(t.i, t.jrow, kk);
@@ -450,23 +461,23 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     if (faces) {
       if (t.cellok) {
         R X[NFV];
-        face_vc<0, NS, F32>(k, g, U, P, alpha, visc, ceq, c, X);
+        face_vc<0, NS, F32, LC>(k, g, U, P, CH, DR, alpha, visc, ceq, c, X);
 #pragma unroll
         for (int q = 0; q < NFV; ++q) xsh[w][q][l + 1] = X[q];
       }
       if (t.xbok) {
         R X[NFV];
-        face_vc<0, NS, F32>(k, g, U, P, alpha, visc, ceq, cxb, X);
+        face_vc<0, NS, F32, LC>(k, g, U, P, CH, DR, alpha, visc, ceq, cxb, X);
 #pragma unroll
         for (int q = 0; q < NFV; ++q) xsh[l][q][0] = X[q];
       }
       if (t.yfok) {
         R Y[NFV];
-        face_vc<1, NS, F32>(k, g, U, P, alpha, visc, ceq, c - g.s1, Y);
+        face_vc<1, NS, F32, LC>(k, g, U, P, CH, DR, alpha, visc, ceq, c - g.s1, Y);
 #pragma unroll
         for (int q = 0; q < NFV; ++q) ysh[w][q][l] = Y[q];
       }
-      if (t.cellok) face_vc<2, NS, F32>(k, g, U, P, alpha, visc, ceq, c, Z);
+      if (t.cellok) face_vc<2, NS, F32, LC>(k, g, U, P, CH, DR, alpha, visc, ceq, c, Z);
       __syncthreads();
     }
     if (t.cellok) {
@@ -509,7 +520,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
           dv[3] += td;
         }
       }
-      if (ceq) {
+      if (ceq && !LC) {
         // calculateRhoGrad, src/ceq3d.hpp:143-186 (rho, nrg, vel planes of P)
         R G[5];
         {
@@ -549,6 +560,8 @@ __global__ void __launch_bounds__(NTHREADS, 1)
                         A::cdv(CU(c - g.s2 + cb) - (R)2 * cc + CU(c + g.s2 + cb), k.d[2], k.rd[2]);
           dv[NV + n] = c1 * (G[n] - cc) + c2 * lap;
         }
+      }
+      if (ceq) {
         // applyCeq, src/ceq3d.hpp:343-365
 #pragma unroll
         for (int ww = 0; ww < 3; ++ww) {
@@ -582,7 +595,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
 #pragma unroll
       for (int q = 0; q < 6 + NS; ++q) st(Po, c + N * q, pr[q]);
       mS = fmax(mS, ms);
-      if (ceq) mC = fmax(mC, fabs(un[NV + 1]));
+      if (ceq && !LC) mC = fmax(mC, fabs(un[NV + 1]));
     }
 #pragma unroll
     for (int q = 0; q < NFV; ++q) Zp[q] = Z[q];
@@ -733,14 +746,15 @@ Launch shape(const Geo &g, int kchunk) {
   return L;
 }
 
-template <int NS, bool F32>
+template <int NS, bool F32, bool LC>
 void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf, const double *U,
-               const double *Un, double *Uo, const double *P, double *Po, double *dvar, const Scalars *scin,
-               Scalars *scout, double h, int stage, int kchunk, Prof *prof, hipStream_t s) {
+               const double *Un, double *Uo, const double *P, double *Po, double *dvar, const float *CH,
+               const float *DR, const Scalars *scin, Scalars *scout, double h, int stage, int kchunk,
+               hipStream_t s) {
   const Launch L = shape(g, kchunk);
   k_adv3d<NS, F32><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
-  k_vcr3d<NS, F32><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, scin, scout, h, stage, cf.visc, cf.ceq,
-                                        kchunk, L.nbx, L.nby, L.nbz);
+  k_vcr3d<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h, stage,
+                                                cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
 }
 
 template <int NS>
@@ -753,16 +767,22 @@ void run_prims(const Grid32 &q, const FC &k, const double *U, double *P, Scalars
 }
 
 using StageFn = void (*)(const Grid32 &, const Geo &, const FC &, const fst_config &, const double *,
-                         const double *, double *, const double *, double *, double *, const Scalars *,
-                         Scalars *, double, int, int, Prof *, hipStream_t);
+                         const double *, double *, const double *, double *, double *, const float *,
+                         const float *, const Scalars *, Scalars *, double, int, int, hipStream_t);
 using PrimFn = void (*)(const Grid32 &, const FC &, const double *, double *, Scalars *, int, Scalars *,
                         hipStream_t);
 
-bool pick(int ns, bool f32, StageFn *sf, PrimFn *pf) {
+template <int NS>
+StageFn pick_ns(bool f32, bool lc) {
+  return f32 ? (lc ? run_stage<NS, true, true> : run_stage<NS, true, false>)
+             : (lc ? run_stage<NS, false, true> : run_stage<NS, false, false>);
+}
+
+bool pick(int ns, bool f32, bool lc, StageFn *sf, PrimFn *pf) {
   switch (ns) {
-    case 1: *sf = f32 ? run_stage<1, true> : run_stage<1, false>; *pf = run_prims<1>; return true;
-    case 2: *sf = f32 ? run_stage<2, true> : run_stage<2, false>; *pf = run_prims<2>; return true;
-    case 3: *sf = f32 ? run_stage<3, true> : run_stage<3, false>; *pf = run_prims<3>; return true;
+    case 1: *sf = pick_ns<1>(f32, lc); *pf = run_prims<1>; return true;
+    case 2: *sf = pick_ns<2>(f32, lc); *pf = run_prims<2>; return true;
+    case 3: *sf = pick_ns<3>(f32, lc); *pf = run_prims<3>; return true;
     default: return false;
   }
 }
@@ -771,21 +791,26 @@ bool pick(int ns, bool f32, StageFn *sf, PrimFn *pf) {
 
 bool fused_supported(const fst_config &cf) {
   if (!(cf.ndim == 3 && cf.grid == 0 && (cf.precision == FST_FP64 || cf.precision == FST_FP32_BULK) &&
-        cf.scheme == FST_WENO5 && cf.ns >= 1 && cf.ns <= 3 && !cf.learned_c))
+        cf.scheme == FST_WENO5 && cf.ns >= 1 && cf.ns <= 3 && (!cf.learned_c || cf.ceq)))
     return false;
   // 32-bit element offsets: the largest array (nvt planes) below 2^29 elements
   const double N = (double)(cf.nci + 2 * cf.ng) * (cf.ncj + 2 * cf.ng) * (cf.nck + 2 * cf.ng);
-  const int nvt = 4 + cf.ns + (cf.ceq ? 5 : 0);
+  const int nvt = 4 + cf.ns + (cf.ceq && !cf.learned_c ? 5 : 0);
   return N * std::max(nvt, 6 + cf.ns) < 536870912.0;
 }
 
-int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb, int nsteps,
-                hipStream_t s, std::string *err) {
+int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb, LearnedBufs *lb,
+                int nsteps, hipStream_t s, std::string *err) {
   StageFn sf;
   PrimFn pf;
-  if (!pick(cf.ns, cf.precision == FST_FP32_BULK, &sf, &pf) || !fused_supported(cf)) {
+  const bool lc = cf.learned_c != 0;
+  if (!pick(cf.ns, cf.precision == FST_FP32_BULK, lc, &sf, &pf) || !fused_supported(cf)) {
     *err = "fused path: configuration not supported";
     return FST_ERR_ARG;
+  }
+  if (lc && (!lb || !lb->have_weights)) {
+    *err = "learned C-indicator: no network weights set";
+    return FST_ERR_STATE;
   }
   const int np = 6 + cf.ns;
   if (!fb.stage) {
@@ -806,7 +831,10 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
   double *Pa = fb.prim, *Pb = fb.prim + (size_t)g.N * np;
   Scalars *S0 = fb.sc, *S1 = fb.sc + 1;
   Prof *prof = b.prof && b.prof->on ? b.prof : nullptr;
-  if (prof) prof->name = "stage: ghost fill + ghost primitives + k_adv3d + k_vcr3d";
+  if (prof)
+    prof->name = lc ? "stage: ghost fill + ghost primitives + [learned C fields, stage 1] + k_adv3d + k_vcr3d"
+                    : "stage: ghost fill + ghost primitives + k_adv3d + k_vcr3d";
+  const float *CH = lc ? lb->chat : nullptr, *DR = lc ? lb->dir : nullptr;
   if (!fb.valid) {  // primitives and maxima of var, all cells
     k_clear<<<1, 64, 0, s>>>(S0);
     pf(q, k, b.var, Pa, S0, 0, nullptr, s);
@@ -818,16 +846,33 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
     if (prof) prof->begin(s);
     launch_apply_bcs(g, cf, b.var, s);       // src/cart3d.cpp:221
     pf(q, k, b.var, Pa, nullptr, 1, S1, s);  // ghost primitives; clears S1
-    sf(q, g, k, cf, b.var, nullptr, fb.stage, Pa, Pb, b.dvar, S0, S1, h1, 1, kchunk, prof, s);
+    if (lc) {  // C_hat and n of U^n, used by both stages of the step
+      const int rc = learned_fields(g, cf, ph, Pa, *lb, s);
+      if (rc) {
+        *err = "learned C-indicator: launch failed";
+        return rc;
+      }
+    }
+    sf(q, g, k, cf, b.var, nullptr, fb.stage, Pa, Pb, b.dvar, CH, DR, lc ? lb->sc : S0, S1, h1, 1, kchunk, s);
     if (prof) prof->end(s);
     // stage 2: U* -> U^{n+1}, written in place into b.var (read pointwise)
     if (prof) prof->begin(s);
     launch_apply_bcs(g, cf, fb.stage, s);
     pf(q, k, fb.stage, Pb, nullptr, 1, S0, s);  // ghost primitives of U*; clears S0
-    sf(q, g, k, cf, fb.stage, b.var, b.var, Pb, Pa, b.dvar, S1, S0, h2, 2, kchunk, prof, s);
+    sf(q, g, k, cf, fb.stage, b.var, b.var, Pb, Pa, b.dvar, CH, DR, lc ? lb->sc : S1, S0, h2, 2, kchunk, s);
     if (prof) prof->end(s);
   }
   return FST_OK;
+}
+
+int fused_primitives(const Geo &g, const Phys &ph, const fst_config &cf, const double *U, double *P, hipStream_t s) {
+  StageFn sf;
+  PrimFn pf;
+  if (!pick(cf.ns, false, false, &sf, &pf) || g.ndim != 3) return FST_ERR_ARG;
+  const FC k = make_fc(ph, cf);
+  const Grid32 q = make_grid(g);
+  pf(q, k, U, P, nullptr, 0, nullptr, s);
+  return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 
 void fused_free(FusedBufs &fb) {

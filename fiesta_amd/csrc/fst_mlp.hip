@@ -1,0 +1,872 @@
+// Learned C-indicator: features, BF16-MFMA MLP inference and training.
+//
+// The MLP stands in for the relaxation solve of the C-equation
+// (updateCeq, src/ceq3d.hpp:189-224): it maps a 5x5x5 stencil of three
+// dimensionless features
+//   f0 = |grad rho| h / rho,  f1 = |grad p| h / p,  f2 = (div u) h / a
+// (2nd-order central differences, a = mixture sound speed, h = min cell size)
+// to the smoothed contact indicator C_hat of the plain method (the variable
+// var(nv+1) that calculateCeqFaces reads, src/ceq3d.hpp:269), predicted as
+// y = C_hat h / rho.  Architecture 375 (padded to 384) -> 64 -> 64 -> 64 -> 1
+// with GELU (tanh form) on the hidden layers.  Products run on MFMA
+// (v_mfma_f32_16x16x16bf16_1k) with BF16 operands and FP32 accumulation;
+// master weights, biases, the output layer and the optimizer are FP32.
+//
+// MFMA 16x16x16 lane layout (lane l, q = 0..3, v = 0..3):
+//   A[i = l%16][k = 4*(l/16)+q], B[k = 4*(l/16)+q][j = l%16],
+//   D[i = 4*(l/16)+v][j = l%16].
+//
+// The direction n of the anisotropic artificial viscosity is the 4th-order
+// density gradient of calculateRhoGrad (src/ceq3d.hpp:145-147, 182-184), the
+// quantity the relaxed variables var(nv+2..nv+4) follow.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fst_mlp.hpp"
+
+namespace fst {
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int W1S = MLP_IN_PAD + 4;  // LDS row stride (shorts) of W1^T: 8-byte reads spread over the banks
+constexpr int WHS = MLP_H + 4;       // LDS row stride of W2^T, W3^T and the activation tiles
+constexpr int INF_W1 = 0;            // inference copy: W1^T [64][384] (zero rows k >= 375), W2^T, W3^T
+constexpr int INF_W2 = MLP_H * MLP_IN_PAD;
+constexpr int INF_W3 = INF_W2 + MLP_H * MLP_H;
+constexpr int INF_N = INF_W3 + MLP_H * MLP_H;
+
+// round-to-nearest-even FP32 -> BF16 (finite inputs)
+__host__ __device__ __forceinline__ short f2bf(float x) {
+  unsigned u = __builtin_bit_cast(unsigned, x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((unsigned)(unsigned short)b) << 16); }
+
+__device__ __forceinline__ float gelu(float x) {
+  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  return 0.5f * x * (1.0f + tanhf(k0 * (x + k1 * x * x * x)));
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  const float t = tanhf(k0 * (x + k1 * x * x * x));
+  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * k0 * (1.0f + 3.0f * k1 * x * x);
+}
+
+// Orders LDS writes and reads among the lanes of one wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float wave_maxf(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+struct G3 {  // plane geometry
+  int ngi, ngj, ngk, ng, nci, ncj, nck;
+  long long N, s1, s2;
+};
+
+G3 make_g3(const Geo &g) {
+  G3 q;
+  q.ngi = g.ngi; q.ngj = g.ngj; q.ngk = g.ngk; q.ng = g.ng;
+  q.nci = g.nci; q.ncj = g.ncj; q.nck = g.nck;
+  q.N = g.N; q.s1 = g.ngi; q.s2 = (long long)g.ngi * g.ngj;
+  return q;
+}
+
+inline unsigned nblk(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------- features
+
+// Per cell: F (cells with one neighbour each way), dir and G1 (cells with two
+// neighbours each way), M = max |F|.  P planes: rho, p, u, v, w, nrg, Y_s.
+// Every value is computed in FP64 in a fixed order and rounded once to FP32.
+__global__ void k_features3d(const G3 g, const FeatConst fc, const double *__restrict__ P, float *__restrict__ F,
+                             float *__restrict__ M, float *__restrict__ Dir, float *__restrict__ G1, Scalars *sc) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  float g1max = 0.f;
+  if (t < g.N) {
+    const long long N = g.N;
+    const int i = (int)(t % g.ngi), j = (int)((t / g.ngi) % g.ngj), k = (int)(t / g.s2);
+    const bool in1 = i >= 1 && j >= 1 && k >= 1 && i < g.ngi - 1 && j < g.ngj - 1 && k < g.ngk - 1;
+    const bool in2 = i >= 2 && j >= 2 && k >= 2 && i < g.ngi - 2 && j < g.ngj - 2 && k < g.ngk - 2;
+    float f0 = 0.f, f1 = 0.f, f2 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f, gg = 0.f;
+    if (in1) {
+      const long long s[3] = {1, g.s1, g.s2};
+      const double *R = P, *Pp = P + N, *E = P + 5 * N;
+      double gr2 = 0.0, gp2 = 0.0, divu = 0.0, dr[3] = {0, 0, 0}, de[3] = {0, 0, 0};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double *V = P + (2 + a) * N;
+        const double h2 = 2.0 * fc.d[a];
+        const double r1 = (R[t + s[a]] - R[t - s[a]]) / h2;
+        const double p1 = (Pp[t + s[a]] - Pp[t - s[a]]) / h2;
+        gr2 = gr2 + r1 * r1;
+        gp2 = gp2 + p1 * p1;
+        divu = divu + (V[t + s[a]] - V[t - s[a]]) / h2;
+        if (in2) {
+          const double h12 = 12.0 * fc.d[a];
+          dr[a] = (R[t - 2 * s[a]] - 8.0 * R[t - s[a]] + 8.0 * R[t + s[a]] - R[t + 2 * s[a]]) / h12;
+          de[a] = (E[t - 2 * s[a]] - 8.0 * E[t - s[a]] + 8.0 * E[t + s[a]] - E[t + 2 * s[a]]) / h12;
+        }
+      }
+      const double rho = R[t], p = Pp[t];
+      double Cp = 0.0, Cv = 0.0;
+      for (int q = 0; q < fc.ns; ++q) {
+        const double Y = P[t + N * (6 + q)];
+        Cp = Cp + Y * fc.cpf[q];
+        Cv = Cv + Y * fc.cvf[q];
+      }
+      const double a_snd = sqrt((Cp / Cv) * p / rho);
+      f0 = (float)(sqrt(gr2) * fc.hmin / rho);
+      f1 = (float)(sqrt(gp2) * fc.hmin / p);
+      f2 = (float)(divu * fc.hmin / a_snd);
+      if (in2) {
+        d0 = (float)dr[0];
+        d1 = (float)dr[1];
+        d2 = (float)dr[2];
+        const double rg2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+        const double dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) * rg2;
+        gg = dnednr < 0.0 ? (float)(sqrt(rg2) * fc.hmin / rho) : 0.f;
+        const bool interior = i >= g.ng && j >= g.ng && k >= g.ng && i < g.ng + g.nci && j < g.ng + g.ncj &&
+                              k < g.ng + g.nck;
+        if (interior) g1max = gg;
+      }
+    }
+    F[t] = f0;
+    F[t + N] = f1;
+    F[t + 2 * N] = f2;
+    M[t] = fmaxf(fabsf(f0), fmaxf(fabsf(f1), fabsf(f2)));
+    Dir[t] = d0;
+    Dir[t + N] = d1;
+    Dir[t + 2 * N] = d2;
+    G1[t] = gg;
+  }
+  g1max = wave_maxf(g1max);
+  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&sc->maxC, (double)g1max);
+}
+
+// Active interior cells: max of M over the 3x3x3 neighbourhood above tau, or
+// (stride > 0) every stride-th interior cell.  Compacted into `list` with one
+// atomic per wave; chat (when given) cleared on every cell.
+__global__ void k_active3d(const G3 g, const float *__restrict__ M, float tau, unsigned stride,
+                           unsigned *__restrict__ list, unsigned *__restrict__ cnt, float *__restrict__ chat) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool act = false;
+  if (t < g.N) {
+    if (chat) chat[t] = 0.f;
+    const int i = (int)(t % g.ngi), j = (int)((t / g.ngi) % g.ngj), k = (int)(t / g.s2);
+    const int ng = g.ng;
+    if (i >= ng && j >= ng && k >= ng && i < ng + g.nci && j < ng + g.ncj && k < ng + g.nck) {
+      float m = 0.f;
+      for (int dk = -1; dk <= 1; ++dk)
+        for (int dj = -1; dj <= 1; ++dj)
+          for (int di = -1; di <= 1; ++di) m = fmaxf(m, M[t + di + dj * g.s1 + dk * g.s2]);
+      act = m > tau;
+      if (stride > 0) {
+        const long long lin = (long long)(i - ng) + (long long)g.nci * ((j - ng) + (long long)g.ncj * (k - ng));
+        act = act || (lin % stride) == 0;
+      }
+    }
+  }
+  const unsigned long long mask = __ballot(act);
+  const int lane = threadIdx.x & 63;
+  if (mask) {
+    const int first = __ffsll((long long)mask) - 1;
+    unsigned base = 0;
+    if (lane == first) base = atomicAdd(cnt, (unsigned)__popcll(mask));
+    base = __shfl(base, first, 64);
+    if (act) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (unsigned)t;
+  }
+}
+
+// Ghost layer next to each face (the cells the face interpolation reads):
+// periodic image or mirror copy, as the ghost fill treats var(nv+1)
+// (src/bc.cpp:33-176).
+__global__ void k_chat_ghost(const G3 g, int xper, int yper, int zper, float *__restrict__ chat) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nx = (long long)g.ncj * g.nck, ny = (long long)g.nci * g.nck, nz = (long long)g.nci * g.ncj;
+  const int ng = g.ng;
+  long long r = t;
+  int face;
+  if (r < 2 * nx) face = 0;
+  else if ((r -= 2 * nx) < 2 * ny) face = 1;
+  else if ((r -= 2 * ny) < 2 * nz) face = 2;
+  else return;
+  const int side = (int)(r & 1);
+  r >>= 1;
+  int i, j, k, si, sj, sk;
+  if (face == 0) {
+    j = ng + (int)(r % g.ncj); k = ng + (int)(r / g.ncj);
+    i = side ? ng + g.nci : ng - 1;
+    si = xper ? (side ? ng : ng + g.nci - 1) : (side ? ng + g.nci - 1 : ng);
+    sj = j; sk = k;
+  } else if (face == 1) {
+    i = ng + (int)(r % g.nci); k = ng + (int)(r / g.nci);
+    j = side ? ng + g.ncj : ng - 1;
+    sj = yper ? (side ? ng : ng + g.ncj - 1) : (side ? ng + g.ncj - 1 : ng);
+    si = i; sk = k;
+  } else {
+    i = ng + (int)(r % g.nci); j = ng + (int)(r / g.nci);
+    k = side ? ng + g.nck : ng - 1;
+    sk = zper ? (side ? ng : ng + g.nck - 1) : (side ? ng + g.nck - 1 : ng);
+    si = i; sj = j;
+  }
+  chat[i + j * g.s1 + k * g.s2] = chat[si + sj * g.s1 + sk * g.s2];
+}
+
+__global__ void k_clear_learned(Scalars *sc, unsigned *cnt) {
+  if (threadIdx.x < 4) {
+    reinterpret_cast<unsigned long long *>(sc)[threadIdx.x] = 0ull;
+    cnt[threadIdx.x] = 0u;
+  }
+}
+
+// FP32 master weights -> BF16 inference copy (transposed, W1 padded to 384).
+__global__ void k_prep_infer(const float *__restrict__ W, short *__restrict__ Wb) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < MLP_H * MLP_IN_PAD) {
+    const int n = t / MLP_IN_PAD, kk = t % MLP_IN_PAD;
+    Wb[INF_W1 + t] = kk < MLP_IN ? f2bf(W[MLP_OFF_W1 + kk * MLP_H + n]) : (short)0;
+  } else if (t < INF_N) {
+    const int q = t - INF_W2;
+    const int layer = q / (MLP_H * MLP_H), r = q % (MLP_H * MLP_H);
+    const int n = r / MLP_H, kk = r % MLP_H;
+    const int off = layer == 0 ? MLP_OFF_W2 : MLP_OFF_W3;
+    Wb[t] = f2bf(W[off + kk * MLP_H + n]);
+  }
+}
+
+// ---------------------------------------------------------------- inference
+
+// Every bf16 array starts at a multiple of 16 bytes (its 8-byte fragment
+// reads stay aligned): 64*388*2, 64*68*2 and 16*68*2 bytes are multiples of 16.
+struct alignas(16) InfLds {
+  short W1[MLP_H * W1S];  // W1^T [n][k]
+  short W2[MLP_H * WHS];  // W2^T
+  short W3[MLP_H * WHS];
+  short act[4][16 * WHS];  // per wave: activations of 16 rows (bf16)
+  float b1[MLP_H], b2[MLP_H], b3[MLP_H], w4[MLP_H];
+  float b4;
+};
+static_assert((MLP_H * W1S * 2) % 16 == 0 && (MLP_H * WHS * 2) % 16 == 0 && (16 * WHS * 2) % 16 == 0,
+              "LDS fragment alignment");
+
+// bias + GELU of the D fragments, to bf16 rows in `act` (row stride WHS)
+__device__ __forceinline__ void store_act(const f32x4 acc[4], const float *b, short *act, int l) {
+  const int i0 = 4 * (l >> 4), j = l & 15;
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) act[(i0 + v) * WHS + nb * 16 + j] = f2bf(gelu(acc[nb][v] + b[nb * 16 + j]));
+}
+
+// acc = act (16 x 64, bf16) x W (64 x 64, given as W^T rows of stride WHS)
+__device__ __forceinline__ void hidden_mm(const short *act, const short *WT, f32x4 acc[4], int l) {
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < MLP_H / 16; ++kb) {
+    const bf16x4 a = *reinterpret_cast<const bf16x4 *>(&act[(l & 15) * WHS + kb * 16 + 4 * (l >> 4)]);
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+      const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&WT[(nb * 16 + (l & 15)) * WHS + kb * 16 + 4 * (l >> 4)]);
+      acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
+    }
+  }
+}
+
+// y of 16 rows through the network; each wave handles tiles of 16 listed
+// cells.  Lanes 0..15 then apply the residual check (maximum principle of the
+// relaxation: 0 <= y <= max G1 over the domain, with tolerance tol) and write
+// C_hat = y rho / h; a failed check takes the local steady state of the
+// C-equation relaxation, the 3x3x3 mean of its source G1.
+__global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__restrict__ F,
+                                                   const unsigned *__restrict__ list, const unsigned *__restrict__ cnt,
+                                                   const short *__restrict__ Wb, const float *__restrict__ W,
+                                                   const float *__restrict__ G1, const double *__restrict__ P,
+                                                   double hmin, float tol, float *__restrict__ chat, Scalars *sc,
+                                                   unsigned *__restrict__ nfallback) {
+  __shared__ InfLds L;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  for (int q = tid; q < MLP_H * MLP_IN_PAD; q += 256) L.W1[(q / MLP_IN_PAD) * W1S + q % MLP_IN_PAD] = Wb[INF_W1 + q];
+  for (int q = tid; q < MLP_H * MLP_H; q += 256) {
+    L.W2[(q / MLP_H) * WHS + q % MLP_H] = Wb[INF_W2 + q];
+    L.W3[(q / MLP_H) * WHS + q % MLP_H] = Wb[INF_W3 + q];
+  }
+  if (tid < MLP_H) {
+    L.b1[tid] = W[MLP_OFF_B1 + tid];
+    L.b2[tid] = W[MLP_OFF_B2 + tid];
+    L.b3[tid] = W[MLP_OFF_B3 + tid];
+    L.w4[tid] = W[MLP_OFF_W4 + tid];
+  }
+  if (tid == 0) L.b4 = W[MLP_OFF_B4];
+  __syncthreads();
+  const unsigned n = cnt[0];
+  const float gmax_all = (float)as_double(sc->maxC);
+  const long long N = g.N, s1 = g.s1, s2 = g.s2;
+  short *act = L.act[w];
+  float mloc = 0.f;
+  unsigned nfb = 0;
+  const unsigned ntiles = (n + 15) / 16;
+  for (unsigned tile = blockIdx.x * 4 + w; tile < ntiles; tile += gridDim.x * 4) {
+    const unsigned r = tile * 16 + (l & 15);
+    // rows past the end repeat the tile's first cell (results discarded)
+    const long long cell = (long long)list[r < n ? r : tile * 16];
+    f32x4 acc[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kb = 0; kb < MLP_IN_PAD / 16; ++kb) {
+      bf16x4 a;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int kk = kb * 16 + 4 * (l >> 4) + q;
+        float x = 0.f;
+        if (kk < MLP_IN) {
+          const int ch = kk / MLP_STEN, o = kk % MLP_STEN;
+          const int dz = o / 25 - 2, dy = (o / 5) % 5 - 2, dx = o % 5 - 2;
+          x = F[cell + N * ch + dx + dy * s1 + dz * s2];
+        }
+        a[q] = f2bf(x);
+      }
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) {
+        const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&L.W1[(nb * 16 + (l & 15)) * W1S + kb * 16 + 4 * (l >> 4)]);
+        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
+      }
+    }
+    store_act(acc, L.b1, act, l);
+    wave_sync();
+    hidden_mm(act, L.W2, acc, l);
+    wave_sync();
+    store_act(acc, L.b2, act, l);
+    wave_sync();
+    hidden_mm(act, L.W3, acc, l);
+    wave_sync();
+    store_act(acc, L.b3, act, l);
+    wave_sync();
+    if (l < 16 && tile * 16 + l < n) {
+      float y = L.b4;
+      for (int q = 0; q < MLP_H; ++q) y += bf2f(act[l * WHS + q]) * L.w4[q];
+      float gsum = 0.f;
+      for (int dk = -1; dk <= 1; ++dk)
+        for (int dj = -1; dj <= 1; ++dj)
+          for (int di = -1; di <= 1; ++di) gsum += G1[cell + di + dj * s1 + dk * s2];
+      const float bound = (1.0f + tol) * gmax_all + tol;
+      float yy;
+      if (isfinite(y) && y >= -tol && y <= bound) {
+        yy = fmaxf(y, 0.f);
+      } else {
+        yy = gsum * (1.0f / 27.0f);
+        ++nfb;
+      }
+      const float ch = (float)((double)yy * P[cell] / hmin);
+      chat[cell] = ch;
+      mloc = fmaxf(mloc, ch);
+    }
+    wave_sync();
+  }
+  mloc = wave_maxf(mloc);
+  for (int o = 32; o > 0; o >>= 1) nfb += __shfl_xor(nfb, o, 64);
+  if (l == 0) {
+    atomic_max_nonneg(&sc->maxCh, (double)mloc);
+    if (nfb) atomicAdd(nfallback, nfb);
+  }
+}
+
+// ---------------------------------------------------------------- training
+
+// Dataset rows: X[r][0..383] (bf16, zeros past 375) = the feature stencil of
+// cell list[r]; T[r] = var(cvar) h / rho of that cell.
+__global__ void k_gather(const G3 g, const float *__restrict__ F, const unsigned *__restrict__ list, unsigned n,
+                         const double *__restrict__ U, int cvar, const double *__restrict__ P, double hmin,
+                         short *__restrict__ X, float *__restrict__ T) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)n * MLP_IN_PAD) return;
+  const unsigned r = (unsigned)(t / MLP_IN_PAD);
+  const int kk = (int)(t % MLP_IN_PAD);
+  const long long cell = list[r];
+  float x = 0.f;
+  if (kk < MLP_IN) {
+    const int ch = kk / MLP_STEN, o = kk % MLP_STEN;
+    const int dz = o / 25 - 2, dy = (o / 5) % 5 - 2, dx = o % 5 - 2;
+    x = F[cell + g.N * ch + dx + dy * g.s1 + dz * g.s2];
+  }
+  X[t] = f2bf(x);
+  if (kk == 0) T[r] = (float)(U[cell + g.N * cvar] * hmin / P[cell]);
+}
+
+// C[m][n] (+)= sum_k A(m,k) B(k,n); bf16 operands at arbitrary strides, one
+// wave per 16x16 tile of C, FP32 accumulation on MFMA.  Each element of C is
+// written by exactly one lane.
+__global__ void k_gemm_bf16(int M, int Nn, int K, const short *__restrict__ A, long long am, long long ak,
+                            const short *__restrict__ B, long long bk, long long bn, float *__restrict__ C,
+                            long long ldc, int accumulate) {
+  const int l = threadIdx.x;
+  const int m0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
+  const int mi = m0 + (l & 15), nj = n0 + (l & 15);
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    bf16x4 a, b;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int kk = k0 + 4 * (l >> 4) + q;
+      a[q] = (mi < M && kk < K) ? A[mi * am + kk * ak] : (short)0;
+      b[q] = (nj < Nn && kk < K) ? B[kk * bk + nj * bn] : (short)0;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int mr = m0 + 4 * (l >> 4) + v;
+    if (mr < M && nj < Nn) {
+      float *cp = C + mr * ldc + nj;
+      *cp = accumulate ? *cp + acc[v] : acc[v];
+    }
+  }
+}
+
+// Z += b (in place), H = bf16(gelu(Z))
+__global__ void k_bias_gelu(long long n, float *__restrict__ Z, const float *__restrict__ b, short *__restrict__ H) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float z = Z[t] + b[t % MLP_H];
+  Z[t] = z;
+  H[t] = f2bf(gelu(z));
+}
+
+// y = H3 . w4 + b4 (same order as inference), dY = 2 (y - T) / M,
+// loss += sum (y - T)^2 / M
+__global__ void k_out_loss(int M, const short *__restrict__ H, const float *__restrict__ W, const float *__restrict__ T,
+                           float *__restrict__ Y, float *__restrict__ dY, float *__restrict__ loss) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  float e2 = 0.f;
+  if (r < M) {
+    float y = W[MLP_OFF_B4];
+    for (int q = 0; q < MLP_H; ++q) y += bf2f(H[(long long)r * MLP_H + q]) * W[MLP_OFF_W4 + q];
+    Y[r] = y;
+    const float e = y - T[r];
+    dY[r] = 2.0f * e / (float)M;
+    e2 = e * e / (float)M;
+  }
+  for (int o = 32; o > 0; o >>= 1) e2 += __shfl_xor(e2, o, 64);
+  if ((threadIdx.x & 63) == 0 && loss) atomicAdd(loss, e2);
+}
+
+// dZ3 = dY w4 gelu'(Z3)
+__global__ void k_out_back(int M, const float *__restrict__ Z, const float *__restrict__ W, const float *__restrict__ dY,
+                           float *__restrict__ dZ, short *__restrict__ dZb) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)M * MLP_H) return;
+  const float d = dY[t / MLP_H] * W[MLP_OFF_W4 + t % MLP_H] * gelu_grad(Z[t]);
+  dZ[t] = d;
+  dZb[t] = f2bf(d);
+}
+
+// dZ = dH gelu'(Z)
+__global__ void k_gelu_back(long long n, const float *__restrict__ dH, const float *__restrict__ Z,
+                            float *__restrict__ dZ, short *__restrict__ dZb) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float v = dH[t] * gelu_grad(Z[t]);
+  dZ[t] = v;
+  dZb[t] = f2bf(v);
+}
+
+// g[n] += sum_r A[r][n] * (Bv ? Bv[r] : 1), one block (4 waves) per column n;
+// A is FP32 or BF16.
+template <bool BF>
+__global__ void k_colsum(int M, int Nn, const void *__restrict__ Av, const float *__restrict__ Bv, float *__restrict__ g) {
+  const int n = blockIdx.x;
+  float s = 0.f;
+  for (int r = threadIdx.x; r < M; r += blockDim.x) {
+    const long long q = (long long)r * Nn + n;
+    const float a = BF ? bf2f(static_cast<const short *>(Av)[q]) : static_cast<const float *>(Av)[q];
+    s += Bv ? a * Bv[r] : a;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  __shared__ float red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) g[n] += (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void k_to_bf16(long long n, const float *__restrict__ a, short *__restrict__ b) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = f2bf(a[t]);
+}
+
+__global__ void k_fill(long long n, float *__restrict__ a, float v) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+// Adam with bias correction, FP32 master weights
+__global__ void k_adam(long long n, float *__restrict__ w, const float *__restrict__ gr, float *__restrict__ m,
+                       float *__restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float g = gr[t];
+  const float mm = b1 * m[t] + (1.0f - b1) * g;
+  const float vv = b2 * v[t] + (1.0f - b2) * g * g;
+  m[t] = mm;
+  v[t] = vv;
+  w[t] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+FeatConst make_feat(const fst_config &cf, const Phys &ph) {
+  FeatConst fc;
+  fc.d[0] = cf.dx;
+  fc.d[1] = cf.dy;
+  fc.d[2] = cf.dz;
+  fc.hmin = std::min(cf.dx, std::min(cf.dy, cf.dz));
+  for (int q = 0; q < FST_MAX_SPECIES; ++q) {
+    fc.cpf[q] = ph.cpf[q];
+    fc.cvf[q] = ph.cvf[q];
+  }
+  fc.ns = cf.ns;
+  return fc;
+}
+
+int features(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, LearnedBufs &L, hipStream_t s) {
+  const G3 q = make_g3(g);
+  k_clear_learned<<<1, 64, 0, s>>>(L.sc, L.cnt);
+  k_features3d<<<nblk(g.N), 256, 0, s>>>(q, make_feat(cf, ph), P, L.F, L.M, L.dir, L.G1, L.sc);
+  return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- solver side
+
+int learned_alloc(const Geo &g, LearnedBufs &L) {
+  if (L.F) return FST_OK;
+  const size_t N = (size_t)g.N;
+  bool ok = hipMalloc((void **)&L.F, N * 3 * sizeof(float)) == hipSuccess &&
+            hipMalloc((void **)&L.M, N * sizeof(float)) == hipSuccess &&
+            hipMalloc((void **)&L.dir, N * 3 * sizeof(float)) == hipSuccess &&
+            hipMalloc((void **)&L.G1, N * sizeof(float)) == hipSuccess &&
+            hipMalloc((void **)&L.chat, N * sizeof(float)) == hipSuccess &&
+            hipMalloc((void **)&L.list, N * sizeof(unsigned)) == hipSuccess &&
+            hipMalloc((void **)&L.cnt, 4 * sizeof(unsigned)) == hipSuccess &&
+            hipMalloc((void **)&L.W, MLP_NPARAM * sizeof(float)) == hipSuccess &&
+            hipMalloc((void **)&L.Wb, INF_N * sizeof(short)) == hipSuccess &&
+            hipMalloc((void **)&L.sc, sizeof(Scalars)) == hipSuccess;
+  if (!ok) return FST_ERR_NOMEM;
+  if (hipMemset(L.chat, 0, N * sizeof(float)) != hipSuccess || hipMemset(L.dir, 0, N * 3 * sizeof(float)) != hipSuccess ||
+      hipMemset(L.sc, 0, sizeof(Scalars)) != hipSuccess || hipMemset(L.cnt, 0, 4 * sizeof(unsigned)) != hipSuccess)
+    return FST_ERR_HIP;
+  return FST_OK;
+}
+
+int learned_set_weights(LearnedBufs &L, const float *host, hipStream_t s) {
+  if (!L.W) return FST_ERR_STATE;
+  if (hipMemcpyAsync(L.W, host, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess)
+    return FST_ERR_HIP;
+  k_prep_infer<<<nblk(INF_N), 256, 0, s>>>(L.W, L.Wb);
+  if (hipStreamSynchronize(s) != hipSuccess) return FST_ERR_HIP;
+  L.have_weights = true;
+  return FST_OK;
+}
+
+int learned_fields(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, LearnedBufs &L,
+                   hipStream_t s) {
+  if (!L.have_weights) return FST_ERR_STATE;
+  int rc = features(g, cf, ph, P, L, s);
+  if (rc) return rc;
+  const G3 q = make_g3(g);
+  k_active3d<<<nblk(g.N), 256, 0, s>>>(q, L.M, L.tau, 0u, L.list, L.cnt, L.chat);
+  const double hmin = std::min(cf.dx, std::min(cf.dy, cf.dz));
+  k_mlp_infer<<<1024, 256, 0, s>>>(q, L.F, L.list, L.cnt, L.Wb, L.W, L.G1, P, hmin, L.tol, L.chat, L.sc,
+                                    L.cnt + 1);
+  const long long nf = 2LL * ((long long)g.ncj * g.nck + (long long)g.nci * g.nck + (long long)g.nci * g.ncj);
+  k_chat_ghost<<<nblk(nf), 256, 0, s>>>(q, cf.xPer, cf.yPer, cf.zPer, L.chat);
+  return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
+}
+
+void learned_free(LearnedBufs &L) {
+  void *ps[] = {L.F, L.M, L.dir, L.G1, L.chat, L.list, L.cnt, L.W, L.Wb, L.sc};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  L = LearnedBufs();
+}
+
+// ---------------------------------------------------------------- trainer
+
+void MlpWork::release() {
+  void *ps[] = {Z1, Z2, Z3, dZ, dH, Y, dY, H1, H2, H3, dZb};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  Z1 = Z2 = Z3 = dZ = dH = Y = dY = nullptr;
+  H1 = H2 = H3 = dZb = nullptr;
+  cap = 0;
+}
+
+namespace {
+
+// splitmix64
+inline unsigned long long smix(unsigned long long &x) {
+  unsigned long long z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline float uni(unsigned long long &x) {
+  return (float)((double)(smix(x) >> 11) * (1.0 / 9007199254740992.0)) * 2.0f - 1.0f;
+}
+
+int ensure_work(Trainer &t, int M) {
+  MlpWork &w = t.work;
+  if (w.cap >= M) return FST_OK;
+  w.release();
+  const size_t f = (size_t)M * MLP_H * sizeof(float), h = (size_t)M * MLP_H * sizeof(short);
+  bool ok = hipMalloc((void **)&w.Z1, f) == hipSuccess && hipMalloc((void **)&w.Z2, f) == hipSuccess &&
+            hipMalloc((void **)&w.Z3, f) == hipSuccess && hipMalloc((void **)&w.dZ, f) == hipSuccess &&
+            hipMalloc((void **)&w.dH, f) == hipSuccess && hipMalloc((void **)&w.H1, h) == hipSuccess &&
+            hipMalloc((void **)&w.H2, h) == hipSuccess && hipMalloc((void **)&w.H3, h) == hipSuccess &&
+            hipMalloc((void **)&w.dZb, h) == hipSuccess && hipMalloc((void **)&w.Y, (size_t)M * 4) == hipSuccess &&
+            hipMalloc((void **)&w.dY, (size_t)M * 4) == hipSuccess;
+  if (!ok) return FST_ERR_NOMEM;
+  w.cap = M;
+  return FST_OK;
+}
+
+// Forward + backward over M rows X (bf16, stride MLP_IN_PAD), targets T.
+// Gradients (parameter-vector layout) accumulate into G when non-null; the
+// mean squared error accumulates into *loss on the device when non-null.
+int fwd_bwd(Trainer &t, const short *X, const float *T, int M, float *G, float *loss) {
+  int rc = ensure_work(t, M);
+  if (rc) return rc;
+  MlpWork &w = t.work;
+  hipStream_t s = t.s;
+  const float *W = t.W;
+  k_to_bf16<<<nblk(MLP_NPARAM), 256, 0, s>>>(MLP_NPARAM, W, t.Wb);
+  const short *W1 = t.Wb + MLP_OFF_W1, *W2 = t.Wb + MLP_OFF_W2, *W3 = t.Wb + MLP_OFF_W3;
+  const dim3 gm((M + 15) / 16, MLP_H / 16);
+  const long long MH = (long long)M * MLP_H;
+  k_gemm_bf16<<<gm, 64, 0, s>>>(M, MLP_H, MLP_IN, X, MLP_IN_PAD, 1, W1, MLP_H, 1, w.Z1, MLP_H, 0);
+  k_bias_gelu<<<nblk(MH), 256, 0, s>>>(MH, w.Z1, W + MLP_OFF_B1, w.H1);
+  k_gemm_bf16<<<gm, 64, 0, s>>>(M, MLP_H, MLP_H, w.H1, MLP_H, 1, W2, MLP_H, 1, w.Z2, MLP_H, 0);
+  k_bias_gelu<<<nblk(MH), 256, 0, s>>>(MH, w.Z2, W + MLP_OFF_B2, w.H2);
+  k_gemm_bf16<<<gm, 64, 0, s>>>(M, MLP_H, MLP_H, w.H2, MLP_H, 1, W3, MLP_H, 1, w.Z3, MLP_H, 0);
+  k_bias_gelu<<<nblk(MH), 256, 0, s>>>(MH, w.Z3, W + MLP_OFF_B3, w.H3);
+  k_out_loss<<<nblk(M), 256, 0, s>>>(M, w.H3, W, T, w.Y, w.dY, loss);
+  if (G) {
+    // output layer: dw4 = H3^T dY, db4 = sum dY
+    k_colsum<true><<<MLP_H, 256, 0, s>>>(M, MLP_H, w.H3, w.dY, G + MLP_OFF_W4);
+    k_colsum<false><<<1, 256, 0, s>>>(M, 1, w.dY, nullptr, G + MLP_OFF_B4);
+    k_out_back<<<nblk(MH), 256, 0, s>>>(M, w.Z3, W, w.dY, w.dZ, w.dZb);  // dZ3
+    k_colsum<false><<<MLP_H, 256, 0, s>>>(M, MLP_H, w.dZ, nullptr, G + MLP_OFF_B3);
+    // dW3 = H2^T dZ3 ; dH2 = dZ3 W3^T
+    k_gemm_bf16<<<dim3(4, 4), 64, 0, s>>>(MLP_H, MLP_H, M, w.H2, 1, MLP_H, w.dZb, MLP_H, 1, G + MLP_OFF_W3, MLP_H, 1);
+    k_gemm_bf16<<<gm, 64, 0, s>>>(M, MLP_H, MLP_H, w.dZb, MLP_H, 1, W3, 1, MLP_H, w.dH, MLP_H, 0);
+    k_gelu_back<<<nblk(MH), 256, 0, s>>>(MH, w.dH, w.Z2, w.dZ, w.dZb);  // dZ2
+    k_colsum<false><<<MLP_H, 256, 0, s>>>(M, MLP_H, w.dZ, nullptr, G + MLP_OFF_B2);
+    k_gemm_bf16<<<dim3(4, 4), 64, 0, s>>>(MLP_H, MLP_H, M, w.H1, 1, MLP_H, w.dZb, MLP_H, 1, G + MLP_OFF_W2, MLP_H, 1);
+    k_gemm_bf16<<<gm, 64, 0, s>>>(M, MLP_H, MLP_H, w.dZb, MLP_H, 1, W2, 1, MLP_H, w.dH, MLP_H, 0);
+    k_gelu_back<<<nblk(MH), 256, 0, s>>>(MH, w.dH, w.Z1, w.dZ, w.dZb);  // dZ1
+    k_colsum<false><<<MLP_H, 256, 0, s>>>(M, MLP_H, w.dZ, nullptr, G + MLP_OFF_B1);
+    // dW1 = X^T dZ1 : (375 x M)(M x 64)
+    k_gemm_bf16<<<dim3((MLP_IN + 15) / 16, 4), 64, 0, s>>>(MLP_IN, MLP_H, M, X, 1, MLP_IN_PAD, w.dZb, MLP_H, 1,
+                                                          G + MLP_OFF_W1, MLP_H, 1);
+  }
+  return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
+}
+
+}  // namespace
+
+int trainer_create(Trainer &t, unsigned long long seed) {
+  if (hipStreamCreateWithFlags(&t.s, hipStreamNonBlocking) != hipSuccess) return FST_ERR_HIP;
+  const size_t pb = MLP_NPARAM * sizeof(float);
+  bool ok = hipMalloc((void **)&t.W, pb) == hipSuccess && hipMalloc((void **)&t.G, pb) == hipSuccess &&
+            hipMalloc((void **)&t.m1, pb) == hipSuccess && hipMalloc((void **)&t.m2, pb) == hipSuccess &&
+            hipMalloc((void **)&t.Wb, MLP_NPARAM * sizeof(short)) == hipSuccess;
+  if (!ok) return FST_ERR_NOMEM;
+  // initial weights: uniform(-a, a), a = sqrt(6 / fan_in) on the hidden
+  // layers, sqrt(3 / fan_in) on the output layer; zero biases
+  std::vector<float> w(MLP_NPARAM, 0.f);
+  unsigned long long x = seed;
+  const float a1 = std::sqrt(6.0f / MLP_IN), a2 = std::sqrt(6.0f / MLP_H), a4 = std::sqrt(3.0f / MLP_H);
+  for (int q = 0; q < MLP_IN * MLP_H; ++q) w[MLP_OFF_W1 + q] = a1 * uni(x);
+  for (int q = 0; q < MLP_H * MLP_H; ++q) w[MLP_OFF_W2 + q] = a2 * uni(x);
+  for (int q = 0; q < MLP_H * MLP_H; ++q) w[MLP_OFF_W3 + q] = a2 * uni(x);
+  for (int q = 0; q < MLP_H; ++q) w[MLP_OFF_W4 + q] = a4 * uni(x);
+  return trainer_set_weights(t, w.data());
+}
+
+void trainer_free(Trainer &t) {
+  void *ps[] = {t.W, t.G, t.m1, t.m2, t.Wb, t.loss, t.X, t.T, t.P};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  t.work.release();
+  learned_free(t.feat);
+  if (t.s) (void)hipStreamDestroy(t.s);
+  t.W = t.G = t.m1 = t.m2 = t.loss = t.T = nullptr;
+  t.Wb = t.X = nullptr;
+  t.P = nullptr;
+  t.s = nullptr;
+}
+
+int trainer_set_weights(Trainer &t, const float *host) {
+  const size_t pb = MLP_NPARAM * sizeof(float);
+  if (hipMemcpyAsync(t.W, host, pb, hipMemcpyHostToDevice, t.s) != hipSuccess) return FST_ERR_HIP;
+  k_fill<<<nblk(MLP_NPARAM), 256, 0, t.s>>>(MLP_NPARAM, t.m1, 0.f);
+  k_fill<<<nblk(MLP_NPARAM), 256, 0, t.s>>>(MLP_NPARAM, t.m2, 0.f);
+  t.adam_t = 0;
+  return hipStreamSynchronize(t.s) == hipSuccess ? FST_OK : FST_ERR_HIP;
+}
+
+int trainer_get_weights(Trainer &t, float *host) {
+  if (hipMemcpyAsync(host, t.W, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost, t.s) != hipSuccess)
+    return FST_ERR_HIP;
+  return hipStreamSynchronize(t.s) == hipSuccess ? FST_OK : FST_ERR_HIP;
+}
+
+int trainer_harvest(Trainer &t, const Geo &g, const fst_config &cf, const Phys &ph, const double *U,
+                    const double *P, unsigned stride, long long max_rows, long long *added) {
+  *added = 0;
+  if (t.feat_cells != g.N) {
+    learned_free(t.feat);
+    t.feat_cells = 0;
+    int rc = learned_alloc(g, t.feat);
+    if (rc) return rc;
+    t.feat_cells = g.N;
+  }
+  hipStream_t s = t.s;
+  int rc = features(g, cf, ph, P, t.feat, s);
+  if (rc) return rc;
+  const G3 q = make_g3(g);
+  k_active3d<<<nblk(g.N), 256, 0, s>>>(q, t.feat.M, t.feat.tau, stride, t.feat.list, t.feat.cnt, nullptr);
+  unsigned n = 0;
+  if (hipMemcpyAsync(&n, t.feat.cnt, sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return FST_ERR_HIP;
+  const long long nn = std::min<long long>(n, std::max<long long>(0, max_rows));
+  if (nn == 0) return FST_OK;
+  if (t.nrows + nn > t.cap) {
+    const long long cap = std::max(t.nrows + nn, 2 * t.cap + 65536);
+    short *X = nullptr;
+    float *T = nullptr;
+    if (hipMalloc((void **)&X, (size_t)cap * MLP_IN_PAD * sizeof(short)) != hipSuccess ||
+        hipMalloc((void **)&T, (size_t)cap * sizeof(float)) != hipSuccess) {
+      if (X) (void)hipFree(X);
+      return FST_ERR_NOMEM;
+    }
+    if (t.nrows) {
+      (void)hipMemcpyAsync(X, t.X, (size_t)t.nrows * MLP_IN_PAD * sizeof(short), hipMemcpyDeviceToDevice, s);
+      (void)hipMemcpyAsync(T, t.T, (size_t)t.nrows * sizeof(float), hipMemcpyDeviceToDevice, s);
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return FST_ERR_HIP;
+    if (t.X) (void)hipFree(t.X);
+    if (t.T) (void)hipFree(t.T);
+    t.X = X;
+    t.T = T;
+    t.cap = cap;
+  }
+  const double hmin = std::min(cf.dx, std::min(cf.dy, cf.dz));
+  k_gather<<<nblk(nn * MLP_IN_PAD), 256, 0, s>>>(q, t.feat.F, t.feat.list, (unsigned)nn, U, g.nv + 1, P, hmin,
+                                                 t.X + (size_t)t.nrows * MLP_IN_PAD, t.T + t.nrows);
+  if (hipStreamSynchronize(s) != hipSuccess) return FST_ERR_HIP;
+  t.nrows += nn;
+  *added = nn;
+  return FST_OK;
+}
+
+int trainer_train(Trainer &t, int epochs, int batch, float lr, unsigned long long seed, float *loss_hist) {
+  if (batch < 16 || t.nrows < batch || epochs < 1) return FST_ERR_ARG;
+  if (t.loss_cap < epochs) {
+    if (t.loss) (void)hipFree(t.loss);
+    t.loss = nullptr;
+    t.loss_cap = 0;
+    if (hipMalloc((void **)&t.loss, (size_t)epochs * sizeof(float)) != hipSuccess) return FST_ERR_NOMEM;
+    t.loss_cap = epochs;
+  }
+  hipStream_t s = t.s;
+  k_fill<<<nblk(epochs), 256, 0, s>>>(epochs, t.loss, 0.f);
+  const long long nb = t.nrows / batch;
+  std::vector<long long> order(nb);
+  unsigned long long x = seed;
+  const float b1 = 0.9f, b2 = 0.999f;
+  for (int ep = 0; ep < epochs; ++ep) {
+    for (long long q = 0; q < nb; ++q) order[q] = q;
+    for (long long q = nb - 1; q > 0; --q) std::swap(order[q], order[smix(x) % (unsigned long long)(q + 1)]);
+    for (long long q = 0; q < nb; ++q) {
+      const long long r0 = order[q] * batch;
+      k_fill<<<nblk(MLP_NPARAM), 256, 0, s>>>(MLP_NPARAM, t.G, 0.f);
+      int rc = fwd_bwd(t, t.X + r0 * MLP_IN_PAD, t.T + r0, batch, t.G, t.loss + ep);
+      if (rc) return rc;
+      t.adam_t += 1;
+      const float c1 = 1.0f - std::pow(b1, (float)t.adam_t), c2 = 1.0f - std::pow(b2, (float)t.adam_t);
+      k_adam<<<nblk(MLP_NPARAM), 256, 0, s>>>(MLP_NPARAM, t.W, t.G, t.m1, t.m2, lr, b1, b2, 1e-8f, c1, c2);
+    }
+  }
+  if (loss_hist &&
+      hipMemcpyAsync(loss_hist, t.loss, (size_t)epochs * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return FST_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return FST_ERR_HIP;
+  if (loss_hist)
+    for (int ep = 0; ep < epochs; ++ep) loss_hist[ep] /= (float)nb;
+  return FST_OK;
+}
+
+int trainer_forward_backward(Trainer &t, const float *Xh, const float *Th, int M, float *Yh, float *Gh, float *lossh) {
+  if (M < 1) return FST_ERR_ARG;
+  std::vector<short> xb((size_t)M * MLP_IN_PAD, 0);
+  for (long long r = 0; r < M; ++r)
+    for (int k = 0; k < MLP_IN; ++k) xb[r * MLP_IN_PAD + k] = f2bf(Xh[r * MLP_IN + k]);
+  short *X = nullptr;
+  float *T = nullptr, *G = nullptr, *L = nullptr;
+  hipStream_t s = t.s;
+  int rc = FST_OK;
+  if (hipMalloc((void **)&X, xb.size() * sizeof(short)) != hipSuccess || hipMalloc((void **)&T, (size_t)M * 4) != hipSuccess ||
+      hipMalloc((void **)&G, MLP_NPARAM * 4) != hipSuccess || hipMalloc((void **)&L, 4) != hipSuccess)
+    rc = FST_ERR_NOMEM;
+  if (!rc) {
+    (void)hipMemcpyAsync(X, xb.data(), xb.size() * sizeof(short), hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(T, Th, (size_t)M * 4, hipMemcpyHostToDevice, s);
+    k_fill<<<nblk(MLP_NPARAM), 256, 0, s>>>(MLP_NPARAM, G, 0.f);
+    k_fill<<<1, 64, 0, s>>>(1, L, 0.f);
+    rc = fwd_bwd(t, X, T, M, G, L);
+    if (!rc) {
+      if (Yh) (void)hipMemcpyAsync(Yh, t.work.Y, (size_t)M * 4, hipMemcpyDeviceToHost, s);
+      if (Gh) (void)hipMemcpyAsync(Gh, G, MLP_NPARAM * 4, hipMemcpyDeviceToHost, s);
+      if (lossh) (void)hipMemcpyAsync(lossh, L, 4, hipMemcpyDeviceToHost, s);
+      rc = hipStreamSynchronize(s) == hipSuccess ? FST_OK : FST_ERR_HIP;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  if (X) (void)hipFree(X);
+  if (T) (void)hipFree(T);
+  if (G) (void)hipFree(G);
+  if (L) (void)hipFree(L);
+  return rc;
+}
+
+int trainer_read_rows(Trainer &t, long long r0, long long n, float *Xh, float *Th) {
+  if (r0 < 0 || n < 0 || r0 + n > t.nrows) return FST_ERR_ARG;
+  if (n == 0) return FST_OK;
+  std::vector<short> xb((size_t)n * MLP_IN_PAD);
+  if (hipMemcpyAsync(xb.data(), t.X + (size_t)r0 * MLP_IN_PAD, xb.size() * sizeof(short), hipMemcpyDeviceToHost, t.s) !=
+          hipSuccess ||
+      (Th && hipMemcpyAsync(Th, t.T + r0, (size_t)n * 4, hipMemcpyDeviceToHost, t.s) != hipSuccess) ||
+      hipStreamSynchronize(t.s) != hipSuccess)
+    return FST_ERR_HIP;
+  if (Xh)
+    for (long long r = 0; r < n; ++r)
+      for (int k = 0; k < MLP_IN; ++k) {
+        const unsigned u = ((unsigned)(unsigned short)xb[r * MLP_IN_PAD + k]) << 16;
+        std::memcpy(&Xh[r * MLP_IN + k], &u, 4);
+      }
+  return FST_OK;
+}
+
+}  // namespace fst

@@ -42,6 +42,24 @@ _EXPORTS = {
     "fst_profile_read": ([C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_char_p, C.c_size_t],
                          C.c_int),
     "fst_copy_probe": ([C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
+    "fst_mlp_nparams": ([], C.c_int),
+    "fst_set_mlp_weights": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
+    "fst_learned_params": ([C.c_void_p, C.c_float, C.c_float], C.c_int),
+    "fst_learned_compute": ([C.c_void_p], C.c_int),
+    "fst_learned_stats": ([C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)],
+                          C.c_int),
+    "fst_learned_fields": ([C.c_void_p] + [C.c_void_p] * 4 + [C.c_size_t], C.c_int),
+    "fst_mlp_create": ([C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
+    "fst_mlp_destroy": ([C.c_void_p], C.c_int),
+    "fst_mlp_last_error": ([C.c_void_p], C.c_char_p),
+    "fst_mlp_set_weights": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
+    "fst_mlp_get_weights": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
+    "fst_mlp_harvest": ([C.c_void_p, C.c_void_p, C.c_uint32, C.c_int64, C.POINTER(C.c_int64)], C.c_int),
+    "fst_mlp_rows": ([C.c_void_p, C.POINTER(C.c_int64)], C.c_int),
+    "fst_mlp_read_rows": ([C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p], C.c_int),
+    "fst_mlp_train": ([C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_uint64, C.c_void_p], C.c_int),
+    "fst_mlp_forward_backward": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                  C.c_void_p], C.c_int),
 }
 
 
@@ -158,14 +176,117 @@ class Module:
 
     def profile_read(self):
         n, ms = C.c_int64(), C.c_double()
-        name = C.create_string_buffer(64)
-        self._chk(lib().fst_profile_read(self.h, C.byref(n), C.byref(ms), name, 64), "fst_profile_read")
+        name = C.create_string_buffer(160)
+        self._chk(lib().fst_profile_read(self.h, C.byref(n), C.byref(ms), name, 160), "fst_profile_read")
         return n.value, ms.value, name.value.decode()
 
     def copy_probe(self, nbytes, reps=10):
         ms = C.c_double()
         self._chk(lib().fst_copy_probe(self.h, nbytes, reps, C.byref(ms)), "fst_copy_probe")
         return ms.value
+
+    # -- learned C-indicator (cfg.learned_c = 1) --------------------------------
+
+    def set_mlp_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._chk(lib().fst_set_mlp_weights(self.h, w.ctypes.data, w.size), "fst_set_mlp_weights")
+
+    def learned_params(self, tau=1e-3, tol=0.05):
+        self._chk(lib().fst_learned_params(self.h, float(tau), float(tol)), "fst_learned_params")
+
+    def learned_compute(self):
+        self._chk(lib().fst_learned_compute(self.h), "fst_learned_compute")
+
+    def learned_stats(self):
+        a, f, m = C.c_int64(), C.c_int64(), C.c_double()
+        self._chk(lib().fst_learned_stats(self.h, C.byref(a), C.byref(f), C.byref(m)), "fst_learned_stats")
+        return a.value, f.value, m.value
+
+    def learned_fields(self):
+        """C_hat (k,j,i), direction (3,k,j,i), features (3,k,j,i), G (k,j,i), float32."""
+        d = self.dims
+        ch = np.empty(d.shape(1), np.float32)
+        dr = np.empty(d.shape(3), np.float32)
+        ft = np.empty(d.shape(3), np.float32)
+        g = np.empty(d.shape(1), np.float32)
+        self._chk(lib().fst_learned_fields(self.h, ch.ctypes.data, dr.ctypes.data, ft.ctypes.data, g.ctypes.data,
+                                           d.N), "fst_learned_fields")
+        return ch[0], dr, ft, g[0]
+
+
+def mlp_nparams():
+    return lib().fst_mlp_nparams()
+
+
+class Trainer:
+    """Dataset + network of the learned C-indicator, trained on the GPU."""
+
+    def __init__(self, seed=1):
+        h = C.c_void_p()
+        rc = lib().fst_mlp_create(int(seed), C.byref(h))
+        if rc != 0:
+            raise FiestaError("fst_mlp_create failed (code %d)" % rc)
+        self.h = h
+        self.nparams = mlp_nparams()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fst_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = lib().fst_mlp_last_error(self.h)
+            raise FiestaError("%s failed (code %d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def weights(self):
+        w = np.empty(self.nparams, np.float32)
+        self._chk(lib().fst_mlp_get_weights(self.h, w.ctypes.data, w.size), "fst_mlp_get_weights")
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._chk(lib().fst_mlp_set_weights(self.h, w.ctypes.data, w.size), "fst_mlp_set_weights")
+
+    def harvest(self, module, stride=0, max_rows=1 << 30):
+        n = C.c_int64()
+        self._chk(lib().fst_mlp_harvest(self.h, module.h, int(stride), int(max_rows), C.byref(n)), "fst_mlp_harvest")
+        return n.value
+
+    def rows(self):
+        n = C.c_int64()
+        self._chk(lib().fst_mlp_rows(self.h, C.byref(n)), "fst_mlp_rows")
+        return n.value
+
+    def read_rows(self, r0, n):
+        X = np.empty((n, 375), np.float32)
+        T = np.empty(n, np.float32)
+        self._chk(lib().fst_mlp_read_rows(self.h, int(r0), int(n), X.ctypes.data, T.ctypes.data), "fst_mlp_read_rows")
+        return X, T
+
+    def train(self, epochs, batch=1024, lr=1e-3, seed=7):
+        loss = np.empty(epochs, np.float32)
+        self._chk(lib().fst_mlp_train(self.h, int(epochs), int(batch), float(lr), int(seed), loss.ctypes.data),
+                  "fst_mlp_train")
+        return loss
+
+    def forward_backward(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        M = T.size
+        assert X.shape == (M, 375)
+        Y = np.empty(M, np.float32)
+        G = np.empty(self.nparams, np.float32)
+        L = np.empty(1, np.float32)
+        self._chk(lib().fst_mlp_forward_backward(self.h, X.ctypes.data, T.ctypes.data, M, Y.ctypes.data,
+                                                 G.ctypes.data, L.ctypes.data), "fst_mlp_forward_backward")
+        return Y, G, float(L[0])
 
 
 def rk_advance(f):

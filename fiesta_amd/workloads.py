@@ -124,11 +124,17 @@ def expansion2d_state(cfg, T0=300.0, r0=0.05):
                           [np.zeros(sh)] * 5 if cfg.ceq else None)
 
 
-def rt3d_config(n=256, ceq=True, visc=True, seed=3, steps_dt=None):
+def rt3d_config(n=256, ceq=True, visc=True, seed=3, steps_dt=None, cfl=0.2):
     """3D Rayleigh-Taylor: 1 m cube, n^3 cells, heavy air (M=28.96 g/mol)
     over light gas (M=4 g/mol), gravity 9.81 along -y, x/z periodic, y
     hydrostatic, viscous (mu 1.8e-5), C-equation (kappa 10, eps 5, alpha 20
-    as the reference's rt3d deck), dt from CFL 0.5 at t=0."""
+    as the reference's rt3d deck), dt = cfl * dx / c_light.
+
+    cfl = 0.2 per direction: with RK2 midpoint and WENO5 the perturbed
+    interface grows without bound at 0.5 per direction (0.5 * 3 in 3D) within
+    ~80 steps, and the C-equation's diffusion (Laplacian divided by d, not d^2)
+    has an RK2 eigenvalue of about 104 * dx * (cfl / 0.5), so coarse grids need
+    the smaller step too."""
     mix = _mix(2, [AIR, HOT])
     cfg = make_config(ndim=3, ns=2, nci=n, ncj=n, nck=n, dx=1.0 / n, dy=1.0 / n, dz=1.0 / n, R=RU,
                       visc=visc, ceq=ceq, buoyancy=True, g=9.81, rhoRef=0.0, kap=10.0, eps=5.0, alpha=20.0,
@@ -136,7 +142,7 @@ def rt3d_config(n=256, ceq=True, visc=True, seed=3, steps_dt=None):
                                                 "outflow", "outflow"), dt=1.0, **mix)
     # c_max = sqrt(gamma_light * p / rho_light) at p ~ 1e5 Pa, T0 = 300 K
     c_light = math.sqrt(1.67 * RU / 0.004 * 300.0)
-    cfg.dt = 0.5 * cfg.dx / c_light if steps_dt is None else steps_dt
+    cfg.dt = cfl * cfg.dx / c_light if steps_dt is None else steps_dt
     return cfg
 
 
@@ -189,3 +195,46 @@ def cfl_dt(cfg, var, cfl):
     c = np.sqrt(gam * p / rho) + np.sqrt(m2) / rho
     dmin = min(cfg.dx, cfg.dy, cfg.dz if cfg.ndim == 3 else cfg.dy)
     return float(cfl * dmin / c.max())
+
+
+def shear3d_config(n=48, nz=4, U=100.0, ceq=True, learned_c=False, precision=0, cfl=0.4):
+    """3D shear layer (Kelvin-Helmholtz): unit square in x-y, n x n x nz cells,
+    heavy air (y > 0.5) moving at +U over light gas moving at -U, equal
+    pressure 1 bar and temperature 300 K, x/z periodic, y outflow, viscous,
+    C-equation with the reference rt3d deck's constants (kappa 10, eps 5,
+    alpha 20)."""
+    mix = _mix(2, [AIR, HOT])
+    cfg = make_config(ndim=3, ns=2, nci=n, ncj=n, nck=nz, dx=1.0 / n, dy=1.0 / n, dz=1.0 / n, R=RU,
+                      visc=True, ceq=ceq, buoyancy=False, kap=10.0, eps=5.0, alpha=20.0, xper=True, zper=True,
+                      bc=("outflow",) * 6, dt=1.0, precision=precision, learned_c=learned_c, **mix)
+    c_light = math.sqrt(1.67 * RU / 0.004 * 300.0)
+    cfg.dt = cfl * cfg.dx / (c_light + U)
+    return cfg
+
+
+def shear3d_state(cfg, seed=1, U=100.0, T0=300.0, p0=1.0e5, nmodes=4, amp=0.1):
+    """Interface y = 0.5 with a tanh profile two cells wide; transverse
+    velocity v = amp*U*sum_m a_m sin(2 pi k_m x + phi_m) exp(-((y-0.5)/0.08)^2)
+    with k_m in {1..6}, a_m ~ U[0.5, 1], phi_m ~ U[0, 2 pi) from the seed."""
+    d = Dims(cfg)
+    rng = np.random.default_rng(seed)
+    x, y, z = _cells(d)
+    delta = 2.0 * cfg.dy
+    s = 0.5 * (1.0 + np.tanh((y - 0.5) / delta))
+    rho_h = p0 / (cfg.R / cfg.M[0] * T0)
+    rho_l = p0 / (cfg.R / cfg.M[1] * T0)
+    sh = np.broadcast(x, y, z).shape
+    rh = (s * rho_h + 1e-6 * rho_h) * np.ones(sh)
+    rl = ((1.0 - s) * rho_l + 1e-6 * rho_l) * np.ones(sh)
+    u = U * np.tanh((y - 0.5) / delta) * np.ones(sh)
+    v = np.zeros(sh)
+    env = np.exp(-((y - 0.5) / 0.08) ** 2)
+    for _ in range(nmodes):
+        k = rng.integers(1, 7)
+        a = rng.uniform(0.5, 1.0)
+        ph = rng.uniform(0.0, 2 * math.pi)
+        v = v + amp * U * a * np.sin(2 * math.pi * k * x + ph) * env
+    w = np.zeros(sh)
+    p = p0 * np.ones(sh)
+    d_nvt_c = Dims(cfg).nvt > Dims(cfg).nv
+    return fill_conserved(cfg, [rh, rl], [u, v, w], p, [np.zeros(sh)] * 5 if d_nvt_c else None)

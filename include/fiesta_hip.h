@@ -32,7 +32,7 @@ extern "C" {
 #endif
 
 #define FST_MAX_SPECIES 8
-#define FST_ABI_VERSION 1
+#define FST_ABI_VERSION 2
 
 /* Boundary types, in the order of `enum class BCType` (src/bc.hpp:31). */
 enum fst_bc_type {
@@ -100,7 +100,9 @@ typedef struct fst_config {
   int32_t bcL, bcR, bcB, bcT, bcH, bcF; /* bc.xmin,xmax,ymin,ymax,zmin,zmax     */
   double dt;             /* time.dt                                             */
   int32_t precision;     /* fst_precision                                       */
-  int32_t learned_c;     /* 1: learned C-indicator replaces the C-equation      */
+  int32_t learned_c;     /* 1: learned C-indicator replaces the C-equation's
+                            relaxation solve (3D, ceq = 1, fused path only;
+                            nvt = nv: no C-equation variables are stored)    */
   int32_t reserved[8];
 } fst_config;
 
@@ -110,7 +112,7 @@ typedef struct fst_ctx fst_ctx;
 typedef struct fst_dims {
   int32_t ngi, ngj, ngk; /* extents with ghosts (ngk = 1 in 2D) */
   int32_t nv;            /* ndim + 1 + ns                       */
-  int32_t nvt;           /* nv + 5 when the C-equation is on    */
+  int32_t nvt;           /* nv + 5 when the C-equation is on (nv in the learned form) */
   int32_t nvx;           /* secondary variables: ndim + 3       */
   int64_t ncells_ghost;  /* ngi*ngj*ngk                         */
   int64_t ncells;        /* nci*ncj*nck                         */
@@ -182,6 +184,65 @@ int fst_profile_read(fst_ctx *ctx, int64_t *launches, double *total_ms, char *na
  * `reps` times, on the context stream; returns the mean ms per copy.  Used to
  * measure the attainable HBM bandwidth and to scale rocprof byte counters. */
 int fst_copy_probe(fst_ctx *ctx, size_t bytes, int reps, double *ms_per_copy);
+
+/* ------------------------------------------------------------------------
+ * Learned C-indicator (no counterpart in the reference: it replaces the
+ * relaxation solve of the C-equation, updateCeq src/ceq3d.hpp:189-224, and
+ * the transport of the five C variables; the artificial-viscosity terms
+ * calculateCeqFaces/Grads/applyCeq src/ceq3d.hpp:226-366 stay).
+ *
+ * Per step, on the stage-1 state: features f0 = |grad rho| h/rho,
+ * f1 = |grad p| h/p, f2 = (div u) h/a (h = min cell size) on every cell; an
+ * MLP (5x5x5 stencil x 3 features -> 64 -> 64 -> 64 -> 1, GELU, BF16 MFMA,
+ * FP32 accumulation) predicts y = C_hat h/rho on the cells whose 3x3x3
+ * neighbourhood has a feature above tau (others: C_hat = 0).  A prediction
+ * outside [-tol, (1+tol) max G + tol] (G = contact-switch source
+ * indicator*|grad rho| h/rho, whose maximum bounds the relaxed C_hat) falls
+ * back to the 3x3x3 mean of G, the local steady state of the relaxation.
+ * The direction n is the 4th-order density gradient.  Both stages of the
+ * step use these fields.
+ * ------------------------------------------------------------------------ */
+
+/* Number of FP32 network parameters: W1[375][64], b1[64], W2[64][64], b2[64],
+ * W3[64][64], b3[64], w4[64], b4 (row-major, input index major). */
+int fst_mlp_nparams(void);
+/* Weights of a learned_c context (copied; required before fst_step). */
+int fst_set_mlp_weights(fst_ctx *ctx, const float *w, size_t n);
+/* Activity threshold tau (default 1e-3) and residual tolerance tol (0.05). */
+int fst_learned_params(fst_ctx *ctx, float tau, float tol);
+/* Ghost fill, primitives and learned fields of the current var, outside a
+ * step (inspection and tests). */
+int fst_learned_compute(fst_ctx *ctx);
+/* Cells evaluated by the MLP, cells that fell back, and max C_hat of the
+ * last field evaluation. */
+int fst_learned_stats(fst_ctx *ctx, int64_t *active, int64_t *fallback, double *maxCh);
+/* Download the last fields (any pointer may be NULL): C_hat [N], direction
+ * [3][N], features [3][N], G [N]; `plane` must equal ncells_ghost. */
+int fst_learned_fields(fst_ctx *ctx, float *chat, float *dir, float *feat, float *g, size_t plane);
+
+/* Trainer: a dataset of (stencil, C_hat h/rho) rows harvested on the device
+ * from plain C-equation runs, and the network being trained (FP32 master
+ * weights, BF16 MFMA forward/backward, Adam).  One HIP stream per trainer. */
+typedef struct fst_mlp fst_mlp;
+int fst_mlp_create(uint64_t seed, fst_mlp **out);
+int fst_mlp_destroy(fst_mlp *m);
+const char *fst_mlp_last_error(const fst_mlp *m);
+int fst_mlp_set_weights(fst_mlp *m, const float *w, size_t n);
+int fst_mlp_get_weights(fst_mlp *m, float *w, size_t n);
+/* Rows from the current var of a plain 3D C-equation context: every active
+ * cell and every `stride`-th interior cell (stride 0: active cells only). */
+int fst_mlp_harvest(fst_mlp *m, fst_ctx *plain, uint32_t stride, int64_t max_rows, int64_t *added);
+int fst_mlp_rows(fst_mlp *m, int64_t *rows);
+/* Rows [r0, r0+n): X [n][375] (BF16 values as FP32), targets [n]. */
+int fst_mlp_read_rows(fst_mlp *m, int64_t r0, int64_t n, float *X, float *T);
+/* Mini-batch Adam over the dataset (seeded batch order); mean batch loss
+ * per epoch into loss_per_epoch (may be NULL). */
+int fst_mlp_train(fst_mlp *m, int epochs, int batch, float lr, uint64_t seed, float *loss_per_epoch);
+/* One forward + backward pass of the current weights over host rows X
+ * [rows][375] with targets T: outputs Y [rows], the gradient of the mean
+ * squared error [nparams] and the loss (any output may be NULL). */
+int fst_mlp_forward_backward(fst_mlp *m, const float *X, const float *T, int64_t rows, float *Y, float *grad,
+                             float *loss);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,10 @@ def lib():
             getattr(L, n).argtypes = [C.c_double] * 5
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_threads.restype = C.c_int
+        L.orc_set_learned.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]
+        L.orc_set_learned.restype = C.c_int
+        L.orc_features.argtypes = [C.c_void_p] + [C.c_void_p] * 4
+        L.orc_features.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -107,6 +111,25 @@ class Oracle:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         lib().orc_maxima(self.h, C.byref(a), C.byref(b), C.byref(c))
         return a.value, b.value, c.value
+
+    def set_learned(self, chat, direction, max_chat):
+        """Learned form: C_hat [N], direction [3][N] (any float dtype; used as
+        doubles) and max C_hat for the following compute() calls."""
+        ch = np.ascontiguousarray(chat, dtype=np.float64).ravel()
+        dr = np.ascontiguousarray(direction, dtype=np.float64).ravel()
+        assert ch.size == self.dims.N and dr.size == 3 * self.dims.N
+        assert lib().orc_set_learned(self.h, ch.ctypes.data, dr.ctypes.data, float(max_chat)) == 0
+
+    def features(self):
+        """Primitives and learned-C features of the current var:
+        F [3][N], M [N], direction [3][N], G [N] (float32, (k,j,i) planes)."""
+        d = self.dims
+        F = np.empty(d.shape(3), np.float32)
+        M = np.empty(d.shape(1), np.float32)
+        Dr = np.empty(d.shape(3), np.float32)
+        G = np.empty(d.shape(1), np.float32)
+        assert lib().orc_features(self.h, F.ctypes.data, M.ctypes.data, Dr.ctypes.data, G.ctypes.data) == 0
+        return F, M[0], Dr, G[0]
 
 
 def weno3d(*f):

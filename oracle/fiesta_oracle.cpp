@@ -58,6 +58,10 @@ struct Orc {
   std::vector<double> mFlux;             // 3D (3,3,ngi,ngj,ngk,3), 2D (2,2,ngi,ngj,2)
   double dxmag;                          // src/cart3d.cpp:173 (no sqrt)
   double maxS = 0, maxC = 0, maxCh = 0, alpha = 0;
+  // learned C-indicator form: C_hat and direction fields given from outside
+  bool learned = false;
+  std::vector<double> lchat, ldir;  // [N], [3][N]
+  double lmaxCh = 0;
 
   size_t id(int i, int j, int k) const {
     return (size_t)i + (size_t)ngi * ((size_t)j + (size_t)ngj * (size_t)k);
@@ -439,10 +443,15 @@ void ceqfaces3d(Orc &o) {
           int ip = face == 0, jp = face == 1, kp = face == 2;
           size_t c = o.id(i, j, k), q = o.id(i + ip, j + jp, k + kp);
           r = (o.rho[c] + o.rho[q]) / 2.0;
-          chat = (o.var[c + N * (nv + 1)] + o.var[q + N * (nv + 1)]) / 2.0;
-          cn[0] = (o.var[c + N * (nv + 2)] + o.var[q + N * (nv + 2)]) / 2.0;
-          cn[1] = (o.var[c + N * (nv + 3)] + o.var[q + N * (nv + 3)]) / 2.0;
-          cn[2] = (o.var[c + N * (nv + 4)] + o.var[q + N * (nv + 4)]) / 2.0;
+          if (o.learned) {  // learned form: the same face composition on the given fields
+            chat = (o.lchat[c] + o.lchat[q]) / 2.0;
+            for (int d = 0; d < 3; ++d) cn[d] = (o.ldir[c + N * d] + o.ldir[q + N * d]) / 2.0;
+          } else {
+            chat = (o.var[c + N * (nv + 1)] + o.var[q + N * (nv + 1)]) / 2.0;
+            cn[0] = (o.var[c + N * (nv + 2)] + o.var[q + N * (nv + 2)]) / 2.0;
+            cn[1] = (o.var[c + N * (nv + 3)] + o.var[q + N * (nv + 3)]) / 2.0;
+            cn[2] = (o.var[c + N * (nv + 4)] + o.var[q + N * (nv + 4)]) / 2.0;
+          }
           cmag = 1.0e-6;
           cmag += cn[0] * cn[0];
           cmag += cn[1] * cn[1];
@@ -983,7 +992,13 @@ void compute3d(Orc &o) {
       applyvisc3d(o, d);
     }
   }
-  if (o.c.ceq) {
+  if (o.c.ceq && o.learned) {
+    // learned form: no relaxation solve; the artificial-viscosity terms of
+    // src/cart3d.cpp:285-292 on the given C_hat, n and max C_hat
+    o.alpha = (o.dxmag / (o.lmaxCh + 1.0e-6)) * o.c.alpha;
+    ceqfaces3d(o);
+    applyceq3d(o);
+  } else if (o.c.ceq) {
     rhograd3d(o);
     maxima3d(o);
     o.alpha = (o.dxmag / (o.maxCh + 1.0e-6)) * o.c.alpha;
@@ -991,6 +1006,75 @@ void compute3d(Orc &o) {
     ceqfaces3d(o);
     applyceq3d(o);
   }
+}
+
+// Features of the learned C-indicator on the current primitives (eos3d and
+// vel3d must have run), in the order and precision of the GPU feature pass:
+// FP64 arithmetic, one rounding to FP32 per output.
+void features3d(Orc &o, float *F, float *M, float *Dir, float *G1) {
+  const size_t N = o.N;
+  const double d[3] = {o.c.dx, o.c.dy, o.c.dz};
+  const double hmin = std::min(d[0], std::min(d[1], d[2]));
+  const long long st[3] = {1, o.ngi, (long long)o.ngi * o.ngj};
+#pragma omp parallel for
+  for (int k = 0; k < o.ngk; ++k)
+    for (int j = 0; j < o.ngj; ++j)
+      for (int i = 0; i < o.ngi; ++i) {
+        const size_t t = o.id(i, j, k);
+        const bool in1 = i >= 1 && j >= 1 && k >= 1 && i < o.ngi - 1 && j < o.ngj - 1 && k < o.ngk - 1;
+        const bool in2 = i >= 2 && j >= 2 && k >= 2 && i < o.ngi - 2 && j < o.ngj - 2 && k < o.ngk - 2;
+        float f0 = 0, f1 = 0, f2 = 0, d0 = 0, d1 = 0, d2 = 0, gg = 0;
+        if (in1) {
+          auto nrg = [&](size_t q) {
+            const double r = o.rho[q], u = o.vel[q], v = o.vel[q + N], w = o.vel[q + 2 * N];
+            return o.var[q + 3 * N] / r - 0.5 * r * (u * u + v * v + w * w);
+          };
+          double gr2 = 0.0, gp2 = 0.0, divu = 0.0, dr[3] = {0, 0, 0}, de[3] = {0, 0, 0};
+          for (int a = 0; a < 3; ++a) {
+            const size_t sp = t + st[a], sm = t - st[a];
+            const double h2 = 2.0 * d[a];
+            const double r1 = (o.rho[sp] - o.rho[sm]) / h2;
+            const double p1 = (o.p[sp] - o.p[sm]) / h2;
+            gr2 = gr2 + r1 * r1;
+            gp2 = gp2 + p1 * p1;
+            divu = divu + (o.vel[sp + N * a] - o.vel[sm + N * a]) / h2;
+            if (in2) {
+              const size_t sp2 = t + 2 * st[a], sm2 = t - 2 * st[a];
+              const double h12 = 12.0 * d[a];
+              dr[a] = (o.rho[sm2] - 8.0 * o.rho[sm] + 8.0 * o.rho[sp] - o.rho[sp2]) / h12;
+              de[a] = (nrg(sm2) - 8.0 * nrg(sm) + 8.0 * nrg(sp) - nrg(sp2)) / h12;
+            }
+          }
+          const double rho = o.rho[t], p = o.p[t];
+          double Cp = 0.0, Cv = 0.0;
+          for (int s = 0; s < o.ns; ++s) {
+            const double gs = o.cd[6 + 3 * s], Rs = o.cd[6 + 3 * s + 1];
+            const double Y = o.var[t + N * (4 + s)] / rho;
+            Cp = Cp + Y * (gs * Rs / (gs - 1));
+            Cv = Cv + Y * (Rs / (gs - 1));
+          }
+          const double a_snd = std::sqrt((Cp / Cv) * p / rho);
+          f0 = (float)(std::sqrt(gr2) * hmin / rho);
+          f1 = (float)(std::sqrt(gp2) * hmin / p);
+          f2 = (float)(divu * hmin / a_snd);
+          if (in2) {
+            d0 = (float)dr[0];
+            d1 = (float)dr[1];
+            d2 = (float)dr[2];
+            const double rg2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+            const double dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) * rg2;
+            gg = dnednr < 0.0 ? (float)(std::sqrt(rg2) * hmin / rho) : 0.f;
+          }
+        }
+        F[t] = f0;
+        F[t + N] = f1;
+        F[t + 2 * N] = f2;
+        M[t] = std::max(std::fabs(f0), std::max(std::fabs(f1), std::fabs(f2)));
+        Dir[t] = d0;
+        Dir[t + N] = d1;
+        Dir[t + 2 * N] = d2;
+        G1[t] = gg;
+      }
 }
 
 // cart2d_func::compute, src/cart2d.cpp:151-237
@@ -1086,7 +1170,8 @@ void *orc_create(const fst_config *cfg) {
   o->ngj = o->ncj + 2 * o->ng;
   o->ngk = cfg->ndim == 3 ? o->nck + 2 * o->ng : 1;
   o->nv = cfg->ndim == 3 ? 4 + o->ns : 3 + o->ns;
-  o->nvt = o->nv + (cfg->ceq ? 5 : 0);
+  o->learned = cfg->learned_c != 0 && cfg->ceq;
+  o->nvt = o->nv + (cfg->ceq && !o->learned ? 5 : 0);
   o->nvx = o->ndim + 3;
   if (cfg->ndim == 2) o->c.dz = cfg->dx;  // src/input.cpp:356
   o->N = (size_t)o->ngi * o->ngj * o->ngk;
@@ -1125,8 +1210,12 @@ void *orc_create(const fst_config *cfg) {
     }
   }
   if (cfg->ceq) {
-    o->gradRho.assign(N * (o->ndim == 3 ? 5 : 4), 0.0);
+    if (!o->learned) o->gradRho.assign(N * (o->ndim == 3 ? 5 : 4), 0.0);
     o->mFlux.assign(o->ndim == 3 ? 27 * N : 8 * N, 0.0);
+  }
+  if (o->learned) {
+    o->lchat.assign(N, 0.0);
+    o->ldir.assign(3 * N, 0.0);
   }
   return o;
 }
@@ -1179,6 +1268,26 @@ void orc_maxima(void *h, double *maxS, double *maxCh, double *alpha) {
   *maxS = o.maxS;
   *maxCh = o.maxCh;
   *alpha = o.alpha;
+}
+
+// Learned form: C_hat [N], direction [3][N] and max C_hat used by compute().
+int orc_set_learned(void *h, const double *chat, const double *dir, double maxCh) {
+  Orc &o = *(Orc *)h;
+  if (!o.learned) return 1;
+  std::memcpy(o.lchat.data(), chat, o.N * sizeof(double));
+  std::memcpy(o.ldir.data(), dir, 3 * o.N * sizeof(double));
+  o.lmaxCh = maxCh;
+  return 0;
+}
+
+// Primitives (eos3d, vel3d) and learned-C features of the current var.
+int orc_features(void *h, float *F, float *M, float *Dir, float *G1) {
+  Orc &o = *(Orc *)h;
+  if (o.ndim != 3) return 1;
+  eos3d(o);
+  vel3d(o);
+  features3d(o, F, M, Dir, G1);
+  return 0;
 }
 
 double orc_weno3d(double f1, double f2, double f3, double f4, double f5) { return weno3d(f1, f2, f3, f4, f5); }

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_struct_size():
     lib = runtime.lib()
-    assert lib.fst_abi_version() == 1
+    assert lib.fst_abi_version() == 2
     # fst_config layout: the ctypes mirror and the C struct must agree in size
     c = FstConfig()
     lib.fst_config_init(C.byref(c))
@@ -78,3 +78,32 @@ def test_null_arguments_are_errors_not_crashes():
     assert lib.fst_create(None, None) == 1
     assert lib.fst_destroy(None) == 1
     assert lib.fst_step(None, 1) == 1
+
+
+def test_learned_form_sizes_and_validation():
+    lib = runtime.lib()
+    cfg = make_config(ndim=3, ns=2, nci=12, ncj=10, nck=8, gamma=(1.4, 1.67), M=(0.029, 0.004),
+                      mu=(1e-5, 1e-5), ceq=True, learned_c=True)
+    d = FstDims()
+    assert lib.fst_config_dims(C.byref(cfg), C.byref(d)) == 0
+    assert d.nvt == d.nv == 6 and Dims(cfg).nvt == 6  # no C-equation variables in the learned form
+    for field, value in (("ceq", 0), ("ndim", 2)):
+        bad = make_config(ndim=3, ns=2, nci=12, ncj=10, nck=8, gamma=(1.4, 1.67), M=(0.029, 0.004),
+                          mu=(1e-5, 1e-5), ceq=True, learned_c=True)
+        setattr(bad, field, value)
+        assert lib.fst_config_dims(C.byref(bad), C.byref(FstDims())) == 1
+    # parameter vector: W1[375][64], b1, W2[64][64], b2, W3[64][64], b3, w4[64], b4
+    assert lib.fst_mlp_nparams() == 375 * 64 + 64 + 2 * (64 * 64 + 64) + 64 + 1
+
+
+def test_learned_entry_points_reject_null_arguments():
+    lib = runtime.lib()
+    assert lib.fst_set_mlp_weights(None, None, 0) == 1
+    assert lib.fst_learned_params(None, 1e-3, 0.05) == 1
+    assert lib.fst_learned_compute(None) == 1
+    assert lib.fst_learned_stats(None, None, None, None) == 1
+    assert lib.fst_learned_fields(None, None, None, None, None, 0) == 1
+    assert lib.fst_mlp_create(1, None) == 1
+    assert lib.fst_mlp_destroy(None) == 1
+    assert lib.fst_mlp_train(None, 1, 16, 1e-3, 1, None) == 1
+    assert lib.fst_mlp_harvest(None, None, 0, 0, None) == 1

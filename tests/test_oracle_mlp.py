@@ -1,0 +1,173 @@
+"""Pins of the learned C-indicator's CPU oracle (oracle/mlp.py, features3d).
+
+- the float64 gradient of the network loss against central finite
+  differences (a dropped term, a transposed operand or a wrong activation
+  derivative fails it);
+- GELU (tanh form) identities: gelu(x) - gelu(-x) = x, gelu(0) = 0, and the
+  derivative against finite differences;
+- bf16 rounding on exactly representable values and ties (round to even);
+- the stencil order of the network inputs;
+- the features on fields whose derivatives the difference formulas
+  reproduce exactly (linear fields);
+- the learned compute path: fed with the plain method's own C variables it
+  reproduces the plain RHS of the flow variables bit for bit.
+"""
+import numpy as np
+
+import oracle
+from oracle import mlp
+from fiesta_amd.config import FST_DVAR, FST_RHO, FST_VAR, Dims, make_config
+
+
+def _params(seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    w = rng.normal(0.0, scale, mlp.NPARAM)
+    w[mlp.OFF_W1:mlp.OFF_B1] *= 0.2
+    return w
+
+
+def test_gradient_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    w = _params(1)
+    X = rng.random((6, mlp.NIN)) * 0.5
+    T = rng.random(6)
+    loss, g, _ = mlp.loss_grad(w, X, T)
+    idx = np.concatenate([rng.choice(mlp.OFF_B1, 60, replace=False),
+                          np.arange(mlp.OFF_B1, mlp.OFF_B1 + 8), rng.choice(np.arange(mlp.OFF_W2, mlp.OFF_B2), 30),
+                          np.arange(mlp.OFF_B2, mlp.OFF_B2 + 8), rng.choice(np.arange(mlp.OFF_W3, mlp.OFF_B3), 30),
+                          np.arange(mlp.OFF_B3, mlp.OFF_B3 + 8), np.arange(mlp.OFF_W4, mlp.NPARAM)])
+    h = 1e-6
+    for q in idx:
+        wp, wm = w.copy(), w.copy()
+        wp[q] += h
+        wm[q] -= h
+        fd = (mlp.loss_grad(wp, X, T)[0] - mlp.loss_grad(wm, X, T)[0]) / (2 * h)
+        assert abs(fd - g[q]) <= 1e-6 * max(1.0, abs(fd)), (q, fd, g[q])
+
+
+def test_gelu_identities():
+    x = np.linspace(-6, 6, 241)
+    assert np.allclose(mlp.gelu(x) - mlp.gelu(-x), x, rtol=0, atol=1e-14)
+    assert mlp.gelu(np.array([0.0]))[0] == 0.0
+    assert abs(mlp.gelu(np.array([20.0]))[0] - 20.0) < 1e-12
+    h = 1e-6
+    fd = (mlp.gelu(x + h) - mlp.gelu(x - h)) / (2 * h)
+    assert np.allclose(fd, mlp.gelu_grad(x), rtol=0, atol=1e-8)
+
+
+def test_bf16_rounding():
+    exact = np.array([0.0, 1.0, -2.5, 0.15625, 3.0 * 2.0 ** -20], np.float32)
+    assert np.array_equal(mlp.bf16(exact), exact.astype(np.float64))
+    one = np.float32(1.0)
+    tie_even = np.float32(1.0 + 2.0 ** -8)           # halfway between 1 and 1 + 2^-7
+    tie_odd = np.float32(1.0 + 3.0 * 2.0 ** -8)      # halfway between 1 + 2^-7 and 1 + 2^-6
+    above = np.float32(1.0 + 2.0 ** -8 + 2.0 ** -20)
+    r = mlp.bf16(np.array([tie_even, tie_odd, above]))
+    assert r[0] == one and r[1] == 1.0 + 2.0 ** -6 and r[2] == 1.0 + 2.0 ** -7
+
+
+def test_stencil_order():
+    cfg = make_config(ndim=3, ns=1, nci=6, ncj=6, nck=6)
+    d = Dims(cfg)
+    F = np.zeros((3, d.ngk, d.ngj, d.ngi))
+    k, j, i = np.meshgrid(np.arange(d.ngk), np.arange(d.ngj), np.arange(d.ngi), indexing="ij")
+    for c in range(3):
+        F[c] = c * 1e6 + i + 100 * j + 10000 * k
+    cell = 4 + d.ngi * (5 + d.ngj * 6)
+    X = mlp.stencils(F, np.array([cell]), d)[0]
+    # input q = ch*125 + (dz+2)*25 + (dy+2)*5 + (dx+2)
+    for ch, dz, dy, dx in ((0, -2, -2, -2), (1, 0, 1, -1), (2, 2, 2, 2), (0, 1, -2, 2)):
+        q = ch * 125 + (dz + 2) * 25 + (dy + 2) * 5 + (dx + 2)
+        assert X[q] == ch * 1e6 + (4 + dx) + 100 * (5 + dy) + 10000 * (6 + dz)
+
+
+def _cfg(**kw):
+    base = dict(ndim=3, ns=2, nci=10, ncj=9, nck=8, dx=0.01, dy=0.012, dz=0.011, gamma=(1.4, 1.67),
+                M=(0.02896, 0.004), mu=(1.8e-5, 2.0e-5), visc=True, ceq=True, kap=10.0, eps=5.0, alpha=20.0,
+                bc=("outflow",) * 6)
+    base.update(kw)
+    return make_config(**base)
+
+
+def _state(cfg, rho, u, p, Y0):
+    """Conserved variables from primitive fields (k,j,i) arrays."""
+    d = Dims(cfg)
+    Ru = cfg.R
+    var = np.zeros(d.shape(d.nvt))
+    cp = Y0 * cfg.gamma[0] * (Ru / cfg.M[0]) / (cfg.gamma[0] - 1) + (1 - Y0) * cfg.gamma[1] * (Ru / cfg.M[1]) / (cfg.gamma[1] - 1)
+    cv = Y0 * (Ru / cfg.M[0]) / (cfg.gamma[0] - 1) + (1 - Y0) * (Ru / cfg.M[1]) / (cfg.gamma[1] - 1)
+    gam = cp / cv
+    for a in range(3):
+        var[a] = rho * u[a]
+    var[3] = p / (gam - 1) + 0.5 * rho * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2)
+    var[4] = rho * Y0
+    var[5] = rho * (1 - Y0)
+    return var, gam
+
+
+def test_features_on_linear_fields():
+    cfg = _cfg(learned_c=True)
+    d = Dims(cfg)
+    k, j, i = np.meshgrid(np.arange(d.ngk), np.arange(d.ngj), np.arange(d.ngi), indexing="ij")
+    x, y, z = i * cfg.dx, j * cfg.dy, k * cfg.dz
+    a = 3.0
+    rho = 1.0 + a * x                       # linear density
+    p = np.full_like(rho, 1.0e5)
+    u = [-40.0 * x, np.zeros_like(x), np.zeros_like(x)]   # uniform compression du/dx = -40
+    var, gam = _state(cfg, rho, u, p, np.ones_like(rho))
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    F, M, Dr, G = o.features()
+    hmin = min(cfg.dx, cfg.dy, cfg.dz)
+    sl = (slice(2, -2),) * 3
+    c = np.sqrt(gam * p / rho)
+    assert np.allclose(F[0][sl], (a * hmin / rho)[sl], rtol=1e-6, atol=0)
+    assert np.allclose(F[1][sl], 0.0, atol=1e-7)
+    assert np.allclose(F[2][sl], (-40.0 * hmin / c)[sl], rtol=1e-6, atol=0)
+    assert np.allclose(Dr[0][sl], a, rtol=1e-6) and np.allclose(Dr[1][sl], 0, atol=1e-6)
+    assert np.allclose(M[sl], np.maximum(np.abs(F[0]), np.abs(F[2]))[sl], rtol=0, atol=0)
+    # zero outside the difference stencils' reach
+    assert np.all(F[:, 0] == 0) and np.all(Dr[:, :, 1] == 0) and np.all(G[:, :, -2] == 0)
+    # contact switch: specific internal energy falls where density rises (p
+    # uniform) -> dnednr < 0 -> G = |grad rho| h / rho
+    cfg2 = _cfg(learned_c=True)
+    var2, _ = _state(cfg2, rho, [np.zeros_like(x)] * 3, p, np.ones_like(rho))
+    o2 = oracle.Oracle(cfg2)
+    o2.set(FST_VAR, var2)
+    _, _, Dr2, G2 = o2.features()
+    assert np.allclose(G2[sl], (np.abs(Dr2[0]) * hmin / rho)[sl], rtol=1e-6)
+
+
+def test_learned_compute_reproduces_plain_terms():
+    """With C_hat, n := the plain method's C variables (after the ghost fill)
+    and max C_hat := theirs, the learned compute equals the plain compute on
+    the flow variables bit for bit (only the relaxation solve is replaced)."""
+    cfg = _cfg(bc=("outflow", "reflective", "hydrostatic", "outflow", "noslip", "outflow"), buoyancy=True,
+               g=9.81)
+    d = Dims(cfg)
+    rng = np.random.default_rng(11)
+    k, j, i = np.meshgrid(np.arange(d.ngk), np.arange(d.ngj), np.arange(d.ngi), indexing="ij")
+    rho = 1.0 + 0.3 * np.tanh((i - 8) / 1.5) + 0.01 * rng.random(i.shape)
+    u = [5 * rng.standard_normal(i.shape) for _ in range(3)]
+    p = 1e5 * (1 + 0.05 * rng.random(i.shape))
+    var, _ = _state(cfg, rho, u, p, 0.5 + 0.4 * np.tanh((j - 6) / 2.0))
+    var[6:] = 50 * rng.random(var[6:].shape)  # C variables
+    plain = oracle.Oracle(cfg)
+    plain.set(FST_VAR, var)
+    plain.compute()
+    want = plain.get(FST_DVAR)[:d.nv]
+    cvars = plain.get(FST_VAR)  # after the ghost fill compute() applied
+    _, maxCh, _ = plain.maxima()
+    lcfg = _cfg(bc=("outflow", "reflective", "hydrostatic", "outflow", "noslip", "outflow"), buoyancy=True,
+                g=9.81, learned_c=True)
+    lrn = oracle.Oracle(lcfg)
+    lrn.set(FST_VAR, var[:d.nv])
+    lrn.set_learned(cvars[d.nv + 1], cvars[d.nv + 2:d.nv + 5], maxCh)
+    lrn.compute()
+    got = lrn.get(FST_DVAR)
+    assert got.shape[0] == d.nv
+    assert np.array_equal(got, want)
+    # and a different C_hat changes the momentum RHS (the term is live)
+    lrn.set_learned(2 * cvars[d.nv + 1], cvars[d.nv + 2:d.nv + 5], maxCh)
+    lrn.compute()
+    assert not np.array_equal(lrn.get(FST_DVAR)[:3], want[:3])
