@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true", help="also time a plain copy kernel")
     ap.add_argument("--forms", default="fp64,fp32,fp32_learned",
                     help="comma list of arithmetic forms to time: fp64 (reported as value), fp32, fp32_learned")
+    ap.add_argument("--save-weights", default=None, help="write the trained network (.npy, float32)")
+    ap.add_argument("--load-weights", default=None, help="use this network (.npy) instead of training one")
     return ap.parse_args()
 
 
@@ -209,9 +211,15 @@ def main():
     forms = [f for f in args.forms.split(",") if f]
     weights, train_info = None, None
     if any(f.endswith("learned") for f in forms):
-        log("training the learned C-indicator")
-        weights, train_info = train_learned()
-        log("trained: %s" % train_info)
+        if args.load_weights:
+            weights = np.load(args.load_weights, allow_pickle=False).astype(np.float32)
+            train_info = {"loaded": os.path.basename(args.load_weights)}
+        else:
+            log("training the learned C-indicator")
+            weights, train_info = train_learned()
+            log("trained: %s" % train_info)
+        if args.save_weights:
+            np.save(args.save_weights, weights)
     results = {}
     for f in forms:
         log("form %s" % f)

@@ -52,4 +52,27 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// Maxima of non-negative values over a 1D block (blockDim.x a multiple of 64,
+// at most 1024): one atomic per block and value, none when the block maximum
+// is 0 (the targets are cleared to 0 before the pass).  Every thread of the
+// block must call it.
+template <int NV>
+__device__ __forceinline__ void block_max_atomic(double (&v)[NV], unsigned long long *const (&dst)[NV]) {
+  __shared__ double red_[NV][16];
+  __syncthreads();  // earlier readers of red_ are done
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const double m = wave_max(v[q]);
+    if ((threadIdx.x & 63) == 0) red_[q][w] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int q = threadIdx.x;
+    double m = 0.0;
+    for (int u = 0; u < (int)(blockDim.x >> 6); ++u) m = fmax(m, red_[q][u]);
+    if (m > 0.0) atomic_max_nonneg(dst[q], m);
+  }
+}
+
 }  // namespace fst

@@ -620,6 +620,391 @@ __global__ void __launch_bounds__(NTHREADS, 1)
   }
 }
 
+// ---------------------------------------------------------------- kernel 2 with an LDS ring
+
+// k_vcr3d with the face fields staged in LDS.  A ring of four planes holds,
+// for rows j0-1..j0+TY and columns i0-1..i0+TX of the tile, u, v, w and, in
+// the FP32 forms, also rho, Y_s and the C-equation fields (plain: the five C
+// variables of U; learned: C_hat and the three direction planes) as FP32.
+// The FP64 form stages u, v, w as doubles (the other fields do not fit next to
+// its FP64 face exchange) and reads the rest from memory.  The block loads one
+// new plane per k step (two steps ahead); faces and the viscous cell terms
+// read the ring.  The arithmetic is k_vcr3d's: the same values in the same
+// operations, so the FP64 form keeps its bitwise parity.
+constexpr int RY = TY + 2, RX = TX + 2, RPL = RY * RX, NSLOT = 4;
+
+template <int NS, bool F32, bool LC>
+struct RingF {
+  static constexpr int NFALL = 4 + NS + (LC ? 4 : 5);  // u v w rho Y_s C...
+  static constexpr int NF = F32 ? NFALL : 3;           // fields held in the ring
+  static constexpr int FY = 4, FC = 4 + NS;            // first Y, first C field
+  static constexpr int NPRE = (NF * RPL + NTHREADS - 1) / NTHREADS;
+  using R = typename Ar<F32>::R;
+};
+
+// Global value of ring field f at cell c (as the form's type R).
+template <int NS, bool F32, bool LC>
+__device__ __forceinline__ typename Ar<F32>::R field_at(uint32_t N, int nv, const double *__restrict__ U,
+                                                        const double *__restrict__ P, const float *__restrict__ CH,
+                                                        const float *__restrict__ DR, int f, uint32_t c) {
+  using RF = RingF<NS, F32, LC>;
+  using R = typename RF::R;
+  if (f < 3) return (R)ld(P, c + N * (2 + f));
+  if (f == 3) return (R)ld(P, c);
+  if (f < RF::FC) return (R)ld(P, c + N * (6 + (f - RF::FY)));
+  if (LC) return (f == RF::FC) ? (R)CH[c] : (R)DR[c + N * (f - RF::FC - 1)];
+  return (R)ld(U, c + N * (nv + (f - RF::FC)));
+}
+
+// Loads of one plane p of the ring (element e = tid + m*NTHREADS).
+template <int NS, bool F32, bool LC>
+__device__ __forceinline__ void ring_fetch(const Grid32 &g, const double *__restrict__ U,
+                                           const double *__restrict__ P, const float *__restrict__ CH,
+                                           const float *__restrict__ DR, int i0, int j0, int p,
+                                           typename Ar<F32>::R *pre) {
+  using RF = RingF<NS, F32, LC>;
+  using R = typename RF::R;
+#pragma unroll
+  for (int m = 0; m < RF::NPRE; ++m) {
+    const int e = threadIdx.x + m * NTHREADS;
+    R v = (R)0;
+    if (e < RF::NF * RPL) {
+      const int f = e / RPL, rem = e - f * RPL;
+      const int r = rem / RX, q = rem - r * RX;
+      const int i = i0 - 1 + q, j = j0 - 1 + r;
+      if (i < g.ngi && j < g.ngj) v = field_at<NS, F32, LC>(g.N, g.nv, U, P, CH, DR, f, g.id(i, j, p));
+    }
+    pre[m] = v;
+  }
+}
+
+template <int NS, bool F32, bool LC>
+__device__ __forceinline__ void ring_store(typename Ar<F32>::R *ring, int slot, const typename Ar<F32>::R *pre) {
+  using RF = RingF<NS, F32, LC>;
+#pragma unroll
+  for (int m = 0; m < RF::NPRE; ++m) {
+    const int e = threadIdx.x + m * NTHREADS;
+    if (e < RF::NF * RPL) ring[slot * RF::NF * RPL + e] = pre[m];
+  }
+}
+
+// Reader of the fields around a cell at ring row r, column q (global index
+// c), with the ring slots of planes kk-1, kk, kk+1: ring fields from LDS,
+// the others from memory.
+template <int NS, bool F32, bool LC>
+struct RingAt {
+  using RF = RingF<NS, F32, LC>;
+  using R = typename RF::R;
+  const R *ring;
+  const double *U, *P;
+  const float *CH, *DR;
+  uint32_t N, s1, s2;
+  int nv;
+  int sl[3];  // slots of relative planes -1, 0, +1
+  int r, q;
+  uint32_t c;
+  __device__ __forceinline__ R operator()(int f, int dp, int dr, int dq) const {
+    if (f < RF::NF) return ring[((sl[dp + 1] * RF::NF + f) * RY + (r + dr)) * RX + (q + dq)];
+    const uint32_t cc = c + (uint32_t)dq + (uint32_t)dr * s1 + (uint32_t)dp * s2;
+    return field_at<NS, F32, LC>(N, nv, U, P, CH, DR, f, cc);
+  }
+};
+
+// face_vc on ring values; D: face direction; the face lies between the cell
+// at (r, q, plane 0) and its +e_D neighbour.
+template <int D, int NS, bool F32, bool LC>
+__device__ __forceinline__ void face_vc_ring(const FC &k, const RingAt<NS, F32, LC> &at,
+                                             typename Ar<F32>::R alpha, int visc, int ceq,
+                                             typename Ar<F32>::R *out) {
+  using A = Ar<F32>;
+  using RF = RingF<NS, F32, LC>;
+  using R = typename A::R;
+  constexpr int ep = D == 2, er = D == 1, eq = D == 0;  // offset of c+e: (plane, row, col)
+  constexpr int T1 = D == 0 ? 1 : 0;
+  constexpr int T2 = D == 2 ? 1 : 2;
+  // offsets of +t for direction dir: x -> col, y -> row, z -> plane
+  auto V = [&](int f, int dp, int dr, int dq) { return at(f, dp, dr, dq); };
+  R der[3][3];
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    der[w][D] = A::cdv(V(w, ep, er, eq) - V(w, 0, 0, 0), k.d[D], k.rd[D]);
+    {
+      constexpr int tp = T1 == 2, tr = T1 == 1, tq = T1 == 0;
+      der[w][T1] = A::cdv((V(w, tp, tr, tq) + V(w, ep + tp, er + tr, eq + tq)) -
+                              (V(w, -tp, -tr, -tq) + V(w, ep - tp, er - tr, eq - tq)),
+                          k.d4[T1], k.rd4[T1]);
+    }
+    {
+      constexpr int tp = T2 == 2, tr = T2 == 1, tq = T2 == 0;
+      der[w][T2] = A::cdv((V(w, tp, tr, tq) + V(w, ep + tp, er + tr, eq + tq)) -
+                              (V(w, -tp, -tr, -tq) + V(w, ep - tp, er - tr, eq - tq)),
+                          k.d4[T2], k.rd4[T2]);
+    }
+  }
+  if (visc) {
+    R mu = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      mu += ((V(RF::FY + s, ep, er, eq) + V(RF::FY + s, 0, 0, 0)) / (R)2.0) * (R)k.mus[s];
+    const R dudx = der[0][0], dudy = der[0][1], dudz = der[0][2];
+    const R dvdx = der[1][0], dvdy = der[1][1], dvdz = der[1][2];
+    const R dwdx = der[2][0], dwdy = der[2][1], dwdz = der[2][2];
+    const R c23 = (R)(2.0 / 3.0), two = (R)2.0;
+    if (D == 0) {
+      out[0] = c23 * mu * (two * dudx - dvdy - dwdz);
+      out[1] = mu * (dudy + dvdx);
+      out[2] = mu * (dudz + dwdx);
+    } else if (D == 1) {
+      out[0] = mu * (dudy + dvdx);
+      out[1] = c23 * mu * (two * dvdy - dudx - dwdz);
+      out[2] = mu * (dvdz + dwdy);
+    } else {
+      out[0] = mu * (dudz + dwdx);
+      out[1] = mu * (dvdz + dwdy);
+      out[2] = c23 * mu * (two * dwdz - dudx - dvdy);
+    }
+  }
+  if (ceq) {
+    constexpr int fch = LC ? RF::FC : RF::FC + 1;  // C_hat
+    constexpr int fcn = fch + 1;                  // direction n
+    const R r = (V(3, 0, 0, 0) + V(3, ep, er, eq)) / (R)2.0;
+    const R chat = (V(fch, 0, 0, 0) + V(fch, ep, er, eq)) / (R)2.0;
+    R cn[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) cn[q] = (V(fcn + q, 0, 0, 0) + V(fcn + q, ep, er, eq)) / (R)2.0;
+    R cmag = (R)1.0e-6;
+    cmag += cn[0] * cn[0];
+    cmag += cn[1] * cn[1];
+    cmag += cn[2] * cn[2];
+    const R arc = alpha * r * chat;
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+      const R dirac = (dir == D) ? (R)1.0 : (R)0.0;
+      const R M = dirac - A::div(cn[D] * cn[dir], cmag);
+      const R X = arc * M;
+#pragma unroll
+      for (int w = 0; w < 3; ++w) out[3 + 3 * dir + w] = X * der[w][dir];
+    }
+  }
+}
+
+template <int NS, bool F32, bool LC>
+__global__ void __launch_bounds__(NTHREADS, 1)
+    k_vcr3d_ring(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ Un,
+                double *__restrict__ Uo, const double *__restrict__ P, double *__restrict__ Po,
+                const double *__restrict__ dvar, const float *__restrict__ CH, const float *__restrict__ DR,
+                const Scalars *scin, Scalars *scout, double h, int stage, int visc, int ceq, int kchunk, int nbx,
+                int nby, int nbz) {
+  using A = Ar<F32>;
+  using RF = RingF<NS, F32, LC>;
+  using R = typename A::R;
+  constexpr int NV = 4 + NS;
+  constexpr int NVT = LC ? NV : NV + 5;
+  __shared__ R ring[NSLOT * RF::NF * RPL];
+  __shared__ R xsh[TY][NFV][TX + 1];
+  __shared__ R ysh[NWAVE][NFV][TX];
+  __shared__ double red[2][NWAVE];
+  const Tile t = place(g, kchunk, nbx, nby, nbz);
+  const int w = t.w, l = t.l;
+  const uint32_t N = g.N;
+  const double maxS = as_double(scin->maxS);
+  const R alpha = (R)((k.dxmag2 / (as_double(scin->maxCh) + 1.0e-6)) * k.alpha_cf);  // src/cart3d.cpp:285
+  const bool faces = visc || ceq;
+  R pre[RF::NPRE];
+  // planes kb-1, kb, kb+1 of the ring
+  for (int p = t.kb - 1; p <= t.kb + 1; ++p) {
+    ring_fetch<NS, F32, LC>(g, U, P, CH, DR, t.i0, t.j0, p, pre);
+    ring_store<NS, F32, LC>(ring, p & 3, pre);
+  }
+  __syncthreads();
+  RingAt<NS, F32, LC> at;
+  at.ring = ring;
+  at.U = U; at.P = P; at.CH = CH; at.DR = DR;
+  at.N = g.N; at.s1 = g.s1; at.s2 = g.s2; at.nv = g.nv;
+  R Z[NFV], Zp[NFV];
+#pragma unroll
+  for (int q = 0; q < NFV; ++q) Z[q] = Zp[q] = (R)0.0;
+  if (faces && t.cellok) {  // z-face between kb-1 and kb
+    at.sl[0] = (t.kb - 2) & 3; at.sl[1] = (t.kb - 1) & 3; at.sl[2] = t.kb & 3;
+    at.r = w + 1; at.q = l + 1;
+    at.c = g.id(t.i, t.jrow, t.kb - 1);
+    face_vc_ring<2, NS, F32, LC>(k, at, alpha, visc, ceq, Zp);
+  }
+  double mS = 0.0, mC = 0.0;
+  for (int kk = t.kb; kk < t.ke; ++kk) {
+    uint32_t c = g.id(t.i, t.jrow, kk);
+    uint32_t cxb = g.id(t.i0 - 1, t.j0 + l, kk);
+    asm volatile("" : "+v"(c), "+v"(cxb));
+    const bool fetch = kk + 2 <= t.ke;  // plane kk+2 is read by iteration kk+1
+    if (fetch) ring_fetch<NS, F32, LC>(g, U, P, CH, DR, t.i0, t.j0, kk + 2, pre);
+    at.sl[0] = (kk - 1) & 3; at.sl[1] = kk & 3; at.sl[2] = (kk + 1) & 3;
+    if (faces) {
+      if (t.cellok) {
+        R X[NFV];
+        at.r = w + 1; at.q = l + 1; at.c = c;
+        face_vc_ring<0, NS, F32, LC>(k, at, alpha, visc, ceq, X);
+#pragma unroll
+        for (int q = 0; q < NFV; ++q) xsh[w][q][l + 1] = X[q];
+      }
+      if (t.xbok) {  // x-face left of the tile, row l
+        R X[NFV];
+        at.r = l + 1; at.q = 0; at.c = cxb;
+        face_vc_ring<0, NS, F32, LC>(k, at, alpha, visc, ceq, X);
+#pragma unroll
+        for (int q = 0; q < NFV; ++q) xsh[l][q][0] = X[q];
+      }
+      if (t.yfok) {  // y-face below row w (between rows w-1 and w)
+        R Y[NFV];
+        at.r = w; at.q = l + 1; at.c = c - g.s1;
+        face_vc_ring<1, NS, F32, LC>(k, at, alpha, visc, ceq, Y);
+#pragma unroll
+        for (int q = 0; q < NFV; ++q) ysh[w][q][l] = Y[q];
+      }
+      if (t.cellok) {
+        at.r = w + 1; at.q = l + 1; at.c = c;
+        face_vc_ring<2, NS, F32, LC>(k, at, alpha, visc, ceq, Z);
+      }
+    }
+    if (fetch) ring_store<NS, F32, LC>(ring, (kk + 2) & 3, pre);
+    __syncthreads();
+    if (t.cellok) {
+      at.r = w + 1; at.q = l + 1; at.c = c;
+      double dv[NVT];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) dv[v] = ld(dvar, c + N * v);
+      auto PV = [&](uint32_t off) { return (R)ld(P, off); };
+      if (visc) {
+#pragma unroll
+        for (int D = 0; D < 3; ++D) {
+          const int dp = D == 2, dr = D == 1, dq = D == 0;
+          R T0c, T0m, T1c, T1m, T2c, T2m;
+          if (D == 0) {
+            T0c = xsh[w][0][l + 1]; T0m = xsh[w][0][l];
+            T1c = xsh[w][1][l + 1]; T1m = xsh[w][1][l];
+            T2c = xsh[w][2][l + 1]; T2m = xsh[w][2][l];
+          } else if (D == 1) {
+            T0c = ysh[w + 1][0][l]; T0m = ysh[w][0][l];
+            T1c = ysh[w + 1][1][l]; T1m = ysh[w][1][l];
+            T2c = ysh[w + 1][2][l]; T2m = ysh[w][2][l];
+          } else {
+            T0c = Z[0]; T0m = Zp[0];
+            T1c = Z[1]; T1m = Zp[1];
+            T2c = Z[2]; T2m = Zp[2];
+          }
+          const R V0 = at(0, 0, 0, 0), V1 = at(1, 0, 0, 0), V2 = at(2, 0, 0, 0);
+          const R ur = (at(0, dp, dr, dq) + V0) / (R)2.0, ul = (at(0, -dp, -dr, -dq) + V0) / (R)2.0;
+          const R vr = (at(1, dp, dr, dq) + V1) / (R)2.0, vl = (at(1, -dp, -dr, -dq) + V1) / (R)2.0;
+          const R wr = (at(2, dp, dr, dq) + V2) / (R)2.0, wl = (at(2, -dp, -dr, -dq) + V2) / (R)2.0;
+          const double dd = k.d[D], rdd = k.rd[D];
+          const R ta = A::cdv(T0c - T0m, dd, rdd);
+          const R tb = A::cdv(T1c - T1m, dd, rdd);
+          const R tc = A::cdv(T2c - T2m, dd, rdd);
+          const R td = A::cdv(ur * T0c - ul * T0m, dd, rdd) + A::cdv(vr * T1c - vl * T1m, dd, rdd) +
+                           A::cdv(wr * T2c - wl * T2m, dd, rdd);
+          dv[0] += ta;
+          dv[1] += tb;
+          dv[2] += tc;
+          dv[3] += td;
+        }
+      }
+      if (ceq && !LC) {
+        // calculateRhoGrad, src/ceq3d.hpp:143-186 (rho, nrg, vel planes of P)
+        R G[5];
+        {
+          R dr[3], de[3], du[3];
+#pragma unroll
+          for (int D = 0; D < 3; ++D) {
+            const uint32_t e = D == 0 ? 1u : (D == 1 ? g.s1 : g.s2);
+            const uint32_t rb = 0, eb = 5 * N, vb = (2 + D) * N;
+            dr[D] = A::cdv(PV(c - 2 * e + rb) - (R)8.0 * PV(c - e + rb) + (R)8.0 * PV(c + e + rb) -
+                               PV(c + 2 * e + rb), k.d12[D], k.rd12[D]);
+            de[D] = A::cdv(PV(c - 2 * e + eb) - (R)8.0 * PV(c - e + eb) + (R)8.0 * PV(c + e + eb) -
+                               PV(c + 2 * e + eb), k.d12[D], k.rd12[D]);
+            du[D] = A::cdv(PV(c - 2 * e + vb) - (R)8.0 * PV(c - e + vb) + (R)8.0 * PV(c + e + vb) -
+                               PV(c + 2 * e + vb), k.d12[D], k.rd12[D]);
+          }
+          const R rgrad = A::sq(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+          const R divu = du[0] + du[1] + du[2];
+          const R dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) *
+                           (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+          const int ind = dnednr < (R)0.0 ? 1 : 0;
+          G[0] = (divu < 0) ? (R)(1 - ind) * rgrad : (R)0;
+          G[1] = (R)ind * rgrad;
+          G[2] = dr[0];
+          G[3] = dr[1];
+          G[4] = dr[2];
+        }
+        // updateCeq, src/ceq3d.hpp:202-223 (C variables from the ring)
+        const R c1 = (R)(maxS / (k.eps * k.dxmag));
+        const R c2 = (R)(k.kap * maxS * k.dxmag);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+          const int f = RF::FC + n;
+          const R cc = at(f, 0, 0, 0);
+          const R lap = A::cdv(at(f, 0, 0, -1) - (R)2 * cc + at(f, 0, 0, 1), k.d[0], k.rd[0]) +
+                        A::cdv(at(f, 0, -1, 0) - (R)2 * cc + at(f, 0, 1, 0), k.d[1], k.rd[1]) +
+                        A::cdv(at(f, -1, 0, 0) - (R)2 * cc + at(f, 1, 0, 0), k.d[2], k.rd[2]);
+          dv[NV + n] = c1 * (G[n] - cc) + c2 * lap;
+        }
+      }
+      if (ceq) {
+        // applyCeq, src/ceq3d.hpp:343-365
+#pragma unroll
+        for (int ww = 0; ww < 3; ++ww) {
+          R diffu = (R)0.0;
+#pragma unroll
+          for (int dir = 0; dir < 3; ++dir)
+            diffu += A::cdv(xsh[w][3 + 3 * dir + ww][l + 1] - xsh[w][3 + 3 * dir + ww][l], k.d[0], k.rd[0]);
+#pragma unroll
+          for (int dir = 0; dir < 3; ++dir)
+            diffu += A::cdv(ysh[w + 1][3 + 3 * dir + ww][l] - ysh[w][3 + 3 * dir + ww][l], k.d[1], k.rd[1]);
+#pragma unroll
+          for (int dir = 0; dir < 3; ++dir)
+            diffu += A::cdv(Z[3 + 3 * dir + ww] - Zp[3 + 3 * dir + ww], k.d[2], k.rd[2]);
+          dv[ww] += diffu;
+        }
+      }
+      // Runge-Kutta stage update and primitives of the new state (FP64)
+      const int nvt = g.nvt;
+      double un[NVT];
+#pragma unroll
+      for (int v = 0; v < NVT; ++v) {
+        if (v < nvt) {
+          const double base = (stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v);
+          un[v] = base + h * dv[v];
+          st(Uo, c + N * v, un[v]);
+        }
+      }
+      double pr[6 + NS];
+      double ms;
+      prims_of<NS>(k, un, pr, ms);
+#pragma unroll
+      for (int q = 0; q < 6 + NS; ++q) st(Po, c + N * q, pr[q]);
+      mS = fmax(mS, ms);
+      if (ceq && !LC) mC = fmax(mC, fabs(un[NV + 1]));
+    }
+#pragma unroll
+    for (int q = 0; q < NFV; ++q) Zp[q] = Z[q];
+    __syncthreads();
+  }
+  mS = wave_max(mS);
+  mC = wave_max(mC);
+  if (l == 0) {
+    red[0][w] = mS;
+    red[1][w] = mC;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0, cm = 0.0;
+    for (int q = 0; q < NWAVE; ++q) {
+      s = fmax(s, red[0][q]);
+      cm = fmax(cm, red[1][q]);
+    }
+    atomic_max_nonneg(&scout->maxS, s);
+    atomic_max_nonneg(&scout->maxCh, cm);
+  }
+}
+
 // ---------------------------------------------------------------- primitives
 
 // Primitives of U over the whole ghost range, with the maxima of the interior
@@ -677,13 +1062,10 @@ __global__ void k_prims3d(const Grid32 g, const FC k, const double *__restrict__
     if (!interior) ms = 0.0;
     else if (g.nvt > g.nv) mc = fabs(ld(U, cell + N * (g.nv + 1)));
   }
-  if (!ghost_only && sc) {
-    ms = wave_max(ms);
-    mc = wave_max(mc);
-    if ((threadIdx.x & 63) == 0) {
-      atomic_max_nonneg(&sc->maxS, ms);
-      atomic_max_nonneg(&sc->maxCh, mc);
-    }
+  if (!ghost_only && sc) {  // uniform over the block
+    double v2[2] = {ms, mc};
+    unsigned long long *const d2[2] = {&sc->maxS, &sc->maxCh};
+    block_max_atomic<2>(v2, d2);
   }
 }
 
@@ -753,8 +1135,12 @@ void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf,
                hipStream_t s) {
   const Launch L = shape(g, kchunk);
   k_adv3d<NS, F32><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
-  k_vcr3d<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h, stage,
-                                                cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
+  if (!getenv("FST_VCR_GLOBAL"))
+    k_vcr3d_ring<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h,
+                                                       stage, cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
+  else
+    k_vcr3d<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h, stage,
+                                                  cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
 }
 
 template <int NS>

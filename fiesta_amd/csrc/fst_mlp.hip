@@ -84,19 +84,23 @@ G3 make_g3(const Geo &g) {
 }
 
 inline unsigned nblk(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+// 3D launch grid of the per-cell passes: block (64, 4) over (i, j), one z-slice per k
+inline dim3 grid3(const Geo &g) { return dim3((g.ngi + 63) / 64, (g.ngj + 3) / 4, g.ngk); }
 
 // ---------------------------------------------------------------- features
 
 // Per cell: F (cells with one neighbour each way), dir and G1 (cells with two
 // neighbours each way), M = max |F|.  P planes: rho, p, u, v, w, nrg, Y_s.
 // Every value is computed in FP64 in a fixed order and rounded once to FP32.
-__global__ void k_features3d(const G3 g, const FeatConst fc, const double *__restrict__ P, float *__restrict__ F,
-                             float *__restrict__ M, float *__restrict__ Dir, float *__restrict__ G1, Scalars *sc) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// Launch: block (64, 4), grid (ceil(ngi/64), ceil(ngj/4), ngk).
+__global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst fc, const double *__restrict__ P,
+                                                    float *__restrict__ F, float *__restrict__ M,
+                                                    float *__restrict__ Dir, float *__restrict__ G1, Scalars *sc) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  const long long t = (long long)i + g.s1 * j + g.s2 * k;
   float g1max = 0.f;
-  if (t < g.N) {
+  if (i < g.ngi && j < g.ngj) {
     const long long N = g.N;
-    const int i = (int)(t % g.ngi), j = (int)((t / g.ngi) % g.ngj), k = (int)(t / g.s2);
     const bool in1 = i >= 1 && j >= 1 && k >= 1 && i < g.ngi - 1 && j < g.ngj - 1 && k < g.ngk - 1;
     const bool in2 = i >= 2 && j >= 2 && k >= 2 && i < g.ngi - 2 && j < g.ngj - 2 && k < g.ngk - 2;
     float f0 = 0.f, f1 = 0.f, f2 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f, gg = 0.f;
@@ -151,20 +155,29 @@ __global__ void k_features3d(const G3 g, const FeatConst fc, const double *__res
     Dir[t + 2 * N] = d2;
     G1[t] = gg;
   }
+  // block maximum, then one atomic per block that has a non-zero value
+  __shared__ float red[4];
   g1max = wave_maxf(g1max);
-  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&sc->maxC, (double)g1max);
+  if (threadIdx.x == 0) red[threadIdx.y] = g1max;
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const float m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    if (m > 0.f) atomic_max_nonneg(&sc->maxC, (double)m);
+  }
 }
 
 // Active interior cells: max of M over the 3x3x3 neighbourhood above tau, or
 // (stride > 0) every stride-th interior cell.  Compacted into `list` with one
 // atomic per wave; chat (when given) cleared on every cell.
-__global__ void k_active3d(const G3 g, const float *__restrict__ M, float tau, unsigned stride,
-                           unsigned *__restrict__ list, unsigned *__restrict__ cnt, float *__restrict__ chat) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// Launch: block (64, 4), grid (ceil(ngi/64), ceil(ngj/4), ngk).
+__global__ void __launch_bounds__(256) k_active3d(const G3 g, const float *__restrict__ M, float tau,
+                                                  unsigned stride, unsigned *__restrict__ list,
+                                                  unsigned *__restrict__ cnt, float *__restrict__ chat) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  const long long t = (long long)i + g.s1 * j + g.s2 * k;
   bool act = false;
-  if (t < g.N) {
+  if (i < g.ngi && j < g.ngj) {
     if (chat) chat[t] = 0.f;
-    const int i = (int)(t % g.ngi), j = (int)((t / g.ngi) % g.ngj), k = (int)(t / g.s2);
     const int ng = g.ng;
     if (i >= ng && j >= ng && k >= ng && i < ng + g.nci && j < ng + g.ncj && k < ng + g.nck) {
       float m = 0.f;
@@ -179,7 +192,7 @@ __global__ void k_active3d(const G3 g, const float *__restrict__ M, float tau, u
     }
   }
   const unsigned long long mask = __ballot(act);
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x;
   if (mask) {
     const int first = __ffsll((long long)mask) - 1;
     unsigned base = 0;
@@ -378,7 +391,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__re
   mloc = wave_maxf(mloc);
   for (int o = 32; o > 0; o >>= 1) nfb += __shfl_xor(nfb, o, 64);
   if (l == 0) {
-    atomic_max_nonneg(&sc->maxCh, (double)mloc);
+    if (mloc > 0.f) atomic_max_nonneg(&sc->maxCh, (double)mloc);
     if (nfb) atomicAdd(nfallback, nfb);
   }
 }
@@ -540,7 +553,7 @@ FeatConst make_feat(const fst_config &cf, const Phys &ph) {
 int features(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, LearnedBufs &L, hipStream_t s) {
   const G3 q = make_g3(g);
   k_clear_learned<<<1, 64, 0, s>>>(L.sc, L.cnt);
-  k_features3d<<<nblk(g.N), 256, 0, s>>>(q, make_feat(cf, ph), P, L.F, L.M, L.dir, L.G1, L.sc);
+  k_features3d<<<grid3(g), dim3(64, 4), 0, s>>>(q, make_feat(cf, ph), P, L.F, L.M, L.dir, L.G1, L.sc);
   return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 
@@ -584,7 +597,7 @@ int learned_fields(const Geo &g, const fst_config &cf, const Phys &ph, const dou
   int rc = features(g, cf, ph, P, L, s);
   if (rc) return rc;
   const G3 q = make_g3(g);
-  k_active3d<<<nblk(g.N), 256, 0, s>>>(q, L.M, L.tau, 0u, L.list, L.cnt, L.chat);
+  k_active3d<<<grid3(g), dim3(64, 4), 0, s>>>(q, L.M, L.tau, 0u, L.list, L.cnt, L.chat);
   const double hmin = std::min(cf.dx, std::min(cf.dy, cf.dz));
   k_mlp_infer<<<1024, 256, 0, s>>>(q, L.F, L.list, L.cnt, L.Wb, L.W, L.G1, P, hmin, L.tol, L.chat, L.sc,
                                     L.cnt + 1);
@@ -745,7 +758,7 @@ int trainer_harvest(Trainer &t, const Geo &g, const fst_config &cf, const Phys &
   int rc = features(g, cf, ph, P, t.feat, s);
   if (rc) return rc;
   const G3 q = make_g3(g);
-  k_active3d<<<nblk(g.N), 256, 0, s>>>(q, t.feat.M, t.feat.tau, stride, t.feat.list, t.feat.cnt, nullptr);
+  k_active3d<<<grid3(g), dim3(64, 4), 0, s>>>(q, t.feat.M, t.feat.tau, stride, t.feat.list, t.feat.cnt, nullptr);
   unsigned n = 0;
   if (hipMemcpyAsync(&n, t.feat.cnt, sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)

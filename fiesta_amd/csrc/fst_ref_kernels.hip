@@ -266,12 +266,9 @@ __global__ void k_rhograd_max3d(Geo g, Phys ph, Range r, const double *__restric
     ms = a + sqrt(var[c] * var[c] + var[c + N] * var[c + N] + var[c + 2 * N] * var[c + 2 * N]) / rho[c];
     mc = fabs(var[c + N * (g.nv + 1)]);
   }
-  ms = wave_max(ms);
-  mc = wave_max(mc);
-  if ((threadIdx.x & 63) == 0) {
-    atomic_max_nonneg(&sc->maxS, ms);
-    atomic_max_nonneg(&sc->maxCh, mc);
-  }
+  double v2[2] = {ms, mc};
+  unsigned long long *const d2[2] = {&sc->maxS, &sc->maxCh};
+  block_max_atomic<2>(v2, d2);
 }
 
 // updateCeq, src/ceq3d.hpp:189-224 (maxS from the device reduction)
@@ -579,14 +576,9 @@ __global__ void k_rhograd_max2d(Geo g, Phys ph, Range r, const double *__restric
     mc = fabs(var[c + N * g.nv]);
     mch = fabs(var[c + N * (g.nv + 1)]);
   }
-  ms = wave_max(ms);
-  mc = wave_max(mc);
-  mch = wave_max(mch);
-  if ((threadIdx.x & 63) == 0) {
-    atomic_max_nonneg(&sc->maxS, ms);
-    atomic_max_nonneg(&sc->maxC, mc);
-    atomic_max_nonneg(&sc->maxCh, mch);
-  }
+  double v3[3] = {ms, mc, mch};
+  unsigned long long *const d3[3] = {&sc->maxS, &sc->maxC, &sc->maxCh};
+  block_max_atomic<3>(v3, d3);
 }
 
 // updateCeq2D, src/ceq.hpp:153-195

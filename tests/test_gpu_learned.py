@@ -186,11 +186,14 @@ def test_training_lowers_loss():
         tr.harvest(m, stride=8)
     n = tr.rows()
     assert n > 1000
-    X, T = tr.read_rows(0, 64)
+    X, T = tr.read_rows(0, n)
     assert np.all(np.isfinite(X)) and np.all(T >= -1e-3 * max(T.max(), 1e-30))
+    _, _, before = tr.forward_backward(X, T)
     loss = tr.train(epochs=20, batch=256, lr=1e-3, seed=3)
     assert np.all(np.isfinite(loss))
-    assert loss[-1] < 0.5 * loss[0], loss
+    _, _, after = tr.forward_backward(X, T)
+    print("dataset loss before %.4g after %.4g (target variance %.4g)" % (before, after, T.var()))
+    assert after < 0.3 * before and after < 0.3 * T.var(), (before, after, T.var())
     m.close()
     tr.close()
 

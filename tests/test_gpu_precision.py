@@ -60,3 +60,45 @@ def test_fp32_form_error_against_fp64_oracle(name, kw):
         l2, linf = rel_errors(a, b)
         print("  %s: rel L2 %.3e  rel Linf %.3e" % (label, l2, linf), flush=True)
         assert l2 < 1e-4, (label, l2)
+
+
+@pytest.mark.parametrize("name,kw,learned,prec", [
+    ("rt_like_plain_fp32", dict(), False, FST_FP32_BULK),
+    ("rt_like_plain_fp64", dict(), False, 0),
+    ("walls_ns3_plain_fp32", dict(ns=3, per=(False, False, False),
+                                  bc=("outflow", "reflective", "hydrostatic", "noslip", "reflective", "outflow")),
+     False, FST_FP32_BULK),
+    ("walls_ns3_plain_fp64", dict(ns=3, per=(False, False, False),
+                                  bc=("outflow", "reflective", "hydrostatic", "noslip", "reflective", "outflow")),
+     False, 0),
+    ("ragged_learned_fp32", dict(nci=70, ncj=19, nck=13), True, FST_FP32_BULK),
+    ("ragged_learned_fp64", dict(nci=70, ncj=19, nck=13), True, 0)])
+def test_ring_kernel_equals_global_kernel(name, kw, learned, prec, monkeypatch):
+    """The viscous/C-equation kernel that stages its face fields in an LDS
+    ring gives the same bits as the one that reads them from memory."""
+    from fiesta_amd.runtime import Trainer
+    base = dict(nci=40, ncj=21, nck=18)
+    base.update(kw)
+    cfg = cfg3d(**base)
+    cfg.precision = prec
+    if learned:
+        cfg.learned_c = 1
+    print("case ring vs global", name, flush=True)
+    d = Dims(cfg)
+    var = W.random_state(cfg, 77)
+    w = Trainer(seed=2).weights() if learned else None
+    outs = []
+    for glob in (False, True):
+        if glob:
+            monkeypatch.setenv("FST_VCR_GLOBAL", "1")
+        else:
+            monkeypatch.delenv("FST_VCR_GLOBAL", raising=False)
+        m = Module(cfg)
+        m.upload(FST_VAR, var[:d.nvt])
+        if learned:
+            m.set_mlp_weights(w)
+        m.pre_sim()
+        m.step(3)
+        outs.append(m.download(FST_VAR))
+        m.close()
+    assert np.array_equal(outs[0], outs[1])
