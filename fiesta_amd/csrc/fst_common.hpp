@@ -46,6 +46,15 @@ __host__ __device__ __forceinline__ double as_double(unsigned long long b) {
   return __builtin_bit_cast(double, b);
 }
 
+// a/b for a run constant b with r = RN(1/b) precomputed: q0 = a*r,
+// e = fma(-b, q0, a), q = fma(e, r, q0) is the correctly rounded quotient
+// (Markstein's theorem), so it equals IEEE a/b bit for bit.
+__host__ __device__ __forceinline__ double cdiv_rn(double a, double b, double rb) {
+  const double q0 = a * rb;
+  const double e = fma(-b, q0, a);
+  return fma(e, rb, q0);
+}
+
 // Wave-wide max over 64 lanes.
 __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));

@@ -31,6 +31,7 @@
 // Addressing: element offsets are 32-bit (checked on the host: every array
 // stays below 2^29 elements) on uniform base pointers.
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstdlib>
 
@@ -70,6 +71,23 @@ __device__ __forceinline__ double ld(const double *__restrict__ base, uint32_t o
 }
 __device__ __forceinline__ void st(double *__restrict__ base, uint32_t off, double v) {
   *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + (off << 3)) = v;
+}
+
+// Partial right-hand side handed from the advection kernel to the viscous
+// kernel: FP64 in the FP64 form, FP32 in the FP32 forms (the pressure-
+// gradient / buoyancy cancellation happens in FP64 before the store; the RK
+// accumulation that adds it stays FP64).
+template <bool F32>
+using DvT = typename std::conditional<F32, float, double>::type;
+template <bool F32>
+__device__ __forceinline__ void st_dv(double *__restrict__ base, uint32_t off, double v) {
+  if (F32) reinterpret_cast<float *>(base)[off] = (float)v;
+  else st(base, off, v);
+}
+template <bool F32>
+__device__ __forceinline__ double ld_dv(const double *__restrict__ base, uint32_t off) {
+  if (F32) return (double)reinterpret_cast<const float *>(base)[off];
+  return ld(base, off);
 }
 
 __device__ __forceinline__ double cdiv(double a, double b, double rb) {
@@ -413,7 +431,176 @@ __global__ void __launch_bounds__(NTHREADS, 1)
         dv[3] += vv * f;
       }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) st(dvar, c + N * v, dv[v]);
+      for (int v = 0; v < NV; ++v) st_dv<F32>(dvar, c + N * v, dv[v]);
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) Zp[q] = Z[q];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- kernel 1 with an LDS plane
+
+// k_adv3d with the current plane staged in LDS: for rows j0-3..j0+TY+2 and
+// columns i0-3..i0+TX+2 of the tile, q (var(v), plus p for the energy), u, v
+// and p as doubles.  The x and y faces and the pressure gradient read it; the
+// z stencil stays in the per-column register queue.  The next plane is loaded
+// into registers during the current one.  Same values, same operations as
+// k_adv3d.
+constexpr int QY = TY + 6, QX = TX + 6, QPL = QY * QX;
+
+template <int NS>
+struct PlaneF {
+  static constexpr int NV = 4 + NS;
+  static constexpr int NQ = NV + 3;  // q_0..q_{nv-1}, u, v, p
+  static constexpr int FU = NV, FV = NV + 1, FP = NV + 2;
+  static constexpr int NPRE = (NQ * QPL + NTHREADS - 1) / NTHREADS;
+};
+
+template <int NS>
+__device__ __forceinline__ void plane_fetch(const Grid32 &g, const double *__restrict__ U,
+                                            const double *__restrict__ P, int i0, int j0, int p, double *pre) {
+  using PF = PlaneF<NS>;
+  const uint32_t N = g.N;
+#pragma unroll
+  for (int m = 0; m < PF::NPRE; ++m) {
+    const int e = threadIdx.x + m * NTHREADS;
+    double v = 0.0;
+    if (e < PF::NQ * QPL) {
+      const int f = e / QPL, rem = e - f * QPL;
+      const int r = rem / QX, q = rem - r * QX;
+      const int i = i0 - 3 + q, j = j0 - 3 + r;
+      if (i < g.ngi && j < g.ngj) {
+        const uint32_t c = g.id(i, j, p);
+        if (f < PF::NV) v = (f == 3) ? ld(U, c + N * f) + ld(P, c + N) : ld(U, c + N * f);
+        else if (f == PF::FU) v = ld(P, c + 2 * N);
+        else if (f == PF::FV) v = ld(P, c + 3 * N);
+        else v = ld(P, c + N);
+      }
+    }
+    pre[m] = v;
+  }
+}
+
+template <int NS>
+__device__ __forceinline__ void plane_store(double *qs, const double *pre) {
+  using PF = PlaneF<NS>;
+#pragma unroll
+  for (int m = 0; m < PF::NPRE; ++m) {
+    const int e = threadIdx.x + m * NTHREADS;
+    if (e < PF::NQ * QPL) qs[e] = pre[m];
+  }
+}
+
+// Advective fluxes on the face between the cell at plane row r, column q and
+// its +e_D neighbour (D = 0: x, 1: y), stencil read from the LDS plane.
+template <int D, int NS, bool F32>
+__device__ __forceinline__ void face_adv_lds(const FC &k, const double *qs, int r, int q, double *out) {
+  using PF = PlaneF<NS>;
+  constexpr int NV = PF::NV;
+  constexpr int sr = D == 1 ? QX : 0, sq = D == 0 ? 1 : 0;  // offset of +e_D in the plane
+  const int base = r * QX + q;
+  constexpr int fv = D == 0 ? PF::FU : PF::FV;
+  const double vd[4] = {qs[fv * QPL + base - (sr + sq)], qs[fv * QPL + base], qs[fv * QPL + base + (sr + sq)],
+                        qs[fv * QPL + base + 2 * (sr + sq)]};
+  double qq[6][NV];
+#pragma unroll
+  for (int t = 0; t < 6; ++t)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) qq[t][v] = qs[v * QPL + base + (t - 2) * (sr + sq)];
+  flux_from_stencil<NS, F32>(k, D, vd, qq, out);
+}
+
+template <int NS, bool F32>
+__global__ void __launch_bounds__(NTHREADS, 1)
+    k_adv3d_lds(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
+                double *__restrict__ dvar, int buoy, int kchunk, int nbx, int nby, int nbz) {
+  using PF = PlaneF<NS>;
+  constexpr int NV = 4 + NS;
+  __shared__ double qs[PF::NQ * QPL];
+  __shared__ double xsh[TY][NV][TX + 1];  // index 0: left of the tile; l+1: right of lane l
+  __shared__ double ysh[NWAVE][NV][TX];   // row w: face below tile row w
+  const Tile t = place(g, kchunk, nbx, nby, nbz);
+  const int w = t.w, l = t.l;
+  const uint32_t N = g.N;
+  double qz[6][NV], vz[4];
+  double Z[NV], Zp[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) Z[q] = Zp[q] = 0.0;
+  const uint32_t c00 = g.id(t.i, t.jrow, t.kb - 1);
+  if (t.cellok) {
+#pragma unroll
+    for (int s = 0; s < 6; ++s) load_q<NS>(g, U, P, c00 + (uint32_t)(s - 2) * g.s2, qz[s]);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) vz[s] = ld(P, c00 + (uint32_t)(s - 1) * g.s2 + 4 * N);
+    flux_from_stencil<NS, F32>(k, 2, vz, qz, Zp);
+  }
+  // prefetch of the next plane when the registers allow it (ns <= 2)
+  constexpr bool PREF = NS <= 2;
+  double pre[PF::NPRE];
+  if (PREF) plane_fetch<NS>(g, U, P, t.i0, t.j0, t.kb, pre);
+  for (int kk = t.kb; kk < t.ke; ++kk) {
+    uint32_t c = g.id(t.i, t.jrow, kk);
+    asm volatile("" : "+v"(c));
+    if (!PREF) plane_fetch<NS>(g, U, P, t.i0, t.j0, kk, pre);
+    plane_store<NS>(qs, pre);
+    __syncthreads();
+    if (PREF && kk + 1 < t.ke) plane_fetch<NS>(g, U, P, t.i0, t.j0, kk + 1, pre);
+    const int rc = w + 3, qc = l + 3;  // plane coordinates of this lane's cell
+    if (t.cellok) {
+      double X[NV];
+      face_adv_lds<0, NS, F32>(k, qs, rc, qc, X);
+#pragma unroll
+      for (int q = 0; q < NV; ++q) xsh[w][q][l + 1] = X[q];
+    }
+    if (t.xbok) {  // x-face left of the tile, row l: cell (i0-1, j0+l)
+      double X[NV];
+      face_adv_lds<0, NS, F32>(k, qs, l + 3, 2, X);
+#pragma unroll
+      for (int q = 0; q < NV; ++q) xsh[l][q][0] = X[q];
+    }
+    if (t.yfok) {  // y-face below row w: cell (i0+l, j0+w-1)
+      double Y[NV];
+      face_adv_lds<1, NS, F32>(k, qs, w + 2, qc, Y);
+#pragma unroll
+      for (int q = 0; q < NV; ++q) ysh[w][q][l] = Y[q];
+    }
+    if (t.cellok) {
+#pragma unroll
+      for (int s = 0; s < 5; ++s)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qz[s][v] = qz[s + 1][v];
+      load_q<NS>(g, U, P, c + 3 * g.s2, qz[5]);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) vz[s] = vz[s + 1];
+      vz[3] = ld(P, c + 2 * g.s2 + 4 * N);
+      flux_from_stencil<NS, F32>(k, 2, vz, qz, Z);
+    }
+    __syncthreads();
+    if (t.cellok) {
+      double dv[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        dv[v] = -((xsh[w][v][l + 1] - xsh[w][v][l]) + (ysh[w + 1][v][l] - ysh[w][v][l]) + (Z[v] - Zp[v]));
+      const double *pp = qs + PF::FP * QPL + rc * QX + qc;
+      const uint32_t pb = N, sz = g.s2;
+      const double dxp = cdiv(pp[-2] - 8.0 * pp[-1] + 8.0 * pp[1] - pp[2], k.pg[0], k.rpg[0]);
+      const double dyp = cdiv(pp[-2 * QX] - 8.0 * pp[-QX] + 8.0 * pp[QX] - pp[2 * QX], k.pg[1], k.rpg[1]);
+      const double dzp = cdiv(ld(P, c - 2 * sz + pb) - 8.0 * ld(P, c - sz + pb) + 8.0 * ld(P, c + sz + pb) -
+                                  ld(P, c + 2 * sz + pb), k.pg[2], k.rpg[2]);
+      dv[0] = dv[0] - dxp;
+      dv[1] = dv[1] - dyp;
+      dv[2] = dv[2] - dzp;
+      if (buoy) {
+        const double rho = ld(P, c);
+        const double vv = qs[1 * QPL + rc * QX + qc] / rho;
+        const double rhop = rho - k.rhoRef;
+        const double f = -k.g * rhop;
+        dv[1] += f;
+        dv[3] += vv * f;
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) st_dv<F32>(dvar, c + N * v, dv[v]);
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) Zp[q] = Z[q];
@@ -483,7 +670,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     if (t.cellok) {
       double dv[NVT];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) dv[v] = ld(dvar, c + N * v);
+      for (int v = 0; v < NV; ++v) dv[v] = ld_dv<F32>(dvar, c + N * v);
       auto PV = [&](uint32_t off) { return (R)ld(P, off); };
       if (visc) {
 #pragma unroll
@@ -837,6 +1024,17 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     asm volatile("" : "+v"(c), "+v"(cxb));
     const bool fetch = kk + 2 <= t.ke;  // plane kk+2 is read by iteration kk+1
     if (fetch) ring_fetch<NS, F32, LC>(g, U, P, CH, DR, t.i0, t.j0, kk + 2, pre);
+    // learned FP32 form (registers to spare): the cell's RHS and stage base
+    // are loaded before the face phase so their latency overlaps it
+    constexpr bool EARLY = F32 && LC;
+    double dvp[EARLY ? NV : 1], bsp[EARLY ? NVT : 1];
+    if (EARLY && t.cellok) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) dvp[v] = ld_dv<F32>(dvar, c + N * v);
+#pragma unroll
+      for (int v = 0; v < NVT; ++v)
+        if (v < g.nvt) bsp[v] = (stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v);
+    }
     at.sl[0] = (kk - 1) & 3; at.sl[1] = kk & 3; at.sl[2] = (kk + 1) & 3;
     if (faces) {
       if (t.cellok) {
@@ -871,7 +1069,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       at.r = w + 1; at.q = l + 1; at.c = c;
       double dv[NVT];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) dv[v] = ld(dvar, c + N * v);
+      for (int v = 0; v < NV; ++v) dv[v] = EARLY ? dvp[EARLY ? v : 0] : ld_dv<F32>(dvar, c + N * v);
       auto PV = [&](uint32_t off) { return (R)ld(P, off); };
       if (visc) {
 #pragma unroll
@@ -970,7 +1168,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
 #pragma unroll
       for (int v = 0; v < NVT; ++v) {
         if (v < nvt) {
-          const double base = (stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v);
+          const double base = EARLY ? bsp[EARLY ? v : 0] : ((stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v));
           un[v] = base + h * dv[v];
           st(Uo, c + N * v, un[v]);
         }
@@ -1134,7 +1332,10 @@ void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf,
                const float *DR, const Scalars *scin, Scalars *scout, double h, int stage, int kchunk,
                hipStream_t s) {
   const Launch L = shape(g, kchunk);
-  k_adv3d<NS, F32><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
+  if (!getenv("FST_ADV_GLOBAL"))
+    k_adv3d_lds<NS, F32><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
+  else
+    k_adv3d<NS, F32><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
   if (!getenv("FST_VCR_GLOBAL"))
     k_vcr3d_ring<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h,
                                                        stage, cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);

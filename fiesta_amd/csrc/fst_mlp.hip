@@ -111,16 +111,16 @@ __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst 
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         const double *V = P + (2 + a) * N;
-        const double h2 = 2.0 * fc.d[a];
-        const double r1 = (R[t + s[a]] - R[t - s[a]]) / h2;
-        const double p1 = (Pp[t + s[a]] - Pp[t - s[a]]) / h2;
+        const double h2 = fc.h2[a], rh2 = fc.rh2[a];  // divisions by run constants: cdiv_rn == IEEE
+        const double r1 = cdiv_rn(R[t + s[a]] - R[t - s[a]], h2, rh2);
+        const double p1 = cdiv_rn(Pp[t + s[a]] - Pp[t - s[a]], h2, rh2);
         gr2 = gr2 + r1 * r1;
         gp2 = gp2 + p1 * p1;
-        divu = divu + (V[t + s[a]] - V[t - s[a]]) / h2;
+        divu = divu + cdiv_rn(V[t + s[a]] - V[t - s[a]], h2, rh2);
         if (in2) {
-          const double h12 = 12.0 * fc.d[a];
-          dr[a] = (R[t - 2 * s[a]] - 8.0 * R[t - s[a]] + 8.0 * R[t + s[a]] - R[t + 2 * s[a]]) / h12;
-          de[a] = (E[t - 2 * s[a]] - 8.0 * E[t - s[a]] + 8.0 * E[t + s[a]] - E[t + 2 * s[a]]) / h12;
+          const double h12 = fc.h12[a], rh12 = fc.rh12[a];
+          dr[a] = cdiv_rn(R[t - 2 * s[a]] - 8.0 * R[t - s[a]] + 8.0 * R[t + s[a]] - R[t + 2 * s[a]], h12, rh12);
+          de[a] = cdiv_rn(E[t - 2 * s[a]] - 8.0 * E[t - s[a]] + 8.0 * E[t + s[a]] - E[t + 2 * s[a]], h12, rh12);
         }
       }
       const double rho = R[t], p = Pp[t];
@@ -541,6 +541,12 @@ FeatConst make_feat(const fst_config &cf, const Phys &ph) {
   fc.d[0] = cf.dx;
   fc.d[1] = cf.dy;
   fc.d[2] = cf.dz;
+  for (int a = 0; a < 3; ++a) {
+    fc.h2[a] = 2.0 * fc.d[a];
+    fc.rh2[a] = 1.0 / fc.h2[a];
+    fc.h12[a] = 12.0 * fc.d[a];
+    fc.rh12[a] = 1.0 / fc.h12[a];
+  }
   fc.hmin = std::min(cf.dx, std::min(cf.dy, cf.dz));
   for (int q = 0; q < FST_MAX_SPECIES; ++q) {
     fc.cpf[q] = ph.cpf[q];

@@ -28,6 +28,7 @@ constexpr int MLP_NPARAM = MLP_OFF_B4 + 1;
 // Constants of the feature pass.
 struct FeatConst {
   double d[3];     // dx, dy, dz
+  double h2[3], rh2[3], h12[3], rh12[3];  // 2d, 12d and their RN reciprocals
   double hmin;     // min(dx, dy, dz): length scale of the dimensionless features
   double cpf[FST_MAX_SPECIES], cvf[FST_MAX_SPECIES];
   int ns;

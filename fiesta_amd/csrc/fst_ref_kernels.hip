@@ -680,6 +680,23 @@ __global__ void k_bc_periodic(Geo g, int dir, double *__restrict__ var) {
   }
 }
 
+// bc_xPer (src/bc.cpp:133-146) with 8 lanes per (j, k, v) row: lanes 0..ng-1
+// fill the low ghosts i = r from i = nci + r, lanes ng..2ng-1 the high ghosts
+// i = nci + ng + r from i = ng + r (sources and targets never overlap: a
+// periodic direction has at least ng interior cells), so a wave touches 8
+// rows with adjacent addresses.
+__global__ void k_bc_xper(Geo g, double *__restrict__ var) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long row = t >> 3;
+  const int r = (int)(t & 7);
+  const long long nrow = (long long)g.ngj * g.ngk * g.nvt;
+  const int ng = g.ng;
+  if (row >= nrow || r >= 2 * ng) return;
+  double *u = var + row * g.ngi;  // rows of one variable plane are contiguous, planes too
+  if (r < ng) u[r] = u[g.nci + r];
+  else u[g.nci + r] = u[r];  // i = nci + ng + (r - ng) from i = ng + (r - ng)
+}
+
 // bc_gen, src/bc.cpp:33-131, on one boundary plane (i0 / j0 / k0 fixed)
 __global__ void k_bc_gen(Geo g, int type, int ih, int jh, int kh, int plane, double *__restrict__ var) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -832,8 +849,13 @@ void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, cons
 void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream_t s) {
   const int bs = 256;
   if (cf.xPer) {
-    long long n = (long long)g.ngj * g.ngk * g.nvt;
-    k_bc_periodic<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, 0, var);
+    if (2 * g.ng <= 8) {
+      long long n = 8LL * g.ngj * g.ngk * g.nvt;
+      k_bc_xper<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, var);
+    } else {
+      long long n = (long long)g.ngj * g.ngk * g.nvt;
+      k_bc_periodic<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, 0, var);
+    }
   }
   if (cf.yPer) {
     long long n = (long long)g.ngi * g.ngk * g.nvt;

@@ -74,8 +74,9 @@ def test_fp32_form_error_against_fp64_oracle(name, kw):
     ("ragged_learned_fp32", dict(nci=70, ncj=19, nck=13), True, FST_FP32_BULK),
     ("ragged_learned_fp64", dict(nci=70, ncj=19, nck=13), True, 0)])
 def test_ring_kernel_equals_global_kernel(name, kw, learned, prec, monkeypatch):
-    """The viscous/C-equation kernel that stages its face fields in an LDS
-    ring gives the same bits as the one that reads them from memory."""
+    """The stage kernels that stage their stencils in LDS (the advection
+    plane, the viscous/C-equation ring) give the same bits as the kernels
+    that read every stencil value from memory."""
     from fiesta_amd.runtime import Trainer
     base = dict(nci=40, ncj=21, nck=18)
     base.update(kw)
@@ -89,10 +90,11 @@ def test_ring_kernel_equals_global_kernel(name, kw, learned, prec, monkeypatch):
     w = Trainer(seed=2).weights() if learned else None
     outs = []
     for glob in (False, True):
-        if glob:
-            monkeypatch.setenv("FST_VCR_GLOBAL", "1")
-        else:
-            monkeypatch.delenv("FST_VCR_GLOBAL", raising=False)
+        for var_name in ("FST_VCR_GLOBAL", "FST_ADV_GLOBAL"):
+            if glob:
+                monkeypatch.setenv(var_name, "1")
+            else:
+                monkeypatch.delenv(var_name, raising=False)
         m = Module(cfg)
         m.upload(FST_VAR, var[:d.nvt])
         if learned:
