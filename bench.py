@@ -127,12 +127,13 @@ def gpu_leg(args, form="fp64", var=None, keep_state=False, weights=None):
     m.sync()
     wall = time.perf_counter() - t0
     launches, kms, kname = m.profile_read()
+    phases = m.profile_phases()
     m.profile(False)
     ev_ms = m.last_step_ms()
     cell_stages = d.ncells * 2 * args.steps
     value = cell_stages / wall / 1e9
     res = dict(cfg=cfg, dims=d, wall=wall, ev_ms=ev_ms, value=value, launches=launches, kernel_ms=kms,
-               kernel=kname, fused=fused_ok and args.path == "fused")
+               kernel=kname, fused=fused_ok and args.path == "fused", phases=phases)
     if learned:
         a, nfb, mch = m.learned_stats()
         res["learned"] = {"active_cells": int(a), "fallback_cells": int(nfb), "max_chat": float("%.5g" % mch)}
@@ -165,6 +166,22 @@ def cpu_leg(args, cfg_size):
     return dict(value=cs / wall / 1e9, unit="Gcell-stage/s", cores=threads, kind="port",
                 sample="%d RK2 steps (after 1 warm-up) of the %d^3 workload with the oracle "
                        "(C++/OpenMP restatement of the reference kernels), %d threads" % (args.cpu_steps, cfg_size, threads))
+
+
+def measured_traffic(size, form):
+    """HBM bytes per RK stage of a form, from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a copy kernel
+    of known size); None when no summary for this size and form exists."""
+    path = os.path.join(ROOT, "profiles", "traffic_rt3d_%d.json" % size)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    e = d.get("forms", {}).get(form)
+    if not e:
+        return None
+    return {"bytes_per_stage": e["bytes_per_stage"], "source": "profiles/%s (%s)" % (os.path.basename(path),
+                                                                                    d.get("command", ""))}
 
 
 def form_errors(cfg, got, want, cfg64):
@@ -241,6 +258,18 @@ def main():
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": r["kernel"],
                 "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes": "%d B per cell-stage x %d cells per stage" % (balg, d.ncells)}
+        ph = r.get("phases") or []
+        if len(ph) == 4:
+            names = ("ghost_fill_and_ghost_primitives", "learned_c_fields", "advection_kernel", "viscous_ceq_rk_kernel")
+            roof["phases_ms"] = {n: round(v / r["launches"], 4) for n, v in zip(names, ph)}
+            ns = r["cfg"].ns
+            roof["dominant_kernel"] = {"name": "k_vcr3d_ring<%d, false, false>" % ns,
+                                       "avg_ms": round(ph[3] / r["launches"], 4),
+                                       "share_of_stage": round(ph[3] / r["kernel_ms"], 3)}
+        tr = measured_traffic(args.size, "fp64")
+        if tr:
+            roof["traffic"] = tr["bytes_per_stage"]
+            roof["traffic_source"] = tr["source"]
     else:
         ach = d.ncells * balg * r["value"] * 1e9 / d.ncells / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -279,6 +308,11 @@ def main():
             e["error_vs_fp64"] = form_errors(q["cfg"], q["state"], ref, r["cfg"])
         if "learned" in q:
             e["learned"] = q["learned"]
+        if len(q.get("phases") or []) == 4 and q["launches"]:
+            e["phases_ms"] = [round(v / q["launches"], 4) for v in q["phases"]]
+        tr = measured_traffic(args.size, f)
+        if tr:
+            e["traffic_bytes_per_stage"] = tr["bytes_per_stage"]
         fm[f] = e
     out["forms"] = fm
     if train_info:

@@ -26,6 +26,8 @@ struct fst_ctx {
   double last_ms = 0.0;
   int device = 0;
   Prof prof;
+  double phase_ms[Prof::MAXPER] = {0};  // per-phase sums of the last fst_profile_read
+  int nphase = 0;
 };
 
 namespace {
@@ -374,19 +376,33 @@ int fst_profile_enable(fst_ctx *ctx, int enable) {
 int fst_profile_read(fst_ctx *ctx, int64_t *launches, double *total_ms, char *name, size_t name_len) {
   if (!ctx || !launches || !total_ms) return FST_ERR_ARG;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const size_t per = (size_t)ctx->prof.per;
   double tot = 0.0;
-  for (size_t q = 0; q + 1 < ctx->prof.used; q += 2) {
+  ctx->nphase = (int)per - 1;
+  for (int q = 0; q < Prof::MAXPER; ++q) ctx->phase_ms[q] = 0.0;
+  for (size_t q = 0; q + per <= ctx->prof.used; q += per) {
     float f = 0.f;
-    HIPCHK(ctx, hipEventElapsedTime(&f, ctx->prof.ev[q], ctx->prof.ev[q + 1]));
+    HIPCHK(ctx, hipEventElapsedTime(&f, ctx->prof.ev[q], ctx->prof.ev[q + per - 1]));
     tot += f;
+    for (size_t ph = 0; ph + 1 < per; ++ph) {
+      HIPCHK(ctx, hipEventElapsedTime(&f, ctx->prof.ev[q + ph], ctx->prof.ev[q + ph + 1]));
+      ctx->phase_ms[ph] += f;
+    }
   }
-  *launches = (int64_t)(ctx->prof.used / 2);
+  *launches = (int64_t)(ctx->prof.used / per);
   *total_ms = tot;
   if (name && name_len) {
     std::strncpy(name, ctx->prof.name, name_len - 1);
     name[name_len - 1] = 0;
   }
   ctx->prof.used = 0;
+  return FST_OK;
+}
+
+int fst_profile_phases(fst_ctx *ctx, double *ms, int n, int *nphase) {
+  if (!ctx || !ms || !nphase || n < 0) return FST_ERR_ARG;
+  *nphase = ctx->nphase;
+  for (int q = 0; q < n && q < ctx->nphase; ++q) ms[q] = ctx->phase_ms[q];
   return FST_OK;
 }
 

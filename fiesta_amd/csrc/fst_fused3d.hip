@@ -1330,12 +1330,13 @@ template <int NS, bool F32, bool LC>
 void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf, const double *U,
                const double *Un, double *Uo, const double *P, double *Po, double *dvar, const float *CH,
                const float *DR, const Scalars *scin, Scalars *scout, double h, int stage, int kchunk,
-               hipStream_t s) {
+               Prof *prof, hipStream_t s) {
   const Launch L = shape(g, kchunk);
   if (!getenv("FST_ADV_GLOBAL"))
     k_adv3d_lds<NS, F32><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
   else
     k_adv3d<NS, F32><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
+  if (prof) prof->mark(s);  // end of the advection kernel
   if (!getenv("FST_VCR_GLOBAL"))
     k_vcr3d_ring<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h,
                                                        stage, cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
@@ -1355,7 +1356,7 @@ void run_prims(const Grid32 &q, const FC &k, const double *U, double *P, Scalars
 
 using StageFn = void (*)(const Grid32 &, const Geo &, const FC &, const fst_config &, const double *,
                          const double *, double *, const double *, double *, double *, const float *,
-                         const float *, const Scalars *, Scalars *, double, int, int, hipStream_t);
+                         const float *, const Scalars *, Scalars *, double, int, int, Prof *, hipStream_t);
 using PrimFn = void (*)(const Grid32 &, const FC &, const double *, double *, Scalars *, int, Scalars *,
                         hipStream_t);
 
@@ -1418,9 +1419,11 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
   double *Pa = fb.prim, *Pb = fb.prim + (size_t)g.N * np;
   Scalars *S0 = fb.sc, *S1 = fb.sc + 1;
   Prof *prof = b.prof && b.prof->on ? b.prof : nullptr;
-  if (prof)
+  if (prof) {
     prof->name = lc ? "stage: ghost fill + ghost primitives + [learned C fields, stage 1] + k_adv3d + k_vcr3d"
                     : "stage: ghost fill + ghost primitives + k_adv3d + k_vcr3d";
+    prof->per = 5;  // begin | ghosts | learned fields | advection | viscous + RK
+  }
   const float *CH = lc ? lb->chat : nullptr, *DR = lc ? lb->dir : nullptr;
   if (!fb.valid) {  // primitives and maxima of var, all cells
     k_clear<<<1, 64, 0, s>>>(S0);
@@ -1433,6 +1436,7 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
     if (prof) prof->begin(s);
     launch_apply_bcs(g, cf, b.var, s);       // src/cart3d.cpp:221
     pf(q, k, b.var, Pa, nullptr, 1, S1, s);  // ghost primitives; clears S1
+    if (prof) prof->mark(s);
     if (lc) {  // C_hat and n of U^n, used by both stages of the step
       const int rc = learned_fields(g, cf, ph, Pa, *lb, s);
       if (rc) {
@@ -1440,13 +1444,18 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
         return rc;
       }
     }
-    sf(q, g, k, cf, b.var, nullptr, fb.stage, Pa, Pb, b.dvar, CH, DR, lc ? lb->sc : S0, S1, h1, 1, kchunk, s);
+    if (prof) prof->mark(s);
+    sf(q, g, k, cf, b.var, nullptr, fb.stage, Pa, Pb, b.dvar, CH, DR, lc ? lb->sc : S0, S1, h1, 1, kchunk, prof, s);
     if (prof) prof->end(s);
     // stage 2: U* -> U^{n+1}, written in place into b.var (read pointwise)
     if (prof) prof->begin(s);
     launch_apply_bcs(g, cf, fb.stage, s);
     pf(q, k, fb.stage, Pb, nullptr, 1, S0, s);  // ghost primitives of U*; clears S0
-    sf(q, g, k, cf, fb.stage, b.var, b.var, Pb, Pa, b.dvar, CH, DR, lc ? lb->sc : S1, S0, h2, 2, kchunk, s);
+    if (prof) {
+      prof->mark(s);
+      prof->mark(s);  // no learned fields in stage 2
+    }
+    sf(q, g, k, cf, fb.stage, b.var, b.var, Pb, Pa, b.dvar, CH, DR, lc ? lb->sc : S1, S0, h2, 2, kchunk, prof, s);
     if (prof) prof->end(s);
   }
   return FST_OK;

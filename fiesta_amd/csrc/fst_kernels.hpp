@@ -8,24 +8,42 @@
 namespace fst {
 
 // Event-bracketed timing of one kernel family on a stream.
+// Live timing of the per-stage work with hipEvents on the context stream.
+// A timed unit records `per` events: begin, (per - 2) phase marks, end.
 struct Prof {
+  static constexpr int MAXPER = 8;
   bool on = false;
   const char *name = "";
-  std::vector<hipEvent_t> ev;  // pairs: begin, end
+  int per = 2;      // events per timed unit
+  int marked = 0;   // marks recorded in the open unit
+  std::vector<hipEvent_t> ev;
   size_t used = 0;
-  void begin(hipStream_t s) {
-    if (!on) return;
-    while (ev.size() < used + 2) {
+  bool grow(size_t n) {
+    while (ev.size() < n) {
       hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
+      if (hipEventCreate(&e) != hipSuccess) { on = false; return false; }
       ev.push_back(e);
     }
+    return true;
+  }
+  void begin(hipStream_t s) {
+    if (!on || !grow(used + per)) return;
+    marked = 0;
     (void)hipEventRecord(ev[used], s);
+  }
+  void mark(hipStream_t s) {
+    if (!on || marked + 2 >= per) return;
+    ++marked;
+    (void)hipEventRecord(ev[used + marked], s);
   }
   void end(hipStream_t s) {
     if (!on) return;
-    (void)hipEventRecord(ev[used + 1], s);
-    used += 2;
+    while (marked + 2 < per) {  // unrecorded marks collapse onto the end
+      ++marked;
+      (void)hipEventRecord(ev[used + marked], s);
+    }
+    (void)hipEventRecord(ev[used + per - 1], s);
+    used += per;
   }
   ~Prof() {
     for (auto e : ev) (void)hipEventDestroy(e);

@@ -92,68 +92,121 @@ inline dim3 grid3(const Geo &g) { return dim3((g.ngi + 63) / 64, (g.ngj + 3) / 4
 // Per cell: F (cells with one neighbour each way), dir and G1 (cells with two
 // neighbours each way), M = max |F|.  P planes: rho, p, u, v, w, nrg, Y_s.
 // Every value is computed in FP64 in a fixed order and rounded once to FP32.
-// Launch: block (64, 4), grid (ceil(ngi/64), ceil(ngj/4), ngk).
+// Launch: block (64, 4) over (i, j); each thread marches its (i, j) column
+// through FEAT_KCH planes, keeping the z stencils (rho and nrg at k-2..k+2,
+// p and w at k-1..k+1) in registers, so each plane is read once for z.
+constexpr int FEAT_KCH = 16;
+
 __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst fc, const double *__restrict__ P,
                                                     float *__restrict__ F, float *__restrict__ M,
                                                     float *__restrict__ Dir, float *__restrict__ G1, Scalars *sc) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  const long long t = (long long)i + g.s1 * j + g.s2 * k;
-  float g1max = 0.f;
-  if (i < g.ngi && j < g.ngj) {
-    const long long N = g.N;
-    const bool in1 = i >= 1 && j >= 1 && k >= 1 && i < g.ngi - 1 && j < g.ngj - 1 && k < g.ngk - 1;
-    const bool in2 = i >= 2 && j >= 2 && k >= 2 && i < g.ngi - 2 && j < g.ngj - 2 && k < g.ngk - 2;
-    float f0 = 0.f, f1 = 0.f, f2 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f, gg = 0.f;
-    if (in1) {
-      const long long s[3] = {1, g.s1, g.s2};
-      const double *R = P, *Pp = P + N, *E = P + 5 * N;
-      double gr2 = 0.0, gp2 = 0.0, divu = 0.0, dr[3] = {0, 0, 0}, de[3] = {0, 0, 0};
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const int k0 = blockIdx.z * FEAT_KCH, k1 = min(k0 + FEAT_KCH, g.ngk);
+  const bool col = i < g.ngi && j < g.ngj;
+  const long long N = g.N;
+  const double *R = P, *Pp = P + N, *W = P + 4 * N, *E = P + 5 * N;
+  const long long cij = (long long)i + g.s1 * j;
+  auto at = [&](const double *A, int k) { return (k >= 0 && k < g.ngk) ? A[cij + g.s2 * k] : 0.0; };
+  double rq[5], eq[5], pq[3], wq[3];  // rho, nrg at k-2..k+2; p, w at k-1..k+1
+  if (col) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const double *V = P + (2 + a) * N;
-        const double h2 = fc.h2[a], rh2 = fc.rh2[a];  // divisions by run constants: cdiv_rn == IEEE
-        const double r1 = cdiv_rn(R[t + s[a]] - R[t - s[a]], h2, rh2);
-        const double p1 = cdiv_rn(Pp[t + s[a]] - Pp[t - s[a]], h2, rh2);
-        gr2 = gr2 + r1 * r1;
-        gp2 = gp2 + p1 * p1;
-        divu = divu + cdiv_rn(V[t + s[a]] - V[t - s[a]], h2, rh2);
+    for (int q = 0; q < 5; ++q) {
+      rq[q] = at(R, k0 - 2 + q);
+      eq[q] = at(E, k0 - 2 + q);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      pq[q] = at(Pp, k0 - 1 + q);
+      wq[q] = at(W, k0 - 1 + q);
+    }
+  }
+  float g1max = 0.f;
+  for (int k = k0; k < k1; ++k) {
+    if (col) {
+      const long long t = cij + g.s2 * k;
+      const bool in1 = i >= 1 && j >= 1 && k >= 1 && i < g.ngi - 1 && j < g.ngj - 1 && k < g.ngk - 1;
+      const bool in2 = i >= 2 && j >= 2 && k >= 2 && i < g.ngi - 2 && j < g.ngj - 2 && k < g.ngk - 2;
+      float f0 = 0.f, f1 = 0.f, f2 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f, gg = 0.f;
+      if (in1) {
+        const long long s[2] = {1, g.s1};
+        double gr2 = 0.0, gp2 = 0.0, divu = 0.0, dr[3] = {0, 0, 0}, de[3] = {0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const double h2 = fc.h2[a], rh2 = fc.rh2[a];  // divisions by run constants: cdiv_rn == IEEE
+          double rp, rm, pp, pm, vp, vm;
+          if (a < 2) {
+            const double *V = P + (2 + a) * N;
+            rp = R[t + s[a]]; rm = R[t - s[a]];
+            pp = Pp[t + s[a]]; pm = Pp[t - s[a]];
+            vp = V[t + s[a]]; vm = V[t - s[a]];
+          } else {
+            rp = rq[3]; rm = rq[1];
+            pp = pq[2]; pm = pq[0];
+            vp = wq[2]; vm = wq[0];
+          }
+          const double r1 = cdiv_rn(rp - rm, h2, rh2);
+          const double p1 = cdiv_rn(pp - pm, h2, rh2);
+          gr2 = gr2 + r1 * r1;
+          gp2 = gp2 + p1 * p1;
+          divu = divu + cdiv_rn(vp - vm, h2, rh2);
+          if (in2) {
+            const double h12 = fc.h12[a], rh12 = fc.rh12[a];
+            if (a < 2) {
+              dr[a] = cdiv_rn(R[t - 2 * s[a]] - 8.0 * rm + 8.0 * rp - R[t + 2 * s[a]], h12, rh12);
+              de[a] = cdiv_rn(E[t - 2 * s[a]] - 8.0 * E[t - s[a]] + 8.0 * E[t + s[a]] - E[t + 2 * s[a]], h12, rh12);
+            } else {
+              dr[a] = cdiv_rn(rq[0] - 8.0 * rq[1] + 8.0 * rq[3] - rq[4], h12, rh12);
+              de[a] = cdiv_rn(eq[0] - 8.0 * eq[1] + 8.0 * eq[3] - eq[4], h12, rh12);
+            }
+          }
+        }
+        const double rho = rq[2], p = pq[1];
+        double Cp = 0.0, Cv = 0.0;
+        for (int q = 0; q < fc.ns; ++q) {
+          const double Y = P[t + N * (6 + q)];
+          Cp = Cp + Y * fc.cpf[q];
+          Cv = Cv + Y * fc.cvf[q];
+        }
+        const double a_snd = sqrt((Cp / Cv) * p / rho);
+        f0 = (float)(sqrt(gr2) * fc.hmin / rho);
+        f1 = (float)(sqrt(gp2) * fc.hmin / p);
+        f2 = (float)(divu * fc.hmin / a_snd);
         if (in2) {
-          const double h12 = fc.h12[a], rh12 = fc.rh12[a];
-          dr[a] = cdiv_rn(R[t - 2 * s[a]] - 8.0 * R[t - s[a]] + 8.0 * R[t + s[a]] - R[t + 2 * s[a]], h12, rh12);
-          de[a] = cdiv_rn(E[t - 2 * s[a]] - 8.0 * E[t - s[a]] + 8.0 * E[t + s[a]] - E[t + 2 * s[a]], h12, rh12);
+          d0 = (float)dr[0];
+          d1 = (float)dr[1];
+          d2 = (float)dr[2];
+          const double rg2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+          const double dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) * rg2;
+          gg = dnednr < 0.0 ? (float)(sqrt(rg2) * fc.hmin / rho) : 0.f;
+          const bool interior = i >= g.ng && j >= g.ng && k >= g.ng && i < g.ng + g.nci && j < g.ng + g.ncj &&
+                                k < g.ng + g.nck;
+          if (interior) g1max = fmaxf(g1max, gg);
         }
       }
-      const double rho = R[t], p = Pp[t];
-      double Cp = 0.0, Cv = 0.0;
-      for (int q = 0; q < fc.ns; ++q) {
-        const double Y = P[t + N * (6 + q)];
-        Cp = Cp + Y * fc.cpf[q];
-        Cv = Cv + Y * fc.cvf[q];
+      F[t] = f0;
+      F[t + N] = f1;
+      F[t + 2 * N] = f2;
+      M[t] = fmaxf(fabsf(f0), fmaxf(fabsf(f1), fabsf(f2)));
+      Dir[t] = d0;
+      Dir[t + N] = d1;
+      Dir[t + 2 * N] = d2;
+      G1[t] = gg;
+      // advance the z stencils
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        rq[q] = rq[q + 1];
+        eq[q] = eq[q + 1];
       }
-      const double a_snd = sqrt((Cp / Cv) * p / rho);
-      f0 = (float)(sqrt(gr2) * fc.hmin / rho);
-      f1 = (float)(sqrt(gp2) * fc.hmin / p);
-      f2 = (float)(divu * fc.hmin / a_snd);
-      if (in2) {
-        d0 = (float)dr[0];
-        d1 = (float)dr[1];
-        d2 = (float)dr[2];
-        const double rg2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-        const double dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) * rg2;
-        gg = dnednr < 0.0 ? (float)(sqrt(rg2) * fc.hmin / rho) : 0.f;
-        const bool interior = i >= g.ng && j >= g.ng && k >= g.ng && i < g.ng + g.nci && j < g.ng + g.ncj &&
-                              k < g.ng + g.nck;
-        if (interior) g1max = gg;
+      rq[4] = at(R, k + 3);
+      eq[4] = at(E, k + 3);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        pq[q] = pq[q + 1];
+        wq[q] = wq[q + 1];
       }
+      pq[2] = at(Pp, k + 2);
+      wq[2] = at(W, k + 2);
     }
-    F[t] = f0;
-    F[t + N] = f1;
-    F[t + 2 * N] = f2;
-    M[t] = fmaxf(fabsf(f0), fmaxf(fabsf(f1), fabsf(f2)));
-    Dir[t] = d0;
-    Dir[t + N] = d1;
-    Dir[t + 2 * N] = d2;
-    G1[t] = gg;
   }
   // block maximum, then one atomic per block that has a non-zero value
   __shared__ float red[4];
@@ -559,7 +612,8 @@ FeatConst make_feat(const fst_config &cf, const Phys &ph) {
 int features(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, LearnedBufs &L, hipStream_t s) {
   const G3 q = make_g3(g);
   k_clear_learned<<<1, 64, 0, s>>>(L.sc, L.cnt);
-  k_features3d<<<grid3(g), dim3(64, 4), 0, s>>>(q, make_feat(cf, ph), P, L.F, L.M, L.dir, L.G1, L.sc);
+  const dim3 fg((g.ngi + 63) / 64, (g.ngj + 3) / 4, (g.ngk + FEAT_KCH - 1) / FEAT_KCH);
+  k_features3d<<<fg, dim3(64, 4), 0, s>>>(q, make_feat(cf, ph), P, L.F, L.M, L.dir, L.G1, L.sc);
   return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 

@@ -807,7 +807,7 @@ void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, cons
   Range cr = cell_range(g), fr = face_range(g);
   if (g.ndim == 3) {
     for (int v = 0; v < g.nv; ++v) {
-      if (b.prof) { b.prof->name = "k_flux3d"; b.prof->begin(s); }
+      if (b.prof) { b.prof->name = "k_flux3d"; b.prof->per = 2; b.prof->begin(s); }
       k_flux3d<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, v, b.var, b.p, b.vel, b.fx, b.fy, b.fz);
       if (b.prof) b.prof->end(s);
       k_advect3d<<<grid_for(cr), 256, 0, s>>>(g, cr, v, b.fx, b.fy, b.fz, b.dvar);

@@ -42,6 +42,7 @@ _EXPORTS = {
     "fst_profile_read": ([C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_char_p, C.c_size_t],
                          C.c_int),
     "fst_copy_probe": ([C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
+    "fst_profile_phases": ([C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "fst_mlp_nparams": ([], C.c_int),
     "fst_set_mlp_weights": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
     "fst_learned_params": ([C.c_void_p, C.c_float, C.c_float], C.c_int),
@@ -179,6 +180,13 @@ class Module:
         name = C.create_string_buffer(160)
         self._chk(lib().fst_profile_read(self.h, C.byref(n), C.byref(ms), name, 160), "fst_profile_read")
         return n.value, ms.value, name.value.decode()
+
+    def profile_phases(self):
+        """Per-phase ms sums of the units read by the last profile_read()."""
+        ms = np.zeros(8)
+        n = C.c_int()
+        self._chk(lib().fst_profile_phases(self.h, ms.ctypes.data, 8, C.byref(n)), "fst_profile_phases")
+        return list(ms[:n.value])
 
     def copy_probe(self, nbytes, reps=10):
         ms = C.c_double()

@@ -179,6 +179,11 @@ int fst_last_step_ms(fst_ctx *ctx, double *ms);
  * duration in ms and the kernel's name; it then clears the counters. */
 int fst_profile_enable(fst_ctx *ctx, int enable);
 int fst_profile_read(fst_ctx *ctx, int64_t *launches, double *total_ms, char *name, size_t name_len);
+/* Per-phase sums (ms) of the units read by the last fst_profile_read.  Fused
+ * stages have 4 phases: ghost fill + ghost primitives, learned C fields (0 when
+ * not used), advection kernel, viscous/C-equation + RK kernel; the reference
+ * sequence has 1 (the flux kernel).  Writes min(n, *nphase) values. */
+int fst_profile_phases(fst_ctx *ctx, double *ms, int n, int *nphase);
 
 /* Device-to-device copy of `bytes` bytes with a plain vectorized HIP kernel,
  * `reps` times, on the context stream; returns the mean ms per copy.  Used to
