@@ -56,6 +56,23 @@ def build_hip(verbose=False):
     return LIB
 
 
+HOST_DIR = os.path.join(PKG, "host")
+RUN_BIN = os.path.join(PKG, "fiesta_run")
+
+
+def build_host():
+    """The host rk_func layer (C++, no HIP code) and its fiesta_run driver,
+    linked against libfiesta_hip.so (rpath: the package directory)."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    srcs = [os.path.join(HOST_DIR, f) for f in ("fiesta_host.cpp", "fiesta_run.cpp")]
+    tmp = RUN_BIN + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + HOST_DIR] + srcs +
+         ["-L" + PKG, "-lfiesta_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link," + os.path.join(rocm, "lib"),
+          "-Wl,-rpath," + os.path.join(rocm, "lib"), "-o", tmp])
+    os.replace(tmp, RUN_BIN)
+    return RUN_BIN
+
+
 def build_oracle():
     d = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-B", "-C", d, "liborc.so"])
@@ -64,8 +81,9 @@ def build_oracle():
 
 def build_all(verbose=False):
     lib = build_hip(verbose)
+    run = build_host()
     orc = build_oracle()
-    return lib, orc
+    return lib, run, orc
 
 
 if __name__ == "__main__":

@@ -1,0 +1,116 @@
+// Host-side rk_func interface over the C-ABI; see fiesta_host.hpp.
+#include "fiesta_host.hpp"
+
+#include <string>
+
+namespace fiesta_hip {
+
+rk_func::rk_func(const fst_config &cf) : cf_(cf) {
+  check(fst_config_dims(&cf_, &d_), "fst_config_dims");
+  const int rc = fst_create(&cf_, &ctx_);
+  if (rc != FST_OK) throw Error("fst_create failed (invalid configuration or no device memory)", rc);
+  var.resize(d_.ngi, d_.ngj, d_.ngk, d_.nvt);
+  dvar.resize(d_.ngi, d_.ngj, d_.ngk, d_.nvt);
+  tmp1.resize(d_.ngi, d_.ngj, d_.ngk, d_.nvt);
+  varx.resize(d_.ngi, d_.ngj, d_.ngk, d_.nvx);
+}
+
+rk_func::~rk_func() {
+  if (ctx_) fst_destroy(ctx_);
+}
+
+void rk_func::check(int rc, const char *what) const {
+  if (rc != FST_OK) {
+    std::string msg = std::string(what) + " failed";
+    if (ctx_) msg += std::string(": ") + fst_last_error(ctx_);
+    throw Error(msg, rc);
+  }
+}
+
+HostArray *rk_func::mirror(int array) {
+  switch (array) {
+    case FST_VAR: return &var;
+    case FST_VARX: return &varx;
+    case FST_DVAR: return &dvar;
+    case FST_TMP1: return &tmp1;
+    default: return nullptr;
+  }
+}
+
+void rk_func::pull(int array) {
+  HostArray *h = mirror(array);
+  if (!h) throw Error("pull: no host mirror for this array", FST_ERR_ARG);
+  check(fst_download(ctx_, array, h->data(), h->size()), "fst_download");
+}
+
+void rk_func::push(int array) {
+  HostArray *h = mirror(array);
+  if (!h) throw Error("push: no host mirror for this array", FST_ERR_ARG);
+  check(fst_upload(ctx_, array, h->data(), h->size()), "fst_upload");
+}
+
+void rk_func::sync() { check(fst_sync(ctx_), "fst_sync"); }
+
+namespace {
+// variable names as src/cart3d.cpp:98-121 and src/cart2d.cpp:41-77 set them
+void names(rk_func &f, const fst_config &cf, const fst_dims &d, const std::vector<std::string> &sp) {
+  auto sname = [&](int s) { return s < (int)sp.size() ? sp[s] : "Species " + std::to_string(s); };
+  if (cf.ndim == 3) {
+    f.varNames = {"X-Momentum", "Y-Momentum", "Z-Momentum", "Energy"};
+    for (int s = 0; s < cf.ns; ++s) f.varNames.push_back(sname(s) + " Density");
+    if (d.nvt > d.nv) for (const char *n : {"C", "C_hat", "C_1", "C_2", "C_3"}) f.varNames.push_back(n);
+    f.varxNames = {"X-Velocity", "Y-Velocity", "Z-Velocity", "Pressure", "Temperature", "Total Density"};
+  } else {
+    f.varNames = {"X-Momentum", "Y-Momentum", "Energy"};
+    for (int s = 0; s < cf.ns; ++s) f.varNames.push_back(sname(s) + " Density");
+    if (d.nvt > d.nv) for (const char *n : {"C", "C_hat", "Tau_1", "Tau_2", "Debug"}) f.varNames.push_back(n);
+    f.varxNames = {"X-Velocity", "Y-Velocity", "Pressure", "Temperature", "Total Density"};
+  }
+}
+}  // namespace
+
+cart3d_func::cart3d_func(const fst_config &cf, const std::vector<std::string> &sp) : rk_func(cf) {
+  if (cf.ndim != 3) throw Error("cart3d_func needs ndim = 3", FST_ERR_ARG);
+  names(*this, cf_, d_, sp);
+}
+void cart3d_func::compute() { check(fst_compute(ctx_), "fst_compute"); }
+void cart3d_func::preStep() { check(fst_pre_step(ctx_), "fst_pre_step"); }
+void cart3d_func::postStep() { check(fst_post_step(ctx_), "fst_post_step"); }
+void cart3d_func::preSim() { check(fst_pre_sim(ctx_), "fst_pre_sim"); }
+void cart3d_func::postSim() {}
+
+cart2d_func::cart2d_func(const fst_config &cf, const std::vector<std::string> &sp) : rk_func(cf) {
+  if (cf.ndim != 2) throw Error("cart2d_func needs ndim = 2", FST_ERR_ARG);
+  names(*this, cf_, d_, sp);
+}
+void cart2d_func::compute() { check(fst_compute(ctx_), "fst_compute"); }
+void cart2d_func::preStep() { check(fst_pre_step(ctx_), "fst_pre_step"); }
+void cart2d_func::postStep() { check(fst_post_step(ctx_), "fst_post_step"); }
+void cart2d_func::preSim() { check(fst_pre_sim(ctx_), "fst_pre_sim"); }
+void cart2d_func::postSim() {}
+
+void rkAdvance(rk_func &f) {
+  fst_ctx *c = f.context();
+  f.compute();
+  if (int rc = fst_rk_stage(c, 1)) throw Error(std::string("fst_rk_stage(1): ") + fst_last_error(c), rc);
+  f.compute();
+  if (int rc = fst_rk_stage(c, 2)) throw Error(std::string("fst_rk_stage(2): ") + fst_last_error(c), rc);
+}
+
+void applyBCs(rk_func &f) {
+  fst_ctx *c = f.context();
+  if (int rc = fst_apply_bcs(c)) throw Error(std::string("fst_apply_bcs: ") + fst_last_error(c), rc);
+}
+
+void step(rk_func &f, int nsteps) {
+  fst_ctx *c = f.context();
+  if (int rc = fst_step(c, nsteps)) throw Error(std::string("fst_step: ") + fst_last_error(c), rc);
+  double ms = 0.0;
+  if (fst_last_step_ms(c, &ms) == FST_OK) {
+    Timer &t = f.timers["step"];
+    t.description = "Fused RK steps (device time)";
+    t.ms += ms;
+  }
+}
+
+}  // namespace fiesta_hip
