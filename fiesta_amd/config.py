@@ -38,11 +38,15 @@ def make_config(ndim=3, ns=1, ng=3, nci=8, ncj=8, nck=8, dx=1.0, dy=1.0, dz=1.0,
                 gamma=(1.4,), M=(0.02896,), mu=(1.8e-5,), visc=False, ceq=False, buoyancy=False,
                 scheme="weno5", kap=1.23456, eps=1.0, alpha=1.0, g=9.81, rhoRef=0.0,
                 xper=False, yper=False, zper=False, bc=("outflow",) * 6, dt=1e-6,
-                precision=FST_FP64, learned_c=False):
-    """Build an FstConfig.  `bc` lists xmin, xmax, ymin, ymax, zmin, zmax."""
+                precision=FST_FP64, learned_c=False, grid=0):
+    """Build an FstConfig.  `bc` lists xmin, xmax, ymin, ymax, zmin, zmax.
+    grid=1 selects the generalized-grid modules (gen2d/gen3d), whose cell
+    sizes are 1 (src/input.cpp:344-350)."""
     c = FstConfig()
-    c.ndim, c.grid, c.ns, c.ng = ndim, 0, ns, ng
+    c.ndim, c.grid, c.ns, c.ng = ndim, int(grid), ns, ng
     c.nci, c.ncj, c.nck = nci, ncj, (nck if ndim == 3 else 1)
+    if grid == 1:
+        dx = dy = dz = 1.0
     c.dx, c.dy = dx, dy
     c.dz = dz if ndim == 3 else dx  # src/input.cpp:356
     c.R = R
@@ -78,6 +82,17 @@ class Dims:
         self.nvx = c.ndim + 3
         self.N = self.ngi * self.ngj * self.ngk
         self.ncells = self.nci * self.ncj * self.nck
+        # grid nodes, src/input.cpp:401-406
+        self.ni, self.nj = self.nci + 1, self.ncj + 1
+        self.nk = self.nck + 1 if c.ndim == 3 else 1
+
+    def grid_shape(self):
+        """numpy shape (v, k, j, i) of the LayoutLeft (ni,nj,nk,3) node array."""
+        return (3, self.nk, self.nj, self.ni)
+
+    def metrics_shape(self):
+        """numpy shape (n, m, k, j, i) of metrics(i,j,k,m,n)."""
+        return (self.ndim, self.ndim, self.ngk, self.ngj, self.ngi)
 
     def shape(self, nvar):
         """numpy shape (v, k, j, i) of a LayoutLeft (i,j,k,v) array."""

@@ -59,7 +59,14 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
     *why = "a periodic direction needs at least ng interior cells";
     return FST_ERR_ARG;
   }
-  if (c.grid != 0) { *why = "only the cartesian grid is supported"; return FST_ERR_ARG; }
+  if (c.grid != 0 && c.grid != 1) {
+    *why = "grid: 0 (cartesian) or 1 (generalized); the terrain grid (2) has no module in the reference";
+    return FST_ERR_ARG;
+  }
+  if (c.grid == 1 && (c.learned_c || c.precision != FST_FP64)) {
+    *why = "generalized grids run the reference arithmetic (FP64, no learned C-indicator)";
+    return FST_ERR_ARG;
+  }
   if (c.learned_c && (c.ndim != 3 || !c.ceq)) {
     *why = "the learned C-indicator replaces the 3D C-equation: needs ndim = 3 and ceq = 1";
     return FST_ERR_ARG;
@@ -82,6 +89,12 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
   return FST_OK;
 }
 
+// grid nodes (ni*nj*nk): one more than cells per direction, nk = 1 in 2D
+// (src/input.cpp:401-406)
+size_t grid_nodes(const Geo &g) {
+  return (size_t)(g.nci + 1) * (size_t)(g.ncj + 1) * (size_t)(g.ndim == 3 ? g.nck + 1 : 1);
+}
+
 double *array_ptr(fst_ctx *ctx, int a, size_t *count) {
   const size_t N = (size_t)ctx->g.N;
   double *p = nullptr;
@@ -96,6 +109,8 @@ double *array_ptr(fst_ctx *ctx, int a, size_t *count) {
     case FST_T: p = ctx->b.T; n = N; break;
     case FST_VEL: p = ctx->b.vel; n = N * ctx->g.ndim; break;
     case FST_GRADRHO: p = ctx->b.gradRho; n = ctx->b.gradRho ? N * (ctx->g.ndim == 3 ? 5 : 4) : 0; break;
+    case FST_GRID: p = ctx->b.grid; n = ctx->b.grid ? grid_nodes(ctx->g) * 3 : 0; break;
+    case FST_METRICS: p = ctx->b.metrics; n = ctx->b.metrics ? N * ctx->g.ndim * ctx->g.ndim : 0; break;
     default: return nullptr;
   }
   *count = n;
@@ -112,6 +127,18 @@ void free_all(fst_ctx *ctx) {
   fused_free(ctx->fb);
   learned_free(ctx->lb);
   if (ctx->lprim) { (void)hipFree(ctx->lprim); ctx->lprim = nullptr; }
+}
+
+// rk_func::compute() of the module the config selects: cart3d applies the
+// boundary conditions first (src/cart3d.cpp:221); cart2d applies them in
+// preStep; gen2d/gen3d never do (src/gen2d.cpp:100-131, src/gen3d.cpp:341-369).
+void compute_seq(fst_ctx *ctx) {
+  if (ctx->cf.grid == 1) {
+    launch_compute_gen(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->stream);
+    return;
+  }
+  if (ctx->g.ndim == 3) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
+  launch_compute_ref(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->stream);
 }
 
 int learned_only(fst_ctx *ctx, const char *what) {
@@ -160,8 +187,9 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
   }
   fst_ctx *ctx = new fst_ctx();
   ctx->cf = *cfg;
+  if (cfg->grid == 1) ctx->cf.dx = ctx->cf.dy = ctx->cf.dz = 1.0;  // src/input.cpp:344-350
   if (cfg->ndim == 2) {  // 2D defaults, src/input.cpp:354-362
-    ctx->cf.dz = cfg->dx;
+    ctx->cf.dz = ctx->cf.dx;
     ctx->cf.nck = 1;
     ctx->cf.zPer = 0;
   }
@@ -205,6 +233,11 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
   const bool ceq_vars = c.ceq && !c.learned_c;
   ok = ok && alloc(&ctx->b.gradRho, ceq_vars ? (g.ndim == 3 ? 5 : 4) : 0);
   ok = ok && alloc(&ctx->b.mflux, ceq_vars ? (g.ndim == 3 ? 27 : 8) : 0);
+  if (ok && c.grid == 1) {  // gen3d.cpp:41-43, gen2d.cpp:25-26
+    ok = alloc(&ctx->b.metrics, (size_t)g.ndim * g.ndim);
+    const size_t gbytes = grid_nodes(g) * 3 * sizeof(double);
+    ok = ok && hipMalloc((void **)&ctx->b.grid, gbytes) == hipSuccess && hipMemset(ctx->b.grid, 0, gbytes) == hipSuccess;
+  }
   if (ok && c.learned_c) {
     if (!fused_supported(c)) {
       fprintf(stderr, "fst_create: the learned C-indicator needs the fused 3D path (ns <= 3, FP64 or FP32_BULK)\n");
@@ -283,6 +316,10 @@ int fst_apply_bcs(fst_ctx *ctx) {
 int fst_pre_sim(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
   ctx->fb.valid = false;
+  if (ctx->cf.grid == 1) {  // gen3d_func::preSim / gen2d_func::preSim: metrics only
+    launch_metrics(ctx->g, ctx->b, ctx->stream);
+    return check_launch(ctx, "fst_pre_sim");
+  }
   if (ctx->g.ndim == 2) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);  // src/cart2d.cpp:137
   launch_secondary(ctx->g, ctx->ph, ctx->b, ctx->stream);
   return check_launch(ctx, "fst_pre_sim");
@@ -291,7 +328,8 @@ int fst_pre_sim(fst_ctx *ctx) {
 int fst_pre_step(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
   ctx->fb.valid = false;
-  if (ctx->g.ndim == 2) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);  // src/cart2d.cpp:147-149
+  // cart2d: src/cart2d.cpp:147-149; cart3d, gen2d, gen3d: nothing
+  if (ctx->g.ndim == 2 && ctx->cf.grid == 0) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
   return check_launch(ctx, "fst_pre_step");
 }
 
@@ -299,8 +337,7 @@ int fst_compute(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
   if (ctx->cf.learned_c) return learned_only(ctx, "fst_compute");
   ctx->fb.valid = false;
-  if (ctx->g.ndim == 3) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);  // src/cart3d.cpp:221
-  launch_compute_ref(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->stream);
+  compute_seq(ctx);
   return check_launch(ctx, "fst_compute");
 }
 
@@ -335,10 +372,9 @@ int fst_step(fst_ctx *ctx, int nsteps) {
   } else {
     ctx->fb.valid = false;
     for (int t = 0; t < nsteps; ++t) {
-      if (ctx->g.ndim == 2) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
+      if (ctx->g.ndim == 2 && ctx->cf.grid == 0) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
       for (int stage = 1; stage <= 2; ++stage) {
-        if (ctx->g.ndim == 3) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
-        launch_compute_ref(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->stream);
+        compute_seq(ctx);
         launch_rk_stage(ctx->g, stage, ctx->cf.dt, ctx->b.var, ctx->b.tmp1, ctx->b.dvar, ctx->stream);
       }
     }

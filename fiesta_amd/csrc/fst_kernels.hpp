@@ -120,5 +120,8 @@ void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream
 void launch_rk_stage(const Geo &g, int stage, double dt, double *var, double *tmp, const double *dvar, hipStream_t s);
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 void launch_secondary(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
+// generalized grids (gen2d/gen3d modules)
+void launch_compute_gen(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);
+void launch_metrics(const Geo &g, const Bufs &b, hipStream_t s);
 
 }  // namespace fst

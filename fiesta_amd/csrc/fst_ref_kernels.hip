@@ -792,6 +792,167 @@ __global__ void k_copy_extra(long long N, int ndim, const double *__restrict__ v
   varx[c + N * (ndim + 2)] = rho[c];
 }
 
+// ------------------------------------------------------------- generalized grids
+//
+// grid(i,j,k,v): node coordinates, LayoutLeft over (ni,nj,nk,3) with
+// ni = nci+1, nj = ncj+1, nk = nck+1 (1 in 2D) (src/gen3d.cpp:41,
+// src/gen2d.cpp:25, src/input.cpp:401-406).  metrics(i,j,k,m,n) = d xi_m / d x_n:
+// plane c + N*(m + ndim*n) (LayoutLeft over (ngi,ngj,ngk,ndim,ndim)).
+
+struct Nodes {
+  int ni, nj, nk;
+  __device__ __forceinline__ long long at(int i, int j, int k, int v) const {
+    return (long long)i + (long long)ni * ((long long)j + (long long)nj * ((long long)k + (long long)nk * v));
+  }
+};
+
+// computeMetrics3D, src/metric.hpp:59-164, over the cell range.  Node indices
+// are cell index - 3 as in the reference (the literal 3, not ng).
+__global__ void k_metrics3d(Geo g, Range r, Nodes nd, const double *__restrict__ grid, double *__restrict__ mt) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const int ii = i - 3, jj = j - 3, kk = k - 3;
+  auto G = [&](int a, int b, int c, int v) { return grid[nd.at(a, b, c, v)]; };
+  double xm[3], xp[3], ym[3], yp[3], zm[3], zp[3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    xm[v] = (G(ii, jj, kk, v) + G(ii, jj + 1, kk, v) + G(ii, jj, kk + 1, v) + G(ii, jj + 1, kk + 1, v)) / 4.0;
+    xp[v] = (G(ii + 1, jj, kk, v) + G(ii + 1, jj + 1, kk, v) + G(ii + 1, jj, kk + 1, v) +
+             G(ii + 1, jj + 1, kk + 1, v)) / 4.0;
+    zm[v] = (G(ii, jj, kk, v) + G(ii + 1, jj, kk, v) + G(ii, jj + 1, kk, v) + G(ii + 1, jj + 1, kk, v)) / 4.0;
+    zp[v] = (G(ii, jj, kk + 1, v) + G(ii + 1, jj, kk + 1, v) + G(ii, jj + 1, kk + 1, v) +
+             G(ii + 1, jj + 1, kk + 1, v)) / 4.0;
+    ym[v] = (G(ii, jj, kk, v) + G(ii + 1, jj, kk, v) + G(ii, jj, kk + 1, v) + G(ii + 1, jj, kk + 1, v)) / 4.0;
+    yp[v] = (G(ii, jj + 1, kk, v) + G(ii + 1, jj + 1, kk, v) + G(ii, jj + 1, kk + 1, v) +
+             G(ii + 1, jj + 1, kk + 1, v)) / 4.0;
+  }
+  const double xxi = xp[0] - xm[0], xet = yp[0] - ym[0], xzt = zp[0] - zm[0];
+  const double yxi = xp[1] - xm[1], yet = yp[1] - ym[1], yzt = zp[1] - zm[1];
+  const double zxi = xp[2] - xm[2], zet = yp[2] - ym[2], zzt = zp[2] - zm[2];
+  const double jac = 1.0 / (xxi * (yet * zzt - yzt * zet) + xet * (yzt * zxi - yxi * zzt) + xzt * (yxi * zet - yet * zxi));
+  const long long c = g.id(i, j, k), N = g.N;
+  mt[c + N * 0] = jac * (yet * zzt - yzt * zet);  // (0,0)
+  mt[c + N * 3] = jac * (xzt * zet - xet * zzt);  // (0,1)
+  mt[c + N * 6] = jac * (xet * yzt - xzt * yet);  // (0,2)
+  mt[c + N * 1] = jac * (yzt * zxi - yxi * zzt);  // (1,0)
+  mt[c + N * 4] = jac * (xxi * zzt - xzt * zxi);  // (1,1)
+  mt[c + N * 7] = jac * (xzt * yxi - xxi * yzt);  // (1,2)
+  mt[c + N * 2] = jac * (yxi * zet - yet * zxi);  // (2,0)
+  mt[c + N * 5] = jac * (xet * zxi - xxi * zet);  // (2,1)
+  mt[c + N * 8] = jac * (xxi * yet - xet * yxi);  // (2,2)
+}
+
+// computeMetrics2D, src/metric.hpp:21-57, over the cell range
+__global__ void k_metrics2d(Geo g, Range r, Nodes nd, const double *__restrict__ grid, double *__restrict__ mt) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const int ii = i - 3, jj = j - 3;
+  auto G = [&](int a, int b, int v) { return grid[nd.at(a, b, 0, v)]; };
+  const double x_xi = (G(ii + 1, jj, 0) + G(ii + 1, jj + 1, 0)) / 2.0 - (G(ii, jj, 0) + G(ii, jj + 1, 0)) / 2.0;
+  const double y_xi = (G(ii + 1, jj, 1) + G(ii + 1, jj + 1, 1)) / 2.0 - (G(ii, jj, 1) + G(ii, jj + 1, 1)) / 2.0;
+  const double x_et = (G(ii, jj + 1, 0) + G(ii + 1, jj + 1, 0)) / 2.0 - (G(ii, jj, 0) + G(ii + 1, jj, 0)) / 2.0;
+  const double y_et = (G(ii, jj + 1, 1) + G(ii + 1, jj + 1, 1)) / 2.0 - (G(ii, jj, 1) + G(ii + 1, jj, 1)) / 2.0;
+  const double jac = 1.0 / (x_xi * y_et - x_et * y_xi);
+  const long long c = g.id(i, j, 0), N = g.N;
+  mt[c + N * 0] = jac * y_et;    // (0,0)
+  mt[c + N * 2] = -jac * x_et;   // (0,1)
+  mt[c + N * 1] = -jac * y_xi;   // (1,0)
+  mt[c + N * 3] = jac * x_xi;    // (1,1)
+}
+
+// Metric ghosts of a rank without neighbours on one face: mirror copy of the
+// ng layers next to it (src/gen3d.cpp:241-300, src/gen2d.cpp:285-329), over the
+// full range of the other two indices.  dir 0/1/2, side -1 (min) or +1 (max).
+__global__ void k_metric_ghost(Geo g, int dir, int side, double *__restrict__ mt) {
+  const int e0 = dir == 0 ? g.ng : g.ngi, e1 = dir == 1 ? g.ng : g.ngj, e2 = dir == 2 ? g.ng : g.ngk;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)e0 * e1 * e2) return;
+  const int a = (int)(t % e0);
+  t /= e0;
+  const int b = (int)(t % e1);
+  const int c = (int)(t / e1);
+  int di = a, dj = b, dk = c, si = a, sj = b, sk = c;
+  if (dir == 0) {
+    di = side < 0 ? g.ng - a - 1 : g.ngi - 1 - a;
+    si = side < 0 ? g.ng + a : g.nci + a;
+  } else if (dir == 1) {
+    dj = side < 0 ? g.ng - b - 1 : g.ngj - 1 - b;
+    sj = side < 0 ? g.ng + b : g.ncj + b;
+  } else {
+    dk = side < 0 ? g.ng - c - 1 : g.ngk - 1 - c;
+    sk = side < 0 ? g.ng + c : g.nck + c;
+  }
+  const long long d = g.id(di, dj, dk), s = g.id(si, sj, sk);
+  const int nm = g.ndim * g.ndim;
+  for (int q = 0; q < nm; ++q) mt[d + g.N * q] = mt[s + g.N * q];
+}
+
+// computeGenVelocity3D / 2D (src/velocity.hpp:23-66): contravariant velocity
+// vel(d) = sum_m metrics(d,m) * var(m) / rho over every cell, after the
+// primitive pass has written rho.
+__global__ void k_genvel(Geo g, Range r, const double *__restrict__ var, const double *__restrict__ rho,
+                         const double *__restrict__ mt, double *__restrict__ vel) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long c = g.id(i, j, k), N = g.N;
+  const double rr = rho[c];
+  if (g.ndim == 3) {
+    const double m0 = var[c], m1 = var[c + N], m2 = var[c + 2 * N];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      vel[c + N * d] = (mt[c + N * (d + 0)] * m0 / rr + mt[c + N * (d + 3)] * m1 / rr + mt[c + N * (d + 6)] * m2 / rr);
+  } else {
+    const double m0 = var[c], m1 = var[c + N];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) vel[c + N * d] = (mt[c + N * (d + 0)] * m0 / rr + mt[c + N * (d + 2)] * m1 / rr);
+  }
+}
+
+// applyGenPressureGradient3D, src/presgrad.hpp:72-147, over the cell range:
+// dvar(n) -= sum_d d/dxi_d (p * metrics(d,n)), 4th-order central in each
+// computational direction.
+__global__ void k_genpresgrad3d(Geo g, Range r, const double *__restrict__ p, const double *__restrict__ mt,
+                                double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long c = g.id(i, j, k), N = g.N;
+  auto P = [&](int a, int b, int cc) { return p[g.id(a, b, cc)]; };
+  auto M = [&](int a, int b, int cc, int m, int n) { return mt[g.id(a, b, cc) + N * (m + 3 * n)]; };
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const double dxi = (P(i - 2, j, k) * M(i - 2, j, k, 0, n) - 8.0 * P(i - 1, j, k) * M(i - 1, j, k, 0, n) +
+                        8.0 * P(i + 1, j, k) * M(i + 1, j, k, 0, n) - P(i + 2, j, k) * M(i + 2, j, k, 0, n)) /
+                       (12.0);
+    const double det = (P(i, j - 2, k) * M(i, j - 2, k, 1, n) - 8.0 * P(i, j - 1, k) * M(i, j - 1, k, 1, n) +
+                        8.0 * P(i, j + 1, k) * M(i, j + 1, k, 1, n) - P(i, j + 2, k) * M(i, j + 2, k, 1, n)) /
+                       (12.0);
+    const double dzt = (P(i, j, k - 2) * M(i, j, k - 2, 2, n) - 8.0 * P(i, j, k - 1) * M(i, j, k - 1, 2, n) +
+                        8.0 * P(i, j, k + 1) * M(i, j, k + 1, 2, n) - P(i, j, k + 2) * M(i, j, k + 2, 2, n)) /
+                       (12.0);
+    dvar[c + N * n] = (dvar[c + N * n] - (dxi + det + dzt));
+  }
+}
+
+// applyGenPressureGradient2D, src/presgrad.hpp:149-190, over the cell range
+__global__ void k_genpresgrad2d(Geo g, Range r, const double *__restrict__ p, const double *__restrict__ mt,
+                                double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long c = g.id(i, j, 0), N = g.N;
+  auto P = [&](int a, int b) { return p[g.id(a, b, 0)]; };
+  auto M = [&](int a, int b, int m, int n) { return mt[g.id(a, b, 0) + N * (m + 2 * n)]; };
+#pragma unroll
+  for (int n = 0; n < 2; ++n) {
+    const double dxi = (P(i - 2, j) * M(i - 2, j, 0, n) - 8.0 * P(i - 1, j) * M(i - 1, j, 0, n) +
+                        8.0 * P(i + 1, j) * M(i + 1, j, 0, n) - P(i + 2, j) * M(i + 2, j, 0, n)) /
+                       (12.0);
+    const double det = (P(i, j - 2) * M(i, j - 2, 1, n) - 8.0 * P(i, j - 1) * M(i, j - 1, 1, n) +
+                        8.0 * P(i, j + 1) * M(i, j + 1, 1, n) - P(i, j + 2) * M(i, j + 2, 1, n)) /
+                       (12.0);
+    dvar[c + N * n] = (dvar[c + N * n] - (dxi + det));
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- launchers
@@ -843,6 +1004,54 @@ void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, cons
       k_ceqfaces2d<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, b.var, b.rho, b.vel, b.sc, b.mflux);
       k_applyceq2d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.mflux, b.dvar);
     }
+  }
+}
+
+// gen3d_func::compute (src/gen3d.cpp:341-369) / gen2d_func::compute
+// (src/gen2d.cpp:100-131): primitives, contravariant velocity, per-variable
+// fluxes and flux differences, metric pressure gradient.  No boundary
+// conditions and no viscous, buoyancy or C-equation terms.  The 3D flux
+// kernel's dz is cf.dx (src/gen3d.cpp:359); both are 1 on these grids.
+void launch_compute_gen(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s) {
+  launch_eos_vel(g, ph, b, s);
+  Range gr = ghost_range(g), cr = cell_range(g), fr = face_range(g);
+  k_genvel<<<grid_for(gr), 256, 0, s>>>(g, gr, b.var, b.rho, b.metrics, b.vel);
+  if (g.ndim == 3) {
+    Phys pg = ph;
+    pg.dz = ph.dx;
+    for (int v = 0; v < g.nv; ++v) {
+      if (b.prof) { b.prof->name = "k_flux3d"; b.prof->per = 2; b.prof->begin(s); }
+      k_flux3d<<<grid_for(fr), 256, 0, s>>>(g, pg, fr, v, b.var, b.p, b.vel, b.fx, b.fy, b.fz);
+      if (b.prof) b.prof->end(s);
+      k_advect3d<<<grid_for(cr), 256, 0, s>>>(g, cr, v, b.fx, b.fy, b.fz, b.dvar);
+    }
+    k_genpresgrad3d<<<grid_for(cr), 256, 0, s>>>(g, cr, b.p, b.metrics, b.dvar);
+  } else {
+    for (int v = 0; v < g.nv; ++v) {
+      k_flux2d<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, v, cf.scheme, b.var, b.p, b.rho, b.vel, b.fx, b.fy);
+      k_advect2d<<<grid_for(cr), 256, 0, s>>>(g, cr, v, b.fx, b.fy, b.dvar);
+    }
+    k_genpresgrad2d<<<grid_for(cr), 256, 0, s>>>(g, cr, b.p, b.metrics, b.dvar);
+  }
+}
+
+// gen3d_func::preSim / gen2d_func::preSim: metrics of the interior cells, then
+// the mirror-copied metric ghosts of a rank without neighbours, 3D in the order
+// x-, x+, y-, y+, z-, z+ (src/gen3d.cpp:131-304), 2D y-, y+, x-, x+
+// (src/gen2d.cpp:187-331).
+void launch_metrics(const Geo &g, const Bufs &b, hipStream_t s) {
+  const Nodes nd{g.nci + 1, g.ncj + 1, g.ndim == 3 ? g.nck + 1 : 1};
+  Range cr = cell_range(g);
+  if (g.ndim == 3) k_metrics3d<<<grid_for(cr), 256, 0, s>>>(g, cr, nd, b.grid, b.metrics);
+  else k_metrics2d<<<grid_for(cr), 256, 0, s>>>(g, cr, nd, b.grid, b.metrics);
+  static const int order3[6][2] = {{0, -1}, {0, 1}, {1, -1}, {1, 1}, {2, -1}, {2, 1}};
+  static const int order2[4][2] = {{1, -1}, {1, 1}, {0, -1}, {0, 1}};
+  const int nf = g.ndim == 3 ? 6 : 4;
+  for (int f = 0; f < nf; ++f) {
+    const int dir = g.ndim == 3 ? order3[f][0] : order2[f][0];
+    const int side = g.ndim == 3 ? order3[f][1] : order2[f][1];
+    const long long n = (long long)(dir == 0 ? g.ng : g.ngi) * (dir == 1 ? g.ng : g.ngj) * (dir == 2 ? g.ng : g.ngk);
+    k_metric_ghost<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, dir, side, b.metrics);
   }
 }
 

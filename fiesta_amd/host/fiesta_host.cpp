@@ -13,6 +13,7 @@ rk_func::rk_func(const fst_config &cf) : cf_(cf) {
   dvar.resize(d_.ngi, d_.ngj, d_.ngk, d_.nvt);
   tmp1.resize(d_.ngi, d_.ngj, d_.ngk, d_.nvt);
   varx.resize(d_.ngi, d_.ngj, d_.ngk, d_.nvx);
+  if (cf_.grid == 1) grid.resize(cf_.nci + 1, cf_.ncj + 1, cf_.ndim == 3 ? cf_.nck + 1 : 1, 3);
 }
 
 rk_func::~rk_func() {
@@ -33,6 +34,7 @@ HostArray *rk_func::mirror(int array) {
     case FST_VARX: return &varx;
     case FST_DVAR: return &dvar;
     case FST_TMP1: return &tmp1;
+    case FST_GRID: return grid.size() ? &grid : nullptr;
     default: return nullptr;
   }
 }
@@ -88,6 +90,48 @@ void cart2d_func::preStep() { check(fst_pre_step(ctx_), "fst_pre_step"); }
 void cart2d_func::postStep() { check(fst_post_step(ctx_), "fst_post_step"); }
 void cart2d_func::preSim() { check(fst_pre_sim(ctx_), "fst_pre_sim"); }
 void cart2d_func::postSim() {}
+
+// variable names as src/gen3d.cpp:67-86 and src/gen2d.cpp:37-78 set them
+void gen_names(rk_func &f, const fst_config &cf, const std::vector<std::string> &sp) {
+  auto sname = [&](int s) { return s < (int)sp.size() ? sp[s] : "Species " + std::to_string(s); };
+  if (cf.ndim == 3) {
+    f.varNames = {"X-Momentum", "Y-Momentum", "Z-Momentum", "Energy"};
+    f.varxNames = {"X-Velocity", "Y-Velocity", "Z-Velocity", "Pressure", "Temperature", "Density"};
+  } else {
+    f.varNames = {"X-Momentum", "Y-Momentum", "Energy"};
+    f.varxNames = {"X-Velocity", "Y-Velocity", "Pressure", "Temperature", "Density"};
+  }
+  for (int s = 0; s < cf.ns; ++s) f.varNames.push_back("Density " + sname(s));
+}
+
+gen3d_func::gen3d_func(const fst_config &cf, const std::vector<std::string> &sp) : rk_func(cf) {
+  if (cf.ndim != 3 || cf.grid != 1) throw Error("gen3d_func needs ndim = 3 and grid = 1", FST_ERR_ARG);
+  gen_names(*this, cf_, sp);
+}
+void gen3d_func::compute() { check(fst_compute(ctx_), "fst_compute"); }
+void gen3d_func::preStep() { check(fst_pre_step(ctx_), "fst_pre_step"); }
+void gen3d_func::postStep() { check(fst_post_step(ctx_), "fst_post_step"); }
+void gen3d_func::preSim() { check(fst_pre_sim(ctx_), "fst_pre_sim"); }
+void gen3d_func::postSim() {}
+
+gen2d_func::gen2d_func(const fst_config &cf, const std::vector<std::string> &sp) : rk_func(cf) {
+  if (cf.ndim != 2 || cf.grid != 1) throw Error("gen2d_func needs ndim = 2 and grid = 1", FST_ERR_ARG);
+  gen_names(*this, cf_, sp);
+}
+void gen2d_func::compute() { check(fst_compute(ctx_), "fst_compute"); }
+void gen2d_func::preStep() { check(fst_pre_step(ctx_), "fst_pre_step"); }
+void gen2d_func::postStep() { check(fst_post_step(ctx_), "fst_post_step"); }
+void gen2d_func::preSim() { check(fst_pre_sim(ctx_), "fst_pre_sim"); }
+void gen2d_func::postSim() {}
+
+rk_func *make_module(const fst_config &cf, const std::vector<std::string> &sp) {
+  if (cf.grid == 1) {
+    if (cf.ndim == 3) return new gen3d_func(cf, sp);
+    return new gen2d_func(cf, sp);
+  }
+  if (cf.ndim == 3) return new cart3d_func(cf, sp);
+  return new cart2d_func(cf, sp);
+}
 
 void rkAdvance(rk_func &f) {
   fst_ctx *c = f.context();

@@ -6,6 +6,7 @@
 //                            postStep, preSim, postSim; var, varx, dvar,
 //                            tmp1, grid; varNames, varxNames; timers)
 //   cart3d_func/cart2d_func  src/cart3d.cpp:44-378, src/cart2d.cpp:30-311
+//   gen3d_func/gen2d_func    src/gen3d.cpp:39-370, src/gen2d.cpp:22-333
 //   rkAdvance(cf, f)         src/rk.cpp:28-72
 //   applyBCs(cf, f)          src/bc.cpp:179-249
 //   Fiesta::step             src/fiesta.cpp:333-340
@@ -77,10 +78,12 @@ class rk_func {
   virtual void postSim() = 0;
 
   HostArray var, varx, dvar, tmp1;
+  HostArray grid;  // node coordinates (ni, nj, nk, 3); generalized grids only
   std::vector<std::string> varNames, varxNames;
   std::map<std::string, Timer> timers;
 
-  // device <-> host copies of one array (FST_VAR, FST_VARX, FST_DVAR, FST_TMP1)
+  // device <-> host copies of one array (FST_VAR, FST_VARX, FST_DVAR, FST_TMP1,
+  // and FST_GRID on generalized grids)
   void pull(int array);
   void push(int array);
   void sync();
@@ -121,6 +124,36 @@ class cart2d_func : public rk_func {
   void preSim() override;
   void postSim() override;
 };
+
+// gen3d_func / gen2d_func: generalized (curvilinear) grids.  preSim computes
+// the metrics of `grid` (push(FST_GRID) first) and their mirrored ghosts
+// (src/gen3d.cpp:131-304, src/gen2d.cpp:187-331); compute() evaluates the EOS,
+// contravariant velocity, WENO fluxes and metric pressure gradient
+// (src/gen3d.cpp:341-369, src/gen2d.cpp:100-131).  Neither module applies
+// boundary conditions; preStep does nothing.
+class gen3d_func : public rk_func {
+ public:
+  explicit gen3d_func(const fst_config &cf, const std::vector<std::string> &speciesName = {});
+  void compute() override;
+  void preStep() override;
+  void postStep() override;
+  void preSim() override;
+  void postSim() override;
+};
+
+class gen2d_func : public rk_func {
+ public:
+  explicit gen2d_func(const fst_config &cf, const std::vector<std::string> &speciesName = {});
+  void compute() override;
+  void preStep() override;
+  void postStep() override;
+  void preSim() override;
+  void postSim() override;
+};
+
+// The module the reference builds for a configuration (src/fiesta.cpp: cart2d,
+// cart3d, gen2d or gen3d from grid.ndim and grid.type).
+rk_func *make_module(const fst_config &cf, const std::vector<std::string> &speciesName = {});
 
 // rkAdvance (src/rk.cpp:28-72): compute, stage-1 update, compute, stage-2 update.
 void rkAdvance(rk_func &f);

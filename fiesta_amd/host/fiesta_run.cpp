@@ -1,9 +1,11 @@
 // fiesta_run: time steps through the host rk_func interface.
 //
-//   fiesta_run CONFIG STATE OUT NSTEPS [rk|fused]
+//   fiesta_run CONFIG STATE OUT NSTEPS [rk|fused] [GRID]
 //
 // CONFIG is a raw fst_config (include/fiesta_hip.h), STATE the initial var
-// (LayoutLeft doubles, nvt planes with ghosts), OUT receives var after NSTEPS
+// (LayoutLeft doubles, nvt planes with ghosts), GRID the node coordinates of
+// a generalized grid (LayoutLeft (ni,nj,nk,3) doubles; required when
+// CONFIG.grid = 1), OUT receives var after NSTEPS
 // steps.  "rk" runs the module sequence of Fiesta::step (src/fiesta.cpp:
 // 333-340): preStep, rkAdvance (src/rk.cpp:28-72), postStep; "fused" runs
 // fst_step.  Prints one JSON line with the step count and device time.
@@ -28,7 +30,7 @@ static bool read_file(const char *path, void *dst, size_t bytes) {
 
 int main(int argc, char **argv) {
   if (argc < 5) {
-    std::fprintf(stderr, "usage: %s CONFIG STATE OUT NSTEPS [rk|fused]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s CONFIG STATE OUT NSTEPS [rk|fused] [GRID]\n", argv[0]);
     return 2;
   }
   const int nsteps = std::atoi(argv[4]);
@@ -43,14 +45,19 @@ int main(int argc, char **argv) {
     return 2;
   }
   try {
-    std::unique_ptr<rk_func> f;
-    if (cf.ndim == 3) f.reset(new cart3d_func(cf));
-    else f.reset(new cart2d_func(cf));
+    std::unique_ptr<rk_func> f(make_module(cf));
     if (!read_file(argv[2], f->var.data(), f->var.size() * sizeof(double))) {
       std::fprintf(stderr, "fiesta_run: %s does not hold %zu doubles\n", argv[2], f->var.size());
       return 2;
     }
     f->push(FST_VAR);
+    if (cf.grid == 1) {
+      if (argc < 7 || !read_file(argv[6], f->grid.data(), f->grid.size() * sizeof(double))) {
+        std::fprintf(stderr, "fiesta_run: a generalized grid needs GRID with %zu doubles\n", f->grid.size());
+        return 2;
+      }
+      f->push(FST_GRID);
+    }
     f->preSim();
     if (mode == "fused") fst_set_fused(f->context(), 1);
     f->sync();

@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 from .config import (Dims, FST_VAR, FST_TMP1, FST_DVAR, FST_VARX, FST_P, FST_RHO, FST_T, FST_VEL,
-                     FST_GRADRHO)
+                     FST_GRADRHO, FST_GRID, FST_METRICS)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfiesta_hip.so")
@@ -89,7 +89,7 @@ def exported_symbols():
 
 
 class Module:
-    """A cart2d/cart3d module instance on the GPU (one HIP stream)."""
+    """A cart2d/cart3d/gen2d/gen3d module instance on the GPU (one HIP stream)."""
 
     def __init__(self, cfg):
         self.cfg = cfg
@@ -127,7 +127,13 @@ class Module:
         self._chk(lib().fst_upload(self.h, array, a.ctypes.data, a.size), "fst_upload")
 
     def download(self, array):
-        out = np.empty(self.dims.shape(self._nvar(array)), dtype=np.float64)
+        if array == FST_GRID:
+            shape = self.dims.grid_shape()
+        elif array == FST_METRICS:
+            shape = self.dims.metrics_shape()
+        else:
+            shape = self.dims.shape(self._nvar(array))
+        out = np.empty(shape, dtype=np.float64)
         self._chk(lib().fst_download(self.h, array, out.ctypes.data, out.size), "fst_download")
         return out
 

@@ -31,11 +31,12 @@ def _cells(d):
     return x[None, None, :], y[None, :, None], z[:, None, None]
 
 
-def fill_conserved(cfg, rho_s, u, p, cvars=None):
+def fill_conserved(cfg, rho_s, u, p, cvars=None, ghosts=False):
     """Pack primitive fields (interior, shape (k,j,i)) into a conserved state array.
 
     rho_s: list of species densities; u: list of ndim velocity components;
     p: pressure.  Energy uses the mixture gamma of src/secondary.hpp:128-141.
+    ghosts=True: the fields cover every cell (shape (ngk,ngj,ngi)).
     """
     d = Dims(cfg)
     var = np.zeros(d.shape(d.nvt), dtype=np.float64)
@@ -50,7 +51,7 @@ def fill_conserved(cfg, rho_s, u, p, cvars=None):
     gam = Cp / Cv
     ke = 0.5 * rho * sum(ui * ui for ui in u)
     E = p / (gam - 1) + ke
-    sl = d.interior()[1:]
+    sl = (slice(None),) * 3 if ghosts else d.interior()[1:]
     for a in range(cfg.ndim):
         var[(a,) + sl] = rho * u[a]
     var[(cfg.ndim,) + sl] = E
@@ -76,6 +77,67 @@ def random_state(cfg, seed=0, amp=0.2):
         cv = [np.abs(rng.normal(0, 5.0, sh)), np.abs(rng.normal(0, 5.0, sh))] + \
              [rng.normal(0, 5.0, sh) for _ in range(3)]
     return fill_conserved(cfg, rho_s, u, p, cv)
+
+
+def random_state_all(cfg, seed=0, amp=0.2, umag=60.0):
+    """Random physical state on every cell, ghosts included.  The generalized
+    grid modules never fill ghosts (src/gen3d.cpp:341-369), so their inputs
+    carry ghost values from the caller."""
+    rng = np.random.default_rng(seed)
+    d = Dims(cfg)
+    sh = (d.ngk, d.ngj, d.ngi)
+    base_rho = [1.2, 0.16, 6.0][: cfg.ns] + [1.0] * max(0, cfg.ns - 3)
+    rho_s = [b * rng.uniform(0.3, 1.0, sh) for b in base_rho[: cfg.ns]]
+    p = 1.0e5 * rng.uniform(1 - amp, 1 + amp, sh)
+    u = [rng.normal(0.0, umag, sh) for _ in range(cfg.ndim)]
+    cv = None
+    if cfg.ceq:
+        cv = [np.abs(rng.normal(0, 5.0, sh)) for _ in range(5)]
+    return fill_conserved(cfg, rho_s, u, p, cv, ghosts=True)
+
+
+def grid_nodes(cfg, fn):
+    """Node array (3, nk, nj, ni) of a generalized grid: fn(i, j, k) -> (x, y, z)
+    on broadcast index arrays (node indices 0..nc)."""
+    d = Dims(cfg)
+    k, j, i = np.meshgrid(np.arange(d.nk, dtype=np.float64), np.arange(d.nj, dtype=np.float64),
+                          np.arange(d.ni, dtype=np.float64), indexing="ij")
+    x, y, z = fn(i, j, k)
+    g = np.zeros(d.grid_shape())
+    g[0], g[1] = x, y
+    if cfg.ndim == 3:
+        g[2] = z
+    return g
+
+
+def terrain_grid(cfg, L=1.0, H=1.0, amp=0.05, beta=2.0):
+    """Terrain-following nodes: ground z_g = amp L sin(2 pi x/L) cos(2 pi y/L),
+    vertical stretch z = z_g + (H - z_g)(1 + tanh(beta (xi - 1))/tanh beta),
+    xi = k/nck (3D); in 2D the ground is y_g = amp L sin(2 pi x/L) under the
+    same stretch in y."""
+    d = Dims(cfg)
+
+    def fn(i, j, k):
+        x = L * i / d.nci
+        if cfg.ndim == 3:
+            y = L * j / d.ncj
+            zg = amp * L * np.sin(2 * np.pi * x / L) * np.cos(2 * np.pi * y / L)
+            xi = k / d.nck
+            z = zg + (H - zg) * (1.0 + np.tanh(beta * (xi - 1.0)) / np.tanh(beta))
+            return x, y, z
+        yg = amp * L * np.sin(2 * np.pi * x / L)
+        eta = j / d.ncj
+        y = yg + (H - yg) * (1.0 + np.tanh(beta * (eta - 1.0)) / np.tanh(beta))
+        return x, y, None
+
+    return grid_nodes(cfg, fn)
+
+
+def gen3d_terrain_config(n=(512, 512, 256), ns=3, dt=None):
+    """3D generalized-grid module over terrain, n = (nci, ncj, nck) cells,
+    ns species (air, hot gas, SF6)."""
+    return make_config(ndim=3, ns=ns, nci=n[0], ncj=n[1], nck=n[2], grid=1, R=RU,
+                       **_mix(ns, [AIR, HOT, SF6]), dt=dt if dt is not None else 1e-7)
 
 
 def sod_config(nci=1000, ncj=4, dt=2e-4, ceq=True):

@@ -3,7 +3,7 @@
  * Runge-Kutta path (gfx950, hand-written HIP kernels).
  *
  * The entry points replace the device side of FIESTA's `rk_func` interface:
- * a host `rk_func` subclass (cart2d / cart3d) keeps its compute() / preStep()
+ * a host `rk_func` subclass (cart2d / cart3d / gen2d / gen3d) keeps its compute() / preStep()
  * / postStep() / preSim() / postSim() methods and forwards each one to the
  * function named below.  Plain pointers and sizes only; every function
  * returns 0 on success and a nonzero FST_ERR_* code otherwise, with a message
@@ -12,6 +12,7 @@
  * Reference interfaces (paths relative to the FIESTA source tree):
  *   class rk_func                      src/rkfunction.hpp:36-69
  *   cart3d_func / cart2d_func          src/cart3d.cpp:44-378, src/cart2d.cpp:30-311
+ *   gen3d_func / gen2d_func            src/gen3d.cpp:39-370, src/gen2d.cpp:22-333
  *   rkAdvance(cf, f)                   src/rk.hpp:26, src/rk.cpp:28-72
  *   applyBCs(cf, f)                    src/bc.hpp:33, src/bc.cpp:179-249
  *   struct inputConfig (fields used)   src/input.hpp:59-148
@@ -62,8 +63,9 @@ enum fst_array {
   FST_T = 6,       /* temperature              [ngi,ngj,ngk]            */
   FST_VEL = 7,     /* velocity                 [ngi,ngj,ngk,ndim]       */
   FST_GRADRHO = 8, /* C-eq source terms        [ngi,ngj,ngk,5 | 4 in 2D] */
-  FST_GRID = 9,    /* grid nodes (generalized) [ni,nj,nk,ndim]          */
-  FST_METRICS = 10 /* inverse Jacobian         [ngi,ngj,ngk,ndim,ndim]  */
+  FST_GRID = 9,    /* grid nodes (generalized) [ni,nj,nk,3], ni = nci+1,
+                      nj = ncj+1, nk = nck+1 (1 in 2D)                  */
+  FST_METRICS = 10 /* inverse Jacobian d xi_m/d x_n [ngi,ngj,ngk,ndim,ndim] */
 };
 
 enum fst_err {
@@ -81,7 +83,9 @@ enum fst_err {
  */
 typedef struct fst_config {
   int32_t ndim;          /* 2 or 3                                  (grid.ndim) */
-  int32_t grid;          /* 0 cartesian, 1 generalized               (cf.grid)  */
+  int32_t grid;          /* 0 cartesian, 1 generalized (gen2d/gen3d: FP64,
+                            dx = dy = dz = 1, no BCs, no viscous, buoyancy
+                            or C-equation terms)                (cf.grid)  */
   int32_t ns;            /* species count, 1..FST_MAX_SPECIES        (cf.ns)    */
   int32_t ng;            /* ghost width, default 3                   (cf.ng)    */
   int32_t nci, ncj, nck; /* interior cells; nck = 1 in 2D                       */
@@ -135,16 +139,19 @@ int fst_get_dims(const fst_ctx *ctx, fst_dims *out);
 int fst_upload(fst_ctx *ctx, int array, const double *host, size_t count);
 int fst_download(fst_ctx *ctx, int array, double *host, size_t count);
 
-/* rk_func::preSim (src/cart3d.cpp:176-185, src/cart2d.cpp:136-145). */
+/* rk_func::preSim (src/cart3d.cpp:176-185, src/cart2d.cpp:136-145).  On a
+ * generalized grid: the metrics of the uploaded FST_GRID and their
+ * mirror-copied ghosts (src/gen3d.cpp:131-304, src/gen2d.cpp:187-331). */
 int fst_pre_sim(fst_ctx *ctx);
-/* rk_func::preStep: 2D applies the boundary conditions (src/cart2d.cpp:147-149),
- * 3D does nothing (src/cart3d.cpp:187-188). */
+/* rk_func::preStep: cart2d applies the boundary conditions (src/cart2d.cpp:147-149),
+ * cart3d, gen2d and gen3d do nothing (src/cart3d.cpp:187-188, src/gen3d.cpp:109). */
 int fst_pre_step(fst_ctx *ctx);
 /* applyBCs (src/bc.cpp:179-249), no-MPI branch: periodic x,y,z then
  * xmin,xmax,ymin,ymax,zmin,zmax planes. */
 int fst_apply_bcs(fst_ctx *ctx);
 /* rk_func::compute(): dvar = R(var), reference kernel sequence
- * (src/cart3d.cpp:220-298, src/cart2d.cpp:151-237). */
+ * (src/cart3d.cpp:220-298, src/cart2d.cpp:151-237, src/gen3d.cpp:341-369,
+ * src/gen2d.cpp:100-131). */
 int fst_compute(fst_ctx *ctx);
 /* One Runge-Kutta stage update on the current dvar (src/rk.cpp:42-50 for
  * stage 1, src/rk.cpp:63-69 for stage 2). */

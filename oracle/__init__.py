@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 from fiesta_amd.config import (Dims, FST_VAR, FST_TMP1, FST_DVAR, FST_VARX, FST_P, FST_RHO, FST_T,
-                               FST_VEL, FST_GRADRHO)
+                               FST_VEL, FST_GRADRHO, FST_GRID, FST_METRICS)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -54,7 +54,7 @@ _NVARS = {FST_VAR: "nvt", FST_TMP1: "nvt", FST_DVAR: "nvt", FST_VARX: "nvx", FST
 
 
 class Oracle:
-    """One module instance (cart2d/cart3d semantics) on the CPU."""
+    """One module instance (cart2d/cart3d/gen2d/gen3d semantics) on the CPU."""
 
     def __init__(self, cfg):
         self.cfg = cfg
@@ -76,13 +76,17 @@ class Oracle:
 
     def set(self, array, a):
         a = np.ascontiguousarray(a, dtype=np.float64)
-        nvar = self._nvar(array)
-        assert a.size == nvar * self.dims.N, (a.shape, nvar)
+        assert a.size == lib().orc_size(self.h, array), (array, a.shape)
         assert lib().orc_set(self.h, array, a.ctypes.data) == 0
 
     def get(self, array):
-        nvar = self._nvar(array)
-        out = np.empty(self.dims.shape(nvar), dtype=np.float64)
+        if array == FST_GRID:
+            shape = self.dims.grid_shape()
+        elif array == FST_METRICS:
+            shape = self.dims.metrics_shape()
+        else:
+            shape = self.dims.shape(self._nvar(array))
+        out = np.empty(shape, dtype=np.float64)
         assert lib().orc_get(self.h, array, out.ctypes.data) == 0
         return out
 

@@ -56,6 +56,11 @@ struct Orc {
   std::vector<double> qx, qy;            // 2D heat flux
   std::vector<double> gradRho;           // (.,5) 3D, (.,4) 2D
   std::vector<double> mFlux;             // 3D (3,3,ngi,ngj,ngk,3), 2D (2,2,ngi,ngj,2)
+  // generalized grids (gen2d/gen3d modules, cf.grid = 1)
+  bool gen = false;
+  int ni = 0, nj = 0, nk = 0;            // grid nodes: nc + 1 (nk = 1 in 2D), src/input.cpp:401-406
+  std::vector<double> grid;              // (ni,nj,nk,3)
+  std::vector<double> metrics;           // (ngi,ngj,ngk,3,3) 3D, (ngi,ngj,2,2) 2D
   double dxmag;                          // src/cart3d.cpp:173 (no sqrt)
   double maxS = 0, maxC = 0, maxCh = 0, alpha = 0;
   // learned C-indicator form: C_hat and direction fields given from outside
@@ -145,9 +150,9 @@ double weno3d(double f1, double f2, double f3, double f4, double f5) {
 
 // calculateFluxesG::operator(), src/flux.hpp:266-309, launched over the tile
 // range [ng-1, n-ng)^3 (src/cart3d.cpp:228-231, 245)
-void flux3d(Orc &o, int v) {
+void flux3d(Orc &o, int v, double dz) {
   const int ng = o.ng;
-  const double dx = o.c.dx, dy = o.c.dy, dz = o.c.dz;
+  const double dx = o.c.dx, dy = o.c.dy;
   double *var = o.var.data() + o.N * v;
   const double *p = o.p.data();
   const double *vel = o.vel.data();
@@ -875,6 +880,215 @@ void ceq2d(Orc &o) {
     }
 }
 
+// ---------------------------------------------------------------- generalized grids
+
+// grid(i,j,k,v) over (ni,nj,nk,3), LayoutLeft (src/gen3d.cpp:41, src/gen2d.cpp:25)
+inline double GR(const Orc &o, int i, int j, int k, int v) {
+  return o.grid[(size_t)i + (size_t)o.ni * ((size_t)j + (size_t)o.nj * ((size_t)k + (size_t)o.nk * (size_t)v))];
+}
+// metrics(i,j,k,m,n) over (ngi,ngj,ngk,3,3) in 3D (src/gen3d.cpp:43) and
+// metrics(i,j,m,n) over (ngi,ngj,2,2) in 2D (src/gen2d.cpp:26)
+inline double &MT(Orc &o, int i, int j, int k, int m, int n) {
+  return o.metrics[o.id(i, j, k) + o.N * ((size_t)m + (size_t)o.ndim * (size_t)n)];
+}
+
+// computeMetrics3D, src/metric.hpp:59-164, over the cell range
+// (src/gen3d.cpp:138-144).  Node offsets use the literal 3 of the reference
+// (ii = i - 3), not ng.
+void metrics3d(Orc &o) {
+#pragma omp parallel for
+  for (int k = o.ng; k < o.ngk - o.ng; ++k)
+    for (int j = o.ng; j < o.ngj - o.ng; ++j)
+      for (int i = o.ng; i < o.ngi - o.ng; ++i) {
+        const int ii = i - 3, jj = j - 3, kk = k - 3;
+        auto g = [&](int a, int b, int c, int v) { return GR(o, a, b, c, v); };
+        double xmx = (g(ii, jj, kk, 0) + g(ii, jj + 1, kk, 0) + g(ii, jj, kk + 1, 0) + g(ii, jj + 1, kk + 1, 0)) / 4.0;
+        double xpx = (g(ii + 1, jj, kk, 0) + g(ii + 1, jj + 1, kk, 0) + g(ii + 1, jj, kk + 1, 0) +
+                      g(ii + 1, jj + 1, kk + 1, 0)) / 4.0;
+        double xmy = (g(ii, jj, kk, 1) + g(ii, jj + 1, kk, 1) + g(ii, jj, kk + 1, 1) + g(ii, jj + 1, kk + 1, 1)) / 4.0;
+        double xpy = (g(ii + 1, jj, kk, 1) + g(ii + 1, jj + 1, kk, 1) + g(ii + 1, jj, kk + 1, 1) +
+                      g(ii + 1, jj + 1, kk + 1, 1)) / 4.0;
+        double xmz = (g(ii, jj, kk, 2) + g(ii, jj + 1, kk, 2) + g(ii, jj, kk + 1, 2) + g(ii, jj + 1, kk + 1, 2)) / 4.0;
+        double xpz = (g(ii + 1, jj, kk, 2) + g(ii + 1, jj + 1, kk, 2) + g(ii + 1, jj, kk + 1, 2) +
+                      g(ii + 1, jj + 1, kk + 1, 2)) / 4.0;
+
+        double zmx = (g(ii, jj, kk, 0) + g(ii + 1, jj, kk, 0) + g(ii, jj + 1, kk, 0) + g(ii + 1, jj + 1, kk, 0)) / 4.0;
+        double zpx = (g(ii, jj, kk + 1, 0) + g(ii + 1, jj, kk + 1, 0) + g(ii, jj + 1, kk + 1, 0) +
+                      g(ii + 1, jj + 1, kk + 1, 0)) / 4.0;
+        double zmy = (g(ii, jj, kk, 1) + g(ii + 1, jj, kk, 1) + g(ii, jj + 1, kk, 1) + g(ii + 1, jj + 1, kk, 1)) / 4.0;
+        double zpy = (g(ii, jj, kk + 1, 1) + g(ii + 1, jj, kk + 1, 1) + g(ii, jj + 1, kk + 1, 1) +
+                      g(ii + 1, jj + 1, kk + 1, 1)) / 4.0;
+        double zmz = (g(ii, jj, kk, 2) + g(ii + 1, jj, kk, 2) + g(ii, jj + 1, kk, 2) + g(ii + 1, jj + 1, kk, 2)) / 4.0;
+        double zpz = (g(ii, jj, kk + 1, 2) + g(ii + 1, jj, kk + 1, 2) + g(ii, jj + 1, kk + 1, 2) +
+                      g(ii + 1, jj + 1, kk + 1, 2)) / 4.0;
+
+        double ymx = (g(ii, jj, kk, 0) + g(ii + 1, jj, kk, 0) + g(ii, jj, kk + 1, 0) + g(ii + 1, jj, kk + 1, 0)) / 4.0;
+        double ypx = (g(ii, jj + 1, kk, 0) + g(ii + 1, jj + 1, kk, 0) + g(ii, jj + 1, kk + 1, 0) +
+                      g(ii + 1, jj + 1, kk + 1, 0)) / 4.0;
+        double ymy = (g(ii, jj, kk, 1) + g(ii + 1, jj, kk, 1) + g(ii, jj, kk + 1, 1) + g(ii + 1, jj, kk + 1, 1)) / 4.0;
+        double ypy = (g(ii, jj + 1, kk, 1) + g(ii + 1, jj + 1, kk, 1) + g(ii, jj + 1, kk + 1, 1) +
+                      g(ii + 1, jj + 1, kk + 1, 1)) / 4.0;
+        double ymz = (g(ii, jj, kk, 2) + g(ii + 1, jj, kk, 2) + g(ii, jj, kk + 1, 2) + g(ii + 1, jj, kk + 1, 2)) / 4.0;
+        double ypz = (g(ii, jj + 1, kk, 2) + g(ii + 1, jj + 1, kk, 2) + g(ii, jj + 1, kk + 1, 2) +
+                      g(ii + 1, jj + 1, kk + 1, 2)) / 4.0;
+
+        double xxi = xpx - xmx, xet = ypx - ymx, xzt = zpx - zmx;
+        double yxi = xpy - xmy, yet = ypy - ymy, yzt = zpy - zmy;
+        double zxi = xpz - xmz, zet = ypz - ymz, zzt = zpz - zmz;
+        double jac = 1.0 / (xxi * (yet * zzt - yzt * zet) + xet * (yzt * zxi - yxi * zzt) +
+                            xzt * (yxi * zet - yet * zxi));
+        MT(o, i, j, k, 0, 0) = jac * (yet * zzt - yzt * zet);
+        MT(o, i, j, k, 0, 1) = jac * (xzt * zet - xet * zzt);
+        MT(o, i, j, k, 0, 2) = jac * (xet * yzt - xzt * yet);
+        MT(o, i, j, k, 1, 0) = jac * (yzt * zxi - yxi * zzt);
+        MT(o, i, j, k, 1, 1) = jac * (xxi * zzt - xzt * zxi);
+        MT(o, i, j, k, 1, 2) = jac * (xzt * yxi - xxi * yzt);
+        MT(o, i, j, k, 2, 0) = jac * (yxi * zet - yet * zxi);
+        MT(o, i, j, k, 2, 1) = jac * (xet * zxi - xxi * zet);
+        MT(o, i, j, k, 2, 2) = jac * (xxi * yet - xet * yxi);
+      }
+}
+
+// computeMetrics2D, src/metric.hpp:21-57, over the cell range (src/gen2d.cpp:192)
+void metrics2d(Orc &o) {
+  for (int j = o.ng; j < o.ngj - o.ng; ++j)
+    for (int i = o.ng; i < o.ngi - o.ng; ++i) {
+      const int ii = i - 3, jj = j - 3;
+      auto g = [&](int a, int b, int v) { return GR(o, a, b, 0, v); };
+      double x_xi = (g(ii + 1, jj, 0) + g(ii + 1, jj + 1, 0)) / 2.0 - (g(ii, jj, 0) + g(ii, jj + 1, 0)) / 2.0;
+      double y_xi = (g(ii + 1, jj, 1) + g(ii + 1, jj + 1, 1)) / 2.0 - (g(ii, jj, 1) + g(ii, jj + 1, 1)) / 2.0;
+      double x_et = (g(ii, jj + 1, 0) + g(ii + 1, jj + 1, 0)) / 2.0 - (g(ii, jj, 0) + g(ii + 1, jj, 0)) / 2.0;
+      double y_et = (g(ii, jj + 1, 1) + g(ii + 1, jj + 1, 1)) / 2.0 - (g(ii, jj, 1) + g(ii + 1, jj, 1)) / 2.0;
+      double jac = 1.0 / (x_xi * y_et - x_et * y_xi);
+      MT(o, i, j, 0, 0, 0) = jac * y_et;
+      MT(o, i, j, 0, 0, 1) = -jac * x_et;
+      MT(o, i, j, 0, 1, 0) = -jac * y_xi;
+      MT(o, i, j, 0, 1, 1) = jac * x_xi;
+    }
+}
+
+// Metric ghost cells of one rank without neighbours: mirror copies, in the
+// reference order -- 3D x-, x+, y-, y+, z-, z+ (src/gen3d.cpp:241-300), 2D
+// y-, y+, x-, x+ (src/gen2d.cpp:285-329) -- each over the full range of the
+// other indices.
+void metric_ghosts(Orc &o) {
+  const int ng = o.ng, nm = o.ndim;
+  auto copy = [&](int id, int jd, int kd, int is, int js, int ks) {
+    for (int m = 0; m < nm; ++m)
+      for (int n = 0; n < nm; ++n) MT(o, id, jd, kd, m, n) = MT(o, is, js, ks, m, n);
+  };
+  auto xm = [&]() {
+    for (int k = 0; k < o.ngk; ++k)
+      for (int j = 0; j < o.ngj; ++j)
+        for (int i = 0; i < ng; ++i) copy(ng - i - 1, j, k, ng + i, j, k);
+  };
+  auto xp = [&]() {
+    for (int k = 0; k < o.ngk; ++k)
+      for (int j = 0; j < o.ngj; ++j)
+        for (int i = 0; i < ng; ++i) copy(o.ngi - 1 - i, j, k, o.nci + i, j, k);
+  };
+  auto ym = [&]() {
+    for (int k = 0; k < o.ngk; ++k)
+      for (int j = 0; j < ng; ++j)
+        for (int i = 0; i < o.ngi; ++i) copy(i, ng - j - 1, k, i, ng + j, k);
+  };
+  auto yp = [&]() {
+    for (int k = 0; k < o.ngk; ++k)
+      for (int j = 0; j < ng; ++j)
+        for (int i = 0; i < o.ngi; ++i) copy(i, o.ngj - 1 - j, k, i, o.ncj + j, k);
+  };
+  if (o.ndim == 3) {
+    xm(); xp(); ym(); yp();
+    for (int k = 0; k < ng; ++k)
+      for (int j = 0; j < o.ngj; ++j)
+        for (int i = 0; i < o.ngi; ++i) copy(i, j, ng - k - 1, i, j, ng + k);
+    for (int k = 0; k < ng; ++k)
+      for (int j = 0; j < o.ngj; ++j)
+        for (int i = 0; i < o.ngi; ++i) copy(i, j, o.ngk - 1 - k, i, j, o.nck + k);
+  } else {
+    ym(); yp(); xm(); xp();
+  }
+}
+
+// computeGenVelocity3D, src/velocity.hpp:23-48: contravariant velocity
+// vel(d) = sum_m metrics(d,m) * m_m / rho, over all cells
+void genvel3d(Orc &o) {
+  const size_t N = o.N;
+#pragma omp parallel for
+  for (int k = 0; k < o.ngk; ++k)
+    for (int j = 0; j < o.ngj; ++j)
+      for (int i = 0; i < o.ngi; ++i) {
+        const size_t c = o.id(i, j, k);
+        const double r = o.rho[c], m0 = o.var[c], m1 = o.var[c + N], m2 = o.var[c + 2 * N];
+        for (int d = 0; d < 3; ++d)
+          o.vel[c + N * d] = (MT(o, i, j, k, d, 0) * m0 / r + MT(o, i, j, k, d, 1) * m1 / r +
+                              MT(o, i, j, k, d, 2) * m2 / r);
+      }
+}
+
+// computeGenVelocity2D, src/velocity.hpp:50-66
+void genvel2d(Orc &o) {
+  const size_t N = o.N;
+  for (int j = 0; j < o.ngj; ++j)
+    for (int i = 0; i < o.ngi; ++i) {
+      const size_t c = o.id(i, j, 0);
+      const double r = o.rho[c], m0 = o.var[c], m1 = o.var[c + N];
+      for (int d = 0; d < 2; ++d)
+        o.vel[c + N * d] = (MT(o, i, j, 0, d, 0) * m0 / r + MT(o, i, j, 0, d, 1) * m1 / r);
+    }
+}
+
+// applyGenPressureGradient3D, src/presgrad.hpp:72-147, over the cell range
+void genpresgrad3d(Orc &o) {
+  const size_t N = o.N;
+  const int ng = o.ng;
+#pragma omp parallel for
+  for (int k = ng; k < o.ngk - ng; ++k)
+    for (int j = ng; j < o.ngj - ng; ++j)
+      for (int i = ng; i < o.ngi - ng; ++i) {
+        auto P = [&](int a, int b, int c) { return o.p[o.id(a, b, c)]; };
+        auto M = [&](int a, int b, int c, int m, int n) { return MT(o, a, b, c, m, n); };
+        double d[3];
+        for (int n = 0; n < 3; ++n) {
+          double dxi = (P(i - 2, j, k) * M(i - 2, j, k, 0, n) - 8.0 * P(i - 1, j, k) * M(i - 1, j, k, 0, n) +
+                        8.0 * P(i + 1, j, k) * M(i + 1, j, k, 0, n) - P(i + 2, j, k) * M(i + 2, j, k, 0, n)) /
+                       (12.0);
+          double det = (P(i, j - 2, k) * M(i, j - 2, k, 1, n) - 8.0 * P(i, j - 1, k) * M(i, j - 1, k, 1, n) +
+                        8.0 * P(i, j + 1, k) * M(i, j + 1, k, 1, n) - P(i, j + 2, k) * M(i, j + 2, k, 1, n)) /
+                       (12.0);
+          double dzt = (P(i, j, k - 2) * M(i, j, k - 2, 2, n) - 8.0 * P(i, j, k - 1) * M(i, j, k - 1, 2, n) +
+                        8.0 * P(i, j, k + 1) * M(i, j, k + 1, 2, n) - P(i, j, k + 2) * M(i, j, k + 2, 2, n)) /
+                       (12.0);
+          d[n] = dxi + det + dzt;
+        }
+        const size_t c = o.id(i, j, k);
+        for (int n = 0; n < 3; ++n) o.dvar[c + N * n] = (o.dvar[c + N * n] - d[n]);
+      }
+}
+
+// applyGenPressureGradient2D, src/presgrad.hpp:149-190, over the cell range
+void genpresgrad2d(Orc &o) {
+  const size_t N = o.N;
+  const int ng = o.ng;
+  for (int j = ng; j < o.ngj - ng; ++j)
+    for (int i = ng; i < o.ngi - ng; ++i) {
+      auto P = [&](int a, int b) { return o.p[o.id(a, b, 0)]; };
+      auto M = [&](int a, int b, int m, int n) { return MT(o, a, b, 0, m, n); };
+      double d[2];
+      for (int n = 0; n < 2; ++n) {
+        double dxi = (P(i - 2, j) * M(i - 2, j, 0, n) - 8.0 * P(i - 1, j) * M(i - 1, j, 0, n) +
+                      8.0 * P(i + 1, j) * M(i + 1, j, 0, n) - P(i + 2, j) * M(i + 2, j, 0, n)) /
+                     (12.0);
+        double det = (P(i, j - 2) * M(i, j - 2, 1, n) - 8.0 * P(i, j - 1) * M(i, j - 1, 1, n) +
+                      8.0 * P(i, j + 1) * M(i, j + 1, 1, n) - P(i, j + 2) * M(i, j + 2, 1, n)) /
+                     (12.0);
+        d[n] = dxi + det;
+      }
+      const size_t c = o.id(i, j, 0);
+      for (int n = 0; n < 2; ++n) o.dvar[c + N * n] = (o.dvar[c + N * n] - d[n]);
+    }
+}
+
 // ---------------------------------------------------------------- ghosts
 
 // bc_gen, src/bc.cpp:33-131, launched on one plane (src/bc.cpp:217-247)
@@ -981,7 +1195,7 @@ void compute3d(Orc &o) {
   eos3d(o);
   vel3d(o);
   for (int v = 0; v < o.nv; ++v) {
-    flux3d(o, v);
+    flux3d(o, v, o.c.dz);
     advect3d(o, v);
   }
   presgrad3d(o);
@@ -1077,6 +1291,30 @@ void features3d(Orc &o, float *F, float *M, float *Dir, float *G1) {
       }
 }
 
+// gen3d_func::compute, src/gen3d.cpp:341-369: no boundary conditions, no
+// viscous, buoyancy or C-equation terms; calculateFluxesG receives cf.dx as
+// its dz argument (:359), and cf.dx = cf.dz = 1 on generalized grids.
+void compute_gen3d(Orc &o) {
+  eos3d(o);
+  genvel3d(o);
+  for (int v = 0; v < o.nv; ++v) {
+    flux3d(o, v, o.c.dx);
+    advect3d(o, v);
+  }
+  genpresgrad3d(o);
+}
+
+// gen2d_func::compute, src/gen2d.cpp:100-131
+void compute_gen2d(Orc &o) {
+  eos2d(o);
+  genvel2d(o);
+  for (int v = 0; v < o.nv; ++v) {
+    flux2d(o, v);
+    advect2d(o, v);
+  }
+  genpresgrad2d(o);
+}
+
 // cart2d_func::compute, src/cart2d.cpp:151-237
 void compute2d(Orc &o) {
   eos2d(o);
@@ -1092,8 +1330,14 @@ void compute2d(Orc &o) {
 }
 
 void compute(Orc &o) {
-  if (o.ndim == 3) compute3d(o);
-  else compute2d(o);
+  if (o.gen) {
+    if (o.ndim == 3) compute_gen3d(o);
+    else compute_gen2d(o);
+  } else if (o.ndim == 3) {
+    compute3d(o);
+  } else {
+    compute2d(o);
+  }
 }
 
 // rkAdvance stage updates, src/rk.cpp:42-50 (stage 1) and :63-69 (stage 2)
@@ -1113,7 +1357,7 @@ void rk_stage(Orc &o, int stage) {
 }
 
 void pre_step(Orc &o) {
-  if (o.ndim == 2) apply_bcs(o);  // src/cart2d.cpp:147-149
+  if (o.ndim == 2 && !o.gen) apply_bcs(o);  // src/cart2d.cpp:147-149; gen2d/gen3d: nothing
 }
 
 // secondary variables: calculateRhoPT + computeVelocity + copyExtraVars
@@ -1131,6 +1375,12 @@ void secondary(Orc &o) {
 }
 
 void pre_sim(Orc &o) {
+  if (o.gen) {  // gen3d_func::preSim (src/gen3d.cpp:131-304), gen2d_func::preSim (src/gen2d.cpp:187-331)
+    if (o.ndim == 3) metrics3d(o);
+    else metrics2d(o);
+    metric_ghosts(o);
+    return;
+  }
   if (o.ndim == 2) apply_bcs(o);  // src/cart2d.cpp:137
   secondary(o);
 }
@@ -1147,6 +1397,8 @@ std::vector<double> *array_of(Orc &o, int a, size_t *count) {
     case FST_T: v = &o.T; break;
     case FST_VEL: v = &o.vel; break;
     case FST_GRADRHO: v = &o.gradRho; break;
+    case FST_GRID: v = &o.grid; break;
+    case FST_METRICS: v = &o.metrics; break;
     default: return nullptr;
   }
   if (count) *count = v->size();
@@ -1173,7 +1425,12 @@ void *orc_create(const fst_config *cfg) {
   o->learned = cfg->learned_c != 0 && cfg->ceq;
   o->nvt = o->nv + (cfg->ceq && !o->learned ? 5 : 0);
   o->nvx = o->ndim + 3;
-  if (cfg->ndim == 2) o->c.dz = cfg->dx;  // src/input.cpp:356
+  o->gen = cfg->grid == 1;
+  if (o->gen) o->c.dx = o->c.dy = o->c.dz = 1.0;  // src/input.cpp:344-350
+  if (cfg->ndim == 2) o->c.dz = o->c.dx;  // src/input.cpp:356
+  o->ni = o->nci + 1;
+  o->nj = o->ncj + 1;
+  o->nk = cfg->ndim == 3 ? o->nck + 1 : 1;
   o->N = (size_t)o->ngi * o->ngj * o->ngk;
   const size_t N = o->N;
   o->cd.assign(6 + 3 * o->ns, 0.0);
@@ -1212,6 +1469,10 @@ void *orc_create(const fst_config *cfg) {
   if (cfg->ceq) {
     if (!o->learned) o->gradRho.assign(N * (o->ndim == 3 ? 5 : 4), 0.0);
     o->mFlux.assign(o->ndim == 3 ? 27 * N : 8 * N, 0.0);
+  }
+  if (o->gen) {
+    o->grid.assign((size_t)o->ni * o->nj * o->nk * 3, 0.0);
+    o->metrics.assign(N * o->ndim * o->ndim, 0.0);
   }
   if (o->learned) {
     o->lchat.assign(N, 0.0);

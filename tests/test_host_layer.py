@@ -65,3 +65,31 @@ def test_driver_steps_equal_oracle(tmp_path, ndim, mode):
     d = Dims(cfg)
     sl = d.interior()
     assert np.array_equal(got[sl], want[sl])
+
+
+@pytest.mark.gpu
+def test_driver_gen3d_module_equals_oracle(tmp_path):
+    """gen3d_func through fiesta_run: grid upload, preSim metrics, rk sequence."""
+    import oracle
+    from fiesta_amd.config import FST_GRID, make_config
+    print("case: fiesta_run gen3d rk", flush=True)
+    cfg = make_config(ndim=3, ns=2, nci=18, ncj=11, nck=9, grid=1, gamma=(1.4, 1.67), M=(0.02896, 0.004),
+                      mu=(0.0, 0.0), dt=2e-7)
+    nodes = W.terrain_grid(cfg)
+    var = W.random_state_all(cfg, seed=6, umag=20.0)
+    gridf = tmp_path / "grid.bin"
+    gridf.write_bytes(np.ascontiguousarray(nodes, np.float64).tobytes())
+    cfgf, stf, outf = tmp_path / "cfg.bin", tmp_path / "state.bin", tmp_path / "out.bin"
+    cfgf.write_bytes(bytes(cfg))
+    stf.write_bytes(np.ascontiguousarray(var, np.float64).tobytes())
+    r = subprocess.run([RUN, str(cfgf), str(stf), str(outf), "3", "rk", str(gridf)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.frombuffer(outf.read_bytes(), np.float64).reshape(var.shape)
+    o = oracle.Oracle(cfg)
+    o.set(FST_GRID, nodes)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.step(3)
+    assert np.array_equal(got, o.get(FST_VAR))
+

@@ -1,0 +1,187 @@
+"""Pins of the oracle's generalized-grid modules (gen2d/gen3d) against closed forms.
+
+The reference has no tests for these modules, so each function is checked
+against what the mathematics fixes, chosen so that a transposed metric, a
+wrong node offset, a swapped mirror index or a misplaced term fails:
+
+  computeMetrics3D/2D (src/metric.hpp:21-164)  affine grids x = A xi + b give
+                                               metrics = inv(A) (A not symmetric);
+                                               separable stretched grids give
+                                               diag(1/(x_{i+1}-x_i), ...)
+  metric ghosts (src/gen3d.cpp:241-300,        separable grid: every ghost cell,
+                 src/gen2d.cpp:285-329)        corners included, holds the value of
+                                               its mirror image in each direction
+  computeGenVelocity3D/2D (src/velocity.hpp:23-66)  affine grid: inv(A) u
+  applyGenPressureGradient3D/2D                fluid at rest, p linear in x:
+    (src/presgrad.hpp:72-190)                  momentum RHS = -grad p, rest 0
+  gen3d/gen2d compute (src/gen3d.cpp:341-369,  on a uniform grid of spacing 2^-5
+    src/gen2d.cpp:100-131)                     the RHS equals the cartesian
+                                               module's bit for bit (the metrics
+                                               are exact powers of two), which
+                                               ties the generalized path to the
+                                               cartesian one pinned elsewhere
+"""
+import numpy as np
+import pytest
+
+import oracle
+from fiesta_amd import workloads as W
+from fiesta_amd.config import FST_DVAR, FST_GRID, FST_METRICS, FST_VAR, FST_VEL, Dims, make_config
+
+RU = 8.314462
+
+
+def _gen_cfg(ndim, n=(9, 7, 6), ns=1, scheme="weno5"):
+    return make_config(ndim=ndim, ns=ns, nci=n[0], ncj=n[1], nck=n[2], grid=1, R=RU,
+                       gamma=(1.4, 1.67)[:ns], M=(0.02896, 0.004)[:ns], mu=(0.0, 0.0)[:ns], scheme=scheme,
+                       dt=1e-7)
+
+
+def _affine(ndim, seed):
+    rng = np.random.default_rng(seed)
+    A = np.eye(ndim) * 0.02 + rng.uniform(-0.006, 0.006, (ndim, ndim))
+    b = rng.uniform(-1, 1, ndim)
+    return A, b
+
+
+def _affine_nodes(cfg, A, b):
+    def fn(i, j, k):
+        xi = [i, j, k][: cfg.ndim]
+        out = [b[r] + sum(A[r, c] * xi[c] for c in range(cfg.ndim)) for r in range(cfg.ndim)]
+        return out[0], out[1], (out[2] if cfg.ndim == 3 else None)
+    return W.grid_nodes(cfg, fn)
+
+
+def _metrics(cfg, nodes):
+    o = oracle.Oracle(cfg)
+    o.set(FST_GRID, nodes)
+    o.pre_sim()
+    return o, o.get(FST_METRICS)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_metrics_affine_grid_is_inverse_jacobian(ndim):
+    cfg = _gen_cfg(ndim)
+    A, b = _affine(ndim, 11 + ndim)
+    _, mt = _metrics(cfg, _affine_nodes(cfg, A, b))
+    inv = np.linalg.inv(A)
+    # metrics(i,j,k,m,n) = d xi_m / d x_n = inv(A)[m, n], in every cell (ghosts are mirror copies)
+    for m in range(ndim):
+        for n in range(ndim):
+            np.testing.assert_allclose(mt[n, m], inv[m, n], rtol=1e-11, atol=1e-9 * np.abs(inv).max())
+    assert not np.allclose(inv, inv.T, rtol=1e-3)  # a transposed metric would fail
+
+
+def _mirror(n_int, ng):
+    """Index map of a ghosted axis: interior cells map to themselves, ghost
+    layer a on each side to its mirror image ng-1-a <- ng+a, ngi-1-a <- nci+a."""
+    m = np.arange(n_int + 2 * ng)
+    for a in range(ng):
+        m[ng - 1 - a] = ng + a
+        m[n_int + 2 * ng - 1 - a] = n_int + a
+    return m
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_metrics_separable_stretch_and_mirrored_ghosts(ndim):
+    cfg = _gen_cfg(ndim, n=(8, 6, 5))
+    d = Dims(cfg)
+    f = lambda t: 0.1 * t + 0.013 * t * t          # noqa: E731  strictly increasing node maps
+    g = lambda t: 0.07 * t + 0.02 * np.sqrt(t + 1.0)  # noqa: E731
+    h = lambda t: 0.05 * t + 0.004 * t ** 3       # noqa: E731
+    nodes = W.grid_nodes(cfg, lambda i, j, k: (f(i), g(j), h(k) if ndim == 3 else None))
+    _, mt = _metrics(cfg, nodes)
+    ng = d.ng
+    spans = [lambda c: f(c - 2) - f(c - 3), lambda c: g(c - 2) - g(c - 3), lambda c: h(c - 2) - h(c - 3)]
+    mir = [_mirror(d.nci, ng), _mirror(d.ncj, ng), _mirror(d.nck, ng)]
+    # diagonal entry m(a,a) in cell (i,j,k) = 1 / node span of axis a at the mirrored index
+    for a in range(ndim):
+        idx = np.arange([d.ngi, d.ngj, d.ngk][a])
+        want = 1.0 / spans[a](mir[a][idx].astype(np.float64))
+        shape = [1, 1, 1]
+        shape[2 - a] = -1
+        full = np.broadcast_to(want.reshape(shape), (d.ngk, d.ngj, d.ngi))
+        np.testing.assert_allclose(mt[a, a], full, rtol=1e-12)
+        for b in range(ndim):
+            if b != a:
+                assert np.all(mt[b, a] == 0.0)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_contravariant_velocity_affine(ndim):
+    cfg = _gen_cfg(ndim, ns=2)
+    d = Dims(cfg)
+    A, b = _affine(ndim, 5)
+    o, _ = _metrics(cfg, _affine_nodes(cfg, A, b))
+    var = W.random_state_all(cfg, seed=4)
+    o.set(FST_VAR, var)
+    o.compute()
+    vel = o.get(FST_VEL)
+    rho = var[ndim + 1:ndim + 1 + cfg.ns].sum(axis=0)
+    u = var[:ndim] / rho
+    want = np.einsum("mn,nkji->mkji", np.linalg.inv(A), u)
+    np.testing.assert_allclose(vel, want, rtol=1e-10, atol=1e-9 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_metric_pressure_gradient_linear_pressure_at_rest(ndim):
+    cfg = _gen_cfg(ndim, ns=1)
+    d = Dims(cfg)
+    A, b = _affine(ndim, 9)
+    nodes = _affine_nodes(cfg, A, b)
+    o, _ = _metrics(cfg, nodes)
+    # cell centres of the affine grid: node map at (cell index - ng + 1/2)
+    idx = [np.arange(n) - d.ng + 0.5 for n in (d.ngi, d.ngj, d.ngk)]
+    K, J, I = np.meshgrid(idx[2], idx[1], idx[0], indexing="ij")
+    xi = [I, J, K][:ndim]
+    x = [b[r] + sum(A[r, c] * xi[c] for c in range(ndim)) for r in range(ndim)]
+    grad = np.array([2.0e3, -1.5e3, 0.7e3][:ndim])
+    p = 1.0e5 + sum(grad[r] * x[r] for r in range(ndim))
+    rho = np.full_like(p, 1.1)
+    var = W.fill_conserved(cfg, [rho], [np.zeros_like(p)] * ndim, p, ghosts=True)
+    o.set(FST_VAR, var)
+    o.compute()
+    dv = o.get(FST_DVAR)[d.interior()]
+    for n in range(ndim):
+        np.testing.assert_allclose(dv[n], -grad[n], rtol=2e-7)
+    assert np.all(dv[ndim:] == 0.0)  # no velocity: zero energy and species flux
+
+
+@pytest.mark.parametrize("ndim,scheme", [(2, "weno5"), (2, "quick"), (3, "weno5")])
+def test_gen_rhs_equals_cartesian_on_uniform_power_of_two_grid(ndim, scheme):
+    h = 2.0 ** -5
+    n = (10, 8, 7)
+    kw = dict(ndim=ndim, ns=2, nci=n[0], ncj=n[1], nck=n[2], R=RU, gamma=(1.4, 1.67), M=(0.02896, 0.004),
+              mu=(0.0, 0.0), scheme=scheme, dt=1e-7)
+    cart = make_config(dx=h, dy=h, dz=h, xper=True, yper=True, zper=True, **kw)
+    gen = make_config(grid=1, **kw)
+    var = W.random_state(cart, seed=21)
+    oc = oracle.Oracle(cart)
+    oc.set(FST_VAR, var)
+    oc.apply_bcs()  # periodic ghosts: cart3d's own ghost fill then leaves them unchanged
+    var = oc.get(FST_VAR)
+    oc.compute()
+    nodes = W.grid_nodes(gen, lambda i, j, k: (h * i, h * j, h * k if ndim == 3 else None))
+    og, mt = _metrics(gen, nodes)
+    assert np.all(mt[0, 0] == 1.0 / h)
+    og.set(FST_VAR, var)
+    og.compute()
+    d = Dims(cart)
+    a = oc.get(FST_DVAR)[d.interior()]
+    b = og.get(FST_DVAR)[d.interior()]
+    assert np.array_equal(a, b)
+
+
+def test_gen_modules_leave_ghosts_untouched():
+    cfg = _gen_cfg(3, ns=2)
+    o, _ = _metrics(cfg, W.terrain_grid(cfg))
+    var = W.random_state_all(cfg, seed=2, umag=5.0)
+    o.set(FST_VAR, var)
+    o.step(2)
+    out = o.get(FST_VAR)
+    d = Dims(cfg)
+    ghost = np.ones(out.shape[1:], bool)
+    ghost[d.interior()[1:]] = False
+    assert np.array_equal(out[:, ghost], var[:, ghost])
+    assert np.all(np.isfinite(out))
+    assert not np.array_equal(out[d.interior()], var[d.interior()])
