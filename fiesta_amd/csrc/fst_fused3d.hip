@@ -626,10 +626,9 @@ __global__ void __launch_bounds__(NTHREADS, 1)
   using A = Ar<F32>;
   using R = typename A::R;
   constexpr int NV = 4 + NS;
-  constexpr int NVT = LC ? NV : NV + 5;
   __shared__ R xsh[TY][NFV][TX + 1];
   __shared__ R ysh[NWAVE][NFV][TX];
-  __shared__ double red[2][NWAVE];
+  __shared__ double red[NWAVE];
   const Tile t = place(g, kchunk, nbx, nby, nbz);
   const int w = t.w, l = t.l;
   const uint32_t N = g.N;
@@ -640,7 +639,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
 #pragma unroll
   for (int q = 0; q < NFV; ++q) Z[q] = Zp[q] = (R)0.0;
   if (faces && t.cellok) face_vc<2, NS, F32, LC>(k, g, U, P, CH, DR, alpha, visc, ceq, g.id(t.i, t.jrow, t.kb - 1), Zp);
-  double mS = 0.0, mC = 0.0;
+  double mS = 0.0;
   for (int kk = t.kb; kk < t.ke; ++kk) {
     uint32_t c = g.id(t.i, t.jrow, kk);
     uint32_t cxb = g.id(t.i0 - 1, t.j0 + l, kk);
@@ -668,7 +667,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       __syncthreads();
     }
     if (t.cellok) {
-      double dv[NVT];
+      double dv[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) dv[v] = ld_dv<F32>(dvar, c + N * v);
       auto PV = [&](uint32_t off) { return (R)ld(P, off); };
@@ -707,47 +706,6 @@ __global__ void __launch_bounds__(NTHREADS, 1)
           dv[3] += td;
         }
       }
-      if (ceq && !LC) {
-        // calculateRhoGrad, src/ceq3d.hpp:143-186 (rho, nrg, vel planes of P)
-        R G[5];
-        {
-          R dr[3], de[3], du[3];
-#pragma unroll
-          for (int D = 0; D < 3; ++D) {
-            const uint32_t e = D == 0 ? 1u : (D == 1 ? g.s1 : g.s2);
-            const uint32_t rb = 0, eb = 5 * N, vb = (2 + D) * N;
-            dr[D] = A::cdv(PV(c - 2 * e + rb) - (R)8.0 * PV(c - e + rb) + (R)8.0 * PV(c + e + rb) -
-                               PV(c + 2 * e + rb), k.d12[D], k.rd12[D]);
-            de[D] = A::cdv(PV(c - 2 * e + eb) - (R)8.0 * PV(c - e + eb) + (R)8.0 * PV(c + e + eb) -
-                               PV(c + 2 * e + eb), k.d12[D], k.rd12[D]);
-            du[D] = A::cdv(PV(c - 2 * e + vb) - (R)8.0 * PV(c - e + vb) + (R)8.0 * PV(c + e + vb) -
-                               PV(c + 2 * e + vb), k.d12[D], k.rd12[D]);
-          }
-          const R rgrad = A::sq(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-          const R divu = du[0] + du[1] + du[2];
-          const R dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) *
-                           (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-          const int ind = dnednr < (R)0.0 ? 1 : 0;
-          G[0] = (divu < 0) ? (R)(1 - ind) * rgrad : (R)0;
-          G[1] = (R)ind * rgrad;
-          G[2] = dr[0];
-          G[3] = dr[1];
-          G[4] = dr[2];
-        }
-        // updateCeq, src/ceq3d.hpp:202-223
-        const R c1 = (R)(maxS / (k.eps * k.dxmag));
-        const R c2 = (R)(k.kap * maxS * k.dxmag);
-#pragma unroll
-        for (int n = 0; n < 5; ++n) {
-          const uint32_t cb = N * (NV + n);
-          auto CU = [&](uint32_t off) { return (R)ld(U, off); };
-          const R cc = CU(c + cb);
-          const R lap = A::cdv(CU(c - 1 + cb) - (R)2 * cc + CU(c + 1 + cb), k.d[0], k.rd[0]) +
-                        A::cdv(CU(c - g.s1 + cb) - (R)2 * cc + CU(c + g.s1 + cb), k.d[1], k.rd[1]) +
-                        A::cdv(CU(c - g.s2 + cb) - (R)2 * cc + CU(c + g.s2 + cb), k.d[2], k.rd[2]);
-          dv[NV + n] = c1 * (G[n] - cc) + c2 * lap;
-        }
-      }
       if (ceq) {
         // applyCeq, src/ceq3d.hpp:343-365
 #pragma unroll
@@ -765,16 +723,14 @@ __global__ void __launch_bounds__(NTHREADS, 1)
           dv[ww] += diffu;
         }
       }
-      // Runge-Kutta stage update and primitives of the new state (FP64)
-      const int nvt = g.nvt;
-      double un[NVT];
+      // Runge-Kutta stage update of the flow variables and primitives of
+      // the new state (FP64); the C variables are updated by k_ceq3d
+      double un[NV];
 #pragma unroll
-      for (int v = 0; v < NVT; ++v) {
-        if (v < nvt) {
-          const double base = (stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v);
-          un[v] = base + h * dv[v];
-          st(Uo, c + N * v, un[v]);
-        }
+      for (int v = 0; v < NV; ++v) {
+        const double base = (stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v);
+        un[v] = base + h * dv[v];
+        st(Uo, c + N * v, un[v]);
       }
       double pr[6 + NS];
       double ms;
@@ -782,7 +738,6 @@ __global__ void __launch_bounds__(NTHREADS, 1)
 #pragma unroll
       for (int q = 0; q < 6 + NS; ++q) st(Po, c + N * q, pr[q]);
       mS = fmax(mS, ms);
-      if (ceq && !LC) mC = fmax(mC, fabs(un[NV + 1]));
     }
 #pragma unroll
     for (int q = 0; q < NFV; ++q) Zp[q] = Z[q];
@@ -790,20 +745,12 @@ __global__ void __launch_bounds__(NTHREADS, 1)
   }
   // block maxima -> one atomic per block and value
   mS = wave_max(mS);
-  mC = wave_max(mC);
-  if (l == 0) {
-    red[0][w] = mS;
-    red[1][w] = mC;
-  }
+  if (l == 0) red[w] = mS;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double s = 0.0, cm = 0.0;
-    for (int q = 0; q < NWAVE; ++q) {
-      s = fmax(s, red[0][q]);
-      cm = fmax(cm, red[1][q]);
-    }
+    double s = 0.0;
+    for (int q = 0; q < NWAVE; ++q) s = fmax(s, red[q]);
     atomic_max_nonneg(&scout->maxS, s);
-    atomic_max_nonneg(&scout->maxCh, cm);
   }
 }
 
@@ -986,11 +933,10 @@ __global__ void __launch_bounds__(NTHREADS, 1)
   using RF = RingF<NS, F32, LC>;
   using R = typename A::R;
   constexpr int NV = 4 + NS;
-  constexpr int NVT = LC ? NV : NV + 5;
   __shared__ R ring[NSLOT * RF::NF * RPL];
   __shared__ R xsh[TY][NFV][TX + 1];
   __shared__ R ysh[NWAVE][NFV][TX];
-  __shared__ double red[2][NWAVE];
+  __shared__ double red[NWAVE];
   const Tile t = place(g, kchunk, nbx, nby, nbz);
   const int w = t.w, l = t.l;
   const uint32_t N = g.N;
@@ -1017,7 +963,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     at.c = g.id(t.i, t.jrow, t.kb - 1);
     face_vc_ring<2, NS, F32, LC>(k, at, alpha, visc, ceq, Zp);
   }
-  double mS = 0.0, mC = 0.0;
+  double mS = 0.0;
   for (int kk = t.kb; kk < t.ke; ++kk) {
     uint32_t c = g.id(t.i, t.jrow, kk);
     uint32_t cxb = g.id(t.i0 - 1, t.j0 + l, kk);
@@ -1027,13 +973,12 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     // learned FP32 form (registers to spare): the cell's RHS and stage base
     // are loaded before the face phase so their latency overlaps it
     constexpr bool EARLY = F32 && LC;
-    double dvp[EARLY ? NV : 1], bsp[EARLY ? NVT : 1];
+    double dvp[EARLY ? NV : 1], bsp[EARLY ? NV : 1];
     if (EARLY && t.cellok) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) dvp[v] = ld_dv<F32>(dvar, c + N * v);
 #pragma unroll
-      for (int v = 0; v < NVT; ++v)
-        if (v < g.nvt) bsp[v] = (stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v);
+      for (int v = 0; v < NV; ++v) bsp[v] = (stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v);
     }
     at.sl[0] = (kk - 1) & 3; at.sl[1] = kk & 3; at.sl[2] = (kk + 1) & 3;
     if (faces) {
@@ -1067,7 +1012,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     __syncthreads();
     if (t.cellok) {
       at.r = w + 1; at.q = l + 1; at.c = c;
-      double dv[NVT];
+      double dv[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) dv[v] = EARLY ? dvp[EARLY ? v : 0] : ld_dv<F32>(dvar, c + N * v);
       auto PV = [&](uint32_t off) { return (R)ld(P, off); };
@@ -1105,46 +1050,6 @@ __global__ void __launch_bounds__(NTHREADS, 1)
           dv[3] += td;
         }
       }
-      if (ceq && !LC) {
-        // calculateRhoGrad, src/ceq3d.hpp:143-186 (rho, nrg, vel planes of P)
-        R G[5];
-        {
-          R dr[3], de[3], du[3];
-#pragma unroll
-          for (int D = 0; D < 3; ++D) {
-            const uint32_t e = D == 0 ? 1u : (D == 1 ? g.s1 : g.s2);
-            const uint32_t rb = 0, eb = 5 * N, vb = (2 + D) * N;
-            dr[D] = A::cdv(PV(c - 2 * e + rb) - (R)8.0 * PV(c - e + rb) + (R)8.0 * PV(c + e + rb) -
-                               PV(c + 2 * e + rb), k.d12[D], k.rd12[D]);
-            de[D] = A::cdv(PV(c - 2 * e + eb) - (R)8.0 * PV(c - e + eb) + (R)8.0 * PV(c + e + eb) -
-                               PV(c + 2 * e + eb), k.d12[D], k.rd12[D]);
-            du[D] = A::cdv(PV(c - 2 * e + vb) - (R)8.0 * PV(c - e + vb) + (R)8.0 * PV(c + e + vb) -
-                               PV(c + 2 * e + vb), k.d12[D], k.rd12[D]);
-          }
-          const R rgrad = A::sq(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-          const R divu = du[0] + du[1] + du[2];
-          const R dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) *
-                           (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-          const int ind = dnednr < (R)0.0 ? 1 : 0;
-          G[0] = (divu < 0) ? (R)(1 - ind) * rgrad : (R)0;
-          G[1] = (R)ind * rgrad;
-          G[2] = dr[0];
-          G[3] = dr[1];
-          G[4] = dr[2];
-        }
-        // updateCeq, src/ceq3d.hpp:202-223 (C variables from the ring)
-        const R c1 = (R)(maxS / (k.eps * k.dxmag));
-        const R c2 = (R)(k.kap * maxS * k.dxmag);
-#pragma unroll
-        for (int n = 0; n < 5; ++n) {
-          const int f = RF::FC + n;
-          const R cc = at(f, 0, 0, 0);
-          const R lap = A::cdv(at(f, 0, 0, -1) - (R)2 * cc + at(f, 0, 0, 1), k.d[0], k.rd[0]) +
-                        A::cdv(at(f, 0, -1, 0) - (R)2 * cc + at(f, 0, 1, 0), k.d[1], k.rd[1]) +
-                        A::cdv(at(f, -1, 0, 0) - (R)2 * cc + at(f, 1, 0, 0), k.d[2], k.rd[2]);
-          dv[NV + n] = c1 * (G[n] - cc) + c2 * lap;
-        }
-      }
       if (ceq) {
         // applyCeq, src/ceq3d.hpp:343-365
 #pragma unroll
@@ -1162,16 +1067,14 @@ __global__ void __launch_bounds__(NTHREADS, 1)
           dv[ww] += diffu;
         }
       }
-      // Runge-Kutta stage update and primitives of the new state (FP64)
-      const int nvt = g.nvt;
-      double un[NVT];
+      // Runge-Kutta stage update of the flow variables and primitives of
+      // the new state (FP64); the C variables are updated by k_ceq3d
+      double un[NV];
 #pragma unroll
-      for (int v = 0; v < NVT; ++v) {
-        if (v < nvt) {
-          const double base = EARLY ? bsp[EARLY ? v : 0] : ((stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v));
-          un[v] = base + h * dv[v];
-          st(Uo, c + N * v, un[v]);
-        }
+      for (int v = 0; v < NV; ++v) {
+        const double base = EARLY ? bsp[EARLY ? v : 0] : ((stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v));
+        un[v] = base + h * dv[v];
+        st(Uo, c + N * v, un[v]);
       }
       double pr[6 + NS];
       double ms;
@@ -1179,28 +1082,195 @@ __global__ void __launch_bounds__(NTHREADS, 1)
 #pragma unroll
       for (int q = 0; q < 6 + NS; ++q) st(Po, c + N * q, pr[q]);
       mS = fmax(mS, ms);
-      if (ceq && !LC) mC = fmax(mC, fabs(un[NV + 1]));
     }
 #pragma unroll
     for (int q = 0; q < NFV; ++q) Zp[q] = Z[q];
     __syncthreads();
   }
   mS = wave_max(mS);
-  mC = wave_max(mC);
-  if (l == 0) {
-    red[0][w] = mS;
-    red[1][w] = mC;
-  }
+  if (l == 0) red[w] = mS;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double s = 0.0, cm = 0.0;
-    for (int q = 0; q < NWAVE; ++q) {
-      s = fmax(s, red[0][q]);
-      cm = fmax(cm, red[1][q]);
-    }
+    double s = 0.0;
+    for (int q = 0; q < NWAVE; ++q) s = fmax(s, red[q]);
     atomic_max_nonneg(&scout->maxS, s);
-    atomic_max_nonneg(&scout->maxCh, cm);
   }
+}
+
+// ---------------------------------------------------------------- kernel 3: C-equation sources
+
+// calculateRhoGrad (src/ceq3d.hpp:143-186) and updateCeq (src/ceq3d.hpp:202-223)
+// with the Runge-Kutta stage update of the five C variables (src/rk.cpp:42-50 /
+// :63-69), and the maximum |C_hat| of the new state.  updateCeq alone assigns
+// the C variables' right-hand side, so this pass is independent of the flow
+// kernels of the stage (which read U's C variables, never Uo's).
+//
+// A block is 64 (i) x CY (j) columns marching k.  The x/y stencil fields of
+// the current plane -- rho, nrg, u, v (4th-order, 2 cells of halo) and the C
+// variables (Laplacian, 1 cell) -- are staged in LDS for rows j0-2..j0+CY+1
+// and columns i0-2..i0+65, the next plane being loaded into registers during
+// the current one.  The z stencils -- rho, nrg, w at k-2..k+2 and the C
+// variables at k-1..k+1 -- stay in per-column register queues, loaded one
+// plane ahead.  Same operations as the reference sequence, so the FP64 form
+// stays bitwise equal.
+constexpr int CY = 8, CNT = 64 * CY;
+constexpr int CPR = CY + 4, CPX = 64 + 4, CPL = CPR * CPX;  // staged plane: rows, columns, cells
+constexpr int CNF = 9;                                       // rho, nrg, u, v, C0..C4
+constexpr int CNPRE = (CNF * CPL + CNT - 1) / CNT;
+
+template <bool F32>
+__device__ __forceinline__ void ceq_fetch(const Grid32 &g, const double *__restrict__ U, const double *__restrict__ P,
+                                          int i0, int j0, int kk, typename Ar<F32>::R *pre) {
+  using R = typename Ar<F32>::R;
+  const uint32_t N = g.N;
+#pragma unroll
+  for (int m = 0; m < CNPRE; ++m) {
+    const int e = threadIdx.x + m * CNT;
+    R v = (R)0;
+    if (e < CNF * CPL) {
+      const int f = e / CPL, rem = e - f * CPL;
+      const int r = rem / CPX, q = rem - r * CPX;
+      const int i = i0 - 2 + q, j = j0 - 2 + r;
+      if (i < g.ngi && j < g.ngj) {
+        const uint32_t c = g.id(i, j, kk);
+        // P planes: rho 0, u 2, v 3, nrg 5; then the C variables of U
+        const uint32_t off = f == 0 ? c : f == 1 ? c + 5 * N : f == 2 ? c + 2 * N : f == 3 ? c + 3 * N : 0u;
+        v = f < 4 ? (R)ld(P, off) : (R)ld(U, c + N * (uint32_t)(g.nv + f - 4));
+      }
+    }
+    pre[m] = v;
+  }
+}
+
+template <bool F32>
+__global__ void __launch_bounds__(CNT)
+    k_ceq3d(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ Un,
+            double *__restrict__ Uo, const double *__restrict__ P, const Scalars *scin, Scalars *scout, double h,
+            int stage, int kchunk, int nbx, int nby, int nbz) {
+  using A = Ar<F32>;
+  using R = typename A::R;
+  __shared__ R pl[CNF * CPL];
+  const int nb = nbx * nby * nbz;
+  const int hb = blockIdx.x;
+  int lb = hb;
+  const int full = (nb / 8) * 8;
+  if (hb < full) lb = (hb % 8) * (full / 8) + hb / 8;  // consecutive tiles on one XCD
+  const int bx = lb % nbx, by = (lb / nbx) % nby, bz = lb / (nbx * nby);
+  const int i0 = g.ng + bx * 64, j0 = g.ng + by * CY;
+  const int l = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+  const int i = i0 + l, j = j0 + w;
+  const int kb = g.ng + bz * kchunk, ke = min(kb + kchunk, g.ng + g.nck);
+  const bool ok = i < g.ng + g.nci && j < g.ng + g.ncj;
+  const uint32_t N = g.N, s2 = g.s2;
+  const uint32_t nvb = N * (uint32_t)g.nv;
+  const double maxS = as_double(scin->maxS);
+  const R c1 = (R)(maxS / (k.eps * k.dxmag));
+  const R c2 = (R)(k.kap * maxS * k.dxmag);
+  auto PV = [&](uint32_t off) { return (R)ld(P, off); };
+  auto CU = [&](uint32_t off) { return (R)ld(U, off); };
+  const uint32_t rb = 0, eb = 5 * N, wb = 4 * N;
+  // queues: [0..4] = planes k-2..k+2 of rho, nrg, w; qc[0..2] = planes k-1..k+1 of C
+  R qr[5], qe[5], qw[5], qc[3][5], nr = 0, ne = 0, nw = 0, nc[5];
+  uint32_t c = g.id(min(i, g.ngi - 1), min(j, g.ngj - 1), kb);
+  if (ok) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t cc = c + (uint32_t)q * s2 - 2u * s2;
+      qr[q] = PV(cc + rb);
+      qe[q] = PV(cc + eb);
+      qw[q] = PV(cc + wb);
+    }
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      qc[0][n] = CU(c - s2 + nvb + N * n);
+      qc[1][n] = CU(c + nvb + N * n);
+    }
+    nr = PV(c + 2 * s2 + rb);
+    ne = PV(c + 2 * s2 + eb);
+    nw = PV(c + 2 * s2 + wb);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) nc[n] = CU(c + s2 + nvb + N * n);
+  }
+  R pre[CNPRE];
+  ceq_fetch<F32>(g, U, P, i0, j0, kb, pre);
+  double mC = 0.0;
+  for (int kk = kb; kk < ke; ++kk, c += s2) {
+    __syncthreads();  // the previous plane's readers are done
+#pragma unroll
+    for (int m = 0; m < CNPRE; ++m) {
+      const int e = threadIdx.x + m * CNT;
+      if (e < CNF * CPL) pl[e] = pre[m];
+    }
+    __syncthreads();
+    if (kk + 1 < ke) ceq_fetch<F32>(g, U, P, i0, j0, kk + 1, pre);
+    if (ok) {
+      qr[4] = nr;
+      qe[4] = ne;
+      qw[4] = nw;
+#pragma unroll
+      for (int n = 0; n < 5; ++n) qc[2][n] = nc[n];
+      double base[5];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) base[n] = (stage == 1) ? ld(U, c + nvb + N * n) : ld(Un, c + nvb + N * n);
+      if (kk + 1 < ke) {  // z-queue entries of the next plane
+        nr = PV(c + 3 * s2 + rb);
+        ne = PV(c + 3 * s2 + eb);
+        nw = PV(c + 3 * s2 + wb);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) nc[n] = CU(c + 2 * s2 + nvb + N * n);
+      }
+      const R *pc = pl + (w + 2) * CPX + (l + 2);  // this column's cell in field 0
+      auto L = [&](int f, int dr, int dq) { return pc[f * CPL + dr * CPX + dq]; };
+      // calculateRhoGrad, src/ceq3d.hpp:143-186: 4th-order gradients of rho
+      // and nrg, divergence of u
+      R dr[3], de[3], du[3];
+      dr[0] = A::cdv(L(0, 0, -2) - (R)8.0 * L(0, 0, -1) + (R)8.0 * L(0, 0, 1) - L(0, 0, 2), k.d12[0], k.rd12[0]);
+      de[0] = A::cdv(L(1, 0, -2) - (R)8.0 * L(1, 0, -1) + (R)8.0 * L(1, 0, 1) - L(1, 0, 2), k.d12[0], k.rd12[0]);
+      du[0] = A::cdv(L(2, 0, -2) - (R)8.0 * L(2, 0, -1) + (R)8.0 * L(2, 0, 1) - L(2, 0, 2), k.d12[0], k.rd12[0]);
+      dr[1] = A::cdv(L(0, -2, 0) - (R)8.0 * L(0, -1, 0) + (R)8.0 * L(0, 1, 0) - L(0, 2, 0), k.d12[1], k.rd12[1]);
+      de[1] = A::cdv(L(1, -2, 0) - (R)8.0 * L(1, -1, 0) + (R)8.0 * L(1, 1, 0) - L(1, 2, 0), k.d12[1], k.rd12[1]);
+      du[1] = A::cdv(L(3, -2, 0) - (R)8.0 * L(3, -1, 0) + (R)8.0 * L(3, 1, 0) - L(3, 2, 0), k.d12[1], k.rd12[1]);
+      dr[2] = A::cdv(qr[0] - (R)8.0 * qr[1] + (R)8.0 * qr[3] - qr[4], k.d12[2], k.rd12[2]);
+      de[2] = A::cdv(qe[0] - (R)8.0 * qe[1] + (R)8.0 * qe[3] - qe[4], k.d12[2], k.rd12[2]);
+      du[2] = A::cdv(qw[0] - (R)8.0 * qw[1] + (R)8.0 * qw[3] - qw[4], k.d12[2], k.rd12[2]);
+      const R rgrad = A::sq(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+      const R divu = du[0] + du[1] + du[2];
+      const R dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) * (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+      const int ind = dnednr < (R)0.0 ? 1 : 0;
+      R G[5];
+      G[0] = (divu < 0) ? (R)(1 - ind) * rgrad : (R)0;
+      G[1] = (R)ind * rgrad;
+      G[2] = dr[0];
+      G[3] = dr[1];
+      G[4] = dr[2];
+      // updateCeq, src/ceq3d.hpp:202-223, then the stage update of the C variables
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        const R cc = qc[1][n];
+        const R lap = A::cdv(L(4 + n, 0, -1) - (R)2 * cc + L(4 + n, 0, 1), k.d[0], k.rd[0]) +
+                      A::cdv(L(4 + n, -1, 0) - (R)2 * cc + L(4 + n, 1, 0), k.d[1], k.rd[1]) +
+                      A::cdv(qc[0][n] - (R)2 * cc + qc[2][n], k.d[2], k.rd[2]);
+        const double dv = c1 * (G[n] - cc) + c2 * lap;
+        const double un = base[n] + h * dv;
+        st(Uo, c + nvb + N * n, un);
+        if (n == 1) mC = fmax(mC, fabs(un));
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        qr[q] = qr[q + 1];
+        qe[q] = qe[q + 1];
+        qw[q] = qw[q + 1];
+      }
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        qc[0][n] = qc[1][n];
+        qc[1][n] = qc[2][n];
+      }
+    }
+  }
+  double v1[1] = {mC};
+  unsigned long long *const d1[1] = {&scout->maxCh};
+  block_max_atomic<1>(v1, d1);
 }
 
 // ---------------------------------------------------------------- primitives
@@ -1343,6 +1413,11 @@ void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf,
   else
     k_vcr3d<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h, stage,
                                                   cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
+  if (cf.ceq && !LC) {  // the C variables: sources, relaxation and stage update
+    const int kc = 16;
+    const int nbx = (g.nci + 63) / 64, nby = (g.ncj + CY - 1) / CY, nbz = (g.nck + kc - 1) / kc;
+    k_ceq3d<F32><<<nbx * nby * nbz, CNT, 0, s>>>(q, k, U, Un, Uo, P, scin, scout, h, stage, kc, nbx, nby, nbz);
+  }
 }
 
 template <int NS>
