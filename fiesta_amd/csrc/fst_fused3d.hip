@@ -901,8 +901,16 @@ __device__ __forceinline__ void face_vc_ring(const FC &k, const RingAt<NS, F32, 
   if (ceq) {
     constexpr int fch = LC ? RF::FC : RF::FC + 1;  // C_hat
     constexpr int fcn = fch + 1;                  // direction n
-    const R r = (V(3, 0, 0, 0) + V(3, ep, er, eq)) / (R)2.0;
     const R chat = (V(fch, 0, 0, 0) + V(fch, ep, er, eq)) / (R)2.0;
+    if (LC && __ballot(chat != (R)0) == 0) {
+      // learned form: C_hat vanishes on this face in every lane of the wave
+      // (inactive cells), so the artificial-viscosity fluxes alpha rho C_hat M
+      // du are zeros; their divergence adds +0 to dvar either way
+#pragma unroll
+      for (int q = 0; q < 9; ++q) out[3 + q] = (R)0;
+      return;
+    }
+    const R r = (V(3, 0, 0, 0) + V(3, ep, er, eq)) / (R)2.0;
     R cn[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) cn[q] = (V(fcn + q, 0, 0, 0) + V(fcn + q, ep, er, eq)) / (R)2.0;
@@ -1050,7 +1058,15 @@ __global__ void __launch_bounds__(NTHREADS, 1)
           dv[3] += td;
         }
       }
-      if (ceq) {
+      bool ceq_cell = ceq;
+      if (LC && ceq) {  // learned form: skip when C_hat is 0 at the cell and its 6 neighbours in the whole wave
+        constexpr int fch = RF::FC;
+        const bool act = at(fch, 0, 0, 0) != (R)0 || at(fch, 0, 0, -1) != (R)0 || at(fch, 0, 0, 1) != (R)0 ||
+                         at(fch, 0, -1, 0) != (R)0 || at(fch, 0, 1, 0) != (R)0 || at(fch, -1, 0, 0) != (R)0 ||
+                         at(fch, 1, 0, 0) != (R)0;
+        ceq_cell = __ballot(act) != 0;
+      }
+      if (ceq_cell) {
         // applyCeq, src/ceq3d.hpp:343-365
 #pragma unroll
         for (int ww = 0; ww < 3; ++ww) {
