@@ -499,7 +499,7 @@ int fst_learned_compute(fst_ctx *ctx) {
   if (!ctx->lprim) HIPCHK(ctx, hipMalloc((void **)&ctx->lprim, np * (size_t)ctx->g.N * sizeof(double)));
   launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
   int rc = fused_primitives(ctx->g, ctx->ph, ctx->cf, ctx->b.var, ctx->lprim, ctx->stream);
-  if (!rc) rc = learned_fields(ctx->g, ctx->cf, ctx->ph, ctx->lprim, ctx->lb, ctx->stream);
+  if (!rc) rc = learned_fields(ctx->g, ctx->cf, ctx->ph, ctx->lprim, false, ctx->lb, ctx->stream);
   if (rc) return fail(ctx, rc, rc == FST_ERR_STATE ? "fst_learned_compute: no network weights set"
                                                    : "fst_learned_compute: launch failed");
   ctx->fb.valid = false;

@@ -55,6 +55,18 @@ __host__ __device__ __forceinline__ double cdiv_rn(double a, double b, double rb
   return fma(e, rb, q0);
 }
 
+// Primitive planes q = 0 rho, 1 p, 2 u, 3 v, 4 w, 5 nrg, 6.. Y_s of cell t
+// (fused path).  FP64 form: doubles, plane q at P + N*q.  FP32 forms: p as
+// doubles (the pressure gradient and buoyancy keep their hydrostatic
+// cancellation in FP64), then the other planes as floats in the order
+// rho, u, v, w, nrg, Y_s.
+template <bool PF>
+__device__ __forceinline__ double prim(const double *__restrict__ P, long long N, int q, long long t) {
+  if (!PF) return P[t + N * q];
+  if (q == 1) return P[t];
+  return (double)reinterpret_cast<const float *>(P + N)[t + N * (q == 0 ? 0 : q - 1)];
+}
+
 // Wave-wide max over 64 lanes.
 __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));

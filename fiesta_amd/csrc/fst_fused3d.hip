@@ -73,6 +73,26 @@ __device__ __forceinline__ void st(double *__restrict__ base, uint32_t off, doub
   *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + (off << 3)) = v;
 }
 
+// Primitive planes of cell c with 32-bit offsets, in the layout of prim()
+// (fst_common.hpp): FP64 form all doubles; FP32 forms p as doubles, then
+// rho, u, v, w, nrg, Y_s as floats.
+template <bool F32>
+__device__ __forceinline__ double pget(const double *__restrict__ P, uint32_t N, int q, uint32_t c) {
+  if (!F32) return ld(P, c + N * (uint32_t)q);
+  if (q == 1) return ld(P, c);
+  return (double)reinterpret_cast<const float *>(P + N)[c + N * (uint32_t)(q == 0 ? 0 : q - 1)];
+}
+template <bool F32>
+__device__ __forceinline__ void pset(double *__restrict__ P, uint32_t N, int q, uint32_t c, double v) {
+  if (!F32) {
+    st(P, c + N * (uint32_t)q, v);
+  } else if (q == 1) {
+    st(P, c, v);
+  } else {
+    reinterpret_cast<float *>(P + N)[c + N * (uint32_t)(q == 0 ? 0 : q - 1)] = (float)v;
+  }
+}
+
 // Partial right-hand side handed from the advection kernel to the viscous
 // kernel: FP64 in the FP64 form, FP32 in the FP32 forms (the pressure-
 // gradient / buoyancy cancellation happens in FP64 before the store; the RK
@@ -209,12 +229,12 @@ __device__ __forceinline__ void flux_from_stencil(const FC &k, int D, const doub
 }
 
 // q of one cell: var(v) + (v==3)*p for v < nv
-template <int NS>
+template <int NS, bool F32>
 __device__ __forceinline__ void load_q(const Grid32 &g, const double *__restrict__ U, const double *__restrict__ P,
                                        uint32_t cc, double *q) {
   const uint32_t N = g.N;
 #pragma unroll
-  for (int v = 0; v < 4 + NS; ++v) q[v] = (v == 3) ? ld(U, cc + N * v) + ld(P, cc + N) : ld(U, cc + N * v);
+  for (int v = 0; v < 4 + NS; ++v) q[v] = (v == 3) ? ld(U, cc + N * v) + pget<F32>(P, N, 1, cc) : ld(U, cc + N * v);
 }
 
 // Advective fluxes on the face between c and c+e_D, stencil read from memory.
@@ -222,11 +242,11 @@ template <int D, int NS, bool F32>
 __device__ __forceinline__ void face_adv(const FC &k, const Grid32 &g, const double *__restrict__ U,
                                          const double *__restrict__ P, uint32_t c, double *out) {
   const uint32_t N = g.N, e = stride<D>(g);
-  const uint32_t vb = N * (2 + D);
-  const double vd[4] = {ld(P, c - e + vb), ld(P, c + vb), ld(P, c + e + vb), ld(P, c + 2 * e + vb)};
+  const double vd[4] = {pget<F32>(P, N, 2 + D, c - e), pget<F32>(P, N, 2 + D, c), pget<F32>(P, N, 2 + D, c + e),
+                        pget<F32>(P, N, 2 + D, c + 2 * e)};
   double q[6][4 + NS];
 #pragma unroll
-  for (int t = 0; t < 6; ++t) load_q<NS>(g, U, P, c + (uint32_t)(t - 2) * e, q[t]);
+  for (int t = 0; t < 6; ++t) load_q<NS, F32>(g, U, P, c + (uint32_t)(t - 2) * e, q[t]);
   flux_from_stencil<NS, F32>(k, D, vd, q, out);
 }
 
@@ -249,21 +269,21 @@ __device__ __forceinline__ void face_vc(const FC &k, const Grid32 &g, const doub
   constexpr int T2 = D == 2 ? 1 : 2;
   const uint32_t t1 = T1 == 1 ? g.s1 : 1u;
   const uint32_t t2 = T2 == 1 ? g.s1 : g.s2;
-  auto V = [&](uint32_t off) { return (R)ld(P, off); };
+  auto V = [&](int q, uint32_t cell) { return (R)pget<F32>(P, N, q, cell); };
   R der[3][3];
 #pragma unroll
   for (int w = 0; w < 3; ++w) {
-    const uint32_t vb = N * (2 + w);
-    der[w][D] = A::cdv(V(c + e + vb) - V(c + vb), k.d[D], k.rd[D]);
-    der[w][T1] = A::cdv((V(c + t1 + vb) + V(c + e + t1 + vb)) - (V(c - t1 + vb) + V(c + e - t1 + vb)), k.d4[T1],
+    const int vq = 2 + w;
+    der[w][D] = A::cdv(V(vq, c + e) - V(vq, c), k.d[D], k.rd[D]);
+    der[w][T1] = A::cdv((V(vq, c + t1) + V(vq, c + e + t1)) - (V(vq, c - t1) + V(vq, c + e - t1)), k.d4[T1],
                         k.rd4[T1]);
-    der[w][T2] = A::cdv((V(c + t2 + vb) + V(c + e + t2 + vb)) - (V(c - t2 + vb) + V(c + e - t2 + vb)), k.d4[T2],
+    der[w][T2] = A::cdv((V(vq, c + t2) + V(vq, c + e + t2)) - (V(vq, c - t2) + V(vq, c + e - t2)), k.d4[T2],
                         k.rd4[T2]);
   }
   if (visc) {
     R mu = 0.0;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) mu += ((V(c + e + N * (6 + s)) + V(c + N * (6 + s))) / (R)2.0) * (R)k.mus[s];
+    for (int s = 0; s < NS; ++s) mu += ((V(6 + s, c + e) + V(6 + s, c)) / (R)2.0) * (R)k.mus[s];
     const R dudx = der[0][0], dudy = der[0][1], dudz = der[0][2];
     const R dvdx = der[1][0], dvdy = der[1][1], dvdz = der[1][2];
     const R dwdx = der[2][0], dwdy = der[2][1], dwdz = der[2][2];
@@ -285,7 +305,7 @@ __device__ __forceinline__ void face_vc(const FC &k, const Grid32 &g, const doub
   if (ceq) {
     const int nv = g.nv;
     auto Uv = [&](uint32_t off) { return (R)ld(U, off); };
-    const R r = (V(c) + V(c + e)) / (R)2.0;
+    const R r = (V(0, c) + V(0, c + e)) / (R)2.0;
     R chat, cn[3];
     if (LC) {
       chat = ((R)CH[c] + (R)CH[c + e]) / (R)2.0;
@@ -367,9 +387,9 @@ __global__ void __launch_bounds__(NTHREADS, 1)
   const uint32_t c00 = g.id(t.i, t.jrow, t.kb - 1);
   if (t.cellok) {
 #pragma unroll
-    for (int s = 0; s < 6; ++s) load_q<NS>(g, U, P, c00 + (uint32_t)(s - 2) * g.s2, qz[s]);
+    for (int s = 0; s < 6; ++s) load_q<NS, F32>(g, U, P, c00 + (uint32_t)(s - 2) * g.s2, qz[s]);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) vz[s] = ld(P, c00 + (uint32_t)(s - 1) * g.s2 + 4 * N);
+    for (int s = 0; s < 4; ++s) vz[s] = pget<F32>(P, N, 4, c00 + (uint32_t)(s - 1) * g.s2);
     flux_from_stencil<NS, F32>(k, 2, vz, qz, Zp);
   }
   for (int kk = t.kb; kk < t.ke; ++kk) {
@@ -399,10 +419,10 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       for (int s = 0; s < 5; ++s)
 #pragma unroll
         for (int v = 0; v < NV; ++v) qz[s][v] = qz[s + 1][v];
-      load_q<NS>(g, U, P, c + 3 * g.s2, qz[5]);
+      load_q<NS, F32>(g, U, P, c + 3 * g.s2, qz[5]);
 #pragma unroll
       for (int s = 0; s < 3; ++s) vz[s] = vz[s + 1];
-      vz[3] = ld(P, c + 2 * g.s2 + 4 * N);
+      vz[3] = pget<F32>(P, N, 4, c + 2 * g.s2);
       flux_from_stencil<NS, F32>(k, 2, vz, qz, Z);
     }
     __syncthreads();
@@ -411,19 +431,19 @@ __global__ void __launch_bounds__(NTHREADS, 1)
 #pragma unroll
       for (int v = 0; v < NV; ++v)
         dv[v] = -((xsh[w][v][l + 1] - xsh[w][v][l]) + (ysh[w + 1][v][l] - ysh[w][v][l]) + (Z[v] - Zp[v]));
-      const uint32_t pb = N;  // pressure plane
+      const double *Pp = F32 ? P : P + N;  // pressure plane (FP64 in every form)
       const uint32_t sx = 1, sy = g.s1, sz = g.s2;
-      const double dxp = cdiv(ld(P, c - 2 * sx + pb) - 8.0 * ld(P, c - sx + pb) + 8.0 * ld(P, c + sx + pb) -
-                                  ld(P, c + 2 * sx + pb), k.pg[0], k.rpg[0]);
-      const double dyp = cdiv(ld(P, c - 2 * sy + pb) - 8.0 * ld(P, c - sy + pb) + 8.0 * ld(P, c + sy + pb) -
-                                  ld(P, c + 2 * sy + pb), k.pg[1], k.rpg[1]);
-      const double dzp = cdiv(ld(P, c - 2 * sz + pb) - 8.0 * ld(P, c - sz + pb) + 8.0 * ld(P, c + sz + pb) -
-                                  ld(P, c + 2 * sz + pb), k.pg[2], k.rpg[2]);
+      const double dxp = cdiv(ld(Pp, c - 2 * sx) - 8.0 * ld(Pp, c - sx) + 8.0 * ld(Pp, c + sx) - ld(Pp, c + 2 * sx),
+                              k.pg[0], k.rpg[0]);
+      const double dyp = cdiv(ld(Pp, c - 2 * sy) - 8.0 * ld(Pp, c - sy) + 8.0 * ld(Pp, c + sy) - ld(Pp, c + 2 * sy),
+                              k.pg[1], k.rpg[1]);
+      const double dzp = cdiv(ld(Pp, c - 2 * sz) - 8.0 * ld(Pp, c - sz) + 8.0 * ld(Pp, c + sz) - ld(Pp, c + 2 * sz),
+                              k.pg[2], k.rpg[2]);
       dv[0] = dv[0] - dxp;
       dv[1] = dv[1] - dyp;
       dv[2] = dv[2] - dzp;
       if (buoy) {
-        const double rho = ld(P, c);
+        const double rho = pget<F32>(P, N, 0, c);
         const double vv = ld(U, c + N) / rho;
         const double rhop = rho - k.rhoRef;
         const double f = -k.g * rhop;
@@ -454,31 +474,36 @@ struct PlaneF {
   static constexpr int NV = 4 + NS;
   static constexpr int NQ = NV + 3;  // q_0..q_{nv-1}, u, v, p
   static constexpr int FU = NV, FV = NV + 1, FP = NV + 2;
-  static constexpr int NPRE = (NQ * QPL + NTHREADS - 1) / NTHREADS;
+  static constexpr int NPC = (QPL + NTHREADS - 1) / NTHREADS;  // plane cells per thread
+  static constexpr int NPRE = NQ * NPC;
 };
 
-template <int NS>
+// Loads of plane p for the LDS plane, field-major: thread t takes the plane
+// cells t, t + NTHREADS, ... of every field, so each load's field (and its
+// array and precision) is a compile-time choice.
+template <int NS, bool F32>
 __device__ __forceinline__ void plane_fetch(const Grid32 &g, const double *__restrict__ U,
                                             const double *__restrict__ P, int i0, int j0, int p, double *pre) {
   using PF = PlaneF<NS>;
   const uint32_t N = g.N;
 #pragma unroll
-  for (int m = 0; m < PF::NPRE; ++m) {
+  for (int m = 0; m < PF::NPC; ++m) {
     const int e = threadIdx.x + m * NTHREADS;
-    double v = 0.0;
-    if (e < PF::NQ * QPL) {
-      const int f = e / QPL, rem = e - f * QPL;
-      const int r = rem / QX, q = rem - r * QX;
-      const int i = i0 - 3 + q, j = j0 - 3 + r;
-      if (i < g.ngi && j < g.ngj) {
-        const uint32_t c = g.id(i, j, p);
-        if (f < PF::NV) v = (f == 3) ? ld(U, c + N * f) + ld(P, c + N) : ld(U, c + N * f);
-        else if (f == PF::FU) v = ld(P, c + 2 * N);
-        else if (f == PF::FV) v = ld(P, c + 3 * N);
-        else v = ld(P, c + N);
+    const int r = e / QX, q = e - r * QX;
+    const int i = i0 - 3 + q, j = j0 - 3 + r;
+    const bool ok = e < QPL && i < g.ngi && j < g.ngj;
+    const uint32_t c = ok ? g.id(i, j, p) : 0u;
+#pragma unroll
+    for (int f = 0; f < PF::NQ; ++f) {
+      double v = 0.0;
+      if (ok) {
+        if (f < PF::NV) v = (f == 3) ? ld(U, c + N * f) + pget<F32>(P, N, 1, c) : ld(U, c + N * f);
+        else if (f == PF::FU) v = pget<F32>(P, N, 2, c);
+        else if (f == PF::FV) v = pget<F32>(P, N, 3, c);
+        else v = pget<F32>(P, N, 1, c);
       }
+      pre[f * PF::NPC + m] = v;
     }
-    pre[m] = v;
   }
 }
 
@@ -486,9 +511,12 @@ template <int NS>
 __device__ __forceinline__ void plane_store(double *qs, const double *pre) {
   using PF = PlaneF<NS>;
 #pragma unroll
-  for (int m = 0; m < PF::NPRE; ++m) {
+  for (int m = 0; m < PF::NPC; ++m) {
     const int e = threadIdx.x + m * NTHREADS;
-    if (e < PF::NQ * QPL) qs[e] = pre[m];
+    if (e < QPL) {
+#pragma unroll
+      for (int f = 0; f < PF::NQ; ++f) qs[f * QPL + e] = pre[f * PF::NPC + m];
+    }
   }
 }
 
@@ -530,22 +558,22 @@ __global__ void __launch_bounds__(NTHREADS, 1)
   const uint32_t c00 = g.id(t.i, t.jrow, t.kb - 1);
   if (t.cellok) {
 #pragma unroll
-    for (int s = 0; s < 6; ++s) load_q<NS>(g, U, P, c00 + (uint32_t)(s - 2) * g.s2, qz[s]);
+    for (int s = 0; s < 6; ++s) load_q<NS, F32>(g, U, P, c00 + (uint32_t)(s - 2) * g.s2, qz[s]);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) vz[s] = ld(P, c00 + (uint32_t)(s - 1) * g.s2 + 4 * N);
+    for (int s = 0; s < 4; ++s) vz[s] = pget<F32>(P, N, 4, c00 + (uint32_t)(s - 1) * g.s2);
     flux_from_stencil<NS, F32>(k, 2, vz, qz, Zp);
   }
   // prefetch of the next plane when the registers allow it (ns <= 2)
   constexpr bool PREF = NS <= 2;
   double pre[PF::NPRE];
-  if (PREF) plane_fetch<NS>(g, U, P, t.i0, t.j0, t.kb, pre);
+  if (PREF) plane_fetch<NS, F32>(g, U, P, t.i0, t.j0, t.kb, pre);
   for (int kk = t.kb; kk < t.ke; ++kk) {
     uint32_t c = g.id(t.i, t.jrow, kk);
     asm volatile("" : "+v"(c));
-    if (!PREF) plane_fetch<NS>(g, U, P, t.i0, t.j0, kk, pre);
+    if (!PREF) plane_fetch<NS, F32>(g, U, P, t.i0, t.j0, kk, pre);
     plane_store<NS>(qs, pre);
     __syncthreads();
-    if (PREF && kk + 1 < t.ke) plane_fetch<NS>(g, U, P, t.i0, t.j0, kk + 1, pre);
+    if (PREF && kk + 1 < t.ke) plane_fetch<NS, F32>(g, U, P, t.i0, t.j0, kk + 1, pre);
     const int rc = w + 3, qc = l + 3;  // plane coordinates of this lane's cell
     if (t.cellok) {
       double X[NV];
@@ -570,10 +598,10 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       for (int s = 0; s < 5; ++s)
 #pragma unroll
         for (int v = 0; v < NV; ++v) qz[s][v] = qz[s + 1][v];
-      load_q<NS>(g, U, P, c + 3 * g.s2, qz[5]);
+      load_q<NS, F32>(g, U, P, c + 3 * g.s2, qz[5]);
 #pragma unroll
       for (int s = 0; s < 3; ++s) vz[s] = vz[s + 1];
-      vz[3] = ld(P, c + 2 * g.s2 + 4 * N);
+      vz[3] = pget<F32>(P, N, 4, c + 2 * g.s2);
       flux_from_stencil<NS, F32>(k, 2, vz, qz, Z);
     }
     __syncthreads();
@@ -583,16 +611,17 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       for (int v = 0; v < NV; ++v)
         dv[v] = -((xsh[w][v][l + 1] - xsh[w][v][l]) + (ysh[w + 1][v][l] - ysh[w][v][l]) + (Z[v] - Zp[v]));
       const double *pp = qs + PF::FP * QPL + rc * QX + qc;
-      const uint32_t pb = N, sz = g.s2;
+      const double *Pp = F32 ? P : P + N;  // pressure plane (FP64 in every form)
+      const uint32_t sz = g.s2;
       const double dxp = cdiv(pp[-2] - 8.0 * pp[-1] + 8.0 * pp[1] - pp[2], k.pg[0], k.rpg[0]);
       const double dyp = cdiv(pp[-2 * QX] - 8.0 * pp[-QX] + 8.0 * pp[QX] - pp[2 * QX], k.pg[1], k.rpg[1]);
-      const double dzp = cdiv(ld(P, c - 2 * sz + pb) - 8.0 * ld(P, c - sz + pb) + 8.0 * ld(P, c + sz + pb) -
-                                  ld(P, c + 2 * sz + pb), k.pg[2], k.rpg[2]);
+      const double dzp = cdiv(ld(Pp, c - 2 * sz) - 8.0 * ld(Pp, c - sz) + 8.0 * ld(Pp, c + sz) - ld(Pp, c + 2 * sz),
+                              k.pg[2], k.rpg[2]);
       dv[0] = dv[0] - dxp;
       dv[1] = dv[1] - dyp;
       dv[2] = dv[2] - dzp;
       if (buoy) {
-        const double rho = ld(P, c);
+        const double rho = pget<F32>(P, N, 0, c);
         const double vv = qs[1 * QPL + rc * QX + qc] / rho;
         const double rhop = rho - k.rhoRef;
         const double f = -k.g * rhop;
@@ -670,7 +699,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       double dv[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) dv[v] = ld_dv<F32>(dvar, c + N * v);
-      auto PV = [&](uint32_t off) { return (R)ld(P, off); };
+      auto PV = [&](int q, uint32_t cell) { return (R)pget<F32>(P, N, q, cell); };
       if (visc) {
 #pragma unroll
         for (int D = 0; D < 3; ++D) {
@@ -689,11 +718,10 @@ __global__ void __launch_bounds__(NTHREADS, 1)
             T1c = Z[1]; T1m = Zp[1];
             T2c = Z[2]; T2m = Zp[2];
           }
-          const uint32_t b0 = 2 * N, b1 = 3 * N, b2 = 4 * N;
-          const R V0 = PV(c + b0), V1 = PV(c + b1), V2 = PV(c + b2);
-          const R ur = (PV(c + e + b0) + V0) / (R)2.0, ul = (PV(c - e + b0) + V0) / (R)2.0;
-          const R vr = (PV(c + e + b1) + V1) / (R)2.0, vl = (PV(c - e + b1) + V1) / (R)2.0;
-          const R wr = (PV(c + e + b2) + V2) / (R)2.0, wl = (PV(c - e + b2) + V2) / (R)2.0;
+          const R V0 = PV(2, c), V1 = PV(3, c), V2 = PV(4, c);
+          const R ur = (PV(2, c + e) + V0) / (R)2.0, ul = (PV(2, c - e) + V0) / (R)2.0;
+          const R vr = (PV(3, c + e) + V1) / (R)2.0, vl = (PV(3, c - e) + V1) / (R)2.0;
+          const R wr = (PV(4, c + e) + V2) / (R)2.0, wl = (PV(4, c - e) + V2) / (R)2.0;
           const double dd = k.d[D], rdd = k.rd[D];
           const R ta = A::cdv(T0c - T0m, dd, rdd);
           const R tb = A::cdv(T1c - T1m, dd, rdd);
@@ -736,7 +764,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       double ms;
       prims_of<NS>(k, un, pr, ms);
 #pragma unroll
-      for (int q = 0; q < 6 + NS; ++q) st(Po, c + N * q, pr[q]);
+      for (int q = 0; q < 6 + NS; ++q) pset<F32>(Po, N, q, c, pr[q]);
       mS = fmax(mS, ms);
     }
 #pragma unroll
@@ -772,7 +800,8 @@ struct RingF {
   static constexpr int NFALL = 4 + NS + (LC ? 4 : 5);  // u v w rho Y_s C...
   static constexpr int NF = F32 ? NFALL : 3;           // fields held in the ring
   static constexpr int FY = 4, FC = 4 + NS;            // first Y, first C field
-  static constexpr int NPRE = (NF * RPL + NTHREADS - 1) / NTHREADS;
+  static constexpr int NPC = (RPL + NTHREADS - 1) / NTHREADS;  // ring cells per thread
+  static constexpr int NPRE = NF * NPC;
   using R = typename Ar<F32>::R;
 };
 
@@ -783,14 +812,15 @@ __device__ __forceinline__ typename Ar<F32>::R field_at(uint32_t N, int nv, cons
                                                         const float *__restrict__ DR, int f, uint32_t c) {
   using RF = RingF<NS, F32, LC>;
   using R = typename RF::R;
-  if (f < 3) return (R)ld(P, c + N * (2 + f));
-  if (f == 3) return (R)ld(P, c);
-  if (f < RF::FC) return (R)ld(P, c + N * (6 + (f - RF::FY)));
+  if (f < 3) return (R)pget<F32>(P, N, 2 + f, c);
+  if (f == 3) return (R)pget<F32>(P, N, 0, c);
+  if (f < RF::FC) return (R)pget<F32>(P, N, 6 + (f - RF::FY), c);
   if (LC) return (f == RF::FC) ? (R)CH[c] : (R)DR[c + N * (f - RF::FC - 1)];
   return (R)ld(U, c + N * (nv + (f - RF::FC)));
 }
 
-// Loads of one plane p of the ring (element e = tid + m*NTHREADS).
+// Loads of one plane p of the ring, field-major: thread t takes the ring
+// cells t, t + NTHREADS, ... of every field (compile-time field choice).
 template <int NS, bool F32, bool LC>
 __device__ __forceinline__ void ring_fetch(const Grid32 &g, const double *__restrict__ U,
                                            const double *__restrict__ P, const float *__restrict__ CH,
@@ -799,16 +829,15 @@ __device__ __forceinline__ void ring_fetch(const Grid32 &g, const double *__rest
   using RF = RingF<NS, F32, LC>;
   using R = typename RF::R;
 #pragma unroll
-  for (int m = 0; m < RF::NPRE; ++m) {
+  for (int m = 0; m < RF::NPC; ++m) {
     const int e = threadIdx.x + m * NTHREADS;
-    R v = (R)0;
-    if (e < RF::NF * RPL) {
-      const int f = e / RPL, rem = e - f * RPL;
-      const int r = rem / RX, q = rem - r * RX;
-      const int i = i0 - 1 + q, j = j0 - 1 + r;
-      if (i < g.ngi && j < g.ngj) v = field_at<NS, F32, LC>(g.N, g.nv, U, P, CH, DR, f, g.id(i, j, p));
-    }
-    pre[m] = v;
+    const int r = e / RX, q = e - r * RX;
+    const int i = i0 - 1 + q, j = j0 - 1 + r;
+    const bool ok = e < RPL && i < g.ngi && j < g.ngj;
+    const uint32_t c = ok ? g.id(i, j, p) : 0u;
+#pragma unroll
+    for (int f = 0; f < RF::NF; ++f)
+      pre[f * RF::NPC + m] = ok ? field_at<NS, F32, LC>(g.N, g.nv, U, P, CH, DR, f, c) : (R)0;
   }
 }
 
@@ -816,9 +845,12 @@ template <int NS, bool F32, bool LC>
 __device__ __forceinline__ void ring_store(typename Ar<F32>::R *ring, int slot, const typename Ar<F32>::R *pre) {
   using RF = RingF<NS, F32, LC>;
 #pragma unroll
-  for (int m = 0; m < RF::NPRE; ++m) {
+  for (int m = 0; m < RF::NPC; ++m) {
     const int e = threadIdx.x + m * NTHREADS;
-    if (e < RF::NF * RPL) ring[slot * RF::NF * RPL + e] = pre[m];
+    if (e < RPL) {
+#pragma unroll
+      for (int f = 0; f < RF::NF; ++f) ring[(slot * RF::NF + f) * RPL + e] = pre[f * RF::NPC + m];
+    }
   }
 }
 
@@ -1023,7 +1055,6 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       double dv[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) dv[v] = EARLY ? dvp[EARLY ? v : 0] : ld_dv<F32>(dvar, c + N * v);
-      auto PV = [&](uint32_t off) { return (R)ld(P, off); };
       if (visc) {
 #pragma unroll
         for (int D = 0; D < 3; ++D) {
@@ -1096,7 +1127,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       double ms;
       prims_of<NS>(k, un, pr, ms);
 #pragma unroll
-      for (int q = 0; q < 6 + NS; ++q) st(Po, c + N * q, pr[q]);
+      for (int q = 0; q < 6 + NS; ++q) pset<F32>(Po, N, q, c, pr[q]);
       mS = fmax(mS, ms);
     }
 #pragma unroll
@@ -1132,7 +1163,8 @@ __global__ void __launch_bounds__(NTHREADS, 1)
 constexpr int CY = 8, CNT = 64 * CY;
 constexpr int CPR = CY + 4, CPX = 64 + 4, CPL = CPR * CPX;  // staged plane: rows, columns, cells
 constexpr int CNF = 9;                                       // rho, nrg, u, v, C0..C4
-constexpr int CNPRE = (CNF * CPL + CNT - 1) / CNT;
+constexpr int CNPC = (CPL + CNT - 1) / CNT;  // plane cells per thread
+constexpr int CNPRE = CNF * CNPC;
 
 template <bool F32>
 __device__ __forceinline__ void ceq_fetch(const Grid32 &g, const double *__restrict__ U, const double *__restrict__ P,
@@ -1140,21 +1172,18 @@ __device__ __forceinline__ void ceq_fetch(const Grid32 &g, const double *__restr
   using R = typename Ar<F32>::R;
   const uint32_t N = g.N;
 #pragma unroll
-  for (int m = 0; m < CNPRE; ++m) {
+  for (int m = 0; m < CNPC; ++m) {  // field-major: compile-time field choice
     const int e = threadIdx.x + m * CNT;
-    R v = (R)0;
-    if (e < CNF * CPL) {
-      const int f = e / CPL, rem = e - f * CPL;
-      const int r = rem / CPX, q = rem - r * CPX;
-      const int i = i0 - 2 + q, j = j0 - 2 + r;
-      if (i < g.ngi && j < g.ngj) {
-        const uint32_t c = g.id(i, j, kk);
-        // P planes: rho 0, u 2, v 3, nrg 5; then the C variables of U
-        const uint32_t off = f == 0 ? c : f == 1 ? c + 5 * N : f == 2 ? c + 2 * N : f == 3 ? c + 3 * N : 0u;
-        v = f < 4 ? (R)ld(P, off) : (R)ld(U, c + N * (uint32_t)(g.nv + f - 4));
-      }
+    const int r = e / CPX, q = e - r * CPX;
+    const int i = i0 - 2 + q, j = j0 - 2 + r;
+    const bool ok = e < CPL && i < g.ngi && j < g.ngj;
+    const uint32_t c = ok ? g.id(i, j, kk) : 0u;
+#pragma unroll
+    for (int f = 0; f < CNF; ++f) {
+      // primitive planes rho 0, nrg 5, u 2, v 3; then the C variables of U
+      const int pq = f == 0 ? 0 : f == 1 ? 5 : f == 2 ? 2 : 3;
+      pre[f * CNPC + m] = !ok ? (R)0 : f < 4 ? (R)pget<F32>(P, N, pq, c) : (R)ld(U, c + N * (uint32_t)(g.nv + f - 4));
     }
-    pre[m] = v;
   }
 }
 
@@ -1182,9 +1211,9 @@ __global__ void __launch_bounds__(CNT)
   const double maxS = as_double(scin->maxS);
   const R c1 = (R)(maxS / (k.eps * k.dxmag));
   const R c2 = (R)(k.kap * maxS * k.dxmag);
-  auto PV = [&](uint32_t off) { return (R)ld(P, off); };
+  auto PV = [&](int q, uint32_t cell) { return (R)pget<F32>(P, N, q, cell); };
   auto CU = [&](uint32_t off) { return (R)ld(U, off); };
-  const uint32_t rb = 0, eb = 5 * N, wb = 4 * N;
+  constexpr int rb = 0, eb = 5, wb = 4;  // primitive planes rho, nrg, w
   // queues: [0..4] = planes k-2..k+2 of rho, nrg, w; qc[0..2] = planes k-1..k+1 of C
   R qr[5], qe[5], qw[5], qc[3][5], nr = 0, ne = 0, nw = 0, nc[5];
   uint32_t c = g.id(min(i, g.ngi - 1), min(j, g.ngj - 1), kb);
@@ -1192,18 +1221,18 @@ __global__ void __launch_bounds__(CNT)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const uint32_t cc = c + (uint32_t)q * s2 - 2u * s2;
-      qr[q] = PV(cc + rb);
-      qe[q] = PV(cc + eb);
-      qw[q] = PV(cc + wb);
+      qr[q] = PV(rb, cc);
+      qe[q] = PV(eb, cc);
+      qw[q] = PV(wb, cc);
     }
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
       qc[0][n] = CU(c - s2 + nvb + N * n);
       qc[1][n] = CU(c + nvb + N * n);
     }
-    nr = PV(c + 2 * s2 + rb);
-    ne = PV(c + 2 * s2 + eb);
-    nw = PV(c + 2 * s2 + wb);
+    nr = PV(rb, c + 2 * s2);
+    ne = PV(eb, c + 2 * s2);
+    nw = PV(wb, c + 2 * s2);
 #pragma unroll
     for (int n = 0; n < 5; ++n) nc[n] = CU(c + s2 + nvb + N * n);
   }
@@ -1213,9 +1242,12 @@ __global__ void __launch_bounds__(CNT)
   for (int kk = kb; kk < ke; ++kk, c += s2) {
     __syncthreads();  // the previous plane's readers are done
 #pragma unroll
-    for (int m = 0; m < CNPRE; ++m) {
+    for (int m = 0; m < CNPC; ++m) {
       const int e = threadIdx.x + m * CNT;
-      if (e < CNF * CPL) pl[e] = pre[m];
+      if (e < CPL) {
+#pragma unroll
+        for (int f = 0; f < CNF; ++f) pl[f * CPL + e] = pre[f * CNPC + m];
+      }
     }
     __syncthreads();
     if (kk + 1 < ke) ceq_fetch<F32>(g, U, P, i0, j0, kk + 1, pre);
@@ -1229,9 +1261,9 @@ __global__ void __launch_bounds__(CNT)
 #pragma unroll
       for (int n = 0; n < 5; ++n) base[n] = (stage == 1) ? ld(U, c + nvb + N * n) : ld(Un, c + nvb + N * n);
       if (kk + 1 < ke) {  // z-queue entries of the next plane
-        nr = PV(c + 3 * s2 + rb);
-        ne = PV(c + 3 * s2 + eb);
-        nw = PV(c + 3 * s2 + wb);
+        nr = PV(rb, c + 3 * s2);
+        ne = PV(eb, c + 3 * s2);
+        nw = PV(wb, c + 3 * s2);
 #pragma unroll
         for (int n = 0; n < 5; ++n) nc[n] = CU(c + 2 * s2 + nvb + N * n);
       }
@@ -1295,7 +1327,7 @@ __global__ void __launch_bounds__(CNT)
 // (ghost_only = 0), or over the ghost cells only (ghost_only = 1: threads
 // enumerate the k slabs, then the j slabs, then the i slabs of the ghost
 // shell).  Threads 0..3 of block 0 clear `clear` when given.
-template <int NS>
+template <int NS, bool F32>
 __global__ void k_prims3d(const Grid32 g, const FC k, const double *__restrict__ U, double *__restrict__ P,
                           Scalars *sc, int ghost_only, Scalars *clear) {
   if (clear && blockIdx.x == 0 && threadIdx.x < 4) reinterpret_cast<unsigned long long *>(clear)[threadIdx.x] = 0ull;
@@ -1342,7 +1374,7 @@ __global__ void k_prims3d(const Grid32 g, const FC k, const double *__restrict__
     double pr[6 + NS];
     prims_of<NS>(k, u, pr, ms);
 #pragma unroll
-    for (int q = 0; q < 6 + NS; ++q) st(P, cell + N * q, pr[q]);
+    for (int q = 0; q < 6 + NS; ++q) pset<F32>(P, N, q, cell, pr[q]);
     if (!interior) ms = 0.0;
     else if (g.nvt > g.nv) mc = fabs(ld(U, cell + N * (g.nv + 1)));
   }
@@ -1436,13 +1468,13 @@ void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf,
   }
 }
 
-template <int NS>
+template <int NS, bool F32>
 void run_prims(const Grid32 &q, const FC &k, const double *U, double *P, Scalars *sc, int ghost_only,
                Scalars *clear, hipStream_t s) {
   uint32_t n = q.N;
   if (ghost_only)
     n = 2u * q.ng * q.s2 + 2u * q.ng * (uint32_t)q.ngi * q.nck + 2u * q.ng * (uint32_t)q.ncj * q.nck;
-  k_prims3d<NS><<<(n + 255) / 256, 256, 0, s>>>(q, k, U, P, sc, ghost_only, clear);
+  k_prims3d<NS, F32><<<(n + 255) / 256, 256, 0, s>>>(q, k, U, P, sc, ghost_only, clear);
 }
 
 using StageFn = void (*)(const Grid32 &, const Geo &, const FC &, const fst_config &, const double *,
@@ -1459,9 +1491,9 @@ StageFn pick_ns(bool f32, bool lc) {
 
 bool pick(int ns, bool f32, bool lc, StageFn *sf, PrimFn *pf) {
   switch (ns) {
-    case 1: *sf = pick_ns<1>(f32, lc); *pf = run_prims<1>; return true;
-    case 2: *sf = pick_ns<2>(f32, lc); *pf = run_prims<2>; return true;
-    case 3: *sf = pick_ns<3>(f32, lc); *pf = run_prims<3>; return true;
+    case 1: *sf = pick_ns<1>(f32, lc); *pf = f32 ? run_prims<1, true> : run_prims<1, false>; return true;
+    case 2: *sf = pick_ns<2>(f32, lc); *pf = f32 ? run_prims<2, true> : run_prims<2, false>; return true;
+    case 3: *sf = pick_ns<3>(f32, lc); *pf = f32 ? run_prims<3, true> : run_prims<3, false>; return true;
     default: return false;
   }
 }
@@ -1529,7 +1561,7 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
     pf(q, k, b.var, Pa, nullptr, 1, S1, s);  // ghost primitives; clears S1
     if (prof) prof->mark(s);
     if (lc) {  // C_hat and n of U^n, used by both stages of the step
-      const int rc = learned_fields(g, cf, ph, Pa, *lb, s);
+      const int rc = learned_fields(g, cf, ph, Pa, cf.precision == FST_FP32_BULK, *lb, s);
       if (rc) {
         *err = "learned C-indicator: launch failed";
         return rc;

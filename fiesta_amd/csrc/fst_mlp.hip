@@ -97,6 +97,7 @@ inline dim3 grid3(const Geo &g) { return dim3((g.ngi + 63) / 64, (g.ngj + 3) / 4
 // p and w at k-1..k+1) in registers, so each plane is read once for z.
 constexpr int FEAT_KCH = 16;
 
+template <bool PF>
 __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst fc, const double *__restrict__ P,
                                                     float *__restrict__ F, float *__restrict__ M,
                                                     float *__restrict__ Dir, float *__restrict__ G1, Scalars *sc) {
@@ -104,20 +105,20 @@ __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst 
   const int k0 = blockIdx.z * FEAT_KCH, k1 = min(k0 + FEAT_KCH, g.ngk);
   const bool col = i < g.ngi && j < g.ngj;
   const long long N = g.N;
-  const double *R = P, *Pp = P + N, *W = P + 4 * N, *E = P + 5 * N;
+  auto PR = [&](int q, long long t) { return prim<PF>(P, N, q, t); };  // primitive plane q at cell t
   const long long cij = (long long)i + g.s1 * j;
-  auto at = [&](const double *A, int k) { return (k >= 0 && k < g.ngk) ? A[cij + g.s2 * k] : 0.0; };
+  auto at = [&](int q, int k) { return (k >= 0 && k < g.ngk) ? PR(q, cij + g.s2 * k) : 0.0; };
   double rq[5], eq[5], pq[3], wq[3];  // rho, nrg at k-2..k+2; p, w at k-1..k+1
   if (col) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-      rq[q] = at(R, k0 - 2 + q);
-      eq[q] = at(E, k0 - 2 + q);
+      rq[q] = at(0, k0 - 2 + q);
+      eq[q] = at(5, k0 - 2 + q);
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      pq[q] = at(Pp, k0 - 1 + q);
-      wq[q] = at(W, k0 - 1 + q);
+      pq[q] = at(1, k0 - 1 + q);
+      wq[q] = at(4, k0 - 1 + q);
     }
   }
   float g1max = 0.f;
@@ -135,10 +136,9 @@ __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst 
           const double h2 = fc.h2[a], rh2 = fc.rh2[a];  // divisions by run constants: cdiv_rn == IEEE
           double rp, rm, pp, pm, vp, vm;
           if (a < 2) {
-            const double *V = P + (2 + a) * N;
-            rp = R[t + s[a]]; rm = R[t - s[a]];
-            pp = Pp[t + s[a]]; pm = Pp[t - s[a]];
-            vp = V[t + s[a]]; vm = V[t - s[a]];
+            rp = PR(0, t + s[a]); rm = PR(0, t - s[a]);
+            pp = PR(1, t + s[a]); pm = PR(1, t - s[a]);
+            vp = PR(2 + a, t + s[a]); vm = PR(2 + a, t - s[a]);
           } else {
             rp = rq[3]; rm = rq[1];
             pp = pq[2]; pm = pq[0];
@@ -152,8 +152,9 @@ __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst 
           if (in2) {
             const double h12 = fc.h12[a], rh12 = fc.rh12[a];
             if (a < 2) {
-              dr[a] = cdiv_rn(R[t - 2 * s[a]] - 8.0 * rm + 8.0 * rp - R[t + 2 * s[a]], h12, rh12);
-              de[a] = cdiv_rn(E[t - 2 * s[a]] - 8.0 * E[t - s[a]] + 8.0 * E[t + s[a]] - E[t + 2 * s[a]], h12, rh12);
+              dr[a] = cdiv_rn(PR(0, t - 2 * s[a]) - 8.0 * rm + 8.0 * rp - PR(0, t + 2 * s[a]), h12, rh12);
+              de[a] = cdiv_rn(PR(5, t - 2 * s[a]) - 8.0 * PR(5, t - s[a]) + 8.0 * PR(5, t + s[a]) - PR(5, t + 2 * s[a]),
+                              h12, rh12);
             } else {
               dr[a] = cdiv_rn(rq[0] - 8.0 * rq[1] + 8.0 * rq[3] - rq[4], h12, rh12);
               de[a] = cdiv_rn(eq[0] - 8.0 * eq[1] + 8.0 * eq[3] - eq[4], h12, rh12);
@@ -163,7 +164,7 @@ __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst 
         const double rho = rq[2], p = pq[1];
         double Cp = 0.0, Cv = 0.0;
         for (int q = 0; q < fc.ns; ++q) {
-          const double Y = P[t + N * (6 + q)];
+          const double Y = PR(6 + q, t);
           Cp = Cp + Y * fc.cpf[q];
           Cv = Cv + Y * fc.cvf[q];
         }
@@ -197,15 +198,15 @@ __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst 
         rq[q] = rq[q + 1];
         eq[q] = eq[q + 1];
       }
-      rq[4] = at(R, k + 3);
-      eq[4] = at(E, k + 3);
+      rq[4] = at(0, k + 3);
+      eq[4] = at(5, k + 3);
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         pq[q] = pq[q + 1];
         wq[q] = wq[q + 1];
       }
-      pq[2] = at(Pp, k + 2);
-      wq[2] = at(W, k + 2);
+      pq[2] = at(1, k + 2);
+      wq[2] = at(4, k + 2);
     }
   }
   // block maximum, then one atomic per block that has a non-zero value
@@ -360,8 +361,8 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__re
                                                    const unsigned *__restrict__ list, const unsigned *__restrict__ cnt,
                                                    const short *__restrict__ Wb, const float *__restrict__ W,
                                                    const float *__restrict__ G1, const double *__restrict__ P,
-                                                   double hmin, float tol, float *__restrict__ chat, Scalars *sc,
-                                                   unsigned *__restrict__ nfallback) {
+                                                   int pf32, double hmin, float tol, float *__restrict__ chat,
+                                                   Scalars *sc, unsigned *__restrict__ nfallback) {
   __shared__ InfLds L;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   for (int q = tid; q < MLP_H * MLP_IN_PAD; q += 256) L.W1[(q / MLP_IN_PAD) * W1S + q % MLP_IN_PAD] = Wb[INF_W1 + q];
@@ -435,7 +436,8 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__re
         yy = gsum * (1.0f / 27.0f);
         ++nfb;
       }
-      const float ch = (float)((double)yy * P[cell] / hmin);
+      const double rho = pf32 ? prim<true>(P, g.N, 0, cell) : prim<false>(P, g.N, 0, cell);
+      const float ch = (float)((double)yy * rho / hmin);
       chat[cell] = ch;
       mloc = fmaxf(mloc, ch);
     }
@@ -609,11 +611,13 @@ FeatConst make_feat(const fst_config &cf, const Phys &ph) {
   return fc;
 }
 
-int features(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, LearnedBufs &L, hipStream_t s) {
+int features(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, bool pf32, LearnedBufs &L,
+             hipStream_t s) {
   const G3 q = make_g3(g);
   k_clear_learned<<<1, 64, 0, s>>>(L.sc, L.cnt);
   const dim3 fg((g.ngi + 63) / 64, (g.ngj + 3) / 4, (g.ngk + FEAT_KCH - 1) / FEAT_KCH);
-  k_features3d<<<fg, dim3(64, 4), 0, s>>>(q, make_feat(cf, ph), P, L.F, L.M, L.dir, L.G1, L.sc);
+  if (pf32) k_features3d<true><<<fg, dim3(64, 4), 0, s>>>(q, make_feat(cf, ph), P, L.F, L.M, L.dir, L.G1, L.sc);
+  else k_features3d<false><<<fg, dim3(64, 4), 0, s>>>(q, make_feat(cf, ph), P, L.F, L.M, L.dir, L.G1, L.sc);
   return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 
@@ -651,16 +655,16 @@ int learned_set_weights(LearnedBufs &L, const float *host, hipStream_t s) {
   return FST_OK;
 }
 
-int learned_fields(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, LearnedBufs &L,
+int learned_fields(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, bool pf32, LearnedBufs &L,
                    hipStream_t s) {
   if (!L.have_weights) return FST_ERR_STATE;
-  int rc = features(g, cf, ph, P, L, s);
+  int rc = features(g, cf, ph, P, pf32, L, s);
   if (rc) return rc;
   const G3 q = make_g3(g);
   k_active3d<<<grid3(g), dim3(64, 4), 0, s>>>(q, L.M, L.tau, 0u, L.list, L.cnt, L.chat);
   const double hmin = std::min(cf.dx, std::min(cf.dy, cf.dz));
-  k_mlp_infer<<<1024, 256, 0, s>>>(q, L.F, L.list, L.cnt, L.Wb, L.W, L.G1, P, hmin, L.tol, L.chat, L.sc,
-                                    L.cnt + 1);
+  k_mlp_infer<<<1024, 256, 0, s>>>(q, L.F, L.list, L.cnt, L.Wb, L.W, L.G1, P, pf32 ? 1 : 0, hmin, L.tol, L.chat,
+                                    L.sc, L.cnt + 1);
   const long long nf = 2LL * ((long long)g.ncj * g.nck + (long long)g.nci * g.nck + (long long)g.nci * g.ncj);
   k_chat_ghost<<<nblk(nf), 256, 0, s>>>(q, cf.xPer, cf.yPer, cf.zPer, L.chat);
   return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
@@ -815,7 +819,7 @@ int trainer_harvest(Trainer &t, const Geo &g, const fst_config &cf, const Phys &
     t.feat_cells = g.N;
   }
   hipStream_t s = t.s;
-  int rc = features(g, cf, ph, P, t.feat, s);
+  int rc = features(g, cf, ph, P, false, t.feat, s);  // plain FP64 contexts: FP64 primitive planes
   if (rc) return rc;
   const G3 q = make_g3(g);
   k_active3d<<<grid3(g), dim3(64, 4), 0, s>>>(q, t.feat.M, t.feat.tau, stride, t.feat.list, t.feat.cnt, nullptr);

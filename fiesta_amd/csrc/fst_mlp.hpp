@@ -55,7 +55,7 @@ int learned_alloc(const Geo &g, LearnedBufs &L);
 // Features of the primitive planes P (rho, p, u, v, w, nrg, Y_s; ghosts
 // filled), then C_hat on the active cells by the MLP, residual check and
 // fallback, ghost layer of C_hat; maxCh into L.sc.  Stream-ordered.
-int learned_fields(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, LearnedBufs &L,
+int learned_fields(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, bool pf32, LearnedBufs &L,
                    hipStream_t s);
 int learned_set_weights(LearnedBufs &L, const float *host, hipStream_t s);
 void learned_free(LearnedBufs &L);
