@@ -1,32 +1,36 @@
-// Fused 3D FP64 stage path (cart3d, src/cart3d.cpp:220-298 + src/rk.cpp:28-72).
+// Fused 3D stage path (cart3d, src/cart3d.cpp:220-298 + src/rk.cpp:28-72).
 //
 // One Runge-Kutta stage = ghost fill (src/bc.cpp:179-249) + ghost primitives +
-// two marching kernels:
+// three marching kernels:
 //   k_adv3d   advective fluxes on every face (WENO5), flux differences,
 //             pressure gradient and buoyancy -> partial RHS of the nv flow
 //             variables (dvar planes);
 //   k_vcr3d   viscous stresses and C-equation fluxes on every face, their
-//             divergences, the C-equation sources, the stage update, and the
-//             primitives + maxima of the new state for the next stage.
-// Every term keeps the reference association and constants, so the result
-// equals the reference kernel sequence bit for bit:
+//             divergences, the stage update of the flow variables, and the
+//             primitives + max wave speed of the new state for the next stage;
+//   k_ceq3d   the C-equation sources and relaxation, the stage update of the
+//             five C variables and max |C_hat| (plain C-equation only).
+// In the FP64 form every term keeps the reference association and constants,
+// so the result equals the reference kernel sequence bit for bit:
 //   * divisions by run constants (dx, 4*dy, 12*dx, 12) use q0 = a*r,
 //     e = fma(-b, q0, a), q = fma(e, r, q0) with r = RN(1/b) (Markstein's
 //     theorem: q is the correctly rounded a/b); other divisions are IEEE.
 //   * per-cell dvar is accumulated in the reference order: advection (x,y,z),
 //     pressure gradient, buoyancy, viscous x, y, z, C-equation.
 //
-// Work decomposition of both kernels (block = 8 waves of 64 lanes, tile =
-// 64 (i) x 7 (j) cells, marching k over a chunk):
-//   waves 0..6  row r: x-face right of its cell, y-face below its row, z-face
-//               above its cell, then the cell itself;
-//   wave 7      y-face above the tile, and the 7 x-faces left of the tile.
+// Work decomposition of the advection and viscous kernels (block = TY+1 waves
+// of 64 lanes, tile = 64 (i) x TY (j) cells, marching k over a chunk; TY = 7,
+// and 11 for the binary32 advection kernel of the FP32 forms):
+//   waves 0..TY-1  row r: x-face right of its cell, y-face below its row,
+//                  z-face above its cell, then the cell itself;
+//   wave TY        y-face above the tile, and the TY x-faces left of the tile.
 // x and y face values are exchanged through LDS; the z-face of the previous
 // plane stays in registers.  Each face is computed exactly once per kernel.
 //
 // Primitive planes P (rho, p, u, v, w, nrg, Y_0..Y_{ns-1}) of the stage input
 // come from the previous stage's epilogue (interior) and a ghost pass; the
-// per-stage maxima (max wave speed, max |C_hat|) likewise.
+// per-stage maxima (max wave speed, max |C_hat|) likewise.  The FP32 forms
+// store them as floats except the pressure (prim(), fst_common.hpp).
 //
 // Addressing: element offsets are 32-bit (checked on the host: every array
 // stays below 2^29 elements) on uniform base pointers.
@@ -129,6 +133,7 @@ struct Tile {
   bool cellok, yfok, xbok;
 };
 
+template <int TYV = TY>
 __device__ __forceinline__ Tile place(const Grid32 &g, int kchunk, int nbx, int nby, int nbz) {
   Tile t;
   const int tid = threadIdx.x;
@@ -141,15 +146,15 @@ __device__ __forceinline__ Tile place(const Grid32 &g, int kchunk, int nbx, int 
   if (hb < full) lb = (hb % 8) * (full / 8) + hb / 8;
   const int bx = lb % nbx, by = (lb / nbx) % nby, bz = lb / (nbx * nby);
   t.i0 = g.ng + bx * TX;
-  t.j0 = g.ng + by * TY;
+  t.j0 = g.ng + by * TYV;
   t.kb = g.ng + bz * kchunk;
   t.ke = min(t.kb + kchunk, g.ng + g.nck);
   const int iend = g.ng + g.nci, jend = g.ng + g.ncj;
   t.i = t.i0 + t.l;
   t.jrow = t.j0 + t.w;
-  t.cellok = (t.w < TY) && (t.jrow < jend) && (t.i < iend);
+  t.cellok = (t.w < TYV) && (t.jrow < jend) && (t.i < iend);
   t.yfok = (t.i < iend) && (t.j0 + t.w - 1 < jend);
-  t.xbok = (t.w == TY) && (t.l < TY) && (t.j0 + t.l < jend);
+  t.xbok = (t.w == TYV) && (t.l < TYV) && (t.j0 + t.l < jend);
   return t;
 }
 
@@ -630,6 +635,197 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) st_dv<F32>(dvar, c + N * v, dv[v]);
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) Zp[q] = Z[q];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- kernel 1, FP32 forms, float plane
+
+// Advection kernel of the FP32 forms with a binary32 LDS plane: q (var(v),
+// plus p for the energy), u and v staged as floats for rows j0-3..j0+TYS+2 and
+// columns i0-3..i0+TX+2, the z stencil as floats in a per-column register
+// queue, WENO5 on deviations from the face-centre value, fluxes in binary32.
+// The pressure gradient and buoyancy, whose cancellation is the hydrostatic
+// balance, stay FP64 (p staged as doubles) and are added to the FP32 flux
+// divergence in FP64; the partial RHS is stored as FP32.
+constexpr int TYS = 11, NWS = TYS + 1, NTS = 64 * NWS;  // tile rows, waves and threads of the float-plane kernel
+constexpr int QYS = TYS + 6, QPLS = QYS * QX;            // its staged plane: rows, cells
+
+template <int NS>
+struct PlaneS {
+  static constexpr int NV = 4 + NS;
+  static constexpr int NQ = NV + 2;  // q_0..q_{nv-1}, u, v
+  static constexpr int FU = NV, FV = NV + 1;
+  static constexpr int NPC = (QPLS + NTS - 1) / NTS;
+};
+
+template <int NS>
+__device__ __forceinline__ void plane_fetch_s(const Grid32 &g, const double *__restrict__ U,
+                                              const double *__restrict__ P, int i0, int j0, int p, float *pre,
+                                              double *prep) {
+  using PS = PlaneS<NS>;
+  const uint32_t N = g.N;
+#pragma unroll
+  for (int m = 0; m < PS::NPC; ++m) {
+    const int e = threadIdx.x + m * NTS;
+    const int r = e / QX, q = e - r * QX;
+    const int i = i0 - 3 + q, j = j0 - 3 + r;
+    const bool ok = e < QPLS && i < g.ngi && j < g.ngj;
+    const uint32_t c = ok ? g.id(i, j, p) : 0u;
+    const double pc = ok ? pget<true>(P, N, 1, c) : 0.0;
+    prep[m] = pc;
+#pragma unroll
+    for (int f = 0; f < PS::NQ; ++f) {
+      float v = 0.f;
+      if (ok) {
+        if (f < PS::NV) v = (float)((f == 3) ? ld(U, c + N * f) + pc : ld(U, c + N * f));
+        else v = (float)pget<true>(P, N, f == PS::FU ? 2 : 3, c);
+      }
+      pre[f * PS::NPC + m] = v;
+    }
+  }
+}
+
+// Fluxes of the nv flow variables on one face from its float stencil
+// (q[t][v] at cells c-2e .. c+3e, vd = vel_D at c-e .. c+2e), divided by d.
+template <int NS>
+__device__ __forceinline__ void flux_s(float rd, const float *vd, const float (*q)[4 + NS], float *out) {
+  const float ud = (-vd[3] + 7.0f * vd[2] + 7.0f * vd[1] - vd[0]) * (1.0f / 12.0f);  // src/flux.hpp:278
+  const bool neg = ud < 0.f;
+#pragma unroll
+  for (int v = 0; v < 4 + NS; ++v) {
+    const float c0 = q[2][v];
+    const float f1 = (neg ? q[5][v] : q[0][v]) - c0;
+    const float f2 = (neg ? q[4][v] : q[1][v]) - c0;
+    const float f3 = (neg ? q[3][v] : q[2][v]) - c0;
+    const float f4 = (neg ? q[2][v] : q[3][v]) - c0;
+    const float f5 = (neg ? q[1][v] : q[4][v]) - c0;
+    out[v] = ud * (c0 + weno5_fast(f1, f2, f3, f4, f5)) * rd;
+  }
+}
+
+template <int NS, int D>
+__device__ __forceinline__ void face_s(const float *qs, float rd, int r, int q, float *out) {
+  using PS = PlaneS<NS>;
+  constexpr int NV = PS::NV;
+  constexpr int st = D == 1 ? QX : 1;
+  constexpr int fv = D == 0 ? PS::FU : PS::FV;
+  const int base = r * QX + q;
+  const float vd[4] = {qs[fv * QPLS + base - st], qs[fv * QPLS + base], qs[fv * QPLS + base + st],
+                       qs[fv * QPLS + base + 2 * st]};
+  float qq[6][NV];
+#pragma unroll
+  for (int tt = 0; tt < 6; ++tt)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) qq[tt][v] = qs[v * QPLS + base + (tt - 2) * st];
+  flux_s<NS>(rd, vd, qq, out);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(NTS, 1)
+    k_adv3d_s(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
+              double *__restrict__ dvar, int buoy, int kchunk, int nbx, int nby, int nbz) {
+  using PS = PlaneS<NS>;
+  constexpr int NV = 4 + NS;
+  __shared__ float qs[PS::NQ * QPLS];
+  __shared__ double ps[QPLS];
+  __shared__ float xsh[TYS][NV][TX + 1];  // index 0: left of the tile; l+1: right of lane l
+  __shared__ float ysh[NWS][NV][TX];   // row w: face below tile row w
+  const Tile t = place<TYS>(g, kchunk, nbx, nby, nbz);
+  const int w = t.w, l = t.l;
+  const uint32_t N = g.N;
+  const float rdx = (float)k.rd[0], rdy = (float)k.rd[1], rdz = (float)k.rd[2];
+  float qz[6][NV], vz[4], Z[NV], Zp[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) Z[q] = Zp[q] = 0.f;
+  auto load_qf = [&](uint32_t cc, float *q) {
+    const double pc = pget<true>(P, N, 1, cc);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) q[v] = (float)((v == 3) ? ld(U, cc + N * v) + pc : ld(U, cc + N * v));
+  };
+  const uint32_t c00 = g.id(t.i, t.jrow, t.kb - 1);
+  if (t.cellok) {
+#pragma unroll
+    for (int s = 0; s < 6; ++s) load_qf(c00 + (uint32_t)(s - 2) * g.s2, qz[s]);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) vz[s] = (float)pget<true>(P, N, 4, c00 + (uint32_t)(s - 1) * g.s2);
+    flux_s<NS>(rdz, vz, qz, Zp);
+  }
+  float pre[PS::NQ * PS::NPC];
+  double prep[PS::NPC];
+  plane_fetch_s<NS>(g, U, P, t.i0, t.j0, t.kb, pre, prep);
+  for (int kk = t.kb; kk < t.ke; ++kk) {
+    uint32_t c = g.id(t.i, t.jrow, kk);
+    asm volatile("" : "+v"(c));
+#pragma unroll
+    for (int m = 0; m < PS::NPC; ++m) {
+      const int e = threadIdx.x + m * NTS;
+      if (e < QPLS) {
+#pragma unroll
+        for (int f = 0; f < PS::NQ; ++f) qs[f * QPLS + e] = pre[f * PS::NPC + m];
+        ps[e] = prep[m];
+      }
+    }
+    __syncthreads();
+    if (kk + 1 < t.ke) plane_fetch_s<NS>(g, U, P, t.i0, t.j0, kk + 1, pre, prep);
+    const int rc = w + 3, qc = l + 3;  // plane coordinates of this lane's cell
+    if (t.cellok) {
+      float X[NV];
+      face_s<NS, 0>(qs, rdx, rc, qc, X);
+#pragma unroll
+      for (int q = 0; q < NV; ++q) xsh[w][q][l + 1] = X[q];
+    }
+    if (t.xbok) {  // x-face left of the tile, row l: cell (i0-1, j0+l)
+      float X[NV];
+      face_s<NS, 0>(qs, rdx, l + 3, 2, X);
+#pragma unroll
+      for (int q = 0; q < NV; ++q) xsh[l][q][0] = X[q];
+    }
+    if (t.yfok) {  // y-face below row w: cell (i0+l, j0+w-1)
+      float Y[NV];
+      face_s<NS, 1>(qs, rdy, w + 2, qc, Y);
+#pragma unroll
+      for (int q = 0; q < NV; ++q) ysh[w][q][l] = Y[q];
+    }
+    if (t.cellok) {
+#pragma unroll
+      for (int s = 0; s < 5; ++s)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qz[s][v] = qz[s + 1][v];
+      load_qf(c + 3 * g.s2, qz[5]);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) vz[s] = vz[s + 1];
+      vz[3] = (float)pget<true>(P, N, 4, c + 2 * g.s2);
+      flux_s<NS>(rdz, vz, qz, Z);
+    }
+    __syncthreads();
+    if (t.cellok) {
+      double dv[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        dv[v] = (double)(-((xsh[w][v][l + 1] - xsh[w][v][l]) + (ysh[w + 1][v][l] - ysh[w][v][l]) + (Z[v] - Zp[v])));
+      const double *pp = ps + rc * QX + qc;
+      const uint32_t sz = g.s2;
+      const double dxp = cdiv(pp[-2] - 8.0 * pp[-1] + 8.0 * pp[1] - pp[2], k.pg[0], k.rpg[0]);
+      const double dyp = cdiv(pp[-2 * QX] - 8.0 * pp[-QX] + 8.0 * pp[QX] - pp[2 * QX], k.pg[1], k.rpg[1]);
+      const double dzp = cdiv(ld(P, c - 2 * sz) - 8.0 * ld(P, c - sz) + 8.0 * ld(P, c + sz) - ld(P, c + 2 * sz),
+                              k.pg[2], k.rpg[2]);  // FP32 forms: P starts with the FP64 pressure plane
+      dv[0] = dv[0] - dxp;
+      dv[1] = dv[1] - dyp;
+      dv[2] = dv[2] - dzp;
+      if (buoy) {
+        const double rho = pget<true>(P, N, 0, c);
+        const double vv = ld(U, c + N) / rho;
+        const double rhop = rho - k.rhoRef;
+        const double f = -k.g * rhop;
+        dv[1] += f;
+        dv[3] += vv * f;
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) st_dv<true>(dvar, c + N * v, dv[v]);
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) Zp[q] = Z[q];
@@ -1450,10 +1646,14 @@ void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf,
                const float *DR, const Scalars *scin, Scalars *scout, double h, int stage, int kchunk,
                Prof *prof, hipStream_t s) {
   const Launch L = shape(g, kchunk);
-  if (!getenv("FST_ADV_GLOBAL"))
-    k_adv3d_lds<NS, F32><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
-  else
-    k_adv3d<NS, F32><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
+  if (F32) {  // FP32 forms: binary32 plane, 11-row tiles (one implementation)
+    const int nbys = (g.ncj + TYS - 1) / TYS, nbs = L.nbx * nbys * L.nbz;
+    k_adv3d_s<NS><<<nbs, NTS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, nbys, L.nbz);
+  } else if (!getenv("FST_ADV_GLOBAL")) {
+    k_adv3d_lds<NS, false><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
+  } else {
+    k_adv3d<NS, false><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
+  }
   if (prof) prof->mark(s);  // end of the advection kernel
   if (!getenv("FST_VCR_GLOBAL"))
     k_vcr3d_ring<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h,
