@@ -259,13 +259,16 @@ def main():
                 "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes": "%d B per cell-stage x %d cells per stage" % (balg, d.ncells)}
         ph = r.get("phases") or []
-        if len(ph) == 4:
-            names = ("ghost_fill_and_ghost_primitives", "learned_c_fields", "advection_kernel", "viscous_ceq_rk_kernel")
+        if len(ph) == 5:
+            names = ("ghost_fill_and_ghost_primitives", "learned_c_fields", "advection_kernel",
+                     "viscous_rk_kernel", "ceq_kernel")
             roof["phases_ms"] = {n: round(v / r["launches"], 4) for n, v in zip(names, ph)}
             ns = r["cfg"].ns
-            roof["dominant_kernel"] = {"name": "k_vcr3d_ring<%d, false, false>" % ns,
-                                       "avg_ms": round(ph[3] / r["launches"], 4),
-                                       "share_of_stage": round(ph[3] / r["kernel_ms"], 3)}
+            kern = {2: "k_adv3d_lds<%d, false>" % ns, 3: "k_vcr3d_ring<%d, false, false>" % ns,
+                    4: "k_ceq3d<false>"}
+            top = max(kern, key=lambda q: ph[q])
+            roof["dominant_kernel"] = {"name": kern[top], "avg_ms": round(ph[top] / r["launches"], 4),
+                                       "share_of_stage": round(ph[top] / r["kernel_ms"], 3)}
         tr = measured_traffic(args.size, "fp64")
         if tr:
             roof["traffic"] = tr["bytes_per_stage"]
@@ -308,7 +311,7 @@ def main():
             e["error_vs_fp64"] = form_errors(q["cfg"], q["state"], ref, r["cfg"])
         if "learned" in q:
             e["learned"] = q["learned"]
-        if len(q.get("phases") or []) == 4 and q["launches"]:
+        if len(q.get("phases") or []) == 5 and q["launches"]:
             e["phases_ms"] = [round(v / q["launches"], 4) for v in q["phases"]]
         tr = measured_traffic(args.size, f)
         if tr:

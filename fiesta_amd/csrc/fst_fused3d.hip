@@ -77,6 +77,19 @@ __device__ __forceinline__ void st(double *__restrict__ base, uint32_t off, doub
   *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + (off << 3)) = v;
 }
 
+// State value at element offset off of U (doubles), or of the float
+// Runge-Kutta stage state of the learned FP32 form (UF).
+template <bool UF>
+__device__ __forceinline__ double uld(const double *__restrict__ U, uint32_t off) {
+  if (UF) return (double)reinterpret_cast<const float *>(U)[off];
+  return ld(U, off);
+}
+template <bool UF>
+__device__ __forceinline__ void ust(double *__restrict__ U, uint32_t off, double v) {
+  if (UF) reinterpret_cast<float *>(U)[off] = (float)v;
+  else st(U, off, v);
+}
+
 // Primitive planes of cell c with 32-bit offsets, in the layout of prim()
 // (fst_common.hpp): FP64 form all doubles; FP32 forms p as doubles, then
 // rho, u, v, w, nrg, Y_s as floats.
@@ -662,7 +675,7 @@ struct PlaneS {
   static constexpr int NPC = (QPLS + NTS - 1) / NTS;
 };
 
-template <int NS>
+template <int NS, bool FI>
 __device__ __forceinline__ void plane_fetch_s(const Grid32 &g, const double *__restrict__ U,
                                               const double *__restrict__ P, int i0, int j0, int p, float *pre,
                                               double *prep) {
@@ -681,7 +694,7 @@ __device__ __forceinline__ void plane_fetch_s(const Grid32 &g, const double *__r
     for (int f = 0; f < PS::NQ; ++f) {
       float v = 0.f;
       if (ok) {
-        if (f < PS::NV) v = (float)((f == 3) ? ld(U, c + N * f) + pc : ld(U, c + N * f));
+        if (f < PS::NV) v = (float)((f == 3) ? uld<FI>(U, c + N * f) + pc : uld<FI>(U, c + N * f));
         else v = (float)pget<true>(P, N, f == PS::FU ? 2 : 3, c);
       }
       pre[f * PS::NPC + m] = v;
@@ -724,7 +737,8 @@ __device__ __forceinline__ void face_s(const float *qs, float rd, int r, int q, 
   flux_s<NS>(rd, vd, qq, out);
 }
 
-template <int NS>
+// FI: U is the float stage state of the learned FP32 form.
+template <int NS, bool FI>
 __global__ void __launch_bounds__(NTS, 1)
     k_adv3d_s(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
               double *__restrict__ dvar, int buoy, int kchunk, int nbx, int nby, int nbz) {
@@ -744,7 +758,7 @@ __global__ void __launch_bounds__(NTS, 1)
   auto load_qf = [&](uint32_t cc, float *q) {
     const double pc = pget<true>(P, N, 1, cc);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) q[v] = (float)((v == 3) ? ld(U, cc + N * v) + pc : ld(U, cc + N * v));
+    for (int v = 0; v < NV; ++v) q[v] = (float)((v == 3) ? uld<FI>(U, cc + N * v) + pc : uld<FI>(U, cc + N * v));
   };
   const uint32_t c00 = g.id(t.i, t.jrow, t.kb - 1);
   if (t.cellok) {
@@ -756,7 +770,7 @@ __global__ void __launch_bounds__(NTS, 1)
   }
   float pre[PS::NQ * PS::NPC];
   double prep[PS::NPC];
-  plane_fetch_s<NS>(g, U, P, t.i0, t.j0, t.kb, pre, prep);
+  plane_fetch_s<NS, FI>(g, U, P, t.i0, t.j0, t.kb, pre, prep);
   for (int kk = t.kb; kk < t.ke; ++kk) {
     uint32_t c = g.id(t.i, t.jrow, kk);
     asm volatile("" : "+v"(c));
@@ -770,7 +784,7 @@ __global__ void __launch_bounds__(NTS, 1)
       }
     }
     __syncthreads();
-    if (kk + 1 < t.ke) plane_fetch_s<NS>(g, U, P, t.i0, t.j0, kk + 1, pre, prep);
+    if (kk + 1 < t.ke) plane_fetch_s<NS, FI>(g, U, P, t.i0, t.j0, kk + 1, pre, prep);
     const int rc = w + 3, qc = l + 3;  // plane coordinates of this lane's cell
     if (t.cellok) {
       float X[NV];
@@ -818,7 +832,7 @@ __global__ void __launch_bounds__(NTS, 1)
       dv[2] = dv[2] - dzp;
       if (buoy) {
         const double rho = pget<true>(P, N, 0, c);
-        const double vv = ld(U, c + N) / rho;
+        const double vv = uld<FI>(U, c + N) / rho;
         const double rhop = rho - k.rhoRef;
         const double f = -k.g * rhop;
         dv[1] += f;
@@ -841,7 +855,8 @@ __global__ void __launch_bounds__(NTS, 1)
 // then the primitives and the maxima of the new state.  dvar accumulates in
 // FP64 in both forms.  LC: learned C-indicator (C_hat, n from CH, DR; no
 // C-equation variables, nvt = nv; maxCh comes from the inference pass).
-template <int NS, bool F32, bool LC>
+// FO: Uo is the float stage state of the learned FP32 form (stage 1).
+template <int NS, bool F32, bool LC, bool FO = false>
 __global__ void __launch_bounds__(NTHREADS, 1)
     k_vcr3d(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ Un,
             double *__restrict__ Uo, const double *__restrict__ P, double *__restrict__ Po,
@@ -954,7 +969,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       for (int v = 0; v < NV; ++v) {
         const double base = (stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v);
         un[v] = base + h * dv[v];
-        st(Uo, c + N * v, un[v]);
+        ust<FO>(Uo, c + N * v, un[v]);
       }
       double pr[6 + NS];
       double ms;
@@ -1158,7 +1173,7 @@ __device__ __forceinline__ void face_vc_ring(const FC &k, const RingAt<NS, F32, 
   }
 }
 
-template <int NS, bool F32, bool LC>
+template <int NS, bool F32, bool LC, bool FO = false>
 __global__ void __launch_bounds__(NTHREADS, 1)
     k_vcr3d_ring(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ Un,
                 double *__restrict__ Uo, const double *__restrict__ P, double *__restrict__ Po,
@@ -1317,7 +1332,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       for (int v = 0; v < NV; ++v) {
         const double base = EARLY ? bsp[EARLY ? v : 0] : ((stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v));
         un[v] = base + h * dv[v];
-        st(Uo, c + N * v, un[v]);
+        ust<FO>(Uo, c + N * v, un[v]);
       }
       double pr[6 + NS];
       double ms;
@@ -1523,7 +1538,8 @@ __global__ void __launch_bounds__(CNT)
 // (ghost_only = 0), or over the ghost cells only (ghost_only = 1: threads
 // enumerate the k slabs, then the j slabs, then the i slabs of the ghost
 // shell).  Threads 0..3 of block 0 clear `clear` when given.
-template <int NS, bool F32>
+// FU: U is the float stage state of the learned FP32 form.
+template <int NS, bool F32, bool FU = false>
 __global__ void k_prims3d(const Grid32 g, const FC k, const double *__restrict__ U, double *__restrict__ P,
                           Scalars *sc, int ghost_only, Scalars *clear) {
   if (clear && blockIdx.x == 0 && threadIdx.x < 4) reinterpret_cast<unsigned long long *>(clear)[threadIdx.x] = 0ull;
@@ -1566,13 +1582,13 @@ __global__ void k_prims3d(const Grid32 g, const FC k, const double *__restrict__
     const uint32_t N = g.N;
     double u[4 + NS];
 #pragma unroll
-    for (int v = 0; v < 4 + NS; ++v) u[v] = ld(U, cell + N * v);
+    for (int v = 0; v < 4 + NS; ++v) u[v] = uld<FU>(U, cell + N * v);
     double pr[6 + NS];
     prims_of<NS>(k, u, pr, ms);
 #pragma unroll
     for (int q = 0; q < 6 + NS; ++q) pset<F32>(P, N, q, cell, pr[q]);
     if (!interior) ms = 0.0;
-    else if (g.nvt > g.nv) mc = fabs(ld(U, cell + N * (g.nv + 1)));
+    else if (g.nvt > g.nv) mc = fabs(uld<FU>(U, cell + N * (g.nv + 1)));
   }
   if (!ghost_only && sc) {  // uniform over the block
     double v2[2] = {ms, mc};
@@ -1640,27 +1656,47 @@ Launch shape(const Geo &g, int kchunk) {
   return L;
 }
 
+// fs: the stage state U* is float (learned FP32 form): stage 1 writes it,
+// stage 2 reads it; U^n and U^{n+1} stay FP64.
 template <int NS, bool F32, bool LC>
 void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf, const double *U,
                const double *Un, double *Uo, const double *P, double *Po, double *dvar, const float *CH,
                const float *DR, const Scalars *scin, Scalars *scout, double h, int stage, int kchunk,
-               Prof *prof, hipStream_t s) {
+               Prof *prof, hipStream_t s, bool fs) {
   const Launch L = shape(g, kchunk);
   if (F32) {  // FP32 forms: binary32 plane, 11-row tiles (one implementation)
     const int nbys = (g.ncj + TYS - 1) / TYS, nbs = L.nbx * nbys * L.nbz;
-    k_adv3d_s<NS><<<nbs, NTS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, nbys, L.nbz);
+    if (fs && stage == 2)
+      k_adv3d_s<NS, true><<<nbs, NTS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, nbys, L.nbz);
+    else
+      k_adv3d_s<NS, false><<<nbs, NTS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, nbys, L.nbz);
   } else if (!getenv("FST_ADV_GLOBAL")) {
     k_adv3d_lds<NS, false><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
   } else {
     k_adv3d<NS, false><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, cf.buoyancy, kchunk, L.nbx, L.nby, L.nbz);
   }
   if (prof) prof->mark(s);  // end of the advection kernel
-  if (!getenv("FST_VCR_GLOBAL"))
+  const bool ring = !getenv("FST_VCR_GLOBAL");
+  if constexpr (F32 && LC) {
+    if (fs && stage == 1) {
+      if (ring)
+        k_vcr3d_ring<NS, F32, LC, true><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout,
+                                                                  h, stage, cf.visc, cf.ceq, kchunk, L.nbx, L.nby,
+                                                                  L.nbz);
+      else
+        k_vcr3d<NS, F32, LC, true><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h,
+                                                             stage, cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
+      if (prof) prof->mark(s);
+      return;
+    }
+  }
+  if (ring)
     k_vcr3d_ring<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h,
                                                        stage, cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
   else
     k_vcr3d<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h, stage,
                                                   cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
+  if (prof) prof->mark(s);  // end of the viscous / stage-update kernel
   if (cf.ceq && !LC) {  // the C variables: sources, relaxation and stage update
     const int kc = 16;
     const int nbx = (g.nci + 63) / 64, nby = (g.ncj + CY - 1) / CY, nbz = (g.nck + kc - 1) / kc;
@@ -1670,18 +1706,21 @@ void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf,
 
 template <int NS, bool F32>
 void run_prims(const Grid32 &q, const FC &k, const double *U, double *P, Scalars *sc, int ghost_only,
-               Scalars *clear, hipStream_t s) {
+               Scalars *clear, hipStream_t s, bool uf) {
   uint32_t n = q.N;
   if (ghost_only)
     n = 2u * q.ng * q.s2 + 2u * q.ng * (uint32_t)q.ngi * q.nck + 2u * q.ng * (uint32_t)q.ncj * q.nck;
-  k_prims3d<NS, F32><<<(n + 255) / 256, 256, 0, s>>>(q, k, U, P, sc, ghost_only, clear);
+  if (F32 && uf)  // the float stage state of the learned FP32 form
+    k_prims3d<NS, F32, true><<<(n + 255) / 256, 256, 0, s>>>(q, k, U, P, sc, ghost_only, clear);
+  else
+    k_prims3d<NS, F32><<<(n + 255) / 256, 256, 0, s>>>(q, k, U, P, sc, ghost_only, clear);
 }
 
 using StageFn = void (*)(const Grid32 &, const Geo &, const FC &, const fst_config &, const double *,
                          const double *, double *, const double *, double *, double *, const float *,
-                         const float *, const Scalars *, Scalars *, double, int, int, Prof *, hipStream_t);
+                         const float *, const Scalars *, Scalars *, double, int, int, Prof *, hipStream_t, bool);
 using PrimFn = void (*)(const Grid32 &, const FC &, const double *, double *, Scalars *, int, Scalars *,
-                        hipStream_t);
+                        hipStream_t, bool);
 
 template <int NS>
 StageFn pick_ns(bool f32, bool lc) {
@@ -1737,28 +1776,33 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
   }
   const FC k = make_fc(ph, cf);
   const Grid32 q = make_grid(g);
-  int kchunk = fb.kchunk > 0 ? fb.kchunk : 8;
+  // planes per block: 8 for the FP64 kernels, 32 for the FP32 forms (k-chunk
+  // sweeps at 256^3 over 4..32 planes)
+  int kchunk = fb.kchunk > 0 ? fb.kchunk : (cf.precision == FST_FP32_BULK ? 32 : 8);
   if (const char *ev = getenv("FST_KCHUNK")) kchunk = std::max(1, atoi(ev));  // tuning override
   double *Pa = fb.prim, *Pb = fb.prim + (size_t)g.N * np;
   Scalars *S0 = fb.sc, *S1 = fb.sc + 1;
   Prof *prof = b.prof && b.prof->on ? b.prof : nullptr;
   if (prof) {
     prof->name = lc ? "stage: ghost fill + ghost primitives + [learned C fields, stage 1] + k_adv3d + k_vcr3d"
-                    : "stage: ghost fill + ghost primitives + k_adv3d + k_vcr3d";
-    prof->per = 5;  // begin | ghosts | learned fields | advection | viscous + RK
+                    : "stage: ghost fill + ghost primitives + k_adv3d + k_vcr3d + k_ceq3d";
+    prof->per = 6;  // begin | ghosts | learned fields | advection | viscous + RK | C-equation
   }
   const float *CH = lc ? lb->chat : nullptr, *DR = lc ? lb->dir : nullptr;
   if (!fb.valid) {  // primitives and maxima of var, all cells
     k_clear<<<1, 64, 0, s>>>(S0);
-    pf(q, k, b.var, Pa, S0, 0, nullptr, s);
+    pf(q, k, b.var, Pa, S0, 0, nullptr, s, false);
     fb.valid = true;
   }
   const double h1 = 0.5f * cf.dt, h2 = cf.dt;
+  // learned FP32 form: the stage state U* is stored as floats (it only feeds
+  // the binary32 right-hand side of stage 2; U^n and U^{n+1} stay FP64)
+  const bool fs = lc && cf.precision == FST_FP32_BULK;
   for (int t = 0; t < nsteps; ++t) {
     // stage 1: U^n (b.var) -> U* (fb.stage); Pa/S0 hold the primitives/maxima of U^n
     if (prof) prof->begin(s);
     launch_apply_bcs(g, cf, b.var, s);       // src/cart3d.cpp:221
-    pf(q, k, b.var, Pa, nullptr, 1, S1, s);  // ghost primitives; clears S1
+    pf(q, k, b.var, Pa, nullptr, 1, S1, s, false);  // ghost primitives; clears S1
     if (prof) prof->mark(s);
     if (lc) {  // C_hat and n of U^n, used by both stages of the step
       const int rc = learned_fields(g, cf, ph, Pa, cf.precision == FST_FP32_BULK, *lb, s);
@@ -1768,17 +1812,20 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
       }
     }
     if (prof) prof->mark(s);
-    sf(q, g, k, cf, b.var, nullptr, fb.stage, Pa, Pb, b.dvar, CH, DR, lc ? lb->sc : S0, S1, h1, 1, kchunk, prof, s);
+    sf(q, g, k, cf, b.var, nullptr, fb.stage, Pa, Pb, b.dvar, CH, DR, lc ? lb->sc : S0, S1, h1, 1, kchunk, prof, s,
+       fs);
     if (prof) prof->end(s);
     // stage 2: U* -> U^{n+1}, written in place into b.var (read pointwise)
     if (prof) prof->begin(s);
-    launch_apply_bcs(g, cf, fb.stage, s);
-    pf(q, k, fb.stage, Pb, nullptr, 1, S0, s);  // ghost primitives of U*; clears S0
+    if (fs) launch_apply_bcs(g, cf, reinterpret_cast<float *>(fb.stage), s);
+    else launch_apply_bcs(g, cf, fb.stage, s);
+    pf(q, k, fb.stage, Pb, nullptr, 1, S0, s, fs);  // ghost primitives of U*; clears S0
     if (prof) {
       prof->mark(s);
       prof->mark(s);  // no learned fields in stage 2
     }
-    sf(q, g, k, cf, fb.stage, b.var, b.var, Pb, Pa, b.dvar, CH, DR, lc ? lb->sc : S1, S0, h2, 2, kchunk, prof, s);
+    sf(q, g, k, cf, fb.stage, b.var, b.var, Pb, Pa, b.dvar, CH, DR, lc ? lb->sc : S1, S0, h2, 2, kchunk, prof, s,
+       fs);
     if (prof) prof->end(s);
   }
   return FST_OK;
@@ -1790,7 +1837,7 @@ int fused_primitives(const Geo &g, const Phys &ph, const fst_config &cf, const d
   if (!pick(cf.ns, false, false, &sf, &pf) || g.ndim != 3) return FST_ERR_ARG;
   const FC k = make_fc(ph, cf);
   const Grid32 q = make_grid(g);
-  pf(q, k, U, P, nullptr, 0, nullptr, s);
+  pf(q, k, U, P, nullptr, 0, nullptr, s, false);
   return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 

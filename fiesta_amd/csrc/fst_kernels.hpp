@@ -117,6 +117,7 @@ __device__ __forceinline__ double weno5_f64c(double f1, double f2, double f3, do
 void launch_eos_vel(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
 void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);
 void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream_t s);
+void launch_apply_bcs(const Geo &g, const fst_config &cf, float *var, hipStream_t s);  // float stage state
 void launch_rk_stage(const Geo &g, int stage, double dt, double *var, double *tmp, const double *dvar, hipStream_t s);
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 void launch_secondary(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);

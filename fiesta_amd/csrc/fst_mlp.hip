@@ -97,6 +97,25 @@ inline dim3 grid3(const Geo &g) { return dim3((g.ngi + 63) / 64, (g.ngj + 3) / 4
 // p and w at k-1..k+1) in registers, so each plane is read once for z.
 constexpr int FEAT_KCH = 16;
 
+// Arithmetic of the feature pass: FP64 with IEEE divisions (FP64 primitive
+// planes; equal to the oracle's features3d bit for bit), or binary32 with
+// hardware reciprocals when the planes are the FP32 forms' (the features
+// feed a BF16 network).
+template <bool PF>
+struct FeatAr {
+  using T = double;
+  static __device__ __forceinline__ T cdv(T a, double b, double rb) { return cdiv_rn(a, b, rb); }
+  static __device__ __forceinline__ T div(T a, T b) { return a / b; }
+  static __device__ __forceinline__ T sq(T a) { return sqrt(a); }
+};
+template <>
+struct FeatAr<true> {
+  using T = float;
+  static __device__ __forceinline__ T cdv(T a, double, double rb) { return a * (float)rb; }
+  static __device__ __forceinline__ T div(T a, T b) { return a * __builtin_amdgcn_rcpf(b); }
+  static __device__ __forceinline__ T sq(T a) { return __builtin_sqrtf(a); }
+};
+
 template <bool PF>
 __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst fc, const double *__restrict__ P,
                                                     float *__restrict__ F, float *__restrict__ M,
@@ -105,10 +124,12 @@ __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst 
   const int k0 = blockIdx.z * FEAT_KCH, k1 = min(k0 + FEAT_KCH, g.ngk);
   const bool col = i < g.ngi && j < g.ngj;
   const long long N = g.N;
-  auto PR = [&](int q, long long t) { return prim<PF>(P, N, q, t); };  // primitive plane q at cell t
+  using A = FeatAr<PF>;
+  using T = typename A::T;
+  auto PR = [&](int q, long long t) { return (T)prim<PF>(P, N, q, t); };  // primitive plane q at cell t
   const long long cij = (long long)i + g.s1 * j;
-  auto at = [&](int q, int k) { return (k >= 0 && k < g.ngk) ? PR(q, cij + g.s2 * k) : 0.0; };
-  double rq[5], eq[5], pq[3], wq[3];  // rho, nrg at k-2..k+2; p, w at k-1..k+1
+  auto at = [&](int q, int k) { return (k >= 0 && k < g.ngk) ? PR(q, cij + g.s2 * k) : (T)0; };
+  T rq[5], eq[5], pq[3], wq[3];  // rho, nrg at k-2..k+2; p, w at k-1..k+1
   if (col) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
@@ -130,11 +151,11 @@ __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst 
       float f0 = 0.f, f1 = 0.f, f2 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f, gg = 0.f;
       if (in1) {
         const long long s[2] = {1, g.s1};
-        double gr2 = 0.0, gp2 = 0.0, divu = 0.0, dr[3] = {0, 0, 0}, de[3] = {0, 0, 0};
+        T gr2 = 0, gp2 = 0, divu = 0, dr[3] = {0, 0, 0}, de[3] = {0, 0, 0};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           const double h2 = fc.h2[a], rh2 = fc.rh2[a];  // divisions by run constants: cdiv_rn == IEEE
-          double rp, rm, pp, pm, vp, vm;
+          T rp, rm, pp, pm, vp, vm;
           if (a < 2) {
             rp = PR(0, t + s[a]); rm = PR(0, t - s[a]);
             pp = PR(1, t + s[a]); pm = PR(1, t - s[a]);
@@ -144,41 +165,42 @@ __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst 
             pp = pq[2]; pm = pq[0];
             vp = wq[2]; vm = wq[0];
           }
-          const double r1 = cdiv_rn(rp - rm, h2, rh2);
-          const double p1 = cdiv_rn(pp - pm, h2, rh2);
+          const T r1 = A::cdv(rp - rm, h2, rh2);
+          const T p1 = A::cdv(pp - pm, h2, rh2);
           gr2 = gr2 + r1 * r1;
           gp2 = gp2 + p1 * p1;
-          divu = divu + cdiv_rn(vp - vm, h2, rh2);
+          divu = divu + A::cdv(vp - vm, h2, rh2);
           if (in2) {
             const double h12 = fc.h12[a], rh12 = fc.rh12[a];
             if (a < 2) {
-              dr[a] = cdiv_rn(PR(0, t - 2 * s[a]) - 8.0 * rm + 8.0 * rp - PR(0, t + 2 * s[a]), h12, rh12);
-              de[a] = cdiv_rn(PR(5, t - 2 * s[a]) - 8.0 * PR(5, t - s[a]) + 8.0 * PR(5, t + s[a]) - PR(5, t + 2 * s[a]),
-                              h12, rh12);
+              dr[a] = A::cdv(PR(0, t - 2 * s[a]) - (T)8.0 * rm + (T)8.0 * rp - PR(0, t + 2 * s[a]), h12, rh12);
+              de[a] = A::cdv(PR(5, t - 2 * s[a]) - (T)8.0 * PR(5, t - s[a]) + (T)8.0 * PR(5, t + s[a]) -
+                                 PR(5, t + 2 * s[a]), h12, rh12);
             } else {
-              dr[a] = cdiv_rn(rq[0] - 8.0 * rq[1] + 8.0 * rq[3] - rq[4], h12, rh12);
-              de[a] = cdiv_rn(eq[0] - 8.0 * eq[1] + 8.0 * eq[3] - eq[4], h12, rh12);
+              dr[a] = A::cdv(rq[0] - (T)8.0 * rq[1] + (T)8.0 * rq[3] - rq[4], h12, rh12);
+              de[a] = A::cdv(eq[0] - (T)8.0 * eq[1] + (T)8.0 * eq[3] - eq[4], h12, rh12);
             }
           }
         }
-        const double rho = rq[2], p = pq[1];
-        double Cp = 0.0, Cv = 0.0;
+        const T rho = rq[2], p = pq[1];
+        T Cp = 0, Cv = 0;
         for (int q = 0; q < fc.ns; ++q) {
-          const double Y = PR(6 + q, t);
-          Cp = Cp + Y * fc.cpf[q];
-          Cv = Cv + Y * fc.cvf[q];
+          const T Y = PR(6 + q, t);
+          Cp = Cp + Y * (T)fc.cpf[q];
+          Cv = Cv + Y * (T)fc.cvf[q];
         }
-        const double a_snd = sqrt((Cp / Cv) * p / rho);
-        f0 = (float)(sqrt(gr2) * fc.hmin / rho);
-        f1 = (float)(sqrt(gp2) * fc.hmin / p);
-        f2 = (float)(divu * fc.hmin / a_snd);
+        const T hmin = (T)fc.hmin;
+        const T a_snd = A::sq(A::div(A::div(Cp, Cv) * p, rho));
+        f0 = (float)A::div(A::sq(gr2) * hmin, rho);
+        f1 = (float)A::div(A::sq(gp2) * hmin, p);
+        f2 = (float)A::div(divu * hmin, a_snd);
         if (in2) {
           d0 = (float)dr[0];
           d1 = (float)dr[1];
           d2 = (float)dr[2];
-          const double rg2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-          const double dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) * rg2;
-          gg = dnednr < 0.0 ? (float)(sqrt(rg2) * fc.hmin / rho) : 0.f;
+          const T rg2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+          const T dnednr = (dr[0] * de[0] + dr[1] * de[1] + dr[2] * de[2]) * rg2;
+          gg = dnednr < (T)0 ? (float)A::div(A::sq(rg2) * hmin, rho) : 0.f;
           const bool interior = i >= g.ng && j >= g.ng && k >= g.ng && i < g.ng + g.nci && j < g.ng + g.ncj &&
                                 k < g.ng + g.nck;
           if (interior) g1max = fmaxf(g1max, gg);

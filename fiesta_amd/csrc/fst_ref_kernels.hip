@@ -654,7 +654,10 @@ __global__ void k_applyceq2d(Geo g, Phys ph, Range r, const double *__restrict__
 // ------------------------------------------------------------- ghosts / RK
 
 // bc_xPer / bc_yPer / bc_zPer, src/bc.cpp:133-176: one thread per (a, b, v)
-__global__ void k_bc_periodic(Geo g, int dir, double *__restrict__ var) {
+// Ghost fills work on double state arrays and, in the learned FP32 form, on
+// its float Runge-Kutta stage state (T = float; arithmetic in double).
+template <typename T>
+__global__ void k_bc_periodic(Geo g, int dir, T *__restrict__ var) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int na = dir == 0 ? g.ngj : g.ngi;
   int nb = dir == 2 ? g.ngj : g.ngk;
@@ -664,7 +667,7 @@ __global__ void k_bc_periodic(Geo g, int dir, double *__restrict__ var) {
   t /= na;
   int b = (int)(t % nb);
   int v = (int)(t / nb);
-  double *u = var + g.N * v;
+  T *u = var + g.N * v;
   const int ng = g.ng;
   for (int gg = 0; gg < ng; ++gg) {
     if (dir == 0) {
@@ -685,20 +688,22 @@ __global__ void k_bc_periodic(Geo g, int dir, double *__restrict__ var) {
 // i = nci + ng + r from i = ng + r (sources and targets never overlap: a
 // periodic direction has at least ng interior cells), so a wave touches 8
 // rows with adjacent addresses.
-__global__ void k_bc_xper(Geo g, double *__restrict__ var) {
+template <typename T>
+__global__ void k_bc_xper(Geo g, T *__restrict__ var) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long row = t >> 3;
   const int r = (int)(t & 7);
   const long long nrow = (long long)g.ngj * g.ngk * g.nvt;
   const int ng = g.ng;
   if (row >= nrow || r >= 2 * ng) return;
-  double *u = var + row * g.ngi;  // rows of one variable plane are contiguous, planes too
+  T *u = var + row * g.ngi;  // rows of one variable plane are contiguous, planes too
   if (r < ng) u[r] = u[g.nci + r];
   else u[g.nci + r] = u[r];  // i = nci + ng + (r - ng) from i = ng + (r - ng)
 }
 
 // bc_gen, src/bc.cpp:33-131, on one boundary plane (i0 / j0 / k0 fixed)
-__global__ void k_bc_gen(Geo g, int type, int ih, int jh, int kh, int plane, double *__restrict__ var) {
+template <typename T>
+__global__ void k_bc_gen(Geo g, int type, int ih, int jh, int kh, int plane, T *__restrict__ var) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (ih != 0) {
@@ -712,7 +717,7 @@ __global__ void k_bc_gen(Geo g, int type, int ih, int jh, int kh, int plane, dou
     k = plane; i = (int)(t % g.ngi); j = (int)(t / g.ngi);
   }
   const int ng = g.ng, nv = g.nvt;
-  auto U = [&](int a, int b, int c, int v) -> double & { return var[g.id(a, b, c) + g.N * v]; };
+  auto U = [&](int a, int b, int c, int v) -> T & { return var[g.id(a, b, c) + g.N * v]; };
   if (type == FST_BC_OUTFLOW) {
     for (int v = 0; v < nv; ++v)
       for (int n = 0; n < ng; ++n)
@@ -721,7 +726,8 @@ __global__ void k_bc_gen(Geo g, int type, int ih, int jh, int kh, int plane, dou
     for (int v = 0; v < nv; ++v) {
       double reflect = (v == 0 || v == 1 || v == 2) ? -1.0 : 1.0;
       for (int n = 0; n < ng; ++n)
-        U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) = reflect * U(i - ih * n, j - jh * n, k - kh * n, v);
+        U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) =
+            (T)(reflect * (double)U(i - ih * n, j - jh * n, k - kh * n, v));
     }
   } else {  // reflective, and the reflection pass of hydrostatic
     for (int v = 0; v < nv; ++v) {
@@ -730,7 +736,8 @@ __global__ void k_bc_gen(Geo g, int type, int ih, int jh, int kh, int plane, dou
       if (jh != 0 && v == 1) reflect = -1.0;
       if (kh != 0 && v == 2) reflect = -1.0;
       for (int n = 0; n < ng; ++n)
-        U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) = reflect * U(i - ih * n, j - jh * n, k - kh * n, v);
+        U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) =
+            (T)(reflect * (double)U(i - ih * n, j - jh * n, k - kh * n, v));
     }
     if (type == FST_BC_HYDROSTATIC) {
       const int edx = g.ndim, rdx = g.ndim + 1;
@@ -739,15 +746,16 @@ __global__ void k_bc_gen(Geo g, int type, int ih, int jh, int kh, int plane, dou
         int i1 = ig - ih, j1 = jg - jh, k1 = kg - kh;
         int i2 = ig - 2 * ih, j2 = jg - 2 * jh, k2 = kg - 2 * kh;
         double ke = 0, ke1 = 0, ke2 = 0;
+        auto D = [&](int a, int b, int c, int v) { return (double)U(a, b, c, v); };
         for (int d = 0; d < g.ndim; ++d) {
-          ke += 0.5 * (U(ig, jg, kg, d) * U(ig, jg, kg, d));
-          ke1 += 0.5 * (U(i1, j1, k1, d) * U(i1, j1, k1, d));
-          ke2 += 0.5 * (U(i2, j2, k2, d) * U(i2, j2, k2, d));
+          ke += 0.5 * (D(ig, jg, kg, d) * D(ig, jg, kg, d));
+          ke1 += 0.5 * (D(i1, j1, k1, d) * D(i1, j1, k1, d));
+          ke2 += 0.5 * (D(i2, j2, k2, d) * D(i2, j2, k2, d));
         }
-        ke = ke / U(ig, jg, kg, rdx);
-        ke1 = ke1 / U(i1, j1, k1, rdx);
-        ke2 = ke2 / U(i2, j2, k2, rdx);
-        U(ig, jg, kg, edx) = 2 * (U(i1, j1, k1, edx) - ke1) - (U(i2, j2, k2, edx) - ke2) + ke;
+        ke = ke / D(ig, jg, kg, rdx);
+        ke1 = ke1 / D(i1, j1, k1, rdx);
+        ke2 = ke2 / D(i2, j2, k2, rdx);
+        U(ig, jg, kg, edx) = (T)(2 * (D(i1, j1, k1, edx) - ke1) - (D(i2, j2, k2, edx) - ke2) + ke);
       }
     }
   }
@@ -1055,7 +1063,8 @@ void launch_metrics(const Geo &g, const Bufs &b, hipStream_t s) {
   }
 }
 
-void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream_t s) {
+template <typename T>
+void apply_bcs_t(const Geo &g, const fst_config &cf, T *var, hipStream_t s) {
   const int bs = 256;
   if (cf.xPer) {
     if (2 * g.ng <= 8) {
@@ -1091,6 +1100,9 @@ void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream
     k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcF, 0, 0, 1, ng + g.nck - 1, var);
   }
 }
+
+void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream_t s) { apply_bcs_t(g, cf, var, s); }
+void launch_apply_bcs(const Geo &g, const fst_config &cf, float *var, hipStream_t s) { apply_bcs_t(g, cf, var, s); }
 
 void launch_rk_stage(const Geo &g, int stage, double dt, double *var, double *tmp, const double *dvar, hipStream_t s) {
   long long n = g.N * g.nvt;

@@ -187,9 +187,10 @@ int fst_last_step_ms(fst_ctx *ctx, double *ms);
 int fst_profile_enable(fst_ctx *ctx, int enable);
 int fst_profile_read(fst_ctx *ctx, int64_t *launches, double *total_ms, char *name, size_t name_len);
 /* Per-phase sums (ms) of the units read by the last fst_profile_read.  Fused
- * stages have 4 phases: ghost fill + ghost primitives, learned C fields (0 when
- * not used), advection kernel, viscous/C-equation + RK kernel; the reference
- * sequence has 1 (the flux kernel).  Writes min(n, *nphase) values. */
+ * stages have 5 phases: ghost fill + ghost primitives, learned C fields (0 when
+ * not used), advection kernel, viscous + flow-variable stage-update kernel,
+ * C-equation kernel (0 in the learned form); the reference sequence has 1 (the
+ * flux kernel).  Writes min(n, *nphase) values. */
 int fst_profile_phases(fst_ctx *ctx, double *ms, int n, int *nphase);
 
 /* Device-to-device copy of `bytes` bytes with a plain vectorized HIP kernel,
