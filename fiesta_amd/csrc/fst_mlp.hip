@@ -85,7 +85,6 @@ G3 make_g3(const Geo &g) {
 
 inline unsigned nblk(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 // 3D launch grid of the per-cell passes: block (64, 4) over (i, j), one z-slice per k
-inline dim3 grid3(const Geo &g) { return dim3((g.ngi + 63) / 64, (g.ngj + 3) / 4, g.ngk); }
 
 // ---------------------------------------------------------------- features
 
@@ -242,39 +241,57 @@ __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst 
   }
 }
 
-// Active interior cells: max of M over the 3x3x3 neighbourhood above tau, or
-// (stride > 0) every stride-th interior cell.  Compacted into `list` with one
-// atomic per wave; chat (when given) cleared on every cell.
-// Launch: block (64, 4), grid (ceil(ngi/64), ceil(ngj/4), ngk).
+constexpr int ACT_KCH = 16;  // planes per block of the activity pass
+
+// Active cells: interior cells whose 3x3x3 neighbourhood has M > tau (every
+// stride-th interior cell as well when stride > 0), appended to `list` with
+// one atomic per wave; C_hat is cleared on every cell.  Each thread marches its
+// (i, j) column through ACT_KCH planes, keeping the 3x3 in-plane maxima of
+// planes k-1, k, k+1 in registers (9 loads per cell instead of 27).
 __global__ void __launch_bounds__(256) k_active3d(const G3 g, const float *__restrict__ M, float tau,
                                                   unsigned stride, unsigned *__restrict__ list,
                                                   unsigned *__restrict__ cnt, float *__restrict__ chat) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  const long long t = (long long)i + g.s1 * j + g.s2 * k;
-  bool act = false;
-  if (i < g.ngi && j < g.ngj) {
-    if (chat) chat[t] = 0.f;
-    const int ng = g.ng;
-    if (i >= ng && j >= ng && k >= ng && i < ng + g.nci && j < ng + g.ncj && k < ng + g.nck) {
-      float m = 0.f;
-      for (int dk = -1; dk <= 1; ++dk)
-        for (int dj = -1; dj <= 1; ++dj)
-          for (int di = -1; di <= 1; ++di) m = fmaxf(m, M[t + di + dj * g.s1 + dk * g.s2]);
-      act = m > tau;
-      if (stride > 0) {
-        const long long lin = (long long)(i - ng) + (long long)g.nci * ((j - ng) + (long long)g.ncj * (k - ng));
-        act = act || (lin % stride) == 0;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const int k0 = blockIdx.z * ACT_KCH, k1 = min(k0 + ACT_KCH, g.ngk);
+  const int ng = g.ng;
+  const bool col = i < g.ngi && j < g.ngj;
+  const bool colin = col && i >= ng && j >= ng && i < ng + g.nci && j < ng + g.ncj;
+  const long long cij = (long long)i + g.s1 * j;
+  auto m9 = [&](int k) {  // max of M over the 3x3 cells around (i, j) in plane k
+    float m = 0.f;
+    if (colin && k >= 0 && k < g.ngk) {
+      const long long t = cij + g.s2 * k;
+      for (int dj = -1; dj <= 1; ++dj)
+        for (int di = -1; di <= 1; ++di) m = fmaxf(m, M[t + di + dj * g.s1]);
+    }
+    return m;
+  };
+  float qm = m9(k0 - 1), qc = m9(k0);
+  const int lane = threadIdx.x;
+  for (int k = k0; k < k1; ++k) {
+    const float qp = m9(k + 1);
+    const long long t = cij + g.s2 * k;
+    bool act = false;
+    if (col) {
+      if (chat) chat[t] = 0.f;
+      if (colin && k >= ng && k < ng + g.nck) {
+        act = fmaxf(qm, fmaxf(qc, qp)) > tau;
+        if (stride > 0) {
+          const long long lin = (long long)(i - ng) + (long long)g.nci * ((j - ng) + (long long)g.ncj * (k - ng));
+          act = act || (lin % stride) == 0;
+        }
       }
     }
-  }
-  const unsigned long long mask = __ballot(act);
-  const int lane = threadIdx.x;
-  if (mask) {
-    const int first = __ffsll((long long)mask) - 1;
-    unsigned base = 0;
-    if (lane == first) base = atomicAdd(cnt, (unsigned)__popcll(mask));
-    base = __shfl(base, first, 64);
-    if (act) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (unsigned)t;
+    const unsigned long long mask = __ballot(act);
+    if (mask) {
+      const int first = __ffsll((long long)mask) - 1;
+      unsigned base = 0;
+      if (lane == first) base = atomicAdd(cnt, (unsigned)__popcll(mask));
+      base = __shfl(base, first, 64);
+      if (act) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (unsigned)t;
+    }
+    qm = qc;
+    qc = qp;
   }
 }
 
@@ -683,7 +700,8 @@ int learned_fields(const Geo &g, const fst_config &cf, const Phys &ph, const dou
   int rc = features(g, cf, ph, P, pf32, L, s);
   if (rc) return rc;
   const G3 q = make_g3(g);
-  k_active3d<<<grid3(g), dim3(64, 4), 0, s>>>(q, L.M, L.tau, 0u, L.list, L.cnt, L.chat);
+  const dim3 ag((g.ngi + 63) / 64, (g.ngj + 3) / 4, (g.ngk + ACT_KCH - 1) / ACT_KCH);
+  k_active3d<<<ag, dim3(64, 4), 0, s>>>(q, L.M, L.tau, 0u, L.list, L.cnt, L.chat);
   const double hmin = std::min(cf.dx, std::min(cf.dy, cf.dz));
   k_mlp_infer<<<1024, 256, 0, s>>>(q, L.F, L.list, L.cnt, L.Wb, L.W, L.G1, P, pf32 ? 1 : 0, hmin, L.tol, L.chat,
                                     L.sc, L.cnt + 1);
@@ -844,7 +862,8 @@ int trainer_harvest(Trainer &t, const Geo &g, const fst_config &cf, const Phys &
   int rc = features(g, cf, ph, P, false, t.feat, s);  // plain FP64 contexts: FP64 primitive planes
   if (rc) return rc;
   const G3 q = make_g3(g);
-  k_active3d<<<grid3(g), dim3(64, 4), 0, s>>>(q, t.feat.M, t.feat.tau, stride, t.feat.list, t.feat.cnt, nullptr);
+  const dim3 ag((g.ngi + 63) / 64, (g.ngj + 3) / 4, (g.ngk + ACT_KCH - 1) / ACT_KCH);
+  k_active3d<<<ag, dim3(64, 4), 0, s>>>(q, t.feat.M, t.feat.tau, stride, t.feat.list, t.feat.cnt, nullptr);
   unsigned n = 0;
   if (hipMemcpyAsync(&n, t.feat.cnt, sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
