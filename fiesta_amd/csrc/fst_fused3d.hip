@@ -539,22 +539,27 @@ __device__ __forceinline__ void plane_store(double *qs, const double *pre) {
 }
 
 // Advective fluxes on the face between the cell at plane row r, column q and
-// its +e_D neighbour (D = 0: x, 1: y), stencil read from the LDS plane.
+// its +e_D neighbour (D = 0: x, 1: y), stencil read from the LDS plane.  The
+// upwind stencil is chosen by address (a0 + t dt, t = 0..4, one choice for all
+// variables): the same values as flux_from_stencil's selection, so the same
+// bits.
 template <int D, int NS, bool F32>
 __device__ __forceinline__ void face_adv_lds(const FC &k, const double *qs, int r, int q, double *out) {
   using PF = PlaneF<NS>;
   constexpr int NV = PF::NV;
-  constexpr int sr = D == 1 ? QX : 0, sq = D == 0 ? 1 : 0;  // offset of +e_D in the plane
+  constexpr int st = D == 1 ? QX : 1;  // offset of +e_D in the plane
   const int base = r * QX + q;
   constexpr int fv = D == 0 ? PF::FU : PF::FV;
-  const double vd[4] = {qs[fv * QPL + base - (sr + sq)], qs[fv * QPL + base], qs[fv * QPL + base + (sr + sq)],
-                        qs[fv * QPL + base + 2 * (sr + sq)]};
-  double qq[6][NV];
+  const double *vp = qs + fv * QPL + base;
+  const double ud = cdiv(-vp[2 * st] + 7.0f * vp[st] + 7.0f * vp[0] - vp[-st], 12.0, k.r12);  // src/flux.hpp:278
+  const bool neg = ud < 0.0;
+  const int a0 = neg ? base + 3 * st : base - 2 * st;
+  const int dt = neg ? -st : st;
 #pragma unroll
-  for (int t = 0; t < 6; ++t)
-#pragma unroll
-    for (int v = 0; v < NV; ++v) qq[t][v] = qs[v * QPL + base + (t - 2) * (sr + sq)];
-  flux_from_stencil<NS, F32>(k, D, vd, qq, out);
+  for (int v = 0; v < NV; ++v) {
+    const double *qv = qs + v * QPL + a0;
+    out[v] = cdiv(ud * weno5_f32c(qv[0], qv[dt], qv[2 * dt], qv[3 * dt], qv[4 * dt]), k.d[D], k.rd[D]);
+  }
 }
 
 template <int NS, bool F32>
@@ -702,24 +707,51 @@ __device__ __forceinline__ void plane_fetch_s(const Grid32 &g, const double *__r
   }
 }
 
-// Fluxes of the nv flow variables on one face from its float stencil
-// (q[t][v] at cells c-2e .. c+3e, vd = vel_D at c-e .. c+2e), divided by d.
+// WENO5 of the binary32 advection kernel on deviations from the upwind-centre
+// value f3 (so f3 = 0 drops out of every smoothness indicator and candidate):
+// the stencils, weights and indicators of calculateFluxesG
+// (src/flux.hpp:240-264), with the linear weights (0.1, 0.6, 0.3) folded into
+// the candidate polynomials and hardware reciprocals.
+__device__ __forceinline__ float weno5_dev(float f1, float f2, float f4, float f5) {
+#pragma clang fp contract(fast)
+  const float eps = 1.0e-6f;
+  float a1 = f1 - 2.0f * f2, a2 = f1 - 4.0f * f2;
+  const float s1 = eps + ((13.0f / 12.0f) * a1 * a1 + 0.25f * a2 * a2);
+  a1 = f2 + f4;
+  a2 = f2 - f4;
+  const float s2 = eps + ((13.0f / 12.0f) * a1 * a1 + 0.25f * a2 * a2);
+  a1 = f5 - 2.0f * f4;
+  a2 = f5 - 4.0f * f4;
+  const float s3 = eps + ((13.0f / 12.0f) * a1 * a1 + 0.25f * a2 * a2);
+  const float r1 = __builtin_amdgcn_rcpf(s1 * s1), r2 = __builtin_amdgcn_rcpf(s2 * s2),
+              r3 = __builtin_amdgcn_rcpf(s3 * s3);
+  const float p1 = (0.1f / 3.0f) * f1 - (0.7f / 6.0f) * f2;  // 0.1 (f1/3 - 7 f2/6 + 11 f3/6)
+  const float p2 = 0.2f * f4 - 0.1f * f2;                    // 0.6 (-f2/6 + 5 f3/6 + f4/3)
+  const float p3 = 0.25f * f4 - 0.05f * f5;                  // 0.3 (f3/3 + 5 f4/6 - f5/6)
+  return (r1 * p1 + r2 * p2 + r3 * p3) * __builtin_amdgcn_rcpf(0.1f * r1 + 0.6f * r2 + 0.3f * r3);
+}
+
+// Fluxes of the nv flow variables on one face from its float stencil in
+// registers (q[t][v] at cells c-2e .. c+3e, vd = vel_D at c-e .. c+2e),
+// divided by d.
 template <int NS>
 __device__ __forceinline__ void flux_s(float rd, const float *vd, const float (*q)[4 + NS], float *out) {
   const float ud = (-vd[3] + 7.0f * vd[2] + 7.0f * vd[1] - vd[0]) * (1.0f / 12.0f);  // src/flux.hpp:278
   const bool neg = ud < 0.f;
+  const float udr = ud * rd;
 #pragma unroll
   for (int v = 0; v < 4 + NS; ++v) {
-    const float c0 = q[2][v];
+    const float c0 = neg ? q[3][v] : q[2][v];
     const float f1 = (neg ? q[5][v] : q[0][v]) - c0;
     const float f2 = (neg ? q[4][v] : q[1][v]) - c0;
-    const float f3 = (neg ? q[3][v] : q[2][v]) - c0;
     const float f4 = (neg ? q[2][v] : q[3][v]) - c0;
     const float f5 = (neg ? q[1][v] : q[4][v]) - c0;
-    out[v] = ud * (c0 + weno5_fast(f1, f2, f3, f4, f5)) * rd;
+    out[v] = udr * (c0 + weno5_dev(f1, f2, f4, f5));
   }
 }
 
+// The same on a face whose stencil lies in the LDS plane: the upwind stencil
+// is chosen by address (a0 + t dt, t = 0..4), one choice for all variables.
 template <int NS, int D>
 __device__ __forceinline__ void face_s(const float *qs, float rd, int r, int q, float *out) {
   using PS = PlaneS<NS>;
@@ -727,14 +759,19 @@ __device__ __forceinline__ void face_s(const float *qs, float rd, int r, int q, 
   constexpr int st = D == 1 ? QX : 1;
   constexpr int fv = D == 0 ? PS::FU : PS::FV;
   const int base = r * QX + q;
-  const float vd[4] = {qs[fv * QPLS + base - st], qs[fv * QPLS + base], qs[fv * QPLS + base + st],
-                       qs[fv * QPLS + base + 2 * st]};
-  float qq[6][NV];
+  const float *vp = qs + fv * QPLS + base;
+  const float ud = (-vp[2 * st] + 7.0f * vp[st] + 7.0f * vp[0] - vp[-st]) * (1.0f / 12.0f);  // src/flux.hpp:278
+  const bool neg = ud < 0.f;
+  const float udr = ud * rd;
+  const int a0 = neg ? base + 3 * st : base - 2 * st;
+  const int dt = neg ? -st : st;
 #pragma unroll
-  for (int tt = 0; tt < 6; ++tt)
-#pragma unroll
-    for (int v = 0; v < NV; ++v) qq[tt][v] = qs[v * QPLS + base + (tt - 2) * st];
-  flux_s<NS>(rd, vd, qq, out);
+  for (int v = 0; v < NV; ++v) {
+    const float *qv = qs + v * QPLS + a0;
+    const float c0 = qv[2 * dt];
+    const float f1 = qv[0] - c0, f2 = qv[dt] - c0, f4 = qv[3 * dt] - c0, f5 = qv[4 * dt] - c0;
+    out[v] = udr * (c0 + weno5_dev(f1, f2, f4, f5));
+  }
 }
 
 // FI: U is the float stage state of the learned FP32 form.
