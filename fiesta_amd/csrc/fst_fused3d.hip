@@ -1270,30 +1270,47 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     }
     at.sl[0] = (kk - 1) & 3; at.sl[1] = kk & 3; at.sl[2] = (kk + 1) & 3;
     if (faces) {
-      if (t.cellok) {
+      if (F32 && t.cellok) {
+        // FP32 forms (registers to spare): one straight-line region for the
+        // three faces of a tile cell, so that the loads of the later faces
+        // are issued ahead of the earlier faces' arithmetic
         R X[NFV];
         at.r = w + 1; at.q = l + 1; at.c = c;
         face_vc_ring<0, NS, F32, LC>(k, at, alpha, visc, ceq, X);
 #pragma unroll
         for (int q = 0; q < NFV; ++q) xsh[w][q][l + 1] = X[q];
-      }
-      if (t.xbok) {  // x-face left of the tile, row l
-        R X[NFV];
-        at.r = l + 1; at.q = 0; at.c = cxb;
-        face_vc_ring<0, NS, F32, LC>(k, at, alpha, visc, ceq, X);
-#pragma unroll
-        for (int q = 0; q < NFV; ++q) xsh[l][q][0] = X[q];
-      }
-      if (t.yfok) {  // y-face below row w (between rows w-1 and w)
-        R Y[NFV];
-        at.r = w; at.q = l + 1; at.c = c - g.s1;
+        face_vc_ring<2, NS, F32, LC>(k, at, alpha, visc, ceq, Z);
+        R Y[NFV];  // y-face below row w (yfok holds for a tile cell)
+        at.r = w; at.c = c - g.s1;
         face_vc_ring<1, NS, F32, LC>(k, at, alpha, visc, ceq, Y);
 #pragma unroll
         for (int q = 0; q < NFV; ++q) ysh[w][q][l] = Y[q];
-      }
-      if (t.cellok) {
-        at.r = w + 1; at.q = l + 1; at.c = c;
-        face_vc_ring<2, NS, F32, LC>(k, at, alpha, visc, ceq, Z);
+      } else {
+        if (!F32 && t.cellok) {
+          R X[NFV];
+          at.r = w + 1; at.q = l + 1; at.c = c;
+          face_vc_ring<0, NS, F32, LC>(k, at, alpha, visc, ceq, X);
+#pragma unroll
+          for (int q = 0; q < NFV; ++q) xsh[w][q][l + 1] = X[q];
+        }
+        if (t.xbok) {  // x-face left of the tile, row l
+          R X[NFV];
+          at.r = l + 1; at.q = 0; at.c = cxb;
+          face_vc_ring<0, NS, F32, LC>(k, at, alpha, visc, ceq, X);
+#pragma unroll
+          for (int q = 0; q < NFV; ++q) xsh[l][q][0] = X[q];
+        }
+        if (t.yfok) {  // y-face below row w (between rows w-1 and w)
+          R Y[NFV];
+          at.r = w; at.q = l + 1; at.c = c - g.s1;
+          face_vc_ring<1, NS, F32, LC>(k, at, alpha, visc, ceq, Y);
+#pragma unroll
+          for (int q = 0; q < NFV; ++q) ysh[w][q][l] = Y[q];
+        }
+        if (!F32 && t.cellok) {
+          at.r = w + 1; at.q = l + 1; at.c = c;
+          face_vc_ring<2, NS, F32, LC>(k, at, alpha, visc, ceq, Z);
+        }
       }
     }
     if (fetch) ring_store<NS, F32, LC>(ring, (kk + 2) & 3, pre);
