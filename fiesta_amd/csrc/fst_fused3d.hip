@@ -598,11 +598,20 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     __syncthreads();
     if (PREF && kk + 1 < t.ke) plane_fetch<NS, F32>(g, U, P, t.i0, t.j0, kk + 1, pre);
     const int rc = w + 3, qc = l + 3;  // plane coordinates of this lane's cell
-    if (t.cellok) {
+    if (t.cellok) {  // x-face right of the cell and z-face above it, one region
       double X[NV];
       face_adv_lds<0, NS, F32>(k, qs, rc, qc, X);
 #pragma unroll
       for (int q = 0; q < NV; ++q) xsh[w][q][l + 1] = X[q];
+#pragma unroll
+      for (int s = 0; s < 5; ++s)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qz[s][v] = qz[s + 1][v];
+      load_q<NS, F32>(g, U, P, c + 3 * g.s2, qz[5]);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) vz[s] = vz[s + 1];
+      vz[3] = pget<F32>(P, N, 4, c + 2 * g.s2);
+      flux_from_stencil<NS, F32>(k, 2, vz, qz, Z);
     }
     if (t.xbok) {  // x-face left of the tile, row l: cell (i0-1, j0+l)
       double X[NV];
@@ -615,17 +624,6 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       face_adv_lds<1, NS, F32>(k, qs, w + 2, qc, Y);
 #pragma unroll
       for (int q = 0; q < NV; ++q) ysh[w][q][l] = Y[q];
-    }
-    if (t.cellok) {
-#pragma unroll
-      for (int s = 0; s < 5; ++s)
-#pragma unroll
-        for (int v = 0; v < NV; ++v) qz[s][v] = qz[s + 1][v];
-      load_q<NS, F32>(g, U, P, c + 3 * g.s2, qz[5]);
-#pragma unroll
-      for (int s = 0; s < 3; ++s) vz[s] = vz[s + 1];
-      vz[3] = pget<F32>(P, N, 4, c + 2 * g.s2);
-      flux_from_stencil<NS, F32>(k, 2, vz, qz, Z);
     }
     __syncthreads();
     if (t.cellok) {
@@ -823,25 +821,11 @@ __global__ void __launch_bounds__(NTS, 1)
     __syncthreads();
     if (kk + 1 < t.ke) plane_fetch_s<NS, FI>(g, U, P, t.i0, t.j0, kk + 1, pre, prep);
     const int rc = w + 3, qc = l + 3;  // plane coordinates of this lane's cell
-    if (t.cellok) {
+    if (t.cellok) {  // the three faces of a tile cell in one region
       float X[NV];
       face_s<NS, 0>(qs, rdx, rc, qc, X);
 #pragma unroll
       for (int q = 0; q < NV; ++q) xsh[w][q][l + 1] = X[q];
-    }
-    if (t.xbok) {  // x-face left of the tile, row l: cell (i0-1, j0+l)
-      float X[NV];
-      face_s<NS, 0>(qs, rdx, l + 3, 2, X);
-#pragma unroll
-      for (int q = 0; q < NV; ++q) xsh[l][q][0] = X[q];
-    }
-    if (t.yfok) {  // y-face below row w: cell (i0+l, j0+w-1)
-      float Y[NV];
-      face_s<NS, 1>(qs, rdy, w + 2, qc, Y);
-#pragma unroll
-      for (int q = 0; q < NV; ++q) ysh[w][q][l] = Y[q];
-    }
-    if (t.cellok) {
 #pragma unroll
       for (int s = 0; s < 5; ++s)
 #pragma unroll
@@ -851,6 +835,23 @@ __global__ void __launch_bounds__(NTS, 1)
       for (int s = 0; s < 3; ++s) vz[s] = vz[s + 1];
       vz[3] = (float)pget<true>(P, N, 4, c + 2 * g.s2);
       flux_s<NS>(rdz, vz, qz, Z);
+      float Y[NV];  // y-face below row w (yfok holds for a tile cell)
+      face_s<NS, 1>(qs, rdy, w + 2, qc, Y);
+#pragma unroll
+      for (int q = 0; q < NV; ++q) ysh[w][q][l] = Y[q];
+    } else {
+      if (t.xbok) {  // x-face left of the tile, row l: cell (i0-1, j0+l)
+        float X[NV];
+        face_s<NS, 0>(qs, rdx, l + 3, 2, X);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) xsh[l][q][0] = X[q];
+      }
+      if (t.yfok) {  // y-face below row w: cell (i0+l, j0+w-1)
+        float Y[NV];
+        face_s<NS, 1>(qs, rdy, w + 2, qc, Y);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) ysh[w][q][l] = Y[q];
+      }
     }
     __syncthreads();
     if (t.cellok) {
