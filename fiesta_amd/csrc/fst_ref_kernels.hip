@@ -667,18 +667,21 @@ __global__ void k_bc_periodic(Geo g, int dir, T *__restrict__ var) {
   t /= na;
   int b = (int)(t % nb);
   int v = (int)(t / nb);
-  T *u = var + g.N * v;
+  // sources (interior cells) and targets (ghosts) never overlap: a periodic
+  // direction has at least ng interior cells
+  const T *__restrict__ src = var + g.N * v;
+  T *__restrict__ dst = var + g.N * v;
   const int ng = g.ng;
   for (int gg = 0; gg < ng; ++gg) {
     if (dir == 0) {
-      u[g.id(gg, a, b)] = u[g.id(g.nci + gg, a, b)];
-      u[g.id(g.nci + ng + gg, a, b)] = u[g.id(ng + gg, a, b)];
+      dst[g.id(gg, a, b)] = src[g.id(g.nci + gg, a, b)];
+      dst[g.id(g.nci + ng + gg, a, b)] = src[g.id(ng + gg, a, b)];
     } else if (dir == 1) {
-      u[g.id(a, gg, b)] = u[g.id(a, g.ncj + gg, b)];
-      u[g.id(a, g.ncj + ng + gg, b)] = u[g.id(a, ng + gg, b)];
+      dst[g.id(a, gg, b)] = src[g.id(a, g.ncj + gg, b)];
+      dst[g.id(a, g.ncj + ng + gg, b)] = src[g.id(a, ng + gg, b)];
     } else {
-      u[g.id(a, b, gg)] = u[g.id(a, b, g.nck + gg)];
-      u[g.id(a, b, g.nck + ng + gg)] = u[g.id(a, b, ng + gg)];
+      dst[g.id(a, b, gg)] = src[g.id(a, b, g.nck + gg)];
+      dst[g.id(a, b, g.nck + ng + gg)] = src[g.id(a, b, ng + gg)];
     }
   }
 }
@@ -701,63 +704,70 @@ __global__ void k_bc_xper(Geo g, T *__restrict__ var) {
   else u[g.nci + r] = u[r];  // i = nci + ng + (r - ng) from i = ng + (r - ng)
 }
 
-// bc_gen, src/bc.cpp:33-131, on one boundary plane (i0 / j0 / k0 fixed)
+// bc_gen, src/bc.cpp:33-131, on one boundary plane (i0 / j0 / k0 fixed).
+// Threads (column, v): blockDim.x columns of the plane times blockDim.y = nvt
+// variables.  Each thread mirrors its variable's ng cells (sources are the
+// interior cells plane - n, targets the ghosts plane + n + 1: they never
+// overlap, so the loads go out before the stores).  For hydrostatic, one
+// thread per column then redoes the energy ghosts in the reference's order
+// after a block barrier; the reflected energy it replaces is never read, so
+// that thread's columns skip it.  Same values and operations as the
+// reference's per-column loops.
 template <typename T>
 __global__ void k_bc_gen(Geo g, int type, int ih, int jh, int kh, int plane, T *__restrict__ var) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  int i, j, k;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = threadIdx.y;
+  int i = 0, j = 0, k = 0;
+  bool ok;
   if (ih != 0) {
-    if (t >= (long long)g.ngj * g.ngk) return;
+    ok = t < (long long)g.ngj * g.ngk;
     i = plane; j = (int)(t % g.ngj); k = (int)(t / g.ngj);
   } else if (jh != 0) {
-    if (t >= (long long)g.ngi * g.ngk) return;
+    ok = t < (long long)g.ngi * g.ngk;
     j = plane; i = (int)(t % g.ngi); k = (int)(t / g.ngi);
   } else {
-    if (t >= (long long)g.ngi * g.ngj) return;
+    ok = t < (long long)g.ngi * g.ngj;
     k = plane; i = (int)(t % g.ngi); j = (int)(t / g.ngi);
   }
-  const int ng = g.ng, nv = g.nvt;
-  auto U = [&](int a, int b, int c, int v) -> T & { return var[g.id(a, b, c) + g.N * v]; };
-  if (type == FST_BC_OUTFLOW) {
-    for (int v = 0; v < nv; ++v)
-      for (int n = 0; n < ng; ++n)
-        U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) = U(i - ih * n, j - jh * n, k - kh * n, v);
-  } else if (type == FST_BC_NOSLIP) {
-    for (int v = 0; v < nv; ++v) {
-      double reflect = (v == 0 || v == 1 || v == 2) ? -1.0 : 1.0;
-      for (int n = 0; n < ng; ++n)
-        U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) =
-            (T)(reflect * (double)U(i - ih * n, j - jh * n, k - kh * n, v));
-    }
-  } else {  // reflective, and the reflection pass of hydrostatic
-    for (int v = 0; v < nv; ++v) {
+  const int ng = g.ng;
+  const bool hydro = type == FST_BC_HYDROSTATIC;
+  const int edx = g.ndim, rdx = g.ndim + 1;
+  if (ok && !(hydro && v == edx)) {
+    const T *__restrict__ src = var + g.N * v;
+    T *__restrict__ dst = var + g.N * v;
+    const long long c0 = g.id(i, j, k);
+    const long long st = (long long)ih + (long long)jh * g.ngi + (long long)kh * g.ngi * g.ngj;  // one cell outward
+    if (type == FST_BC_OUTFLOW) {
+      for (int n = 0; n < ng; ++n) dst[c0 + st * (n + 1)] = src[c0 - st * n];
+    } else {
       double reflect = 1.0;
-      if (ih != 0 && v == 0) reflect = -1.0;
-      if (jh != 0 && v == 1) reflect = -1.0;
-      if (kh != 0 && v == 2) reflect = -1.0;
-      for (int n = 0; n < ng; ++n)
-        U(i + ih * (n + 1), j + jh * (n + 1), k + kh * (n + 1), v) =
-            (T)(reflect * (double)U(i - ih * n, j - jh * n, k - kh * n, v));
-    }
-    if (type == FST_BC_HYDROSTATIC) {
-      const int edx = g.ndim, rdx = g.ndim + 1;
-      for (int n = 0; n < ng; ++n) {
-        int ig = i + ih * (n + 1), jg = j + jh * (n + 1), kg = k + kh * (n + 1);
-        int i1 = ig - ih, j1 = jg - jh, k1 = kg - kh;
-        int i2 = ig - 2 * ih, j2 = jg - 2 * jh, k2 = kg - 2 * kh;
-        double ke = 0, ke1 = 0, ke2 = 0;
-        auto D = [&](int a, int b, int c, int v) { return (double)U(a, b, c, v); };
-        for (int d = 0; d < g.ndim; ++d) {
-          ke += 0.5 * (D(ig, jg, kg, d) * D(ig, jg, kg, d));
-          ke1 += 0.5 * (D(i1, j1, k1, d) * D(i1, j1, k1, d));
-          ke2 += 0.5 * (D(i2, j2, k2, d) * D(i2, j2, k2, d));
-        }
-        ke = ke / D(ig, jg, kg, rdx);
-        ke1 = ke1 / D(i1, j1, k1, rdx);
-        ke2 = ke2 / D(i2, j2, k2, rdx);
-        U(ig, jg, kg, edx) = (T)(2 * (D(i1, j1, k1, edx) - ke1) - (D(i2, j2, k2, edx) - ke2) + ke);
+      if (type == FST_BC_NOSLIP) {
+        if (v == 0 || v == 1 || v == 2) reflect = -1.0;
+      } else {  // reflective, and the reflection pass of hydrostatic
+        if ((ih != 0 && v == 0) || (jh != 0 && v == 1) || (kh != 0 && v == 2)) reflect = -1.0;
       }
+      for (int n = 0; n < ng; ++n) dst[c0 + st * (n + 1)] = (T)(reflect * (double)src[c0 - st * n]);
     }
+  }
+  if (!hydro) return;
+  __syncthreads();
+  if (!ok || v != 0) return;
+  auto U = [&](int a, int b, int c, int w) -> T & { return var[g.id(a, b, c) + g.N * w]; };
+  auto D = [&](int a, int b, int c, int w) { return (double)U(a, b, c, w); };
+  for (int n = 0; n < ng; ++n) {
+    int ig = i + ih * (n + 1), jg = j + jh * (n + 1), kg = k + kh * (n + 1);
+    int i1 = ig - ih, j1 = jg - jh, k1 = kg - kh;
+    int i2 = ig - 2 * ih, j2 = jg - 2 * jh, k2 = kg - 2 * kh;
+    double ke = 0, ke1 = 0, ke2 = 0;
+    for (int d = 0; d < g.ndim; ++d) {
+      ke += 0.5 * (D(ig, jg, kg, d) * D(ig, jg, kg, d));
+      ke1 += 0.5 * (D(i1, j1, k1, d) * D(i1, j1, k1, d));
+      ke2 += 0.5 * (D(i2, j2, k2, d) * D(i2, j2, k2, d));
+    }
+    ke = ke / D(ig, jg, kg, rdx);
+    ke1 = ke1 / D(i1, j1, k1, rdx);
+    ke2 = ke2 / D(i2, j2, k2, rdx);
+    U(ig, jg, kg, edx) = (T)(2 * (D(i1, j1, k1, edx) - ke1) - (D(i2, j2, k2, edx) - ke2) + ke);
   }
 }
 
@@ -1084,20 +1094,25 @@ void apply_bcs_t(const Geo &g, const fst_config &cf, T *var, hipStream_t s) {
     k_bc_periodic<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, 2, var);
   }
   const int ng = g.ng;
+  // k_bc_gen: 32 columns x nvt variables per block (nvt <= 17)
+  const dim3 gb(32, g.nvt);
+  auto gen = [&](long long ncol, int type, int ih, int jh, int kh, int plane) {
+    k_bc_gen<<<(unsigned)((ncol + 31) / 32), gb, 0, s>>>(g, type, ih, jh, kh, plane, var);
+  };
   if (!cf.xPer) {
-    long long n = (long long)g.ngj * g.ngk;
-    k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcL, -1, 0, 0, ng, var);
-    k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcR, 1, 0, 0, ng + g.nci - 1, var);
+    const long long n = (long long)g.ngj * g.ngk;
+    gen(n, cf.bcL, -1, 0, 0, ng);
+    gen(n, cf.bcR, 1, 0, 0, ng + g.nci - 1);
   }
   if (!cf.yPer) {
-    long long n = (long long)g.ngi * g.ngk;
-    k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcB, 0, -1, 0, ng, var);
-    k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcT, 0, 1, 0, ng + g.ncj - 1, var);
+    const long long n = (long long)g.ngi * g.ngk;
+    gen(n, cf.bcB, 0, -1, 0, ng);
+    gen(n, cf.bcT, 0, 1, 0, ng + g.ncj - 1);
   }
   if (g.ndim == 3 && !cf.zPer) {
-    long long n = (long long)g.ngi * g.ngj;
-    k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcH, 0, 0, -1, ng, var);
-    k_bc_gen<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(g, cf.bcF, 0, 0, 1, ng + g.nck - 1, var);
+    const long long n = (long long)g.ngi * g.ngj;
+    gen(n, cf.bcH, 0, 0, -1, ng);
+    gen(n, cf.bcF, 0, 0, 1, ng + g.nck - 1);
   }
 }
 
