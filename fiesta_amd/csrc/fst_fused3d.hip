@@ -1259,9 +1259,10 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     asm volatile("" : "+v"(c), "+v"(cxb));
     const bool fetch = kk + 2 <= t.ke;  // plane kk+2 is read by iteration kk+1
     if (fetch) ring_fetch<NS, F32, LC>(g, U, P, CH, DR, t.i0, t.j0, kk + 2, pre);
-    // learned FP32 form (registers to spare): the cell's RHS and stage base
-    // are loaded before the face phase so their latency overlaps it
-    constexpr bool EARLY = F32 && LC;
+    // FP32 forms (registers to spare): the cell's partial RHS and stage base
+    // are loaded before the face phase so that their latency overlaps it (in
+    // the FP64 form this measured slower)
+    constexpr bool EARLY = F32;
     double dvp[EARLY ? NV : 1], bsp[EARLY ? NV : 1];
     if (EARLY && t.cellok) {
 #pragma unroll
@@ -1753,7 +1754,7 @@ void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf,
                                                   cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
   if (prof) prof->mark(s);  // end of the viscous / stage-update kernel
   if (cf.ceq && !LC) {  // the C variables: sources, relaxation and stage update
-    const int kc = 16;
+    const int kc = 32;
     const int nbx = (g.nci + 63) / 64, nby = (g.ncj + CY - 1) / CY, nbz = (g.nck + kc - 1) / kc;
     k_ceq3d<F32><<<nbx * nby * nbz, CNT, 0, s>>>(q, k, U, Un, Uo, P, scin, scout, h, stage, kc, nbx, nby, nbz);
   }
