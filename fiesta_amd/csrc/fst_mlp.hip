@@ -362,6 +362,7 @@ struct alignas(16) InfLds {
   short W3[MLP_H * WHS];
   short act[4][16 * WHS];  // per wave: activations of 16 rows (bf16)
   float b1[MLP_H], b2[MLP_H], b3[MLP_H], w4[MLP_H];
+  long long off[MLP_IN_PAD];  // offset of input kk from its cell in F (channel plane + stencil)
   float b4;
 };
 static_assert((MLP_H * W1S * 2) % 16 == 0 && (MLP_H * WHS * 2) % 16 == 0 && (16 * WHS * 2) % 16 == 0,
@@ -416,10 +417,19 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__re
     L.w4[tid] = W[MLP_OFF_W4 + tid];
   }
   if (tid == 0) L.b4 = W[MLP_OFF_B4];
+  const long long N = g.N, s1 = g.s1, s2 = g.s2;
+  for (int kk = tid; kk < MLP_IN_PAD; kk += 256) {
+    long long o = 0;
+    if (kk < MLP_IN) {
+      const int ch = kk / MLP_STEN, r = kk % MLP_STEN;
+      const int dz = r / 25 - 2, dy = (r / 5) % 5 - 2, dx = r % 5 - 2;
+      o = N * ch + dx + dy * s1 + dz * s2;
+    }
+    L.off[kk] = o;
+  }
   __syncthreads();
   const unsigned n = cnt[0];
   const float gmax_all = (float)as_double(sc->maxC);
-  const long long N = g.N, s1 = g.s1, s2 = g.s2;
   short *act = L.act[w];
   float mloc = 0.f;
   unsigned nfb = 0;
@@ -431,23 +441,31 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__re
     f32x4 acc[4];
 #pragma unroll
     for (int nb = 0; nb < 4; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < MLP_IN_PAD / 16; ++kb) {
-      bf16x4 a;
+    // the lane's inputs are gathered in batches of KBB k-blocks (4 KBB loads
+    // in flight together), then fed to the MFMAs
+    constexpr int KBB = 12;
+    static_assert((MLP_IN_PAD / 16) % KBB == 0, "k-block batches");
+    for (int kb0 = 0; kb0 < MLP_IN_PAD / 16; kb0 += KBB) {
+      float x[KBB][4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int kk = kb * 16 + 4 * (l >> 4) + q;
-        float x = 0.f;
-        if (kk < MLP_IN) {
-          const int ch = kk / MLP_STEN, o = kk % MLP_STEN;
-          const int dz = o / 25 - 2, dy = (o / 5) % 5 - 2, dx = o % 5 - 2;
-          x = F[cell + N * ch + dx + dy * s1 + dz * s2];
+      for (int b = 0; b < KBB; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int kk = (kb0 + b) * 16 + 4 * (l >> 4) + q;
+          x[b][q] = kk < MLP_IN ? F[cell + L.off[kk]] : 0.f;
         }
-        a[q] = f2bf(x);
-      }
 #pragma unroll
-      for (int nb = 0; nb < 4; ++nb) {
-        const bf16x4 b = *reinterpret_cast<const bf16x4 *>(&L.W1[(nb * 16 + (l & 15)) * W1S + kb * 16 + 4 * (l >> 4)]);
-        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
+      for (int b = 0; b < KBB; ++b) {
+        const int kb = kb0 + b;
+        bf16x4 a;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = f2bf(x[b][q]);
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+          const bf16x4 bw =
+              *reinterpret_cast<const bf16x4 *>(&L.W1[(nb * 16 + (l & 15)) * W1S + kb * 16 + 4 * (l >> 4)]);
+          acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[nb], 0, 0, 0);
+        }
       }
     }
     store_act(acc, L.b1, act, l);
