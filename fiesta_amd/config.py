@@ -30,7 +30,8 @@ class FstConfig(C.Structure):
         ("bcH", C.c_int32), ("bcF", C.c_int32),
         ("dt", C.c_double),
         ("precision", C.c_int32), ("learned_c", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("rk_order", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -38,10 +39,11 @@ def make_config(ndim=3, ns=1, ng=3, nci=8, ncj=8, nck=8, dx=1.0, dy=1.0, dz=1.0,
                 gamma=(1.4,), M=(0.02896,), mu=(1.8e-5,), visc=False, ceq=False, buoyancy=False,
                 scheme="weno5", kap=1.23456, eps=1.0, alpha=1.0, g=9.81, rhoRef=0.0,
                 xper=False, yper=False, zper=False, bc=("outflow",) * 6, dt=1e-6,
-                precision=FST_FP64, learned_c=False, grid=0):
+                precision=FST_FP64, learned_c=False, grid=0, rk=2):
     """Build an FstConfig.  `bc` lists xmin, xmax, ymin, ymax, zmin, zmax.
     grid=1 selects the generalized-grid modules (gen2d/gen3d), whose cell
-    sizes are 1 (src/input.cpp:344-350)."""
+    sizes are 1 (src/input.cpp:344-350).  rk: 2 = the reference's RK2
+    midpoint, 3 = SSP-RK3 (an added option)."""
     c = FstConfig()
     c.ndim, c.grid, c.ns, c.ng = ndim, int(grid), ns, ng
     c.nci, c.ncj, c.nck = nci, ncj, (nck if ndim == 3 else 1)
@@ -63,6 +65,7 @@ def make_config(ndim=3, ns=1, ng=3, nci=8, ncj=8, nck=8, dx=1.0, dy=1.0, dz=1.0,
     c.dt = dt
     c.precision = precision
     c.learned_c = int(learned_c)
+    c.rk_order = int(rk)
     return c
 
 

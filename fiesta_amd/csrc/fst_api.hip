@@ -75,6 +75,10 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
     *why = "advection scheme: 3D supports weno5; 2D weno5, centered4, quick";
     return FST_ERR_ARG;
   }
+  if (c.rk_order != 0 && c.rk_order != 2 && c.rk_order != 3) {
+    *why = "rk_order: 2 (RK2 midpoint, the reference's rkAdvance; 0 means 2) or 3 (SSP-RK3)";
+    return FST_ERR_ARG;
+  }
   const int bcs[6] = {c.bcL, c.bcR, c.bcB, c.bcT, c.bcH, c.bcF};
   for (int q = 0; q < 6; ++q)
     if (bcs[q] < 0 || bcs[q] > 3) { *why = "unknown boundary type"; return FST_ERR_ARG; }
@@ -342,9 +346,9 @@ int fst_compute(fst_ctx *ctx) {
 }
 
 int fst_rk_stage(fst_ctx *ctx, int stage) {
-  if (!ctx || (stage != 1 && stage != 2)) return FST_ERR_ARG;
+  if (!ctx || stage < 1 || stage > rk_stages(ctx->cf)) return FST_ERR_ARG;
   ctx->fb.valid = false;
-  launch_rk_stage(ctx->g, stage, ctx->cf.dt, ctx->b.var, ctx->b.tmp1, ctx->b.dvar, ctx->stream);
+  launch_rk_stage(ctx->g, rk_stages(ctx->cf), stage, ctx->cf.dt, ctx->b.var, ctx->b.tmp1, ctx->b.dvar, ctx->stream);
   return check_launch(ctx, "fst_rk_stage");
 }
 
@@ -373,9 +377,10 @@ int fst_step(fst_ctx *ctx, int nsteps) {
     ctx->fb.valid = false;
     for (int t = 0; t < nsteps; ++t) {
       if (ctx->g.ndim == 2 && ctx->cf.grid == 0) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
-      for (int stage = 1; stage <= 2; ++stage) {
+      const int ns = rk_stages(ctx->cf);
+      for (int stage = 1; stage <= ns; ++stage) {
         compute_seq(ctx);
-        launch_rk_stage(ctx->g, stage, ctx->cf.dt, ctx->b.var, ctx->b.tmp1, ctx->b.dvar, ctx->stream);
+        launch_rk_stage(ctx->g, ns, stage, ctx->cf.dt, ctx->b.var, ctx->b.tmp1, ctx->b.dvar, ctx->stream);
       }
     }
   }

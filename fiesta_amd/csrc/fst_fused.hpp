@@ -8,18 +8,21 @@
 namespace fst {
 
 struct FusedBufs {
-  double *stage = nullptr;  // U* (stage-1 output), nvt planes
-  double *prim = nullptr;   // two sets of primitive planes rho, p, u, v, w, nrg, Y_s
-  Scalars *sc = nullptr;    // two sets of per-stage maxima
-  int kchunk = 0;           // k planes per block (0: default)
-  bool valid = false;       // prim[0] / sc[0] describe the current var (interior)
+  double *stage = nullptr;   // U* (stage-1 output), nvt planes
+  double *stage2 = nullptr;  // SSP-RK3: U2 (stage-2 output), nvt planes
+  double *prim = nullptr;    // two sets of primitive planes rho, p, u, v, w, nrg, Y_s
+  Scalars *sc = nullptr;     // two sets of per-stage maxima
+  int kchunk = 0;            // k planes per block (0: default)
+  bool valid = false;        // prim[cur] / sc[cur] describe the current var (interior)
+  int cur = 0;               // which set describes var (an SSP-RK3 step flips it)
 };
 
 struct LearnedBufs;
 
 bool fused_supported(const fst_config &cf);
-// nsteps RK2 steps of the fused path; lb carries the learned C-indicator
-// fields when cf.learned_c is set (else may be null).
+// nsteps time steps of the fused path (RK2 midpoint, or SSP-RK3 with
+// rk_order = 3); lb carries the learned C-indicator fields when cf.learned_c
+// is set (else may be null).
 int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb, LearnedBufs *lb,
                 int nsteps, hipStream_t s, std::string *err);
 // Primitive planes (rho, p, u, v, w, nrg, Y_s) of U over every cell (3D).

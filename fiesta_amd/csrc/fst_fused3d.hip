@@ -133,6 +133,19 @@ __device__ __forceinline__ double cdiv(double a, double b, double rb) {
   return fma(e, rb, q0);
 }
 
+// Stage update of the fused kernels' epilogues, one value: form 1: U + h dv
+// (the first stage of either scheme), 2: Un + h dv (rkAdvance's second stage),
+// 3: a Un + b (U + h dv) (SSP-RK3 stages 2 and 3); U is the stage input, Un
+// the state at the start of the step.
+struct Upd {
+  double h, a, b;
+  int form;
+};
+__device__ __forceinline__ double upd_val(const Upd &u, double us, double ub, double dv) {
+  if (u.form == 3) return u.a * ub + u.b * (us + u.h * dv);
+  return (u.form == 1 ? us : ub) + u.h * dv;
+}
+
 template <int D>
 __device__ __forceinline__ uint32_t stride(const Grid32 &g) {
   return D == 0 ? 1u : (D == 1 ? g.s1 : g.s2);
@@ -894,12 +907,12 @@ __global__ void __launch_bounds__(NTS, 1)
 // FP64 in both forms.  LC: learned C-indicator (C_hat, n from CH, DR; no
 // C-equation variables, nvt = nv; maxCh comes from the inference pass).
 // FO: Uo is the float stage state of the learned FP32 form (stage 1).
-template <int NS, bool F32, bool LC, bool FO = false>
+template <int NS, bool F32, bool LC, bool FO = false, bool BL = false>
 __global__ void __launch_bounds__(NTHREADS, 1)
     k_vcr3d(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ Un,
             double *__restrict__ Uo, const double *__restrict__ P, double *__restrict__ Po,
             const double *__restrict__ dvar, const float *__restrict__ CH, const float *__restrict__ DR,
-            const Scalars *scin, Scalars *scout, double h, int stage, int visc, int ceq, int kchunk, int nbx,
+            const Scalars *scin, Scalars *scout, Upd up, int visc, int ceq, int kchunk, int nbx,
             int nby, int nbz) {
   using A = Ar<F32>;
   using R = typename A::R;
@@ -1005,8 +1018,8 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       double un[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        const double base = (stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v);
-        un[v] = base + h * dv[v];
+        if constexpr (BL) un[v] = upd_val(up, ld(U, c + N * v), ld(Un, c + N * v), dv[v]);
+        else un[v] = ((up.form == 1) ? ld(U, c + N * v) : ld(Un, c + N * v)) + up.h * dv[v];
         ust<FO>(Uo, c + N * v, un[v]);
       }
       double pr[6 + NS];
@@ -1211,12 +1224,12 @@ __device__ __forceinline__ void face_vc_ring(const FC &k, const RingAt<NS, F32, 
   }
 }
 
-template <int NS, bool F32, bool LC, bool FO = false>
+template <int NS, bool F32, bool LC, bool FO = false, bool BL = false>
 __global__ void __launch_bounds__(NTHREADS, 1)
     k_vcr3d_ring(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ Un,
                 double *__restrict__ Uo, const double *__restrict__ P, double *__restrict__ Po,
                 const double *__restrict__ dvar, const float *__restrict__ CH, const float *__restrict__ DR,
-                const Scalars *scin, Scalars *scout, double h, int stage, int visc, int ceq, int kchunk, int nbx,
+                const Scalars *scin, Scalars *scout, Upd up, int visc, int ceq, int kchunk, int nbx,
                 int nby, int nbz) {
   using A = Ar<F32>;
   using RF = RingF<NS, F32, LC>;
@@ -1267,8 +1280,10 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     if (EARLY && t.cellok) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) dvp[v] = ld_dv<F32>(dvar, c + N * v);
+      if (!BL) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) bsp[v] = (stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v);
+        for (int v = 0; v < NV; ++v) bsp[v] = (up.form == 1) ? ld(U, c + N * v) : ld(Un, c + N * v);
+      }
     }
     at.sl[0] = (kk - 1) & 3; at.sl[1] = kk & 3; at.sl[2] = (kk + 1) & 3;
     if (faces) {
@@ -1386,8 +1401,12 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       double un[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        const double base = EARLY ? bsp[EARLY ? v : 0] : ((stage == 1) ? ld(U, c + N * v) : ld(Un, c + N * v));
-        un[v] = base + h * dv[v];
+        if constexpr (BL) {
+          un[v] = upd_val(up, ld(U, c + N * v), ld(Un, c + N * v), dv[v]);
+        } else {
+          const double base = EARLY ? bsp[EARLY ? v : 0] : ((up.form == 1) ? ld(U, c + N * v) : ld(Un, c + N * v));
+          un[v] = base + up.h * dv[v];
+        }
         ust<FO>(Uo, c + N * v, un[v]);
       }
       double pr[6 + NS];
@@ -1454,11 +1473,11 @@ __device__ __forceinline__ void ceq_fetch(const Grid32 &g, const double *__restr
   }
 }
 
-template <bool F32>
+template <bool F32, bool BL = false>
 __global__ void __launch_bounds__(CNT)
     k_ceq3d(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ Un,
-            double *__restrict__ Uo, const double *__restrict__ P, const Scalars *scin, Scalars *scout, double h,
-            int stage, int kchunk, int nbx, int nby, int nbz) {
+            double *__restrict__ Uo, const double *__restrict__ P, const Scalars *scin, Scalars *scout, Upd up,
+            int kchunk, int nbx, int nby, int nbz) {
   using A = Ar<F32>;
   using R = typename A::R;
   __shared__ R pl[CNF * CPL];
@@ -1524,9 +1543,17 @@ __global__ void __launch_bounds__(CNT)
       qw[4] = nw;
 #pragma unroll
       for (int n = 0; n < 5; ++n) qc[2][n] = nc[n];
-      double base[5];
+      // the cell's C variables in U (stage input) and, for the SSP-RK3 blend, Un
+      double base[5], ub[BL ? 5 : 1];
 #pragma unroll
-      for (int n = 0; n < 5; ++n) base[n] = (stage == 1) ? ld(U, c + nvb + N * n) : ld(Un, c + nvb + N * n);
+      for (int n = 0; n < 5; ++n) {
+        if constexpr (BL) {
+          base[n] = ld(U, c + nvb + N * n);
+          ub[n] = ld(Un, c + nvb + N * n);
+        } else {
+          base[n] = (up.form == 1) ? ld(U, c + nvb + N * n) : ld(Un, c + nvb + N * n);
+        }
+      }
       if (kk + 1 < ke) {  // z-queue entries of the next plane
         nr = PV(rb, c + 3 * s2);
         ne = PV(eb, c + 3 * s2);
@@ -1566,7 +1593,9 @@ __global__ void __launch_bounds__(CNT)
                       A::cdv(L(4 + n, -1, 0) - (R)2 * cc + L(4 + n, 1, 0), k.d[1], k.rd[1]) +
                       A::cdv(qc[0][n] - (R)2 * cc + qc[2][n], k.d[2], k.rd[2]);
         const double dv = c1 * (G[n] - cc) + c2 * lap;
-        const double un = base[n] + h * dv;
+        double un;
+        if constexpr (BL) un = upd_val(up, base[n], ub[n], dv);
+        else un = base[n] + up.h * dv;
         st(Uo, c + nvb + N * n, un);
         if (n == 1) mC = fmax(mC, fabs(un));
       }
@@ -1590,6 +1619,50 @@ __global__ void __launch_bounds__(CNT)
 
 // ---------------------------------------------------------------- primitives
 
+// Cells of the ghost shell: thread t enumerates the k slabs (k < ng or
+// k >= ng+nck), then the j slabs of the interior k range, then the i slabs of
+// the interior j and k ranges.  Returns false past the last ghost cell.
+__device__ __forceinline__ bool ghost_cell(const Grid32 &g, uint32_t t, uint32_t *cell) {
+  const int ng = g.ng;
+  const uint32_t nk = 2u * ng * g.s2;                     // k < ng or k >= ng+nck
+  const uint32_t nj = 2u * ng * (uint32_t)g.ngi * g.nck;  // interior k, j ghost
+  const uint32_t ni = 2u * ng * (uint32_t)g.ncj * g.nck;  // interior k and j, i ghost
+  if (t < nk) {
+    const uint32_t kq = t / g.s2, r = t % g.s2;
+    const uint32_t kk = kq < (uint32_t)ng ? kq : kq + g.nck;
+    *cell = r + kk * g.s2;
+  } else if (t < nk + nj) {
+    t -= nk;
+    const uint32_t i = t % g.ngi, q = t / g.ngi;
+    const uint32_t jq = q % (2u * ng), kq = q / (2u * ng);
+    const uint32_t j = jq < (uint32_t)ng ? jq : jq + g.ncj;
+    *cell = i + j * g.s1 + (kq + ng) * g.s2;
+  } else if (t < nk + nj + ni) {
+    t -= nk + nj;
+    const uint32_t iq = t % (2u * ng), q = t / (2u * ng);
+    const uint32_t j = q % g.ncj, kq = q / g.ncj;
+    const uint32_t i = iq < (uint32_t)ng ? iq : iq + g.nci;
+    *cell = i + (j + ng) * g.s1 + (kq + ng) * g.s2;
+  } else {
+    return false;
+  }
+  return true;
+}
+
+// Ghost cells of the last SSP-RK3 stage.  The stage update covers every cell
+// (as rkAdvance's does) with dvar = 0 on the ghosts: U^{n+1} = a Un + b (U + h 0)
+// with Un the step's ghost-filled initial state (Uo, in place) and U the
+// ghost-filled stage input.  Earlier stages need no such pass: the next
+// stage's ghost fill rewrites every ghost cell.
+__global__ void k_rk_ghost(const Grid32 g, const double *__restrict__ U, double *__restrict__ Uo, Upd up) {
+  uint32_t cell;
+  if (!ghost_cell(g, blockIdx.x * blockDim.x + threadIdx.x, &cell)) return;
+  for (int v = 0; v < g.nvt; ++v) {
+    const uint32_t c = cell + g.N * v;
+    Uo[c] = upd_val(up, U[c], Uo[c], 0.0);
+  }
+}
+
 // Primitives of U over the whole ghost range, with the maxima of the interior
 // (ghost_only = 0), or over the ghost cells only (ghost_only = 1: threads
 // enumerate the k slabs, then the j slabs, then the i slabs of the ghost
@@ -1604,27 +1677,7 @@ __global__ void k_prims3d(const Grid32 g, const FC k, const double *__restrict__
   bool inrange, interior = false;
   uint32_t cell = 0;
   if (ghost_only) {
-    const uint32_t nk = 2u * ng * g.s2;                          // k < ng or k >= ng+nck
-    const uint32_t nj = 2u * ng * (uint32_t)g.ngi * g.nck;       // interior k, j ghost
-    const uint32_t ni = 2u * ng * (uint32_t)g.ncj * g.nck;       // interior k and j, i ghost
-    inrange = t < nk + nj + ni;
-    if (t < nk) {
-      const uint32_t kq = t / g.s2, r = t % g.s2;
-      const uint32_t kk = kq < (uint32_t)ng ? kq : kq + g.nck;
-      cell = r + kk * g.s2;
-    } else if (t < nk + nj) {
-      t -= nk;
-      const uint32_t i = t % g.ngi, q = t / g.ngi;
-      const uint32_t jq = q % (2u * ng), kq = q / (2u * ng);
-      const uint32_t j = jq < (uint32_t)ng ? jq : jq + g.ncj;
-      cell = i + j * g.s1 + (kq + ng) * g.s2;
-    } else if (inrange) {
-      t -= nk + nj;
-      const uint32_t iq = t % (2u * ng), q = t / (2u * ng);
-      const uint32_t j = q % g.ncj, kq = q / g.ncj;
-      const uint32_t i = iq < (uint32_t)ng ? iq : iq + g.nci;
-      cell = i + (j + ng) * g.s1 + (kq + ng) * g.s2;
-    }
+    inrange = ghost_cell(g, t, &cell);
   } else {
     inrange = t < g.N;
     cell = t;
@@ -1717,7 +1770,7 @@ Launch shape(const Geo &g, int kchunk) {
 template <int NS, bool F32, bool LC>
 void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf, const double *U,
                const double *Un, double *Uo, const double *P, double *Po, double *dvar, const float *CH,
-               const float *DR, const Scalars *scin, Scalars *scout, double h, int stage, int kchunk,
+               const float *DR, const Scalars *scin, Scalars *scout, Upd up, int stage, int kchunk,
                Prof *prof, hipStream_t s, bool fs) {
   const Launch L = shape(g, kchunk);
   if (F32) {  // FP32 forms: binary32 plane, 11-row tiles (one implementation)
@@ -1737,26 +1790,37 @@ void run_stage(const Grid32 &q, const Geo &g, const FC &k, const fst_config &cf,
     if (fs && stage == 1) {
       if (ring)
         k_vcr3d_ring<NS, F32, LC, true><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout,
-                                                                  h, stage, cf.visc, cf.ceq, kchunk, L.nbx, L.nby,
+                                                                  up, cf.visc, cf.ceq, kchunk, L.nbx, L.nby,
                                                                   L.nbz);
       else
-        k_vcr3d<NS, F32, LC, true><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h,
-                                                             stage, cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
+        k_vcr3d<NS, F32, LC, true><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, up,
+                                                             cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
       if (prof) prof->mark(s);
       return;
     }
   }
-  if (ring)
-    k_vcr3d_ring<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h,
-                                                       stage, cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
+  const bool bl = up.form == 3;  // SSP-RK3 blend of U^n and the stage result
+  if (ring && bl)
+    k_vcr3d_ring<NS, F32, LC, false, true><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin,
+                                                                     scout, up, cf.visc, cf.ceq, kchunk, L.nbx,
+                                                                     L.nby, L.nbz);
+  else if (ring)
+    k_vcr3d_ring<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, up,
+                                                       cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
+  else if (bl)
+    k_vcr3d<NS, F32, LC, false, true><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout,
+                                                                up, cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
   else
-    k_vcr3d<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, h, stage,
+    k_vcr3d<NS, F32, LC><<<L.nb, NTHREADS, 0, s>>>(q, k, U, Un, Uo, P, Po, dvar, CH, DR, scin, scout, up,
                                                   cf.visc, cf.ceq, kchunk, L.nbx, L.nby, L.nbz);
   if (prof) prof->mark(s);  // end of the viscous / stage-update kernel
   if (cf.ceq && !LC) {  // the C variables: sources, relaxation and stage update
     const int kc = 32;
     const int nbx = (g.nci + 63) / 64, nby = (g.ncj + CY - 1) / CY, nbz = (g.nck + kc - 1) / kc;
-    k_ceq3d<F32><<<nbx * nby * nbz, CNT, 0, s>>>(q, k, U, Un, Uo, P, scin, scout, h, stage, kc, nbx, nby, nbz);
+    if (bl)
+      k_ceq3d<F32, true><<<nbx * nby * nbz, CNT, 0, s>>>(q, k, U, Un, Uo, P, scin, scout, up, kc, nbx, nby, nbz);
+    else
+      k_ceq3d<F32><<<nbx * nby * nbz, CNT, 0, s>>>(q, k, U, Un, Uo, P, scin, scout, up, kc, nbx, nby, nbz);
   }
 }
 
@@ -1774,7 +1838,7 @@ void run_prims(const Grid32 &q, const FC &k, const double *U, double *P, Scalars
 
 using StageFn = void (*)(const Grid32 &, const Geo &, const FC &, const fst_config &, const double *,
                          const double *, double *, const double *, double *, double *, const float *,
-                         const float *, const Scalars *, Scalars *, double, int, int, Prof *, hipStream_t, bool);
+                         const float *, const Scalars *, Scalars *, Upd, int, int, Prof *, hipStream_t, bool);
 using PrimFn = void (*)(const Grid32 &, const FC &, const double *, double *, Scalars *, int, Scalars *,
                         hipStream_t, bool);
 
@@ -1819,8 +1883,9 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
     return FST_ERR_STATE;
   }
   const int np = 6 + cf.ns;
+  const bool rk3 = rk_stages(cf) == 3;
+  const size_t bytes = (size_t)g.N * sizeof(double);
   if (!fb.stage) {
-    const size_t bytes = (size_t)g.N * sizeof(double);
     if (hipMalloc((void **)&fb.stage, bytes * g.nvt) != hipSuccess ||
         hipMalloc((void **)&fb.prim, bytes * np * 2) != hipSuccess ||
         hipMalloc((void **)&fb.sc, sizeof(Scalars) * 2) != hipSuccess) {
@@ -1830,14 +1895,21 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
     (void)hipMemsetAsync(fb.stage, 0, bytes * g.nvt, s);
     (void)hipMemsetAsync(fb.prim, 0, bytes * np * 2, s);
   }
+  if (rk3 && !fb.stage2) {
+    if (hipMalloc((void **)&fb.stage2, bytes * g.nvt) != hipSuccess) {
+      *err = "fused path: device allocation failed";
+      return FST_ERR_NOMEM;
+    }
+    (void)hipMemsetAsync(fb.stage2, 0, bytes * g.nvt, s);
+  }
   const FC k = make_fc(ph, cf);
   const Grid32 q = make_grid(g);
   // planes per block: 8 for the FP64 kernels, 32 for the FP32 forms (k-chunk
   // sweeps at 256^3 over 4..32 planes)
   int kchunk = fb.kchunk > 0 ? fb.kchunk : (cf.precision == FST_FP32_BULK ? 32 : 8);
   if (const char *ev = getenv("FST_KCHUNK")) kchunk = std::max(1, atoi(ev));  // tuning override
-  double *Pa = fb.prim, *Pb = fb.prim + (size_t)g.N * np;
-  Scalars *S0 = fb.sc, *S1 = fb.sc + 1;
+  double *const Pset[2] = {fb.prim, fb.prim + (size_t)g.N * np};
+  Scalars *const Sset[2] = {fb.sc, fb.sc + 1};
   Prof *prof = b.prof && b.prof->on ? b.prof : nullptr;
   if (prof) {
     prof->name = lc ? "stage: ghost fill + ghost primitives + [learned C fields, stage 1] + k_adv3d + k_vcr3d"
@@ -1846,43 +1918,58 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
   }
   const float *CH = lc ? lb->chat : nullptr, *DR = lc ? lb->dir : nullptr;
   if (!fb.valid) {  // primitives and maxima of var, all cells
-    k_clear<<<1, 64, 0, s>>>(S0);
-    pf(q, k, b.var, Pa, S0, 0, nullptr, s, false);
+    fb.cur = 0;
+    k_clear<<<1, 64, 0, s>>>(Sset[0]);
+    pf(q, k, b.var, Pset[0], Sset[0], 0, nullptr, s, false);
     fb.valid = true;
   }
-  const double h1 = 0.5f * cf.dt, h2 = cf.dt;
-  // learned FP32 form: the stage state U* is stored as floats (it only feeds
-  // the binary32 right-hand side of stage 2; U^n and U^{n+1} stay FP64)
-  const bool fs = lc && cf.precision == FST_FP32_BULK;
-  for (int t = 0; t < nsteps; ++t) {
-    // stage 1: U^n (b.var) -> U* (fb.stage); Pa/S0 hold the primitives/maxima of U^n
+  // learned FP32 form (RK2): the stage state U* is stored as floats (it only
+  // feeds the binary32 right-hand side of stage 2; U^n and U^{n+1} stay FP64)
+  const bool fs = lc && cf.precision == FST_FP32_BULK && !rk3;
+  // stage s: input Uin (ghosts filled here), update form up, output Uout;
+  // P[cur]/S[cur] describe Uin on entry, P[cur^1]/S[cur^1] receive Uout's
+  auto stage_run = [&](int st, const double *Uin, double *Uout, Upd up, bool fin, bool fout) -> int {
+    double *Pi = Pset[fb.cur], *Po = Pset[fb.cur ^ 1];
+    Scalars *Si = Sset[fb.cur], *So = Sset[fb.cur ^ 1];
     if (prof) prof->begin(s);
-    launch_apply_bcs(g, cf, b.var, s);       // src/cart3d.cpp:221
-    pf(q, k, b.var, Pa, nullptr, 1, S1, s, false);  // ghost primitives; clears S1
+    if (fin) launch_apply_bcs(g, cf, reinterpret_cast<float *>(const_cast<double *>(Uin)), s);
+    else launch_apply_bcs(g, cf, const_cast<double *>(Uin), s);  // src/cart3d.cpp:221
+    pf(q, k, Uin, Pi, nullptr, 1, So, s, fin);                     // ghost primitives; clears So
     if (prof) prof->mark(s);
-    if (lc) {  // C_hat and n of U^n, used by both stages of the step
-      const int rc = learned_fields(g, cf, ph, Pa, cf.precision == FST_FP32_BULK, *lb, s);
+    if (lc && st == 1) {  // C_hat and n of U^n, used by every stage of the step
+      const int rc = learned_fields(g, cf, ph, Pi, cf.precision == FST_FP32_BULK, *lb, s);
       if (rc) {
         *err = "learned C-indicator: launch failed";
         return rc;
       }
     }
     if (prof) prof->mark(s);
-    sf(q, g, k, cf, b.var, nullptr, fb.stage, Pa, Pb, b.dvar, CH, DR, lc ? lb->sc : S0, S1, h1, 1, kchunk, prof, s,
-       fs);
-    if (prof) prof->end(s);
-    // stage 2: U* -> U^{n+1}, written in place into b.var (read pointwise)
-    if (prof) prof->begin(s);
-    if (fs) launch_apply_bcs(g, cf, reinterpret_cast<float *>(fb.stage), s);
-    else launch_apply_bcs(g, cf, fb.stage, s);
-    pf(q, k, fb.stage, Pb, nullptr, 1, S0, s, fs);  // ghost primitives of U*; clears S0
-    if (prof) {
-      prof->mark(s);
-      prof->mark(s);  // no learned fields in stage 2
+    sf(q, g, k, cf, Uin, b.var, Uout, Pi, Po, b.dvar, CH, DR, lc ? lb->sc : Si, So, up, st, kchunk, prof, s,
+       fs && (fout || fin));
+    fb.cur ^= 1;
+    return FST_OK;
+  };
+  for (int t = 0; t < nsteps; ++t) {
+    int rc;
+    if (!rk3) {
+      // rkAdvance: U* = U^n + dt/2 R(U^n) (fb.stage), then U^{n+1} = U^n + dt R(U*)
+      // written in place into b.var (read pointwise)
+      if ((rc = stage_run(1, b.var, fb.stage, Upd{0.5f * cf.dt, 0.0, 0.0, 1}, false, fs))) return rc;
+      if (prof) prof->end(s);
+      if ((rc = stage_run(2, fb.stage, b.var, Upd{cf.dt, 0.0, 0.0, 2}, fs, false))) return rc;
+      if (prof) prof->end(s);
+    } else {
+      // SSP-RK3: U1 (fb.stage), U2 (fb.stage2), U^{n+1} in place into b.var
+      if ((rc = stage_run(1, b.var, fb.stage, Upd{cf.dt, 0.0, 0.0, 1}, false, false))) return rc;
+      if (prof) prof->end(s);
+      if ((rc = stage_run(2, fb.stage, fb.stage2, Upd{cf.dt, RK3_A2, RK3_B2, 3}, false, false))) return rc;
+      if (prof) prof->end(s);
+      const Upd u3{cf.dt, RK3_A3, RK3_B3, 3};
+      if ((rc = stage_run(3, fb.stage2, b.var, u3, false, false))) return rc;
+      const uint32_t ngh = 2u * q.ng * q.s2 + 2u * q.ng * (uint32_t)q.ngi * q.nck + 2u * q.ng * (uint32_t)q.ncj * q.nck;
+      k_rk_ghost<<<(ngh + 255) / 256, 256, 0, s>>>(q, fb.stage2, b.var, u3);
+      if (prof) prof->end(s);
     }
-    sf(q, g, k, cf, fb.stage, b.var, b.var, Pb, Pa, b.dvar, CH, DR, lc ? lb->sc : S1, S0, h2, 2, kchunk, prof, s,
-       fs);
-    if (prof) prof->end(s);
   }
   return FST_OK;
 }
@@ -1899,9 +1986,10 @@ int fused_primitives(const Geo &g, const Phys &ph, const fst_config &cf, const d
 
 void fused_free(FusedBufs &fb) {
   if (fb.stage) (void)hipFree(fb.stage);
+  if (fb.stage2) (void)hipFree(fb.stage2);
   if (fb.prim) (void)hipFree(fb.prim);
   if (fb.sc) (void)hipFree(fb.sc);
-  fb.stage = fb.prim = nullptr;
+  fb.stage = fb.stage2 = fb.prim = nullptr;
   fb.sc = nullptr;
 }
 

@@ -118,7 +118,12 @@ void launch_eos_vel(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
 void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);
 void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream_t s);
 void launch_apply_bcs(const Geo &g, const fst_config &cf, float *var, hipStream_t s);  // float stage state
-void launch_rk_stage(const Geo &g, int stage, double dt, double *var, double *tmp, const double *dvar, hipStream_t s);
+// Stages per time step: 2 (RK2 midpoint, rkAdvance) or 3 (SSP-RK3, rk_order = 3).
+inline int rk_stages(const fst_config &c) { return c.rk_order == 3 ? 3 : 2; }
+// SSP-RK3 (Shu-Osher) stage s = 2, 3: var = A*tmp1 + B*(var + dt*dvar)
+constexpr double RK3_A2 = 0.75, RK3_B2 = 0.25, RK3_A3 = 1.0 / 3.0, RK3_B3 = 2.0 / 3.0;
+void launch_rk_stage(const Geo &g, int nstages, int stage, double dt, double *var, double *tmp, const double *dvar,
+                     hipStream_t s);
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 void launch_secondary(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
 // generalized grids (gen2d/gen3d modules)

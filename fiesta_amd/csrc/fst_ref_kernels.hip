@@ -779,16 +779,27 @@ __global__ void k_zero_scalars(Scalars *sc) {
 }
 
 // rkAdvance Loop1 / Loop2, src/rk.cpp:42-50 and :63-69, over every cell
-__global__ void k_rk_stage(long long n, int stage, double dt, double *__restrict__ var, double *__restrict__ tmp,
-                           const double *__restrict__ dvar) {
+// rkAdvance stage updates over every cell (src/rk.cpp:42-50, :63-69); with
+// three stages the SSP-RK3 updates (no counterpart in the reference)
+__global__ void k_rk_stage(long long n, int nstages, int stage, double dt, double *__restrict__ var,
+                           double *__restrict__ tmp, const double *__restrict__ dvar) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  if (stage == 1) {
+  if (nstages == 2) {
+    if (stage == 1) {
+      double u = var[t];
+      tmp[t] = u;
+      var[t] = u + 0.5f * dt * dvar[t];
+    } else {
+      var[t] = tmp[t] + dt * dvar[t];
+    }
+  } else if (stage == 1) {
     double u = var[t];
     tmp[t] = u;
-    var[t] = u + 0.5f * dt * dvar[t];
+    var[t] = u + dt * dvar[t];
   } else {
-    var[t] = tmp[t] + dt * dvar[t];
+    const double a = stage == 2 ? RK3_A2 : RK3_A3, b = stage == 2 ? RK3_B2 : RK3_B3;
+    var[t] = a * tmp[t] + b * (var[t] + dt * dvar[t]);
   }
 }
 
@@ -1119,9 +1130,10 @@ void apply_bcs_t(const Geo &g, const fst_config &cf, T *var, hipStream_t s) {
 void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream_t s) { apply_bcs_t(g, cf, var, s); }
 void launch_apply_bcs(const Geo &g, const fst_config &cf, float *var, hipStream_t s) { apply_bcs_t(g, cf, var, s); }
 
-void launch_rk_stage(const Geo &g, int stage, double dt, double *var, double *tmp, const double *dvar, hipStream_t s) {
+void launch_rk_stage(const Geo &g, int nstages, int stage, double dt, double *var, double *tmp, const double *dvar,
+                     hipStream_t s) {
   long long n = g.N * g.nvt;
-  k_rk_stage<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, stage, dt, var, tmp, dvar);
+  k_rk_stage<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, nstages, stage, dt, var, tmp, dvar);
 }
 
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {

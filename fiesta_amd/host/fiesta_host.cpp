@@ -135,10 +135,12 @@ rk_func *make_module(const fst_config &cf, const std::vector<std::string> &sp) {
 
 void rkAdvance(rk_func &f) {
   fst_ctx *c = f.context();
-  f.compute();
-  if (int rc = fst_rk_stage(c, 1)) throw Error(std::string("fst_rk_stage(1): ") + fst_last_error(c), rc);
-  f.compute();
-  if (int rc = fst_rk_stage(c, 2)) throw Error(std::string("fst_rk_stage(2): ") + fst_last_error(c), rc);
+  const int nstages = f.config().rk_order == 3 ? 3 : 2;  // RK2 midpoint, or the SSP-RK3 option
+  for (int s = 1; s <= nstages; ++s) {
+    f.compute();
+    if (int rc = fst_rk_stage(c, s))
+      throw Error("fst_rk_stage(" + std::to_string(s) + "): " + fst_last_error(c), rc);
+  }
 }
 
 void applyBCs(rk_func &f) {

@@ -155,7 +155,8 @@ class gen2d_func : public rk_func {
 // cart3d, gen2d or gen3d from grid.ndim and grid.type).
 rk_func *make_module(const fst_config &cf, const std::vector<std::string> &speciesName = {});
 
-// rkAdvance (src/rk.cpp:28-72): compute, stage-1 update, compute, stage-2 update.
+// rkAdvance (src/rk.cpp:28-72): compute, stage-1 update, compute, stage-2
+// update; with rk_order = 3, the three compute + stage-update pairs of SSP-RK3.
 void rkAdvance(rk_func &f);
 // applyBCs (src/bc.cpp:179-249).
 void applyBCs(rk_func &f);

@@ -304,8 +304,8 @@ class Trainer:
 
 
 def rk_advance(f):
-    """rkAdvance (src/rk.cpp:28-72): two compute() calls, two stage updates."""
-    f.compute()
-    f.rk_stage(1)
-    f.compute()
-    f.rk_stage(2)
+    """rkAdvance (src/rk.cpp:28-72): two compute() calls, two stage updates
+    (three of each with the SSP-RK3 option, rk_order = 3)."""
+    for s in range(1, (3 if f.cfg.rk_order == 3 else 2) + 1):
+        f.compute()
+        f.rk_stage(s)

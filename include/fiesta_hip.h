@@ -33,7 +33,7 @@ extern "C" {
 #endif
 
 #define FST_MAX_SPECIES 8
-#define FST_ABI_VERSION 2
+#define FST_ABI_VERSION 3
 
 /* Boundary types, in the order of `enum class BCType` (src/bc.hpp:31). */
 enum fst_bc_type {
@@ -107,7 +107,10 @@ typedef struct fst_config {
   int32_t learned_c;     /* 1: learned C-indicator replaces the C-equation's
                             relaxation solve (3D, ceq = 1, fused path only;
                             nvt = nv: no C-equation variables are stored)    */
-  int32_t reserved[8];
+  int32_t rk_order;      /* 2 (or 0): the reference's RK2 midpoint (rkAdvance);
+                            3: three-stage SSP-RK3 (Shu-Osher), an option the
+                            reference does not have                          */
+  int32_t reserved[7];
 } fst_config;
 
 typedef struct fst_ctx fst_ctx;
@@ -153,15 +156,20 @@ int fst_apply_bcs(fst_ctx *ctx);
  * (src/cart3d.cpp:220-298, src/cart2d.cpp:151-237, src/gen3d.cpp:341-369,
  * src/gen2d.cpp:100-131). */
 int fst_compute(fst_ctx *ctx);
-/* One Runge-Kutta stage update on the current dvar (src/rk.cpp:42-50 for
- * stage 1, src/rk.cpp:63-69 for stage 2). */
+/* One Runge-Kutta stage update on the current dvar, over every cell
+ * (src/rk.cpp:42-50 for stage 1, src/rk.cpp:63-69 for stage 2).  With
+ * rk_order = 3, stages 1..3 of SSP-RK3 (tmp1 = U^n):
+ *   1: tmp1 = var; var = var + dt*dvar
+ *   2: var = 0.75*tmp1 + 0.25*(var + dt*dvar)
+ *   3: var = (1.0/3.0)*tmp1 + (2.0/3.0)*(var + dt*dvar)                  */
 int fst_rk_stage(fst_ctx *ctx, int stage);
 /* rk_func::postStep: secondary variables varx (src/cart3d.cpp:300-317). */
 int fst_post_step(fst_ctx *ctx);
 
 /* `nsteps` full time steps: Fiesta::step (src/fiesta.cpp:333-340) =
- * preStep + rkAdvance, on the fused fast path (one RHS+RK kernel per stage).
- * Results equal the compute()/rk_stage() sequence. */
+ * preStep + rkAdvance (rk_order stages of compute() + rk_stage), on the fused
+ * fast path (one RHS+RK kernel per stage).  Results equal the
+ * compute()/rk_stage() sequence. */
 int fst_step(fst_ctx *ctx, int nsteps);
 
 /* Per-stage global maxima of the C-equation (max wave speed, max |C_hat|)

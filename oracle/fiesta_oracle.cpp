@@ -1340,8 +1340,35 @@ void compute(Orc &o) {
   }
 }
 
+int rk_stages(const Orc &o) { return o.c.rk_order == 3 ? 3 : 2; }
+
+// SSP-RK3 (Shu & Osher 1988) over every cell, as rkAdvance updates every
+// cell; tmp1 keeps U^n.  Not in the reference (rk_order = 3 is an option):
+//   U1 = U^n + dt R(U^n)
+//   U2 = 3/4 U^n + 1/4 (U1 + dt R(U1))
+//   U^{n+1} = 1/3 U^n + 2/3 (U2 + dt R(U2))
+void rk3_stage(Orc &o, int stage) {
+  const double dt = o.c.dt;
+  const size_t tot = o.N * o.nvt;
+  if (stage == 1) {
+#pragma omp parallel for
+    for (size_t q = 0; q < tot; ++q) {
+      o.tmp1[q] = o.var[q];
+      o.var[q] = o.var[q] + dt * o.dvar[q];
+    }
+  } else {
+    const double a = stage == 2 ? 0.75 : 1.0 / 3.0, b = stage == 2 ? 0.25 : 2.0 / 3.0;
+#pragma omp parallel for
+    for (size_t q = 0; q < tot; ++q) o.var[q] = a * o.tmp1[q] + b * (o.var[q] + dt * o.dvar[q]);
+  }
+}
+
 // rkAdvance stage updates, src/rk.cpp:42-50 (stage 1) and :63-69 (stage 2)
 void rk_stage(Orc &o, int stage) {
+  if (rk_stages(o) == 3) {
+    rk3_stage(o, stage);
+    return;
+  }
   const double dt = o.c.dt;
   const size_t tot = o.N * o.nvt;
   if (stage == 1) {
@@ -1517,10 +1544,10 @@ void orc_step(void *h, int nsteps) {
   Orc &o = *(Orc *)h;
   for (int t = 0; t < nsteps; ++t) {
     pre_step(o);
-    compute(o);
-    rk_stage(o, 1);
-    compute(o);
-    rk_stage(o, 2);
+    for (int s = 1; s <= rk_stages(o); ++s) {
+      compute(o);
+      rk_stage(o, s);
+    }
   }
 }
 

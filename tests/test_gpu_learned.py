@@ -125,10 +125,12 @@ def test_inference_matches_oracle():
     tr.close()
 
 
+@pytest.mark.parametrize("rk", [2, 3])
 @pytest.mark.parametrize("precision", [0])
-def test_learned_steps_equal_oracle(precision):
-    print("case: learned steps shear 24x24x8 fp64", flush=True)
+def test_learned_steps_equal_oracle(precision, rk):
+    print("case: learned steps shear 24x24x8 fp64, rk", rk, flush=True)
     cfg, var = _learned_case(n=24, nz=8, seed=4, precision=precision)
+    cfg.rk_order = rk
     tr = Trainer(seed=11)
     m = Module(cfg)
     m.upload(FST_VAR, var)
@@ -152,13 +154,15 @@ def test_learned_steps_equal_oracle(precision):
     tr.close()
 
 
-def test_fp32_learned_close_to_fp64_learned():
-    print("case: fp32 learned vs fp64 learned shear 24x24x8", flush=True)
+@pytest.mark.parametrize("rk", [2, 3])
+def test_fp32_learned_close_to_fp64_learned(rk):
+    print("case: fp32 learned vs fp64 learned shear 24x24x8, rk", rk, flush=True)
     outs = []
     tr = Trainer(seed=11)
     w = tr.weights()
     for prec in (0, FST_FP32_BULK):
         cfg, var = _learned_case(n=24, nz=8, seed=4, precision=prec)
+        cfg.rk_order = rk
         m = Module(cfg)
         m.upload(FST_VAR, var)
         m.set_mlp_weights(w)

@@ -37,9 +37,13 @@ def rel_errors(a, b):
 
 
 @pytest.mark.parametrize("name,kw", [("rt_like", dict()), ("walls_ns3", dict(ns=3, per=(False, False, False),
-                                     bc=("outflow", "reflective", "hydrostatic", "noslip", "reflective", "outflow")))])
+                                     bc=("outflow", "reflective", "hydrostatic", "noslip", "reflective", "outflow"))),
+                                     ("rt_like_ssp_rk3", dict(rk=3))])
 def test_fp32_form_error_against_fp64_oracle(name, kw):
+    kw = dict(kw)
+    rk = kw.pop("rk", 2)
     cfg = cfg3d(nci=40, ncj=21, nck=18, **kw)
+    cfg.rk_order = rk
     print("case fp32 form", name, flush=True)
     var = W.random_state(cfg, 51)
     o = oracle.Oracle(cfg)

@@ -261,6 +261,28 @@ def test_fluid_at_rest_under_gravity_rk2_closed_form():
     np.testing.assert_allclose(u[3] - E0, 0.5 * f * f * dt * dt / rho, atol=4 * np.spacing(E0))
 
 
+def test_fluid_at_rest_under_gravity_ssp_rk3_closed_form():
+    """SSP-RK3 (rk_order = 3) on the same state: the momentum grows by f t and
+    the energy by f^2 t^2 / (2 rho) exactly (the scheme integrates quadratics
+    in t exactly); the stage weights 3/4, 1/4 and 1/3, 2/3 all enter."""
+    g0, rref, dt = 9.81, 0.4, 1e-3
+    cfg = make_config(ndim=3, ns=1, nci=5, ncj=5, nck=5, gamma=(1.4,), M=(0.02896,), mu=(0.0,), buoyancy=True,
+                      g=g0, rhoRef=rref, xper=True, yper=True, zper=True, dt=dt, rk=3)
+    d = Dims(cfg)
+    one = np.ones((d.nck, d.ncj, d.nci))
+    rho = 1.2
+    var = W.fill_conserved(cfg, [rho * one], [0 * one] * 3, 1e5 * one)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.step(1)
+    u = o.get(FST_VAR)[(slice(None),) + d.interior()[1:]]
+    f = -g0 * (rho - rref)
+    np.testing.assert_allclose(u[1], f * dt, rtol=1e-10)
+    E0 = 1e5 / 0.4
+    np.testing.assert_allclose(u[3] - E0, 0.5 * f * f * dt * dt / rho, rtol=1e-6, atol=4 * np.spacing(E0))
+
+
 # ------------------------------------------------------------------ C-equation
 
 def test_uniform_c_field_relaxes_with_rk2_factor():
@@ -285,6 +307,48 @@ def test_uniform_c_field_relaxes_with_rk2_factor():
         np.testing.assert_allclose(out[d.nv + n], C0[n] * (1 - z + z * z / 2), rtol=1e-12)
     # uniform flow: no artificial viscosity, momentum unchanged
     np.testing.assert_allclose(out[0], rho * u0, rtol=1e-14)
+
+
+def test_uniform_c_field_relaxes_with_ssp_rk3_factor():
+    """With rk_order = 3 the uniform C field decays by the SSP-RK3 stability
+    polynomial 1 - z + z^2/2 - z^3/6 (RK2 gives 1 - z + z^2/2)."""
+    eps, kap = 5.0, 10.0
+    dt = 2e-6
+    cfg = make_config(ndim=3, ns=1, nci=6, ncj=6, nck=6, dx=1e-2, dy=1e-2, dz=1e-2, gamma=(1.4,), M=(0.02896,),
+                      mu=(0.0,), ceq=True, kap=kap, eps=eps, alpha=20.0, xper=True, yper=True, zper=True, dt=dt,
+                      rk=3)
+    d = Dims(cfg)
+    one = np.ones((d.nck, d.ncj, d.nci))
+    rho, u0 = 1.2, 30.0
+    C0 = [0.7 * one, 0.3 * one, 0.2 * one, -0.1 * one, 0.5 * one]
+    var = W.fill_conserved(cfg, [rho * one], [u0 * one, 0 * one, 0 * one], 1e5 * one, C0)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.step(1)
+    out = o.get(FST_VAR)[(slice(None),) + d.interior()[1:]]
+    c = np.sqrt(1.4 * 1e5 / rho)
+    z = (c + u0) / (eps * np.sqrt(3) * 1e-2) * dt
+    for n in range(5):
+        np.testing.assert_allclose(out[d.nv + n], C0[n] * (1 - z + z * z / 2 - z ** 3 / 6), rtol=1e-12)
+    # a third-order term is there: RK2's factor is off by z^3/6 ~ 1e-5 relative here
+    assert abs(z ** 3 / 6) > 1e-9
+    np.testing.assert_allclose(out[0], rho * u0, rtol=1e-14)
+
+
+def test_ssp_rk3_periodic_steps_conserve_mass_momentum_energy():
+    cfg = make_config(ndim=3, ns=2, nci=8, ncj=6, nck=5, gamma=(1.4, 1.67), M=(0.02896, 0.004),
+                      mu=(0.0, 0.0), xper=True, yper=True, zper=True, dt=1e-7, rk=3)
+    d = Dims(cfg)
+    var = W.random_state(cfg, 9, amp=0.05)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    before = o.get(FST_VAR)[(slice(None),) + d.interior()[1:]].sum(axis=(1, 2, 3))
+    o.step(3)
+    after = o.get(FST_VAR)[(slice(None),) + d.interior()[1:]].sum(axis=(1, 2, 3))
+    scale = np.abs(var[(slice(None),) + d.interior()[1:]]).sum(axis=(1, 2, 3))
+    assert (np.abs(after - before) <= 1e-12 * scale).all(), (after - before) / scale
 
 
 @pytest.mark.parametrize("kind", ["shock", "contact"])
