@@ -13,7 +13,9 @@ cell-stage (stage 1 reads U, writes U*; stage 2 reads U*, U^n, writes U^{n+1}).
 
 Forms: fp64 (reported as `value`), fp32 (FP32 stencils, FP64 EOS/RK) and
 fp32_learned (fp32 + the learned C-indicator in place of the C-equation's
-relaxation solve: nvt = nv).  The learned network is trained first, outside
+relaxation solve: nvt = nv); a suffix _rk3 (fp64_rk3, fp32_learned_rk3) runs
+the three-stage SSP-RK3 option in place of the reference's RK2 midpoint and
+is compared with fp64_rk3.  The learned network is trained first, outside
 the timed region, on rows harvested on the GPU from plain runs (a 64^3 RT
 run and four 48x48x4 shear layers); each form's final state is compared with
 the FP64 plain final state (density, pressure, mass fractions).
@@ -42,8 +44,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--path", choices=["fused", "reference"], default="fused")
     ap.add_argument("--copy-probe", action="store_true", help="also time a plain copy kernel")
-    ap.add_argument("--forms", default="fp64,fp32,fp32_learned",
-                    help="comma list of arithmetic forms to time: fp64 (reported as value), fp32, fp32_learned")
+    ap.add_argument("--forms", default="fp64,fp32,fp32_learned,fp64_rk3,fp32_learned_rk3",
+                    help="comma list of arithmetic forms to time: fp64 (reported as value), fp32, fp32_learned; "
+                         "suffix _rk3 runs the SSP-RK3 option instead of the reference's RK2 midpoint")
     ap.add_argument("--save-weights", default=None, help="write the trained network (.npy, float32)")
     ap.add_argument("--load-weights", default=None, help="use this network (.npy) instead of training one")
     return ap.parse_args()
@@ -98,9 +101,11 @@ def gpu_leg(args, form="fp64", var=None, keep_state=False, weights=None):
     from fiesta_amd.runtime import Module
 
     cfg = W.rt3d_config(args.size)
-    if form.startswith("fp32"):
+    base, nst = (form[:-4], 3) if form.endswith("_rk3") else (form, 2)
+    cfg.rk_order = nst
+    if base.startswith("fp32"):
         cfg.precision = FST_FP32_BULK
-    learned = form.endswith("learned")
+    learned = base.endswith("learned")
     if learned:
         cfg.learned_c = 1
     d = Dims(cfg)
@@ -130,7 +135,7 @@ def gpu_leg(args, form="fp64", var=None, keep_state=False, weights=None):
     phases = m.profile_phases()
     m.profile(False)
     ev_ms = m.last_step_ms()
-    cell_stages = d.ncells * 2 * args.steps
+    cell_stages = d.ncells * nst * args.steps
     value = cell_stages / wall / 1e9
     res = dict(cfg=cfg, dims=d, wall=wall, ev_ms=ev_ms, value=value, launches=launches, kernel_ms=kms,
                kernel=kname, fused=fused_ok and args.path == "fused", phases=phases)
@@ -227,7 +232,7 @@ def main():
     log("state generated in %.1fs (%d^3, nvt=%d)" % (time.time() - t0, args.size, Dims(cfg0).nvt))
     forms = [f for f in args.forms.split(",") if f]
     weights, train_info = None, None
-    if any(f.endswith("learned") for f in forms):
+    if any("learned" in f for f in forms):
         if args.load_weights:
             weights = np.load(args.load_weights, allow_pickle=False).astype(np.float32)
             train_info = {"loaded": os.path.basename(args.load_weights)}
@@ -264,8 +269,8 @@ def main():
                      "viscous_rk_kernel", "ceq_kernel")
             roof["phases_ms"] = {n: round(v / r["launches"], 4) for n, v in zip(names, ph)}
             ns = r["cfg"].ns
-            kern = {2: "k_adv3d_lds<%d, false>" % ns, 3: "k_vcr3d_ring<%d, false, false>" % ns,
-                    4: "k_ceq3d<false>"}
+            kern = {2: "k_adv3d_lds<%d, false>" % ns, 3: "k_vcr3d_ring<%d, false, false, false, false>" % ns,
+                    4: "k_ceq3d<false, false>"}
             top = max(kern, key=lambda q: ph[q])
             roof["dominant_kernel"] = {"name": kern[top], "avg_ms": round(ph[top] / r["launches"], 4),
                                        "share_of_stage": round(ph[top] / r["kernel_ms"], 3)}
@@ -298,17 +303,21 @@ def main():
     }
     if "copy_gbs" in r:
         out["copy_kernel_gbs"] = round(r["copy_gbs"], 1)
-    ref = r.get("state")
     fm = {}
     for f, q in results.items():
+        # RK2 forms are compared with fp64, SSP-RK3 forms (suffix _rk3) with fp64_rk3
+        rk3 = f.endswith("_rk3")
+        r0 = results.get("fp64_rk3" if rk3 else "fp64")
         stage_ms = q["kernel_ms"] / max(q["launches"], 1)
         b_form = 20 * q["dims"].nvt
         e = {"value": round(q["value"], 4), "ms_per_step": round(q["wall"] * 1e3 / args.steps, 4),
-             "speedup_vs_fp64": round(q["value"] / r["value"], 3), "stage_ms": round(stage_ms, 4),
+             "integrator": "ssp-rk3" if rk3 else "rk2-midpoint", "stage_ms": round(stage_ms, 4),
              "algorithmic_bytes_per_cell_stage": b_form,
              "hbm_frac": round(d.ncells * b_form / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        if f != "fp64" and ref is not None and "state" in q:
-            e["error_vs_fp64"] = form_errors(q["cfg"], q["state"], ref, r["cfg"])
+        if r0 is not None:
+            e["speedup_vs_fp64"] = round(q["value"] / r0["value"], 3)
+        if r0 is not None and r0 is not q and r0.get("state") is not None and "state" in q:
+            e["error_vs_fp64"] = form_errors(q["cfg"], q["state"], r0["state"], r0["cfg"])
         if "learned" in q:
             e["learned"] = q["learned"]
         if len(q.get("phases") or []) == 5 and q["launches"]:
