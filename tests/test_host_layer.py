@@ -44,11 +44,12 @@ def _run_driver(tmp_path, cfg, var, nsteps, mode):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("ndim,mode", [(3, "rk"), (3, "fused"), (2, "rk")])
-def test_driver_steps_equal_oracle(tmp_path, ndim, mode):
+@pytest.mark.parametrize("ndim,mode,rk", [(3, "rk", 2), (3, "fused", 2), (2, "rk", 2), (3, "rk", 3), (3, "fused", 3)])
+def test_driver_steps_equal_oracle(tmp_path, ndim, mode, rk):
     import oracle
-    print("case: fiesta_run %dD %s" % (ndim, mode), flush=True)
+    print("case: fiesta_run %dD %s rk%d" % (ndim, mode, rk), flush=True)
     cfg = cfg3d(nci=24, ncj=13, nck=10) if ndim == 3 else cfg2d(nci=40, ncj=12)
+    cfg.rk_order = rk
     var = W.random_state(cfg, 5)
     got = _run_driver(tmp_path, cfg, var, 3, mode)
     o = oracle.Oracle(cfg)
@@ -56,10 +57,9 @@ def test_driver_steps_equal_oracle(tmp_path, ndim, mode):
     o.pre_sim()
     for _ in range(3):
         o.pre_step()
-        o.compute()
-        o.rk_stage(1)
-        o.compute()
-        o.rk_stage(2)
+        for s in range(1, rk + 1):
+            o.compute()
+            o.rk_stage(s)
         o.post_step()
     want = o.get(FST_VAR)
     d = Dims(cfg)
