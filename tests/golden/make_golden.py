@@ -15,11 +15,15 @@ tests check that the oracle still reproduces them.
 
 Run from the repository root after building the oracle:
     python3 tests/golden/make_golden.py
+The files are written with fixed zip timestamps: an unchanged oracle
+reproduces them byte for byte.
 """
 import hashlib
+import io
 import json
 import os
 import sys
+import zipfile
 
 import numpy as np
 
@@ -79,6 +83,8 @@ CASES = {
     "gen3d_terrain_ns2": (_gen(3, (10, 8, 6), 2), 107, 3,
                           "gen3d on a terrain-following, vertically stretched grid"),
     "gen2d_terrain_ns2": (_gen(2, (12, 10, 1), 2), 108, 3, "gen2d on a terrain-following grid"),
+    "cart3d_rt_ns2_ssp_rk3": (_cart3d(rk=3), 109, 3,
+                              "as cart3d_rt_ns2 with the SSP-RK3 option (rk_order = 3) in place of RK2"),
 }
 
 
@@ -122,6 +128,18 @@ def run_oracle(name, var_in, grid):
     return out
 
 
+def write_npz(path, arrays):
+    """An .npz (numpy's zip of .npy members, deflated) with fixed member
+    timestamps, so that the same arrays give the same bytes."""
+    with zipfile.ZipFile(path, "w", compression=zipfile.ZIP_DEFLATED) as z:
+        for name in sorted(arrays):
+            buf = io.BytesIO()
+            np.lib.format.write_array(buf, np.asanyarray(arrays[name]), allow_pickle=False)
+            info = zipfile.ZipInfo(name + ".npy", date_time=(1980, 1, 1, 0, 0, 0))
+            info.compress_type = zipfile.ZIP_DEFLATED
+            z.writestr(info, buf.getvalue())
+
+
 def sha256(path):
     h = hashlib.sha256()
     with open(path, "rb") as f:
@@ -141,7 +159,7 @@ def main():
         for k, a in arrays.items():
             assert np.isfinite(a).all(), (name, k)
         path = os.path.join(HERE, name + ".npz")
-        np.savez_compressed(path, config=np.array(json.dumps(kw, sort_keys=True)), **arrays)
+        write_npz(path, dict(config=np.array(json.dumps(kw, sort_keys=True)), **arrays))
         manifest["cases"][name] = {"file": name + ".npz", "sha256": sha256(path), "seed": seed, "steps": nsteps,
                                    "description": desc, "arrays": sorted(arrays)}
         print("%-24s %8d bytes" % (name, os.path.getsize(path)))
