@@ -245,9 +245,7 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
   if (ok && c.learned_c) {
     if (!fused_supported(c)) {
       fprintf(stderr, "fst_create: the learned C-indicator needs the fused 3D path (ns <= 3, FP64 or FP32_BULK)\n");
-      free_all(ctx);
-      if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-      delete ctx;
+      (void)fst_destroy(ctx);
       return FST_ERR_ARG;
     }
     ok = learned_alloc(g, ctx->lb) == FST_OK;
@@ -257,9 +255,7 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
        hipMemset(ctx->b.sc, 0, sizeof(Scalars)) == hipSuccess;
   if (!ok) {
     fprintf(stderr, "fst_create: device allocation failed\n");
-    free_all(ctx);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    (void)fst_destroy(ctx);
     return FST_ERR_NOMEM;
   }
   ctx->fused = fused_supported(c) ? 1 : 0;
