@@ -75,6 +75,10 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
     *why = "advection scheme: 3D supports weno5; 2D weno5, centered4, quick";
     return FST_ERR_ARG;
   }
+  if (c.noise && c.n_nt < 0) {
+    *why = "noise.nt must be >= 0";
+    return FST_ERR_ARG;
+  }
   if (c.rk_order != 0 && c.rk_order != 2 && c.rk_order != 3) {
     *why = "rk_order: 2 (RK2 midpoint, the reference's rkAdvance; 0 means 2) or 3 (SSP-RK3)";
     return FST_ERR_ARG;
@@ -92,6 +96,10 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
   d->ncells = (int64_t)c.nci * c.ncj * (c.ndim == 3 ? c.nck : 1);
   return FST_OK;
 }
+
+// The noise filter runs in the cartesian modules' and gen2d's postStep
+// (src/cart3d.cpp:319, src/cart2d.cpp:257, src/gen2d.cpp:150); gen3d has none.
+bool noise_on(const fst_config &c) { return c.noise && !(c.grid == 1 && c.ndim == 3); }
 
 // grid nodes (ni*nj*nk): one more than cells per direction, nk = 1 in 2D
 // (src/input.cpp:401-406)
@@ -124,9 +132,11 @@ double *array_ptr(fst_ctx *ctx, int a, size_t *count) {
 void free_all(fst_ctx *ctx) {
   double **ps[] = {&ctx->b.var, &ctx->b.tmp1, &ctx->b.dvar, &ctx->b.varx, &ctx->b.p, &ctx->b.rho, &ctx->b.T,
                    &ctx->b.vel, &ctx->b.fx, &ctx->b.fy, &ctx->b.fz, &ctx->b.stress, &ctx->b.gradRho,
-                   &ctx->b.mflux, &ctx->b.grid, &ctx->b.metrics};
+                   &ctx->b.mflux, &ctx->b.grid, &ctx->b.metrics, &ctx->b.ndn};
   for (auto pp : ps)
     if (*pp) { (void)hipFree(*pp); *pp = nullptr; }
+  if (ctx->b.nflag) { (void)hipFree(ctx->b.nflag); ctx->b.nflag = nullptr; }
+  if (ctx->b.nmax) { (void)hipFree(ctx->b.nmax); ctx->b.nmax = nullptr; }
   if (ctx->b.sc) { (void)hipFree(ctx->b.sc); ctx->b.sc = nullptr; }
   fused_free(ctx->fb);
   learned_free(ctx->lb);
@@ -242,6 +252,13 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
     const size_t gbytes = grid_nodes(g) * 3 * sizeof(double);
     ok = ok && hipMalloc((void **)&ctx->b.grid, gbytes) == hipSuccess && hipMemset(ctx->b.grid, 0, gbytes) == hipSuccess;
   }
+  if (ok && noise_on(c)) {  // noise filter buffers (src/cart3d.cpp:91-93, src/cart2d.cpp:64-66)
+    ok = alloc(&ctx->b.ndn, 1);
+    const size_t nf = (size_t)std::max(noise_flag_count(g, c.grid == 1), 1);
+    ok = ok && hipMalloc((void **)&ctx->b.nflag, nf) == hipSuccess &&
+         hipMalloc((void **)&ctx->b.nmax, sizeof(unsigned long long)) == hipSuccess &&
+         hipMemset(ctx->b.nmax, 0, sizeof(unsigned long long)) == hipSuccess;
+  }
   if (ok && c.learned_c) {
     if (!fused_supported(c)) {
       fprintf(stderr, "fst_create: the learned C-indicator needs the fused 3D path (ns <= 3, FP64 or FP32_BULK)\n");
@@ -351,6 +368,10 @@ int fst_rk_stage(fst_ctx *ctx, int stage) {
 int fst_post_step(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
   launch_secondary(ctx->g, ctx->ph, ctx->b, ctx->stream);
+  if (noise_on(ctx->cf)) {
+    ctx->fb.valid = false;  // var changes
+    launch_noise(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->stream);
+  }
   return check_launch(ctx, "fst_post_step");
 }
 

@@ -59,6 +59,11 @@ struct Bufs {
   double *gradRho = nullptr;
   double *mflux = nullptr;   // 3D: 27 planes ((face*3+dir)*3+w); 2D: 8 planes
   double *grid = nullptr, *metrics = nullptr;
+  // noise filter (cf.noise): smoothing increments (1 plane), coarse-point
+  // flags, and the maximum of the C variable that sets the cut-off
+  double *ndn = nullptr;
+  unsigned char *nflag = nullptr;
+  unsigned long long *nmax = nullptr;
   Scalars *sc = nullptr;     // per-stage maxima (device)
   Prof *prof = nullptr;      // timing of the dominant kernel (host side)
 };
@@ -124,6 +129,9 @@ inline int rk_stages(const fst_config &c) { return c.rk_order == 3 ? 3 : 2; }
 constexpr double RK3_A2 = 0.75, RK3_B2 = 0.25, RK3_A3 = 1.0 / 3.0, RK3_B3 = 2.0 / 3.0;
 void launch_rk_stage(const Geo &g, int nstages, int stage, double dt, double *var, double *tmp, const double *dvar,
                      hipStream_t s);
+// Noise filter of postStep on var (cartesian modules; cf.noise = 1).
+int noise_flag_count(const Geo &g, bool gen);
+void launch_noise(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 void launch_secondary(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
 // generalized grids (gen2d/gen3d modules)

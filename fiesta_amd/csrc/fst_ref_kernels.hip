@@ -803,6 +803,138 @@ __global__ void k_rk_stage(long long n, int nstages, int stage, double dt, doubl
   }
 }
 
+// ------------------------------------------------------------- noise filter
+// postStep's noise filter (src/cart3d.cpp:319-374, src/cart2d.cpp:257-307,
+// detectNoise/removeNoise src/noise.hpp:26-261) with an order-free definition
+// of its overlapping block writes and in-place sweeps: a cell is flagged when
+// any coarse point whose 3x3(x3) block covers it flags it (and, with coff > 0,
+// its C value is below coff), and each sweep reads the values from before it.
+
+// coarse points per direction: 0..noise_half(nc), src/cart3d.cpp:325-341
+__host__ __device__ __forceinline__ int noise_half(int nc) { return (nc + 1) % 2 == 0 ? (nc + 1) / 2 : nc / 2; }
+
+// detectNoise3D's stencil in the reference's summation order (noise.hpp:63-82):
+// (di, dj, dk, coefficient)
+__constant__ signed char NOISE3D_ST[27][4] = {
+    {-1, 1, 1, 2}, {0, 1, 1, 2}, {1, 1, 1, 2}, {-1, 0, 1, 2}, {0, 0, 1, 8}, {1, 0, 1, 2},
+    {-1, -1, 1, 2}, {0, -1, 1, 2}, {1, -1, 1, 2}, {-1, 1, 0, 2}, {0, 1, 0, 8}, {1, 1, 0, 2},
+    {-1, 0, 0, 8}, {0, 0, 0, -88}, {1, 0, 0, 8}, {-1, -1, 0, 2}, {0, -1, 0, 8}, {1, -1, 0, 2},
+    {-1, 1, -1, 2}, {0, 1, -1, 2}, {1, 1, -1, 2}, {-1, 0, -1, 2}, {0, 0, -1, 8}, {1, 0, -1, 2},
+    {-1, -1, -1, 2}, {0, -1, -1, 2}, {1, -1, -1, 2}};
+
+__global__ void k_fill_u64(unsigned long long *a, int n, unsigned long long v) {
+  if ((int)threadIdx.x < n) a[threadIdx.x] = v;
+}
+
+// max |var(.., cv)| over the cells (maxCvar3D(var, 6), src/cart3d.cpp:343-352;
+// maxCvar2D(var, 1) = C_hat, src/cart2d.cpp:274-283), one rank
+__global__ void k_noise_cmax(Geo g, const double *__restrict__ var, int cv, unsigned long long *out) {
+  const long long n = (long long)g.nci * g.ncj * g.nck;
+  double m = 0.0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % g.nci) + g.ng, j = (int)((t / g.nci) % g.ncj) + g.ng;
+    const int k = g.ndim == 3 ? (int)(t / ((long long)g.nci * g.ncj)) + g.ng : 0;
+    m = fmax(m, fabs(var[g.id(i, j, k) + g.N * cv]));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0.0) atomic_max_nonneg(out, m);
+}
+
+// detectNoise3D / detectNoise2D's test on each coarse point (ii, jj, kk)
+// px, py, pz: coarse points per direction (pz = 1 in 2D)
+__global__ void k_noise_flags(Geo g, Phys ph, const fst_config cf, const double *__restrict__ var, int v, int px,
+                              int py, int pz, unsigned char *__restrict__ flag) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)px * py * pz) return;
+  const int ii = (int)(t % px), jj = (int)((t / px) % py), kk = (int)(t / ((long long)px * py));
+  const int i = 2 * ii + g.ng, j = 2 * jj + g.ng;
+  const double dx = ph.cd[1], dy = ph.cd[2], dz = ph.cd[3];
+  const double *U = var + g.N * v;
+  bool f;
+  if (g.ndim == 3) {
+    const int k = 2 * kk + g.ng;
+    const double a = sqrt(165.0 * dx * dy * dz) / 15840.0;
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const double term =
+          (double)NOISE3D_ST[q][3] * U[g.id(i + NOISE3D_ST[q][0], j + NOISE3D_ST[q][1], k + NOISE3D_ST[q][2])];
+      sum = q == 0 ? term : sum + term;
+    }
+    const double c = -a * sum;
+    const double cref = cf.n_dh * sqrt(dx * dy * dz / 12.0);
+    f = fabs(c) >= cref;
+  } else {
+    auto W = [&](int a, int b) { return U[g.id(a, b, 0)]; };
+    const double a = sqrt(6.0 * dx * dy);
+    const double c = -(a / 192.0) * (W(i - 1, j + 1) + 2 * W(i, j + 1) + W(i + 1, j + 1) + 2 * W(i - 1, j) -
+                                     12 * W(i, j) + 2 * W(i + 1, j) + W(i - 1, j - 1) + 2 * W(i, j - 1) +
+                                     W(i + 1, j - 1));
+    const double cref = cf.n_dh * a / 16.0;
+    f = fabs(c) >= cref;
+  }
+  flag[t] = f ? 1 : 0;
+}
+
+// removeNoise3D / removeNoise2D increments of one sweep on the cells: dn
+__global__ void k_noise_dn(Geo g, Phys ph, const fst_config cf, const double *__restrict__ var, int v, int px,
+                           int py, int pz, const unsigned char *__restrict__ flag, const unsigned long long *cmax,
+                           int use_coff, double *__restrict__ dn) {
+  const long long n = (long long)g.nci * g.ncj * g.nck;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.nci) + g.ng, j = (int)((t / g.nci) % g.ncj) + g.ng;
+  const int k = g.ndim == 3 ? (int)(t / ((long long)g.nci * g.ncj)) + g.ng : 0;
+  const int ng = g.ng;
+  // the coarse points covering (i, j, k): point q sits at ng + 2q and covers
+  // ng + 2q - 1 .. ng + 2q + 1, so per axis those of x-1, x, x+1 on the lattice
+  bool cov = false;
+  for (int ci = i - 1; ci <= i + 1; ++ci) {
+    const int di = ci - ng;
+    if (di < 0 || (di & 1) || di / 2 >= px) continue;
+    for (int cj = j - 1; cj <= j + 1; ++cj) {
+      const int dj = cj - ng;
+      if (dj < 0 || (dj & 1) || dj / 2 >= py) continue;
+      for (int ck = (g.ndim == 3 ? k - 1 : 0); ck <= (g.ndim == 3 ? k + 1 : 0); ++ck) {
+        const int dk = g.ndim == 3 ? ck - ng : 0;
+        if (dk < 0 || (dk & 1) || dk / 2 >= pz) continue;
+        cov = cov || flag[di / 2 + (long long)px * (dj / 2 + (long long)py * (dk / 2))] != 0;
+      }
+    }
+  }
+  const double coff = use_coff ? cf.n_coff * as_double(*cmax) : 0.0;
+  const double dx = ph.cd[1], dy = ph.cd[2], dz = ph.cd[3];
+  const double *U = var + g.N * v;
+  const long long c = g.id(i, j, k);
+  if (g.ndim == 3) {
+    const double ch = fabs(var[c + g.N * (g.nv + 1)]);
+    const int noise = cov && (coff == 0.0 || ch < coff) ? 1 : 0;
+    const long long sj = g.ngi, sk = (long long)g.ngi * g.ngj;
+    const double lap = (U[c - 1] - 2 * U[c] + U[c + 1]) / (dx * dx) + (U[c - sj] - 2 * U[c] + U[c + sj]) / (dy * dy) +
+                       (U[c - sk] - 2 * U[c] + U[c + sk]) / (dz * dz);
+    dn[c] = cf.n_eta * (dx * dx + dy * dy + dz * dz) * noise * lap;
+  } else {
+    const double ch = var[c + g.N * (g.nv + 1)];  // no abs in the reference (noise.hpp:222)
+    const int noise = cov && (coff == 0.0 || ch < coff) ? 1 : 0;
+    const long long sj = g.ngi;
+    const double lgrad = (U[c] - U[c - 1]) / dx;
+    const double rgrad = (U[c + 1] - U[c]) / dx;
+    const double bgrad = (U[c] - U[c - sj]) / dy;
+    const double tgrad = (U[c + sj] - U[c]) / dy;
+    dn[c] = cf.n_eta * (dx * dx + dy * dy) * noise * ((rgrad - lgrad) / dx + (tgrad - bgrad) / dy);
+  }
+}
+
+__global__ void k_noise_add(Geo g, double *__restrict__ var, int v, const double *__restrict__ dn) {
+  const long long n = (long long)g.nci * g.ncj * g.nck;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.nci) + g.ng, j = (int)((t / g.nci) % g.ncj) + g.ng;
+  const int k = g.ndim == 3 ? (int)(t / ((long long)g.nci * g.ncj)) + g.ng : 0;
+  const long long c = g.id(i, j, k);
+  var[c + g.N * v] += dn[c];
+}
+
 // plain copy, 16 bytes per lane per iteration (bandwidth probe)
 __global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long long n) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1134,6 +1266,58 @@ void launch_rk_stage(const Geo &g, int nstages, int stage, double dt, double *va
                      hipStream_t s) {
   long long n = g.N * g.nvt;
   k_rk_stage<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, nstages, stage, dt, var, tmp, dvar);
+}
+
+// Coarse points per direction: cartesian modules 0..noise_half(nc)
+// (src/cart3d.cpp:325-341, src/cart2d.cpp:261-272); gen2d 0..M-1 with
+// M = (nc-1)/2 or nc/2 (src/gen2d.cpp:155-164).
+void noise_points(const Geo &g, bool gen, int p[3]) {
+  const int nc[3] = {g.nci, g.ncj, g.nck};
+  for (int a = 0; a < 3; ++a) {
+    if (a == 2 && g.ndim == 2) { p[a] = 1; continue; }
+    p[a] = gen ? ((nc[a] - 1) % 2 == 0 ? (nc[a] - 1) / 2 : nc[a] / 2) : noise_half(nc[a]) + 1;
+  }
+}
+
+int noise_flag_count(const Geo &g, bool gen) {
+  int p[3];
+  noise_points(g, gen, p);
+  return p[0] * p[1] * p[2];
+}
+
+void launch_noise(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s) {
+  const bool gen = cf.grid == 1;
+  const long long ncell = (long long)g.nci * g.ncj * g.nck;
+  const unsigned nb = (unsigned)((ncell + 255) / 256);
+  // the C_hat cut-off: cartesian modules with the C-equation on; gen2d uses 0
+  const bool use_coff = !gen && cf.ceq;
+  if (use_coff) {
+    k_fill_u64<<<1, 64, 0, s>>>(b.nmax, 1, 0ull);
+    k_noise_cmax<<<std::min(nb, 1024u), 256, 0, s>>>(g, b.var, g.ndim == 3 ? 6 : g.nv + 1, b.nmax);
+  }
+  int p[3];
+  noise_points(g, gen, p);
+  const int nf = p[0] * p[1] * p[2];
+  // variables: mode 1 the momentum components, else the energy (cartesian);
+  // gen2d always the two momentum components
+  int vars[3], nvars;
+  if (gen || cf.n_mode == 1) {
+    nvars = g.ndim;
+    for (int q = 0; q < nvars; ++q) vars[q] = q;
+  } else {
+    nvars = 1;
+    vars[0] = g.ndim;
+  }
+  if (nf == 0 || ncell == 0) return;
+  for (int q = 0; q < nvars; ++q) {
+    k_noise_flags<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(g, ph, cf, b.var, vars[q], p[0], p[1], p[2],
+                                                                b.nflag);
+    for (int tau = 0; tau < cf.n_nt; ++tau) {
+      k_noise_dn<<<nb, 256, 0, s>>>(g, ph, cf, b.var, vars[q], p[0], p[1], p[2], b.nflag, b.nmax,
+                                    use_coff ? 1 : 0, b.ndn);
+      k_noise_add<<<nb, 256, 0, s>>>(g, b.var, vars[q], b.ndn);
+    }
+  }
 }
 
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {

@@ -33,7 +33,7 @@ extern "C" {
 #endif
 
 #define FST_MAX_SPECIES 8
-#define FST_ABI_VERSION 3
+#define FST_ABI_VERSION 4
 
 /* Boundary types, in the order of `enum class BCType` (src/bc.hpp:31). */
 enum fst_bc_type {
@@ -110,7 +110,14 @@ typedef struct fst_config {
   int32_t rk_order;      /* 2 (or 0): the reference's RK2 midpoint (rkAdvance);
                             3: three-stage SSP-RK3 (Shu-Osher), an option the
                             reference does not have                          */
-  int32_t reserved[7];
+  int32_t noise;         /* noise.enabled: the noise filter of postStep
+                            (cartesian modules, src/noise.hpp)               */
+  int32_t n_nt;          /* noise.nt, smoothing sweeps per variable (1)      */
+  int32_t n_mode;        /* noise.mode: 1 momentum components, else energy   */
+  double n_dh;           /* noise.dh, detection threshold                     */
+  double n_eta;          /* noise.eta, smoothing coefficient                  */
+  double n_coff;         /* noise.coff, C_hat cut-off factor (ceq on)         */
+  int32_t reserved[4];
 } fst_config;
 
 typedef struct fst_ctx fst_ctx;
@@ -163,7 +170,14 @@ int fst_compute(fst_ctx *ctx);
  *   2: var = 0.75*tmp1 + 0.25*(var + dt*dvar)
  *   3: var = (1.0/3.0)*tmp1 + (2.0/3.0)*(var + dt*dvar)                  */
 int fst_rk_stage(fst_ctx *ctx, int stage);
-/* rk_func::postStep: secondary variables varx (src/cart3d.cpp:300-317). */
+/* rk_func::postStep: secondary variables varx (src/cart3d.cpp:300-317,
+ * src/cart2d.cpp:239-255), then, with noise = 1, the noise filter
+ * (src/cart3d.cpp:319-374, src/cart2d.cpp:257-307, src/noise.hpp).  Its
+ * detection writes overlapping 3x3(x3) blocks and its smoothing updates var in
+ * place inside parallel loops, so the reference's result depends on the loop
+ * order; this build defines it order-free: a cell is flagged when any block
+ * covering it flags it, and each smoothing sweep reads the values from before
+ * the sweep. */
 int fst_post_step(fst_ctx *ctx);
 
 /* `nsteps` full time steps: Fiesta::step (src/fiesta.cpp:333-340) =

@@ -1401,6 +1401,164 @@ void secondary(Orc &o) {
   }
 }
 
+// ---------------------------------------------------------------- noise filter
+// postStep's noise filter, src/cart3d.cpp:319-374 / src/cart2d.cpp:257-307
+// with detectNoise3D/2D and removeNoise3D/2D (src/noise.hpp:26-261).  The
+// reference writes the overlapping 3x3(x3) blocks of neighbouring coarse points
+// and updates var in place inside parallel loops, so its result depends on the
+// loop order.  Defined here without order: a cell is flagged when any coarse
+// point whose block covers it flags it (and, with coff > 0, its |C_hat| is
+// below coff), and each smoothing sweep reads the values from before the sweep.
+
+// coarse points per direction: 0..noise_half(nc), src/cart3d.cpp:325-341
+int noise_half(int nc) { return (nc + 1) % 2 == 0 ? (nc + 1) / 2 : nc / 2; }
+
+// detectNoise3D's stencil, in the reference's summation order (noise.hpp:63-82)
+const int NOISE3D_ST[27][4] = {
+    {-1, 1, 1, 2}, {0, 1, 1, 2}, {1, 1, 1, 2}, {-1, 0, 1, 2}, {0, 0, 1, 8}, {1, 0, 1, 2},
+    {-1, -1, 1, 2}, {0, -1, 1, 2}, {1, -1, 1, 2}, {-1, 1, 0, 2}, {0, 1, 0, 8}, {1, 1, 0, 2},
+    {-1, 0, 0, 8}, {0, 0, 0, -88}, {1, 0, 0, 8}, {-1, -1, 0, 2}, {0, -1, 0, 8}, {1, -1, 0, 2},
+    {-1, 1, -1, 2}, {0, 1, -1, 2}, {1, 1, -1, 2}, {-1, 0, -1, 2}, {0, 0, -1, 8}, {1, 0, -1, 2},
+    {-1, -1, -1, 2}, {0, -1, -1, 2}, {1, -1, -1, 2}};
+
+// Is some coarse point covering cell x along one axis flagged?  Coarse point
+// q sits at ng + 2q (0 <= q <= qmax) and covers ng + 2q - 1 .. ng + 2q + 1.
+template <typename F>
+bool any_cover(int x, int ng, int qmax, F &&f) {
+  for (int c = x - 1; c <= x + 1; ++c) {
+    const int d = c - ng;
+    if (d < 0 || (d & 1) || d / 2 > qmax) continue;
+    if (f(d / 2)) return true;
+  }
+  return false;
+}
+
+void noise3d(Orc &o) {
+  const int ng = o.ng, M = noise_half(o.nci), Nn = noise_half(o.ncj), L = noise_half(o.nck);
+  const double dx = o.cd[1], dy = o.cd[2], dz = o.cd[3];
+  double coff = 0.0;
+  if (o.c.ceq) {  // maxCvar3D(var, 6) over the cells (src/cart3d.cpp:343-352), one rank
+    double mx = 0.0;
+    for (int k = ng; k < o.ngk - ng; ++k)
+      for (int j = ng; j < o.ngj - ng; ++j)
+        for (int i = ng; i < o.ngi - ng; ++i) {
+          const double a = std::abs(o.V(o.var, i, j, k, 6));
+          if (a > mx) mx = a;
+        }
+    coff = o.c.n_coff * mx;
+  }
+  std::vector<int> vars;
+  if (o.c.n_mode == 1) vars = {0, 1, 2};
+  else vars = {3};
+  const int nv = o.nv;
+  std::vector<char> flag((size_t)(M + 1) * (Nn + 1) * (L + 1));
+  std::vector<double> dn(o.N, 0.0);
+  for (int v : vars) {
+    for (int kk = 0; kk <= L; ++kk)
+      for (int jj = 0; jj <= Nn; ++jj)
+        for (int ii = 0; ii <= M; ++ii) {
+          const int i = 2 * ii + ng, j = 2 * jj + ng, k = 2 * kk + ng;
+          const double a = std::sqrt(165.0 * dx * dy * dz) / 15840.0;
+          double sum = 0.0;
+          for (int t = 0; t < 27; ++t) {
+            const double term = (double)NOISE3D_ST[t][3] * o.V(o.var, i + NOISE3D_ST[t][0], j + NOISE3D_ST[t][1],
+                                                             k + NOISE3D_ST[t][2], v);
+            sum = t == 0 ? term : sum + term;
+          }
+          const double c = -a * sum;
+          const double cref = o.c.n_dh * std::sqrt(dx * dy * dz / 12.0);
+          flag[(size_t)ii + (size_t)(M + 1) * ((size_t)jj + (size_t)(Nn + 1) * kk)] = std::abs(c) >= cref;
+        }
+    for (int tau = 0; tau < o.c.n_nt; ++tau) {
+      for (int k = ng; k < o.ngk - ng; ++k)
+        for (int j = ng; j < o.ngj - ng; ++j)
+          for (int i = ng; i < o.ngi - ng; ++i) {
+            const bool cov = any_cover(i, ng, M, [&](int ii) {
+              return any_cover(j, ng, Nn, [&](int jj) {
+                return any_cover(k, ng, L, [&](int kk) {
+                  return flag[(size_t)ii + (size_t)(M + 1) * ((size_t)jj + (size_t)(Nn + 1) * kk)] != 0;
+                });
+              });
+            });
+            const int noise = cov && (coff == 0.0 || std::abs(o.V(o.var, i, j, k, nv + 1)) < coff) ? 1 : 0;
+            auto U = [&](int a, int b, int c) { return o.V(o.var, a, b, c, v); };
+            const double lap = (U(i - 1, j, k) - 2 * U(i, j, k) + U(i + 1, j, k)) / (dx * dx) +
+                               (U(i, j - 1, k) - 2 * U(i, j, k) + U(i, j + 1, k)) / (dy * dy) +
+                               (U(i, j, k - 1) - 2 * U(i, j, k) + U(i, j, k + 1)) / (dz * dz);
+            dn[o.id(i, j, k)] = o.c.n_eta * (dx * dx + dy * dy + dz * dz) * noise * lap;
+          }
+      for (int k = ng; k < o.ngk - ng; ++k)
+        for (int j = ng; j < o.ngj - ng; ++j)
+          for (int i = ng; i < o.ngi - ng; ++i) o.V(o.var, i, j, k, v) += dn[o.id(i, j, k)];
+    }
+  }
+}
+
+// cart2d (src/cart2d.cpp:257-307) and gen2d (src/gen2d.cpp:150-177: coarse
+// points 0..M-1 with M = (nc-1)/2 or nc/2, coff = 0, both momentum components)
+void noise2d(Orc &o) {
+  const int ng = o.ng;
+  auto gen_pts = [](int nc) { return (nc - 1) % 2 == 0 ? (nc - 1) / 2 : nc / 2; };
+  // M, Nn: the largest coarse index per direction
+  const int M = o.gen ? gen_pts(o.nci) - 1 : noise_half(o.nci), Nn = o.gen ? gen_pts(o.ncj) - 1 : noise_half(o.ncj);
+  const double dx = o.cd[1], dy = o.cd[2];
+  double coff = 0.0;
+  if (o.c.ceq && !o.gen) {  // maxCvar2D(var, 1, cd): |C_hat| over the cells (src/cart2d.cpp:274-283)
+    double mx = 0.0;
+    for (int j = ng; j < o.ngj - ng; ++j)
+      for (int i = ng; i < o.ngi - ng; ++i) {
+        const double a = std::abs(o.V(o.var, i, j, 0, o.nv + 1));
+        if (a > mx) mx = a;
+      }
+    coff = o.c.n_coff * mx;
+  }
+  std::vector<int> vars;
+  if (o.c.n_mode == 1 || o.gen) vars = {0, 1};
+  else vars = {2};
+  if (M < 0 || Nn < 0) return;  // no coarse points
+  std::vector<char> flag((size_t)(M + 1) * (Nn + 1));
+  std::vector<double> dn(o.N, 0.0);
+  for (int v : vars) {
+    auto U = [&](int a, int b) { return o.V(o.var, a, b, 0, v); };
+    for (int jj = 0; jj <= Nn; ++jj)
+      for (int ii = 0; ii <= M; ++ii) {
+        const int i = 2 * ii + ng, j = 2 * jj + ng;
+        const double a = std::sqrt(6.0 * dx * dy);
+        const double c = -(a / 192.0) * (U(i - 1, j + 1) + 2 * U(i, j + 1) + U(i + 1, j + 1) + 2 * U(i - 1, j) -
+                                         12 * U(i, j) + 2 * U(i + 1, j) + U(i - 1, j - 1) + 2 * U(i, j - 1) +
+                                         U(i + 1, j - 1));
+        const double cref = o.c.n_dh * a / 16.0;
+        flag[(size_t)ii + (size_t)(M + 1) * jj] = std::abs(c) >= cref;
+      }
+    for (int tau = 0; tau < o.c.n_nt; ++tau) {
+      for (int j = ng; j < o.ngj - ng; ++j)
+        for (int i = ng; i < o.ngi - ng; ++i) {
+          const bool cov = any_cover(i, ng, M, [&](int ii) {
+            return any_cover(j, ng, Nn, [&](int jj) { return flag[(size_t)ii + (size_t)(M + 1) * jj] != 0; });
+          });
+          // no abs here in the reference (noise.hpp:222)
+          const int noise = cov && (coff == 0.0 || o.V(o.var, i, j, 0, o.nv + 1) < coff) ? 1 : 0;
+          const double lgrad = (U(i, j) - U(i - 1, j)) / dx;
+          const double rgrad = (U(i + 1, j) - U(i, j)) / dx;
+          const double bgrad = (U(i, j) - U(i, j - 1)) / dy;
+          const double tgrad = (U(i, j + 1) - U(i, j)) / dy;
+          dn[o.id(i, j, 0)] = o.c.n_eta * (dx * dx + dy * dy) * noise * ((rgrad - lgrad) / dx + (tgrad - bgrad) / dy);
+        }
+      for (int j = ng; j < o.ngj - ng; ++j)
+        for (int i = ng; i < o.ngi - ng; ++i) o.V(o.var, i, j, 0, v) += dn[o.id(i, j, 0)];
+    }
+  }
+}
+
+// rk_func::postStep: secondary variables, then the noise filter (cartesian modules)
+void post_step(Orc &o) {
+  secondary(o);
+  if (o.c.noise && !(o.gen && o.ndim == 3)) {  // gen3d has no noise filter
+    if (o.ndim == 3) noise3d(o);
+    else noise2d(o);
+  }
+}
+
 void pre_sim(Orc &o) {
   if (o.gen) {  // gen3d_func::preSim (src/gen3d.cpp:131-304), gen2d_func::preSim (src/gen2d.cpp:187-331)
     if (o.ndim == 3) metrics3d(o);
@@ -1537,7 +1695,7 @@ void orc_compute(void *h) { compute(*(Orc *)h); }
 void orc_rk_stage(void *h, int stage) { rk_stage(*(Orc *)h, stage); }
 void orc_pre_sim(void *h) { pre_sim(*(Orc *)h); }
 void orc_pre_step(void *h) { pre_step(*(Orc *)h); }
-void orc_post_step(void *h) { secondary(*(Orc *)h); }
+void orc_post_step(void *h) { post_step(*(Orc *)h); }
 
 // Fiesta::step (src/fiesta.cpp:333-340): preStep, rkAdvance, postStep
 void orc_step(void *h, int nsteps) {
