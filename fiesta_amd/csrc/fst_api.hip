@@ -28,6 +28,8 @@ struct fst_ctx {
   Prof prof;
   double phase_ms[Prof::MAXPER] = {0};  // per-phase sums of the last fst_profile_read
   int nphase = 0;
+  void *pack = nullptr;  // device staging of fst_pack
+  size_t pack_bytes = 0;
 };
 
 namespace {
@@ -141,6 +143,7 @@ void free_all(fst_ctx *ctx) {
   fused_free(ctx->fb);
   learned_free(ctx->lb);
   if (ctx->lprim) { (void)hipFree(ctx->lprim); ctx->lprim = nullptr; }
+  if (ctx->pack) { (void)hipFree(ctx->pack); ctx->pack = nullptr; ctx->pack_bytes = 0; }
 }
 
 // rk_func::compute() of the module the config selects: cart3d applies the
@@ -461,6 +464,55 @@ int fst_profile_phases(fst_ctx *ctx, double *ms, int n, int *nphase) {
   if (!ctx || !ms || !nphase || n < 0) return FST_ERR_ARG;
   *nphase = ctx->nphase;
   for (int q = 0; q < n && q < ctx->nphase; ++q) ms[q] = ctx->phase_ms[q];
+  return FST_OK;
+}
+
+int fst_pack(fst_ctx *ctx, int array, int vn, const int64_t start[3], const int64_t end[3], const int64_t stride[3],
+             int average, int out_f32, void *out, size_t count) {
+  if (!ctx || !start || !end || !stride || !out) return FST_ERR_ARG;
+  const Geo &g = ctx->g;
+  const int nd = g.ndim;
+  // the plane of variable vn, its extents and the offset of index 0
+  const double *src = nullptr;
+  long long nx = g.ngi, ny = g.ngj, lim[3] = {g.nci, g.ncj, g.nck};
+  int nvar = 0, off = g.ng;
+  if (array == FST_VAR) { src = ctx->b.var; nvar = g.nvt; }
+  else if (array == FST_VARX) { src = ctx->b.varx; nvar = g.nvx; }
+  else if (array == FST_GRID && ctx->b.grid) {  // nodes (ni, nj, nk, 3)
+    src = ctx->b.grid; nvar = 3; off = 0;
+    nx = g.nci + 1; ny = g.ncj + 1;
+    lim[0] = g.nci + 1; lim[1] = g.ncj + 1; lim[2] = nd == 3 ? g.nck + 1 : 1;
+  }
+  if (!src) return fail(ctx, FST_ERR_ARG, "fst_pack: array not available (FST_VAR, FST_VARX, FST_GRID)");
+  if (vn < 0 || vn >= nvar) return fail(ctx, FST_ERR_ARG, "fst_pack: variable index out of range");
+  const long long plane = array == FST_GRID ? nx * ny * lim[2] : (long long)g.N;
+  PackBox bx;
+  long long n = 1;
+  for (int a = 0; a < 3; ++a) {
+    if (a >= nd) { bx.start[a] = 0; bx.stride[a] = 1; bx.ext[a] = 1; continue; }
+    if (stride[a] < 1 || start[a] < 0 || end[a] < start[a]) return fail(ctx, FST_ERR_ARG, "fst_pack: bad box");
+    bx.start[a] = start[a];
+    bx.stride[a] = stride[a];
+    bx.ext[a] = (end[a] - start[a]) / stride[a] + 1;
+    // every read stays inside the plane: the last point (plus its averaging block)
+    const long long last = start[a] + (bx.ext[a] - 1) * stride[a] + (average ? stride[a] - 1 : 0);
+    const long long room = array == FST_GRID ? lim[a] : lim[a] + g.ng;  // var/varx may reach into the ghosts
+    if (last >= room) return fail(ctx, FST_ERR_ARG, "fst_pack: box leaves the array");
+    n *= bx.ext[a];
+  }
+  if ((size_t)n != count) return fail(ctx, FST_ERR_ARG, "fst_pack: count does not match the packed size");
+  const size_t bytes = (size_t)n * (out_f32 ? sizeof(float) : sizeof(double));
+  if (bytes > ctx->pack_bytes) {
+    if (ctx->pack) (void)hipFree(ctx->pack);
+    ctx->pack = nullptr;
+    ctx->pack_bytes = 0;
+    if (hipMalloc(&ctx->pack, bytes) != hipSuccess) return fail(ctx, FST_ERR_NOMEM, "fst_pack: staging allocation");
+    ctx->pack_bytes = bytes;
+  }
+  launch_pack(src + plane * vn, nx, ny, nd, off, bx, average, out_f32 != 0, ctx->pack, ctx->stream);
+  if (int rc = check_launch(ctx, "fst_pack")) return rc;
+  HIPCHK(ctx, hipMemcpyAsync(out, ctx->pack, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return FST_OK;
 }
 

@@ -129,6 +129,12 @@ inline int rk_stages(const fst_config &c) { return c.rk_order == 3 ? 3 : 2; }
 constexpr double RK3_A2 = 0.75, RK3_B2 = 0.25, RK3_A3 = 1.0 / 3.0, RK3_B3 = 2.0 / 3.0;
 void launch_rk_stage(const Geo &g, int nstages, int stage, double dt, double *var, double *tmp, const double *dvar,
                      hipStream_t s);
+// Output pack box (inclusive start..end, stride, extent per direction).
+struct PackBox {
+  long long start[3], stride[3], ext[3];
+};
+void launch_pack(const double *src, long long nx, long long ny, int ndim, int off, const PackBox &bx, int average,
+                 bool f32, void *dst, hipStream_t s);
 // Noise filter of postStep on var (cartesian modules; cf.noise = 1).
 int noise_flag_count(const Geo &g, bool gen);
 void launch_noise(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);

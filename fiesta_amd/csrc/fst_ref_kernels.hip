@@ -935,6 +935,35 @@ __global__ void k_noise_add(Geo g, double *__restrict__ var, int v, const double
   var[c + g.N * v] += dn[c];
 }
 
+// ------------------------------------------------------------- output pack
+// blockWriter::dataPack (src/block.cpp:404-454): one thread per packed value;
+// the mean sums the stride block in the reference's loop order (ia, ja, ka)
+// and divides by the block size.  src: plane vn of the array, with cell
+// offset `off` (ng for var/varx, 0 for the grid) and extents nx, ny.
+template <typename T>
+__global__ void k_pack(const double *__restrict__ src, long long nx, long long ny, int ndim, int off, PackBox bx,
+                       int average, T *__restrict__ dst) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long e0 = bx.ext[0], e1 = bx.ext[1], e2 = ndim == 3 ? bx.ext[2] : 1;
+  if (t >= e0 * e1 * e2) return;
+  const long long ii = t % e0, jj = (t / e0) % e1, kk = t / (e0 * e1);
+  const long long i = bx.start[0] + ii * bx.stride[0], j = bx.start[1] + jj * bx.stride[1];
+  const long long k = ndim == 3 ? bx.start[2] + kk * bx.stride[2] : 0;
+  auto at = [&](long long a, long long b, long long c) {
+    return src[(a + off) + nx * ((b + off) + ny * (ndim == 3 ? c + off : 0))];
+  };
+  if (average) {
+    double mean = 0.0;
+    const long long s2 = ndim == 3 ? bx.stride[2] : 1;
+    for (long long ia = 0; ia < bx.stride[0]; ++ia)
+      for (long long ja = 0; ja < bx.stride[1]; ++ja)
+        for (long long ka = 0; ka < s2; ++ka) mean += at(i + ia, j + ja, k + ka);
+    dst[t] = (T)(mean / (double)(ndim == 3 ? bx.stride[0] * bx.stride[1] * bx.stride[2] : bx.stride[0] * bx.stride[1]));
+  } else {
+    dst[t] = (T)at(i, j, k);
+  }
+}
+
 // plain copy, 16 bytes per lane per iteration (bandwidth probe)
 __global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long long n) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1318,6 +1347,14 @@ void launch_noise(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs
       k_noise_add<<<nb, 256, 0, s>>>(g, b.var, vars[q], b.ndn);
     }
   }
+}
+
+void launch_pack(const double *src, long long nx, long long ny, int ndim, int off, const PackBox &bx, int average,
+                 bool f32, void *dst, hipStream_t s) {
+  const long long n = bx.ext[0] * bx.ext[1] * (ndim == 3 ? bx.ext[2] : 1);
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (f32) k_pack<float><<<nb, 256, 0, s>>>(src, nx, ny, ndim, off, bx, average, (float *)dst);
+  else k_pack<double><<<nb, 256, 0, s>>>(src, nx, ny, ndim, off, bx, average, (double *)dst);
 }
 
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {

@@ -42,6 +42,8 @@ _EXPORTS = {
     "fst_profile_read": ([C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_char_p, C.c_size_t],
                          C.c_int),
     "fst_copy_probe": ([C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
+    "fst_pack": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                  C.c_size_t], C.c_int),
     "fst_profile_phases": ([C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "fst_mlp_nparams": ([], C.c_int),
     "fst_set_mlp_weights": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
@@ -157,6 +159,15 @@ class Module:
 
     def post_sim(self):
         pass
+
+    def pack(self, array, vn, start, end, stride, average=False, f32=True):
+        """blockWriter::dataPack on the device (fst_pack); returns the packed
+        block shaped (k, j, i) (2D: (j, i))."""
+        shape, args = pack_args(self.cfg.ndim, start, end, stride)
+        out = np.empty(int(np.prod(shape)), np.float32 if f32 else np.float64)
+        self._chk(lib().fst_pack(self.h, array, vn, *[a.ctypes.data for a in args], int(average), int(f32),
+                                 out.ctypes.data, out.size), "fst_pack")
+        return out.reshape(shape)
 
     def step(self, n=1):
         self._chk(lib().fst_step(self.h, n), "fst_step")
@@ -301,6 +312,18 @@ class Trainer:
         self._chk(lib().fst_mlp_forward_backward(self.h, X.ctypes.data, T.ctypes.data, M, Y.ctypes.data,
                                                  G.ctypes.data, L.ctypes.data), "fst_mlp_forward_backward")
         return Y, G, float(L[0])
+
+
+def pack_args(ndim, start, end, stride):
+    """(k, j, i) shape of a packed block and its (start, end, stride) int64
+    triples, padded to three directions."""
+    args = [np.zeros(3, np.int64) for _ in range(3)]
+    for a, v in zip(args, (start, end, stride)):
+        a[:ndim] = v[:ndim]
+    args[2][ndim:] = 1
+    # an empty or inverted box gives extent 0; the library rejects it
+    ext = [max((int(args[1][a]) - int(args[0][a])) // max(int(args[2][a]), 1) + 1, 0) for a in range(ndim)]
+    return tuple(reversed(ext)), args
 
 
 def rk_advance(f):

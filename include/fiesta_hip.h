@@ -215,6 +215,18 @@ int fst_profile_read(fst_ctx *ctx, int64_t *launches, double *total_ms, char *na
  * flux kernel).  Writes min(n, *nphase) values. */
 int fst_profile_phases(fst_ctx *ctx, double *ms, int n, int *nphase);
 
+/* blockWriter::dataPack (src/block.cpp:404-454) on the device, for the
+ * writers (src/block.cpp:340-352): variable `vn` of FST_VAR, FST_VARX (cell
+ * indices relative to the first interior cell) or FST_GRID (node indices),
+ * at i = start[a], start[a] + stride[a], ..., end[a] (inclusive) per
+ * direction, or, with average = 1, the mean of each stride[0] x stride[1]
+ * (x stride[2]) block there; packed i fastest (extent (end - start)/stride
+ * + 1 per direction) and copied to `out` as float (out_f32 = 1) or double.
+ * `count` is the number of values `out` holds; it must equal the packed size.
+ * Only the packed values cross to the host.  2D uses the first two entries. */
+int fst_pack(fst_ctx *ctx, int array, int vn, const int64_t start[3], const int64_t end[3],
+             const int64_t stride[3], int average, int out_f32, void *out, size_t count);
+
 /* Device-to-device copy of `bytes` bytes with a plain vectorized HIP kernel,
  * `reps` times, on the context stream; returns the mean ms per copy.  Used to
  * measure the attainable HBM bandwidth and to scale rocprof byte counters. */

@@ -45,6 +45,9 @@ def lib():
         L.orc_set_learned.restype = C.c_int
         L.orc_features.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         L.orc_features.restype = C.c_int
+        L.orc_pack.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                               C.c_void_p, C.c_size_t]
+        L.orc_pack.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -123,6 +126,18 @@ class Oracle:
         dr = np.ascontiguousarray(direction, dtype=np.float64).ravel()
         assert ch.size == self.dims.N and dr.size == 3 * self.dims.N
         assert lib().orc_set_learned(self.h, ch.ctypes.data, dr.ctypes.data, float(max_chat)) == 0
+
+    def pack(self, array, vn, start, end, stride, average=False, f32=True):
+        """blockWriter::dataPack (src/block.cpp:404-454); packed block shaped
+        (k, j, i) (2D: (j, i))."""
+        from fiesta_amd.runtime import pack_args
+        shape, args = pack_args(self.cfg.ndim, start, end, stride)
+        out = np.empty(int(np.prod(shape)), np.float32 if f32 else np.float64)
+        rc = lib().orc_pack(self.h, array, vn, *[a.ctypes.data for a in args], int(average), int(f32),
+                            out.ctypes.data, out.size)
+        if rc:
+            raise ValueError("orc_pack failed (%d)" % rc)
+        return out.reshape(shape)
 
     def features(self):
         """Primitives and learned-C features of the current var:

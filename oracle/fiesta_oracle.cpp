@@ -1736,6 +1736,69 @@ int orc_features(void *h, float *F, float *M, float *Dir, float *G1) {
   return 0;
 }
 
+// blockWriter::dataPack (src/block.cpp:404-454) of variable vn of var, varx
+// (cell indices from the first interior cell, offset ng) or the grid nodes
+// (offset 0), into out (float or double), i fastest.
+int orc_pack(void *h, int array, int vn, const int64_t *start, const int64_t *end, const int64_t *stride,
+             int average, int out_f32, void *out, size_t count) {
+  Orc &o = *(Orc *)h;
+  const std::vector<double> *src = nullptr;
+  long long nx = o.ngi, ny = o.ngj, nz = o.ngk;
+  int ng = o.ng;
+  if (array == FST_VAR) src = &o.var;
+  else if (array == FST_VARX) src = &o.varx;
+  else if (array == FST_GRID && o.gen) {
+    src = &o.grid;
+    nx = o.ni; ny = o.nj; nz = o.nk;
+    ng = 0;
+  }
+  if (!src) return 1;
+  const size_t plane = (size_t)nx * ny * nz;
+  auto S = [&](long long i, long long j, long long k) {
+    return (*src)[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k) + plane * vn];
+  };
+  long long ext[3] = {1, 1, 1};
+  for (int a = 0; a < o.ndim; ++a) ext[a] = (end[a] - start[a]) / stride[a] + 1;
+  if ((size_t)(ext[0] * ext[1] * ext[2]) != count) return 2;
+  auto put = [&](long long idx, double v) {
+    if (out_f32) ((float *)out)[idx] = (float)v;
+    else ((double *)out)[idx] = v;
+  };
+  if (o.ndim == 3) {
+    for (long long i = start[0]; i < end[0] + 1; i += stride[0])
+      for (long long j = start[1]; j < end[1] + 1; j += stride[1])
+        for (long long k = start[2]; k < end[2] + 1; k += stride[2]) {
+          const long long ii = (i - start[0]) / stride[0], jj = (j - start[1]) / stride[1],
+                          kk = (k - start[2]) / stride[2];
+          const long long idx = (ext[0] * ext[1]) * kk + ext[0] * jj + ii;
+          if (average) {
+            double mean = 0.0;
+            for (long long ia = 0; ia < stride[0]; ++ia)
+              for (long long ja = 0; ja < stride[1]; ++ja)
+                for (long long ka = 0; ka < stride[2]; ++ka) mean += S(i + ia + ng, j + ja + ng, k + ka + ng);
+            put(idx, mean / (double)(stride[0] * stride[1] * stride[2]));
+          } else {
+            put(idx, S(i + ng, j + ng, k + ng));
+          }
+        }
+  } else {
+    for (long long i = start[0]; i < end[0] + 1; i += stride[0])
+      for (long long j = start[1]; j < end[1] + 1; j += stride[1]) {
+        const long long ii = (i - start[0]) / stride[0], jj = (j - start[1]) / stride[1];
+        const long long idx = ext[0] * jj + ii;
+        if (average) {
+          double mean = 0.0;
+          for (long long ia = 0; ia < stride[0]; ++ia)
+            for (long long ja = 0; ja < stride[1]; ++ja) mean += S(i + ia + ng, j + ja + ng, 0);
+          put(idx, mean / (double)(stride[0] * stride[1]));
+        } else {
+          put(idx, S(i + ng, j + ng, 0));
+        }
+      }
+  }
+  return 0;
+}
+
 double orc_weno3d(double f1, double f2, double f3, double f4, double f5) { return weno3d(f1, f2, f3, f4, f5); }
 double orc_weno2d(double f1, double f2, double f3, double f4, double f5) { return weno2d(f1, f2, f3, f4, f5); }
 
