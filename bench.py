@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--forms", default="fp64,fp32,fp32_learned,fp64_rk3,fp32_learned_rk3",
                     help="comma list of arithmetic forms to time: fp64 (reported as value), fp32, fp32_learned; "
                          "suffix _rk3 runs the SSP-RK3 option instead of the reference's RK2 midpoint")
+    ap.add_argument("--extras", action="store_true",
+                    help="also time postStep's noise filter and the device-side output pack (fst_pack) on the "
+                         "fp64 workload")
     ap.add_argument("--save-weights", default=None, help="write the trained network (.npy, float32)")
     ap.add_argument("--load-weights", default=None, help="use this network (.npy) instead of training one")
     return ap.parse_args()
@@ -150,6 +153,43 @@ def gpu_leg(args, form="fp64", var=None, keep_state=False, weights=None):
         res["copy_gbs"] = 2 * nbytes / (ms * 1e-3) / 1e9
     m.close()
     return res
+
+
+def extras_leg(args, var):
+    """postStep with the noise filter (mode 1: three momentum components, one
+    sweep, C_hat cut-off on) and fst_pack of every variable (float32, full
+    interior) on the fp64 workload; wall-clock ms per call after one warm-up."""
+    from fiesta_amd import workloads as W
+    from fiesta_amd.config import FST_VAR, Dims
+    from fiesta_amd.runtime import Module
+
+    cfg = W.rt3d_config(args.size)
+    cfg.noise, cfg.n_dh, cfg.n_eta, cfg.n_coff, cfg.n_nt, cfg.n_mode = 1, 1e-3, 0.05, 0.1, 1, 1
+    d = Dims(cfg)
+    m = Module(cfg)
+    m.upload(FST_VAR, var[:d.nvt])
+    m.pre_sim()
+    m.post_step()
+    m.sync()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m.post_step()
+    m.sync()
+    post_ms = (time.perf_counter() - t0) * 1e3 / reps
+    n = args.size
+    box = ((0, 0, 0), (n - 1, n - 1, n - 1), (1, 1, 1))
+    m.pack(FST_VAR, 0, *box)
+    t0 = time.perf_counter()
+    for v in range(d.nvt):
+        m.pack(FST_VAR, v, *box)
+    pack_ms = (time.perf_counter() - t0) * 1e3
+    m.close()
+    return {"post_step_with_noise_ms": round(post_ms, 3),
+            "pack_all_vars_f32_ms": round(pack_ms, 3),
+            "pack_bytes_to_host": int(d.nvt * d.ncells * 4),
+            "note": "postStep = secondary variables + noise filter (3 variables, 1 sweep); pack = %d variables, "
+                    "float32, whole interior, each copied to pageable host memory" % d.nvt}
 
 
 def cpu_leg(args, cfg_size):
@@ -329,6 +369,9 @@ def main():
     out["forms"] = fm
     if train_info:
         out["learned_training"] = train_info
+    if args.extras:
+        log("extras: noise filter and output pack")
+        out["extras"] = extras_leg(args, var)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(args, args.size)
         out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 5)
