@@ -59,12 +59,19 @@ def test_derived_sizes_match_input_cpp(ndim, ns, ceq):
 
 
 @pytest.mark.parametrize("field,value", [("ndim", 4), ("ns", 0), ("ns", 9), ("ng", 2), ("scheme", 7),
-                                         ("bcL", 5), ("nci", 0)])
+                                         ("bcL", 5), ("nci", 0), ("rk_order", 4), ("rk_order", 1)])
 def test_invalid_configurations_are_rejected(field, value):
     cfg = make_config(ndim=3, ns=1, nci=8, ncj=8, nck=8)
     setattr(cfg, field, value)
     d = FstDims()
     assert runtime.lib().fst_config_dims(C.byref(cfg), C.byref(d)) == 1  # FST_ERR_ARG
+
+
+def test_noise_filter_needs_nonnegative_sweeps():
+    cfg = make_config(ndim=3, ns=1, nci=8, ncj=8, nck=8, noise=True, n_nt=-1)
+    assert runtime.lib().fst_config_dims(C.byref(cfg), C.byref(FstDims())) == 1
+    cfg.n_nt = 0
+    assert runtime.lib().fst_config_dims(C.byref(cfg), C.byref(FstDims())) == 0
 
 
 def test_periodic_direction_needs_ng_cells():
