@@ -1021,6 +1021,9 @@ __global__ void __launch_bounds__(NTHREADS, 1)
         if constexpr (BL) un[v] = upd_val(up, ld(U, c + N * v), ld(Un, c + N * v), dv[v]);
         else un[v] = ((up.form == 1) ? ld(U, c + N * v) : ld(Un, c + N * v)) + up.h * dv[v];
         ust<FO>(Uo, c + N * v, un[v]);
+        // float stage state: the primitives (and maxima) describe the stored,
+        // rounded state, as the ghost-cell primitives computed from it do
+        if (FO) un[v] = (double)(float)un[v];
       }
       double pr[6 + NS];
       double ms;
@@ -1408,6 +1411,9 @@ __global__ void __launch_bounds__(NTHREADS, 1)
           un[v] = base + up.h * dv[v];
         }
         ust<FO>(Uo, c + N * v, un[v]);
+        // float stage state: the primitives (and maxima) describe the stored,
+        // rounded state, as the ghost-cell primitives computed from it do
+        if (FO) un[v] = (double)(float)un[v];
       }
       double pr[6 + NS];
       double ms;
