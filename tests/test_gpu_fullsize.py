@@ -91,6 +91,36 @@ def test_fullsize_periodic_shift_invariance(rt, form):
     assert_bitwise(shifted[sl], _roll_x(base, d, s)[sl], form + " shifted run vs shifted result")
 
 
+def _roll_z(a, d, s):
+    out = a.copy()
+    g = d.ng
+    out[:, g:g + d.nck] = np.roll(a[:, g:g + d.nck], s, axis=1)
+    return out
+
+
+@pytest.mark.parametrize("form", ["fp64_rk3", "fp32_learned_rk3"])
+def test_fullsize_z_shift_invariance_ssp_rk3(rt, form):
+    """The same along periodic z (across the k-chunks of the 2.5D march), with
+    the SSP-RK3 option (blend kernels, ghost-cell update)."""
+    cfg, var = rt
+    cfg = W.rt3d_config(N)
+    cfg.rk_order = 3
+    weights = None
+    if form.startswith("fp32"):
+        cfg.precision = FST_FP32_BULK
+        cfg.learned_c = 1
+        tr = Trainer(seed=5)
+        weights = tr.weights()
+        tr.close()
+    d = Dims(cfg)
+    _say("case full size: z shift invariance,", form)
+    s = 13  # not a multiple of the k-chunk
+    base = _run(cfg, var, 2, weights=weights)
+    shifted = _run(cfg, _roll_z(var[:d.nvt], d, s), 2, weights=weights)
+    sl = d.interior()
+    assert_bitwise(shifted[sl], _roll_z(base, d, s)[sl], form + " shifted run vs shifted result")
+
+
 def test_fullsize_species_mass_is_conserved(rt):
     cfg, var = rt
     d = Dims(cfg)
