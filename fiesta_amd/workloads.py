@@ -300,3 +300,120 @@ def shear3d_state(cfg, seed=1, U=100.0, T0=300.0, p0=1.0e5, nmodes=4, amp=0.1):
     p = p0 * np.ones(sh)
     d_nvt_c = Dims(cfg).nvt > Dims(cfg).nv
     return fill_conserved(cfg, [rh, rl], [u, v, w], p, [np.zeros(sh)] * 5 if d_nvt_c else None)
+
+
+def riemann_ideal(rho_l, p_l, rho_r, p_r, gamma):
+    """Exact solution of the Riemann problem for one ideal gas at rest on both
+    sides (Toro, Riemann Solvers and Numerical Methods, ch. 4): returns the
+    star pressure p*, velocity u*, the star densities left and right of the
+    contact, and the right shock's speed (p_l > p_r: a left rarefaction and a
+    right shock)."""
+    g = gamma
+    a_l, a_r = math.sqrt(g * p_l / rho_l), math.sqrt(g * p_r / rho_r)
+
+    def f(p, rho, pk, a):
+        if p > pk:
+            A, B = 2.0 / ((g + 1.0) * rho), (g - 1.0) / (g + 1.0) * pk
+            return (p - pk) * math.sqrt(A / (p + B))
+        return 2.0 * a / (g - 1.0) * ((p / pk) ** ((g - 1.0) / (2.0 * g)) - 1.0)
+
+    lo, hi = min(p_l, p_r), max(p_l, p_r)
+    for _ in range(200):  # bisection: f_l + f_r is increasing in p
+        mid = 0.5 * (lo + hi)
+        if f(mid, rho_l, p_l, a_l) + f(mid, rho_r, p_r, a_r) > 0.0:
+            hi = mid
+        else:
+            lo = mid
+    ps = 0.5 * (lo + hi)
+    us = 0.5 * (f(ps, rho_r, p_r, a_r) - f(ps, rho_l, p_l, a_l))
+    gm = (g - 1.0) / (g + 1.0)
+    rho_sr = rho_r * (ps / p_r + gm) / (gm * ps / p_r + 1.0)
+    rho_sl = rho_l * (ps / p_l) ** (1.0 / g)
+    s_r = a_r * math.sqrt((g + 1.0) / (2.0 * g) * ps / p_r + (g - 1.0) / (2.0 * g))
+    return dict(p=ps, u=us, rho_l=rho_sl, rho_r=rho_sr, s_shock=s_r)
+
+
+def shock3d_config(pr, nci=200, ncj=4, nck=4, steps=300, x0=0.4, travel=0.3, ceq=True, learned_c=False,
+                   precision=0, T0=300.0, p_r=1.0e5):
+    """3D shock tube along x: air at rest, temperature T0 on both sides,
+    pressure pr * p_r left of x0 and p_r right of it; 1 m long, nci x ncj x
+    nck cubic cells, y/z periodic, x outflow, inviscid, C-equation with the
+    reference rt3d deck's constants (kappa 10, eps 5, alpha 20).  dt is set
+    so that the exact shock travels `travel` in `steps` steps, whatever pr
+    is (CFL about 0.3 for pr <= 12)."""
+    cfg = make_config(ndim=3, ns=1, nci=nci, ncj=ncj, nck=nck, dx=1.0 / nci, dy=1.0 / nci, dz=1.0 / nci, R=RU,
+                      gamma=(AIR["gamma"],), M=(AIR["M"],), mu=(0.0,), visc=False, ceq=ceq, kap=10.0, eps=5.0,
+                      alpha=20.0, yper=True, zper=True, bc=("outflow",) * 6, dt=1.0, precision=precision,
+                      learned_c=learned_c)
+    ex = shock3d_exact(cfg, pr, T0, p_r)
+    cfg.dt = travel / (ex["s_shock"] * steps)
+    return cfg
+
+
+def shock3d_exact(cfg, pr, T0=300.0, p_r=1.0e5):
+    rho_r = p_r / (cfg.R / cfg.M[0] * T0)
+    return riemann_ideal(pr * rho_r, pr * p_r, rho_r, p_r, cfg.gamma[0])
+
+
+def shock3d_state(cfg, pr, x0=0.4, T0=300.0, p_r=1.0e5):
+    d = Dims(cfg)
+    x, y, z = _cells(d)
+    sh = (d.nck, d.ncj, d.nci)
+    left = np.broadcast_to(x < x0, sh)
+    p = np.where(left, pr * p_r, p_r)
+    rho = p / (cfg.R / cfg.M[0] * T0)
+    u = [np.zeros(sh)] * 3
+    return fill_conserved(cfg, [rho], u, p, [np.zeros(sh)] * 5 if d.nvt > d.nv else None)
+
+
+def rm3d_config(nci=128, ncj=32, nck=4, pr=5.0, ceq=True, learned_c=False, precision=0, cfl=0.3, T0=300.0,
+                p_r=1.0e5):
+    """3D Richtmyer-Meshkov: a shock in air (pressure ratio pr) runs into a
+    corrugated air/SF6 interface.  x in [0, 1] with nci cells, cubic cells
+    (y extent ncj/nci, z extent nck/nci), y/z periodic, x outflow, inviscid,
+    C-equation with the reference rt3d deck's constants.  dt = cfl dx /
+    (u* + a*), the fastest signal behind the incident shock."""
+    mix = _mix(2, [AIR, SF6])
+    cfg = make_config(ndim=3, ns=2, nci=nci, ncj=ncj, nck=nck, dx=1.0 / nci, dy=1.0 / nci, dz=1.0 / nci, R=RU,
+                      visc=False, ceq=ceq, kap=10.0, eps=5.0, alpha=20.0, yper=True, zper=True,
+                      bc=("outflow",) * 6, dt=1.0, precision=precision, learned_c=learned_c, **mix)
+    ex = rm3d_incident(cfg, pr, T0, p_r)
+    a_s = math.sqrt(cfg.gamma[0] * ex["p"] / ex["rho_r"])
+    cfg.dt = cfl * cfg.dx / (ex["u"] + a_s)
+    return cfg
+
+
+def rm3d_incident(cfg, pr, T0=300.0, p_r=1.0e5):
+    """The incident shock: the Riemann problem of the driver (pr p_r) and the
+    driven air (p_r), both at T0."""
+    rho_r = p_r / (cfg.R / cfg.M[0] * T0)
+    return riemann_ideal(pr * rho_r, pr * p_r, rho_r, p_r, cfg.gamma[0])
+
+
+def rm3d_state(cfg, seed=1, pr=5.0, x_shock=0.12, x_int=0.25, amp=0.02, T0=300.0, p_r=1.0e5):
+    """Driver air (pr p_r) left of x_shock, air at p_r up to the interface
+    x = x_int + eta(y, z), SF6 at p_r beyond; eta = amp * (sum of two modes
+    along y and one along z, wavenumbers and phases from the seed), smoothed
+    over one cell with tanh."""
+    d = Dims(cfg)
+    rng = np.random.default_rng(seed)
+    x, _, _ = _cells(d)
+    ly, lz = d.ncj / d.nci, d.nck / d.nci
+    y = ((np.arange(d.ncj) + 0.5) / d.nci)[None, :, None]
+    z = ((np.arange(d.nck) + 0.5) / d.nci)[:, None, None]
+    eta = np.zeros((d.nck, d.ncj, 1))
+    for _ in range(2):
+        k = rng.integers(1, 4)
+        eta = eta + rng.uniform(0.5, 1.0) * np.sin(2 * math.pi * k * y / ly + rng.uniform(0, 2 * math.pi))
+    eta = eta + 0.3 * np.sin(2 * math.pi * z / lz + rng.uniform(0, 2 * math.pi))
+    eta = amp * eta / 2.3
+    sh = (d.nck, d.ncj, d.nci)
+    s = 0.5 * (1.0 + np.tanh((x - x_int - eta) / cfg.dx)) * np.ones(sh)  # SF6 fraction
+    drv = np.broadcast_to(x < x_shock, sh)
+    p = np.where(drv, pr * p_r, p_r)
+    rho_air = p / (cfg.R / cfg.M[0] * T0)
+    rho_sf6 = p / (cfg.R / cfg.M[1] * T0)
+    ra = (1.0 - s) * rho_air + 1e-6 * rho_air
+    rs = s * rho_sf6 + 1e-6 * rho_sf6
+    u = [np.zeros(sh)] * 3
+    return fill_conserved(cfg, [ra, rs], u, p, [np.zeros(sh)] * 5 if d.nvt > d.nv else None)

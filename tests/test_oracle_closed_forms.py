@@ -20,6 +20,9 @@ sign, index or species stride, a swapped stage coefficient) fails:
   shock/contact switch (src/ceq3d.hpp:143-186) linear ramps
   full 2D step (src/cart2d.cpp)        Sod shock tube vs the exact Riemann
                                        solution at t = 0.2
+  full 3D step (src/cart3d.cpp)        planar shock tubes vs the exact Riemann
+                                       solution (shock position, star state);
+                                       the C-equation leaves a planar shock alone
 """
 from fractions import Fraction
 
@@ -444,3 +447,56 @@ def test_sod_shock_tube_against_exact_riemann_solution():
     assert np.all(np.abs(rho[x < 0.2] - 1.0) < 1e-3)
     assert np.all(np.abs(rho[x > 0.95] - 0.125) < 1e-3)
     assert np.abs(rho[(x > 0.5) & (x < 0.56)] - 0.42632).max() < 0.005  # left star state
+
+
+def test_riemann_solver_reproduces_toro_test_1():
+    """The workloads' exact solver on Sod's problem: Toro's tabulated star
+    state (p* 0.30313, u* 0.92745, rho*L 0.42632, rho*R 0.26557)."""
+    ex = W.riemann_ideal(1.0, 1.0, 0.125, 0.1, 1.4)
+    assert ex["p"] == pytest.approx(0.30313, abs=1e-5) and ex["u"] == pytest.approx(0.92745, abs=1e-5)
+    assert ex["rho_l"] == pytest.approx(0.42632, abs=1e-5) and ex["rho_r"] == pytest.approx(0.26557, abs=1e-5)
+    _, _, _, sh = _sod_exact(np.zeros(1), 0.2)
+    assert ex["s_shock"] == pytest.approx(sh, rel=1e-9)
+
+
+def _ideal_p(cfg, a):
+    rho = a[4]
+    return (cfg.gamma[0] - 1.0) * (a[3] - 0.5 * (a[0] ** 2 + a[1] ** 2 + a[2] ** 2) / rho)
+
+
+@pytest.mark.parametrize("pr", [2.0, 12.0])
+def test_3d_shock_tube_against_exact_riemann_solution(pr):
+    """Full 3D steps (src/cart3d.cpp) on a planar shock tube along x, 200x4x4,
+    air at 300 K on both sides: after 300 steps the shock (the p = (p* + p_R)/2
+    crossing) sits at the exact position to a quarter cell, and the plateau
+    between contact and shock holds the exact star pressure and density
+    (mean to 5e-4).  The C-equation's artificial viscosity (src/ceq3d.hpp:226-366) only
+    diffuses along the interface (M = I - n n^T with n the C-gradient
+    direction), so on a planar shock it leaves the solution unchanged to
+    round-off."""
+    outs = {}
+    for ceq in (True, False):
+        cfg = W.shock3d_config(pr, ceq=ceq)
+        d = Dims(cfg)
+        o = oracle.Oracle(cfg)
+        o.set(FST_VAR, W.shock3d_state(cfg, pr))
+        o.pre_sim()
+        o.step(300)
+        outs[ceq] = o.get(FST_VAR)[d.interior()]
+    a = outs[True]
+    ex = W.shock3d_exact(cfg, pr)
+    p = _ideal_p(cfg, a)
+    assert np.ptp(p, axis=(0, 1)).max() <= 1e-9 * p.max()  # planar: every (y, z) row alike
+    p = p[0, 0]
+    pm = 0.5 * (ex["p"] + 1.0e5)
+    i = np.nonzero(p >= pm)[0].max()
+    xs = (i + 0.5 + (p[i] - pm) / (p[i] - p[i + 1])) / cfg.nci
+    assert abs(xs - 0.7) < 0.25 / cfg.nci, xs  # 0.4 + the 0.3 the exact shock travels
+    xc = 0.4 + ex["u"] * 300 * cfg.dt  # contact
+    plateau = slice(int((xc + 0.03) * cfg.nci), int(0.67 * cfg.nci))
+    rho = a[4, 0, 0]
+    # the mean to 5e-4; pointwise, the post-shock wiggles of WENO5 to 2%
+    assert abs(p[plateau].mean() / ex["p"] - 1) < 5e-4 and abs(rho[plateau].mean() / ex["rho_r"] - 1) < 5e-4
+    assert np.abs(p[plateau] / ex["p"] - 1).max() < 0.02
+    b = outs[False]
+    assert np.abs(b[:5] - a[:5]).max() <= 1e-8 * np.abs(a[:5]).max()
