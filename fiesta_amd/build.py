@@ -58,17 +58,47 @@ def build_hip(verbose=False):
 
 HOST_DIR = os.path.join(PKG, "host")
 RUN_BIN = os.path.join(PKG, "fiesta_run")
+HOST_LIB = os.path.join(PKG, "libfiesta_host.so")
+# HDF5 (serial, C API) for the solution/restart files.  The library is linked
+# through a directory holding only a link to it, so the run-time search path
+# of the host code names no directory of unrelated shared libraries.
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
+HDF5_SONAME = "libhdf5.so.103"
+HDF5_LINK_DIR = os.path.join(PKG, "_hdf5")
+
+
+def _hdf5_link():
+    inc = os.path.join(HDF5_DIR, "include")
+    lib = os.path.join(HDF5_DIR, "lib", HDF5_SONAME)
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(lib)):
+        raise RuntimeError("HDF5 not found under %s (set HDF5_DIR): the writers need hdf5.h and %s" %
+                           (HDF5_DIR, HDF5_SONAME))
+    os.makedirs(HDF5_LINK_DIR, exist_ok=True)
+    link = os.path.join(HDF5_LINK_DIR, HDF5_SONAME)
+    if os.path.lexists(link) and os.readlink(link) != lib:
+        os.remove(link)
+    if not os.path.lexists(link):
+        os.symlink(lib, link)
+    return inc, link
 
 
 def build_host():
-    """The host rk_func layer (C++, no HIP code) and its fiesta_run driver,
-    linked against libfiesta_hip.so (rpath: the package directory)."""
+    """The host rk_func layer and the HDF5/XDMF writers and restart reader
+    (C++, no HIP code) as libfiesta_host.so, and the fiesta_run driver;
+    linked against libfiesta_hip.so (run-time path: the package directory)."""
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
-    srcs = [os.path.join(HOST_DIR, f) for f in ("fiesta_host.cpp", "fiesta_run.cpp")]
+    inc, h5 = _hdf5_link()
+    common = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + HOST_DIR]
+    rpath = ["-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/_hdf5", "-Wl,-rpath-link," + os.path.join(rocm, "lib"),
+             "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    srcs = [os.path.join(HOST_DIR, f) for f in ("fiesta_host.cpp", "fiesta_io.cpp")]
+    tmp = HOST_LIB + ".tmp"
+    _run(common + ["-fPIC", "-shared", "-isystem", inc] + srcs + [h5, "-L" + PKG, "-lfiesta_hip"] + rpath +
+         ["-o", tmp])
+    os.replace(tmp, HOST_LIB)
     tmp = RUN_BIN + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + HOST_DIR] + srcs +
-         ["-L" + PKG, "-lfiesta_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link," + os.path.join(rocm, "lib"),
-          "-Wl,-rpath," + os.path.join(rocm, "lib"), "-o", tmp])
+    _run(common + [os.path.join(HOST_DIR, "fiesta_run.cpp"), "-L" + PKG, "-lfiesta_host", "-lfiesta_hip"] + rpath +
+         ["-o", tmp])
     os.replace(tmp, RUN_BIN)
     return RUN_BIN
 
