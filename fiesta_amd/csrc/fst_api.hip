@@ -139,6 +139,7 @@ void free_all(fst_ctx *ctx) {
     if (*pp) { (void)hipFree(*pp); *pp = nullptr; }
   if (ctx->b.nflag) { (void)hipFree(ctx->b.nflag); ctx->b.nflag = nullptr; }
   if (ctx->b.nmax) { (void)hipFree(ctx->b.nmax); ctx->b.nmax = nullptr; }
+  if (ctx->b.mm) { (void)hipFree(ctx->b.mm); ctx->b.mm = nullptr; }
   if (ctx->b.sc) { (void)hipFree(ctx->b.sc); ctx->b.sc = nullptr; }
   fused_free(ctx->fb);
   learned_free(ctx->lb);
@@ -270,6 +271,7 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
     }
     ok = learned_alloc(g, ctx->lb) == FST_OK;
   }
+  ok = ok && hipMalloc((void **)&ctx->b.mm, 2 * sizeof(unsigned long long) * std::max(g.nvt, g.nvx)) == hipSuccess;
   ok = ok && hipDeviceSynchronize() == hipSuccess;
   ok = ok && hipMalloc((void **)&ctx->b.sc, sizeof(Scalars)) == hipSuccess &&
        hipMemset(ctx->b.sc, 0, sizeof(Scalars)) == hipSuccess;
@@ -513,6 +515,25 @@ int fst_pack(fst_ctx *ctx, int array, int vn, const int64_t start[3], const int6
   if (int rc = check_launch(ctx, "fst_pack")) return rc;
   HIPCHK(ctx, hipMemcpyAsync(out, ctx->pack, bytes, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return FST_OK;
+}
+
+int fst_status(fst_ctx *ctx, int array, double *min, double *max, int nvar) {
+  if (!ctx || !min || !max) return FST_ERR_ARG;
+  const Geo &g = ctx->g;
+  const double *a = array == FST_VAR ? ctx->b.var : array == FST_VARX ? ctx->b.varx : nullptr;
+  const int na = array == FST_VAR ? g.nvt : g.nvx;
+  if (!a) return fail(ctx, FST_ERR_ARG, "fst_status: array must be FST_VAR or FST_VARX");
+  if (nvar < 1 || nvar > na) return fail(ctx, FST_ERR_ARG, "fst_status: nvar out of range");
+  launch_minmax(g, a, nvar, ctx->b.mm, ctx->stream);
+  if (int rc = check_launch(ctx, "fst_status")) return rc;
+  unsigned long long k[2 * 64];
+  HIPCHK(ctx, hipMemcpyAsync(k, ctx->b.mm, 2 * sizeof(unsigned long long) * nvar, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int v = 0; v < nvar; ++v) {
+    max[v] = key_to_double(k[2 * v]);
+    min[v] = key_to_double(k[2 * v + 1]);
+  }
   return FST_OK;
 }
 

@@ -64,6 +64,7 @@ struct Bufs {
   double *ndn = nullptr;
   unsigned char *nflag = nullptr;
   unsigned long long *nmax = nullptr;
+  unsigned long long *mm = nullptr;  // status check: (max, min) order keys per variable
   Scalars *sc = nullptr;     // per-stage maxima (device)
   Prof *prof = nullptr;      // timing of the dominant kernel (host side)
 };
@@ -139,6 +140,14 @@ void launch_pack(const double *src, long long nx, long long ny, int ndim, int of
 int noise_flag_count(const Geo &g, bool gen);
 void launch_noise(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
+// statusCheck reductions: max and min over the interior cells of each of the
+// first nvar variables of `a` (planes of g.N values), into mm as order keys
+// (mm[2v] max, mm[2v+1] min); decode with key_to_double.
+void launch_minmax(const Geo &g, const double *a, int nvar, unsigned long long *mm, hipStream_t s);
+inline double key_to_double(unsigned long long k) {
+  const unsigned long long b = (k >> 63) ? (k & ~(1ull << 63)) : ~k;
+  return __builtin_bit_cast(double, b);
+}
 void launch_secondary(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
 // generalized grids (gen2d/gen3d modules)
 void launch_compute_gen(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);

@@ -3,6 +3,9 @@
 // fst_compute()/fst_rk_stage() (the rk_func::compute contract, dvar written
 // to HBM) and every 2D configuration.  The fused 3D stage kernel lives in
 // fst_fused3d.hip and must agree with these bit for bit.
+#include <algorithm>
+#include <cfloat>
+
 #include "fst_common.hpp"
 #include "fst_kernels.hpp"
 
@@ -1143,6 +1146,65 @@ __global__ void k_genpresgrad2d(Geo g, Range r, const double *__restrict__ p, co
   }
 }
 
+
+// ---------------------------------------------------------------- status check
+
+// Order key of a double: unsigned order of the key is numeric order (negative
+// values reversed; -0 orders just below +0).
+__device__ __forceinline__ unsigned long long order_key(double d) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(d);
+  return (b >> 63) ? ~b : (b | (1ull << 63));
+}
+
+__global__ void k_minmax_init(int nvar, unsigned long long *__restrict__ mm) {
+  const int v = threadIdx.x;
+  if (v < nvar) {
+    mm[2 * v] = order_key(-DBL_MAX);  // the identities of Kokkos::Max / Kokkos::Min
+    mm[2 * v + 1] = order_key(DBL_MAX);
+  }
+}
+
+// maxVarFunctor / minVarFunctor (src/status.cpp:40-98) over the cell range:
+// one block per group of (j, k) rows of variable blockIdx.y, threads along i.
+// `s > max` and `s < min` are false for NaN, so NaN cells leave both
+// reductions unchanged, as in the reference's join.
+__global__ void __launch_bounds__(256) k_minmax(const double *__restrict__ a, long long N, int ndim, int ng, int ngi,
+                                                int ngj, int nci, int ncj, int rows, unsigned long long *__restrict__ mm) {
+  const int v = blockIdx.y;
+  const double *p = a + N * v;
+  const int ko = ndim == 3 ? ng : 0;
+  double mx = -DBL_MAX, mn = DBL_MAX;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int j = r % ncj, k = r / ncj;
+    const double *row = p + (long long)ngi * ((j + ng) + (long long)ngj * (k + ko)) + ng;
+    for (int i = threadIdx.x; i < nci; i += blockDim.x) {
+      const double x = row[i];
+      if (x > mx) mx = x;
+      if (x < mn) mn = x;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double x = __shfl_xor(mx, o), y = __shfl_xor(mn, o);
+    if (x > mx) mx = x;
+    if (y < mn) mn = y;
+  }
+  __shared__ double smx[4], smn[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    smx[w] = mx;
+    smn[w] = mn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+      if (smx[q] > mx) mx = smx[q];
+      if (smn[q] < mn) mn = smn[q];
+    }
+    atomicMax(&mm[2 * v], order_key(mx));
+    atomicMin(&mm[2 * v + 1], order_key(mn));
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- launchers
@@ -1360,6 +1422,13 @@ void launch_pack(const double *src, long long nx, long long ny, int ndim, int of
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {
   long long n = (long long)(bytes / 16);
   k_copy<<<256 * 8 * 4, 256, 0, s>>>((const double2 *)src, (double2 *)dst, n);
+}
+
+void launch_minmax(const Geo &g, const double *a, int nvar, unsigned long long *mm, hipStream_t s) {
+  k_minmax_init<<<1, 64, 0, s>>>(nvar, mm);
+  const int rows = g.ncj * (g.ndim == 3 ? g.nck : 1);
+  const unsigned nb = (unsigned)std::min(rows, 2048);
+  k_minmax<<<dim3(nb, (unsigned)nvar), 256, 0, s>>>(a, g.N, g.ndim, g.ng, g.ngi, g.ngj, g.nci, g.ncj, rows, mm);
 }
 
 void launch_secondary(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s) {

@@ -1,6 +1,7 @@
 // Host-side rk_func interface over the C-ABI; see fiesta_host.hpp.
 #include "fiesta_host.hpp"
 
+#include <cmath>
 #include <string>
 
 namespace fiesta_hip {
@@ -157,6 +158,42 @@ void step(rk_func &f, int nsteps) {
     t.description = "Fused RK steps (device time)";
     t.ms += ms;
   }
+}
+
+bool statusIsBad(double v) {  // isBad, src/status.cpp:100-105
+  return !((std::isnormal(v) || v == 0) && (v > -1.0e+200 && v < 1.0e+200));
+}
+
+bool statusIsConcern(double v) { return v < -1.0e+16 || v > 1.0e+16; }  // isConcern, src/status.cpp:107-112
+
+std::vector<VarStatus> statusCheck(rk_func &f, std::FILE *out, bool color, long long t, long long tend, double time) {
+  std::vector<VarStatus> st;
+  const struct {
+    int array;
+    const std::vector<std::string> *names;
+  } sets[2] = {{FST_VAR, &f.varNames}, {FST_VARX, &f.varxNames}};
+  for (const auto &set : sets) {
+    const int n = (int)set.names->size();
+    std::vector<double> mn(n), mx(n);
+    if (int rc = fst_status(f.context(), set.array, mn.data(), mx.data(), n))
+      throw Error(std::string("fst_status: ") + fst_last_error(f.context()), rc);
+    for (int v = 0; v < n; ++v) st.push_back({(*set.names)[v], mn[v], mx[v]});
+  }
+  if (!out) return st;
+  auto col = [&](double v) {
+    if (!color) return "";
+    return statusIsBad(v) ? "\033[0;31m" : statusIsConcern(v) ? "\033[0;33m" : "\033[0;35m";
+  };
+  const char *reset = color ? "\033[0m" : "";
+  std::fprintf(out, "[%lld] Reporting Status\n", t);
+  std::fprintf(out, "%8sTimestep:  %lld/%lld (%.0f%%)\n", "", t, tend, tend > 0 ? 100.0 * (double)t / (double)tend : 0.0);
+  std::fprintf(out, "%8sSim Time:  %.2gs\n", "", time);
+  std::fprintf(out, "%8s%-16s%11s%11s\n", "", "", "Min", "Max");
+  for (const VarStatus &s : st)
+    std::fprintf(out, "%8s%-16s%s%11.2e%s%s%11.2e%s\n", "", s.name.c_str(), col(s.min), s.min, reset, col(s.max), s.max,
+                 reset);
+  std::fflush(out);
+  return st;
 }
 
 }  // namespace fiesta_hip

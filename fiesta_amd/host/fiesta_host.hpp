@@ -17,6 +17,7 @@
 // the C-ABI are turned into fiesta_hip::Error exceptions at this layer.
 #pragma once
 #include <cstdint>
+#include <cstdio>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -164,5 +165,19 @@ void applyBCs(rk_func &f);
 // (one call, results equal to preStep + rkAdvance + postStep without the
 // postStep secondary-variable pass, which runs when called explicitly).
 void step(rk_func &f, int nsteps);
+
+// statusCheck (src/status.cpp:155-251): min and max of every variable of var
+// and varx over the interior cells, reduced on the device (fst_status, so no
+// array is copied to the host), printed to `out` (NULL: not printed) as the
+// reference's table. isBad values (not normal and not 0, or beyond 1e200
+// in magnitude) print red, isConcern values (beyond 1e16) yellow, others
+// magenta, when `color` is set. varx holds what the last postStep computed.
+struct VarStatus {
+  std::string name;
+  double min = 0.0, max = 0.0;
+};
+bool statusIsBad(double v);
+bool statusIsConcern(double v);
+std::vector<VarStatus> statusCheck(rk_func &f, std::FILE *out, bool color, long long t, long long tend, double time);
 
 }  // namespace fiesta_hip

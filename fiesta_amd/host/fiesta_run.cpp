@@ -19,6 +19,7 @@
 //   --restart-in FILE         read var (and grid) from a restart file (readRestart)
 //   --restart-out PATH NAME   after the steps, write the restart file PATH/NAME-<step>.h5
 //   --view NAME PATH AVG      after the steps, write an IO view of the whole domain
+//   --status                  after the steps, print statusCheck's table (--status-color: in colour)
 //   --view-box NAME PATH AVG START LIMIT STRIDE
 //                             an IO view of cells START..LIMIT (1-based, inclusive,
 //                             comma lists as the deck's) every STRIDE cells
@@ -77,7 +78,7 @@ struct ViewArg {
 int usage(const char *prog) {
   std::fprintf(stderr,
                "usage: %s CONFIG STATE OUT NSTEPS [rk|fused] [GRID] [--species A,B] [--title T] [--tend N]\n"
-               "       [--restart-in FILE] [--restart-out PATH NAME] [--view NAME PATH AVG]\n"
+               "       [--restart-in FILE] [--restart-out PATH NAME] [--view NAME PATH AVG] [--status[-color]]\n"
                "       [--view-box NAME PATH AVG START LIMIT STRIDE]\n",
                prog);
   return 2;
@@ -91,6 +92,7 @@ int main(int argc, char **argv) {
   std::vector<ViewArg> views;
   IoConfig io;
   std::string restart_in, rpath, rname;
+  bool status = false, status_color = false;
   for (int a = 1; a < argc; ++a) {
     const std::string k = argv[a];
     auto need = [&](int n) { return a + n < argc; };
@@ -102,6 +104,9 @@ int main(int argc, char **argv) {
       io.title = argv[++a];
     } else if (k == "--tend" && need(1)) {
       io.tend = std::atoll(argv[++a]);
+    } else if (k == "--status" || k == "--status-color") {
+      status = true;
+      status_color = k == "--status-color";
     } else if (k == "--restart-in" && need(1)) {
       restart_in = argv[++a];
     } else if (k == "--restart-out" && need(2)) {
@@ -201,7 +206,7 @@ int main(int argc, char **argv) {
     f->sync();
     const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     const bool files = !writers.empty() || !rpath.empty();
-    if (mode == "fused" && files) f->postStep();
+    if (mode == "fused" && (files || status)) f->postStep();
     // checkIO at the last step (src/fiesta.cpp:194-272): the restart, then the views
     const int tdx = tstart + nsteps;
     const double time = time0 + nsteps * cf.dt;
@@ -211,6 +216,7 @@ int main(int argc, char **argv) {
       written.push_back(restart.write(*f, tdx, time));
     }
     for (auto &w : writers) written.push_back(w.write(*f, tdx, time));
+    if (status) statusCheck(*f, stdout, status_color, tdx, io.tend, time);
     f->pull(FST_VAR);
     f->postSim();
     std::ofstream o(pos[2], std::ios::binary);

@@ -42,6 +42,7 @@ _EXPORTS = {
     "fst_profile_read": ([C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_char_p, C.c_size_t],
                          C.c_int),
     "fst_copy_probe": ([C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
+    "fst_status": ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int], C.c_int),
     "fst_pack": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                   C.c_size_t], C.c_int),
     "fst_profile_phases": ([C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
@@ -159,6 +160,16 @@ class Module:
 
     def post_sim(self):
         pass
+
+    def status(self, array=FST_VAR, nvar=None):
+        """statusCheck's per-variable (min, max) over the interior cells
+        (src/status.cpp:155-231), reduced on the device."""
+        d = Dims(self.cfg)
+        if nvar is None:
+            nvar = d.nvt if array == FST_VAR else d.ndim + 3
+        mn, mx = np.empty(nvar), np.empty(nvar)
+        self._chk(lib().fst_status(self.h, array, mn.ctypes.data, mx.ctypes.data, int(nvar)), "fst_status")
+        return mn, mx
 
     def pack(self, array, vn, start, end, stride, average=False, f32=True):
         """blockWriter::dataPack on the device (fst_pack); returns the packed

@@ -33,7 +33,7 @@ extern "C" {
 #endif
 
 #define FST_MAX_SPECIES 8
-#define FST_ABI_VERSION 4
+#define FST_ABI_VERSION 5
 
 /* Boundary types, in the order of `enum class BCType` (src/bc.hpp:31). */
 enum fst_bc_type {
@@ -226,6 +226,14 @@ int fst_profile_phases(fst_ctx *ctx, double *ms, int n, int *nphase);
  * Only the packed values cross to the host.  2D uses the first two entries. */
 int fst_pack(fst_ctx *ctx, int array, int vn, const int64_t start[3], const int64_t end[3],
              const int64_t stride[3], int average, int out_f32, void *out, size_t count);
+
+/* statusCheck's reductions (src/status.cpp:155-231): the maximum and minimum
+ * of each of the first `nvar` variables of FST_VAR or FST_VARX over the
+ * interior cells, in one pass on the device; only 2*nvar values cross to the
+ * host.  As the reference's Kokkos::Max/Min reductions, NaN values are
+ * skipped (a variable with no other value reports -DBL_MAX / DBL_MAX); -0
+ * counts as smaller than +0. */
+int fst_status(fst_ctx *ctx, int array, double *min, double *max, int nvar);
 
 /* Device-to-device copy of `bytes` bytes with a plain vectorized HIP kernel,
  * `reps` times, on the context stream; returns the mean ms per copy.  Used to

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_struct_size():
     lib = runtime.lib()
-    assert lib.fst_abi_version() == 4
+    assert lib.fst_abi_version() == 5
     # fst_config layout: the ctypes mirror and the C struct must agree in size
     c = FstConfig()
     lib.fst_config_init(C.byref(c))
