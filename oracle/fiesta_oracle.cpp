@@ -24,11 +24,14 @@
  *   viscous         linear shear (uniform stress: zero divergence), quadratic
  *                   profile with analytic divergence
  *   C-equation      uniform state (zero sources), planar-shock switch logic,
- *                   relaxation of a uniform C field
- *   full 2D step    Sod shock tube against the exact Riemann solution
- *   Parity unpinned (no independent check beyond the above): the exact
- *   bit-level composition of the C-equation artificial-viscosity faces
- *   (calculateCeqFaces/Grads/applyCeq) and the 2D Q7/Q6 quirk values.
+ *                   relaxation of a uniform C field; artificial-viscosity
+ *                   faces (calculateCeqFaces/Grads/applyCeq and the 2D set)
+ *                   on quadratic velocities: the anisotropic Laplacian
+ *                   alpha rho C_hat sum M_fd H_fd exactly
+ *   full 2D/3D step Sod and planar 3D shock tubes against the exact Riemann
+ *                   solution
+ *   Parity unpinned (no independent check beyond the above): the values of
+ *   the 2D viscous and wave-speed quirks (Q7, Q6).
  */
 #include "fiesta_hip.h"
 

@@ -500,3 +500,61 @@ def test_3d_shock_tube_against_exact_riemann_solution(pr):
     assert np.abs(p[plateau] / ex["p"] - 1).max() < 0.02
     b = outs[False]
     assert np.abs(b[:5] - a[:5]).max() <= 1e-8 * np.abs(a[:5]).max()
+
+
+@pytest.mark.parametrize("ndim", [3, 2])
+def test_ceq_artificial_viscosity_is_an_anisotropic_laplacian(ndim):
+    """The C-equation's artificial viscosity (3D: calculateCeqFaces, Grads and
+    applyCeq, src/ceq3d.hpp:226-366; 2D: computeCeqFlux2D, computeCeqFaces2D,
+    applyCeq2D, src/ceq.hpp:197-293) on uniform density, uniform C_hat and
+    direction, and quadratic velocities u_w = b_w + x^T H_w x / 2.  The face
+    differences are exact on quadratics, so the momentum RHS gains
+      alpha rho C_hat sum_{f,d} M_fd (H_w)_fd,
+      M = I - n n^T / (|n|^2 + 1e-6) (3D), t t^T / (|t|^2 + 1e-6) (2D),
+      alpha = (sum of d^2) / (C_hat + 1e-6) * cf.alpha.
+    A term on the wrong face, derivative direction, velocity component,
+    projector or sign fails; the difference of RHS with and without the
+    C-equation isolates the term (the flow fluxes do not read C)."""
+    rng = np.random.default_rng(7 + ndim)
+    d_ = (0.01, 0.012, 0.009)[:ndim]
+    chat, rho0, p0 = 0.7, 1.2, 1.0e5
+    nvec = np.array([0.3, -0.8, 0.5][:ndim]) if ndim == 3 else np.array([0.6, -0.8])
+    H = []
+    for _ in range(ndim):
+        a = rng.uniform(-2e4, 2e4, (ndim, ndim))
+        H.append(0.5 * (a + a.T))
+    b = rng.uniform(-30, 30, ndim)
+    outs = {}
+    for ceq in (True, False):
+        cfg = make_config(ndim=ndim, ns=1, nci=12, ncj=11, nck=10, dx=d_[0], dy=d_[1], dz=d_[-1] if ndim == 3 else 1.0,
+                          gamma=(1.4,), M=(0.02896,), mu=(0.0,), visc=False, ceq=ceq, buoyancy=False, kap=10.0,
+                          eps=5.0, alpha=20.0, bc=("outflow",) * 6, dt=1e-7)
+        d = Dims(cfg)
+        idx = np.meshgrid(*[np.arange(n) for n in ((d.ngk, d.ngj, d.ngi) if ndim == 3 else (d.ngj, d.ngi))],
+                          indexing="ij")
+        x = [(idx[-1 - a] - d.ng + 0.5) * d_[a] for a in range(ndim)]  # x, y (, z) of every cell
+        u = []
+        for w in range(ndim):
+            q = b[w] + sum(0.5 * H[w][f][e] * x[f] * x[e] for f in range(ndim) for e in range(ndim))
+            u.append(q.reshape((d.ngk, d.ngj, d.ngi)))
+        one = np.ones((d.ngk, d.ngj, d.ngi))
+        cv = [0.0 * one, chat * one] + [c * one for c in nvec] + ([0.0 * one] if ndim == 2 else [])
+        var = W.fill_conserved(cfg, [rho0 * one], u, p0 * one, cv if ceq else None, ghosts=True)
+        o = oracle.Oracle(cfg)
+        o.set(FST_VAR, var)
+        o.pre_sim()
+        o.compute()
+        outs[ceq] = (cfg, d, o.get(FST_DVAR))
+    cfg, d, on = outs[True]
+    off = outs[False][2]
+    nn = nvec @ nvec + 1e-6
+    M = (np.eye(ndim) - np.outer(nvec, nvec) / nn) if ndim == 3 else np.outer(nvec, nvec) / nn
+    alpha = sum(v * v for v in d_) / (chat + 1e-6) * cfg.alpha
+    g = d.ng + 2  # cells whose stencils stay clear of the boundary ghosts
+    inner = (slice(g, -g),) * ndim if ndim == 3 else (slice(0, 1), slice(g, -g), slice(g, -g))
+    for w in range(ndim):
+        want = alpha * rho0 * chat * float(np.sum(M * H[w]))
+        got = (on[w] - off[w])[inner]
+        scale = np.abs(off[w][inner]).max()
+        assert np.abs(got - want).max() <= 1e-9 * abs(want) + 1e-13 * scale, (w, want, got.min(), got.max())
+        assert abs(want) > 1.0
