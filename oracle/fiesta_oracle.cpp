@@ -30,8 +30,10 @@
  *                   alpha rho C_hat sum M_fd H_fd exactly
  *   full 2D/3D step Sod and planar 3D shock tubes against the exact Riemann
  *                   solution
- *   Parity unpinned (no independent check beyond the above): the values of
- *   the 2D viscous and wave-speed quirks (Q7, Q6).
+ *   2D quirks       fixed mu = 2.928e-5 and k = 0.026 (quadratic u, quadratic
+ *                   T), the x-face dudy index (linear v gives -mu s/(4 dy)),
+ *                   maxWaveSpeed2D's stride-2 table reads and |m| (uniform
+ *                   state: the C relaxation exposes maxS)
  */
 #include "fiesta_hip.h"
 

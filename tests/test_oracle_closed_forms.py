@@ -558,3 +558,95 @@ def test_ceq_artificial_viscosity_is_an_anisotropic_laplacian(ndim):
         scale = np.abs(off[w][inner]).max()
         assert np.abs(got - want).max() <= 1e-9 * abs(want) + 1e-13 * scale, (w, want, got.min(), got.max())
         assert abs(want) > 1.0
+
+
+# ------------------------------------------------------------------ 2D quirks (Q6, Q7)
+
+def _visc2d_delta(fields, mu_species=1.0e-3):
+    """RHS of one 2D compute() with the viscous terms minus without them.
+    fields(x, y) -> (rho, u, v, p) on every cell."""
+    outs = {}
+    for visc in (True, False):
+        cfg = make_config(ndim=2, ns=1, nci=14, ncj=12, dx=0.01, dy=0.012, gamma=(1.4,), M=(0.02896,),
+                          mu=(mu_species,), visc=visc, ceq=False, buoyancy=False, bc=("outflow",) * 6, dt=1e-7)
+        d = Dims(cfg)
+        jj, ii = np.meshgrid(np.arange(d.ngj), np.arange(d.ngi), indexing="ij")
+        x = (ii - d.ng + 0.5) * cfg.dx
+        y = (jj - d.ng + 0.5) * cfg.dy
+        rho, u, v, p = fields(x, y)
+        sh = (1, d.ngj, d.ngi)
+        var = W.fill_conserved(cfg, [np.broadcast_to(rho, sh)], [np.broadcast_to(u, sh), np.broadcast_to(v, sh)],
+                               np.broadcast_to(p, sh), ghosts=True)
+        o = oracle.Oracle(cfg)
+        o.set(FST_VAR, var)
+        o.pre_sim()
+        o.compute()
+        outs[visc] = o.get(FST_DVAR)
+    g = d.ng + 2
+    inner = (slice(0, 1), slice(g, -g), slice(g, -g))
+    return cfg, [(outs[True][w] - outs[False][w])[inner] for w in range(3)], y[g:-g, g:-g]
+
+
+def test_2d_viscous_uses_the_fixed_viscosity_and_conductivity():
+    """calculateStressTensor2dv / calculateHeatFlux2dv / applyViscousTerm2dv
+    (src/viscosity.hpp:25-199): mu = 2.928e-5 and k = 0.026 whatever the
+    species' mu (Q7).  u = a y^2: the x-momentum gains mu * 2a (the second
+    difference is exact); T = T0 + tau x^2 at rest: the energy gains k * 2 tau."""
+    a, mu = 1.0e3, 2.928e-5
+    rho0, p0 = 1.2, 1.0e5
+    cfg, dv, _ = _visc2d_delta(lambda x, y: (rho0, a * y * y, 0.0 * x, p0))
+    np.testing.assert_allclose(dv[0], 2 * a * mu, rtol=1e-8)
+    assert np.abs(dv[1]).max() <= 1e-12
+    T0, tau = 300.0, 1.0e3
+    R = cfg.R / cfg.M[0]
+    cfg, dv, _ = _visc2d_delta(lambda x, y: (rho0, 0.0 * x, 0.0 * x, rho0 * R * (T0 + tau * x * x)))
+    assert np.abs(dv[0]).max() <= 1e-12 and np.abs(dv[1]).max() <= 1e-12
+    np.testing.assert_allclose(dv[2], 2 * 0.026 * tau, rtol=1e-6)
+
+
+def test_2d_viscous_dudy_index_quirk():
+    """The x-face dudy of calculateStressTensor2dv reads vel(i, j-1, 1) (the v
+    component) where u belongs (Q7, src/viscosity.hpp:68).  With u = 0 and
+    v = V0 + s x, the correct stress has zero divergence; the reference's
+    gives the y-momentum -mu s / (4 dy) (a uniform-in-j shift of the x-face
+    shear stress by -v/(4 dy))."""
+    V0, s, mu = 5.0, 200.0, 2.928e-5
+    cfg, dv, _ = _visc2d_delta(lambda x, y: (1.2, 0.0 * x, V0 + s * x + 0.0 * y, 1.0e5))
+    np.testing.assert_allclose(dv[1], -mu * s / (4 * cfg.dy), rtol=1e-8)
+    assert np.abs(dv[0]).max() <= 1e-12
+
+
+def test_2d_wave_speed_quirk():
+    """maxWaveSpeed2D (src/ceq.hpp:43-78, Q6): species s reads gamma and R/M
+    from cd(6+2s), cd(7+2s) of the (gamma, R/M, mu) x ns table, and adds
+    |m| (momentum, not velocity).  On a uniform state the C relaxation
+    (updateCeq2D, src/ceq.hpp:153-195) gives dvar(C_n) = -maxS C0 / (eps |dx|)
+    exactly, which exposes maxS."""
+    g0, g1, M0, M1, mu0, mu1 = 1.4, 1.67, 0.02896, 0.004, 1.8e-5, 2.1e-5
+    cfg = make_config(ndim=2, ns=2, nci=10, ncj=9, dx=0.01, dy=0.012, gamma=(g0, g1), M=(M0, M1), mu=(mu0, mu1),
+                      visc=False, ceq=True, buoyancy=False, kap=10.0, eps=5.0, alpha=2.0, bc=("outflow",) * 6,
+                      dt=1e-7)
+    d = Dims(cfg)
+    sh = (1, d.ngj, d.ngi)
+    one = np.ones(sh)
+    r0, r1, u, v, p, C0 = 0.9, 0.15, 30.0, -10.0, 1.0e5, 0.5
+    var = W.fill_conserved(cfg, [r0 * one, r1 * one], [u * one, v * one], p * one, [C0 * one] * 5, ghosts=True)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.compute()
+    dv = o.get(FST_DVAR)[d.nv:d.nv + 4][(slice(None), 0) + d.interior()[2:]]
+    dxmag = np.sqrt(cfg.dx ** 2 + cfg.dy ** 2)
+    maxS = -dv * cfg.eps * dxmag / C0
+    rho = r0 + r1
+    table = [g0, cfg.R / M0, mu0, g1, cfg.R / M1, mu1]
+    gs, Rs = [table[2 * s] for s in range(2)], [table[2 * s + 1] for s in range(2)]  # the stride-2 reads
+    Y = [r0 / rho, r1 / rho]
+    cp = sum(Y[s] * gs[s] * Rs[s] / (gs[s] - 1) for s in range(2))
+    cv = sum(Y[s] * Rs[s] / (gs[s] - 1) for s in range(2))
+    want = np.sqrt(cp / cv * p / rho) + np.hypot(rho * u, rho * v)
+    np.testing.assert_allclose(maxS, want, rtol=1e-12)
+    # the textbook speed (right table entries, |u|) differs: the test can tell them apart
+    cpt = sum(Y[s] * (g0, g1)[s] * cfg.R / (M0, M1)[s] / ((g0, g1)[s] - 1) for s in range(2))
+    cvt = sum(Y[s] * cfg.R / (M0, M1)[s] / ((g0, g1)[s] - 1) for s in range(2))
+    assert abs(np.sqrt(cpt / cvt * p / rho) + np.hypot(u, v) - want) > 1e-3 * want
