@@ -47,8 +47,10 @@ def parse():
     ap.add_argument("--forms", default="fp64,fp32,fp32_learned,fp64_rk3,fp32_learned_rk3",
                     help="comma list of arithmetic forms to time: fp64 (reported as value), fp32, fp32_learned; "
                          "suffix _rk3 runs the SSP-RK3 option instead of the reference's RK2 midpoint")
+    ap.add_argument("--gen3d-size", type=int, nargs=3, default=[512, 512, 256],
+                    help="cells of the gen3d leg of --extras")
     ap.add_argument("--extras", action="store_true",
-                    help="also time postStep's noise filter and the device-side output pack (fst_pack) on the "
+                    help="also time postStep's noise filter, the device-side output pack (fst_pack) and the gen3d module on the "
                          "fp64 workload")
     ap.add_argument("--save-weights", default=None, help="write the trained network (.npy, float32)")
     ap.add_argument("--load-weights", default=None, help="use this network (.npy) instead of training one")
@@ -190,6 +192,43 @@ def extras_leg(args, var):
             "pack_bytes_to_host": int(d.nvt * d.ncells * 4),
             "note": "postStep = secondary variables + noise filter (3 variables, 1 sweep); pack = %d variables, "
                     "float32, whole interior, each copied to pageable host memory" % d.nvt}
+
+
+def gen3d_leg(args):
+    """The generalized-grid module (gen3d: EOS, contravariant velocity, WENO
+    fluxes, metric pressure gradient; src/gen3d.cpp:341-369) on the 3D
+    terrain workload, 512 x 512 x 256 cells, 3 species, FP64: EOS and
+    contravariant velocity, the fused advection kernel (fluxes of every
+    variable in one launch), the metric pressure gradient, the RK update.
+    Device ms per step from hipEvents over 3 steps after one warm-up step."""
+    from fiesta_amd import workloads as W
+    from fiesta_amd.config import FST_GRID, FST_VAR, Dims
+    from fiesta_amd.runtime import Module
+
+    n = tuple(args.gen3d_size)
+    cfg = W.gen3d_terrain_config(n=n, ns=3, dt=1e-7)
+    d = Dims(cfg)
+    m = Module(cfg)
+    m.upload(FST_GRID, W.terrain_grid(cfg))
+    m.upload(FST_VAR, W.random_state_all(cfg, seed=5, umag=20.0))
+    m.pre_sim()
+    m.step(1)
+    m.sync()
+    steps = 3
+    m.step(steps)
+    ms_step = m.last_step_ms() / steps
+    m.close()
+    cs = d.ncells * 2
+    balg = 20 * d.nvt          # U read/write of the RK2 stages
+    bmet = balg + 9 * 8        # plus one read of the 9 metric terms per stage
+    stage_ms = ms_step / 2
+    return {"workload": "gen3d terrain %dx%dx%d, 3 species, FP64 (fused advection + metric pressure gradient + RK kernels)" % n,
+            "ms_per_stage": round(stage_ms, 4), "value": round(cs / (ms_step * 1e-3) / 1e9, 4),
+            "unit": "Gcell-stage/s",
+            "roofline": {"bound": "hbm", "algorithmic_bytes_per_cell_stage": balg,
+                         "with_metrics_bytes_per_cell_stage": bmet,
+                         "achieved_GBps": round(d.ncells * balg / (stage_ms * 1e-3) / 1e9, 1),
+                         "frac": round(d.ncells * balg / (stage_ms * 1e-3) / 8.0e12, 4)}}
 
 
 def cpu_leg(args, cfg_size):
@@ -372,6 +411,8 @@ def main():
     if args.extras:
         log("extras: noise filter and output pack")
         out["extras"] = extras_leg(args, var)
+        log("extras: generalized-grid module (gen3d)")
+        out["extras"]["gen3d"] = gen3d_leg(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(args, args.size)
         out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 5)

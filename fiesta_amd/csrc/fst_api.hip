@@ -132,6 +132,10 @@ double *array_ptr(fst_ctx *ctx, int a, size_t *count) {
 }
 
 void free_all(fst_ctx *ctx) {
+  if (ctx->b.prim) {  // rho, p and vel live in this block
+    (void)hipFree(ctx->b.prim);
+    ctx->b.prim = ctx->b.rho = ctx->b.p = ctx->b.vel = nullptr;
+  }
   double **ps[] = {&ctx->b.var, &ctx->b.tmp1, &ctx->b.dvar, &ctx->b.varx, &ctx->b.p, &ctx->b.rho, &ctx->b.T,
                    &ctx->b.vel, &ctx->b.fx, &ctx->b.fy, &ctx->b.fz, &ctx->b.stress, &ctx->b.gradRho,
                    &ctx->b.mflux, &ctx->b.grid, &ctx->b.metrics, &ctx->b.ndn};
@@ -244,8 +248,18 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
   ok = ok && hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess;
   ok = ok && hipEventCreate(&ctx->ev0) == hipSuccess && hipEventCreate(&ctx->ev1) == hipSuccess;
   ok = ok && alloc(&ctx->b.var, g.nvt) && alloc(&ctx->b.tmp1, g.nvt) && alloc(&ctx->b.dvar, g.nvt);
-  ok = ok && alloc(&ctx->b.varx, g.nvx) && alloc(&ctx->b.p, 1) && alloc(&ctx->b.rho, 1) && alloc(&ctx->b.T, 1);
-  ok = ok && alloc(&ctx->b.vel, g.ndim) && alloc(&ctx->b.fx, 1) && alloc(&ctx->b.fy, 1);
+  if (gen_adv_supported(c, g)) {  // gen3d: the fused advection reads rho, p, vel as one block of planes
+    ok = ok && alloc(&ctx->b.prim, 5);
+    if (ok) {
+      ctx->b.rho = ctx->b.prim;
+      ctx->b.p = ctx->b.prim + g.N;
+      ctx->b.vel = ctx->b.prim + 2 * g.N;
+    }
+  } else {
+    ok = ok && alloc(&ctx->b.p, 1) && alloc(&ctx->b.rho, 1) && alloc(&ctx->b.vel, g.ndim);
+  }
+  ok = ok && alloc(&ctx->b.varx, g.nvx) && alloc(&ctx->b.T, 1);
+  ok = ok && alloc(&ctx->b.fx, 1) && alloc(&ctx->b.fy, 1);
   ok = ok && alloc(&ctx->b.fz, g.ndim == 3 ? 1 : 0);
   ok = ok && alloc(&ctx->b.stress, c.visc ? (g.ndim == 3 ? 6 : 8) : 0);
   const bool ceq_vars = c.ceq && !c.learned_c;

@@ -575,7 +575,10 @@ __device__ __forceinline__ void face_adv_lds(const FC &k, const double *qs, int 
   }
 }
 
-template <int NS, bool F32>
+// PG = false (the generalized-grid module): fluxes and their divergence only;
+// gen3d adds its metric pressure gradient in a kernel of its own
+// (applyGenPressureGradient3D) and has no buoyancy.
+template <int NS, bool F32, bool PG = true>
 __global__ void __launch_bounds__(NTHREADS, 1)
     k_adv3d_lds(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
                 double *__restrict__ dvar, int buoy, int kchunk, int nbx, int nby, int nbz) {
@@ -644,6 +647,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
 #pragma unroll
       for (int v = 0; v < NV; ++v)
         dv[v] = -((xsh[w][v][l + 1] - xsh[w][v][l]) + (ysh[w + 1][v][l] - ysh[w][v][l]) + (Z[v] - Zp[v]));
+      if constexpr (PG) {
       const double *pp = qs + PF::FP * QPL + rc * QX + qc;
       const double *Pp = F32 ? P : P + N;  // pressure plane (FP64 in every form)
       const uint32_t sz = g.s2;
@@ -654,7 +658,8 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       dv[0] = dv[0] - dxp;
       dv[1] = dv[1] - dyp;
       dv[2] = dv[2] - dzp;
-      if (buoy) {
+      }
+      if (PG && buoy) {
         const double rho = pget<F32>(P, N, 0, c);
         const double vv = qs[1 * QPL + rc * QX + qc] / rho;
         const double rhop = rho - k.rhoRef;
@@ -1864,6 +1869,32 @@ bool pick(int ns, bool f32, bool lc, StageFn *sf, PrimFn *pf) {
 }
 
 }  // namespace
+
+bool gen_adv_supported(const fst_config &cf, const Geo &g) {
+  // 32-bit element offsets: the largest array (nvt planes, or the 5 primitive
+  // planes) below 2^29 elements
+  return cf.ndim == 3 && cf.grid == 1 && cf.ns >= 1 && cf.ns <= 3 && cf.scheme == FST_WENO5 &&
+         (double)g.N * std::max(g.nvt, 5) < 536870912.0;
+}
+
+// gen3d_func::compute's fluxes and their divergence (calculateFluxesG +
+// advect3D for every flow variable, src/gen3d.cpp:352-362) in one launch of
+// the fused advection kernel: P holds rho, p and the contravariant velocity
+// (planes 0, 1, 2..4), the cell sizes are 1, and the metric pressure gradient
+// is left to its own kernel.  The same operations as k_flux3d + k_advect3d,
+// so the result is bit for bit the reference sequence's.
+void launch_gen_adv(const Geo &g, const Phys &ph, const fst_config &cf, const double *U, const double *P,
+                    double *dvar, hipStream_t s) {
+  const Grid32 q = make_grid(g);
+  const FC k = make_fc(ph, cf);
+  constexpr int kchunk = 8;
+  const Launch L = shape(g, kchunk);
+  switch (cf.ns) {
+    case 1: k_adv3d_lds<1, false, false><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, 0, kchunk, L.nbx, L.nby, L.nbz); break;
+    case 2: k_adv3d_lds<2, false, false><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, 0, kchunk, L.nbx, L.nby, L.nbz); break;
+    default: k_adv3d_lds<3, false, false><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, 0, kchunk, L.nbx, L.nby, L.nbz); break;
+  }
+}
 
 bool fused_supported(const fst_config &cf) {
   if (!(cf.ndim == 3 && cf.grid == 0 && (cf.precision == FST_FP64 || cf.precision == FST_FP32_BULK) &&

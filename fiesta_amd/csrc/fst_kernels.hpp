@@ -64,6 +64,7 @@ struct Bufs {
   double *ndn = nullptr;
   unsigned char *nflag = nullptr;
   unsigned long long *nmax = nullptr;
+  double *prim = nullptr;  // gen3d: rho, p, vel[3] in one 5-plane block (rho, p, vel point into it)
   unsigned long long *mm = nullptr;  // status check: (max, min) order keys per variable
   Scalars *sc = nullptr;     // per-stage maxima (device)
   Prof *prof = nullptr;      // timing of the dominant kernel (host side)
