@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--gen3d-size", type=int, nargs=3, default=[512, 512, 256],
                     help="cells of the gen3d leg of --extras")
     ap.add_argument("--extras", action="store_true",
-                    help="also time postStep's noise filter, the device-side output pack (fst_pack) and the gen3d module on the "
+                    help="also time postStep's noise filter, the device-side output pack (fst_pack), the gen3d module and the 2D expansion on the "
                          "fp64 workload")
     ap.add_argument("--save-weights", default=None, help="write the trained network (.npy, float32)")
     ap.add_argument("--load-weights", default=None, help="use this network (.npy) instead of training one")
@@ -227,6 +227,39 @@ def gen3d_leg(args):
             "unit": "Gcell-stage/s",
             "roofline": {"bound": "hbm", "algorithmic_bytes_per_cell_stage": balg,
                          "with_metrics_bytes_per_cell_stage": bmet,
+                         "achieved_GBps": round(d.ncells * balg / (stage_ms * 1e-3) / 1e9, 1),
+                         "frac": round(d.ncells * balg / (stage_ms * 1e-3) / 8.0e12, 4)}}
+
+
+def expansion2d_leg(args):
+    """The 2D module (cart2d: BCs once per step, then per stage EOS, WENO5,
+    pressure gradient, viscous terms and the C-equation; src/cart2d.cpp:
+    136-237) on the 2D ideal expansion, 1024 x 1024 cells, 2 species,
+    viscous, C-equation, SSP-RK3 (3 stages per step), dt from CFL 0.5; the
+    reference-sequence kernels.  Device ms per step from hipEvents over 20
+    steps after 2 warm-up steps."""
+    from fiesta_amd import workloads as W
+    from fiesta_amd.config import FST_VAR, Dims
+    from fiesta_amd.runtime import Module
+
+    cfg = W.expansion2d_config(1024, ceq=True, visc=True, ns=2)
+    cfg.rk_order = 3
+    d = Dims(cfg)
+    m = Module(cfg)
+    m.upload(FST_VAR, W.expansion2d_state(cfg))
+    m.pre_sim()
+    m.step(2)
+    m.sync()
+    steps = 20
+    m.step(steps)
+    ms_step = m.last_step_ms() / steps
+    m.close()
+    stage_ms = ms_step / 3
+    balg = 64 * d.nvt // 3  # SSP-RK3: (2 + 3 + 3) / 3 x 8 nvt bytes per cell-stage
+    return {"workload": "cart2d expansion 1024x1024, 2 species, viscous, C-equation, SSP-RK3, FP64",
+            "ms_per_stage": round(stage_ms, 4), "value": round(d.ncells / (stage_ms * 1e-3) / 1e9, 4),
+            "unit": "Gcell-stage/s",
+            "roofline": {"bound": "hbm", "algorithmic_bytes_per_cell_stage": balg,
                          "achieved_GBps": round(d.ncells * balg / (stage_ms * 1e-3) / 1e9, 1),
                          "frac": round(d.ncells * balg / (stage_ms * 1e-3) / 8.0e12, 4)}}
 
@@ -413,6 +446,8 @@ def main():
         out["extras"] = extras_leg(args, var)
         log("extras: generalized-grid module (gen3d)")
         out["extras"]["gen3d"] = gen3d_leg(args)
+        log("extras: 2D module (expansion)")
+        out["extras"]["cart2d_expansion"] = expansion2d_leg(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(args, args.size)
         out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 5)
