@@ -381,7 +381,7 @@ def main():
                      "viscous_rk_kernel", "ceq_kernel")
             roof["phases_ms"] = {n: round(v / r["launches"], 4) for n, v in zip(names, ph)}
             ns = r["cfg"].ns
-            kern = {2: "k_adv3d_lds<%d, false>" % ns, 3: "k_vcr3d_ring<%d, false, false, false, false>" % ns,
+            kern = {2: "k_adv3d_lds<%d, false, true>" % ns, 3: "k_vcr3d_ring<%d, false, false, false, false>" % ns,
                     4: "k_ceq3d<false, false>"}
             top = max(kern, key=lambda q: ph[q])
             roof["dominant_kernel"] = {"name": kern[top], "avg_ms": round(ph[top] / r["launches"], 4),

@@ -647,18 +647,20 @@ __global__ void __launch_bounds__(NTHREADS, 1)
 #pragma unroll
       for (int v = 0; v < NV; ++v)
         dv[v] = -((xsh[w][v][l + 1] - xsh[w][v][l]) + (ysh[w + 1][v][l] - ysh[w][v][l]) + (Z[v] - Zp[v]));
-      if constexpr (PG) {
       const double *pp = qs + PF::FP * QPL + rc * QX + qc;
       const double *Pp = F32 ? P : P + N;  // pressure plane (FP64 in every form)
       const uint32_t sz = g.s2;
-      const double dxp = cdiv(pp[-2] - 8.0 * pp[-1] + 8.0 * pp[1] - pp[2], k.pg[0], k.rpg[0]);
-      const double dyp = cdiv(pp[-2 * QX] - 8.0 * pp[-QX] + 8.0 * pp[QX] - pp[2 * QX], k.pg[1], k.rpg[1]);
-      const double dzp = cdiv(ld(Pp, c - 2 * sz) - 8.0 * ld(Pp, c - sz) + 8.0 * ld(Pp, c + sz) - ld(Pp, c + 2 * sz),
-                              k.pg[2], k.rpg[2]);
+      // !PG: the terms are +0, and x - 0.0 is x bit for bit
+      const double dxp = !PG ? 0.0 : cdiv(pp[-2] - 8.0 * pp[-1] + 8.0 * pp[1] - pp[2], k.pg[0], k.rpg[0]);
+      const double dyp =
+          !PG ? 0.0 : cdiv(pp[-2 * QX] - 8.0 * pp[-QX] + 8.0 * pp[QX] - pp[2 * QX], k.pg[1], k.rpg[1]);
+      const double dzp =
+          !PG ? 0.0
+              : cdiv(ld(Pp, c - 2 * sz) - 8.0 * ld(Pp, c - sz) + 8.0 * ld(Pp, c + sz) - ld(Pp, c + 2 * sz), k.pg[2],
+                     k.rpg[2]);
       dv[0] = dv[0] - dxp;
       dv[1] = dv[1] - dyp;
       dv[2] = dv[2] - dzp;
-      }
       if (PG && buoy) {
         const double rho = pget<F32>(P, N, 0, c);
         const double vv = qs[1 * QPL + rc * QX + qc] / rho;
@@ -1776,6 +1778,8 @@ Launch shape(const Geo &g, int kchunk) {
   return L;
 }
 
+#ifndef FST_GEN3D_TU  // the cartesian stage drivers
+
 // fs: the stage state U* is float (learned FP32 form): stage 1 writes it,
 // stage 2 reads it; U^n and U^{n+1} stay FP64.
 template <int NS, bool F32, bool LC>
@@ -1868,15 +1872,11 @@ bool pick(int ns, bool f32, bool lc, StageFn *sf, PrimFn *pf) {
   }
 }
 
+#endif  // FST_GEN3D_TU
+
 }  // namespace
 
-bool gen_adv_supported(const fst_config &cf, const Geo &g) {
-  // 32-bit element offsets: the largest array (nvt planes, or the 5 primitive
-  // planes) below 2^29 elements
-  return cf.ndim == 3 && cf.grid == 1 && cf.ns >= 1 && cf.ns <= 3 && cf.scheme == FST_WENO5 &&
-         (double)g.N * std::max(g.nvt, 5) < 536870912.0;
-}
-
+#ifdef FST_GEN3D_TU
 // gen3d_func::compute's fluxes and their divergence (calculateFluxesG +
 // advect3D for every flow variable, src/gen3d.cpp:352-362) in one launch of
 // the fused advection kernel: P holds rho, p and the contravariant velocity
@@ -1894,6 +1894,14 @@ void launch_gen_adv(const Geo &g, const Phys &ph, const fst_config &cf, const do
     case 2: k_adv3d_lds<2, false, false><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, 0, kchunk, L.nbx, L.nby, L.nbz); break;
     default: k_adv3d_lds<3, false, false><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, 0, kchunk, L.nbx, L.nby, L.nbz); break;
   }
+}
+
+#else
+bool gen_adv_supported(const fst_config &cf, const Geo &g) {
+  // 32-bit element offsets: the largest array (nvt planes, or the 5 primitive
+  // planes) below 2^29 elements
+  return cf.ndim == 3 && cf.grid == 1 && cf.ns >= 1 && cf.ns <= 3 && cf.scheme == FST_WENO5 &&
+         (double)g.N * std::max(g.nvt, 5) < 536870912.0;
 }
 
 bool fused_supported(const fst_config &cf) {
@@ -2029,5 +2037,7 @@ void fused_free(FusedBufs &fb) {
   fb.stage = fb.stage2 = fb.prim = nullptr;
   fb.sc = nullptr;
 }
+
+#endif  // FST_GEN3D_TU
 
 }  // namespace fst
