@@ -20,11 +20,12 @@ struct FusedBufs {
 struct LearnedBufs;
 
 bool fused_supported(const fst_config &cf);
-// gen3d's advection on the fused kernel (rho, p, contravariant velocity in
-// the 5 planes of P); gen_adv_supported says whether the configuration fits.
+// gen3d's advection and metric pressure gradient on the fused kernel (rho, p,
+// contravariant velocity in the 5 planes of P, metric terms MT);
+// gen_adv_supported says whether the configuration fits.
 bool gen_adv_supported(const fst_config &cf, const Geo &g);
 void launch_gen_adv(const Geo &g, const Phys &ph, const fst_config &cf, const double *U, const double *P,
-                    double *dvar, hipStream_t s);
+                    const double *MT, double *dvar, hipStream_t s);
 // nsteps time steps of the fused path (RK2 midpoint, or SSP-RK3 with
 // rk_order = 3); lb carries the learned C-indicator fields when cf.learned_c
 // is set (else may be null).

@@ -575,13 +575,13 @@ __device__ __forceinline__ void face_adv_lds(const FC &k, const double *qs, int 
   }
 }
 
-// PG = false (the generalized-grid module): fluxes and their divergence only;
-// gen3d adds its metric pressure gradient in a kernel of its own
-// (applyGenPressureGradient3D) and has no buoyancy.
-template <int NS, bool F32, bool PG = true>
-__global__ void __launch_bounds__(NTHREADS, 1)
-    k_adv3d_lds(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
-                double *__restrict__ dvar, int buoy, int kchunk, int nbx, int nby, int nbz) {
+// PG = false (the generalized-grid module): the fluxes, their divergence and
+// applyGenPressureGradient3D (src/presgrad.hpp:72-147) on the metric terms MT
+// (9 planes, m(i,j,k,a,n) at MT + N (a + 3 n)); gen3d has no buoyancy.
+template <int NS, bool F32, bool PG>
+__device__ __forceinline__ void adv_tile(const Grid32 &g, const FC &k, const double *__restrict__ U,
+                                         const double *__restrict__ P, double *__restrict__ dvar, int buoy,
+                                         int kchunk, int nbx, int nby, int nbz, const double *__restrict__ MT) {
   using PF = PlaneF<NS>;
   constexpr int NV = 4 + NS;
   __shared__ double qs[PF::NQ * QPL];
@@ -650,17 +650,35 @@ __global__ void __launch_bounds__(NTHREADS, 1)
       const double *pp = qs + PF::FP * QPL + rc * QX + qc;
       const double *Pp = F32 ? P : P + N;  // pressure plane (FP64 in every form)
       const uint32_t sz = g.s2;
-      // !PG: the terms are +0, and x - 0.0 is x bit for bit
-      const double dxp = !PG ? 0.0 : cdiv(pp[-2] - 8.0 * pp[-1] + 8.0 * pp[1] - pp[2], k.pg[0], k.rpg[0]);
-      const double dyp =
-          !PG ? 0.0 : cdiv(pp[-2 * QX] - 8.0 * pp[-QX] + 8.0 * pp[QX] - pp[2 * QX], k.pg[1], k.rpg[1]);
-      const double dzp =
-          !PG ? 0.0
-              : cdiv(ld(Pp, c - 2 * sz) - 8.0 * ld(Pp, c - sz) + 8.0 * ld(Pp, c + sz) - ld(Pp, c + 2 * sz), k.pg[2],
-                     k.rpg[2]);
-      dv[0] = dv[0] - dxp;
-      dv[1] = dv[1] - dyp;
-      dv[2] = dv[2] - dzp;
+      if constexpr (PG) {
+        const double dxp = cdiv(pp[-2] - 8.0 * pp[-1] + 8.0 * pp[1] - pp[2], k.pg[0], k.rpg[0]);
+        const double dyp = cdiv(pp[-2 * QX] - 8.0 * pp[-QX] + 8.0 * pp[QX] - pp[2 * QX], k.pg[1], k.rpg[1]);
+        const double dzp = cdiv(ld(Pp, c - 2 * sz) - 8.0 * ld(Pp, c - sz) + 8.0 * ld(Pp, c + sz) - ld(Pp, c + 2 * sz),
+                                k.pg[2], k.rpg[2]);
+        dv[0] = dv[0] - dxp;
+        dv[1] = dv[1] - dyp;
+        dv[2] = dv[2] - dzp;
+      } else {
+        // metric pressure gradient: 4th-order differences of p m(.,a,n) along
+        // xi, eta, zeta, divided by 12 (IEEE: cdiv with RN(1/12))
+        const uint32_t sy = g.s1;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+          const double *m0 = MT + (size_t)N * (size_t)(0 + 3 * n);
+          const double *m1 = MT + (size_t)N * (size_t)(1 + 3 * n);
+          const double *m2 = MT + (size_t)N * (size_t)(2 + 3 * n);
+          const double dxi = cdiv(pp[-2] * ld(m0, c - 2) - 8.0 * pp[-1] * ld(m0, c - 1) + 8.0 * pp[1] * ld(m0, c + 1) -
+                                      pp[2] * ld(m0, c + 2),
+                                  12.0, k.r12);
+          const double det = cdiv(pp[-2 * QX] * ld(m1, c - 2 * sy) - 8.0 * pp[-QX] * ld(m1, c - sy) +
+                                      8.0 * pp[QX] * ld(m1, c + sy) - pp[2 * QX] * ld(m1, c + 2 * sy),
+                                  12.0, k.r12);
+          const double dzt = cdiv(ld(Pp, c - 2 * sz) * ld(m2, c - 2 * sz) - 8.0 * ld(Pp, c - sz) * ld(m2, c - sz) +
+                                      8.0 * ld(Pp, c + sz) * ld(m2, c + sz) - ld(Pp, c + 2 * sz) * ld(m2, c + 2 * sz),
+                                  12.0, k.r12);
+          dv[n] = dv[n] - (dxi + det + dzt);
+        }
+      }
       if (PG && buoy) {
         const double rho = pget<F32>(P, N, 0, c);
         const double vv = qs[1 * QPL + rc * QX + qc] / rho;
@@ -676,6 +694,22 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     for (int q = 0; q < NV; ++q) Zp[q] = Z[q];
     __syncthreads();
   }
+}
+
+template <int NS, bool F32, bool PG = true>
+__global__ void __launch_bounds__(NTHREADS, 1)
+    k_adv3d_lds(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
+                double *__restrict__ dvar, int buoy, int kchunk, int nbx, int nby, int nbz) {
+  adv_tile<NS, F32, PG>(g, k, U, P, dvar, buoy, kchunk, nbx, nby, nbz, nullptr);
+}
+
+// gen3d_func::compute's fluxes, their divergence and the metric pressure
+// gradient (src/gen3d.cpp:352-366) in one launch
+template <int NS>
+__global__ void __launch_bounds__(NTHREADS, 1)
+    k_adv3d_gen(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
+                double *__restrict__ dvar, const double *__restrict__ MT, int kchunk, int nbx, int nby, int nbz) {
+  adv_tile<NS, false, false>(g, k, U, P, dvar, 0, kchunk, nbx, nby, nbz, MT);
 }
 
 // ---------------------------------------------------------------- kernel 1, FP32 forms, float plane
@@ -1877,22 +1911,23 @@ bool pick(int ns, bool f32, bool lc, StageFn *sf, PrimFn *pf) {
 }  // namespace
 
 #ifdef FST_GEN3D_TU
-// gen3d_func::compute's fluxes and their divergence (calculateFluxesG +
-// advect3D for every flow variable, src/gen3d.cpp:352-362) in one launch of
-// the fused advection kernel: P holds rho, p and the contravariant velocity
-// (planes 0, 1, 2..4), the cell sizes are 1, and the metric pressure gradient
-// is left to its own kernel.  The same operations as k_flux3d + k_advect3d,
-// so the result is bit for bit the reference sequence's.
+// gen3d_func::compute's fluxes, their divergence and the metric pressure
+// gradient (calculateFluxesG + advect3D for every flow variable, then
+// applyGenPressureGradient3D; src/gen3d.cpp:352-366) in one launch: P holds
+// rho, p and the contravariant velocity (planes 0, 1, 2..4), MT the 9 metric
+// planes, the cell sizes are 1.  The same operations as k_flux3d +
+// k_advect3d + k_genpresgrad3d, so the result is bit for bit the reference
+// sequence's.
 void launch_gen_adv(const Geo &g, const Phys &ph, const fst_config &cf, const double *U, const double *P,
-                    double *dvar, hipStream_t s) {
+                    const double *MT, double *dvar, hipStream_t s) {
   const Grid32 q = make_grid(g);
   const FC k = make_fc(ph, cf);
   constexpr int kchunk = 8;
   const Launch L = shape(g, kchunk);
   switch (cf.ns) {
-    case 1: k_adv3d_lds<1, false, false><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, 0, kchunk, L.nbx, L.nby, L.nbz); break;
-    case 2: k_adv3d_lds<2, false, false><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, 0, kchunk, L.nbx, L.nby, L.nbz); break;
-    default: k_adv3d_lds<3, false, false><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, 0, kchunk, L.nbx, L.nby, L.nbz); break;
+    case 1: k_adv3d_gen<1><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, MT, kchunk, L.nbx, L.nby, L.nbz); break;
+    case 2: k_adv3d_gen<2><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, MT, kchunk, L.nbx, L.nby, L.nbz); break;
+    default: k_adv3d_gen<3><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, MT, kchunk, L.nbx, L.nby, L.nbz); break;
   }
 }
 

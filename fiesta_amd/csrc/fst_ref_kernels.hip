@@ -1269,11 +1269,10 @@ void launch_compute_gen(const Geo &g, const Phys &ph, const fst_config &cf, cons
   launch_eos_vel(g, ph, b, s);
   Range gr = ghost_range(g), cr = cell_range(g), fr = face_range(g);
   k_genvel<<<grid_for(gr), 256, 0, s>>>(g, gr, b.var, b.rho, b.metrics, b.vel);
-  if (g.ndim == 3 && b.prim) {  // fluxes + divergence of every variable in one fused launch
-    if (b.prof) { b.prof->name = "k_adv3d_lds (gen3d)"; b.prof->per = 2; b.prof->begin(s); }
-    launch_gen_adv(g, ph, cf, b.var, b.prim, b.dvar, s);
+  if (g.ndim == 3 && b.prim) {  // fluxes + divergence + metric pressure gradient in one fused launch
+    if (b.prof) { b.prof->name = "k_adv3d_gen"; b.prof->per = 2; b.prof->begin(s); }
+    launch_gen_adv(g, ph, cf, b.var, b.prim, b.metrics, b.dvar, s);
     if (b.prof) b.prof->end(s);
-    k_genpresgrad3d<<<grid_for(cr), 256, 0, s>>>(g, cr, b.p, b.metrics, b.dvar);
   } else if (g.ndim == 3) {
     Phys pg = ph;
     pg.dz = ph.dx;
