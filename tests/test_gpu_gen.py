@@ -30,8 +30,8 @@ def _say(*a):
 
 def _cfg(ndim, n, ns, scheme="weno5", dt=2e-7):
     return make_config(ndim=ndim, ns=ns, nci=n[0], ncj=n[1], nck=n[2], grid=1, R=RU,
-                       gamma=(1.4, 1.67, 1.092)[:ns], M=(0.02896, 0.004, 0.14606)[:ns],
-                       mu=(1.8e-5, 2.1e-5, 1.5e-5)[:ns], scheme=scheme, dt=dt)
+                       gamma=(1.4, 1.67, 1.092, 1.3)[:ns], M=(0.02896, 0.004, 0.14606, 0.044)[:ns],
+                       mu=(1.8e-5, 2.1e-5, 1.5e-5, 1.4e-5)[:ns], scheme=scheme, dt=dt)
 
 
 def _wavy2d(cfg):
@@ -44,6 +44,8 @@ def _wavy2d(cfg):
 CASES = {
     "gen3d_terrain_ns2": dict(ndim=3, n=(20, 14, 11), ns=2, nodes="terrain"),
     "gen3d_terrain_ns3_ragged": dict(ndim=3, n=(67, 9, 13), ns=3, nodes="terrain"),
+    # 4 species: outside the fused kernel's range, the per-variable flux kernels
+    "gen3d_terrain_ns4_per_variable": dict(ndim=3, n=(18, 11, 9), ns=4, nodes="terrain"),
     "gen2d_terrain_weno": dict(ndim=2, n=(24, 18, 1), ns=2, nodes="terrain"),
     "gen2d_wavy_quick": dict(ndim=2, n=(19, 23, 1), ns=1, nodes="wavy", scheme="quick"),
     "gen2d_wavy_centered4": dict(ndim=2, n=(16, 16, 1), ns=3, nodes="wavy", scheme="centered4"),
