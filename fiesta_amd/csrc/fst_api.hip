@@ -65,6 +65,10 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
     *why = "grid: 0 (cartesian) or 1 (generalized); the terrain grid (2) has no module in the reference";
     return FST_ERR_ARG;
   }
+  if (c.precision != FST_FP64 && c.precision != FST_FP32_BULK) {
+    *why = "precision: 0 (FP64) or 1 (FP32 bulk, FP64 accumulation)";
+    return FST_ERR_ARG;
+  }
   if (c.grid == 1 && (c.learned_c || c.precision != FST_FP64)) {
     *why = "generalized grids run the reference arithmetic (FP64, no learned C-indicator)";
     return FST_ERR_ARG;
@@ -88,6 +92,11 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
   const int bcs[6] = {c.bcL, c.bcR, c.bcB, c.bcT, c.bcH, c.bcF};
   for (int q = 0; q < 6; ++q)
     if (bcs[q] < 0 || bcs[q] > 3) { *why = "unknown boundary type"; return FST_ERR_ARG; }
+  if (c.precision == FST_FP32_BULK && !fused_supported(c)) {
+    *why = "the FP32 precision form runs on the fused 3D stage kernels only (cartesian, WENO5, ns <= 3, every "
+           "array below 2^29 elements)";
+    return FST_ERR_ARG;
+  }
   d->ngi = c.nci + 2 * c.ng;
   d->ngj = c.ncj + 2 * c.ng;
   d->ngk = c.ndim == 3 ? c.nck + 2 * c.ng : 1;
@@ -165,6 +174,12 @@ void compute_seq(fst_ctx *ctx) {
 
 int learned_only(fst_ctx *ctx, const char *what) {
   return fail(ctx, FST_ERR_ARG, std::string(what) + ": the learned C-indicator runs through fst_step only");
+}
+
+// The FP32 forms exist only in the fused stage kernels; the per-kernel
+// sequence computes in FP64 whatever the precision field says.
+int fp32_only(fst_ctx *ctx, const char *what) {
+  return fail(ctx, FST_ERR_ARG, std::string(what) + ": the FP32 precision form runs through fst_step only");
 }
 
 }  // namespace
@@ -372,6 +387,7 @@ int fst_pre_step(fst_ctx *ctx) {
 int fst_compute(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
   if (ctx->cf.learned_c) return learned_only(ctx, "fst_compute");
+  if (ctx->cf.precision == FST_FP32_BULK) return fp32_only(ctx, "fst_compute");
   ctx->fb.valid = false;
   compute_seq(ctx);
   return check_launch(ctx, "fst_compute");
@@ -399,6 +415,7 @@ int fst_set_fused(fst_ctx *ctx, int fused) {
   if (fused && !fused_supported(ctx->cf))
     return fail(ctx, FST_ERR_ARG, "fused path supports 3D cartesian FP64 with ceq, visc and buoyancy flags only");
   if (!fused && ctx->cf.learned_c) return learned_only(ctx, "fst_set_fused(0)");
+  if (!fused && ctx->cf.precision == FST_FP32_BULK) return fp32_only(ctx, "fst_set_fused(0)");
   ctx->fused = fused ? 1 : 0;
   return FST_OK;
 }
@@ -699,8 +716,10 @@ int fst_mlp_get_weights(fst_mlp *m, float *w, size_t n) {
 int fst_mlp_harvest(fst_mlp *m, fst_ctx *ctx, uint32_t stride, int64_t max_rows, int64_t *added) {
   if (!m || !ctx) return FST_ERR_ARG;
   const fst_config &c = ctx->cf;
-  if (c.ndim != 3 || !c.ceq || c.learned_c || c.ns > 3)
-    return mfail(m, FST_ERR_ARG, "fst_mlp_harvest: needs a 3D context with the C-equation (ns <= 3)");
+  if (c.ndim != 3 || c.grid != 0 || !c.ceq || c.learned_c || c.ns > 3 || !fused_supported(c))
+    return mfail(m, FST_ERR_ARG,
+                 "fst_mlp_harvest: needs a 3D cartesian context with the C-equation (ns <= 3, every array "
+                 "below 2^29 elements)");
   const long long np = 6 + c.ns;
   if (m->t.Pcells != ctx->g.N || m->t.Pplanes != np) {
     if (m->t.P) (void)hipFree(m->t.P);

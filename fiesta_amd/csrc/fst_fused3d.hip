@@ -2057,7 +2057,10 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
 int fused_primitives(const Geo &g, const Phys &ph, const fst_config &cf, const double *U, double *P, hipStream_t s) {
   StageFn sf;
   PrimFn pf;
-  if (!pick(cf.ns, false, false, &sf, &pf) || g.ndim != 3) return FST_ERR_ARG;
+  // 32-bit element offsets in k_prims3d: 6 + ns planes of N cells below 2^29
+  if (!pick(cf.ns, false, false, &sf, &pf) || g.ndim != 3 || cf.grid != 0 ||
+      (double)g.N * (6 + cf.ns) >= 536870912.0)
+    return FST_ERR_ARG;
   const FC k = make_fc(ph, cf);
   const Grid32 q = make_grid(g);
   pf(q, k, U, P, nullptr, 0, nullptr, s, false);

@@ -9,8 +9,9 @@
 // doubles; required when CONFIG.grid = 1 without --restart-in), OUT receives
 // var after NSTEPS steps.  "rk" runs the module sequence of Fiesta::step
 // (src/fiesta.cpp:333-340): preStep, rkAdvance (src/rk.cpp:28-72), postStep;
-// "fused" runs fst_step (then one postStep when files are written, as the
-// output step's postStep computes varx, src/cart3d.cpp:300-317).
+// "fused" runs fst_step (with the noise filter on, one step and one postStep
+// at a time; otherwise all steps at once and one postStep when files are
+// written, as the output step's postStep computes varx, src/cart3d.cpp:300-317).
 //
 // Options (the deck keys of src/input.cpp:130-207 and src/luaReader.cpp:208-312):
 //   --species A,B,..          species names (variable names "<A> Density")
@@ -191,7 +192,12 @@ int main(int argc, char **argv) {
       }
     }
     f->preSim();
-    if (mode == "fused") fst_set_fused(f->context(), 1);
+    if (mode == "fused") {
+      if (int rc = fst_set_fused(f->context(), 1)) {
+        std::fprintf(stderr, "fiesta_run: fused mode: %s (code %d); use \"rk\"\n", fst_last_error(f->context()), rc);
+        return 2;
+      }
+    }
     f->sync();
     const auto t0 = std::chrono::steady_clock::now();
     if (mode == "rk") {
@@ -200,13 +206,21 @@ int main(int argc, char **argv) {
         rkAdvance(*f);
         f->postStep();
       }
+    } else if (cf.noise) {
+      // the noise filter is part of every step's postStep (Fiesta::step,
+      // src/fiesta.cpp:333-340), so the fused steps go one at a time
+      for (int t = 0; t < nsteps; ++t) {
+        step(*f, 1);
+        f->postStep();
+      }
     } else {
       step(*f, nsteps);
     }
     f->sync();
     const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     const bool files = !writers.empty() || !rpath.empty();
-    if (mode == "fused" && (files || status)) f->postStep();
+    // without the filter postStep only computes varx, which the output step needs
+    if (mode == "fused" && !cf.noise && (files || status)) f->postStep();
     // checkIO at the last step (src/fiesta.cpp:194-272): the restart, then the views
     const int tdx = tstart + nsteps;
     const double time = time0 + nsteps * cf.dt;

@@ -180,10 +180,11 @@ int fst_rk_stage(fst_ctx *ctx, int stage);
  * the sweep. */
 int fst_post_step(fst_ctx *ctx);
 
-/* `nsteps` full time steps: Fiesta::step (src/fiesta.cpp:333-340) =
- * preStep + rkAdvance (rk_order stages of compute() + rk_stage), on the fused
- * fast path (one RHS+RK kernel per stage).  Results equal the
- * compute()/rk_stage() sequence. */
+/* `nsteps` time steps of preStep + rkAdvance (rk_order stages of compute()
+ * + rk_stage) on the fused fast path (one RHS+RK kernel per stage).  Results
+ * equal the compute()/rk_stage() sequence.  Fiesta::step
+ * (src/fiesta.cpp:333-340) also runs postStep after every step: with the
+ * noise filter on, call fst_step(ctx, 1) and fst_post_step(ctx) per step. */
 int fst_step(fst_ctx *ctx, int nsteps);
 
 /* Per-stage global maxima of the C-equation (max wave speed, max |C_hat|)

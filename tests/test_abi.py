@@ -114,3 +114,34 @@ def test_learned_entry_points_reject_null_arguments():
     assert lib.fst_mlp_destroy(None) == 1
     assert lib.fst_mlp_train(None, 1, 16, 1e-3, 1, None) == 1
     assert lib.fst_mlp_harvest(None, None, 0, 0, None) == 1
+
+
+@pytest.mark.parametrize("field,value", [("precision", 2), ("precision", -1)])
+def test_unknown_precision_is_rejected(field, value):
+    cfg = make_config(ndim=3, ns=1, nci=8, ncj=8, nck=8)
+    setattr(cfg, field, value)
+    assert runtime.lib().fst_config_dims(C.byref(cfg), C.byref(FstDims())) == 1
+
+
+@pytest.mark.parametrize("case", ["2d", "ns4", "gen3d", "over_2_29"])
+def test_fp32_form_needs_the_fused_3d_path(case):
+    """The FP32 form exists only in the fused 3D stage kernels; elsewhere the
+    FP64 sequence would run without notice, so the configuration is refused."""
+    kw = dict(ndim=3, ns=2, nci=8, ncj=8, nck=8, gamma=(1.4, 1.67), M=(0.029, 0.004), mu=(1e-5, 1e-5), ceq=True)
+    if case == "2d":
+        kw["ndim"] = 2
+    elif case == "ns4":
+        kw.update(ns=4, gamma=(1.4,) * 4, M=(0.029,) * 4, mu=(1e-5,) * 4)
+    elif case == "gen3d":
+        kw["grid"] = 1
+    else:
+        kw.update(nci=400, ncj=400, nck=400)  # 406^3 cells x 11 planes > 2^29
+    lib = runtime.lib()
+    ok = make_config(**kw)
+    if case == "over_2_29":
+        assert lib.fst_config_dims(C.byref(ok), C.byref(FstDims())) == 0  # FP64 runs on the sequence kernels
+    cfg = make_config(precision=1, **kw)
+    assert lib.fst_config_dims(C.byref(cfg), C.byref(FstDims())) == 1
+    good = make_config(precision=1, **dict(kw, ndim=3, grid=0, nci=8, ncj=8, nck=8, ns=2, gamma=(1.4, 1.67),
+                                           M=(0.029, 0.004), mu=(1e-5, 1e-5)))
+    assert lib.fst_config_dims(C.byref(good), C.byref(FstDims())) == 0

@@ -329,3 +329,21 @@ def test_learned_shock_position_and_density_profiles():
         assert abs(dx_lrn.mean()) < 0.05 and np.abs(dx_lrn).max() < 0.25
         assert e3["lrn"] < 0.6 * e3["noc"], (s, e3)
         assert e1["lrn"] < 0.6 * e1["noc"], (s, e1)
+
+
+def test_harvest_rejects_contexts_without_the_fused_primitives():
+    """The harvest runs the 32-bit-offset primitives pass of the fused
+    cartesian path: generalized-grid and 2D contexts are refused with an
+    error instead of harvesting cartesian features."""
+    from fiesta_amd.config import make_config
+    from fiesta_amd.runtime import FiestaError
+    print("case: harvest rejection (gen3d, 2D)", flush=True)
+    tr = Trainer(seed=1)
+    for kw in (dict(ndim=3, grid=1, nck=6), dict(ndim=2)):
+        cfg = make_config(ns=2, nci=12, ncj=10, gamma=(1.4, 1.67), M=(0.029, 0.004), mu=(1e-5, 1e-5), ceq=True,
+                          **kw)
+        m = Module(cfg)
+        with pytest.raises(FiestaError, match="fst_mlp_harvest"):
+            tr.harvest(m)
+        m.close()
+    tr.close()

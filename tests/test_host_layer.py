@@ -44,12 +44,17 @@ def _run_driver(tmp_path, cfg, var, nsteps, mode):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("ndim,mode,rk", [(3, "rk", 2), (3, "fused", 2), (2, "rk", 2), (3, "rk", 3), (3, "fused", 3)])
-def test_driver_steps_equal_oracle(tmp_path, ndim, mode, rk):
+@pytest.mark.parametrize("ndim,mode,rk,noise", [(3, "rk", 2, 0), (3, "fused", 2, 0), (2, "rk", 2, 0), (3, "rk", 3, 0),
+                                               (3, "fused", 3, 0), (3, "rk", 2, 1), (3, "fused", 2, 1)])
+def test_driver_steps_equal_oracle(tmp_path, ndim, mode, rk, noise):
+    """Fiesta::step (src/fiesta.cpp:333-340) runs postStep after every step;
+    with the noise filter on it changes var, so both modes must run it per step."""
     import oracle
-    print("case: fiesta_run %dD %s rk%d" % (ndim, mode, rk), flush=True)
+    print("case: fiesta_run %dD %s rk%d noise %d" % (ndim, mode, rk, noise), flush=True)
     cfg = cfg3d(nci=24, ncj=13, nck=10) if ndim == 3 else cfg2d(nci=40, ncj=12)
     cfg.rk_order = rk
+    if noise:
+        cfg.noise, cfg.n_dh, cfg.n_eta, cfg.n_coff, cfg.n_nt = 1, 2e-3, 0.05, 0.3, 2
     var = W.random_state(cfg, 5)
     got = _run_driver(tmp_path, cfg, var, 3, mode)
     o = oracle.Oracle(cfg)

@@ -1,0 +1,285 @@
+"""Closed-form pins of the oracle where the reference departs from the
+textbook formula, or where only a specific data shape exposes a slip.
+
+Each test is built so that the mistake it names changes the result by far
+more than its tolerance:
+
+  3D pressure gradient, z term over cd(2) = dy (src/presgrad.hpp:44-71, :56):
+      p linear in z with dz != dy gives dvar(2) = -(dp/dz) dz/dy.
+  QUICK faces (src/flux.hpp:169-221): 0.75 f2 + 0.375 f3 - 0.125 f1 is the
+      exact face value of quadratic data on either upwind side, so a
+      uniformly advected quadratic gives dvar = -u . grad q exactly.
+  centered4 faces (src/flux.hpp:108-167): (-q[i+2] + 7 q[i+1] + 7 q[i] -
+      q[i-1]) / 12 is the face value minus h^2/24 times the second
+      derivative there for cubic data, and the face difference then gives
+      dvar = -u . grad q exactly; a two-point face average fails on cubics.
+  2D buoyancy (src/buoyancy.hpp:49-73): v = m_y / var(3) (the first species'
+      density, not rho), the energy term uses |v|, and |rho - rhoRef| < 1e-6
+      switches the term off.
+  3D viscous split (src/viscosity.hpp:416-663): a general quadratic
+      velocity field makes every face stress exact, so the momentum term is
+      mu (lap u + grad(div u)/3) and the energy term is the continuous
+      d_j(u_i tau_ij) plus the truncation terms that the two-cell face
+      average of u and the face differencing add (derived below).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from fiesta_amd import workloads as W
+from fiesta_amd.config import FST_DVAR, FST_VAR, Dims, make_config
+
+
+def _centres(d, h, axis):
+    """Cell-centre coordinate of every cell (ghosts included) along one axis;
+    shape broadcastable to (ngk, ngj, ngi)."""
+    n = (d.ngi, d.ngj, d.ngk)[axis]
+    x = (np.arange(n) - d.ng + 0.5) * h
+    shape = [1, 1, 1]
+    shape[2 - axis] = n
+    return x.reshape(shape)
+
+
+# ------------------------------------------------------------------ 3D pressure gradient
+
+def test_presgrad3d_z_term_is_divided_by_dy():
+    """At rest, p = p0 + a z: advection fluxes vanish (u_f = 0) and the only
+    right-hand side is the pressure gradient.  src/presgrad.hpp:56 sets
+    dz := cd(2) = dy, so the z-momentum gets -a dz/dy, not -a."""
+    dx, dy, dz = 1.0e-2, 2.0e-2, 0.5e-2
+    a = 1.0e6
+    cfg = make_config(ndim=3, ns=1, nci=6, ncj=6, nck=10, dx=dx, dy=dy, dz=dz, gamma=(1.4,), M=(0.02896,),
+                      mu=(0.0,), xper=True, yper=True, bc=("outflow",) * 6)
+    d = Dims(cfg)
+    one = np.ones((d.ngk, d.ngj, d.ngi))
+    z = _centres(d, dz, 2) * one
+    p = 1.0e5 + a * z
+    var = W.fill_conserved(cfg, [1.2 * one], [0 * one] * 3, p, ghosts=True)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.compute()
+    dv = o.get(FST_DVAR)
+    g = d.ng
+    # cells whose 5-point z stencil stays inside (ghosts are refilled by the BCs)
+    core = (slice(g + 2, g + d.nck - 2), slice(g, g + d.ncj), slice(g, g + d.nci))
+    want = -a * dz / dy
+    np.testing.assert_allclose(dv[2][core], want, rtol=1e-9)
+    assert abs(want - (-a)) > 0.5 * a  # the textbook dz would be far outside the tolerance
+    for v in (0, 1):
+        assert np.abs(dv[v][core]).max() <= 1e-9 * abs(want)
+    for v in (3, 4):
+        assert np.abs(dv[v][core]).max() == 0.0
+
+
+# ------------------------------------------------------------------ 2D QUICK and centered4
+
+def _passive_scalar_case(scheme, q_of, U, V, dx=1.0e-2, dy=1.5e-2):
+    """Two species with the same gamma and M (so p stays uniform), uniform
+    rho and velocity; the first species' density is q(x, y).  Every cell,
+    ghosts included, is set from the closed form, and compute() alone runs
+    (the 2D module fills ghosts in preStep, not in compute).  Returns the
+    interior dvar of species 0 and the centre coordinates."""
+    cfg = make_config(ndim=2, ns=2, nci=12, ncj=10, dx=dx, dy=dy, gamma=(1.4, 1.4), M=(0.02896, 0.02896),
+                      mu=(0.0, 0.0), scheme=scheme)
+    d = Dims(cfg)
+    one = np.ones((1, d.ngj, d.ngi))
+    x = _centres(d, dx, 0) * one
+    y = _centres(d, dy, 1) * one
+    q = q_of(x, y)
+    rho = 1.2
+    var = W.fill_conserved(cfg, [q, rho - q], [U * one, V * one], 1.0e5 * one, ghosts=True)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.compute()
+    dv = o.get(FST_DVAR)
+    sl = d.interior()[1:]
+    return dv[3][sl], x[sl], y[sl], dv
+
+
+@pytest.mark.parametrize("U,V", [(30.0, -20.0), (-25.0, 15.0)])
+def test_quick_faces_exact_for_quadratic_data_on_both_upwind_sides(U, V):
+    al, be, c1, c2 = 8.0, -5.0, 1.5, 0.7
+
+    def q(x, y):
+        return 0.5 + c1 * x + c2 * y + al * x * x + be * y * y
+
+    got, x, y, dv = _passive_scalar_case("quick", q, U, V)
+    want = -(U * (c1 + 2 * al * x) + V * (c2 + 2 * be * y))
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
+    # a first-order-upwind or 0.75/0.125/0.125 face would be off by O(h q'')
+    assert np.abs(want).min() > 0.1 * al * 1e-2 * min(abs(U), abs(V))
+
+
+def test_centered4_faces_exact_for_cubic_data():
+    al, be, c1 = 40.0, -30.0, 2.0
+    dx, dy = 1.0e-2, 1.5e-2
+    U, V = 30.0, -20.0
+
+    def q(x, y):
+        return 0.5 + c1 * x + al * x ** 3 + be * y ** 3
+
+    got, x, y, dv = _passive_scalar_case("centered4", q, U, V, dx, dy)
+    # For a cubic the face value is q(f) - h^2 q2(f)/24, q2 = d2q/dx2 (the
+    # odd terms cancel about the face); ((x+h/2)^3 - (x-h/2)^3)/h = 3x^2 +
+    # h^2/4, and the q2 term takes back h^2 q3/24 = h^2/4 per unit al:
+    # dvar = -u . grad q exactly.
+    want = -(U * (c1 + 3 * al * x * x) + V * 3 * be * y * y)
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
+    # The two-point face average q(f) + h^2 q2(f)/8 would shift dvar by
+    # -(U al dx^2 + V be dy^2); the point-value interpolant (-1, 9, 9, -1)/16
+    # (exact for a cubic) by +(U al dx^2 + V be dy^2)/4.
+    shift = U * al * dx * dx + V * be * dy * dy
+    assert abs(shift) > 1e4 * 1e-10 * np.abs(want).max()
+
+
+# ------------------------------------------------------------------ 2D buoyancy
+
+def _buoyancy2d(rhoRef, V=-15.0, U=10.0, rho0=0.4, rho1=0.8, g=9.81):
+    cfg = make_config(ndim=2, ns=2, nci=6, ncj=5, gamma=(1.4, 1.67), M=(0.02896, 0.004), mu=(0.0, 0.0),
+                      buoyancy=True, g=g, rhoRef=rhoRef, xper=True, yper=True)
+    d = Dims(cfg)
+    one = np.ones((1, d.ngj, d.ngi))
+    var = W.fill_conserved(cfg, [rho0 * one, rho1 * one], [U * one, V * one], 1.0e5 * one, ghosts=True)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.pre_step()
+    o.compute()
+    dv = o.get(FST_DVAR)[(slice(None),) + d.interior()[1:]]
+    return dv, var[(slice(None),) + d.interior()[1:]]
+
+
+def test_buoyancy2d_uses_first_species_density_and_abs_v():
+    """Uniform state: the flux terms vanish.  f = -g (rho - rhoRef) enters
+    the y-momentum, and |m_y / var(3)| f the energy (var(3) is the first
+    species' density, 0.4 here, while rho = 1.2; m_y < 0)."""
+    g, rr, V = 9.81, 0.3, -15.0
+    dv, var = _buoyancy2d(rr, V=V, g=g)
+    rho = 1.2
+    f = -g * (rho - rr)
+    np.testing.assert_allclose(dv[1], f, rtol=1e-14)
+    np.testing.assert_allclose(dv[2], abs(rho * V / 0.4) * f, rtol=1e-14)   # = 45 f, not -15 f or +15 f
+    assert np.abs(dv[0]).max() == 0.0
+    assert np.abs(dv[3:]).max() == 0.0
+
+
+@pytest.mark.parametrize("off,active", [(5.0e-7, False), (-5.0e-7, False), (2.0e-6, True), (-2.0e-6, True)])
+def test_buoyancy2d_gate_on_rho_minus_rhoref(off, active):
+    """|rho - rhoRef| < 1e-6 switches the 2D term off entirely."""
+    g = 9.81
+    dv, _ = _buoyancy2d(1.2 - off, g=g)
+    if active:
+        np.testing.assert_allclose(dv[1], -g * off, rtol=1e-8)
+        np.testing.assert_allclose(dv[2], 45.0 * (-g * off), rtol=1e-8)
+    else:
+        assert np.abs(dv[1:3]).max() == 0.0
+
+
+# ------------------------------------------------------------------ 3D viscous terms
+
+class _QuadVel:
+    """u_i(x) = b_i + A_ij x_j + 1/2 H_ijk x_j x_k (H symmetric in j, k)."""
+
+    def __init__(self, rng, scale_A=8.0, scale_H=80.0):
+        self.b = rng.normal(0.0, 1.0, 3)
+        self.A = rng.normal(0.0, scale_A, (3, 3))
+        H = rng.normal(0.0, scale_H, (3, 3, 3))
+        self.H = 0.5 * (H + H.transpose(0, 2, 1))
+
+    def u(self, X):
+        """X: (3, ...) coordinates -> (3, ...) velocity."""
+        lin = np.einsum("ij,j...->i...", self.A, X)
+        quad = 0.5 * np.einsum("ijk,j...,k...->i...", self.H, X, X)
+        return self.b.reshape((3,) + (1,) * (X.ndim - 1)) + lin + quad
+
+    def grad(self, X):
+        """G_ij = d u_i / d x_j, shape (3, 3, ...)."""
+        return self.A.reshape((3, 3) + (1,) * (X.ndim - 1)) + np.einsum("ijk,k...->ij...", self.H, X)
+
+
+def _stress(G, mu):
+    div = G[0, 0] + G[1, 1] + G[2, 2]
+    tau = mu * (G + G.transpose((1, 0) + tuple(range(2, G.ndim))))
+    for i in range(3):
+        tau[i, i] = tau[i, i] - (2.0 / 3.0) * mu * div
+    return tau
+
+
+def _viscous_closed_form(qv, X, mu, h):
+    """Momentum and energy of the face-based viscous split for a quadratic u.
+
+    Every face stress is the exact stress at the face centre (normal
+    derivatives are two-point differences, tangential ones four-point
+    averages: both exact for quadratics), so the momentum term is the
+    continuous d_j tau_ij = mu (lap u_i + d_i(div u)/3).
+
+    The energy flux through a face uses the two-cell average of u, which is
+    u(face) + h^2/8 d_jj u_i for a quadratic; the difference of the cubic
+    g = u_i tau_ij over +-h/2 divided by h is g' + h^2/24 g''' with
+    g''' = 3 d_jj u_i d_j tau_ij.  Together:
+        E = d_j(u_i tau_ij) + sum_j h_j^2/4 sum_i d_jj u_i d_j tau_ij.
+    """
+    H = qv.H
+    trH = np.einsum("mml->l", H)  # d_l (div u)
+    dtau = np.empty((3, 3, 3))  # dtau[i, j, l] = d_l tau_ij
+    for i in range(3):
+        for j in range(3):
+            for l in range(3):
+                dtau[i, j, l] = mu * (H[i, j, l] + H[j, i, l]) - (2.0 / 3.0) * mu * (i == j) * trH[l]
+    mom = np.array([sum(dtau[i, j, j] for j in range(3)) for i in range(3)])
+    assert np.allclose(mom, mu * (np.einsum("ijj->i", H) + trH / 3.0))
+    G = qv.grad(X)
+    U = qv.u(X)
+    tau = _stress(G, mu)
+    energy = np.einsum("ij...,ij...->...", G, tau)
+    for i in range(3):
+        for j in range(3):
+            energy = energy + U[i] * dtau[i, j, j]
+    for j in range(3):
+        for i in range(3):
+            energy = energy + h[j] ** 2 / 4.0 * H[i, j, j] * dtau[i, j, j]
+    return mom, energy
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_viscous3d_quadratic_velocity_closed_form(seed):
+    """dvar with the viscous terms minus dvar without them, on a general
+    quadratic velocity field (every entry of A and H nonzero, dx, dy, dz
+    distinct): momentum mu (lap u + grad(div u)/3), energy as derived in
+    _viscous_closed_form.  A missing 2/3 in the normal stresses, a swapped
+    stress component on any face family or a wrong face spacing fails."""
+    rng = np.random.default_rng(seed)
+    qv = _QuadVel(rng)
+    h = (1.0e-2, 1.3e-2, 0.8e-2)
+    mus = (0.5, 2.0)
+    Y0 = 0.3
+    mu = Y0 * mus[0] + (1 - Y0) * mus[1]
+    base = dict(ndim=3, ns=2, nci=7, ncj=6, nck=8, dx=h[0], dy=h[1], dz=h[2], gamma=(1.4, 1.67),
+                M=(0.02896, 0.004), mu=mus, bc=("outflow",) * 6)
+    cfgs = [make_config(visc=v, **base) for v in (True, False)]
+    d = Dims(cfgs[0])
+    one = np.ones((d.ngk, d.ngj, d.ngi))
+    X = np.stack([_centres(d, h[a], a) * one for a in range(3)])
+    Uv = qv.u(X)
+    rho = 1.2
+    var = W.fill_conserved(cfgs[0], [Y0 * rho * one, (1 - Y0) * rho * one], list(Uv), 1.0e3 * one, ghosts=True)
+    dvs = []
+    for cfg in cfgs:
+        o = oracle.Oracle(cfg)
+        o.set(FST_VAR, var)
+        o.pre_sim()
+        o.compute()
+        dvs.append(o.get(FST_DVAR))
+    g = d.ng
+    # cells whose viscous stencil (one cell in every direction) is interior
+    core = (slice(g + 1, g + d.nck - 1), slice(g + 1, g + d.ncj - 1), slice(g + 1, g + d.nci - 1))
+    visc = dvs[0] - dvs[1]
+    mom, energy = _viscous_closed_form(qv, X[(slice(None),) + core], mu, h)
+    scale = np.abs(mom).max()
+    for i in range(3):
+        np.testing.assert_allclose(visc[i][core], mom[i], rtol=0, atol=1e-9 * scale)
+    escale = np.abs(energy).max()
+    np.testing.assert_allclose(visc[3][core], energy, rtol=0, atol=1e-9 * escale)
+    # the species equations carry no viscous term
+    assert np.abs(visc[4:6][(slice(None),) + core]).max() == 0.0
