@@ -184,6 +184,42 @@ int fp32_only(fst_ctx *ctx, const char *what) {
 
 }  // namespace
 
+namespace fst {
+
+int module_check(const fst_config &in, fst_dims *d, std::string *why) { return derive(in, d, why); }
+
+void module_setup(const fst_config &in, const fst_dims &d, fst_config *cf, Geo *g, Phys *ph) {
+  *cf = in;
+  fst_config &c = *cf;
+  if (in.grid == 1) c.dx = c.dy = c.dz = 1.0;  // src/input.cpp:344-350
+  if (in.ndim == 2) {  // 2D defaults, src/input.cpp:354-362
+    c.dz = c.dx;
+    c.nck = 1;
+    c.zPer = 0;
+  }
+  g->ngi = d.ngi; g->ngj = d.ngj; g->ngk = d.ngk; g->ng = c.ng;
+  g->nci = c.nci; g->ncj = c.ncj; g->nck = c.ndim == 3 ? c.nck : 1; g->ndim = c.ndim;
+  g->nv = d.nv; g->nvt = d.nvt; g->ns = c.ns; g->nvx = d.nvx;
+  g->N = d.ncells_ghost;
+  std::memset(ph, 0, sizeof(*ph));
+  // device config array, src/cart3d.cpp:152-171 / src/cart2d.cpp:92-110
+  ph->cd[0] = c.ns; ph->cd[1] = c.dx; ph->cd[2] = c.dy; ph->cd[3] = c.dz; ph->cd[4] = d.nv; ph->cd[5] = c.ng;
+  for (int s = 0; s < c.ns; ++s) {
+    double gammas = c.gamma[s], Rs = c.R / c.M[s];
+    ph->cd[6 + 3 * s] = gammas;
+    ph->cd[6 + 3 * s + 1] = Rs;
+    ph->cd[6 + 3 * s + 2] = c.mu[s];
+    ph->cpf[s] = gammas * Rs / (gammas - 1);
+    ph->cvf[s] = Rs / (gammas - 1);
+    ph->mus[s] = c.mu[s];
+  }
+  ph->dx = c.dx; ph->dy = c.dy; ph->dz = c.dz;
+  ph->dxmag2 = c.dx * c.dx + c.dy * c.dy + c.dz * c.dz;
+  ph->kap = c.kap; ph->eps = c.eps; ph->alpha = c.alpha; ph->g = c.gAccel; ph->rhoRef = c.rhoRef; ph->dt = c.dt;
+}
+
+}  // namespace fst
+
 extern "C" {
 
 int fst_abi_version(void) { return FST_ABI_VERSION; }
@@ -223,35 +259,9 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
     return rc;
   }
   fst_ctx *ctx = new fst_ctx();
-  ctx->cf = *cfg;
-  if (cfg->grid == 1) ctx->cf.dx = ctx->cf.dy = ctx->cf.dz = 1.0;  // src/input.cpp:344-350
-  if (cfg->ndim == 2) {  // 2D defaults, src/input.cpp:354-362
-    ctx->cf.dz = ctx->cf.dx;
-    ctx->cf.nck = 1;
-    ctx->cf.zPer = 0;
-  }
+  module_setup(*cfg, d, &ctx->cf, &ctx->g, &ctx->ph);
   const fst_config &c = ctx->cf;
-  Geo &g = ctx->g;
-  g.ngi = d.ngi; g.ngj = d.ngj; g.ngk = d.ngk; g.ng = c.ng;
-  g.nci = c.nci; g.ncj = c.ncj; g.nck = c.ndim == 3 ? c.nck : 1; g.ndim = c.ndim;
-  g.nv = d.nv; g.nvt = d.nvt; g.ns = c.ns; g.nvx = d.nvx;
-  g.N = d.ncells_ghost;
-  Phys &ph = ctx->ph;
-  std::memset(&ph, 0, sizeof(ph));
-  // device config array, src/cart3d.cpp:152-171 / src/cart2d.cpp:92-110
-  ph.cd[0] = c.ns; ph.cd[1] = c.dx; ph.cd[2] = c.dy; ph.cd[3] = c.dz; ph.cd[4] = d.nv; ph.cd[5] = c.ng;
-  for (int s = 0; s < c.ns; ++s) {
-    double gammas = c.gamma[s], Rs = c.R / c.M[s];
-    ph.cd[6 + 3 * s] = gammas;
-    ph.cd[6 + 3 * s + 1] = Rs;
-    ph.cd[6 + 3 * s + 2] = c.mu[s];
-    ph.cpf[s] = gammas * Rs / (gammas - 1);
-    ph.cvf[s] = Rs / (gammas - 1);
-    ph.mus[s] = c.mu[s];
-  }
-  ph.dx = c.dx; ph.dy = c.dy; ph.dz = c.dz;
-  ph.dxmag2 = c.dx * c.dx + c.dy * c.dy + c.dz * c.dz;
-  ph.kap = c.kap; ph.eps = c.eps; ph.alpha = c.alpha; ph.g = c.gAccel; ph.rhoRef = c.rhoRef; ph.dt = c.dt;
+  const Geo &g = ctx->g;
 
   auto alloc = [&](double **p, size_t planes) -> bool {
     if (planes == 0) return true;

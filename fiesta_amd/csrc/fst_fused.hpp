@@ -35,4 +35,11 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
 int fused_primitives(const Geo &g, const Phys &ph, const fst_config &cf, const double *U, double *P, hipStream_t s);
 void fused_free(FusedBufs &fb);
 
+// Validation and derived sizes of a configuration (fst_config_dims), and the
+// module constants fst_create builds from it: the effective config (2D and
+// generalized-grid defaults applied), the launch geometry and the device
+// constants.
+int module_check(const fst_config &in, fst_dims *d, std::string *why);
+void module_setup(const fst_config &in, const fst_dims &d, fst_config *cf, Geo *g, Phys *ph);
+
 }  // namespace fst

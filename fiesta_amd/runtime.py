@@ -34,6 +34,7 @@ _EXPORTS = {
     "fst_rk_stage": ([C.c_void_p, C.c_int], C.c_int),
     "fst_post_step": ([C.c_void_p], C.c_int),
     "fst_step": ([C.c_void_p, C.c_int], C.c_int),
+    "fst_step_static": ([C.c_void_p, C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "fst_last_maxima": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "fst_sync": ([C.c_void_p], C.c_int),
     "fst_set_fused": ([C.c_void_p, C.c_int], C.c_int),

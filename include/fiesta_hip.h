@@ -187,6 +187,31 @@ int fst_post_step(fst_ctx *ctx);
  * noise filter on, call fst_step(ctx, 1) and fst_post_step(ctx) per step. */
 int fst_step(fst_ctx *ctx, int nsteps);
 
+/* Size in MiB of the device scratch array fst_step_static uses on each GPU. */
+#define FST_STATIC_SCRATCH_MB 256
+
+/* fst_step without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * `nsteps` time steps of the fused 3D cartesian path (FP64 or FP32 form, RK2
+ * or SSP-RK3; not the learned C-indicator) on `var`, a device pointer to the
+ * caller's state (LayoutLeft, nvt planes with ghosts, as fst_upload takes
+ * it), updated in place; `stream` is a hipStream_t (NULL: the default
+ * stream).  The result equals fst_step on a context holding the same var
+ * after fst_pre_sim.  The call is asynchronous on `stream`.
+ *
+ * The stage state, primitive planes, partial right-hand side and stage
+ * maxima live in one static device array of FST_STATIC_SCRATCH_MB MiB per
+ * GPU, held until the process ends and zeroed at the start of every call.
+ * A grid that needs more (about 264 bytes per cell with ghosts at nvt = 11,
+ * RK2; 390 at nvt = 12 with SSP-RK3) is refused with FST_ERR_ARG before
+ * anything is launched.  Every call on one GPU shares that array: calls
+ * that overlap in time on one GPU (other streams, other host threads) race
+ * and give wrong results; serialize them.  Errors go to stderr (there is no
+ * context to hold a message). */
+int fst_step_static(const fst_config *cfg, double *var, int nsteps, void *stream);
+
 /* Per-stage global maxima of the C-equation (max wave speed, max |C_hat|)
  * of the last computed stage (fsMax, src/cart3d.cpp:190-198, 281-283). */
 int fst_last_maxima(fst_ctx *ctx, double *maxS, double *maxCh);

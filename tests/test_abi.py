@@ -145,3 +145,21 @@ def test_fp32_form_needs_the_fused_3d_path(case):
     good = make_config(precision=1, **dict(kw, ndim=3, grid=0, nci=8, ncj=8, nck=8, ns=2, gamma=(1.4, 1.67),
                                            M=(0.029, 0.004), mu=(1e-5, 1e-5)))
     assert lib.fst_config_dims(C.byref(good), C.byref(FstDims())) == 0
+
+
+def test_static_step_refuses_before_touching_a_device():
+    """fst_step_static checks the configuration and the scratch size on the
+    host, before any HIP call (so these run without a GPU)."""
+    lib = runtime.lib()
+    kw = dict(ndim=3, ns=2, gamma=(1.4, 1.67), M=(0.029, 0.004), mu=(1e-5, 1e-5), ceq=True, visc=True)
+    dummy = C.c_void_p(0x1000)
+    big = make_config(nci=100, ncj=100, nck=100, **kw)  # 106^3 cells x 264 B > 256 MiB
+    assert lib.fst_step_static(C.byref(big), dummy, 1, None) == 1
+    for bad in (make_config(nci=8, ncj=8, nck=8, learned_c=True, **kw),
+                make_config(**dict(kw, ndim=2), nci=8, ncj=8),
+                make_config(nci=8, ncj=8, nck=8, grid=1, **kw)):
+        assert lib.fst_step_static(C.byref(bad), dummy, 1, None) == 1
+    ok = make_config(nci=8, ncj=8, nck=8, **kw)
+    assert lib.fst_step_static(C.byref(ok), None, 1, None) == 1  # null state
+    assert lib.fst_step_static(C.byref(ok), dummy, -1, None) == 1
+    assert lib.fst_step_static(None, dummy, 1, None) == 1
