@@ -7,7 +7,9 @@ A step is one full RK2 time step (two RHS evaluations + stage updates); the
 metric counts interior cells x 2 stages / wall second (Gcell-stage/s).
 
 Prints ONE JSON line.  `value` is the FP64 plain path (bitwise parity with
-the oracle).  The roofline block covers the dominant kernel, timed live with
+the oracle).  The fast FP64 form (fp64_fast: FMA contraction and
+reciprocal-based division in the WENO5 and viscous face terms) is reported
+beside it with its error after the configuration's 500 steps.  The roofline block covers the dominant kernel, timed live with
 HIP events on the stream it runs on; algorithmic bytes are 20*nvt per
 cell-stage (stage 1 reads U, writes U*; stage 2 reads U*, U^n, writes U^{n+1}).
 
@@ -44,9 +46,12 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--path", choices=["fused", "reference"], default="fused")
     ap.add_argument("--copy-probe", action="store_true", help="also time a plain copy kernel")
-    ap.add_argument("--forms", default="fp64,fp32,fp32_learned,fp64_rk3,fp32_learned_rk3",
-                    help="comma list of arithmetic forms to time: fp64 (reported as value), fp32, fp32_learned; "
-                         "suffix _rk3 runs the SSP-RK3 option instead of the reference's RK2 midpoint")
+    ap.add_argument("--forms", default="fp64,fp32,fp32_learned,fp64_fast,fp64_rk3,fp32_learned_rk3",
+                    help="comma list of arithmetic forms to time: fp64 (reported as value), fp32, fp32_learned, "
+                         "fp64_fast; suffix _rk3 runs the SSP-RK3 option instead of the reference's RK2 midpoint")
+    ap.add_argument("--accuracy-steps", type=int, default=500,
+                    help="steps of the untimed accuracy leg: each RK2 form against fp64 after this many steps "
+                         "(0: skip); the Rayleigh-Taylor configuration runs 500")
     ap.add_argument("--gen3d-size", type=int, nargs=3, default=[512, 512, 256],
                     help="cells of the gen3d leg of --extras")
     ap.add_argument("--extras", action="store_true",
@@ -102,7 +107,7 @@ def train_learned():
 
 def gpu_leg(args, form="fp64", var=None, keep_state=False, weights=None):
     from fiesta_amd import workloads as W
-    from fiesta_amd.config import FST_FP32_BULK, FST_VAR, Dims
+    from fiesta_amd.config import FST_FP32_BULK, FST_FP64_FAST, FST_VAR, Dims
     from fiesta_amd.runtime import Module
 
     cfg = W.rt3d_config(args.size)
@@ -110,6 +115,8 @@ def gpu_leg(args, form="fp64", var=None, keep_state=False, weights=None):
     cfg.rk_order = nst
     if base.startswith("fp32"):
         cfg.precision = FST_FP32_BULK
+    elif base == "fp64_fast":
+        cfg.precision = FST_FP64_FAST
     learned = base.endswith("learned")
     if learned:
         cfg.learned_c = 1
@@ -264,12 +271,21 @@ def expansion2d_leg(args):
                          "frac": round(d.ncells * balg / (stage_ms * 1e-3) / 8.0e12, 4)}}
 
 
+def cpu_threads():
+    """The host's stated core count: OMP_NUM_THREADS when the environment sets
+    it (the GPU hosts state their CPU share there), else os.cpu_count()."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
 def cpu_leg(args, cfg_size):
+    # thread placement is read when the OpenMP runtime starts (first load of the oracle)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     import oracle
     from fiesta_amd import workloads as W
     from fiesta_amd.config import FST_VAR
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     oracle.set_threads(threads)
     cfg = W.rt3d_config(cfg_size)
     o = oracle.Oracle(cfg)
@@ -282,7 +298,91 @@ def cpu_leg(args, cfg_size):
     cs = cfg.nci * cfg.ncj * cfg.nck * 2 * args.cpu_steps
     return dict(value=cs / wall / 1e9, unit="Gcell-stage/s", cores=threads, kind="port",
                 sample="%d RK2 steps (after 1 warm-up) of the %d^3 workload with the oracle "
-                       "(C++/OpenMP restatement of the reference kernels), %d threads" % (args.cpu_steps, cfg_size, threads))
+                       "(C++/OpenMP restatement of the reference kernels), %d threads, OMP_PROC_BIND=%s"
+                       % (args.cpu_steps, cfg_size, threads, os.environ.get("OMP_PROC_BIND")))
+
+
+def sod_leg():
+    """Configuration 1 in full: Sod shock tube, 1000 x 4 cells, WENO5 +
+    C-equation, RK2, dt 2e-4, 1000 steps, on the GPU (the cart2d module's
+    kernels) and on the CPU with the oracle; the final states compared bit
+    for bit and against the exact Riemann solution at t = 0.2."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    import oracle
+    from fiesta_amd import workloads as W
+    from fiesta_amd.config import FST_VAR, Dims
+    from fiesta_amd.runtime import Module
+
+    cfg = W.sod_config()
+    d = Dims(cfg)
+    var = W.sod_state(cfg)
+    steps = 1000
+    m = Module(cfg)
+    m.upload(FST_VAR, var)
+    m.pre_sim()
+    m.sync()
+    t0 = time.perf_counter()
+    m.step(steps)
+    m.sync()
+    gpu_s = time.perf_counter() - t0
+    got = m.download(FST_VAR)
+    m.close()
+    threads = cpu_threads()
+    oracle.set_threads(threads)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    t0 = time.perf_counter()
+    o.step(steps)
+    cpu_s = time.perf_counter() - t0
+    want = o.get(FST_VAR)
+    rho = got[3, 0, d.ng, d.ng:d.ng + d.nci]  # density (one species) along x, first row
+    x = (np.arange(d.nci) + 0.5) / d.nci
+    ex = W.riemann_ideal(1.0, 1.0, 0.125, 0.1, 1.4)
+    mid = 0.5 * (ex["rho_r"] + 0.125)
+    shock = float(x[np.where((x > 0.7) & (rho < mid))[0][0]])
+    cs = d.ncells * 2 * steps
+    return {"workload": "Sod shock tube 1000x4, WENO5 + C-equation, RK2, dt 2e-4, 1000 steps, FP64",
+            "gpu_s": round(gpu_s, 4), "gpu_gcell_stage_per_s": round(cs / gpu_s / 1e9, 5),
+            "cpu_s": round(cpu_s, 4), "cpu_gcell_stage_per_s": round(cs / cpu_s / 1e9, 5), "cpu_threads": threads,
+            "bitwise_equal_to_oracle": bool(np.array_equal(got, want)),
+            "shock_position": round(shock, 4), "exact_shock_position": round(0.5 + 0.2 * ex["s_shock"], 4)}
+
+
+def accuracy_leg(args, var, forms, weights):
+    """Each RK2 form run for args.accuracy_steps steps (untimed) from the
+    same state, and its final density, pressure and mass fractions compared
+    with the FP64 path's (equal to the oracle bit for bit)."""
+    from fiesta_amd import workloads as W
+    from fiesta_amd.config import FST_FP32_BULK, FST_FP64_FAST, FST_VAR, Dims
+    from fiesta_amd.runtime import Module
+
+    n = args.accuracy_steps
+    states = {}
+    for f in ["fp64"] + [f for f in forms if f != "fp64" and not f.endswith("_rk3")]:
+        cfg = W.rt3d_config(args.size)
+        if f.startswith("fp32"):
+            cfg.precision = FST_FP32_BULK
+        elif f == "fp64_fast":
+            cfg.precision = FST_FP64_FAST
+        if f.endswith("learned"):
+            cfg.learned_c = 1
+        d = Dims(cfg)
+        m = Module(cfg)
+        m.upload(FST_VAR, var[:d.nvt])
+        if cfg.learned_c:
+            m.set_mlp_weights(weights)
+        m.pre_sim()
+        m.step(n)
+        states[f] = (cfg, m.download(FST_VAR))
+        m.close()
+        log("accuracy leg: %s, %d steps" % (f, n))
+    cfg64, ref = states["fp64"]
+    out = {}
+    for f, (cfg, st) in states.items():
+        if f != "fp64":
+            out[f] = form_errors(cfg, st, ref, cfg64)
+    return out
 
 
 def measured_traffic(size, form):
@@ -441,6 +541,13 @@ def main():
     out["forms"] = fm
     if train_info:
         out["learned_training"] = train_info
+    if args.accuracy_steps > 0:
+        acc = accuracy_leg(args, var, forms, weights)
+        for f, e in acc.items():
+            if f in fm:
+                fm[f]["error_vs_fp64_%d_steps" % args.accuracy_steps] = e
+    log("configuration 1 (Sod, CPU and GPU)")
+    out["sod_c1"] = sod_leg()
     if args.extras:
         log("extras: noise filter and output pack")
         out["extras"] = extras_leg(args, var)

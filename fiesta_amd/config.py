@@ -12,7 +12,7 @@ BC = {"outflow": 0, "reflective": 1, "noslip": 2, "hydrostatic": 3}
 SCHEME = {"weno5": 1, "centered4": 2, "quick": 3}
 
 FST_VAR, FST_TMP1, FST_DVAR, FST_VARX, FST_P, FST_RHO, FST_T, FST_VEL, FST_GRADRHO, FST_GRID, FST_METRICS = range(11)
-FST_FP64, FST_FP32_BULK = 0, 1
+FST_FP64, FST_FP32_BULK, FST_FP64_FAST = 0, 1, 2
 
 
 class FstConfig(C.Structure):

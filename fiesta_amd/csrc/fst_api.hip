@@ -65,8 +65,12 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
     *why = "grid: 0 (cartesian) or 1 (generalized); the terrain grid (2) has no module in the reference";
     return FST_ERR_ARG;
   }
-  if (c.precision != FST_FP64 && c.precision != FST_FP32_BULK) {
-    *why = "precision: 0 (FP64) or 1 (FP32 bulk, FP64 accumulation)";
+  if (c.precision != FST_FP64 && c.precision != FST_FP32_BULK && c.precision != FST_FP64_FAST) {
+    *why = "precision: 0 (FP64), 1 (FP32 bulk, FP64 accumulation) or 2 (fast FP64)";
+    return FST_ERR_ARG;
+  }
+  if (c.precision == FST_FP64_FAST && c.learned_c) {
+    *why = "the learned C-indicator runs in the FP64 (0) or FP32 bulk (1) forms";
     return FST_ERR_ARG;
   }
   if (c.grid == 1 && (c.learned_c || c.precision != FST_FP64)) {
@@ -92,9 +96,9 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
   const int bcs[6] = {c.bcL, c.bcR, c.bcB, c.bcT, c.bcH, c.bcF};
   for (int q = 0; q < 6; ++q)
     if (bcs[q] < 0 || bcs[q] > 3) { *why = "unknown boundary type"; return FST_ERR_ARG; }
-  if (c.precision == FST_FP32_BULK && !fused_supported(c)) {
-    *why = "the FP32 precision form runs on the fused 3D stage kernels only (cartesian, WENO5, ns <= 3, every "
-           "array below 2^29 elements)";
+  if (c.precision != FST_FP64 && !fused_supported(c)) {
+    *why = "the FP32 and fast FP64 forms run on the fused 3D stage kernels only (cartesian, WENO5, ns <= 3, "
+           "every array below 2^29 elements)";
     return FST_ERR_ARG;
   }
   d->ngi = c.nci + 2 * c.ng;
@@ -176,10 +180,11 @@ int learned_only(fst_ctx *ctx, const char *what) {
   return fail(ctx, FST_ERR_ARG, std::string(what) + ": the learned C-indicator runs through fst_step only");
 }
 
-// The FP32 forms exist only in the fused stage kernels; the per-kernel
-// sequence computes in FP64 whatever the precision field says.
+// The FP32 and fast FP64 forms exist only in the fused stage kernels; the
+// per-kernel sequence computes the reference arithmetic whatever the
+// precision field says.
 int fp32_only(fst_ctx *ctx, const char *what) {
-  return fail(ctx, FST_ERR_ARG, std::string(what) + ": the FP32 precision form runs through fst_step only");
+  return fail(ctx, FST_ERR_ARG, std::string(what) + ": the FP32 and fast FP64 forms run through fst_step only");
 }
 
 }  // namespace
@@ -397,7 +402,7 @@ int fst_pre_step(fst_ctx *ctx) {
 int fst_compute(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
   if (ctx->cf.learned_c) return learned_only(ctx, "fst_compute");
-  if (ctx->cf.precision == FST_FP32_BULK) return fp32_only(ctx, "fst_compute");
+  if (ctx->cf.precision != FST_FP64) return fp32_only(ctx, "fst_compute");
   ctx->fb.valid = false;
   compute_seq(ctx);
   return check_launch(ctx, "fst_compute");
@@ -425,7 +430,7 @@ int fst_set_fused(fst_ctx *ctx, int fused) {
   if (fused && !fused_supported(ctx->cf))
     return fail(ctx, FST_ERR_ARG, "fused path supports 3D cartesian FP64 with ceq, visc and buoyancy flags only");
   if (!fused && ctx->cf.learned_c) return learned_only(ctx, "fst_set_fused(0)");
-  if (!fused && ctx->cf.precision == FST_FP32_BULK) return fp32_only(ctx, "fst_set_fused(0)");
+  if (!fused && ctx->cf.precision != FST_FP64) return fp32_only(ctx, "fst_set_fused(0)");
   ctx->fused = fused ? 1 : 0;
   return FST_OK;
 }

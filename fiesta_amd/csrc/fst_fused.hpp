@@ -31,6 +31,10 @@ void launch_gen_adv(const Geo &g, const Phys &ph, const fst_config &cf, const do
 // is set (else may be null).
 int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb, LearnedBufs *lb,
                 int nsteps, hipStream_t s, std::string *err);
+// The same for precision = FST_FP64_FAST (fst_fast64.hip); fused_steps
+// forwards to it.
+int fused_steps_fast64(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb,
+                       LearnedBufs *lb, int nsteps, hipStream_t s, std::string *err);
 // Primitive planes (rho, p, u, v, w, nrg, Y_s) of U over every cell (3D).
 int fused_primitives(const Geo &g, const Phys &ph, const fst_config &cf, const double *U, double *P, hipStream_t s);
 void fused_free(FusedBufs &fb);

@@ -34,6 +34,20 @@
 //
 // Addressing: element offsets are 32-bit (checked on the host: every array
 // stays below 2^29 elements) on uniform base pointers.
+//
+// Fast FP64 form (precision = FST_FP64_FAST): this file compiled a second time
+// as fst_fast64.hip with FST_FAST64_TU defined.  There the WENO5 fluxes
+// (weno5_onediv: one division per face value instead of four) and the
+// viscous / artificial-viscosity face terms (face_vc, face_vc_ring) contract
+// a*b+c into FMAs and divide through the hardware reciprocal (fast_div).
+// Everything that sets the pressure and its gradient keeps the reference
+// arithmetic: the EOS and primitives, the pressure gradient and buoyancy,
+// the C-equation sensors and relaxation, and the RK update.  Near hydrostatic
+// balance the momentum right-hand side is the small difference of the
+// pressure gradient and gravity, so one ulp of p there would move it far
+// more than the flux rounding does.  The form is held to relative L2
+// <= 1e-12 against the reference arithmetic.  Only fused_steps is exported
+// from that unit, as fused_steps_fast64.
 #include <algorithm>
 #include <type_traits>
 #include <cmath>
@@ -45,6 +59,14 @@
 namespace fst {
 
 namespace {
+
+// First statement of a function body whose arithmetic the fast FP64 form
+// contracts (a no-op in the reference-arithmetic units).
+#ifdef FST_FAST64_TU
+#define FST_FAST_FP _Pragma("clang fp contract(fast)")
+#else
+#define FST_FAST_FP
+#endif
 
 constexpr int TX = 64;
 constexpr int TY = 7;
@@ -192,11 +214,31 @@ __device__ __forceinline__ Tile place(const Grid32 &g, int kchunk, int nbx, int 
 // assembly, pressure gradient, RK accumulation and EOS stay FP64.
 template <bool F32>
 struct Ar;
+#ifdef FST_FAST64_TU
+// a/b through the hardware reciprocal: r = rcp(b) refined by two
+// Newton-Raphson steps r <- r + r (1 - b r), then q = a r with one residual
+// correction q <- q + r (a - b q).  Within about an ulp of the IEEE quotient
+// for the normal, finite operands of the face terms.
+__device__ __forceinline__ double fast_div(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(e, r, r);
+  e = fma(-b, r, 1.0);
+  r = fma(e, r, r);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+#endif
+
 template <>
 struct Ar<false> {
   using R = double;
   static __device__ __forceinline__ R cdv(R a, double b, double rb) { return cdiv(a, b, rb); }
+#ifdef FST_FAST64_TU
+  static __device__ __forceinline__ R div(R a, R b) { return fast_div(a, b); }
+#else
   static __device__ __forceinline__ R div(R a, R b) { return a / b; }
+#endif
   static __device__ __forceinline__ R sq(R a) { return sqrt(a); }
 };
 template <>
@@ -231,6 +273,39 @@ __device__ __forceinline__ float weno5_fast(float f1, float f2, float f3, float 
   return (w1 * p1 + w2 * p2 + w3 * p3) * __builtin_amdgcn_rcpf(w1 + w2 + w3);
 }
 
+#ifdef FST_FAST64_TU
+// WENO5 of calculateFluxesG (src/flux.hpp:240-264) with the reference's
+// binary32-rounded constants, normalised with one division: for
+// d_i = (eps + b_i)^2 and weights w_i = g_i / d_i,
+//   (w1 p1 + w2 p2 + w3 p3) / (w1 + w2 + w3)
+//     = (g1 p1 d2 d3 + g2 p2 d1 d3 + g3 p3 d1 d2) / (g1 d2 d3 + g2 d1 d3 + g3 d1 d2)
+// (numerator and denominator times d1 d2 d3).  d_i >= 1e-12 and the flux
+// variables stay far below 1e70, so the products neither underflow nor
+// overflow.
+__device__ __forceinline__ double weno5_onediv(double f1, double f2, double f3, double f4, double f5) {
+  FST_FAST_FP
+  const double eps = 0.000001;
+  const double c13 = 13.0f / 12.0f, q = 0.25f;
+  double a1 = f1 - 2.0 * f2 + f3, a2 = f1 - 4.0 * f2 + 3.0 * f3;
+  const double b1 = c13 * a1 * a1 + q * a2 * a2;
+  a1 = f2 - 2.0 * f3 + f4;
+  a2 = f2 - f4;
+  const double b2 = c13 * a1 * a1 + q * a2 * a2;
+  a1 = f3 - 2.0 * f4 + f5;
+  a2 = 3.0 * f3 - 4.0 * f4 + f5;
+  const double b3 = c13 * a1 * a1 + q * a2 * a2;
+  const double e1 = eps + b1, e2 = eps + b2, e3 = eps + b3;
+  const double d1 = e1 * e1, d2 = e2 * e2, d3 = e3 * e3;
+  const double d23 = d2 * d3, d13 = d1 * d3, d12 = d1 * d2;
+  const double g1 = 0.1f, g2 = 0.6f, g3 = 0.3f;
+  const double p1 = (1.0f / 3.0f) * f1 + (-7.0f / 6.0f) * f2 + (11.0f / 6.0f) * f3;
+  const double p2 = (-1.0f / 6.0f) * f2 + (5.0f / 6.0f) * f3 + (1.0f / 3.0f) * f4;
+  const double p3 = (1.0f / 3.0f) * f3 + (5.0f / 6.0f) * f4 + (-1.0f / 6.0f) * f5;
+  const double W1 = g1 * d23, W2 = g2 * d13, W3 = g3 * d12;
+  return fast_div(W1 * p1 + W2 * p2 + W3 * p3, W1 + W2 + W3);
+}
+#endif
+
 // ---------------------------------------------------------------- faces
 
 // Advective fluxes of the nv flow variables on one face (calculateFluxesG,
@@ -249,7 +324,11 @@ __device__ __forceinline__ void flux_from_stencil(const FC &k, int D, const doub
     const double f4 = neg ? q[2][v] : q[3][v];
     const double f5 = neg ? q[1][v] : q[4][v];
     if (!F32) {
+#ifdef FST_FAST64_TU
+      out[v] = (ud * weno5_onediv(f1, f2, f3, f4, f5)) * k.rd[D];
+#else
       out[v] = cdiv(ud * weno5_f32c(f1, f2, f3, f4, f5), k.d[D], k.rd[D]);
+#endif
     } else {
       const double c0 = q[2][v];
       const float w = weno5_fast((float)(f1 - c0), (float)(f2 - c0), (float)(f3 - c0), (float)(f4 - c0),
@@ -293,6 +372,7 @@ __device__ __forceinline__ void face_vc(const FC &k, const Grid32 &g, const doub
                                         const double *__restrict__ P, const float *__restrict__ CH,
                                         const float *__restrict__ DR, double alpha, int visc, int ceq,
                                         uint32_t c, typename Ar<F32>::R *out) {
+  FST_FAST_FP
   using A = Ar<F32>;
   using R = typename A::R;
   const uint32_t N = g.N, e = stride<D>(g);
@@ -571,7 +651,11 @@ __device__ __forceinline__ void face_adv_lds(const FC &k, const double *qs, int 
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     const double *qv = qs + v * QPL + a0;
+#ifdef FST_FAST64_TU
+    out[v] = (ud * weno5_onediv(qv[0], qv[dt], qv[2 * dt], qv[3 * dt], qv[4 * dt])) * k.rd[D];
+#else
     out[v] = cdiv(ud * weno5_f32c(qv[0], qv[dt], qv[2 * dt], qv[3 * dt], qv[4 * dt]), k.d[D], k.rd[D]);
+#endif
   }
 }
 
@@ -1188,6 +1272,7 @@ template <int D, int NS, bool F32, bool LC>
 __device__ __forceinline__ void face_vc_ring(const FC &k, const RingAt<NS, F32, LC> &at,
                                              typename Ar<F32>::R alpha, int visc, int ceq,
                                              typename Ar<F32>::R *out) {
+  FST_FAST_FP
   using A = Ar<F32>;
   using RF = RingF<NS, F32, LC>;
   using R = typename A::R;
@@ -1891,6 +1976,17 @@ using StageFn = void (*)(const Grid32 &, const Geo &, const FC &, const fst_conf
 using PrimFn = void (*)(const Grid32 &, const FC &, const double *, double *, Scalars *, int, Scalars *,
                         hipStream_t, bool);
 
+#ifdef FST_FAST64_TU  // the fast FP64 form: FP64 plain stage kernels only
+bool pick(int ns, bool f32, bool lc, StageFn *sf, PrimFn *pf) {
+  if (f32 || lc) return false;
+  switch (ns) {
+    case 1: *sf = run_stage<1, false, false>; *pf = run_prims<1, false>; return true;
+    case 2: *sf = run_stage<2, false, false>; *pf = run_prims<2, false>; return true;
+    case 3: *sf = run_stage<3, false, false>; *pf = run_prims<3, false>; return true;
+    default: return false;
+  }
+}
+#else
 template <int NS>
 StageFn pick_ns(bool f32, bool lc) {
   return f32 ? (lc ? run_stage<NS, true, true> : run_stage<NS, true, false>)
@@ -1905,6 +2001,7 @@ bool pick(int ns, bool f32, bool lc, StageFn *sf, PrimFn *pf) {
     default: return false;
   }
 }
+#endif  // FST_FAST64_TU
 
 #endif  // FST_GEN3D_TU
 
@@ -1932,6 +2029,7 @@ void launch_gen_adv(const Geo &g, const Phys &ph, const fst_config &cf, const do
 }
 
 #else
+#ifndef FST_FAST64_TU
 bool gen_adv_supported(const fst_config &cf, const Geo &g) {
   // 32-bit element offsets: the largest array (nvt planes, or the 5 primitive
   // planes) below 2^29 elements
@@ -1940,7 +2038,8 @@ bool gen_adv_supported(const fst_config &cf, const Geo &g) {
 }
 
 bool fused_supported(const fst_config &cf) {
-  if (!(cf.ndim == 3 && cf.grid == 0 && (cf.precision == FST_FP64 || cf.precision == FST_FP32_BULK) &&
+  if (!(cf.ndim == 3 && cf.grid == 0 &&
+        (cf.precision == FST_FP64 || cf.precision == FST_FP32_BULK || cf.precision == FST_FP64_FAST) &&
         cf.scheme == FST_WENO5 && cf.ns >= 1 && cf.ns <= 3 && (!cf.learned_c || cf.ceq)))
     return false;
   // 32-bit element offsets: the largest array (nvt planes) below 2^29 elements
@@ -1949,8 +2048,16 @@ bool fused_supported(const fst_config &cf) {
   return N * std::max(nvt, 6 + cf.ns) < 536870912.0;
 }
 
+#endif  // FST_FAST64_TU
+
+#ifdef FST_FAST64_TU
+int fused_steps_fast64(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb,
+                       LearnedBufs *lb, int nsteps, hipStream_t s, std::string *err) {
+#else
 int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb, LearnedBufs *lb,
                 int nsteps, hipStream_t s, std::string *err) {
+  if (cf.precision == FST_FP64_FAST) return fused_steps_fast64(g, ph, cf, b, fb, lb, nsteps, s, err);
+#endif
   StageFn sf;
   PrimFn pf;
   const bool lc = cf.learned_c != 0;
@@ -2054,6 +2161,7 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
   return FST_OK;
 }
 
+#ifndef FST_FAST64_TU
 int fused_primitives(const Geo &g, const Phys &ph, const fst_config &cf, const double *U, double *P, hipStream_t s) {
   StageFn sf;
   PrimFn pf;
@@ -2075,6 +2183,7 @@ void fused_free(FusedBufs &fb) {
   fb.stage = fb.stage2 = fb.prim = nullptr;
   fb.sc = nullptr;
 }
+#endif  // FST_FAST64_TU
 
 #endif  // FST_GEN3D_TU
 

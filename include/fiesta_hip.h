@@ -48,8 +48,12 @@ enum fst_scheme { FST_WENO5 = 1, FST_CENTERED4 = 2, FST_QUICK = 3 };
 
 /* Arithmetic forms of the path. */
 enum fst_precision {
-  FST_FP64 = 0,      /* reference arithmetic, bitwise parity target */
-  FST_FP32_BULK = 1  /* FP32 stencils/fluxes, FP64 EOS and RK accumulation */
+  FST_FP64 = 0,       /* reference arithmetic, bitwise parity target */
+  FST_FP32_BULK = 1,  /* FP32 stencils/fluxes, FP64 EOS and RK accumulation */
+  FST_FP64_FAST = 2   /* FP64 throughout with FMA contraction, reciprocal-based
+                         division and one-division WENO5 weights: within
+                         relative L2 1e-12 of the reference arithmetic, not
+                         bitwise (fused 3D path only) */
 };
 
 /* Arrays reachable through fst_upload / fst_download.  Extents in brackets. */

@@ -116,15 +116,16 @@ def test_learned_entry_points_reject_null_arguments():
     assert lib.fst_mlp_harvest(None, None, 0, 0, None) == 1
 
 
-@pytest.mark.parametrize("field,value", [("precision", 2), ("precision", -1)])
+@pytest.mark.parametrize("field,value", [("precision", 3), ("precision", -1)])
 def test_unknown_precision_is_rejected(field, value):
     cfg = make_config(ndim=3, ns=1, nci=8, ncj=8, nck=8)
     setattr(cfg, field, value)
     assert runtime.lib().fst_config_dims(C.byref(cfg), C.byref(FstDims())) == 1
 
 
+@pytest.mark.parametrize("precision", [1, 2])
 @pytest.mark.parametrize("case", ["2d", "ns4", "gen3d", "over_2_29"])
-def test_fp32_form_needs_the_fused_3d_path(case):
+def test_fp32_and_fast64_forms_need_the_fused_3d_path(case, precision):
     """The FP32 form exists only in the fused 3D stage kernels; elsewhere the
     FP64 sequence would run without notice, so the configuration is refused."""
     kw = dict(ndim=3, ns=2, nci=8, ncj=8, nck=8, gamma=(1.4, 1.67), M=(0.029, 0.004), mu=(1e-5, 1e-5), ceq=True)
@@ -140,9 +141,9 @@ def test_fp32_form_needs_the_fused_3d_path(case):
     ok = make_config(**kw)
     if case == "over_2_29":
         assert lib.fst_config_dims(C.byref(ok), C.byref(FstDims())) == 0  # FP64 runs on the sequence kernels
-    cfg = make_config(precision=1, **kw)
+    cfg = make_config(precision=precision, **kw)
     assert lib.fst_config_dims(C.byref(cfg), C.byref(FstDims())) == 1
-    good = make_config(precision=1, **dict(kw, ndim=3, grid=0, nci=8, ncj=8, nck=8, ns=2, gamma=(1.4, 1.67),
+    good = make_config(precision=precision, **dict(kw, ndim=3, grid=0, nci=8, ncj=8, nck=8, ns=2, gamma=(1.4, 1.67),
                                            M=(0.029, 0.004), mu=(1e-5, 1e-5)))
     assert lib.fst_config_dims(C.byref(good), C.byref(FstDims())) == 0
 
@@ -163,3 +164,10 @@ def test_static_step_refuses_before_touching_a_device():
     assert lib.fst_step_static(C.byref(ok), None, 1, None) == 1  # null state
     assert lib.fst_step_static(C.byref(ok), dummy, -1, None) == 1
     assert lib.fst_step_static(None, dummy, 1, None) == 1
+
+
+def test_fast64_form_has_no_learned_variant():
+    kw = dict(ndim=3, ns=2, nci=8, ncj=8, nck=8, gamma=(1.4, 1.67), M=(0.029, 0.004), mu=(1e-5, 1e-5), ceq=True)
+    lib = runtime.lib()
+    assert lib.fst_config_dims(C.byref(make_config(precision=2, **kw)), C.byref(FstDims())) == 0
+    assert lib.fst_config_dims(C.byref(make_config(precision=2, learned_c=True, **kw)), C.byref(FstDims())) == 1
