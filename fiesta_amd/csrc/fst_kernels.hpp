@@ -140,7 +140,8 @@ void launch_pack(const double *src, long long nx, long long ny, int ndim, int of
 // Noise filter of postStep on var (cartesian modules; cf.noise = 1).
 int noise_flag_count(const Geo &g, bool gen);
 void launch_noise(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);
-void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
+void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s);   // 32-byte elements
+void launch_copy8(const void *src, void *dst, size_t bytes, hipStream_t s);  // 8-byte elements
 // statusCheck reductions: max and min over the interior cells of each of the
 // first nvar variables of `a` (planes of g.N values), into mm as order keys
 // (mm[2v] max, mm[2v+1] min); decode with key_to_double.

@@ -969,9 +969,34 @@ __global__ void k_pack(const double *__restrict__ src, long long nx, long long n
 }
 
 // plain copy, 16 bytes per lane per iteration (bandwidth probe)
-__global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long long n) {
+// Streaming copies for the bandwidth ceiling.  k_copy32: 32-byte elements,
+// four in flight per thread per iteration (consecutive threads on
+// consecutive elements); the fastest of the shapes swept on MI355X (8- and
+// 32-byte elements, 1 or 4 per thread, 1024..16384 blocks of 256, with and
+// without nontemporal hints).  k_copy8: one double per thread per iteration,
+// the access width of the stage kernels, for scaling the byte counters.
+typedef double dv4 __attribute__((ext_vector_type(4)));
+__global__ void k_copy32(const dv4 *__restrict__ src, dv4 *__restrict__ dst, long long n) {
+  constexpr int U = 4;
+  long long base = (long long)blockIdx.x * blockDim.x * U + threadIdx.x;
+  const long long step = (long long)gridDim.x * blockDim.x * U;
+  for (; base < n; base += step) {
+    dv4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = base + (long long)u * blockDim.x;
+      if (i < n) v[u] = src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = base + (long long)u * blockDim.x;
+      if (i < n) dst[i] = v[u];
+    }
+  }
+}
+__global__ void k_copy8(const double *__restrict__ src, double *__restrict__ dst, long long n) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long stride = (long long)gridDim.x * blockDim.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
   for (; t < n; t += stride) dst[t] = src[t];
 }
 
@@ -1425,8 +1450,11 @@ void launch_pack(const double *src, long long nx, long long ny, int ndim, int of
 }
 
 void launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {
-  long long n = (long long)(bytes / 16);
-  k_copy<<<256 * 8 * 4, 256, 0, s>>>((const double2 *)src, (double2 *)dst, n);
+  k_copy32<<<8192, 256, 0, s>>>((const dv4 *)src, (dv4 *)dst, (long long)(bytes / 32));
+}
+
+void launch_copy8(const void *src, void *dst, size_t bytes, hipStream_t s) {
+  k_copy8<<<8192, 256, 0, s>>>((const double *)src, (double *)dst, (long long)(bytes / 8));
 }
 
 void launch_minmax(const Geo &g, const double *a, int nvar, unsigned long long *mm, hipStream_t s) {

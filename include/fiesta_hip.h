@@ -265,9 +265,11 @@ int fst_pack(fst_ctx *ctx, int array, int vn, const int64_t start[3], const int6
  * counts as smaller than +0. */
 int fst_status(fst_ctx *ctx, int array, double *min, double *max, int nvar);
 
-/* Device-to-device copy of `bytes` bytes with a plain vectorized HIP kernel,
- * `reps` times, on the context stream; returns the mean ms per copy.  Used to
- * measure the attainable HBM bandwidth and to scale rocprof byte counters. */
+/* Device-to-device copy of `bytes` bytes (rounded down to a multiple of 32)
+ * with a streaming HIP kernel (32-byte elements, four in flight per thread),
+ * `reps` times, on the context stream; returns the mean ms per copy.  One
+ * untimed copy with 8-byte elements runs first.  Used to measure the
+ * attainable HBM bandwidth and to scale rocprof byte counters. */
 int fst_copy_probe(fst_ctx *ctx, size_t bytes, int reps, double *ms_per_copy);
 
 /* ------------------------------------------------------------------------
