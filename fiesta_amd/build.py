@@ -58,6 +58,7 @@ def build_hip(verbose=False):
 
 HOST_DIR = os.path.join(PKG, "host")
 RUN_BIN = os.path.join(PKG, "fiesta_run")
+MAIN_BIN = os.path.join(PKG, "fiesta")  # runs a Lua input deck (the reference's main program)
 HOST_LIB = os.path.join(PKG, "libfiesta_host.so")
 # HDF5 (serial, C API) for the solution/restart files.  The library is linked
 # through a directory holding only a link to it, so the run-time search path
@@ -83,23 +84,25 @@ def _hdf5_link():
 
 
 def build_host():
-    """The host rk_func layer and the HDF5/XDMF writers and restart reader
-    (C++, no HIP code) as libfiesta_host.so, and the fiesta_run driver;
+    """The host rk_func layer, the HDF5/XDMF writers and restart reader and
+    the Lua deck reader (C++, no HIP code) as libfiesta_host.so, and the
+    fiesta_run and fiesta drivers;
     linked against libfiesta_hip.so (run-time path: the package directory)."""
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
     inc, h5 = _hdf5_link()
-    common = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + HOST_DIR]
+    common = ["g++", "-O2", "-std=c++17", "-Wall", "-pthread", "-I" + os.path.join(ROOT, "include"), "-I" + HOST_DIR]
     rpath = ["-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/_hdf5", "-Wl,-rpath-link," + os.path.join(rocm, "lib"),
              "-Wl,-rpath," + os.path.join(rocm, "lib")]
-    srcs = [os.path.join(HOST_DIR, f) for f in ("fiesta_host.cpp", "fiesta_io.cpp")]
+    srcs = [os.path.join(HOST_DIR, f) for f in ("fiesta_host.cpp", "fiesta_io.cpp", "fiesta_lua.cpp", "fiesta_deck.cpp")]
     tmp = HOST_LIB + ".tmp"
     _run(common + ["-fPIC", "-shared", "-isystem", inc] + srcs + [h5, "-L" + PKG, "-lfiesta_hip"] + rpath +
          ["-o", tmp])
     os.replace(tmp, HOST_LIB)
-    tmp = RUN_BIN + ".tmp"
-    _run(common + [os.path.join(HOST_DIR, "fiesta_run.cpp"), "-L" + PKG, "-lfiesta_host", "-lfiesta_hip"] + rpath +
-         ["-o", tmp])
-    os.replace(tmp, RUN_BIN)
+    for src, dst in (("fiesta_run.cpp", RUN_BIN), ("fiesta_main.cpp", MAIN_BIN)):
+        tmp = dst + ".tmp"
+        _run(common + [os.path.join(HOST_DIR, src), "-L" + PKG, "-lfiesta_host", "-lfiesta_hip"] + rpath +
+             ["-o", tmp])
+        os.replace(tmp, dst)
     return RUN_BIN
 
 

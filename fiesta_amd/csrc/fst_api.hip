@@ -253,6 +253,15 @@ int fst_config_dims(const fst_config *cfg, fst_dims *out) {
   return derive(*cfg, out, &why);
 }
 
+int fst_config_check(const fst_config *cfg, char *why, size_t len) {
+  if (!cfg) return FST_ERR_ARG;
+  fst_dims d;
+  std::string w;
+  const int rc = derive(*cfg, &d, &w);
+  if (why && len) std::snprintf(why, len, "%s", w.c_str());
+  return rc;
+}
+
 int fst_create(const fst_config *cfg, fst_ctx **out) {
   if (!cfg || !out) return FST_ERR_ARG;
   *out = nullptr;

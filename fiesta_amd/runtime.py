@@ -21,6 +21,7 @@ _EXPORTS = {
     "fst_abi_version": ([], C.c_int),
     "fst_config_init": ([C.c_void_p], None),
     "fst_config_dims": ([C.c_void_p, C.c_void_p], C.c_int),
+    "fst_config_check": ([C.c_void_p, C.c_char_p, C.c_size_t], C.c_int),
     "fst_create": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "fst_destroy": ([C.c_void_p], C.c_int),
     "fst_last_error": ([C.c_void_p], C.c_char_p),

@@ -139,6 +139,9 @@ typedef struct fst_dims {
 int fst_abi_version(void);
 void fst_config_init(fst_config *cfg);
 int fst_config_dims(const fst_config *cfg, fst_dims *out);
+/* The same validation as fst_config_dims, with the reason for a refusal
+ * written to `why` (NUL-terminated, empty on success).  Host only. */
+int fst_config_check(const fst_config *cfg, char *why, size_t len);
 
 /* Allocate the device arrays of a module (cart3d_func ctor, src/cart3d.cpp:44-174;
  * cart2d_func ctor, src/cart2d.cpp:30-134).  One HIP stream per context. */
