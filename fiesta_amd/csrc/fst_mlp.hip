@@ -393,16 +393,13 @@ __device__ __forceinline__ void hidden_mm(const short *act, const short *WT, f32
 }
 
 // y of 16 rows through the network; each wave handles tiles of 16 listed
-// cells.  Lanes 0..15 then apply the residual check (maximum principle of the
-// relaxation: 0 <= y <= max G1 over the domain, with tolerance tol) and write
-// C_hat = y rho / h; a failed check takes the local steady state of the
-// C-equation relaxation, the 3x3x3 mean of its source G1.
+// cells.  Lanes 0..15 write C_hat = max(y, 0) rho / h (NaN for a non-finite
+// y, which the residual check then rejects).
 __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__restrict__ F,
                                                    const unsigned *__restrict__ list, const unsigned *__restrict__ cnt,
                                                    const short *__restrict__ Wb, const float *__restrict__ W,
                                                    const float *__restrict__ G1, const double *__restrict__ P,
-                                                   int pf32, double hmin, float tol, float *__restrict__ chat,
-                                                   Scalars *sc, unsigned *__restrict__ nfallback) {
+                                                   int pf32, double hmin, float *__restrict__ chat) {
   __shared__ InfLds L;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   for (int q = tid; q < MLP_H * MLP_IN_PAD; q += 256) L.W1[(q / MLP_IN_PAD) * W1S + q % MLP_IN_PAD] = Wb[INF_W1 + q];
@@ -429,10 +426,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__re
   }
   __syncthreads();
   const unsigned n = cnt[0];
-  const float gmax_all = (float)as_double(sc->maxC);
   short *act = L.act[w];
-  float mloc = 0.f;
-  unsigned nfb = 0;
   const unsigned ntiles = (n + 15) / 16;
   for (unsigned tile = blockIdx.x * 4 + w; tile < ntiles; tile += gridDim.x * 4) {
     const unsigned r = tile * 16 + (l & 15);
@@ -481,30 +475,82 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__re
     if (l < 16 && tile * 16 + l < n) {
       float y = L.b4;
       for (int q = 0; q < MLP_H; ++q) y += bf2f(act[l * WHS + q]) * L.w4[q];
-      float gsum = 0.f;
-      for (int dk = -1; dk <= 1; ++dk)
-        for (int dj = -1; dj <= 1; ++dj)
-          for (int di = -1; di <= 1; ++di) gsum += G1[cell + di + dj * s1 + dk * s2];
-      const float bound = (1.0f + tol) * gmax_all + tol;
-      float yy;
-      if (isfinite(y) && y >= -tol && y <= bound) {
-        yy = fmaxf(y, 0.f);
-      } else {
-        yy = gsum * (1.0f / 27.0f);
-        ++nfb;
-      }
       const double rho = pf32 ? prim<true>(P, g.N, 0, cell) : prim<false>(P, g.N, 0, cell);
-      const float ch = (float)((double)yy * rho / hmin);
-      chat[cell] = ch;
-      mloc = fmaxf(mloc, ch);
+      chat[cell] = isfinite(y) ? (float)((double)fmaxf(y, 0.f) * rho / hmin) : __builtin_nanf("");
     }
     wave_sync();
   }
+}
+
+// Residual check of the prediction against the plain method, and its
+// fallback.  The relaxation of updateCeq (src/ceq3d.hpp:189-224),
+//   d C_hat / dt = maxS / (eps |dx|) (G - C_hat) + kap maxS |dx| lap(C_hat),
+//   lap(c) = sum_d (c[-e_d] - 2 c + c[+e_d]) / d_d,   |dx|^2 = dx^2 + dy^2 + dz^2,
+// is at rest where r = G - C_hat + a lap(C_hat) = 0 with a = kap eps |dx|^2
+// (G: the contact source indicator |grad rho| = G1 rho / h).  The plain
+// method's own C_hat keeps |r| within 2 % of max G once past its start-up
+// transient.  A listed cell whose prediction has |r| h / rho > tol max G1 (or
+// is not finite) takes the plain C-equation's value there: the local solution
+// of the resting relaxation with the neighbours held,
+//   C_hat = (G + a sum_d (c[-e_d] + c[+e_d]) / d_d) / (1 + 2 a sum_d 1 / d_d).
+// Reads chat (ghosts filled), writes out[cell] for the listed cells (Jacobi:
+// every cell sees the predictions of its neighbours); max C_hat and the
+// fallback count go to sc / nfallback.
+struct CheckConst {
+  double a, rd[3], rden, hmin;
+  float tol;
+};
+__global__ void __launch_bounds__(256) k_ceq_check(const G3 g, const unsigned *__restrict__ list,
+                                                   const unsigned *__restrict__ cnt, const float *__restrict__ chat,
+                                                   const float *__restrict__ G1, const double *__restrict__ P,
+                                                   int pf32, CheckConst cc, float *__restrict__ out, Scalars *sc,
+                                                   unsigned *__restrict__ nfallback) {
+  const unsigned n = cnt[0];
+  const float gmax = (float)as_double(sc->maxC);
+  const long long st[3] = {1, g.s1, g.s2};
+  float mloc = 0.f;
+  unsigned nfb = 0;
+  for (unsigned r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const long long cell = (long long)list[r];
+    const double rho = pf32 ? prim<true>(P, g.N, 0, cell) : prim<false>(P, g.N, 0, cell);
+    const double G = (double)G1[cell] * rho / cc.hmin;
+    const double c = (double)chat[cell];
+    double lap = 0.0, nsum = 0.0;
+    bool fin = isfinite(c);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double m = (double)chat[cell - st[d]], p = (double)chat[cell + st[d]];
+      fin = fin && isfinite(m) && isfinite(p);
+      if (!isfinite(m)) m = 0.0;
+      if (!isfinite(p)) p = 0.0;
+      lap += (m - 2.0 * c + p) * cc.rd[d];
+      nsum += (m + p) * cc.rd[d];
+    }
+    const double res = G - c + cc.a * lap;
+    double v = c;
+    if (!(fin && fabs(res) * cc.hmin / rho <= (double)cc.tol * (double)gmax)) {
+      v = (G + cc.a * nsum) * cc.rden;
+      ++nfb;
+    }
+    const float f = (float)v;
+    out[cell] = f;
+    mloc = fmaxf(mloc, f);
+  }
   mloc = wave_maxf(mloc);
   for (int o = 32; o > 0; o >>= 1) nfb += __shfl_xor(nfb, o, 64);
-  if (l == 0) {
+  if ((threadIdx.x & 63) == 0) {
     if (mloc > 0.f) atomic_max_nonneg(&sc->maxCh, (double)mloc);
     if (nfb) atomicAdd(nfallback, nfb);
+  }
+}
+
+// chat[cell] = src[cell] for the listed cells
+__global__ void k_copy_listed(const unsigned *__restrict__ list, const unsigned *__restrict__ cnt,
+                              const float *__restrict__ src, float *__restrict__ chat) {
+  const unsigned n = cnt[0];
+  for (unsigned r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const unsigned c = list[r];
+    chat[c] = src[c];
   }
 }
 
@@ -690,6 +736,7 @@ int learned_alloc(const Geo &g, LearnedBufs &L) {
             hipMalloc((void **)&L.dir, N * 3 * sizeof(float)) == hipSuccess &&
             hipMalloc((void **)&L.G1, N * sizeof(float)) == hipSuccess &&
             hipMalloc((void **)&L.chat, N * sizeof(float)) == hipSuccess &&
+            hipMalloc((void **)&L.chat2, N * sizeof(float)) == hipSuccess &&
             hipMalloc((void **)&L.list, N * sizeof(unsigned)) == hipSuccess &&
             hipMalloc((void **)&L.cnt, 4 * sizeof(unsigned)) == hipSuccess &&
             hipMalloc((void **)&L.W, MLP_NPARAM * sizeof(float)) == hipSuccess &&
@@ -721,15 +768,25 @@ int learned_fields(const Geo &g, const fst_config &cf, const Phys &ph, const dou
   const dim3 ag((g.ngi + 63) / 64, (g.ngj + 3) / 4, (g.ngk + ACT_KCH - 1) / ACT_KCH);
   k_active3d<<<ag, dim3(64, 4), 0, s>>>(q, L.M, L.tau, 0u, L.list, L.cnt, L.chat);
   const double hmin = std::min(cf.dx, std::min(cf.dy, cf.dz));
-  k_mlp_infer<<<1024, 256, 0, s>>>(q, L.F, L.list, L.cnt, L.Wb, L.W, L.G1, P, pf32 ? 1 : 0, hmin, L.tol, L.chat,
-                                    L.sc, L.cnt + 1);
+  k_mlp_infer<<<1024, 256, 0, s>>>(q, L.F, L.list, L.cnt, L.Wb, L.W, L.G1, P, pf32 ? 1 : 0, hmin, L.chat);
   const long long nf = 2LL * ((long long)g.ncj * g.nck + (long long)g.nci * g.nck + (long long)g.nci * g.ncj);
+  k_chat_ghost<<<nblk(nf), 256, 0, s>>>(q, cf.xPer, cf.yPer, cf.zPer, L.chat);
+  CheckConst cc;
+  cc.a = cf.kap * cf.eps * (cf.dx * cf.dx + cf.dy * cf.dy + cf.dz * cf.dz);
+  cc.rd[0] = 1.0 / cf.dx;
+  cc.rd[1] = 1.0 / cf.dy;
+  cc.rd[2] = 1.0 / cf.dz;
+  cc.rden = 1.0 / (1.0 + 2.0 * cc.a * (cc.rd[0] + cc.rd[1] + cc.rd[2]));
+  cc.hmin = hmin;
+  cc.tol = L.tol;
+  k_ceq_check<<<1024, 256, 0, s>>>(q, L.list, L.cnt, L.chat, L.G1, P, pf32 ? 1 : 0, cc, L.chat2, L.sc, L.cnt + 1);
+  k_copy_listed<<<1024, 256, 0, s>>>(L.list, L.cnt, L.chat2, L.chat);
   k_chat_ghost<<<nblk(nf), 256, 0, s>>>(q, cf.xPer, cf.yPer, cf.zPer, L.chat);
   return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 
 void learned_free(LearnedBufs &L) {
-  void *ps[] = {L.F, L.M, L.dir, L.G1, L.chat, L.list, L.cnt, L.W, L.Wb, L.sc};
+  void *ps[] = {L.F, L.M, L.dir, L.G1, L.chat, L.chat2, L.list, L.cnt, L.W, L.Wb, L.sc};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   L = LearnedBufs();

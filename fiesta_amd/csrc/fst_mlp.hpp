@@ -41,13 +41,14 @@ struct LearnedBufs {
   float *dir = nullptr;   // 3 planes: 4th-order grad rho (the direction n)
   float *G1 = nullptr;    // contact-switch source, dimensionless
   float *chat = nullptr;  // C_hat, dimensional (units of |grad rho|)
+  float *chat2 = nullptr; // the residual check's output for the listed cells
   unsigned *list = nullptr;
   unsigned *cnt = nullptr;  // [0] active cells, [1] fallbacks
   float *W = nullptr;       // FP32 parameters
   short *Wb = nullptr;      // BF16 copy used by inference
   Scalars *sc = nullptr;    // maxCh (max C_hat), maxC (max G1 as double)
   float tau = 1.0e-3f;      // activity threshold on the features
-  float tol = 0.05f;        // residual-check tolerance
+  float tol = 0.05f;        // residual-check tolerance (fraction of max G1)
   bool have_weights = false;
 };
 

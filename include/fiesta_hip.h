@@ -285,10 +285,13 @@ int fst_copy_probe(fst_ctx *ctx, size_t bytes, int reps, double *ms_per_copy);
  * f1 = |grad p| h/p, f2 = (div u) h/a (h = min cell size) on every cell; an
  * MLP (5x5x5 stencil x 3 features -> 64 -> 64 -> 64 -> 1, GELU, BF16 MFMA,
  * FP32 accumulation) predicts y = C_hat h/rho on the cells whose 3x3x3
- * neighbourhood has a feature above tau (others: C_hat = 0).  A prediction
- * outside [-tol, (1+tol) max G + tol] (G = contact-switch source
- * indicator*|grad rho| h/rho, whose maximum bounds the relaxed C_hat) falls
- * back to the 3x3x3 mean of G, the local steady state of the relaxation.
+ * neighbourhood has a feature above tau (others: C_hat = 0).  The residual
+ * check: the prediction must nearly satisfy the C-equation relaxation at
+ * rest, |G - C_hat + a lap(C_hat)| h / rho <= tol max G1 (a = kap eps |dx|^2,
+ * G = contact-switch source indicator*|grad rho|, lap with the reference's
+ * 1/d, src/ceq3d.hpp:189-224); a cell that fails (or is not finite) takes
+ * the plain C-equation's value there, the local solution of that resting
+ * equation with the neighbouring predictions held.
  * The direction n is the 4th-order density gradient.  Both stages of the
  * step use these fields.
  * ------------------------------------------------------------------------ */
@@ -298,7 +301,8 @@ int fst_copy_probe(fst_ctx *ctx, size_t bytes, int reps, double *ms_per_copy);
 int fst_mlp_nparams(void);
 /* Weights of a learned_c context (copied; required before fst_step). */
 int fst_set_mlp_weights(fst_ctx *ctx, const float *w, size_t n);
-/* Activity threshold tau (default 1e-3) and residual tolerance tol (0.05). */
+/* Activity threshold tau (default 1e-3) and residual tolerance tol (0.05,
+ * a fraction of max G1; a huge value switches the fallback off). */
 int fst_learned_params(fst_ctx *ctx, float tau, float tol);
 /* Ghost fill, primitives and learned fields of the current var, outside a
  * step (inspection and tests). */

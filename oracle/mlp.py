@@ -2,7 +2,8 @@
 
 The network (5x5x5 stencil x 3 features -> 64 -> 64 -> 64 -> 1, GELU tanh
 form on the hidden layers, mean-squared-error loss), its gradient, the
-inference-side residual check with its fallback and the ghost layer of C_hat,
+inference-side residual check (the resting C-equation relaxation) with its
+fallback to the plain method's local solution, and the ghost layer of C_hat,
 all in float64.  With `bf16=True` the operands of every matrix product are
 rounded to bfloat16 at the points where the GPU kernels round them (inputs,
 weights W1..W3, hidden activations, the back-propagated dZ), so GPU results
@@ -119,26 +120,8 @@ def active_cells(M, dims, tau):
     return np.flatnonzero((mx > tau) & inter)
 
 
-def chat_field(w, F, M, G, rho, dims, hmin, tau=1e-3, tol=0.05, xper=False, yper=False, zper=False):
-    """C_hat over the grid as the GPU inference computes it (bf16 network),
-    with the residual check / fallback and the face ghost layer.  Returns
-    (chat, y, fallback mask, active cells, bound)."""
-    cells = active_cells(M, dims, tau)
-    X = stencils(F, cells, dims)
-    y, _ = forward(w, X, bf=True)
-    Gf = np.asarray(G, np.float64).ravel()
-    g_inter = np.asarray(G, np.float32)[dims.ng:dims.ng + dims.nck, dims.ng:dims.ng + dims.ncj,
-                                        dims.ng:dims.ng + dims.nci]
-    gmax = float(g_inter.max()) if g_inter.size else 0.0
-    bound = (1.0 + tol) * gmax + tol
-    ok = np.isfinite(y) & (y >= -tol) & (y <= bound)
-    s1, s2 = dims.ngi, dims.ngi * dims.ngj
-    offs = np.array([dx + dy * s1 + dz * s2 for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)])
-    gmean = Gf[cells[:, None] + offs[None, :]].sum(axis=1) / 27.0
-    yy = np.where(ok, np.maximum(y, 0.0), gmean)
-    ch = np.zeros(dims.N)
-    ch[cells] = yy * np.asarray(rho, np.float64).ravel()[cells] / hmin
-    ch = ch.reshape(dims.ngk, dims.ngj, dims.ngi)
+def _chat_ghost(ch, dims, xper, yper, zper):
+    """The face ghost layer of C_hat: periodic image or mirror copy."""
     g = dims.ng
     I, J, K = slice(g, g + dims.nci), slice(g, g + dims.ncj), slice(g, g + dims.nck)
     ch[K, J, g - 1] = ch[K, J, g + dims.nci - 1] if xper else ch[K, J, g]
@@ -147,4 +130,64 @@ def chat_field(w, F, M, G, rho, dims, hmin, tau=1e-3, tol=0.05, xper=False, yper
     ch[K, g + dims.ncj, I] = ch[K, g, I] if yper else ch[K, g + dims.ncj - 1, I]
     ch[g - 1, J, I] = ch[g + dims.nck - 1, J, I] if zper else ch[g, J, I]
     ch[g + dims.nck, J, I] = ch[g, J, I] if zper else ch[g + dims.nck - 1, J, I]
-    return ch, y, ~ok, cells, bound
+
+
+def residual_check(ch, G, rho, cells, dims, hmin, kap, eps, d, tol):
+    """The residual check of the prediction and its fallback, as the GPU pass
+    does it (float64 arithmetic on the float32 fields).  The C-equation
+    relaxation (src/ceq3d.hpp:189-224) rests where
+    r = G - C + a lap(C) = 0, a = kap eps (dx^2 + dy^2 + dz^2),
+    lap(c) = sum_d (c[-e_d] - 2c + c[+e_d]) / d_d, G = G1 rho / h.  A cell with
+    |r| h / rho > tol max G1 (or a non-finite value around it) takes the
+    resting relaxation's local solution with its neighbours held:
+    (G + a sum_d (c[-e_d] + c[+e_d]) / d_d) / (1 + 2 a sum_d 1 / d_d).
+    Returns (new C_hat values of `cells` as float32, fallback mask, r h / rho)."""
+    chf = np.asarray(ch, np.float32).ravel().astype(np.float64)
+    Gf = np.asarray(G, np.float32).ravel().astype(np.float64)
+    rf = np.asarray(rho, np.float64).ravel()
+    g = dims.ng
+    gmax = float(np.asarray(G, np.float32)[g:g + dims.nck, g:g + dims.ncj, g:g + dims.nci].max())
+    a = kap * eps * (d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+    st = (1, dims.ngi, dims.ngi * dims.ngj)
+    c = chf[cells]
+    fin = np.isfinite(c)
+    lap = np.zeros(len(cells))
+    nsum = np.zeros(len(cells))
+    for q in range(3):
+        m = chf[cells - st[q]]
+        p = chf[cells + st[q]]
+        fin &= np.isfinite(m) & np.isfinite(p)
+        m = np.where(np.isfinite(m), m, 0.0)
+        p = np.where(np.isfinite(p), p, 0.0)
+        lap += (m - 2.0 * c + p) * (1.0 / d[q])
+        nsum += (m + p) * (1.0 / d[q])
+    Gd = Gf[cells] * rf[cells] / hmin
+    res = Gd - c + a * lap
+    rn = np.abs(res) * hmin / rf[cells]
+    with np.errstate(invalid="ignore"):
+        ok = fin & (rn <= tol * gmax)
+    rden = 1.0 / (1.0 + 2.0 * a * (1.0 / d[0] + 1.0 / d[1] + 1.0 / d[2]))
+    v = np.where(ok, c, (Gd + a * nsum) * rden)
+    return v.astype(np.float32), ~ok, rn, tol * gmax
+
+
+def chat_field(w, F, M, G, rho, dims, hmin, kap, eps, d, tau=1e-3, tol=0.05, xper=False, yper=False, zper=False):
+    """C_hat over the grid as the GPU inference computes it (bf16 network):
+    C_hat = max(y, 0) rho / h on the active cells, the face ghost layer, the
+    residual check with its fallback, the ghost layer again.  Returns
+    (chat, y, fallback mask, active cells, normalised residuals, threshold)."""
+    cells = active_cells(M, dims, tau)
+    X = stencils(F, cells, dims)
+    y, _ = forward(w, X, bf=True)
+    rf = np.asarray(rho, np.float64).ravel()
+    ch = np.zeros(dims.N, np.float32)
+    with np.errstate(invalid="ignore"):
+        ch[cells] = np.where(np.isfinite(y), np.maximum(y, 0.0) * rf[cells] / hmin, np.nan).astype(np.float32)
+    ch = ch.reshape(dims.ngk, dims.ngj, dims.ngi)
+    _chat_ghost(ch, dims, xper, yper, zper)
+    v, fb, rn, thr = residual_check(ch, G, rho, cells, dims, hmin, kap, eps, d, tol)
+    out = ch.ravel().copy()
+    out[cells] = v
+    out = out.reshape(dims.ngk, dims.ngj, dims.ngi)
+    _chat_ghost(out, dims, xper, yper, zper)
+    return out, y, fb, cells, rn, thr

@@ -171,3 +171,40 @@ def test_learned_compute_reproduces_plain_terms():
     lrn.set_learned(2 * cvars[d.nv + 1], cvars[d.nv + 2:d.nv + 5], maxCh)
     lrn.compute()
     assert not np.array_equal(lrn.get(FST_DVAR)[:3], want[:3])
+
+
+def test_residual_check_rests_on_the_steady_relaxation():
+    """A field that satisfies the resting C-equation relaxation exactly
+    (G = C - a lap(C), G built from C) passes the check, and the fallback
+    value of every cell of it is the field itself (the local solution of the
+    resting equation); a spike planted at one cell fails there."""
+    from fiesta_amd.config import Dims, make_config
+    cfg = make_config(ndim=3, ns=1, nci=10, ncj=9, nck=8, dx=0.01, dy=0.012, dz=0.009, kap=10.0, eps=5.0)
+    d = Dims(cfg)
+    rng = np.random.default_rng(4)
+    C = rng.uniform(0.5, 2.0, d.shape(1)[1:])
+    h = min(cfg.dx, cfg.dy, cfg.dz)
+    dd = (cfg.dx, cfg.dy, cfg.dz)
+    a = cfg.kap * cfg.eps * sum(x * x for x in dd)
+    lap = np.zeros_like(C)
+    for ax, hh in zip((2, 1, 0), dd):
+        lap += (np.roll(C, 1, ax) - 2 * C + np.roll(C, -1, ax)) / hh
+    rho = rng.uniform(1.0, 1.5, C.shape)
+    G = (C - a * lap) * h / rho            # dimensionless source G1
+    g = d.ng
+    inner = np.zeros(C.shape, bool)
+    inner[g:-g, g:-g, g:-g] = True
+    cells = np.flatnonzero(inner)
+    C32 = C.astype(np.float32)
+    G = np.where(inner, G, 0.0)
+    # float32 storage of C and G leaves a residual of a few ulps of the terms
+    v, fb, rn, thr = mlp.residual_check(C32, G.astype(np.float32), rho, cells, d, h, cfg.kap, cfg.eps, dd, 1e-4)
+    assert not fb.any()
+    v0, fb0, _, _ = mlp.residual_check(C32, G.astype(np.float32), rho, cells, d, h, cfg.kap, cfg.eps, dd, 0.0)
+    assert fb0.all()
+    np.testing.assert_allclose(v0, C32.ravel()[cells], rtol=2e-5)
+    spike = C32.copy()
+    c0 = cells[len(cells) // 2]
+    spike.ravel()[c0] *= 1.5
+    _, fbs, _, _ = mlp.residual_check(spike, G.astype(np.float32), rho, cells, d, h, cfg.kap, cfg.eps, dd, 1e-3)
+    assert fbs[list(cells).index(c0)]
