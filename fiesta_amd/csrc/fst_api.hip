@@ -77,8 +77,16 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
     *why = "generalized grids run the reference arithmetic (FP64, no learned C-indicator)";
     return FST_ERR_ARG;
   }
-  if (c.learned_c && (c.ndim != 3 || !c.ceq)) {
-    *why = "the learned C-indicator replaces the 3D C-equation: needs ndim = 3 and ceq = 1";
+  if (c.learned_c && !c.ceq) {
+    *why = "the learned C-indicator replaces the C-equation's relaxation solve: needs ceq = 1";
+    return FST_ERR_ARG;
+  }
+  if (c.learned_c && c.ndim == 2 && c.precision != FST_FP64) {
+    *why = "the 2D learned C-indicator runs on the FP64 cart2d kernel sequence (precision 0)";
+    return FST_ERR_ARG;
+  }
+  if (c.learned_c && c.noise && c.ceq) {
+    *why = "the noise filter's C_hat cut-off reads the C-equation variables, which the learned form does not store";
     return FST_ERR_ARG;
   }
   if (c.scheme < 1 || c.scheme > 3 || (c.ndim == 3 && c.scheme != FST_WENO5)) {
@@ -167,13 +175,30 @@ void free_all(fst_ctx *ctx) {
 // rk_func::compute() of the module the config selects: cart3d applies the
 // boundary conditions first (src/cart3d.cpp:221); cart2d applies them in
 // preStep; gen2d/gen3d never do (src/gen2d.cpp:100-131, src/gen3d.cpp:341-369).
-void compute_seq(fst_ctx *ctx) {
+// The 2D learned form evaluates its fields when `fields` is set (stage 1 of
+// a step) and uses the last ones otherwise.
+int compute_seq(fst_ctx *ctx, bool fields = false) {
   if (ctx->cf.grid == 1) {
     launch_compute_gen(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->stream);
-    return;
+    return FST_OK;
   }
   if (ctx->g.ndim == 3) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
+  if (ctx->cf.learned_c && ctx->g.ndim == 2) {
+    const Bufs &b = ctx->b;
+    launch_eos_vel(ctx->g, ctx->ph, b, ctx->stream);
+    if (fields) {
+      const int rc = learned_fields2d(ctx->g, ctx->cf, ctx->ph, b.var, b.rho, b.p, b.vel, ctx->lb, ctx->stream);
+      if (rc) return rc;
+    }
+    Ceq2dFields lf;
+    lf.chat = ctx->lb.chat;
+    lf.dir = ctx->lb.dir;
+    lf.sc = ctx->lb.sc;
+    launch_compute_ref(ctx->g, ctx->ph, ctx->cf, b, ctx->stream, &lf, true);
+    return FST_OK;
+  }
   launch_compute_ref(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->stream);
+  return FST_OK;
 }
 
 int learned_only(fst_ctx *ctx, const char *what) {
@@ -303,7 +328,8 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
   ok = ok && alloc(&ctx->b.stress, c.visc ? (g.ndim == 3 ? 6 : 8) : 0);
   const bool ceq_vars = c.ceq && !c.learned_c;
   ok = ok && alloc(&ctx->b.gradRho, ceq_vars ? (g.ndim == 3 ? 5 : 4) : 0);
-  ok = ok && alloc(&ctx->b.mflux, ceq_vars ? (g.ndim == 3 ? 27 : 8) : 0);
+  // mFlux: the reference sequence's C-equation faces (the 2D learned form too)
+  ok = ok && alloc(&ctx->b.mflux, c.ceq && (ceq_vars || g.ndim == 2) ? (g.ndim == 3 ? 27 : 8) : 0);
   if (ok && c.grid == 1) {  // gen3d.cpp:41-43, gen2d.cpp:25-26
     ok = alloc(&ctx->b.metrics, (size_t)g.ndim * g.ndim);
     const size_t gbytes = grid_nodes(g) * 3 * sizeof(double);
@@ -317,8 +343,8 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
          hipMemset(ctx->b.nmax, 0, sizeof(unsigned long long)) == hipSuccess;
   }
   if (ok && c.learned_c) {
-    if (!fused_supported(c)) {
-      fprintf(stderr, "fst_create: the learned C-indicator needs the fused 3D path (ns <= 3, FP64 or FP32_BULK)\n");
+    if (g.ndim == 3 && !fused_supported(c)) {
+      fprintf(stderr, "fst_create: the 3D learned C-indicator needs the fused 3D path (ns <= 3, FP64 or FP32_BULK)\n");
       (void)fst_destroy(ctx);
       return FST_ERR_ARG;
     }
@@ -438,7 +464,7 @@ int fst_set_fused(fst_ctx *ctx, int fused) {
   if (!ctx) return FST_ERR_ARG;
   if (fused && !fused_supported(ctx->cf))
     return fail(ctx, FST_ERR_ARG, "fused path supports 3D cartesian FP64 with ceq, visc and buoyancy flags only");
-  if (!fused && ctx->cf.learned_c) return learned_only(ctx, "fst_set_fused(0)");
+  if (!fused && ctx->cf.learned_c && ctx->g.ndim == 3) return learned_only(ctx, "fst_set_fused(0)");
   if (!fused && ctx->cf.precision != FST_FP64) return fp32_only(ctx, "fst_set_fused(0)");
   ctx->fused = fused ? 1 : 0;
   return FST_OK;
@@ -456,7 +482,9 @@ int fst_step(fst_ctx *ctx, int nsteps) {
       if (ctx->g.ndim == 2 && ctx->cf.grid == 0) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
       const int ns = rk_stages(ctx->cf);
       for (int stage = 1; stage <= ns; ++stage) {
-        compute_seq(ctx);
+        const int rc = compute_seq(ctx, stage == 1);
+        if (rc)
+          return fail(ctx, rc, rc == FST_ERR_STATE ? "fst_step: no network weights set" : "fst_step: launch failed");
         launch_rk_stage(ctx->g, ns, stage, ctx->cf.dt, ctx->b.var, ctx->b.tmp1, ctx->b.dvar, ctx->stream);
       }
     }
@@ -627,10 +655,13 @@ int fst_sync(fst_ctx *ctx) {
 
 int fst_mlp_nparams(void) { return MLP_NPARAM; }
 
+int fst_mlp_nparams_nd(int ndim) { return ndim == 2 || ndim == 3 ? mlp_nparams(ndim) : -1; }
+
 int fst_set_mlp_weights(fst_ctx *ctx, const float *w, size_t n) {
   if (!ctx || !w) return FST_ERR_ARG;
   if (!ctx->cf.learned_c) return fail(ctx, FST_ERR_ARG, "fst_set_mlp_weights: context has learned_c = 0");
-  if (n != (size_t)MLP_NPARAM) return fail(ctx, FST_ERR_ARG, "fst_set_mlp_weights: wrong parameter count");
+  if (n != (size_t)mlp_nparams(ctx->lb.nd))
+    return fail(ctx, FST_ERR_ARG, "fst_set_mlp_weights: wrong parameter count for the context's dimension");
   const int rc = learned_set_weights(ctx->lb, w, ctx->stream);
   return rc ? fail(ctx, rc, "fst_set_mlp_weights: copy failed") : FST_OK;
 }
@@ -646,11 +677,18 @@ int fst_learned_params(fst_ctx *ctx, float tau, float tol) {
 int fst_learned_compute(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
   if (!ctx->cf.learned_c) return fail(ctx, FST_ERR_ARG, "fst_learned_compute: context has learned_c = 0");
-  const size_t np = 6 + (size_t)ctx->cf.ns;
-  if (!ctx->lprim) HIPCHK(ctx, hipMalloc((void **)&ctx->lprim, np * (size_t)ctx->g.N * sizeof(double)));
+  int rc = FST_OK;
   launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
-  int rc = fused_primitives(ctx->g, ctx->ph, ctx->cf, ctx->b.var, ctx->lprim, ctx->stream);
-  if (!rc) rc = learned_fields(ctx->g, ctx->cf, ctx->ph, ctx->lprim, false, ctx->lb, ctx->stream);
+  if (ctx->g.ndim == 2) {
+    const Bufs &b = ctx->b;
+    launch_eos_vel(ctx->g, ctx->ph, b, ctx->stream);
+    rc = learned_fields2d(ctx->g, ctx->cf, ctx->ph, b.var, b.rho, b.p, b.vel, ctx->lb, ctx->stream);
+  } else {
+    const size_t np = 6 + (size_t)ctx->cf.ns;
+    if (!ctx->lprim) HIPCHK(ctx, hipMalloc((void **)&ctx->lprim, np * (size_t)ctx->g.N * sizeof(double)));
+    rc = fused_primitives(ctx->g, ctx->ph, ctx->cf, ctx->b.var, ctx->lprim, ctx->stream);
+    if (!rc) rc = learned_fields(ctx->g, ctx->cf, ctx->ph, ctx->lprim, false, ctx->lb, ctx->stream);
+  }
   if (rc) return fail(ctx, rc, rc == FST_ERR_STATE ? "fst_learned_compute: no network weights set"
                                                    : "fst_learned_compute: launch failed");
   ctx->fb.valid = false;
@@ -700,11 +738,13 @@ int mfail(fst_mlp *m, int code, const std::string &msg) {
 
 extern "C" {
 
-int fst_mlp_create(uint64_t seed, fst_mlp **out) {
-  if (!out) return FST_ERR_ARG;
+int fst_mlp_create(uint64_t seed, fst_mlp **out) { return fst_mlp_create_nd(3, seed, out); }
+
+int fst_mlp_create_nd(int ndim, uint64_t seed, fst_mlp **out) {
+  if (!out || (ndim != 2 && ndim != 3)) return FST_ERR_ARG;
   *out = nullptr;
   fst_mlp *m = new fst_mlp();
-  const int rc = trainer_create(m->t, seed);
+  const int rc = trainer_create(m->t, ndim, seed);
   if (rc) {
     trainer_free(m->t);
     delete m;
@@ -724,16 +764,18 @@ int fst_mlp_destroy(fst_mlp *m) {
 
 const char *fst_mlp_last_error(const fst_mlp *m) { return m ? m->err.c_str() : "null trainer"; }
 
+int fst_mlp_ndim(const fst_mlp *m) { return m ? m->t.nd : -1; }
+
 int fst_mlp_set_weights(fst_mlp *m, const float *w, size_t n) {
   if (!m || !w) return FST_ERR_ARG;
-  if (n != (size_t)MLP_NPARAM) return mfail(m, FST_ERR_ARG, "fst_mlp_set_weights: wrong parameter count");
+  if (n != (size_t)mlp_nparams(m->t.nd)) return mfail(m, FST_ERR_ARG, "fst_mlp_set_weights: wrong parameter count");
   const int rc = trainer_set_weights(m->t, w);
   return rc ? mfail(m, rc, "fst_mlp_set_weights: copy failed") : FST_OK;
 }
 
 int fst_mlp_get_weights(fst_mlp *m, float *w, size_t n) {
   if (!m || !w) return FST_ERR_ARG;
-  if (n != (size_t)MLP_NPARAM) return mfail(m, FST_ERR_ARG, "fst_mlp_get_weights: wrong parameter count");
+  if (n != (size_t)mlp_nparams(m->t.nd)) return mfail(m, FST_ERR_ARG, "fst_mlp_get_weights: wrong parameter count");
   const int rc = trainer_get_weights(m->t, w);
   return rc ? mfail(m, rc, "fst_mlp_get_weights: copy failed") : FST_OK;
 }
@@ -741,7 +783,23 @@ int fst_mlp_get_weights(fst_mlp *m, float *w, size_t n) {
 int fst_mlp_harvest(fst_mlp *m, fst_ctx *ctx, uint32_t stride, int64_t max_rows, int64_t *added) {
   if (!m || !ctx) return FST_ERR_ARG;
   const fst_config &c = ctx->cf;
-  if (c.ndim != 3 || c.grid != 0 || !c.ceq || c.learned_c || c.ns > 3 || !fused_supported(c))
+  if (c.ndim != m->t.nd)
+    return mfail(m, FST_ERR_ARG, "fst_mlp_harvest: the context's dimension differs from the network's");
+  if (c.ndim == 2) {
+    if (c.grid != 0 || !c.ceq || c.learned_c)
+      return mfail(m, FST_ERR_ARG, "fst_mlp_harvest: needs a 2D cartesian context with the C-equation");
+    // cart2d's state as its compute() sees it: ghosts filled in preStep, then primitives
+    launch_apply_bcs(ctx->g, c, ctx->b.var, ctx->stream);
+    launch_eos_vel(ctx->g, ctx->ph, ctx->b, ctx->stream);
+    int rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? FST_OK : FST_ERR_HIP;
+    long long a = 0;
+    if (!rc)
+      rc = trainer_harvest2d(m->t, ctx->g, c, ctx->ph, ctx->b.var, ctx->b.rho, ctx->b.p, ctx->b.vel, stride,
+                             max_rows, &a);
+    if (added) *added = a;
+    return rc ? mfail(m, rc, "fst_mlp_harvest: failed") : FST_OK;
+  }
+  if (c.grid != 0 || !c.ceq || c.learned_c || c.ns > 3 || !fused_supported(c))
     return mfail(m, FST_ERR_ARG,
                  "fst_mlp_harvest: needs a 3D cartesian context with the C-equation (ns <= 3, every array "
                  "below 2^29 elements)");

@@ -122,7 +122,17 @@ __device__ __forceinline__ double weno5_f64c(double f1, double f2, double f3, do
 }
 
 void launch_eos_vel(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
-void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);
+// Learned C-indicator fields of the 2D form: C_hat [N], the tangent [2][N]
+// and max C_hat (sc->maxCh), all on the device.
+struct Ceq2dFields {
+  const float *chat = nullptr, *dir = nullptr;
+  const Scalars *sc = nullptr;
+};
+// cart3d/cart2d compute() (src/cart3d.cpp:220-298, src/cart2d.cpp:151-237).
+// lf: the 2D learned form's fields (nullptr: the plain method); eos_done: the
+// primitives of this state are already in b.
+void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s,
+                        const Ceq2dFields *lf = nullptr, bool eos_done = false);
 void launch_apply_bcs(const Geo &g, const fst_config &cf, double *var, hipStream_t s);
 void launch_apply_bcs(const Geo &g, const fst_config &cf, float *var, hipStream_t s);  // float stage state
 // Stages per time step: 2 (RK2 midpoint, rkAdvance) or 3 (SSP-RK3, rk_order = 3).

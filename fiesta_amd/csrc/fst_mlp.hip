@@ -33,12 +33,10 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int W1S = MLP_IN_PAD + 4;  // LDS row stride (shorts) of W1^T: 8-byte reads spread over the banks
-constexpr int WHS = MLP_H + 4;       // LDS row stride of W2^T, W3^T and the activation tiles
-constexpr int INF_W1 = 0;            // inference copy: W1^T [64][384] (zero rows k >= 375), W2^T, W3^T
-constexpr int INF_W2 = MLP_H * MLP_IN_PAD;
-constexpr int INF_W3 = INF_W2 + MLP_H * MLP_H;
-constexpr int INF_N = INF_W3 + MLP_H * MLP_H;
+// LDS row stride (shorts) of W1^T: 8-byte reads spread over the banks
+template <int ND>
+constexpr int W1S = Net<ND>::IN_PAD + 4;
+constexpr int WHS = MLP_H + 4;  // LDS row stride of W2^T, W3^T and the activation tiles
 
 // round-to-nearest-even FP32 -> BF16 (finite inputs)
 __host__ __device__ __forceinline__ short f2bf(float x) {
@@ -295,13 +293,134 @@ __global__ void __launch_bounds__(256) k_active3d(const G3 g, const float *__res
   }
 }
 
+// 2D features (cart2d state, src/cart2d.cpp:151-237), one thread per cell:
+// F (cells with one neighbour each way), the tangent and G1 (two
+// neighbours each way), M = max |F|, from var (energy, species densities),
+// rho, p and vel.  The tangent (-d rho/dy, d rho/dx) and the contact switch
+// follow calculateRhoGrad2D (src/ceq.hpp:80-151): 4th-order gradients of rho
+// and of nrg = E / rho - rho |u|^2 / 2, G1 = |grad rho| h / rho where
+// (grad rho . grad nrg) |grad rho|^2 < 0.  FP64 arithmetic in a fixed order
+// (divisions by run constants correctly rounded, cdiv_rn), one rounding to
+// FP32 per output.
+__global__ void __launch_bounds__(256) k_features2d(const G3 g, const FeatConst fc, const double *__restrict__ var,
+                                                    const double *__restrict__ rho, const double *__restrict__ p,
+                                                    const double *__restrict__ vel, float *__restrict__ F,
+                                                    float *__restrict__ M, float *__restrict__ Dir,
+                                                    float *__restrict__ G1, Scalars *sc) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const long long N = g.N;
+  float g1max = 0.f;
+  if (i < g.ngi && j < g.ngj) {
+    const long long t = (long long)i + g.s1 * j;
+    const bool in1 = i >= 1 && j >= 1 && i < g.ngi - 1 && j < g.ngj - 1;
+    const bool in2 = i >= 2 && j >= 2 && i < g.ngi - 2 && j < g.ngj - 2;
+    float f0 = 0.f, f1 = 0.f, f2 = 0.f, d0 = 0.f, d1 = 0.f, gg = 0.f;
+    if (in1) {
+      auto nrg = [&](long long q) {
+        const double r = rho[q], u = vel[q], v = vel[q + N];
+        return var[q + 2 * N] / r - 0.5 * r * (u * u + v * v);
+      };
+      const long long st[2] = {1, g.s1};
+      double gr2 = 0.0, gp2 = 0.0, divu = 0.0, dr[2] = {0.0, 0.0}, de[2] = {0.0, 0.0};
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const long long sp = t + st[a], sm = t - st[a];
+        const double h2 = fc.h2[a], rh2 = fc.rh2[a];
+        const double r1 = cdiv_rn(rho[sp] - rho[sm], h2, rh2);
+        const double p1 = cdiv_rn(p[sp] - p[sm], h2, rh2);
+        gr2 = gr2 + r1 * r1;
+        gp2 = gp2 + p1 * p1;
+        divu = divu + cdiv_rn(vel[sp + N * a] - vel[sm + N * a], h2, rh2);
+        if (in2) {
+          const long long sp2 = t + 2 * st[a], sm2 = t - 2 * st[a];
+          const double h12 = fc.h12[a], rh12 = fc.rh12[a];
+          dr[a] = cdiv_rn(rho[sm2] - 8.0 * rho[sm] + 8.0 * rho[sp] - rho[sp2], h12, rh12);
+          de[a] = cdiv_rn(nrg(sm2) - 8.0 * nrg(sm) + 8.0 * nrg(sp) - nrg(sp2), h12, rh12);
+        }
+      }
+      const double rc = rho[t], pc = p[t];
+      double Cp = 0.0, Cv = 0.0;
+      for (int q = 0; q < fc.ns; ++q) {
+        const double Y = var[t + N * (3 + q)] / rc;
+        Cp = Cp + Y * fc.cpf[q];
+        Cv = Cv + Y * fc.cvf[q];
+      }
+      const double hmin = fc.hmin;
+      const double a_snd = sqrt((Cp / Cv) * pc / rc);
+      f0 = (float)(sqrt(gr2) * hmin / rc);
+      f1 = (float)(sqrt(gp2) * hmin / pc);
+      f2 = (float)(divu * hmin / a_snd);
+      if (in2) {
+        d0 = (float)(-dr[1]);
+        d1 = (float)dr[0];
+        const double rg2 = dr[0] * dr[0] + dr[1] * dr[1];
+        const double dnednr = (dr[0] * de[0] + dr[1] * de[1]) * rg2;
+        gg = dnednr < 0.0 ? (float)(sqrt(rg2) * hmin / rc) : 0.f;
+        const bool interior = i >= g.ng && j >= g.ng && i < g.ng + g.nci && j < g.ng + g.ncj;
+        if (interior) g1max = fmaxf(g1max, gg);
+      }
+    }
+    F[t] = f0;
+    F[t + N] = f1;
+    F[t + 2 * N] = f2;
+    M[t] = fmaxf(fabsf(f0), fmaxf(fabsf(f1), fabsf(f2)));
+    Dir[t] = d0;
+    Dir[t + N] = d1;
+    G1[t] = gg;
+  }
+  __shared__ float red[4];
+  g1max = wave_maxf(g1max);
+  if (threadIdx.x == 0) red[threadIdx.y] = g1max;
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const float m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    if (m > 0.f) atomic_max_nonneg(&sc->maxC, (double)m);
+  }
+}
+
+// 2D active cells: interior cells whose 3x3 neighbourhood has M > tau (every
+// stride-th interior cell as well when stride > 0), appended to `list` with
+// one atomic per wave; C_hat is cleared on every cell.
+__global__ void __launch_bounds__(256) k_active2d(const G3 g, const float *__restrict__ M, float tau,
+                                                  unsigned stride, unsigned *__restrict__ list,
+                                                  unsigned *__restrict__ cnt, float *__restrict__ chat) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const int ng = g.ng;
+  const long long t = (long long)i + g.s1 * j;
+  bool act = false;
+  if (i < g.ngi && j < g.ngj) {
+    if (chat) chat[t] = 0.f;
+    if (i >= ng && j >= ng && i < ng + g.nci && j < ng + g.ncj) {
+      float m = 0.f;
+      for (int dj = -1; dj <= 1; ++dj)
+        for (int di = -1; di <= 1; ++di) m = fmaxf(m, M[t + di + dj * g.s1]);
+      act = m > tau;
+      if (stride > 0) {
+        const long long lin = (long long)(i - ng) + (long long)g.nci * (j - ng);
+        act = act || (lin % stride) == 0;
+      }
+    }
+  }
+  const int lane = threadIdx.y * 64 + threadIdx.x;
+  const unsigned long long mask = __ballot(act);
+  if (mask) {
+    const int first = __ffsll((long long)mask) - 1;
+    unsigned base = 0;
+    if ((lane & 63) == first) base = atomicAdd(cnt, (unsigned)__popcll(mask));
+    base = __shfl(base, first, 64);
+    if (act) list[base + __popcll(mask & ((1ull << (lane & 63)) - 1ull))] = (unsigned)t;
+  }
+}
+
 // Ghost layer next to each face (the cells the face interpolation reads):
 // periodic image or mirror copy, as the ghost fill treats var(nv+1)
-// (src/bc.cpp:33-176).
+// (src/bc.cpp:33-176).  2D grids (ngk = 1) have x and y faces only.
 __global__ void k_chat_ghost(const G3 g, int xper, int yper, int zper, float *__restrict__ chat) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long nx = (long long)g.ncj * g.nck, ny = (long long)g.nci * g.nck, nz = (long long)g.nci * g.ncj;
+  const long long nx = (long long)g.ncj * g.nck, ny = (long long)g.nci * g.nck;
+  const long long nz = g.ngk > 1 ? (long long)g.nci * g.ncj : 0;  // 2D: no z faces
   const int ng = g.ng;
+  const int k0 = g.ngk > 1 ? ng : 0;  // first interior plane (2D: the only plane)
   long long r = t;
   int face;
   if (r < 2 * nx) face = 0;
@@ -312,12 +431,12 @@ __global__ void k_chat_ghost(const G3 g, int xper, int yper, int zper, float *__
   r >>= 1;
   int i, j, k, si, sj, sk;
   if (face == 0) {
-    j = ng + (int)(r % g.ncj); k = ng + (int)(r / g.ncj);
+    j = ng + (int)(r % g.ncj); k = k0 + (int)(r / g.ncj);
     i = side ? ng + g.nci : ng - 1;
     si = xper ? (side ? ng : ng + g.nci - 1) : (side ? ng + g.nci - 1 : ng);
     sj = j; sk = k;
   } else if (face == 1) {
-    i = ng + (int)(r % g.nci); k = ng + (int)(r / g.nci);
+    i = ng + (int)(r % g.nci); k = k0 + (int)(r / g.nci);
     j = side ? ng + g.ncj : ng - 1;
     sj = yper ? (side ? ng : ng + g.ncj - 1) : (side ? ng + g.ncj - 1 : ng);
     si = i; sk = k;
@@ -337,17 +456,19 @@ __global__ void k_clear_learned(Scalars *sc, unsigned *cnt) {
   }
 }
 
-// FP32 master weights -> BF16 inference copy (transposed, W1 padded to 384).
+// FP32 master weights -> BF16 inference copy (transposed, W1 padded to IN_PAD).
+template <int ND>
 __global__ void k_prep_infer(const float *__restrict__ W, short *__restrict__ Wb) {
+  using NT = Net<ND>;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < MLP_H * MLP_IN_PAD) {
-    const int n = t / MLP_IN_PAD, kk = t % MLP_IN_PAD;
-    Wb[INF_W1 + t] = kk < MLP_IN ? f2bf(W[MLP_OFF_W1 + kk * MLP_H + n]) : (short)0;
-  } else if (t < INF_N) {
-    const int q = t - INF_W2;
+  if (t < MLP_H * NT::IN_PAD) {
+    const int n = t / NT::IN_PAD, kk = t % NT::IN_PAD;
+    Wb[NT::INF_W1 + t] = kk < NT::IN ? f2bf(W[NT::OFF_W1 + kk * MLP_H + n]) : (short)0;
+  } else if (t < NT::INF_N) {
+    const int q = t - NT::INF_W2;
     const int layer = q / (MLP_H * MLP_H), r = q % (MLP_H * MLP_H);
     const int n = r / MLP_H, kk = r % MLP_H;
-    const int off = layer == 0 ? MLP_OFF_W2 : MLP_OFF_W3;
+    const int off = layer == 0 ? NT::OFF_W2 : NT::OFF_W3;
     Wb[t] = f2bf(W[off + kk * MLP_H + n]);
   }
 }
@@ -355,18 +476,31 @@ __global__ void k_prep_infer(const float *__restrict__ W, short *__restrict__ Wb
 // ---------------------------------------------------------------- inference
 
 // Every bf16 array starts at a multiple of 16 bytes (its 8-byte fragment
-// reads stay aligned): 64*388*2, 64*68*2 and 16*68*2 bytes are multiples of 16.
+// reads stay aligned): 64*388*2 (3D), 64*84*2 (2D), 64*68*2 and 16*68*2 bytes
+// are multiples of 16.
+template <int ND>
 struct alignas(16) InfLds {
-  short W1[MLP_H * W1S];  // W1^T [n][k]
-  short W2[MLP_H * WHS];  // W2^T
+  short W1[MLP_H * W1S<ND>];  // W1^T [n][k]
+  short W2[MLP_H * WHS];      // W2^T
   short W3[MLP_H * WHS];
   short act[4][16 * WHS];  // per wave: activations of 16 rows (bf16)
   float b1[MLP_H], b2[MLP_H], b3[MLP_H], w4[MLP_H];
-  long long off[MLP_IN_PAD];  // offset of input kk from its cell in F (channel plane + stencil)
+  long long off[Net<ND>::IN_PAD];  // offset of input kk from its cell in F (channel plane + stencil)
   float b4;
 };
-static_assert((MLP_H * W1S * 2) % 16 == 0 && (MLP_H * WHS * 2) % 16 == 0 && (16 * WHS * 2) % 16 == 0,
+static_assert((MLP_H * W1S<3> * 2) % 16 == 0 && (MLP_H * W1S<2> * 2) % 16 == 0 && (MLP_H * WHS * 2) % 16 == 0 &&
+                  (16 * WHS * 2) % 16 == 0,
               "LDS fragment alignment");
+
+// Offset of network input kk (channel-major, then the stencil with x
+// fastest) from its cell in the feature planes.
+template <int ND>
+__host__ __device__ __forceinline__ long long input_offset(int kk, long long N, long long s1, long long s2) {
+  using NT = Net<ND>;
+  const int ch = kk / NT::STEN, r = kk % NT::STEN;
+  const int dx = r % 5 - 2, dy = (r / 5) % 5 - 2, dz = ND == 3 ? r / 25 - 2 : 0;
+  return N * ch + dx + dy * s1 + dz * s2;
+}
 
 // bias + GELU of the D fragments, to bf16 rows in `act` (row stride WHS)
 __device__ __forceinline__ void store_act(const f32x4 acc[4], const float *b, short *act, int l) {
@@ -395,35 +529,31 @@ __device__ __forceinline__ void hidden_mm(const short *act, const short *WT, f32
 // y of 16 rows through the network; each wave handles tiles of 16 listed
 // cells.  Lanes 0..15 write C_hat = max(y, 0) rho / h (NaN for a non-finite
 // y, which the residual check then rejects).
+// Rho is plane 0 of P (the fused primitive planes in 3D, the rho array in 2D).
+template <int ND>
 __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__restrict__ F,
                                                    const unsigned *__restrict__ list, const unsigned *__restrict__ cnt,
                                                    const short *__restrict__ Wb, const float *__restrict__ W,
                                                    const float *__restrict__ G1, const double *__restrict__ P,
                                                    int pf32, double hmin, float *__restrict__ chat) {
-  __shared__ InfLds L;
+  using NT = Net<ND>;
+  constexpr int IP = NT::IN_PAD, S1 = W1S<ND>;
+  __shared__ InfLds<ND> L;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-  for (int q = tid; q < MLP_H * MLP_IN_PAD; q += 256) L.W1[(q / MLP_IN_PAD) * W1S + q % MLP_IN_PAD] = Wb[INF_W1 + q];
+  for (int q = tid; q < MLP_H * IP; q += 256) L.W1[(q / IP) * S1 + q % IP] = Wb[NT::INF_W1 + q];
   for (int q = tid; q < MLP_H * MLP_H; q += 256) {
-    L.W2[(q / MLP_H) * WHS + q % MLP_H] = Wb[INF_W2 + q];
-    L.W3[(q / MLP_H) * WHS + q % MLP_H] = Wb[INF_W3 + q];
+    L.W2[(q / MLP_H) * WHS + q % MLP_H] = Wb[NT::INF_W2 + q];
+    L.W3[(q / MLP_H) * WHS + q % MLP_H] = Wb[NT::INF_W3 + q];
   }
   if (tid < MLP_H) {
-    L.b1[tid] = W[MLP_OFF_B1 + tid];
-    L.b2[tid] = W[MLP_OFF_B2 + tid];
-    L.b3[tid] = W[MLP_OFF_B3 + tid];
-    L.w4[tid] = W[MLP_OFF_W4 + tid];
+    L.b1[tid] = W[NT::OFF_B1 + tid];
+    L.b2[tid] = W[NT::OFF_B2 + tid];
+    L.b3[tid] = W[NT::OFF_B3 + tid];
+    L.w4[tid] = W[NT::OFF_W4 + tid];
   }
-  if (tid == 0) L.b4 = W[MLP_OFF_B4];
+  if (tid == 0) L.b4 = W[NT::OFF_B4];
   const long long N = g.N, s1 = g.s1, s2 = g.s2;
-  for (int kk = tid; kk < MLP_IN_PAD; kk += 256) {
-    long long o = 0;
-    if (kk < MLP_IN) {
-      const int ch = kk / MLP_STEN, r = kk % MLP_STEN;
-      const int dz = r / 25 - 2, dy = (r / 5) % 5 - 2, dx = r % 5 - 2;
-      o = N * ch + dx + dy * s1 + dz * s2;
-    }
-    L.off[kk] = o;
-  }
+  for (int kk = tid; kk < IP; kk += 256) L.off[kk] = kk < NT::IN ? input_offset<ND>(kk, N, s1, s2) : 0;
   __syncthreads();
   const unsigned n = cnt[0];
   short *act = L.act[w];
@@ -437,16 +567,16 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__re
     for (int nb = 0; nb < 4; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
     // the lane's inputs are gathered in batches of KBB k-blocks (4 KBB loads
     // in flight together), then fed to the MFMAs
-    constexpr int KBB = 12;
-    static_assert((MLP_IN_PAD / 16) % KBB == 0, "k-block batches");
-    for (int kb0 = 0; kb0 < MLP_IN_PAD / 16; kb0 += KBB) {
+    constexpr int KBB = ND == 3 ? 12 : 5;
+    static_assert((IP / 16) % KBB == 0, "k-block batches");
+    for (int kb0 = 0; kb0 < IP / 16; kb0 += KBB) {
       float x[KBB][4];
 #pragma unroll
       for (int b = 0; b < KBB; ++b)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int kk = (kb0 + b) * 16 + 4 * (l >> 4) + q;
-          x[b][q] = kk < MLP_IN ? F[cell + L.off[kk]] : 0.f;
+          x[b][q] = kk < NT::IN ? F[cell + L.off[kk]] : 0.f;
         }
 #pragma unroll
       for (int b = 0; b < KBB; ++b) {
@@ -457,7 +587,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__re
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb) {
           const bf16x4 bw =
-              *reinterpret_cast<const bf16x4 *>(&L.W1[(nb * 16 + (l & 15)) * W1S + kb * 16 + 4 * (l >> 4)]);
+              *reinterpret_cast<const bf16x4 *>(&L.W1[(nb * 16 + (l & 15)) * S1 + kb * 16 + 4 * (l >> 4)]);
           acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[nb], 0, 0, 0);
         }
       }
@@ -483,9 +613,10 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__re
 }
 
 // Residual check of the prediction against the plain method, and its
-// fallback.  The relaxation of updateCeq (src/ceq3d.hpp:189-224),
+// fallback.  The relaxation of updateCeq (src/ceq3d.hpp:189-224; in 2D
+// updateCeq2D, src/ceq.hpp:153-195, the same form over two directions),
 //   d C_hat / dt = maxS / (eps |dx|) (G - C_hat) + kap maxS |dx| lap(C_hat),
-//   lap(c) = sum_d (c[-e_d] - 2 c + c[+e_d]) / d_d,   |dx|^2 = dx^2 + dy^2 + dz^2,
+//   lap(c) = sum_d (c[-e_d] - 2 c + c[+e_d]) / d_d,   |dx|^2 = sum_d d_d^2,
 // is at rest where r = G - C_hat + a lap(C_hat) = 0 with a = kap eps |dx|^2
 // (G: the contact source indicator |grad rho| = G1 rho / h).  The plain
 // method's own C_hat keeps |r| within 2 % of max G once past its start-up
@@ -499,6 +630,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const G3 g, const float *__re
 struct CheckConst {
   double a, rd[3], rden, hmin;
   float tol;
+  int nd;  // directions of the Laplacian (2 or 3)
 };
 __global__ void __launch_bounds__(256) k_ceq_check(const G3 g, const unsigned *__restrict__ list,
                                                    const unsigned *__restrict__ cnt, const float *__restrict__ chat,
@@ -508,6 +640,7 @@ __global__ void __launch_bounds__(256) k_ceq_check(const G3 g, const unsigned *_
   const unsigned n = cnt[0];
   const float gmax = (float)as_double(sc->maxC);
   const long long st[3] = {1, g.s1, g.s2};
+  const int nd = cc.nd;
   float mloc = 0.f;
   unsigned nfb = 0;
   for (unsigned r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
@@ -517,8 +650,7 @@ __global__ void __launch_bounds__(256) k_ceq_check(const G3 g, const unsigned *_
     const double c = (double)chat[cell];
     double lap = 0.0, nsum = 0.0;
     bool fin = isfinite(c);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < nd; ++d) {
       double m = (double)chat[cell - st[d]], p = (double)chat[cell + st[d]];
       fin = fin && isfinite(m) && isfinite(p);
       if (!isfinite(m)) m = 0.0;
@@ -556,22 +688,19 @@ __global__ void k_copy_listed(const unsigned *__restrict__ list, const unsigned 
 
 // ---------------------------------------------------------------- training
 
-// Dataset rows: X[r][0..383] (bf16, zeros past 375) = the feature stencil of
-// cell list[r]; T[r] = var(cvar) h / rho of that cell.
+// Dataset rows: X[r][0..IN_PAD) (bf16, zeros past IN) = the feature stencil
+// of cell list[r]; T[r] = var(cvar) h / rho of that cell (rho = P[cell]).
+template <int ND>
 __global__ void k_gather(const G3 g, const float *__restrict__ F, const unsigned *__restrict__ list, unsigned n,
                          const double *__restrict__ U, int cvar, const double *__restrict__ P, double hmin,
                          short *__restrict__ X, float *__restrict__ T) {
+  using NT = Net<ND>;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)n * MLP_IN_PAD) return;
-  const unsigned r = (unsigned)(t / MLP_IN_PAD);
-  const int kk = (int)(t % MLP_IN_PAD);
+  if (t >= (long long)n * NT::IN_PAD) return;
+  const unsigned r = (unsigned)(t / NT::IN_PAD);
+  const int kk = (int)(t % NT::IN_PAD);
   const long long cell = list[r];
-  float x = 0.f;
-  if (kk < MLP_IN) {
-    const int ch = kk / MLP_STEN, o = kk % MLP_STEN;
-    const int dz = o / 25 - 2, dy = (o / 5) % 5 - 2, dx = o % 5 - 2;
-    x = F[cell + g.N * ch + dx + dy * g.s1 + dz * g.s2];
-  }
+  const float x = kk < NT::IN ? F[cell + input_offset<ND>(kk, g.N, g.s1, g.s2)] : 0.f;
   X[t] = f2bf(x);
   if (kk == 0) T[r] = (float)(U[cell + g.N * cvar] * hmin / P[cell]);
 }
@@ -616,14 +745,14 @@ __global__ void k_bias_gelu(long long n, float *__restrict__ Z, const float *__r
 }
 
 // y = H3 . w4 + b4 (same order as inference), dY = 2 (y - T) / M,
-// loss += sum (y - T)^2 / M
-__global__ void k_out_loss(int M, const short *__restrict__ H, const float *__restrict__ W, const float *__restrict__ T,
+// loss += sum (y - T)^2 / M.  w4 points at the output weights, b4 = w4[64].
+__global__ void k_out_loss(int M, const short *__restrict__ H, const float *__restrict__ w4, const float *__restrict__ T,
                            float *__restrict__ Y, float *__restrict__ dY, float *__restrict__ loss) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   float e2 = 0.f;
   if (r < M) {
-    float y = W[MLP_OFF_B4];
-    for (int q = 0; q < MLP_H; ++q) y += bf2f(H[(long long)r * MLP_H + q]) * W[MLP_OFF_W4 + q];
+    float y = w4[MLP_H];
+    for (int q = 0; q < MLP_H; ++q) y += bf2f(H[(long long)r * MLP_H + q]) * w4[q];
     Y[r] = y;
     const float e = y - T[r];
     dY[r] = 2.0f * e / (float)M;
@@ -634,11 +763,11 @@ __global__ void k_out_loss(int M, const short *__restrict__ H, const float *__re
 }
 
 // dZ3 = dY w4 gelu'(Z3)
-__global__ void k_out_back(int M, const float *__restrict__ Z, const float *__restrict__ W, const float *__restrict__ dY,
+__global__ void k_out_back(int M, const float *__restrict__ Z, const float *__restrict__ w4, const float *__restrict__ dY,
                            float *__restrict__ dZ, short *__restrict__ dZb) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)M * MLP_H) return;
-  const float d = dY[t / MLP_H] * W[MLP_OFF_W4 + t % MLP_H] * gelu_grad(Z[t]);
+  const float d = dY[t / MLP_H] * w4[t % MLP_H] * gelu_grad(Z[t]);
   dZ[t] = d;
   dZb[t] = f2bf(d);
 }
@@ -724,13 +853,56 @@ int features(const Geo &g, const fst_config &cf, const Phys &ph, const double *P
   return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 
+int features2d(const Geo &g, const fst_config &cf, const Phys &ph, const double *var, const double *rho,
+               const double *p, const double *vel, LearnedBufs &L, hipStream_t s) {
+  const G3 q = make_g3(g);
+  k_clear_learned<<<1, 64, 0, s>>>(L.sc, L.cnt);
+  const dim3 fg((g.ngi + 63) / 64, (g.ngj + 3) / 4);
+  k_features2d<<<fg, dim3(64, 4), 0, s>>>(q, make_feat(cf, ph), var, rho, p, vel, L.F, L.M, L.dir, L.G1, L.sc);
+  return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
+}
+
+double hmin_of(const fst_config &cf) {
+  return cf.ndim == 3 ? std::min(cf.dx, std::min(cf.dy, cf.dz)) : std::min(cf.dx, cf.dy);
+}
+
+// C_hat ghost layer, residual check with fallback, ghost layer again; rho is
+// plane 0 of P.
+int check_fields(const Geo &g, const fst_config &cf, const double *P, bool pf32, LearnedBufs &L, hipStream_t s) {
+  const G3 q = make_g3(g);
+  long long nf = 2LL * ((long long)g.ncj * g.nck + (long long)g.nci * g.nck);
+  if (g.ndim == 3) nf += 2LL * g.nci * g.ncj;
+  k_chat_ghost<<<nblk(nf), 256, 0, s>>>(q, cf.xPer, cf.yPer, cf.zPer, L.chat);
+  CheckConst cc;
+  const double d[3] = {cf.dx, cf.dy, cf.dz};
+  cc.nd = g.ndim;
+  double dx2 = 0.0, rsum = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    cc.rd[a] = a < cc.nd ? 1.0 / d[a] : 0.0;
+    if (a < cc.nd) {
+      dx2 += d[a] * d[a];
+      rsum += cc.rd[a];
+    }
+  }
+  cc.a = cf.kap * cf.eps * dx2;
+  cc.rden = 1.0 / (1.0 + 2.0 * cc.a * rsum);
+  cc.hmin = hmin_of(cf);
+  cc.tol = L.tol;
+  k_ceq_check<<<1024, 256, 0, s>>>(q, L.list, L.cnt, L.chat, L.G1, P, pf32 ? 1 : 0, cc, L.chat2, L.sc, L.cnt + 1);
+  k_copy_listed<<<1024, 256, 0, s>>>(L.list, L.cnt, L.chat2, L.chat);
+  k_chat_ghost<<<nblk(nf), 256, 0, s>>>(q, cf.xPer, cf.yPer, cf.zPer, L.chat);
+  return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- solver side
 
 int learned_alloc(const Geo &g, LearnedBufs &L) {
   if (L.F) return FST_OK;
+  L.nd = g.ndim;
   const size_t N = (size_t)g.N;
+  const size_t np = (size_t)mlp_nparams(L.nd), ninf = L.nd == 2 ? Net<2>::INF_N : Net<3>::INF_N;
   bool ok = hipMalloc((void **)&L.F, N * 3 * sizeof(float)) == hipSuccess &&
             hipMalloc((void **)&L.M, N * sizeof(float)) == hipSuccess &&
             hipMalloc((void **)&L.dir, N * 3 * sizeof(float)) == hipSuccess &&
@@ -739,8 +911,8 @@ int learned_alloc(const Geo &g, LearnedBufs &L) {
             hipMalloc((void **)&L.chat2, N * sizeof(float)) == hipSuccess &&
             hipMalloc((void **)&L.list, N * sizeof(unsigned)) == hipSuccess &&
             hipMalloc((void **)&L.cnt, 4 * sizeof(unsigned)) == hipSuccess &&
-            hipMalloc((void **)&L.W, MLP_NPARAM * sizeof(float)) == hipSuccess &&
-            hipMalloc((void **)&L.Wb, INF_N * sizeof(short)) == hipSuccess &&
+            hipMalloc((void **)&L.W, np * sizeof(float)) == hipSuccess &&
+            hipMalloc((void **)&L.Wb, ninf * sizeof(short)) == hipSuccess &&
             hipMalloc((void **)&L.sc, sizeof(Scalars)) == hipSuccess;
   if (!ok) return FST_ERR_NOMEM;
   if (hipMemset(L.chat, 0, N * sizeof(float)) != hipSuccess || hipMemset(L.dir, 0, N * 3 * sizeof(float)) != hipSuccess ||
@@ -751,9 +923,10 @@ int learned_alloc(const Geo &g, LearnedBufs &L) {
 
 int learned_set_weights(LearnedBufs &L, const float *host, hipStream_t s) {
   if (!L.W) return FST_ERR_STATE;
-  if (hipMemcpyAsync(L.W, host, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess)
+  if (hipMemcpyAsync(L.W, host, (size_t)mlp_nparams(L.nd) * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess)
     return FST_ERR_HIP;
-  k_prep_infer<<<nblk(INF_N), 256, 0, s>>>(L.W, L.Wb);
+  if (L.nd == 2) k_prep_infer<2><<<nblk(Net<2>::INF_N), 256, 0, s>>>(L.W, L.Wb);
+  else k_prep_infer<3><<<nblk(Net<3>::INF_N), 256, 0, s>>>(L.W, L.Wb);
   if (hipStreamSynchronize(s) != hipSuccess) return FST_ERR_HIP;
   L.have_weights = true;
   return FST_OK;
@@ -762,27 +935,27 @@ int learned_set_weights(LearnedBufs &L, const float *host, hipStream_t s) {
 int learned_fields(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, bool pf32, LearnedBufs &L,
                    hipStream_t s) {
   if (!L.have_weights) return FST_ERR_STATE;
+  if (L.nd != 3 || g.ndim != 3) return FST_ERR_ARG;
   int rc = features(g, cf, ph, P, pf32, L, s);
   if (rc) return rc;
   const G3 q = make_g3(g);
   const dim3 ag((g.ngi + 63) / 64, (g.ngj + 3) / 4, (g.ngk + ACT_KCH - 1) / ACT_KCH);
   k_active3d<<<ag, dim3(64, 4), 0, s>>>(q, L.M, L.tau, 0u, L.list, L.cnt, L.chat);
-  const double hmin = std::min(cf.dx, std::min(cf.dy, cf.dz));
-  k_mlp_infer<<<1024, 256, 0, s>>>(q, L.F, L.list, L.cnt, L.Wb, L.W, L.G1, P, pf32 ? 1 : 0, hmin, L.chat);
-  const long long nf = 2LL * ((long long)g.ncj * g.nck + (long long)g.nci * g.nck + (long long)g.nci * g.ncj);
-  k_chat_ghost<<<nblk(nf), 256, 0, s>>>(q, cf.xPer, cf.yPer, cf.zPer, L.chat);
-  CheckConst cc;
-  cc.a = cf.kap * cf.eps * (cf.dx * cf.dx + cf.dy * cf.dy + cf.dz * cf.dz);
-  cc.rd[0] = 1.0 / cf.dx;
-  cc.rd[1] = 1.0 / cf.dy;
-  cc.rd[2] = 1.0 / cf.dz;
-  cc.rden = 1.0 / (1.0 + 2.0 * cc.a * (cc.rd[0] + cc.rd[1] + cc.rd[2]));
-  cc.hmin = hmin;
-  cc.tol = L.tol;
-  k_ceq_check<<<1024, 256, 0, s>>>(q, L.list, L.cnt, L.chat, L.G1, P, pf32 ? 1 : 0, cc, L.chat2, L.sc, L.cnt + 1);
-  k_copy_listed<<<1024, 256, 0, s>>>(L.list, L.cnt, L.chat2, L.chat);
-  k_chat_ghost<<<nblk(nf), 256, 0, s>>>(q, cf.xPer, cf.yPer, cf.zPer, L.chat);
-  return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
+  k_mlp_infer<3><<<1024, 256, 0, s>>>(q, L.F, L.list, L.cnt, L.Wb, L.W, L.G1, P, pf32 ? 1 : 0, hmin_of(cf), L.chat);
+  return check_fields(g, cf, P, pf32, L, s);
+}
+
+int learned_fields2d(const Geo &g, const fst_config &cf, const Phys &ph, const double *var, const double *rho,
+                     const double *p, const double *vel, LearnedBufs &L, hipStream_t s) {
+  if (!L.have_weights) return FST_ERR_STATE;
+  if (L.nd != 2 || g.ndim != 2) return FST_ERR_ARG;
+  int rc = features2d(g, cf, ph, var, rho, p, vel, L, s);
+  if (rc) return rc;
+  const G3 q = make_g3(g);
+  const dim3 ag((g.ngi + 63) / 64, (g.ngj + 3) / 4);
+  k_active2d<<<ag, dim3(64, 4), 0, s>>>(q, L.M, L.tau, 0u, L.list, L.cnt, L.chat);
+  k_mlp_infer<2><<<1024, 256, 0, s>>>(q, L.F, L.list, L.cnt, L.Wb, L.W, L.G1, rho, 0, hmin_of(cf), L.chat);
+  return check_fields(g, cf, rho, false, L, s);
 }
 
 void learned_free(LearnedBufs &L) {
@@ -832,66 +1005,81 @@ int ensure_work(Trainer &t, int M) {
   return FST_OK;
 }
 
-// Forward + backward over M rows X (bf16, stride MLP_IN_PAD), targets T.
+// Forward + backward over M rows X (bf16, stride IN_PAD), targets T.
 // Gradients (parameter-vector layout) accumulate into G when non-null; the
 // mean squared error accumulates into *loss on the device when non-null.
-int fwd_bwd(Trainer &t, const short *X, const float *T, int M, float *G, float *loss) {
+template <int ND>
+int fwd_bwd_nd(Trainer &t, const short *X, const float *T, int M, float *G, float *loss) {
+  using NT = Net<ND>;
   int rc = ensure_work(t, M);
   if (rc) return rc;
   MlpWork &w = t.work;
   hipStream_t s = t.s;
   const float *W = t.W;
-  k_to_bf16<<<nblk(MLP_NPARAM), 256, 0, s>>>(MLP_NPARAM, W, t.Wb);
-  const short *W1 = t.Wb + MLP_OFF_W1, *W2 = t.Wb + MLP_OFF_W2, *W3 = t.Wb + MLP_OFF_W3;
+  k_to_bf16<<<nblk(NT::NPARAM), 256, 0, s>>>(NT::NPARAM, W, t.Wb);
+  const short *W1 = t.Wb + NT::OFF_W1, *W2 = t.Wb + NT::OFF_W2, *W3 = t.Wb + NT::OFF_W3;
   const dim3 gm((M + 15) / 16, MLP_H / 16);
   const long long MH = (long long)M * MLP_H;
-  k_gemm_bf16<<<gm, 64, 0, s>>>(M, MLP_H, MLP_IN, X, MLP_IN_PAD, 1, W1, MLP_H, 1, w.Z1, MLP_H, 0);
-  k_bias_gelu<<<nblk(MH), 256, 0, s>>>(MH, w.Z1, W + MLP_OFF_B1, w.H1);
+  k_gemm_bf16<<<gm, 64, 0, s>>>(M, MLP_H, NT::IN, X, NT::IN_PAD, 1, W1, MLP_H, 1, w.Z1, MLP_H, 0);
+  k_bias_gelu<<<nblk(MH), 256, 0, s>>>(MH, w.Z1, W + NT::OFF_B1, w.H1);
   k_gemm_bf16<<<gm, 64, 0, s>>>(M, MLP_H, MLP_H, w.H1, MLP_H, 1, W2, MLP_H, 1, w.Z2, MLP_H, 0);
-  k_bias_gelu<<<nblk(MH), 256, 0, s>>>(MH, w.Z2, W + MLP_OFF_B2, w.H2);
+  k_bias_gelu<<<nblk(MH), 256, 0, s>>>(MH, w.Z2, W + NT::OFF_B2, w.H2);
   k_gemm_bf16<<<gm, 64, 0, s>>>(M, MLP_H, MLP_H, w.H2, MLP_H, 1, W3, MLP_H, 1, w.Z3, MLP_H, 0);
-  k_bias_gelu<<<nblk(MH), 256, 0, s>>>(MH, w.Z3, W + MLP_OFF_B3, w.H3);
-  k_out_loss<<<nblk(M), 256, 0, s>>>(M, w.H3, W, T, w.Y, w.dY, loss);
+  k_bias_gelu<<<nblk(MH), 256, 0, s>>>(MH, w.Z3, W + NT::OFF_B3, w.H3);
+  k_out_loss<<<nblk(M), 256, 0, s>>>(M, w.H3, W + NT::OFF_W4, T, w.Y, w.dY, loss);
   if (G) {
     // output layer: dw4 = H3^T dY, db4 = sum dY
-    k_colsum<true><<<MLP_H, 256, 0, s>>>(M, MLP_H, w.H3, w.dY, G + MLP_OFF_W4);
-    k_colsum<false><<<1, 256, 0, s>>>(M, 1, w.dY, nullptr, G + MLP_OFF_B4);
-    k_out_back<<<nblk(MH), 256, 0, s>>>(M, w.Z3, W, w.dY, w.dZ, w.dZb);  // dZ3
-    k_colsum<false><<<MLP_H, 256, 0, s>>>(M, MLP_H, w.dZ, nullptr, G + MLP_OFF_B3);
+    k_colsum<true><<<MLP_H, 256, 0, s>>>(M, MLP_H, w.H3, w.dY, G + NT::OFF_W4);
+    k_colsum<false><<<1, 256, 0, s>>>(M, 1, w.dY, nullptr, G + NT::OFF_B4);
+    k_out_back<<<nblk(MH), 256, 0, s>>>(M, w.Z3, W + NT::OFF_W4, w.dY, w.dZ, w.dZb);  // dZ3
+    k_colsum<false><<<MLP_H, 256, 0, s>>>(M, MLP_H, w.dZ, nullptr, G + NT::OFF_B3);
     // dW3 = H2^T dZ3 ; dH2 = dZ3 W3^T
-    k_gemm_bf16<<<dim3(4, 4), 64, 0, s>>>(MLP_H, MLP_H, M, w.H2, 1, MLP_H, w.dZb, MLP_H, 1, G + MLP_OFF_W3, MLP_H, 1);
+    k_gemm_bf16<<<dim3(4, 4), 64, 0, s>>>(MLP_H, MLP_H, M, w.H2, 1, MLP_H, w.dZb, MLP_H, 1, G + NT::OFF_W3, MLP_H, 1);
     k_gemm_bf16<<<gm, 64, 0, s>>>(M, MLP_H, MLP_H, w.dZb, MLP_H, 1, W3, 1, MLP_H, w.dH, MLP_H, 0);
     k_gelu_back<<<nblk(MH), 256, 0, s>>>(MH, w.dH, w.Z2, w.dZ, w.dZb);  // dZ2
-    k_colsum<false><<<MLP_H, 256, 0, s>>>(M, MLP_H, w.dZ, nullptr, G + MLP_OFF_B2);
-    k_gemm_bf16<<<dim3(4, 4), 64, 0, s>>>(MLP_H, MLP_H, M, w.H1, 1, MLP_H, w.dZb, MLP_H, 1, G + MLP_OFF_W2, MLP_H, 1);
+    k_colsum<false><<<MLP_H, 256, 0, s>>>(M, MLP_H, w.dZ, nullptr, G + NT::OFF_B2);
+    k_gemm_bf16<<<dim3(4, 4), 64, 0, s>>>(MLP_H, MLP_H, M, w.H1, 1, MLP_H, w.dZb, MLP_H, 1, G + NT::OFF_W2, MLP_H, 1);
     k_gemm_bf16<<<gm, 64, 0, s>>>(M, MLP_H, MLP_H, w.dZb, MLP_H, 1, W2, 1, MLP_H, w.dH, MLP_H, 0);
     k_gelu_back<<<nblk(MH), 256, 0, s>>>(MH, w.dH, w.Z1, w.dZ, w.dZb);  // dZ1
-    k_colsum<false><<<MLP_H, 256, 0, s>>>(M, MLP_H, w.dZ, nullptr, G + MLP_OFF_B1);
-    // dW1 = X^T dZ1 : (375 x M)(M x 64)
-    k_gemm_bf16<<<dim3((MLP_IN + 15) / 16, 4), 64, 0, s>>>(MLP_IN, MLP_H, M, X, 1, MLP_IN_PAD, w.dZb, MLP_H, 1,
-                                                          G + MLP_OFF_W1, MLP_H, 1);
+    k_colsum<false><<<MLP_H, 256, 0, s>>>(M, MLP_H, w.dZ, nullptr, G + NT::OFF_B1);
+    // dW1 = X^T dZ1 : (IN x M)(M x 64)
+    k_gemm_bf16<<<dim3((NT::IN + 15) / 16, 4), 64, 0, s>>>(NT::IN, MLP_H, M, X, 1, NT::IN_PAD, w.dZb, MLP_H, 1,
+                                                          G + NT::OFF_W1, MLP_H, 1);
   }
   return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 
+int fwd_bwd(Trainer &t, const short *X, const float *T, int M, float *G, float *loss) {
+  return t.nd == 2 ? fwd_bwd_nd<2>(t, X, T, M, G, loss) : fwd_bwd_nd<3>(t, X, T, M, G, loss);
+}
+
+template <int ND>
+std::vector<float> initial_weights(unsigned long long seed) {
+  using NT = Net<ND>;
+  // uniform(-a, a), a = sqrt(6 / fan_in) on the hidden layers, sqrt(3 /
+  // fan_in) on the output layer; zero biases
+  std::vector<float> w(NT::NPARAM, 0.f);
+  unsigned long long x = seed;
+  const float a1 = std::sqrt(6.0f / NT::IN), a2 = std::sqrt(6.0f / MLP_H), a4 = std::sqrt(3.0f / MLP_H);
+  for (int q = 0; q < NT::IN * MLP_H; ++q) w[NT::OFF_W1 + q] = a1 * uni(x);
+  for (int q = 0; q < MLP_H * MLP_H; ++q) w[NT::OFF_W2 + q] = a2 * uni(x);
+  for (int q = 0; q < MLP_H * MLP_H; ++q) w[NT::OFF_W3 + q] = a2 * uni(x);
+  for (int q = 0; q < MLP_H; ++q) w[NT::OFF_W4 + q] = a4 * uni(x);
+  return w;
+}
+
 }  // namespace
 
-int trainer_create(Trainer &t, unsigned long long seed) {
+int trainer_create(Trainer &t, int nd, unsigned long long seed) {
+  if (nd != 2 && nd != 3) return FST_ERR_ARG;
+  t.nd = nd;
   if (hipStreamCreateWithFlags(&t.s, hipStreamNonBlocking) != hipSuccess) return FST_ERR_HIP;
-  const size_t pb = MLP_NPARAM * sizeof(float);
+  const size_t np = (size_t)mlp_nparams(nd), pb = np * sizeof(float);
   bool ok = hipMalloc((void **)&t.W, pb) == hipSuccess && hipMalloc((void **)&t.G, pb) == hipSuccess &&
             hipMalloc((void **)&t.m1, pb) == hipSuccess && hipMalloc((void **)&t.m2, pb) == hipSuccess &&
-            hipMalloc((void **)&t.Wb, MLP_NPARAM * sizeof(short)) == hipSuccess;
+            hipMalloc((void **)&t.Wb, np * sizeof(short)) == hipSuccess;
   if (!ok) return FST_ERR_NOMEM;
-  // initial weights: uniform(-a, a), a = sqrt(6 / fan_in) on the hidden
-  // layers, sqrt(3 / fan_in) on the output layer; zero biases
-  std::vector<float> w(MLP_NPARAM, 0.f);
-  unsigned long long x = seed;
-  const float a1 = std::sqrt(6.0f / MLP_IN), a2 = std::sqrt(6.0f / MLP_H), a4 = std::sqrt(3.0f / MLP_H);
-  for (int q = 0; q < MLP_IN * MLP_H; ++q) w[MLP_OFF_W1 + q] = a1 * uni(x);
-  for (int q = 0; q < MLP_H * MLP_H; ++q) w[MLP_OFF_W2 + q] = a2 * uni(x);
-  for (int q = 0; q < MLP_H * MLP_H; ++q) w[MLP_OFF_W3 + q] = a2 * uni(x);
-  for (int q = 0; q < MLP_H; ++q) w[MLP_OFF_W4 + q] = a4 * uni(x);
+  const std::vector<float> w = nd == 2 ? initial_weights<2>(seed) : initial_weights<3>(seed);
   return trainer_set_weights(t, w.data());
 }
 
@@ -909,36 +1097,40 @@ void trainer_free(Trainer &t) {
 }
 
 int trainer_set_weights(Trainer &t, const float *host) {
-  const size_t pb = MLP_NPARAM * sizeof(float);
+  const int np = mlp_nparams(t.nd);
+  const size_t pb = (size_t)np * sizeof(float);
   if (hipMemcpyAsync(t.W, host, pb, hipMemcpyHostToDevice, t.s) != hipSuccess) return FST_ERR_HIP;
-  k_fill<<<nblk(MLP_NPARAM), 256, 0, t.s>>>(MLP_NPARAM, t.m1, 0.f);
-  k_fill<<<nblk(MLP_NPARAM), 256, 0, t.s>>>(MLP_NPARAM, t.m2, 0.f);
+  k_fill<<<nblk(np), 256, 0, t.s>>>(np, t.m1, 0.f);
+  k_fill<<<nblk(np), 256, 0, t.s>>>(np, t.m2, 0.f);
   t.adam_t = 0;
   return hipStreamSynchronize(t.s) == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 
 int trainer_get_weights(Trainer &t, float *host) {
-  if (hipMemcpyAsync(host, t.W, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost, t.s) != hipSuccess)
+  if (hipMemcpyAsync(host, t.W, (size_t)mlp_nparams(t.nd) * sizeof(float), hipMemcpyDeviceToHost, t.s) != hipSuccess)
     return FST_ERR_HIP;
   return hipStreamSynchronize(t.s) == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 
-int trainer_harvest(Trainer &t, const Geo &g, const fst_config &cf, const Phys &ph, const double *U,
-                    const double *P, unsigned stride, long long max_rows, long long *added) {
-  *added = 0;
-  if (t.feat_cells != g.N) {
-    learned_free(t.feat);
-    t.feat_cells = 0;
-    int rc = learned_alloc(g, t.feat);
-    if (rc) return rc;
-    t.feat_cells = g.N;
-  }
-  hipStream_t s = t.s;
-  int rc = features(g, cf, ph, P, false, t.feat, s);  // plain FP64 contexts: FP64 primitive planes
+namespace {
+
+// Feature buffers of the trainer sized for g.
+int feat_bufs(Trainer &t, const Geo &g) {
+  if (t.feat_cells == g.N && t.feat.nd == g.ndim) return FST_OK;
+  learned_free(t.feat);
+  t.feat_cells = 0;
+  int rc = learned_alloc(g, t.feat);
   if (rc) return rc;
-  const G3 q = make_g3(g);
-  const dim3 ag((g.ngi + 63) / 64, (g.ngj + 3) / 4, (g.ngk + ACT_KCH - 1) / ACT_KCH);
-  k_active3d<<<ag, dim3(64, 4), 0, s>>>(q, t.feat.M, t.feat.tau, stride, t.feat.list, t.feat.cnt, nullptr);
+  t.feat_cells = g.N;
+  return FST_OK;
+}
+
+// Appends the rows of the listed cells (t.feat.list, count t.feat.cnt[0],
+// features t.feat.F) with targets var(nv+1) h / rho (rho = plane 0 of P).
+int append_rows(Trainer &t, const Geo &g, const fst_config &cf, const double *U, const double *P,
+                long long max_rows, long long *added) {
+  hipStream_t s = t.s;
+  const int ip = mlp_in_pad(t.nd);
   unsigned n = 0;
   if (hipMemcpyAsync(&n, t.feat.cnt, sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
@@ -949,13 +1141,13 @@ int trainer_harvest(Trainer &t, const Geo &g, const fst_config &cf, const Phys &
     const long long cap = std::max(t.nrows + nn, 2 * t.cap + 65536);
     short *X = nullptr;
     float *T = nullptr;
-    if (hipMalloc((void **)&X, (size_t)cap * MLP_IN_PAD * sizeof(short)) != hipSuccess ||
+    if (hipMalloc((void **)&X, (size_t)cap * ip * sizeof(short)) != hipSuccess ||
         hipMalloc((void **)&T, (size_t)cap * sizeof(float)) != hipSuccess) {
       if (X) (void)hipFree(X);
       return FST_ERR_NOMEM;
     }
     if (t.nrows) {
-      (void)hipMemcpyAsync(X, t.X, (size_t)t.nrows * MLP_IN_PAD * sizeof(short), hipMemcpyDeviceToDevice, s);
+      (void)hipMemcpyAsync(X, t.X, (size_t)t.nrows * ip * sizeof(short), hipMemcpyDeviceToDevice, s);
       (void)hipMemcpyAsync(T, t.T, (size_t)t.nrows * sizeof(float), hipMemcpyDeviceToDevice, s);
     }
     if (hipStreamSynchronize(s) != hipSuccess) return FST_ERR_HIP;
@@ -965,13 +1157,51 @@ int trainer_harvest(Trainer &t, const Geo &g, const fst_config &cf, const Phys &
     t.T = T;
     t.cap = cap;
   }
-  const double hmin = std::min(cf.dx, std::min(cf.dy, cf.dz));
-  k_gather<<<nblk(nn * MLP_IN_PAD), 256, 0, s>>>(q, t.feat.F, t.feat.list, (unsigned)nn, U, g.nv + 1, P, hmin,
-                                                 t.X + (size_t)t.nrows * MLP_IN_PAD, t.T + t.nrows);
+  const G3 q = make_g3(g);
+  const double hmin = hmin_of(cf);
+  if (t.nd == 2)
+    k_gather<2><<<nblk(nn * ip), 256, 0, s>>>(q, t.feat.F, t.feat.list, (unsigned)nn, U, g.nv + 1, P, hmin,
+                                              t.X + (size_t)t.nrows * ip, t.T + t.nrows);
+  else
+    k_gather<3><<<nblk(nn * ip), 256, 0, s>>>(q, t.feat.F, t.feat.list, (unsigned)nn, U, g.nv + 1, P, hmin,
+                                              t.X + (size_t)t.nrows * ip, t.T + t.nrows);
   if (hipStreamSynchronize(s) != hipSuccess) return FST_ERR_HIP;
   t.nrows += nn;
   *added = nn;
   return FST_OK;
+}
+
+}  // namespace
+
+int trainer_harvest(Trainer &t, const Geo &g, const fst_config &cf, const Phys &ph, const double *U,
+                    const double *P, unsigned stride, long long max_rows, long long *added) {
+  *added = 0;
+  if (t.nd != 3 || g.ndim != 3) return FST_ERR_ARG;
+  int rc = feat_bufs(t, g);
+  if (rc) return rc;
+  hipStream_t s = t.s;
+  rc = features(g, cf, ph, P, false, t.feat, s);  // plain FP64 contexts: FP64 primitive planes
+  if (rc) return rc;
+  const G3 q = make_g3(g);
+  const dim3 ag((g.ngi + 63) / 64, (g.ngj + 3) / 4, (g.ngk + ACT_KCH - 1) / ACT_KCH);
+  k_active3d<<<ag, dim3(64, 4), 0, s>>>(q, t.feat.M, t.feat.tau, stride, t.feat.list, t.feat.cnt, nullptr);
+  return append_rows(t, g, cf, U, P, max_rows, added);
+}
+
+int trainer_harvest2d(Trainer &t, const Geo &g, const fst_config &cf, const Phys &ph, const double *U,
+                      const double *rho, const double *p, const double *vel, unsigned stride, long long max_rows,
+                      long long *added) {
+  *added = 0;
+  if (t.nd != 2 || g.ndim != 2) return FST_ERR_ARG;
+  int rc = feat_bufs(t, g);
+  if (rc) return rc;
+  hipStream_t s = t.s;
+  rc = features2d(g, cf, ph, U, rho, p, vel, t.feat, s);
+  if (rc) return rc;
+  const G3 q = make_g3(g);
+  const dim3 ag((g.ngi + 63) / 64, (g.ngj + 3) / 4);
+  k_active2d<<<ag, dim3(64, 4), 0, s>>>(q, t.feat.M, t.feat.tau, stride, t.feat.list, t.feat.cnt, nullptr);
+  return append_rows(t, g, cf, U, rho, max_rows, added);
 }
 
 int trainer_train(Trainer &t, int epochs, int batch, float lr, unsigned long long seed, float *loss_hist) {
@@ -989,17 +1219,18 @@ int trainer_train(Trainer &t, int epochs, int batch, float lr, unsigned long lon
   std::vector<long long> order(nb);
   unsigned long long x = seed;
   const float b1 = 0.9f, b2 = 0.999f;
+  const int np = mlp_nparams(t.nd), ip = mlp_in_pad(t.nd);
   for (int ep = 0; ep < epochs; ++ep) {
     for (long long q = 0; q < nb; ++q) order[q] = q;
     for (long long q = nb - 1; q > 0; --q) std::swap(order[q], order[smix(x) % (unsigned long long)(q + 1)]);
     for (long long q = 0; q < nb; ++q) {
       const long long r0 = order[q] * batch;
-      k_fill<<<nblk(MLP_NPARAM), 256, 0, s>>>(MLP_NPARAM, t.G, 0.f);
-      int rc = fwd_bwd(t, t.X + r0 * MLP_IN_PAD, t.T + r0, batch, t.G, t.loss + ep);
+      k_fill<<<nblk(np), 256, 0, s>>>(np, t.G, 0.f);
+      int rc = fwd_bwd(t, t.X + r0 * ip, t.T + r0, batch, t.G, t.loss + ep);
       if (rc) return rc;
       t.adam_t += 1;
       const float c1 = 1.0f - std::pow(b1, (float)t.adam_t), c2 = 1.0f - std::pow(b2, (float)t.adam_t);
-      k_adam<<<nblk(MLP_NPARAM), 256, 0, s>>>(MLP_NPARAM, t.W, t.G, t.m1, t.m2, lr, b1, b2, 1e-8f, c1, c2);
+      k_adam<<<nblk(np), 256, 0, s>>>(np, t.W, t.G, t.m1, t.m2, lr, b1, b2, 1e-8f, c1, c2);
     }
   }
   if (loss_hist &&
@@ -1013,25 +1244,26 @@ int trainer_train(Trainer &t, int epochs, int batch, float lr, unsigned long lon
 
 int trainer_forward_backward(Trainer &t, const float *Xh, const float *Th, int M, float *Yh, float *Gh, float *lossh) {
   if (M < 1) return FST_ERR_ARG;
-  std::vector<short> xb((size_t)M * MLP_IN_PAD, 0);
+  const int np = mlp_nparams(t.nd), in = mlp_in(t.nd), ip = mlp_in_pad(t.nd);
+  std::vector<short> xb((size_t)M * ip, 0);
   for (long long r = 0; r < M; ++r)
-    for (int k = 0; k < MLP_IN; ++k) xb[r * MLP_IN_PAD + k] = f2bf(Xh[r * MLP_IN + k]);
+    for (int k = 0; k < in; ++k) xb[r * ip + k] = f2bf(Xh[r * in + k]);
   short *X = nullptr;
   float *T = nullptr, *G = nullptr, *L = nullptr;
   hipStream_t s = t.s;
   int rc = FST_OK;
   if (hipMalloc((void **)&X, xb.size() * sizeof(short)) != hipSuccess || hipMalloc((void **)&T, (size_t)M * 4) != hipSuccess ||
-      hipMalloc((void **)&G, MLP_NPARAM * 4) != hipSuccess || hipMalloc((void **)&L, 4) != hipSuccess)
+      hipMalloc((void **)&G, (size_t)np * 4) != hipSuccess || hipMalloc((void **)&L, 4) != hipSuccess)
     rc = FST_ERR_NOMEM;
   if (!rc) {
     (void)hipMemcpyAsync(X, xb.data(), xb.size() * sizeof(short), hipMemcpyHostToDevice, s);
     (void)hipMemcpyAsync(T, Th, (size_t)M * 4, hipMemcpyHostToDevice, s);
-    k_fill<<<nblk(MLP_NPARAM), 256, 0, s>>>(MLP_NPARAM, G, 0.f);
+    k_fill<<<nblk(np), 256, 0, s>>>(np, G, 0.f);
     k_fill<<<1, 64, 0, s>>>(1, L, 0.f);
     rc = fwd_bwd(t, X, T, M, G, L);
     if (!rc) {
       if (Yh) (void)hipMemcpyAsync(Yh, t.work.Y, (size_t)M * 4, hipMemcpyDeviceToHost, s);
-      if (Gh) (void)hipMemcpyAsync(Gh, G, MLP_NPARAM * 4, hipMemcpyDeviceToHost, s);
+      if (Gh) (void)hipMemcpyAsync(Gh, G, (size_t)np * 4, hipMemcpyDeviceToHost, s);
       if (lossh) (void)hipMemcpyAsync(lossh, L, 4, hipMemcpyDeviceToHost, s);
       rc = hipStreamSynchronize(s) == hipSuccess ? FST_OK : FST_ERR_HIP;
     }
@@ -1047,17 +1279,18 @@ int trainer_forward_backward(Trainer &t, const float *Xh, const float *Th, int M
 int trainer_read_rows(Trainer &t, long long r0, long long n, float *Xh, float *Th) {
   if (r0 < 0 || n < 0 || r0 + n > t.nrows) return FST_ERR_ARG;
   if (n == 0) return FST_OK;
-  std::vector<short> xb((size_t)n * MLP_IN_PAD);
-  if (hipMemcpyAsync(xb.data(), t.X + (size_t)r0 * MLP_IN_PAD, xb.size() * sizeof(short), hipMemcpyDeviceToHost, t.s) !=
+  const int in = mlp_in(t.nd), ip = mlp_in_pad(t.nd);
+  std::vector<short> xb((size_t)n * ip);
+  if (hipMemcpyAsync(xb.data(), t.X + (size_t)r0 * ip, xb.size() * sizeof(short), hipMemcpyDeviceToHost, t.s) !=
           hipSuccess ||
       (Th && hipMemcpyAsync(Th, t.T + r0, (size_t)n * 4, hipMemcpyDeviceToHost, t.s) != hipSuccess) ||
       hipStreamSynchronize(t.s) != hipSuccess)
     return FST_ERR_HIP;
   if (Xh)
     for (long long r = 0; r < n; ++r)
-      for (int k = 0; k < MLP_IN; ++k) {
-        const unsigned u = ((unsigned)(unsigned short)xb[r * MLP_IN_PAD + k]) << 16;
-        std::memcpy(&Xh[r * MLP_IN + k], &u, 4);
+      for (int k = 0; k < in; ++k) {
+        const unsigned u = ((unsigned)(unsigned short)xb[r * ip + k]) << 16;
+        std::memcpy(&Xh[r * in + k], &u, 4);
       }
   return FST_OK;
 }

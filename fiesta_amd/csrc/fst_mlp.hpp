@@ -7,23 +7,43 @@
 
 namespace fst {
 
-// Network: 5x5x5 stencil of 3 features (375 inputs, padded to 384 for the
-// 16-deep MFMA k-steps) -> 64 -> 64 -> 64 -> 1, GELU on the hidden layers.
+// Network: a 5^ND stencil of 3 features (375 inputs in 3D, 75 in 2D, padded
+// to a multiple of 16 for the 16-deep MFMA k-steps) -> 64 -> 64 -> 64 -> 1,
+// GELU on the hidden layers.  FP32 parameter vector: W1[IN][64], b1[64],
+// W2[64][64], b2, W3[64][64], b3, w4[64], b4.
+template <int ND>
+struct Net {
+  static constexpr int NF = 3;
+  static constexpr int STEN = ND == 3 ? 125 : 25;
+  static constexpr int IN = NF * STEN;
+  static constexpr int IN_PAD = (IN + 15) / 16 * 16;
+  static constexpr int H = 64;
+  static constexpr int OFF_W1 = 0;
+  static constexpr int OFF_B1 = OFF_W1 + IN * H;
+  static constexpr int OFF_W2 = OFF_B1 + H;
+  static constexpr int OFF_B2 = OFF_W2 + H * H;
+  static constexpr int OFF_W3 = OFF_B2 + H;
+  static constexpr int OFF_B3 = OFF_W3 + H * H;
+  static constexpr int OFF_W4 = OFF_B3 + H;
+  static constexpr int OFF_B4 = OFF_W4 + H;
+  static constexpr int NPARAM = OFF_B4 + 1;
+  // BF16 inference copy: W1^T [64][IN_PAD] (zero columns k >= IN), W2^T, W3^T
+  static constexpr int INF_W1 = 0;
+  static constexpr int INF_W2 = H * IN_PAD;
+  static constexpr int INF_W3 = INF_W2 + H * H;
+  static constexpr int INF_N = INF_W3 + H * H;
+};
 constexpr int MLP_NF = 3;
-constexpr int MLP_STEN = 125;
-constexpr int MLP_IN = MLP_NF * MLP_STEN;
-constexpr int MLP_IN_PAD = 384;
 constexpr int MLP_H = 64;
-// FP32 parameter vector: W1[375][64], b1[64], W2[64][64], b2, W3[64][64], b3, w4[64], b4
-constexpr int MLP_OFF_W1 = 0;
-constexpr int MLP_OFF_B1 = MLP_OFF_W1 + MLP_IN * MLP_H;
-constexpr int MLP_OFF_W2 = MLP_OFF_B1 + MLP_H;
-constexpr int MLP_OFF_B2 = MLP_OFF_W2 + MLP_H * MLP_H;
-constexpr int MLP_OFF_W3 = MLP_OFF_B2 + MLP_H;
-constexpr int MLP_OFF_B3 = MLP_OFF_W3 + MLP_H * MLP_H;
-constexpr int MLP_OFF_W4 = MLP_OFF_B3 + MLP_H;
-constexpr int MLP_OFF_B4 = MLP_OFF_W4 + MLP_H;
-constexpr int MLP_NPARAM = MLP_OFF_B4 + 1;
+constexpr int MLP_STEN = Net<3>::STEN;
+constexpr int MLP_IN = Net<3>::IN;
+constexpr int MLP_IN_PAD = Net<3>::IN_PAD;
+constexpr int MLP_NPARAM = Net<3>::NPARAM;
+static_assert(Net<3>::IN_PAD == 384 && Net<2>::IN_PAD == 80, "padded input widths");
+// parameter count / padded input width / input width of the ND network
+inline int mlp_nparams(int nd) { return nd == 2 ? Net<2>::NPARAM : Net<3>::NPARAM; }
+inline int mlp_in(int nd) { return nd == 2 ? Net<2>::IN : Net<3>::IN; }
+inline int mlp_in_pad(int nd) { return nd == 2 ? Net<2>::IN_PAD : Net<3>::IN_PAD; }
 
 // Constants of the feature pass.
 struct FeatConst {
@@ -36,9 +56,11 @@ struct FeatConst {
 
 // Learned-C fields of one solver context.
 struct LearnedBufs {
+  int nd = 3;             // network / grid dimension
   float *F = nullptr;     // 3 feature planes
   float *M = nullptr;     // per-cell max |feature|
-  float *dir = nullptr;   // 3 planes: 4th-order grad rho (the direction n)
+  float *dir = nullptr;   // 3D: 4th-order grad rho (the direction n); 2D: the
+                          // tangent (-d rho/dy, d rho/dx), 4th order (2 of 3 planes)
   float *G1 = nullptr;    // contact-switch source, dimensionless
   float *chat = nullptr;  // C_hat, dimensional (units of |grad rho|)
   float *chat2 = nullptr; // the residual check's output for the listed cells
@@ -58,6 +80,10 @@ int learned_alloc(const Geo &g, LearnedBufs &L);
 // fallback, ghost layer of C_hat; maxCh into L.sc.  Stream-ordered.
 int learned_fields(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, bool pf32, LearnedBufs &L,
                    hipStream_t s);
+// 2D (cart2d, reference-sequence state): features of var, rho, p and vel
+// (ghosts filled, primitives current), then C_hat as above.
+int learned_fields2d(const Geo &g, const fst_config &cf, const Phys &ph, const double *var, const double *rho,
+                     const double *p, const double *vel, LearnedBufs &L, hipStream_t s);
 int learned_set_weights(LearnedBufs &L, const float *host, hipStream_t s);
 void learned_free(LearnedBufs &L);
 
@@ -71,12 +97,13 @@ struct MlpWork {
 };
 
 struct Trainer {
+  int nd = 3;  // network dimension (5^nd stencils)
   hipStream_t s = nullptr;
   float *W = nullptr, *G = nullptr, *m1 = nullptr, *m2 = nullptr;
   short *Wb = nullptr;
   float *loss = nullptr;  // device loss accumulators
   int loss_cap = 0;
-  short *X = nullptr;  // rows x MLP_IN_PAD (bf16)
+  short *X = nullptr;  // rows x IN_PAD (bf16)
   float *T = nullptr;  // targets
   long long nrows = 0, cap = 0;
   long long adam_t = 0;
@@ -90,15 +117,19 @@ struct Trainer {
   std::string err;
 };
 
-int trainer_create(Trainer &t, unsigned long long seed);
+int trainer_create(Trainer &t, int nd, unsigned long long seed);
 void trainer_free(Trainer &t);
 int trainer_set_weights(Trainer &t, const float *host);
 int trainer_get_weights(Trainer &t, float *host);
 // Rows from the state U of a plain (C-equation) solver with primitives P:
 // every active cell plus every stride-th interior cell; targets
-// var(nv+1) * hmin / rho.
+// var(nv+1) * hmin / rho.  3D: P = the fused primitive planes; 2D: the
+// reference-sequence rho, p, vel (P2 = {rho, p, vel}).
 int trainer_harvest(Trainer &t, const Geo &g, const fst_config &cf, const Phys &ph, const double *U,
                     const double *P, unsigned stride, long long max_rows, long long *added);
+int trainer_harvest2d(Trainer &t, const Geo &g, const fst_config &cf, const Phys &ph, const double *U,
+                      const double *rho, const double *p, const double *vel, unsigned stride, long long max_rows,
+                      long long *added);
 int trainer_train(Trainer &t, int epochs, int batch, float lr, unsigned long long seed, float *loss_hist);
 int trainer_forward_backward(Trainer &t, const float *X, const float *T, int M, float *Y, float *grad, float *loss);
 int trainer_read_rows(Trainer &t, long long r0, long long n, float *X, float *T);

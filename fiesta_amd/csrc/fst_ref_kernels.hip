@@ -608,8 +608,12 @@ __global__ void k_updateceq2d(Geo g, Phys ph, Range r, const double *__restrict_
 }
 
 // computeCeqFlux2D (src/ceq.hpp:197-245) and computeCeqFaces2D (:247-267)
-// fused per face; m planes ((f*2+d)*2+w)
-__global__ void k_ceqfaces2d(Geo g, Phys ph, Range r, const double *__restrict__ var, const double *__restrict__ rho,
+// fused per face; m planes ((f*2+d)*2+w).  LRN: the learned form, the same
+// face composition on the given C_hat and tangent planes (float) in place of
+// var(nv+1..nv+3).
+template <bool LRN>
+__global__ void k_ceqfaces2d(Geo g, Phys ph, Range r, const double *__restrict__ var, const float *__restrict__ lchat,
+                             const float *__restrict__ ldir, const double *__restrict__ rho,
                              const double *__restrict__ vel, const Scalars *sc, double *__restrict__ m) {
   int i, j, k;
   if (!decode(r, i, j, k)) return;
@@ -623,10 +627,16 @@ __global__ void k_ceqfaces2d(Geo g, Phys ph, Range r, const double *__restrict__
   for (int f = 0; f < 2; ++f) {
     const long long q = g.id(i + (f == 0), j + (f == 1), 0);
     double rr = (rho[c] + rho[q]) / 2.0;
-    double chat = (var[c + N * (nv + 1)] + var[q + N * (nv + 1)]) / 2.0;
-    double ct[2];
-    ct[0] = (var[c + N * (nv + 2)] + var[q + N * (nv + 2)]) / 2.0;
-    ct[1] = (var[c + N * (nv + 3)] + var[q + N * (nv + 3)]) / 2.0;
+    double chat, ct[2];
+    if (LRN) {
+      chat = ((double)lchat[c] + (double)lchat[q]) / 2.0;
+      ct[0] = ((double)ldir[c] + (double)ldir[q]) / 2.0;
+      ct[1] = ((double)ldir[c + N] + (double)ldir[q + N]) / 2.0;
+    } else {
+      chat = (var[c + N * (nv + 1)] + var[q + N * (nv + 1)]) / 2.0;
+      ct[0] = (var[c + N * (nv + 2)] + var[q + N * (nv + 2)]) / 2.0;
+      ct[1] = (var[c + N * (nv + 3)] + var[q + N * (nv + 3)]) / 2.0;
+    }
     double ctmag = (ct[0] * ct[0] + ct[1] * ct[1]) + 1.0e-6;
     for (int d = 0; d < 2; ++d) X[f][d] = alpha * rr * chat * (ct[f] * ct[d]) / ctmag;
   }
@@ -1241,8 +1251,9 @@ void launch_eos_vel(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s) 
   else k_eos_vel2d<<<grid_for(r), 256, 0, s>>>(g, ph, r, b.var, b.p, b.rho, b.T, b.vel);
 }
 
-void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s) {
-  launch_eos_vel(g, ph, b, s);
+void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s,
+                        const Ceq2dFields *lf, bool eos_done) {
+  if (!eos_done) launch_eos_vel(g, ph, b, s);
   Range cr = cell_range(g), fr = face_range(g);
   if (g.ndim == 3) {
     for (int v = 0; v < g.nv; ++v) {
@@ -1275,11 +1286,18 @@ void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, cons
       k_stress2d<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, b.vel, b.T, b.stress);
       k_applyvisc2d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.vel, b.stress, b.dvar);
     }
-    if (cf.ceq) {
+    if (cf.ceq && lf) {
+      // learned form: no relaxation solve; the artificial-viscosity terms
+      // (src/cart2d.cpp:223-236) on the given C_hat, tangent and max C_hat
+      k_ceqfaces2d<true><<<grid_for(fr), 256, 0, s>>>(g, ph, fr, b.var, lf->chat, lf->dir, b.rho, b.vel, lf->sc,
+                                                      b.mflux);
+      k_applyceq2d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.mflux, b.dvar);
+    } else if (cf.ceq) {
       k_zero_scalars<<<1, 64, 0, s>>>(b.sc);
       k_rhograd_max2d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.var, b.p, b.rho, b.vel, b.gradRho, b.sc);
       k_updateceq2d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.var, b.gradRho, b.sc, b.dvar);
-      k_ceqfaces2d<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, b.var, b.rho, b.vel, b.sc, b.mflux);
+      k_ceqfaces2d<false><<<grid_for(fr), 256, 0, s>>>(g, ph, fr, b.var, nullptr, nullptr, b.rho, b.vel, b.sc,
+                                                       b.mflux);
       k_applyceq2d<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.mflux, b.dvar);
     }
   }

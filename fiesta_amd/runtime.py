@@ -49,6 +49,7 @@ _EXPORTS = {
                   C.c_size_t], C.c_int),
     "fst_profile_phases": ([C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "fst_mlp_nparams": ([], C.c_int),
+    "fst_mlp_nparams_nd": ([C.c_int], C.c_int),
     "fst_set_mlp_weights": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
     "fst_learned_params": ([C.c_void_p, C.c_float, C.c_float], C.c_int),
     "fst_learned_compute": ([C.c_void_p], C.c_int),
@@ -56,6 +57,8 @@ _EXPORTS = {
                           C.c_int),
     "fst_learned_fields": ([C.c_void_p] + [C.c_void_p] * 4 + [C.c_size_t], C.c_int),
     "fst_mlp_create": ([C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
+    "fst_mlp_create_nd": ([C.c_int, C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
+    "fst_mlp_ndim": ([C.c_void_p], C.c_int),
     "fst_mlp_destroy": ([C.c_void_p], C.c_int),
     "fst_mlp_last_error": ([C.c_void_p], C.c_char_p),
     "fst_mlp_set_weights": ([C.c_void_p, C.c_void_p, C.c_size_t], C.c_int),
@@ -252,20 +255,22 @@ class Module:
         return ch[0], dr, ft, g[0]
 
 
-def mlp_nparams():
-    return lib().fst_mlp_nparams()
+def mlp_nparams(ndim=3):
+    return lib().fst_mlp_nparams() if ndim == 3 else lib().fst_mlp_nparams_nd(int(ndim))
 
 
 class Trainer:
     """Dataset + network of the learned C-indicator, trained on the GPU."""
 
-    def __init__(self, seed=1):
+    def __init__(self, seed=1, ndim=3):
         h = C.c_void_p()
-        rc = lib().fst_mlp_create(int(seed), C.byref(h))
+        rc = lib().fst_mlp_create_nd(int(ndim), int(seed), C.byref(h))
         if rc != 0:
-            raise FiestaError("fst_mlp_create failed (code %d)" % rc)
+            raise FiestaError("fst_mlp_create_nd failed (code %d)" % rc)
         self.h = h
-        self.nparams = mlp_nparams()
+        self.ndim = lib().fst_mlp_ndim(h)
+        self.nparams = mlp_nparams(self.ndim)
+        self.nin = 375 if self.ndim == 3 else 75
 
     def close(self):
         if getattr(self, "h", None):
@@ -303,7 +308,7 @@ class Trainer:
         return n.value
 
     def read_rows(self, r0, n):
-        X = np.empty((n, 375), np.float32)
+        X = np.empty((n, self.nin), np.float32)
         T = np.empty(n, np.float32)
         self._chk(lib().fst_mlp_read_rows(self.h, int(r0), int(n), X.ctypes.data, T.ctypes.data), "fst_mlp_read_rows")
         return X, T
@@ -318,7 +323,8 @@ class Trainer:
         X = np.ascontiguousarray(X, dtype=np.float32)
         T = np.ascontiguousarray(T, dtype=np.float32)
         M = T.size
-        assert X.shape == (M, 375)
+        if X.shape != (M, self.nin):
+            raise ValueError("X must be (rows, %d) for the %dD network" % (self.nin, self.ndim))
         Y = np.empty(M, np.float32)
         G = np.empty(self.nparams, np.float32)
         L = np.empty(1, np.float32)

@@ -186,6 +186,52 @@ def expansion2d_state(cfg, T0=300.0, r0=0.05):
                           [np.zeros(sh)] * 5 if cfg.ceq else None)
 
 
+def multiblast2d_config(n=2048, ceq=True, learned_c=False, visc=True, rk=3, seed=4, nblobs=8, cfl=0.5):
+    """2D multi-blast: 1 m box, n^2 cells, three species (air, hot gas, SF6):
+    `nblobs` Gaussian high-pressure blobs of hot gas (radius 0.05, pressure
+    ratio ~ U(10, 100)) at seeded uniform positions in [0.1, 0.9]^2, in air
+    at 1 atm below a heavy SF6 layer (y > 0.7, tanh interface of width
+    0.01); outflow on all sides, viscous, the C-equation as in the 2D Sod and
+    expansion decks (kappa 10, eps 1, alpha 2), SSP-RK3 (rk=3) and dt from
+    CFL `cfl` at t = 0."""
+    mix = _mix(3, [AIR, HOT, SF6])
+    cfg = make_config(ndim=2, ns=3, nci=n, ncj=n, dx=1.0 / n, dy=1.0 / n, R=RU, visc=visc, ceq=ceq,
+                      learned_c=learned_c, kap=10.0, eps=1.0, alpha=2.0, bc=("outflow",) * 6, dt=1.0, rk=rk,
+                      **mix)
+    cfg.dt = cfl_dt(cfg, multiblast2d_state(cfg, seed, nblobs), cfl)
+    return cfg
+
+
+def multiblast2d_blobs(seed=4, nblobs=8):
+    """(x, y, pressure ratio) of each blob, from the seed."""
+    rng = np.random.default_rng(seed)
+    xy = rng.uniform(0.1, 0.9, (nblobs, 2))
+    pr = rng.uniform(10.0, 100.0, nblobs)
+    return [(float(xy[b, 0]), float(xy[b, 1]), float(pr[b])) for b in range(nblobs)]
+
+
+def multiblast2d_state(cfg, seed=4, nblobs=8, T0=300.0, r0=0.05, p0=101325.0):
+    d = Dims(cfg)
+    x, y, _ = _cells(d)
+    x = np.broadcast_to(x, (1, d.ncj, d.nci))
+    y = np.broadcast_to(y, (1, d.ncj, d.nci))
+    yhot = np.zeros((1, d.ncj, d.nci))
+    excess = np.zeros_like(yhot)  # p / p0 - 1
+    for bx, by, pr in multiblast2d_blobs(seed, nblobs):
+        g = np.exp(-((x - bx) ** 2 + (y - by) ** 2) / r0 ** 2)
+        yhot = np.maximum(yhot, g)
+        excess = excess + (pr - 1.0) * g
+    p = p0 * (1.0 + excess)
+    ysf6 = (1.0 - yhot) * 0.5 * (1.0 + np.tanh((y - 0.7) / 0.01))
+    yair = 1.0 - yhot - ysf6
+    Ys = [yair, yhot, ysf6]
+    Rm = sum(Ys[s] * cfg.R / cfg.M[s] for s in range(3))
+    rho = p / (Rm * T0)
+    rho_s = [rho * Ys[s] for s in range(3)]
+    sh = rho.shape
+    return fill_conserved(cfg, rho_s, [np.zeros(sh), np.zeros(sh)], p, [np.zeros(sh)] * 5 if cfg.ceq else None)
+
+
 def rt3d_config(n=256, ceq=True, visc=True, seed=3, steps_dt=None, cfl=0.2):
     """3D Rayleigh-Taylor: 1 m cube, n^3 cells, heavy air (M=28.96 g/mol)
     over light gas (M=4 g/mol), gravity 9.81 along -y, x/z periodic, y

@@ -33,7 +33,7 @@ extern "C" {
 #endif
 
 #define FST_MAX_SPECIES 8
-#define FST_ABI_VERSION 5
+#define FST_ABI_VERSION 6
 
 /* Boundary types, in the order of `enum class BCType` (src/bc.hpp:31). */
 enum fst_bc_type {
@@ -109,8 +109,9 @@ typedef struct fst_config {
   double dt;             /* time.dt                                             */
   int32_t precision;     /* fst_precision                                       */
   int32_t learned_c;     /* 1: learned C-indicator replaces the C-equation's
-                            relaxation solve (3D, ceq = 1, fused path only;
-                            nvt = nv: no C-equation variables are stored)    */
+                            relaxation solve (ceq = 1; 3D: fused path only;
+                            2D: FP64; no noise filter; nvt = nv: no
+                            C-equation variables are stored)                 */
   int32_t rk_order;      /* 2 (or 0): the reference's RK2 midpoint (rkAdvance);
                             3: three-stage SSP-RK3 (Shu-Osher), an option the
                             reference does not have                          */
@@ -294,12 +295,25 @@ int fst_copy_probe(fst_ctx *ctx, size_t bytes, int reps, double *ms_per_copy);
  * equation with the neighbouring predictions held.
  * The direction n is the 4th-order density gradient.  Both stages of the
  * step use these fields.
+ *
+ * 2D (cart2d, src/ceq.hpp:80-293, src/cart2d.cpp:201-236): the same on 5x5
+ * stencils (75 inputs) and 3x3 neighbourhoods; the network stands in for
+ * updateCeq2D's relaxation of var(nv+1) (the contact indicator), the check
+ * uses a = kap eps (dx^2 + dy^2) and the two-direction Laplacian, and the
+ * tangent (-d rho/dy, d rho/dx) (4th order) takes the place of the relaxed
+ * var(nv+2..nv+3) in computeCeqFlux2D.  Fields are evaluated at stage 1 of
+ * each fst_step step, after the ghost fill of preStep.
  * ------------------------------------------------------------------------ */
 
-/* Number of FP32 network parameters: W1[375][64], b1[64], W2[64][64], b2[64],
- * W3[64][64], b3[64], w4[64], b4 (row-major, input index major). */
+/* Number of FP32 network parameters of the 3D network: W1[375][64], b1[64],
+ * W2[64][64], b2[64], W3[64][64], b3[64], w4[64], b4 (row-major, input index
+ * major; input index = channel*125 + (dz+2)*25 + (dy+2)*5 + (dx+2)). */
 int fst_mlp_nparams(void);
-/* Weights of a learned_c context (copied; required before fst_step). */
+/* The same for the ndim network (2: W1[75][64], input index = channel*25 +
+ * (dy+2)*5 + (dx+2); 3: as above); -1 for another ndim. */
+int fst_mlp_nparams_nd(int ndim);
+/* Weights of a learned_c context (copied; required before fst_step); n is
+ * fst_mlp_nparams_nd(the context's ndim). */
 int fst_set_mlp_weights(fst_ctx *ctx, const float *w, size_t n);
 /* Activity threshold tau (default 1e-3) and residual tolerance tol (0.05,
  * a fraction of max G1; a huge value switches the fallback off). */
@@ -311,29 +325,37 @@ int fst_learned_compute(fst_ctx *ctx);
  * last field evaluation. */
 int fst_learned_stats(fst_ctx *ctx, int64_t *active, int64_t *fallback, double *maxCh);
 /* Download the last fields (any pointer may be NULL): C_hat [N], direction
- * [3][N], features [3][N], G [N]; `plane` must equal ncells_ghost. */
+ * [3][N] (2D: the tangent in planes 0-1, plane 2 zero), features [3][N],
+ * G [N]; `plane` must equal ncells_ghost. */
 int fst_learned_fields(fst_ctx *ctx, float *chat, float *dir, float *feat, float *g, size_t plane);
 
 /* Trainer: a dataset of (stencil, C_hat h/rho) rows harvested on the device
  * from plain C-equation runs, and the network being trained (FP32 master
  * weights, BF16 MFMA forward/backward, Adam).  One HIP stream per trainer. */
 typedef struct fst_mlp fst_mlp;
+/* A 3D network (5x5x5 stencils). */
 int fst_mlp_create(uint64_t seed, fst_mlp **out);
+/* A network for ndim = 2 (5x5 stencils) or 3; harvests only from contexts of
+ * that ndim. */
+int fst_mlp_create_nd(int ndim, uint64_t seed, fst_mlp **out);
+int fst_mlp_ndim(const fst_mlp *m);
 int fst_mlp_destroy(fst_mlp *m);
 const char *fst_mlp_last_error(const fst_mlp *m);
 int fst_mlp_set_weights(fst_mlp *m, const float *w, size_t n);
 int fst_mlp_get_weights(fst_mlp *m, float *w, size_t n);
-/* Rows from the current var of a plain 3D C-equation context: every active
- * cell and every `stride`-th interior cell (stride 0: active cells only). */
+/* Rows from the current var of a plain C-equation context (cartesian, the
+ * network's ndim; 3D: ns <= 3): every active cell and every `stride`-th
+ * interior cell (stride 0: active cells only).  The context's ghost cells
+ * are filled first, as its next compute() would see them. */
 int fst_mlp_harvest(fst_mlp *m, fst_ctx *plain, uint32_t stride, int64_t max_rows, int64_t *added);
 int fst_mlp_rows(fst_mlp *m, int64_t *rows);
-/* Rows [r0, r0+n): X [n][375] (BF16 values as FP32), targets [n]. */
+/* Rows [r0, r0+n): X [n][375] (2D: [n][75]; BF16 values as FP32), targets [n]. */
 int fst_mlp_read_rows(fst_mlp *m, int64_t r0, int64_t n, float *X, float *T);
 /* Mini-batch Adam over the dataset (seeded batch order); mean batch loss
  * per epoch into loss_per_epoch (may be NULL). */
 int fst_mlp_train(fst_mlp *m, int epochs, int batch, float lr, uint64_t seed, float *loss_per_epoch);
 /* One forward + backward pass of the current weights over host rows X
- * [rows][375] with targets T: outputs Y [rows], the gradient of the mean
+ * [rows][375] (2D: [rows][75]) with targets T: outputs Y [rows], the gradient of the mean
  * squared error [nparams] and the loss (any output may be NULL). */
 int fst_mlp_forward_backward(fst_mlp *m, const float *X, const float *T, int64_t rows, float *Y, float *grad,
                              float *loss);

@@ -755,10 +755,75 @@ void visc2d(Orc &o) {
 // (:43-78, stride-2 species lookup and |m| kept), maxCvar2D (:23-41),
 // updateCeq2D (:153-195), computeCeqFlux2D (:197-245), computeCeqFaces2D
 // (:247-267), applyCeq2D (:269-293); sequence src/cart2d.cpp:201-236
+// computeCeqFlux2D, computeCeqFaces2D and applyCeq2D (src/ceq.hpp:197-293)
+// over the face range; the learned form reads C_hat and the tangent from
+// lchat / ldir in place of var(nv+1..nv+3).
+void ceqfaces2d(Orc &o) {
+  const int ng = o.ng, nv = o.nv;
+  const size_t N = o.N;
+  const double dx = o.c.dx, dy = o.c.dy;
+  const double alpha = o.alpha;
+#pragma omp parallel for
+  for (int j = ng - 1; j < o.ngj - ng; ++j)
+    for (int i = ng - 1; i < o.ngi - ng; ++i) {
+      for (int f = 0; f < 2; ++f) {
+        int ih = f == 0, jh = f == 1;
+        size_t c = o.id(i, j, 0), q = o.id(i + ih, j + jh, 0);
+        double r = (o.rho[c] + o.rho[q]) / 2.0;
+        double chat, ct[2];
+        if (o.learned) {
+          chat = (o.lchat[c] + o.lchat[q]) / 2.0;
+          ct[0] = (o.ldir[c] + o.ldir[q]) / 2.0;
+          ct[1] = (o.ldir[c + N] + o.ldir[q + N]) / 2.0;
+        } else {
+          chat = (o.var[c + N * (nv + 1)] + o.var[q + N * (nv + 1)]) / 2.0;
+          ct[0] = (o.var[c + N * (nv + 2)] + o.var[q + N * (nv + 2)]) / 2.0;
+          ct[1] = (o.var[c + N * (nv + 3)] + o.var[q + N * (nv + 3)]) / 2.0;
+        }
+        double ctmag = (ct[0] * ct[0] + ct[1] * ct[1]) + 1.0e-6;
+        for (int d = 0; d < 2; ++d)
+          for (int w = 0; w < 2; ++w) o.M2(f, d, i, j, w) = alpha * r * chat * (ct[f] * ct[d]) / ctmag;
+      }
+    }
+  {
+    const double cdx = o.cd[1], cdy = o.cd[2];
+#pragma omp parallel for
+    for (int j = ng - 1; j < o.ngj - ng; ++j)
+      for (int i = ng - 1; i < o.ngi - ng; ++i) {
+        auto v = [&](int a, int b, int w) { return o.vel[o.id(a, b, 0) + N * w]; };
+        for (int w = 0; w < 2; ++w) {
+          o.M2(0, 0, i, j, w) *= (v(i + 1, j, w) - v(i, j, w)) / cdx;
+          o.M2(0, 1, i, j, w) *= ((v(i, j + 1, w) + v(i + 1, j + 1, w)) - (v(i, j - 1, w) + v(i + 1, j - 1, w))) / (4.0 * cdy);
+          o.M2(1, 0, i, j, w) *= ((v(i + 1, j, w) + v(i + 1, j + 1, w)) - (v(i - 1, j, w) + v(i - 1, j + 1, w))) / (4.0 * cdx);
+          o.M2(1, 1, i, j, w) *= (v(i, j + 1, w) - v(i, j, w)) / cdy;
+        }
+      }
+  }
+#pragma omp parallel for
+  for (int j = ng; j < o.ngj - ng; ++j)
+    for (int i = ng; i < o.ngi - ng; ++i) {
+      for (int w = 0; w < 2; ++w) {
+        double diffu = 0;
+        diffu += (o.M2(0, 0, i, j, w) - o.M2(0, 0, i - 1, j, w)) / dx;
+        diffu += (o.M2(0, 1, i, j, w) - o.M2(0, 1, i - 1, j, w)) / dx;
+        diffu += (o.M2(1, 0, i, j, w) - o.M2(1, 0, i, j - 1, w)) / dy;
+        diffu += (o.M2(1, 1, i, j, w) - o.M2(1, 1, i, j - 1, w)) / dy;
+        o.dvar[o.id(i, j, 0) + N * w] += diffu;
+      }
+    }
+}
+
 void ceq2d(Orc &o) {
   const int ng = o.ng, ns = o.ns;
   const size_t N = o.N;
   const double dx = o.c.dx, dy = o.c.dy;
+  if (o.learned) {
+    // learned form: no relaxation solve; the artificial-viscosity terms of
+    // src/cart2d.cpp:223-236 on the given C_hat, tangent and max C_hat
+    o.alpha = (o.c.dx * o.c.dx + o.c.dy * o.c.dy) / (o.lmaxCh + 1.0e-6) * o.c.alpha;
+    ceqfaces2d(o);
+    return;
+  }
 #pragma omp parallel for
   for (int j = ng; j < o.ngj - ng; ++j)
     for (int i = ng; i < o.ngi - ng; ++i) {
@@ -840,49 +905,7 @@ void ceq2d(Orc &o) {
         }
       }
   }
-  const double alpha = o.alpha;
-#pragma omp parallel for
-  for (int j = ng - 1; j < o.ngj - ng; ++j)
-    for (int i = ng - 1; i < o.ngi - ng; ++i) {
-      for (int f = 0; f < 2; ++f) {
-        int ih = f == 0, jh = f == 1;
-        size_t c = o.id(i, j, 0), q = o.id(i + ih, j + jh, 0);
-        double r = (o.rho[c] + o.rho[q]) / 2.0;
-        double chat = (o.var[c + N * (nv + 1)] + o.var[q + N * (nv + 1)]) / 2.0;
-        double ct[2];
-        ct[0] = (o.var[c + N * (nv + 2)] + o.var[q + N * (nv + 2)]) / 2.0;
-        ct[1] = (o.var[c + N * (nv + 3)] + o.var[q + N * (nv + 3)]) / 2.0;
-        double ctmag = (ct[0] * ct[0] + ct[1] * ct[1]) + 1.0e-6;
-        for (int d = 0; d < 2; ++d)
-          for (int w = 0; w < 2; ++w) o.M2(f, d, i, j, w) = alpha * r * chat * (ct[f] * ct[d]) / ctmag;
-      }
-    }
-  {
-    const double cdx = o.cd[1], cdy = o.cd[2];
-#pragma omp parallel for
-    for (int j = ng - 1; j < o.ngj - ng; ++j)
-      for (int i = ng - 1; i < o.ngi - ng; ++i) {
-        auto v = [&](int a, int b, int w) { return o.vel[o.id(a, b, 0) + N * w]; };
-        for (int w = 0; w < 2; ++w) {
-          o.M2(0, 0, i, j, w) *= (v(i + 1, j, w) - v(i, j, w)) / cdx;
-          o.M2(0, 1, i, j, w) *= ((v(i, j + 1, w) + v(i + 1, j + 1, w)) - (v(i, j - 1, w) + v(i + 1, j - 1, w))) / (4.0 * cdy);
-          o.M2(1, 0, i, j, w) *= ((v(i + 1, j, w) + v(i + 1, j + 1, w)) - (v(i - 1, j, w) + v(i - 1, j + 1, w))) / (4.0 * cdx);
-          o.M2(1, 1, i, j, w) *= (v(i, j + 1, w) - v(i, j, w)) / cdy;
-        }
-      }
-  }
-#pragma omp parallel for
-  for (int j = ng; j < o.ngj - ng; ++j)
-    for (int i = ng; i < o.ngi - ng; ++i) {
-      for (int w = 0; w < 2; ++w) {
-        double diffu = 0;
-        diffu += (o.M2(0, 0, i, j, w) - o.M2(0, 0, i - 1, j, w)) / dx;
-        diffu += (o.M2(0, 1, i, j, w) - o.M2(0, 1, i - 1, j, w)) / dx;
-        diffu += (o.M2(1, 0, i, j, w) - o.M2(1, 0, i, j - 1, w)) / dy;
-        diffu += (o.M2(1, 1, i, j, w) - o.M2(1, 1, i, j - 1, w)) / dy;
-        o.dvar[o.id(i, j, 0) + N * w] += diffu;
-      }
-    }
+  ceqfaces2d(o);
 }
 
 // ---------------------------------------------------------------- generalized grids
@@ -1294,6 +1317,76 @@ void features3d(Orc &o, float *F, float *M, float *Dir, float *G1) {
         Dir[t + 2 * N] = d2;
         G1[t] = gg;
       }
+}
+
+// Features of the 2D learned C-indicator on the current primitives (eos2d and
+// vel2d must have run), in the order and precision of the GPU feature pass.
+// The tangent (-d rho/dy, d rho/dx) and the contact switch follow
+// calculateRhoGrad2D (src/ceq.hpp:80-151): 4th-order gradients of rho and of
+// nrg = E / rho - rho |u|^2 / 2; G1 = |grad rho| h / rho where
+// (grad rho . grad nrg) |grad rho|^2 < 0.
+void features2d(Orc &o, float *F, float *M, float *Dir, float *G1) {
+  const size_t N = o.N;
+  const double d[2] = {o.c.dx, o.c.dy};
+  const double hmin = std::min(d[0], d[1]);
+  const long long st[2] = {1, o.ngi};
+#pragma omp parallel for
+  for (int j = 0; j < o.ngj; ++j)
+    for (int i = 0; i < o.ngi; ++i) {
+      const size_t t = o.id(i, j, 0);
+      const bool in1 = i >= 1 && j >= 1 && i < o.ngi - 1 && j < o.ngj - 1;
+      const bool in2 = i >= 2 && j >= 2 && i < o.ngi - 2 && j < o.ngj - 2;
+      float f0 = 0, f1 = 0, f2 = 0, d0 = 0, d1 = 0, gg = 0;
+      if (in1) {
+        auto nrg = [&](size_t q) {
+          const double r = o.rho[q], u = o.vel[q], v = o.vel[q + N];
+          return o.var[q + 2 * N] / r - 0.5 * r * (u * u + v * v);
+        };
+        double gr2 = 0.0, gp2 = 0.0, divu = 0.0, dr[2] = {0, 0}, de[2] = {0, 0};
+        for (int a = 0; a < 2; ++a) {
+          const size_t sp = t + st[a], sm = t - st[a];
+          const double h2 = 2.0 * d[a];
+          const double r1 = (o.rho[sp] - o.rho[sm]) / h2;
+          const double p1 = (o.p[sp] - o.p[sm]) / h2;
+          gr2 = gr2 + r1 * r1;
+          gp2 = gp2 + p1 * p1;
+          divu = divu + (o.vel[sp + N * a] - o.vel[sm + N * a]) / h2;
+          if (in2) {
+            const size_t sp2 = t + 2 * st[a], sm2 = t - 2 * st[a];
+            const double h12 = 12.0 * d[a];
+            dr[a] = (o.rho[sm2] - 8.0 * o.rho[sm] + 8.0 * o.rho[sp] - o.rho[sp2]) / h12;
+            de[a] = (nrg(sm2) - 8.0 * nrg(sm) + 8.0 * nrg(sp) - nrg(sp2)) / h12;
+          }
+        }
+        const double rho = o.rho[t], p = o.p[t];
+        double Cp = 0.0, Cv = 0.0;
+        for (int s = 0; s < o.ns; ++s) {
+          const double gs = o.cd[6 + 3 * s], Rs = o.cd[6 + 3 * s + 1];
+          const double Y = o.var[t + N * (3 + s)] / rho;
+          Cp = Cp + Y * (gs * Rs / (gs - 1));
+          Cv = Cv + Y * (Rs / (gs - 1));
+        }
+        const double a_snd = std::sqrt((Cp / Cv) * p / rho);
+        f0 = (float)(std::sqrt(gr2) * hmin / rho);
+        f1 = (float)(std::sqrt(gp2) * hmin / p);
+        f2 = (float)(divu * hmin / a_snd);
+        if (in2) {
+          d0 = (float)(-dr[1]);
+          d1 = (float)dr[0];
+          const double rg2 = dr[0] * dr[0] + dr[1] * dr[1];
+          const double dnednr = (dr[0] * de[0] + dr[1] * de[1]) * rg2;
+          gg = dnednr < 0.0 ? (float)(std::sqrt(rg2) * hmin / rho) : 0.f;
+        }
+      }
+      F[t] = f0;
+      F[t + N] = f1;
+      F[t + 2 * N] = f2;
+      M[t] = std::max(std::fabs(f0), std::max(std::fabs(f1), std::fabs(f2)));
+      Dir[t] = d0;
+      Dir[t + N] = d1;
+      Dir[t + 2 * N] = 0.f;
+      G1[t] = gg;
+    }
 }
 
 // gen3d_func::compute, src/gen3d.cpp:341-369: no boundary conditions, no
@@ -1731,13 +1824,20 @@ int orc_set_learned(void *h, const double *chat, const double *dir, double maxCh
   return 0;
 }
 
-// Primitives (eos3d, vel3d) and learned-C features of the current var.
+// Primitives (eos3d, vel3d / eos2d, vel2d) and learned-C features of the
+// current var (Dir: 3 planes; in 2D the tangent, plane 2 zero).
 int orc_features(void *h, float *F, float *M, float *Dir, float *G1) {
   Orc &o = *(Orc *)h;
-  if (o.ndim != 3) return 1;
-  eos3d(o);
-  vel3d(o);
-  features3d(o, F, M, Dir, G1);
+  if (o.gen) return 1;
+  if (o.ndim == 3) {
+    eos3d(o);
+    vel3d(o);
+    features3d(o, F, M, Dir, G1);
+  } else {
+    eos2d(o);
+    vel2d(o);
+    features2d(o, F, M, Dir, G1);
+  }
   return 0;
 }
 

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_struct_size():
     lib = runtime.lib()
-    assert lib.fst_abi_version() == 5
+    assert lib.fst_abi_version() == 6
     # fst_config layout: the ctypes mirror and the C struct must agree in size
     c = FstConfig()
     lib.fst_config_init(C.byref(c))
@@ -94,13 +94,25 @@ def test_learned_form_sizes_and_validation():
     d = FstDims()
     assert lib.fst_config_dims(C.byref(cfg), C.byref(d)) == 0
     assert d.nvt == d.nv == 6 and Dims(cfg).nvt == 6  # no C-equation variables in the learned form
-    for field, value in (("ceq", 0), ("ndim", 2)):
+    for field, value in (("ceq", 0), ("noise", 1)):
         bad = make_config(ndim=3, ns=2, nci=12, ncj=10, nck=8, gamma=(1.4, 1.67), M=(0.029, 0.004),
                           mu=(1e-5, 1e-5), ceq=True, learned_c=True)
         setattr(bad, field, value)
         assert lib.fst_config_dims(C.byref(bad), C.byref(FstDims())) == 1
-    # parameter vector: W1[375][64], b1, W2[64][64], b2, W3[64][64], b3, w4[64], b4
-    assert lib.fst_mlp_nparams() == 375 * 64 + 64 + 2 * (64 * 64 + 64) + 64 + 1
+    # 2D: the cart2d sequence in FP64, 5x5 stencils, no C-equation variables
+    cfg2 = make_config(ndim=2, ns=3, nci=16, ncj=12, gamma=(1.4, 1.67, 1.092), M=(0.029, 0.004, 0.146),
+                       mu=(1e-5,) * 3, ceq=True, learned_c=True)
+    d2 = FstDims()
+    assert lib.fst_config_dims(C.byref(cfg2), C.byref(d2)) == 0
+    assert d2.nvt == d2.nv == 6 and Dims(cfg2).nvt == 6
+    cfg2.precision = 1
+    assert lib.fst_config_dims(C.byref(cfg2), C.byref(FstDims())) == 1
+    # parameter vector: W1[IN][64], b1, W2[64][64], b2, W3[64][64], b3, w4[64], b4
+    rest = 64 + 2 * (64 * 64 + 64) + 64 + 1
+    assert lib.fst_mlp_nparams() == 375 * 64 + rest
+    assert lib.fst_mlp_nparams_nd(3) == 375 * 64 + rest
+    assert lib.fst_mlp_nparams_nd(2) == 75 * 64 + rest
+    assert lib.fst_mlp_nparams_nd(1) == -1 and lib.fst_mlp_nparams_nd(4) == -1
 
 
 def test_learned_entry_points_reject_null_arguments():
@@ -111,6 +123,10 @@ def test_learned_entry_points_reject_null_arguments():
     assert lib.fst_learned_stats(None, None, None, None) == 1
     assert lib.fst_learned_fields(None, None, None, None, None, 0) == 1
     assert lib.fst_mlp_create(1, None) == 1
+    assert lib.fst_mlp_create_nd(2, 1, None) == 1
+    h = C.c_void_p()
+    assert lib.fst_mlp_create_nd(4, 1, C.byref(h)) == 1 and not h.value  # checked before any device call
+    assert lib.fst_mlp_ndim(None) == -1
     assert lib.fst_mlp_destroy(None) == 1
     assert lib.fst_mlp_train(None, 1, 16, 1e-3, 1, None) == 1
     assert lib.fst_mlp_harvest(None, None, 0, 0, None) == 1

@@ -1,0 +1,210 @@
+"""GPU tests of the 2D learned C-indicator (cart2d), all through the C-ABI.
+
+- forward/backward of the 2D (5x5 stencil) network on the BF16-MFMA
+  training kernels against the numpy oracle with the same BF16 rounding
+  points;
+- the 2D feature pass (|grad rho|, |grad p|, div u, the tangent, the contact
+  source) equals the C oracle's bit for bit;
+- 2D inference (active cells, network, residual check with the
+  two-direction Laplacian, fallback, ghost layer) against the numpy oracle;
+- FP64 learned steps of the cart2d sequence (RK2 and SSP-RK3) equal the
+  oracle's learned steps bit for bit when the oracle is given the same
+  C_hat, tangent and max C_hat;
+- rows harvested from plain 2D runs (the Sod tube and the ideal expansion)
+  train the network: the loss falls.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp
+from fiesta_amd import workloads as W
+from fiesta_amd.config import FST_RHO, FST_VAR, Dims
+from fiesta_amd.runtime import Module, Trainer
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_forward_backward_2d_matches_oracle():
+    print("case: forward_backward 2D M=260 seed=6", flush=True)
+    tr = Trainer(seed=6, ndim=2)
+    assert tr.ndim == 2 and tr.nparams == mlp.NPARAM2
+    w = tr.weights()
+    rng = np.random.default_rng(1)
+    M = 260
+    X = (rng.random((M, 75)) * rng.random((M, 1))).astype(np.float32)
+    T = (0.2 * rng.random(M)).astype(np.float32)
+    Y, G, L = tr.forward_backward(X, T)
+    loss, g, y = mlp.loss_grad(w, X, T, bf=True)
+    assert _rel(Y, y) < 1e-3, _rel(Y, y)
+    assert abs(L - loss) <= 2e-3 * loss
+    lay = mlp.layout(2)
+    blocks = [("W1", "OFF_W1", "OFF_B1"), ("b1", "OFF_B1", "OFF_W2"), ("W2", "OFF_W2", "OFF_B2"),
+              ("b2", "OFF_B2", "OFF_W3"), ("W3", "OFF_W3", "OFF_B3"), ("b3", "OFF_B3", "OFF_W4"),
+              ("w4", "OFF_W4", "OFF_B4"), ("b4", "OFF_B4", "NPARAM")]
+    for name, a, b in blocks:
+        sa, sb = lay[a], lay[b]
+        assert _rel(G[sa:sb], g[sa:sb]) < 5e-3, (name, _rel(G[sa:sb], g[sa:sb]))
+    tr.close()
+
+
+def _case2d(n=40, seed=3, nblobs=3):
+    cfg = W.multiblast2d_config(n, learned_c=True, seed=seed, nblobs=nblobs)
+    var = W.multiblast2d_state(cfg, seed, nblobs)
+    rng = np.random.default_rng(seed)
+    d = Dims(cfg)
+    sl = d.interior()
+    var[sl][0:2] += 20.0 * rng.standard_normal(var[sl][0:2].shape)  # momenta: a non-zero div u
+    return cfg, var
+
+
+def test_features2d_equal_oracle():
+    print("case: features 2D multiblast 40x40", flush=True)
+    cfg, var = _case2d()
+    tr = Trainer(seed=3, ndim=2)
+    m = Module(cfg)
+    m.upload(FST_VAR, var)
+    m.set_mlp_weights(tr.weights())
+    m.learned_compute()
+    chat, dr, ft, g = m.learned_fields()
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.apply_bcs()
+    F, M, Dr, G = o.features()
+    assert np.array_equal(ft, F)
+    assert np.array_equal(dr, Dr)
+    assert np.array_equal(g, G)
+    assert np.abs(F[0]).max() > 0.05 and np.abs(F[2]).max() > 0.0
+    assert np.abs(dr[0]).max() > 0 and np.abs(dr[1]).max() > 0
+    m.close()
+    tr.close()
+
+
+def test_inference2d_matches_oracle():
+    print("case: inference 2D multiblast 40x40", flush=True)
+    cfg, var = _case2d()
+    d = Dims(cfg)
+    tr = Trainer(seed=9, ndim=2)
+    w = tr.weights()
+    m = Module(cfg)
+    m.upload(FST_VAR, var)
+    m.set_mlp_weights(w)
+    tau, tol = 1e-3, 0.05
+    m.learned_params(tau, tol)
+    m.learned_compute()
+    chat, dr, ft, g = m.learned_fields()
+    active, nfb, maxch = m.learned_stats()
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.apply_bcs()
+    F, M, Dr, G = o.features()
+    rho = o.get(FST_RHO)[0]
+    hmin = min(cfg.dx, cfg.dy)
+    want, y, fb, cells, rn, thr = mlp.chat_field(w, F, M, G, rho, d, hmin, cfg.kap, cfg.eps, (cfg.dx, cfg.dy),
+                                                 tau, tol, False, False, False)
+    assert active == cells.size and active > 0
+    near = np.abs(rn - thr) < 5e-3 * max(thr, 1e-30)
+    assert abs(nfb - int(fb.sum())) <= int(near.sum())
+    assert 0 < fb.sum() < fb.size  # untrained weights: both branches run
+    got = chat.ravel()
+    ref = want.ravel()
+    keep = np.ones(d.N, bool)
+    s1 = d.ngi
+    for c in cells[near].tolist():
+        for off in (0, 1, -1, s1, -s1):
+            keep[c + off] = False
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref)[keep]
+    assert np.all(err <= 2e-3 * np.abs(ref[keep]) + 1e-5 * scale), float(err.max())
+    assert np.isclose(maxch, got.max(), rtol=1e-7)
+    fb_cells = cells[fb & ~near]
+    assert fb_cells.size > 0
+    np.testing.assert_allclose(got[fb_cells], ref[fb_cells], rtol=2e-3, atol=1e-5 * scale)
+    m.close()
+    tr.close()
+
+
+@pytest.mark.parametrize("rk", [2, 3])
+def test_learned2d_steps_equal_oracle(rk):
+    print("case: learned 2D steps multiblast 48x40, rk", rk, flush=True)
+    cfg, var = _case2d(n=48, seed=5)
+    cfg.rk_order = rk
+    tr = Trainer(seed=11, ndim=2)
+    m = Module(cfg)
+    m.upload(FST_VAR, var)
+    m.set_mlp_weights(tr.weights())
+    m.pre_sim()
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    d = Dims(cfg)
+    sl = d.interior()
+    for s in range(3):
+        m.step(1)
+        chat, dr, _, _ = m.learned_fields()
+        _, _, mch = m.learned_stats()
+        o.set_learned(chat, dr, mch)
+        o.step(1)
+        got, want = m.download(FST_VAR), o.get(FST_VAR)
+        assert np.array_equal(got[sl], want[sl]), ("step", s, int((got[sl] != want[sl]).sum()))
+        assert mch > 0
+    m.close()
+    tr.close()
+
+
+def test_learned2d_needs_weights_and_rejects_the_other_dimension():
+    print("case: 2D learned context without weights / 3D weights", flush=True)
+    cfg, var = _case2d(n=32)
+    m = Module(cfg)
+    m.upload(FST_VAR, var)
+    with pytest.raises(Exception):
+        m.step(1)
+    with pytest.raises(Exception):
+        m.set_mlp_weights(np.zeros(mlp.NPARAM, np.float32))  # the 3D network's count
+    tr3 = Trainer(seed=1, ndim=3)
+    plain = Module(W.sod_config(64, 4))
+    plain.upload(FST_VAR, W.sod_state(plain.cfg))
+    with pytest.raises(Exception):
+        tr3.harvest(plain)  # a 3D network cannot take 2D rows
+    tr3.close()
+    plain.close()
+    m.close()
+
+
+def test_harvest2d_and_training_lowers_loss():
+    print("case: harvest 2D Sod 400x4 + expansion 96^2, train", flush=True)
+    tr = Trainer(seed=2, ndim=2)
+    runs = [(W.sod_config(400, 4, dt=5e-4), W.sod_state), (W.expansion2d_config(96), W.expansion2d_state)]
+    for cfg, st in runs:
+        cfg.rk_order = 3
+        m = Module(cfg)
+        m.upload(FST_VAR, st(cfg))
+        m.pre_sim()
+        for _ in range(6):
+            m.step(10)
+            tr.harvest(m, stride=7)
+        m.close()
+    n = tr.rows()
+    assert n > 2000
+    X, T = tr.read_rows(0, n)
+    assert X.shape[1] == 75 and np.isfinite(X).all() and np.isfinite(T).all()
+    # Negative targets come from the plain 2D method itself: its wave speed
+    # (maxWaveSpeed2D, src/ceq.hpp:43-78) reads the species constants with
+    # stride 2, so with two species the relaxation rate maxS / (eps |dx|) is
+    # far off where the hot gas dominates and the relaxed C_hat oscillates
+    # there.  The network's output is clipped at 0.
+    print("targets: max %.4g min %.4g, %d negative" % (T.max(), T.min(), int((T < 0).sum())))
+    assert T.max() > 0
+    _, _, before = tr.forward_backward(X, T)
+    loss = tr.train(20, batch=256, lr=1e-3, seed=3)
+    assert np.all(np.isfinite(loss))
+    _, _, after = tr.forward_backward(X, T)
+    print("rows %d, dataset loss before %.4g after %.4g (target variance %.4g)" % (n, before, after, T.var()))
+    assert after < 0.3 * before and after < 0.5 * T.var(), (before, after, T.var())
+    tr.close()
