@@ -57,6 +57,11 @@ def parse():
     ap.add_argument("--extras", action="store_true",
                     help="also time postStep's noise filter, the device-side output pack (fst_pack), the gen3d module and the 2D expansion on the "
                          "fp64 workload")
+    ap.add_argument("--multiblast", action="store_true",
+                    help="the 2D learned C-indicator on the multi-blast (trained on 2D Sod and expansion runs), "
+                         "against the plain 2D C-equation; also part of --extras")
+    ap.add_argument("--multiblast-size", type=int, default=2048)
+    ap.add_argument("--multiblast-steps", type=int, default=400)
     ap.add_argument("--save-weights", default=None, help="write the trained network (.npy, float32)")
     ap.add_argument("--load-weights", default=None, help="use this network (.npy) instead of training one")
     return ap.parse_args()
@@ -103,6 +108,158 @@ def train_learned():
             "data": "plain FP64 runs: RT 64^3 (200 steps) + 4 shear layers 48x48x4 (300 steps), "
                     "harvested every 20 steps from step 40"}
     return w, info
+
+
+def train_learned2d(max_rows=2_000_000, c2_size=1024, c2_steps=2000, sod_steps=1000):
+    """The 2D network (5x5 stencils) on rows harvested from plain FP64 cart2d
+    runs with SSP-RK3: the Sod tube (1000 x 4, dt 2e-4, `sod_steps` steps;
+    seed 1 Sod's states, seed 2 drawn ones), harvested every 20 steps, every
+    4th smooth cell; and the ideal expansion (c2_size^2, CFL 0.5, `c2_steps`
+    steps; seed 1 the centred 50 atm blob, seed 2 a drawn position and
+    ratio), harvested every 100 steps from step 100, every 64th smooth cell;
+    at most max_rows rows."""
+    from fiesta_amd import workloads as W
+    from fiesta_amd.config import FST_VAR
+    from fiesta_amd.runtime import Module, Trainer
+
+    t0 = time.perf_counter()
+    tr = Trainer(seed=1, ndim=2)
+    plan = []
+    for seed in (1, 2):
+        cfg = W.sod_config(1000, 4, dt=2e-4)
+        cfg.rk_order = 3
+        plan.append((cfg, W.sod_state(cfg, seed), sod_steps, 20, 20, 4))
+    for seed in (1, 2):
+        cfg = W.expansion2d_config(c2_size)
+        cfg.rk_order = 3
+        plan.append((cfg, W.expansion2d_state(cfg, seed=seed), c2_steps, 100, 100, 64))
+    per_run = []
+    for cfg, var, steps, every, first, stride in plan:
+        m = Module(cfg)
+        m.upload(FST_VAR, var)
+        m.pre_sim()
+        n0 = tr.rows()
+        for done in range(every, steps + 1, every):
+            m.step(every)
+            # only states whose plain C_hat is bounded: the 2D expansion's
+            # plain C-equation diverges for a while where the hot gas is
+            # nearly pure (its wave speed misreads the species constants)
+            if done >= first and tr.rows() < max_rows and m.maxima()[1] < 1e6:
+                tr.harvest(m, stride=stride, max_rows=max_rows - tr.rows())
+        m.close()
+        per_run.append(int(tr.rows() - n0))
+    rows = tr.rows()
+    t1 = time.perf_counter()
+    l1 = tr.train(epochs=12, batch=1024, lr=1e-3, seed=40)
+    l2 = tr.train(epochs=6, batch=1024, lr=3e-4, seed=20)
+    t2 = time.perf_counter()
+    w = tr.weights()
+    tr.close()
+    info = {"rows": int(rows), "rows_per_run": per_run, "harvest_s": round(t1 - t0, 2),
+            "train_s": round(t2 - t1, 2), "epochs": 18, "batch": 1024,
+            "loss_first_epoch": float("%.4g" % l1[0]), "loss_last_epoch": float("%.4g" % l2[-1]),
+            "data": "plain FP64 cart2d SSP-RK3 runs: Sod 1000x4 seeds 1-2 (%d steps), expansion %d^2 seeds 1-2 "
+                    "(%d steps)" % (sod_steps, c2_size, c2_steps)}
+    return w, info
+
+
+def _blast_rays(cfg, var, blobs, rmax=0.3):
+    """Pressure and density along four rays (+-x, +-y) from each blob centre
+    (nearest cells, one per cell width) and the shock position on each ray:
+    the steepest pressure drop beyond the blob radius, in cells."""
+    from fiesta_amd.config import Dims
+    d = Dims(cfg)
+    g = d.ng
+    ns = cfg.ns
+    v = var[:, 0]
+    rho = v[3:3 + ns].sum(axis=0)
+    Cp = 0.0
+    Cv = 0.0
+    for q in range(ns):
+        Rs = cfg.R / cfg.M[q]
+        Cp = Cp + v[3 + q] / rho * (cfg.gamma[q] * Rs / (cfg.gamma[q] - 1))
+        Cv = Cv + v[3 + q] / rho * (Rs / (cfg.gamma[q] - 1))
+    p = (Cp / Cv - 1) * (v[2] - 0.5 * (v[0] ** 2 + v[1] ** 2) / rho)
+    n = d.nci
+    out = []
+    for bx, by, _ in blobs:
+        ci, cj = int(bx * n), int(by * n)
+        for dx, dy in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+            lim = min(int(rmax * n), (n - 1 - ci) if dx > 0 else ci if dx < 0 else n,
+                      (n - 1 - cj) if dy > 0 else cj if dy < 0 else n)
+            r = np.arange(lim)
+            ii, jj = g + ci + dx * r, g + cj + dy * r
+            pr, rr = p[jj, ii], rho[jj, ii]
+            dp = np.diff(pr)
+            start = int(0.05 * n)
+            k = start + int(np.argmin(dp[start:])) if dp.size > start + 1 else 0
+            out.append((k, pr, rr))
+    return out
+
+
+def multiblast_leg(args, weights, train_info):
+    """Configuration 4: the 2D multi-blast (n^2 cells, 8 blobs, 3 species,
+    viscous, SSP-RK3) with the plain 2D C-equation, the learned C-indicator
+    (FP64 cart2d sequence) and no C-equation; the learned and C-equation-free
+    runs judged by the plain run on density (field and ray profiles) and
+    shock positions, and timed (device ms per step)."""
+    from fiesta_amd import workloads as W
+    from fiesta_amd.config import FST_VAR, Dims
+    from fiesta_amd.runtime import Module
+
+    n, steps = args.multiblast_size, args.multiblast_steps
+    blobs = W.multiblast2d_blobs(4, 8)
+    res = {}
+    for form in ("plain", "learned", "no_ceq"):
+        cfg = W.multiblast2d_config(n, ceq=form != "no_ceq", learned_c=form == "learned")
+        d = Dims(cfg)
+        m = Module(cfg)
+        m.upload(FST_VAR, W.multiblast2d_state(cfg))
+        if form == "learned":
+            m.set_mlp_weights(weights)
+        m.pre_sim()
+        m.step(steps)
+        ms = m.last_step_ms()
+        e = {"ms_per_step": round(ms / steps, 4), "ms_per_stage": round(ms / steps / 3, 4),
+             "value": round(d.ncells * 3 * steps / (ms * 1e-3) / 1e9, 4), "nvt": d.nvt}
+        if form == "learned":
+            act, fb, mch = m.learned_stats()
+            e["active_cells_last_step"] = int(act)
+            e["fallback_cells_last_step"] = int(fb)
+        res[form] = (cfg, m.download(FST_VAR), e)
+        m.close()
+        log("multiblast %s: %.3f ms/step" % (form, ms / steps))
+    cfg_p, vp, ep = res["plain"]
+    dp = Dims(cfg_p)
+    sl = dp.interior()[1:]
+    rho_p = vp[3:6][(slice(None),) + sl].sum(axis=0)
+    mom_p = np.hypot(vp[0][sl], vp[1][sl])
+    rays_p = _blast_rays(cfg_p, vp, blobs)
+    out = {"workload": "2D multi-blast %d^2, 8 blobs (seed 4, p ratio U(10,100)), air / hot gas / SF6, viscous, "
+                       "SSP-RK3, %d steps, dt %.4g" % (n, steps, cfg_p.dt),
+           "plain_ceq": ep, "training": train_info}
+    for form in ("learned", "no_ceq"):
+        cfg, v, e = res[form]
+        rho = v[3:6][(slice(None),) + sl].sum(axis=0)
+        mom = np.hypot(v[0][sl], v[1][sl])
+        rays = _blast_rays(cfg, v, blobs)
+        dk = np.array([abs(a[0] - b[0]) for a, b in zip(rays, rays_p)])
+        prof = np.concatenate([a[2] - b[2] for a, b in zip(rays, rays_p)])
+        prof_ref = np.concatenate([b[2] for b in rays_p])
+        e.update({"density_rel_l2_vs_plain": float("%.4g" % (np.linalg.norm(rho - rho_p) / np.linalg.norm(rho_p))),
+                  "density_linf_rel_vs_plain": float("%.4g" % (np.abs(rho - rho_p).max() / np.abs(rho_p).max())),
+                  "momentum_rel_l2_vs_plain": float("%.4g" % (np.linalg.norm(mom - mom_p) / np.linalg.norm(mom_p))),
+                  "ray_density_profile_rel_l2_vs_plain": float("%.4g" % (np.linalg.norm(prof) /
+                                                                         np.linalg.norm(prof_ref))),
+                  "shock_position_cells_vs_plain": {"mean_abs": round(float(dk.mean()), 3), "max_abs": int(dk.max()),
+                                                    "rays": int(dk.size)},
+                  "speedup_vs_plain": round(e["value"] / ep["value"], 3)})
+        out[form] = e
+    l, z = out["learned"], out["no_ceq"]
+    out["learned_error_over_no_ceq_error"] = {
+        k: round(l[k] / z[k], 4) if z[k] else None
+        for k in ("density_rel_l2_vs_plain", "momentum_rel_l2_vs_plain", "ray_density_profile_rel_l2_vs_plain")}
+    return out
 
 
 def gpu_leg(args, form="fp64", var=None, keep_state=False, weights=None):
@@ -561,6 +718,11 @@ def main():
         out["extras"]["gen3d"] = gen3d_leg(args)
         log("extras: 2D module (expansion)")
         out["extras"]["cart2d_expansion"] = expansion2d_leg(args)
+    if args.multiblast or args.extras:
+        log("configuration 4: 2D learned C-indicator on the multi-blast")
+        w2, info2 = train_learned2d()
+        log("trained 2D: %s" % info2)
+        out.setdefault("extras", {})["multiblast_c4"] = multiblast_leg(args, w2, info2)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(args, args.size)
         out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 5)

@@ -150,13 +150,21 @@ def sod_config(nci=1000, ncj=4, dt=2e-4, ceq=True):
                        dt=dt)
 
 
-def sod_state(cfg):
+def sod_state(cfg, seed=1):
+    """Seed 1: Sod's states (rho, p) = (1, 1) | (0.125, 0.1); another seed:
+    a shock tube with left and right states drawn from it (rho_l, p_l in
+    [0.5, 2], rho_r, p_r in [0.05, 0.3])."""
     d = Dims(cfg)
     x, y, z = _cells(d)
     sh = (1, d.ncj, d.nci)
     left = np.broadcast_to(x < 0.5, sh)
-    rho = np.where(left, 1.0, 0.125)
-    p = np.where(left, 1.0, 0.1)
+    rl, pl, rr, pr_ = 1.0, 1.0, 0.125, 0.1
+    if seed != 1:
+        rng = np.random.default_rng(seed)
+        rl, pl = rng.uniform(0.5, 2.0, 2)
+        rr, pr_ = rng.uniform(0.05, 0.3, 2)
+    rho = np.where(left, rl, rr)
+    p = np.where(left, pl, pr_)
     u = [np.zeros(sh), np.zeros(sh)]
     return fill_conserved(cfg, [rho], u, p, [np.zeros(sh)] * 5 if cfg.ceq else None)
 
@@ -172,12 +180,20 @@ def expansion2d_config(n=1024, ceq=True, visc=True, ns=2):
     return cfg
 
 
-def expansion2d_state(cfg, T0=300.0, r0=0.05):
+def expansion2d_state(cfg, T0=300.0, r0=0.05, seed=1):
+    """Seed 1: the blob at the centre, 50 atm; another seed: the blob at a
+    position drawn from it in [0.35, 0.65]^2 with a pressure ratio in
+    [10, 100]."""
     d = Dims(cfg)
     x, y, _ = _cells(d)
-    r2 = (x - 0.5) ** 2 + (y - 0.5) ** 2
+    cx, cy, pr = 0.5, 0.5, 50.0
+    if seed != 1:
+        rng = np.random.default_rng(seed)
+        cx, cy = rng.uniform(0.35, 0.65, 2)
+        pr = rng.uniform(10.0, 100.0)
+    r2 = (x - cx) ** 2 + (y - cy) ** 2
     yh = np.exp(-r2 / r0 ** 2)[None, :, :] * np.ones((1, 1, 1))
-    p = 101325.0 * (1.0 + 49.0 * yh)
+    p = 101325.0 * (1.0 + (pr - 1.0) * yh)
     Rm = (1 - yh) * cfg.R / cfg.M[0] + yh * cfg.R / cfg.M[1]
     rho = p / (Rm * T0)
     rho_s = [rho * (1 - yh), rho * yh]
@@ -186,17 +202,23 @@ def expansion2d_state(cfg, T0=300.0, r0=0.05):
                           [np.zeros(sh)] * 5 if cfg.ceq else None)
 
 
-def multiblast2d_config(n=2048, ceq=True, learned_c=False, visc=True, rk=3, seed=4, nblobs=8, cfl=0.5):
+def multiblast2d_config(n=2048, ceq=True, learned_c=False, visc=True, rk=3, seed=4, nblobs=8, cfl=0.5, kap=10.0,
+                        eps=5.0, alpha=20.0):
     """2D multi-blast: 1 m box, n^2 cells, three species (air, hot gas, SF6):
     `nblobs` Gaussian high-pressure blobs of hot gas (radius 0.05, pressure
     ratio ~ U(10, 100)) at seeded uniform positions in [0.1, 0.9]^2, in air
     at 1 atm below a heavy SF6 layer (y > 0.7, tanh interface of width
-    0.01); outflow on all sides, viscous, the C-equation as in the 2D Sod and
-    expansion decks (kappa 10, eps 1, alpha 2), SSP-RK3 (rk=3) and dt from
-    CFL `cfl` at t = 0."""
+    0.01); every cell keeps at least 2 % air (see multiblast2d_state);
+    outflow on all sides, viscous, SSP-RK3 (rk=3) and dt from CFL `cfl` at
+    t = 0.  The C-equation takes the 3D Rayleigh-Taylor deck's constants
+    (kappa 10, eps 5, alpha 20, samples/rt3d/fiesta.lua:42-45): the plain 2D
+    method's wave speed includes |rho u| in place of |u| (src/ceq.hpp:72-73),
+    which reaches ~1e4 m/s in the shocked SF6, and with eps = 1 (the 2D Sod
+    deck) its explicit relaxation, rate maxS dt / (eps |dx|) ~ 2.8 at CFL
+    0.4, leaves SSP-RK3's stability interval and diverges."""
     mix = _mix(3, [AIR, HOT, SF6])
     cfg = make_config(ndim=2, ns=3, nci=n, ncj=n, dx=1.0 / n, dy=1.0 / n, R=RU, visc=visc, ceq=ceq,
-                      learned_c=learned_c, kap=10.0, eps=1.0, alpha=2.0, bc=("outflow",) * 6, dt=1.0, rk=rk,
+                      learned_c=learned_c, kap=kap, eps=eps, alpha=alpha, bc=("outflow",) * 6, dt=1.0, rk=rk,
                       **mix)
     cfg.dt = cfl_dt(cfg, multiblast2d_state(cfg, seed, nblobs), cfl)
     return cfg
@@ -210,7 +232,15 @@ def multiblast2d_blobs(seed=4, nblobs=8):
     return [(float(xy[b, 0]), float(xy[b, 1]), float(pr[b])) for b in range(nblobs)]
 
 
-def multiblast2d_state(cfg, seed=4, nblobs=8, T0=300.0, r0=0.05, p0=101325.0):
+def multiblast2d_state(cfg, seed=4, nblobs=8, T0=300.0, r0=0.05, p0=101325.0, air_min=0.02):
+    """The blobs' hot gas and the SF6 layer share the cells with at least
+    `air_min` air.  The plain 2D C-equation needs it: its wave speed
+    (maxWaveSpeed2D, src/ceq.hpp:43-78) reads the species constants with
+    stride 2, so species 1 enters Cv as gamma_1 / (mu_0 - 1) < 0 and Cv
+    passes zero where species 1 exceeds about 99.8 %; there the wave speed,
+    and with it the C-equation's explicit relaxation rate, becomes
+    unbounded (6.7e5 m/s at 2048^2 with pure hot gas at the blob centres)
+    and the plain run diverges within 25 steps."""
     d = Dims(cfg)
     x, y, _ = _cells(d)
     x = np.broadcast_to(x, (1, d.ncj, d.nci))
@@ -222,7 +252,8 @@ def multiblast2d_state(cfg, seed=4, nblobs=8, T0=300.0, r0=0.05, p0=101325.0):
         yhot = np.maximum(yhot, g)
         excess = excess + (pr - 1.0) * g
     p = p0 * (1.0 + excess)
-    ysf6 = (1.0 - yhot) * 0.5 * (1.0 + np.tanh((y - 0.7) / 0.01))
+    yhot = (1.0 - air_min) * yhot
+    ysf6 = (1.0 - air_min - yhot) * 0.5 * (1.0 + np.tanh((y - 0.7) / 0.01))
     yair = 1.0 - yhot - ysf6
     Ys = [yair, yhot, ysf6]
     Rm = sum(Ys[s] * cfg.R / cfg.M[s] for s in range(3))

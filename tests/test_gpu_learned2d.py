@@ -208,3 +208,74 @@ def test_harvest2d_and_training_lowers_loss():
     print("rows %d, dataset loss before %.4g after %.4g (target variance %.4g)" % (n, before, after, T.var()))
     assert after < 0.3 * before and after < 0.5 * T.var(), (before, after, T.var())
     tr.close()
+
+
+def _train_small():
+    tr = Trainer(seed=4, ndim=2)
+    for seed in (1, 2):
+        cfg = W.sod_config(400, 4, dt=5e-4)
+        cfg.rk_order = 3
+        m = Module(cfg)
+        m.upload(FST_VAR, W.sod_state(cfg, seed))
+        m.pre_sim()
+        for _ in range(20):
+            m.step(10)
+            tr.harvest(m, stride=4)
+        m.close()
+    for seed in (1, 2):
+        cfg = W.expansion2d_config(160)
+        cfg.rk_order = 3
+        m = Module(cfg)
+        m.upload(FST_VAR, W.expansion2d_state(cfg, seed=seed))
+        m.pre_sim()
+        for _ in range(12):
+            m.step(20)
+            if m.maxima()[1] < 1e6:  # states whose plain C_hat is bounded
+                tr.harvest(m, stride=8)
+        m.close()
+    tr.train(24, batch=512, lr=1e-3, seed=5)
+    tr.train(8, batch=512, lr=3e-4, seed=6)
+    w = tr.weights()
+    rows = tr.rows()
+    tr.close()
+    return w, rows
+
+
+def test_learned2d_gain_on_multiblast():
+    """Configuration 4 in small: trained on 2D Sod and expansion runs, the
+    learned run on an unseen multi-blast (4 blobs, 3 species, 256^2) is closer
+    to the plain C-equation run than a run without the C-equation terms, and
+    keeps the blast fronts where the plain run has them.  CFL 0.25: the plain
+    2D C-equation's Laplacian term (divided by d, src/ceq.hpp:184-191) has an
+    explicit eigenvalue ~113 maxS dt, which its inflated wave speed pushes
+    past SSP-RK3's stability interval on coarse grids at CFL 0.5."""
+    print("case: 2D learned gain, multiblast 256^2 4 blobs CFL 0.25 240 steps", flush=True)
+    w, rows = _train_small()
+    n, steps, nb = 256, 240, 4
+    out = {}
+    for form in ("plain", "learned", "no_ceq"):
+        cfg = W.multiblast2d_config(n, ceq=form != "no_ceq", learned_c=form == "learned", nblobs=nb, cfl=0.25)
+        m = Module(cfg)
+        m.upload(FST_VAR, W.multiblast2d_state(cfg, 4, nb))
+        if form == "learned":
+            m.set_mlp_weights(w)
+        m.pre_sim()
+        m.step(steps)
+        v = m.download(FST_VAR)
+        stats = m.learned_stats() if form == "learned" else None
+        m.close()
+        d = Dims(cfg)
+        sl = d.interior()[1:]
+        rho = v[3:6][(slice(None),) + sl].sum(axis=0)
+        mom = np.stack([v[0][sl], v[1][sl]])
+        out[form] = (rho, mom, stats)
+        assert np.isfinite(v).all(), form
+    rp, mp, _ = out["plain"]
+    err = {f: (_rel(out[f][0], rp), _rel(out[f][1], mp)) for f in ("learned", "no_ceq")}
+    act, nfb, mch = out["learned"][2]
+    print("rows %d; density / momentum rel L2 vs plain: learned %.3g / %.3g, no C-equation %.3g / %.3g; "
+          "active %d fallback %d" % (rows, err["learned"][0], err["learned"][1], err["no_ceq"][0],
+                                     err["no_ceq"][1], act, nfb), flush=True)
+    assert err["no_ceq"][1] > 0  # the C-equation terms act on this flow
+    assert err["learned"][1] < err["no_ceq"][1]
+    assert err["learned"][0] < err["no_ceq"][0]
