@@ -359,7 +359,7 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
     (void)fst_destroy(ctx);
     return FST_ERR_NOMEM;
   }
-  ctx->fused = fused_supported(c) ? 1 : 0;
+  ctx->fused = fused_supported(c) || fused2d_supported(c) ? 1 : 0;
   ctx->b.prof = &ctx->prof;
   *out = ctx;
   return FST_OK;
@@ -462,8 +462,10 @@ int fst_post_step(fst_ctx *ctx) {
 
 int fst_set_fused(fst_ctx *ctx, int fused) {
   if (!ctx) return FST_ERR_ARG;
-  if (fused && !fused_supported(ctx->cf))
-    return fail(ctx, FST_ERR_ARG, "fused path supports 3D cartesian FP64 with ceq, visc and buoyancy flags only");
+  if (fused && !fused_supported(ctx->cf) && !fused2d_supported(ctx->cf))
+    return fail(ctx, FST_ERR_ARG,
+                "the fused path supports cartesian WENO5 configurations with ns <= 3 (2D: FP64) and every array "
+                "below 2^29 elements (3D)");
   if (!fused && ctx->cf.learned_c && ctx->g.ndim == 3) return learned_only(ctx, "fst_set_fused(0)");
   if (!fused && ctx->cf.precision != FST_FP64) return fp32_only(ctx, "fst_set_fused(0)");
   ctx->fused = fused ? 1 : 0;
@@ -474,7 +476,9 @@ int fst_step(fst_ctx *ctx, int nsteps) {
   if (!ctx || nsteps < 0) return FST_ERR_ARG;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (ctx->fused) {
-    int rc = fused_steps(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->fb, &ctx->lb, nsteps, ctx->stream, &ctx->err);
+    int rc = ctx->g.ndim == 2
+                 ? fused2d_steps(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->fb, &ctx->lb, nsteps, ctx->stream, &ctx->err)
+                 : fused_steps(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->fb, &ctx->lb, nsteps, ctx->stream, &ctx->err);
     if (rc) return rc;
   } else {
     ctx->fb.valid = false;

@@ -20,6 +20,12 @@ struct FusedBufs {
 struct LearnedBufs;
 
 bool fused_supported(const fst_config &cf);
+// The fused cart2d stage (fst_fused2d.hip): cartesian 2D, WENO5, FP64,
+// ns <= 3; plain, learned or no C-equation.  fb.stage / fb.stage2 hold the
+// stage states, fb.sc three maxima records used in rotation (fb.cur).
+bool fused2d_supported(const fst_config &cf);
+int fused2d_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb, LearnedBufs *lb,
+                  int nsteps, hipStream_t s, std::string *err);
 // gen3d's advection and metric pressure gradient on the fused kernel (rho, p,
 // contravariant velocity in the 5 planes of P, metric terms MT);
 // gen_adv_supported says whether the configuration fits.
