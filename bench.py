@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--extras", action="store_true",
                     help="also time postStep's noise filter, the device-side output pack (fst_pack), the gen3d module and the 2D expansion on the "
                          "fp64 workload")
+    ap.add_argument("--cart2d", action="store_true",
+                    help="the 2D module on the expansion (fused stage and kernel sequence); also part of --extras")
     ap.add_argument("--multiblast", action="store_true",
                     help="the 2D learned C-indicator on the multi-blast (trained on 2D Sod and expansion runs), "
                          "against the plain 2D C-equation; also part of --extras")
@@ -399,33 +401,44 @@ def expansion2d_leg(args):
     """The 2D module (cart2d: BCs once per step, then per stage EOS, WENO5,
     pressure gradient, viscous terms and the C-equation; src/cart2d.cpp:
     136-237) on the 2D ideal expansion, 1024 x 1024 cells, 2 species,
-    viscous, C-equation, SSP-RK3 (3 stages per step), dt from CFL 0.5; the
-    reference-sequence kernels.  Device ms per step from hipEvents over 20
-    steps after 2 warm-up steps."""
+    viscous, C-equation, SSP-RK3 (3 stages per step), dt from CFL 0.5: the
+    fused cart2d stage kernel and the reference-sequence kernels.  Device ms
+    per step from hipEvents over 20 steps after 2 warm-up steps."""
     from fiesta_amd import workloads as W
     from fiesta_amd.config import FST_VAR, Dims
+
     from fiesta_amd.runtime import Module
 
     cfg = W.expansion2d_config(1024, ceq=True, visc=True, ns=2)
     cfg.rk_order = 3
     d = Dims(cfg)
-    m = Module(cfg)
-    m.upload(FST_VAR, W.expansion2d_state(cfg))
-    m.pre_sim()
-    m.step(2)
-    m.sync()
-    steps = 20
-    m.step(steps)
-    ms_step = m.last_step_ms() / steps
-    m.close()
-    stage_ms = ms_step / 3
     balg = 64 * d.nvt // 3  # SSP-RK3: (2 + 3 + 3) / 3 x 8 nvt bytes per cell-stage
-    return {"workload": "cart2d expansion 1024x1024, 2 species, viscous, C-equation, SSP-RK3, FP64",
-            "ms_per_stage": round(stage_ms, 4), "value": round(d.ncells / (stage_ms * 1e-3) / 1e9, 4),
-            "unit": "Gcell-stage/s",
-            "roofline": {"bound": "hbm", "algorithmic_bytes_per_cell_stage": balg,
-                         "achieved_GBps": round(d.ncells * balg / (stage_ms * 1e-3) / 1e9, 1),
-                         "frac": round(d.ncells * balg / (stage_ms * 1e-3) / 8.0e12, 4)}}
+    out = {"workload": "cart2d expansion 1024x1024, 2 species, viscous, C-equation, SSP-RK3, FP64",
+           "unit": "Gcell-stage/s", "algorithmic_bytes_per_cell_stage": balg}
+    states = {}
+    for path in ("fused", "reference_sequence"):
+        m = Module(cfg)
+        m.upload(FST_VAR, W.expansion2d_state(cfg))
+        m.pre_sim()
+        m.set_fused(path == "fused")
+        m.step(2)
+        m.sync()
+        steps = 20
+        m.step(steps)
+        ms_step = m.last_step_ms() / steps
+        states[path] = m.download(FST_VAR)
+        m.close()
+        stage_ms = ms_step / 3
+        ach = d.ncells * balg / (stage_ms * 1e-3) / 1e9
+        out[path] = {"ms_per_stage": round(stage_ms, 4), "value": round(d.ncells / (stage_ms * 1e-3) / 1e9, 4),
+                     "roofline": {"bound": "hbm", "achieved_GBps": round(ach, 1),
+                                  "frac": round(ach * 1e9 / 8.0e12, 4)}}
+    out["fused_equals_reference_sequence"] = bool(np.array_equal(states["fused"], states["reference_sequence"]))
+    out["speedup_fused"] = round(out["fused"]["value"] / out["reference_sequence"]["value"], 3)
+    out["value"] = out["fused"]["value"]
+    out["ms_per_stage"] = out["fused"]["ms_per_stage"]
+    out["roofline"] = out["fused"]["roofline"]
+    return out
 
 
 def cpu_threads():
@@ -716,8 +729,9 @@ def main():
         out["extras"] = extras_leg(args, var)
         log("extras: generalized-grid module (gen3d)")
         out["extras"]["gen3d"] = gen3d_leg(args)
-        log("extras: 2D module (expansion)")
-        out["extras"]["cart2d_expansion"] = expansion2d_leg(args)
+    if args.extras or args.cart2d:
+        log("2D module (expansion)")
+        out.setdefault("extras", {})["cart2d_expansion"] = expansion2d_leg(args)
     if args.multiblast or args.extras:
         log("configuration 4: 2D learned C-indicator on the multi-blast")
         w2, info2 = train_learned2d()

@@ -187,7 +187,7 @@ int compute_seq(fst_ctx *ctx, bool fields = false) {
     const Bufs &b = ctx->b;
     launch_eos_vel(ctx->g, ctx->ph, b, ctx->stream);
     if (fields) {
-      const int rc = learned_fields2d(ctx->g, ctx->cf, ctx->ph, b.var, b.rho, b.p, b.vel, ctx->lb, ctx->stream);
+      const int rc = learned_fields2d(ctx->g, ctx->cf, ctx->ph, b.var, b.rho, ctx->lb, ctx->stream);
       if (rc) return rc;
     }
     Ceq2dFields lf;
@@ -684,9 +684,7 @@ int fst_learned_compute(fst_ctx *ctx) {
   int rc = FST_OK;
   launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
   if (ctx->g.ndim == 2) {
-    const Bufs &b = ctx->b;
-    launch_eos_vel(ctx->g, ctx->ph, b, ctx->stream);
-    rc = learned_fields2d(ctx->g, ctx->cf, ctx->ph, b.var, b.rho, b.p, b.vel, ctx->lb, ctx->stream);
+    rc = learned_fields2d(ctx->g, ctx->cf, ctx->ph, ctx->b.var, ctx->b.rho, ctx->lb, ctx->stream);
   } else {
     const size_t np = 6 + (size_t)ctx->cf.ns;
     if (!ctx->lprim) HIPCHK(ctx, hipMalloc((void **)&ctx->lprim, np * (size_t)ctx->g.N * sizeof(double)));
@@ -792,14 +790,14 @@ int fst_mlp_harvest(fst_mlp *m, fst_ctx *ctx, uint32_t stride, int64_t max_rows,
   if (c.ndim == 2) {
     if (c.grid != 0 || !c.ceq || c.learned_c)
       return mfail(m, FST_ERR_ARG, "fst_mlp_harvest: needs a 2D cartesian context with the C-equation");
-    // cart2d's state as its compute() sees it: ghosts filled in preStep, then primitives
+    // cart2d's state as its compute() sees it: ghosts filled in preStep (the
+    // feature pass computes the primitives and writes rho into the context's
+    // rho plane, which compute() rewrites with the same values)
     launch_apply_bcs(ctx->g, c, ctx->b.var, ctx->stream);
-    launch_eos_vel(ctx->g, ctx->ph, ctx->b, ctx->stream);
+    ctx->fb.valid = false;
     int rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? FST_OK : FST_ERR_HIP;
     long long a = 0;
-    if (!rc)
-      rc = trainer_harvest2d(m->t, ctx->g, c, ctx->ph, ctx->b.var, ctx->b.rho, ctx->b.p, ctx->b.vel, stride,
-                             max_rows, &a);
+    if (!rc) rc = trainer_harvest2d(m->t, ctx->g, c, ctx->ph, ctx->b.var, ctx->b.rho, stride, max_rows, &a);
     if (added) *added = a;
     return rc ? mfail(m, rc, "fst_mlp_harvest: failed") : FST_OK;
   }

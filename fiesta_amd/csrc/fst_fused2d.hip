@@ -46,8 +46,9 @@ constexpr int RW = F2X + 2 * F2H, RH = F2Y + 2 * F2H;
 constexpr int RN = RW * RH;                        // staged cells
 constexpr int NXF = (F2X + 1) * F2Y;               // x faces: i0-1 .. i0+63
 constexpr int NYF = F2X * (F2Y + 1);               // y faces: j0-1 .. j0+7
-constexpr int WORK = 4 * NYF;                      // doubles of the work area
-static_assert(RN + NXF + NYF <= WORK && 3 * NYF <= WORK && 4 * NXF <= WORK, "work area");
+constexpr int ADV = RN + NXF + NYF;                // one advection buffer set: q, x and y fluxes
+constexpr int WORK = 2 * ADV;                      // doubles of the work area
+static_assert(3 * NXF + 3 * NYF <= WORK && 4 * NYF <= WORK && 4 * NXF <= WORK, "work area");
 constexpr int NT2 = F2X * F2Y;                     // threads per block
 
 enum { UPD_RK2_1 = 0, UPD_ADD = 1, UPD_RK3_1 = 2, UPD_BLEND = 3 };
@@ -154,27 +155,40 @@ __global__ void __launch_bounds__(NT2) k_stage2d(K2 k, const double *__restrict_
   const long long c = i + (long long)k.ngi * j;
   if (hb == 0 && tid < 4) reinterpret_cast<unsigned long long *>(scClr)[tid] = 0ull;
 
-  // ---- primitives of the region (calculateRhoPT2D, computeVelocity2D)
-  for (int r = tid; r < RN; r += NT2) {
-    const int gi = i0 - F2H + r % RW, gj = j0 - F2H + r / RW;
-    if (gi < k.ngi && gj < k.ngj) {
-      const long long q = gi + (long long)k.ngi * gj;
-      double rr = 0.0, ys[NS];
+  // ---- primitives of the region (calculateRhoPT2D, computeVelocity2D); the
+  // thread's two staged cells (r = tid, tid + NT2) load everything first
+  static_assert(RN <= 2 * NT2, "thread slots");
+  {
+    double w[2][3 + NS];
+    bool ok[2];
+    int rr_[2];
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        ys[s] = U[q + N * (3 + s)];
-        rr = rr + ys[s];
-      }
+    for (int h = 0; h < 2; ++h) {
+      const int r = tid + h * NT2;
+      rr_[h] = r;
+      const int gi = i0 - F2H + r % RW, gj = j0 - F2H + r / RW;
+      ok[h] = r < RN && gi < k.ngi && gj < k.ngj;
+      const long long q = gi + (long long)k.ngi * gj;
+#pragma unroll
+      for (int v = 0; v < 3 + NS; ++v) w[h][v] = ok[h] ? U[q + N * v] : 1.0;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (!ok[h]) continue;
+      const int r = rr_[h];
+      double rr = 0.0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) rr = rr + w[h][3 + s];
       double Cp = 0, Cv = 0;
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
-        const double Y = ys[s] / rr;
+        const double Y = w[h][3 + s] / rr;
         Cp = Cp + Y * k.cpf[s];
         Cv = Cv + Y * k.cvf[s];
       }
       const double gamma = Cp / Cv;
-      const double m0 = U[q], m1 = U[q + N];
-      const double pp = (gamma - 1) * (U[q + 2 * N] - (0.5 / rr) * (m0 * m0 + m1 * m1));
+      const double m0 = w[h][0], m1 = w[h][1];
+      const double pp = (gamma - 1) * (w[h][2] - (0.5 / rr) * (m0 * m0 + m1 * m1));
       sR[r] = rr;
       sP[r] = pp;
       sT[r] = pp / ((Cp - Cv) * rr);
@@ -228,12 +242,27 @@ __global__ void __launch_bounds__(NT2) k_stage2d(K2 k, const double *__restrict_
   const bool oka = gia < k.ngi && gja < k.ngj, okb = rb < RN && gib < k.ngi && gjb < k.ngj;
   const long long qia = gia + (long long)k.ngi * gja, qib = gib + (long long)k.ngi * gjb;
   double qa = oka ? U[qia] : 0.0, qb = okb ? U[qib] : 0.0;
-#pragma unroll
-  for (int v = 0; v < NV; ++v) {
+  // two buffer sets (q, fluxes) alternate between variables: per variable
+  // one barrier separates staging q(v) from its faces, and the faces of v from
+  // the differences of v (read after the next barrier, together with the
+  // staging of v + 1)
+  auto stage_q = [&](int v, double *sq) {
     const double isE = (double)(v == 2);
-    if (oka) sQ[ra] = qa + isE * sP[ra];
-    if (okb) sQ[rb] = qb + isE * sP[rb];
+    if (oka) sq[ra] = qa + isE * sP[ra];
+    if (okb) sq[rb] = qb + isE * sP[rb];
+  };
+  stage_q(0, sW);
+#pragma unroll
+  for (int v = 0; v <= NV; ++v) {
     __syncthreads();
+    double *const sQ = sW + (v & 1) * ADV, *const sFX = sQ + RN, *const sFY = sQ + RN + NXF;
+    if (v > 0) {  // differences of variable v - 1
+      const double *const pX = sW + ((v - 1) & 1) * ADV + RN, *const pY = pX + NXF;
+      const int xr = ci + 1 + (F2X + 1) * cj, xl = ci + (F2X + 1) * cj;
+      const int yt = ci + F2X * (cj + 1), yb = ci + F2X * cj;
+      dv[v - 1] = -((pX[xr] - pX[xl]) + (pY[yt] - pY[yb]));
+    }
+    if (v == NV) break;
     if (v + 1 < NV) {
       if (oka) qa = U[qia + N * (v + 1)];
       if (okb) qb = U[qib + N * (v + 1)];
@@ -259,12 +288,7 @@ __global__ void __launch_bounds__(NT2) k_stage2d(K2 k, const double *__restrict_
         sFY[tid + q * NT2 - NXF] = cdiv_rn(fvq[q] * weno5_f64c(f1, f2, f3, f4, f5), k.dy, k.rdy);
       }
     }
-    __syncthreads();
-    {
-      const int xr = ci + 1 + (F2X + 1) * cj, xl = ci + (F2X + 1) * cj;
-      const int yt = ci + F2X * (cj + 1), yb = ci + F2X * cj;
-      dv[v] = -((sFX[xr] - sFX[xl]) + (sFY[yt] - sFY[yb]));
-    }
+    if (v + 1 < NV) stage_q(v + 1, sW + ((v + 1) & 1) * ADV);
   }
 
   const int rc = rix(i0, j0, i, j);
@@ -291,18 +315,37 @@ __global__ void __launch_bounds__(NT2) k_stage2d(K2 k, const double *__restrict_
   // applyViscousTerm2dv): x faces, then y faces
   if (k.visc) {
     const double mur = 2.928e-5, mut = 2.928e-5, kk = 0.026;
+    double *const SX = sW, *const SY = sW + 3 * NXF;  // x faces: sx00, sx01, qx; y faces: sy11, sy01, qy
     __syncthreads();  // the last variable's fluxes are read
-    for (int f = tid; f < NXF; f += NT2) {
-      const int fi = i0 - 1 + f % (F2X + 1), fj = j0 + f / (F2X + 1);
-      if (fi >= ie || fj >= je) continue;
-      const int b0 = rix(i0, j0, fi, fj);
-      const double dudx = cdiv_rn(sU[b0 + 1] - sU[b0], k.dx, k.rdx);
-      const double dvdx = cdiv_rn(sV[b0 + 1] - sV[b0], k.dx, k.rdx);
-      const double dudy = cdiv_rn((sU[b0 + RW] + sU[b0 + 1 + RW]) - (sV[b0 - RW] + sU[b0 + 1 - RW]), k.dy4, k.rdy4);
-      const double dvdy = cdiv_rn((sV[b0 + RW] + sV[b0 + 1 + RW]) - (sV[b0 - RW] + sV[b0 + 1 - RW]), k.dy4, k.rdy4);
-      sW[f] = (2.0 / 3.0) * mur * (2.0 * dudx - dvdy);
-      sW[NXF + f] = mur * (dudy + dvdx);
-      sW[2 * NXF + f] = cdiv_rn(-kk * (sT[b0 + 1] - sT[b0]), k.dx, k.rdx);
+    for (int f = tid; f < NXF + NYF; f += NT2) {
+      if (f < NXF) {
+        const int fi = i0 - 1 + f % (F2X + 1), fj = j0 + f / (F2X + 1);
+        if (fi >= ie || fj >= je) continue;
+        const int b0 = rix(i0, j0, fi, fj);
+        const double dudx = cdiv_rn(sU[b0 + 1] - sU[b0], k.dx, k.rdx);
+        const double dvdx = cdiv_rn(sV[b0 + 1] - sV[b0], k.dx, k.rdx);
+        const double dudy =
+            cdiv_rn((sU[b0 + RW] + sU[b0 + 1 + RW]) - (sV[b0 - RW] + sU[b0 + 1 - RW]), k.dy4, k.rdy4);
+        const double dvdy =
+            cdiv_rn((sV[b0 + RW] + sV[b0 + 1 + RW]) - (sV[b0 - RW] + sV[b0 + 1 - RW]), k.dy4, k.rdy4);
+        SX[f] = (2.0 / 3.0) * mur * (2.0 * dudx - dvdy);
+        SX[NXF + f] = mur * (dudy + dvdx);
+        SX[2 * NXF + f] = cdiv_rn(-kk * (sT[b0 + 1] - sT[b0]), k.dx, k.rdx);
+      } else {
+        const int g = f - NXF;
+        const int fi = i0 + g % F2X, fj = j0 - 1 + g / F2X;
+        if (fi >= ie || fj >= je) continue;
+        const int b0 = rix(i0, j0, fi, fj);
+        const double dudx =
+            cdiv_rn((sU[b0 + 1] + sU[b0 + 1 + RW]) - (sU[b0 - 1] + sU[b0 - 1 + RW]), k.dx4, k.rdx4);
+        const double dvdx =
+            cdiv_rn((sV[b0 + 1] + sV[b0 + 1 + RW]) - (sV[b0 - 1] + sV[b0 - 1 + RW]), k.dx4, k.rdx4);
+        const double dudy = cdiv_rn(sU[b0 + RW] - sU[b0], k.dy, k.rdy);
+        const double dvdy = cdiv_rn(sV[b0 + RW] - sV[b0], k.dy, k.rdy);
+        SY[g] = (2.0 / 3.0) * mut * (2.0 * dvdy - dudx);
+        SY[NYF + g] = mut * (dudy + dvdx);
+        SY[2 * NYF + g] = cdiv_rn(-kk * (sT[b0 + RW] - sT[b0]), k.dy, k.rdy);
+      }
     }
     __syncthreads();
     const double ur = (sU[rc + 1] + sU[rc]) / 2.0, ul = (sU[rc - 1] + sU[rc]) / 2.0;
@@ -310,31 +353,15 @@ __global__ void __launch_bounds__(NT2) k_stage2d(K2 k, const double *__restrict_
     const double ut = (sU[rc + RW] + sU[rc]) / 2.0, ub = (sU[rc - RW] + sU[rc]) / 2.0;
     const double vt = (sV[rc + RW] + sV[rc]) / 2.0, vb = (sV[rc - RW] + sV[rc]) / 2.0;
     const int xr = ci + 1 + (F2X + 1) * cj, xl = ci + (F2X + 1) * cj;
-    const double ax = cdiv_rn(sW[xr] - sW[xl], k.dx, k.rdx);
-    const double bx_ = cdiv_rn(sW[NXF + xr] - sW[NXF + xl], k.dx, k.rdx);
-    const double c1x = cdiv_rn(ur * sW[xr] - ul * sW[xl], k.dx, k.rdx) +
-                       cdiv_rn(vr * sW[NXF + xr] - vl * sW[NXF + xl], k.dx, k.rdx);
-    const double c2x = cdiv_rn(sW[2 * NXF + xr] - sW[2 * NXF + xl], k.dx, k.rdx);
-    __syncthreads();
-    for (int f = tid; f < NYF; f += NT2) {
-      const int fi = i0 + f % F2X, fj = j0 - 1 + f / F2X;
-      if (fi >= ie || fj >= je) continue;
-      const int b0 = rix(i0, j0, fi, fj);
-      const double dudx = cdiv_rn((sU[b0 + 1] + sU[b0 + 1 + RW]) - (sU[b0 - 1] + sU[b0 - 1 + RW]), k.dx4, k.rdx4);
-      const double dvdx = cdiv_rn((sV[b0 + 1] + sV[b0 + 1 + RW]) - (sV[b0 - 1] + sV[b0 - 1 + RW]), k.dx4, k.rdx4);
-      const double dudy = cdiv_rn(sU[b0 + RW] - sU[b0], k.dy, k.rdy);
-      const double dvdy = cdiv_rn(sV[b0 + RW] - sV[b0], k.dy, k.rdy);
-      sW[f] = (2.0 / 3.0) * mut * (2.0 * dvdy - dudx);      // sy11
-      sW[NYF + f] = mut * (dudy + dvdx);                     // sy01
-      sW[2 * NYF + f] = cdiv_rn(-kk * (sT[b0 + RW] - sT[b0]), k.dy, k.rdy);
-    }
-    __syncthreads();
     const int yt = ci + F2X * (cj + 1), yb = ci + F2X * cj;
-    const double a = ax + cdiv_rn(sW[NYF + yt] - sW[NYF + yb], k.dy, k.rdy);
-    const double b = bx_ + cdiv_rn(sW[yt] - sW[yb], k.dy, k.rdy);
-    const double c1 = (c1x + cdiv_rn(ut * sW[NYF + yt] - ub * sW[NYF + yb], k.dy, k.rdy)) +
-                      cdiv_rn(vt * sW[yt] - vb * sW[yb], k.dy, k.rdy);
-    const double c2 = c2x + cdiv_rn(sW[2 * NYF + yt] - sW[2 * NYF + yb], k.dy, k.rdy);
+    const double a = cdiv_rn(SX[xr] - SX[xl], k.dx, k.rdx) + cdiv_rn(SY[NYF + yt] - SY[NYF + yb], k.dy, k.rdy);
+    const double b = cdiv_rn(SX[NXF + xr] - SX[NXF + xl], k.dx, k.rdx) + cdiv_rn(SY[yt] - SY[yb], k.dy, k.rdy);
+    const double c1 = cdiv_rn(ur * SX[xr] - ul * SX[xl], k.dx, k.rdx) +
+                      cdiv_rn(vr * SX[NXF + xr] - vl * SX[NXF + xl], k.dx, k.rdx) +
+                      cdiv_rn(ut * SY[NYF + yt] - ub * SY[NYF + yb], k.dy, k.rdy) +
+                      cdiv_rn(vt * SY[yt] - vb * SY[yb], k.dy, k.rdy);
+    const double c2 =
+        cdiv_rn(SX[2 * NXF + xr] - SX[2 * NXF + xl], k.dx, k.rdx) + cdiv_rn(SY[2 * NYF + yt] - SY[2 * NYF + yb], k.dy, k.rdy);
     dv[0] = dv[0] + a;
     dv[1] = dv[1] + b;
     dv[2] = dv[2] + c1 - c2;
@@ -617,8 +644,7 @@ int fused2d_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, F
       if (st == 1) {
         launch_apply_bcs(g, cf, b.var, s);  // cart2d preStep (src/cart2d.cpp:147-149)
         if (cm == 2) {
-          launch_eos_vel(g, ph, b, s);
-          const int rc = learned_fields2d(g, cf, ph, b.var, b.rho, b.p, b.vel, *lb, s);
+          const int rc = learned_fields2d(g, cf, ph, b.var, b.rho, *lb, s);
           if (rc) {
             *err = "learned C-indicator: field evaluation failed";
             return rc;

@@ -295,50 +295,79 @@ __global__ void __launch_bounds__(256) k_active3d(const G3 g, const float *__res
 
 // 2D features (cart2d state, src/cart2d.cpp:151-237), one thread per cell:
 // F (cells with one neighbour each way), the tangent and G1 (two
-// neighbours each way), M = max |F|, from var (energy, species densities),
-// rho, p and vel.  The tangent (-d rho/dy, d rho/dx) and the contact switch
-// follow calculateRhoGrad2D (src/ceq.hpp:80-151): 4th-order gradients of rho
-// and of nrg = E / rho - rho |u|^2 / 2, G1 = |grad rho| h / rho where
+// neighbours each way), M = max |F|, from var.  The tangent (-d rho/dy,
+// d rho/dx) and the contact switch follow calculateRhoGrad2D
+// (src/ceq.hpp:80-151): 4th-order gradients of rho and of
+// nrg = E / rho - rho |u|^2 / 2, G1 = |grad rho| h / rho where
 // (grad rho . grad nrg) |grad rho|^2 < 0.  FP64 arithmetic in a fixed order
 // (divisions by run constants correctly rounded, cdiv_rn), one rounding to
-// FP32 per output.
+// FP32 per output.  A block (64 x 4 cells) stages rho, p, u, v and nrg of its
+// cells and a 2-cell halo in LDS, computed from var as calculateRhoPT2D and
+// computeVelocity2D compute them (src/secondary.hpp:60-107,
+// src/velocity.hpp:85-99), and writes rho of its cells to rho_out (the
+// network's and the residual check's scale).
+constexpr int FW2 = 64 + 4, FH2 = 4 + 4, FN2 = FW2 * FH2;
 __global__ void __launch_bounds__(256) k_features2d(const G3 g, const FeatConst fc, const double *__restrict__ var,
-                                                    const double *__restrict__ rho, const double *__restrict__ p,
-                                                    const double *__restrict__ vel, float *__restrict__ F,
+                                                    double *__restrict__ rho_out, float *__restrict__ F,
                                                     float *__restrict__ M, float *__restrict__ Dir,
                                                     float *__restrict__ G1, Scalars *sc) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  __shared__ double sR[FN2], sP[FN2], sU[FN2], sV[FN2], sE[FN2];
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;
+  const int tid = threadIdx.x + 64 * threadIdx.y;
   const long long N = g.N;
+  for (int r = tid; r < FN2; r += 256) {
+    const int gi = i0 - 2 + r % FW2, gj = j0 - 2 + r / FW2;
+    if (gi >= 0 && gj >= 0 && gi < g.ngi && gj < g.ngj) {
+      const long long q = gi + g.s1 * gj;
+      double rr = 0.0;
+      for (int s = 0; s < fc.ns; ++s) rr = rr + var[q + N * (3 + s)];
+      double Cp = 0, Cv = 0;
+      for (int s = 0; s < fc.ns; ++s) {
+        const double Y = var[q + N * (3 + s)] / rr;
+        Cp = Cp + Y * fc.cpf[s];
+        Cv = Cv + Y * fc.cvf[s];
+      }
+      const double gamma = Cp / Cv;
+      const double m0 = var[q], m1 = var[q + N];
+      const double pp = (gamma - 1) * (var[q + 2 * N] - (0.5 / rr) * (m0 * m0 + m1 * m1));
+      const double u = m0 / rr, v = m1 / rr;
+      sR[r] = rr;
+      sP[r] = pp;
+      sU[r] = u;
+      sV[r] = v;
+      sE[r] = var[q + 2 * N] / rr - 0.5 * rr * (u * u + v * v);
+    }
+  }
+  __syncthreads();
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
   float g1max = 0.f;
   if (i < g.ngi && j < g.ngj) {
     const long long t = (long long)i + g.s1 * j;
+    const int c = (threadIdx.x + 2) + FW2 * (threadIdx.y + 2);
     const bool in1 = i >= 1 && j >= 1 && i < g.ngi - 1 && j < g.ngj - 1;
     const bool in2 = i >= 2 && j >= 2 && i < g.ngi - 2 && j < g.ngj - 2;
     float f0 = 0.f, f1 = 0.f, f2 = 0.f, d0 = 0.f, d1 = 0.f, gg = 0.f;
     if (in1) {
-      auto nrg = [&](long long q) {
-        const double r = rho[q], u = vel[q], v = vel[q + N];
-        return var[q + 2 * N] / r - 0.5 * r * (u * u + v * v);
-      };
-      const long long st[2] = {1, g.s1};
+      const int st[2] = {1, FW2};
       double gr2 = 0.0, gp2 = 0.0, divu = 0.0, dr[2] = {0.0, 0.0}, de[2] = {0.0, 0.0};
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
-        const long long sp = t + st[a], sm = t - st[a];
+        const int sp = c + st[a], sm = c - st[a];
         const double h2 = fc.h2[a], rh2 = fc.rh2[a];
-        const double r1 = cdiv_rn(rho[sp] - rho[sm], h2, rh2);
-        const double p1 = cdiv_rn(p[sp] - p[sm], h2, rh2);
+        const double *vel = a == 0 ? sU : sV;
+        const double r1 = cdiv_rn(sR[sp] - sR[sm], h2, rh2);
+        const double p1 = cdiv_rn(sP[sp] - sP[sm], h2, rh2);
         gr2 = gr2 + r1 * r1;
         gp2 = gp2 + p1 * p1;
-        divu = divu + cdiv_rn(vel[sp + N * a] - vel[sm + N * a], h2, rh2);
+        divu = divu + cdiv_rn(vel[sp] - vel[sm], h2, rh2);
         if (in2) {
-          const long long sp2 = t + 2 * st[a], sm2 = t - 2 * st[a];
+          const int sp2 = c + 2 * st[a], sm2 = c - 2 * st[a];
           const double h12 = fc.h12[a], rh12 = fc.rh12[a];
-          dr[a] = cdiv_rn(rho[sm2] - 8.0 * rho[sm] + 8.0 * rho[sp] - rho[sp2], h12, rh12);
-          de[a] = cdiv_rn(nrg(sm2) - 8.0 * nrg(sm) + 8.0 * nrg(sp) - nrg(sp2), h12, rh12);
+          dr[a] = cdiv_rn(sR[sm2] - 8.0 * sR[sm] + 8.0 * sR[sp] - sR[sp2], h12, rh12);
+          de[a] = cdiv_rn(sE[sm2] - 8.0 * sE[sm] + 8.0 * sE[sp] - sE[sp2], h12, rh12);
         }
       }
-      const double rc = rho[t], pc = p[t];
+      const double rc = sR[c], pc = sP[c];
       double Cp = 0.0, Cv = 0.0;
       for (int q = 0; q < fc.ns; ++q) {
         const double Y = var[t + N * (3 + q)] / rc;
@@ -360,6 +389,7 @@ __global__ void __launch_bounds__(256) k_features2d(const G3 g, const FeatConst 
         if (interior) g1max = fmaxf(g1max, gg);
       }
     }
+    rho_out[t] = sR[c];
     F[t] = f0;
     F[t + N] = f1;
     F[t + 2 * N] = f2;
@@ -378,38 +408,69 @@ __global__ void __launch_bounds__(256) k_features2d(const G3 g, const FeatConst 
   }
 }
 
-// 2D active cells: interior cells whose 3x3 neighbourhood has M > tau (every
-// stride-th interior cell as well when stride > 0), appended to `list` with
-// one atomic per wave; C_hat is cleared on every cell.
-__global__ void __launch_bounds__(256) k_active2d(const G3 g, const float *__restrict__ M, float tau,
+constexpr int ACT2_ROWS = 8;  // rows per thread of the 2D activity pass
+
+// Launch: block (64, 4) over (i, j); each thread takes ACT2_ROWS rows of its
+// column, keeping the 3-wide row maxima of rows j-1, j, j+1 in registers, and
+// the block appends its active cells with one atomic (a prefix sum over the
+// block gives each thread its slots).
+__global__ void __launch_bounds__(256) k_active2d(const G3 g, const float *__restrict__ A, float tau,
                                                   unsigned stride, unsigned *__restrict__ list,
                                                   unsigned *__restrict__ cnt, float *__restrict__ chat) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int jb = (blockIdx.y * 4 + threadIdx.y) * ACT2_ROWS;
   const int ng = g.ng;
-  const long long t = (long long)i + g.s1 * j;
-  bool act = false;
-  if (i < g.ngi && j < g.ngj) {
-    if (chat) chat[t] = 0.f;
-    if (i >= ng && j >= ng && i < ng + g.nci && j < ng + g.ncj) {
-      float m = 0.f;
-      for (int dj = -1; dj <= 1; ++dj)
-        for (int di = -1; di <= 1; ++di) m = fmaxf(m, M[t + di + dj * g.s1]);
-      act = m > tau;
-      if (stride > 0) {
-        const long long lin = (long long)(i - ng) + (long long)g.nci * (j - ng);
-        act = act || (lin % stride) == 0;
+  const bool colin = i >= ng && i < ng + g.nci;
+  auto h3 = [&](int j) {  // max of A over (i-1..i+1, j)
+    float m = 0.f;
+    if (colin && j >= 0 && j < g.ngj) {
+      const long long t = (long long)i + g.s1 * j;
+      m = fmaxf(A[t - 1], fmaxf(A[t], A[t + 1]));
+    }
+    return m;
+  };
+  unsigned bits = 0;  // active rows of this thread
+  float hm = h3(jb - 1), hc = h3(jb);
+#pragma unroll
+  for (int r = 0; r < ACT2_ROWS; ++r) {
+    const int j = jb + r;
+    const float hp = h3(j + 1);
+    if (i < g.ngi && j < g.ngj) {
+      const long long t = (long long)i + g.s1 * j;
+      if (chat) chat[t] = 0.f;
+      if (colin && j >= ng && j < ng + g.ncj) {
+        bool act = fmaxf(hm, fmaxf(hc, hp)) > tau;
+        if (stride > 0) {
+          const long long lin = (long long)(i - ng) + (long long)g.nci * (j - ng);
+          act = act || (lin % stride) == 0;
+        }
+        if (act) bits |= 1u << r;
       }
     }
+    hm = hc;
+    hc = hp;
   }
-  const int lane = threadIdx.y * 64 + threadIdx.x;
-  const unsigned long long mask = __ballot(act);
-  if (mask) {
-    const int first = __ffsll((long long)mask) - 1;
-    unsigned base = 0;
-    if ((lane & 63) == first) base = atomicAdd(cnt, (unsigned)__popcll(mask));
-    base = __shfl(base, first, 64);
-    if (act) list[base + __popcll(mask & ((1ull << (lane & 63)) - 1ull))] = (unsigned)t;
+  // block prefix sum of the counts
+  const unsigned n = (unsigned)__popc(bits);
+  const int lane = threadIdx.x, w = threadIdx.y;
+  unsigned inc = n;
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
   }
+  __shared__ unsigned wsum[4], base;
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const unsigned tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    base = tot ? atomicAdd(cnt, tot) : 0u;
+  }
+  __syncthreads();
+  unsigned off = base + inc - n;
+  for (int q = 0; q < w; ++q) off += wsum[q];
+#pragma unroll
+  for (int r = 0; r < ACT2_ROWS; ++r)
+    if (bits & (1u << r)) list[off++] = (unsigned)((long long)i + g.s1 * (jb + r));
 }
 
 // Ghost layer next to each face (the cells the face interpolation reads):
@@ -853,12 +914,12 @@ int features(const Geo &g, const fst_config &cf, const Phys &ph, const double *P
   return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 
-int features2d(const Geo &g, const fst_config &cf, const Phys &ph, const double *var, const double *rho,
-               const double *p, const double *vel, LearnedBufs &L, hipStream_t s) {
+int features2d(const Geo &g, const fst_config &cf, const Phys &ph, const double *var, double *rho_out,
+               LearnedBufs &L, hipStream_t s) {
   const G3 q = make_g3(g);
   k_clear_learned<<<1, 64, 0, s>>>(L.sc, L.cnt);
   const dim3 fg((g.ngi + 63) / 64, (g.ngj + 3) / 4);
-  k_features2d<<<fg, dim3(64, 4), 0, s>>>(q, make_feat(cf, ph), var, rho, p, vel, L.F, L.M, L.dir, L.G1, L.sc);
+  k_features2d<<<fg, dim3(64, 4), 0, s>>>(q, make_feat(cf, ph), var, rho_out, L.F, L.M, L.dir, L.G1, L.sc);
   return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 
@@ -945,15 +1006,15 @@ int learned_fields(const Geo &g, const fst_config &cf, const Phys &ph, const dou
   return check_fields(g, cf, P, pf32, L, s);
 }
 
-int learned_fields2d(const Geo &g, const fst_config &cf, const Phys &ph, const double *var, const double *rho,
-                     const double *p, const double *vel, LearnedBufs &L, hipStream_t s) {
+int learned_fields2d(const Geo &g, const fst_config &cf, const Phys &ph, const double *var, double *rho,
+                     LearnedBufs &L, hipStream_t s) {
   if (!L.have_weights) return FST_ERR_STATE;
   if (L.nd != 2 || g.ndim != 2) return FST_ERR_ARG;
-  int rc = features2d(g, cf, ph, var, rho, p, vel, L, s);
+  int rc = features2d(g, cf, ph, var, rho, L, s);
   if (rc) return rc;
   const G3 q = make_g3(g);
-  const dim3 ag((g.ngi + 63) / 64, (g.ngj + 3) / 4);
-  k_active2d<<<ag, dim3(64, 4), 0, s>>>(q, L.M, L.tau, 0u, L.list, L.cnt, L.chat);
+  const dim3 ag((g.ngi + 63) / 64, (g.ngj + 4 * ACT2_ROWS - 1) / (4 * ACT2_ROWS));
+  k_active2d<<<ag, dim3(64, 4), 0, s>>>(q, L.G1, L.tau, 0u, L.list, L.cnt, L.chat);
   k_mlp_infer<2><<<1024, 256, 0, s>>>(q, L.F, L.list, L.cnt, L.Wb, L.W, L.G1, rho, 0, hmin_of(cf), L.chat);
   return check_fields(g, cf, rho, false, L, s);
 }
@@ -1189,18 +1250,17 @@ int trainer_harvest(Trainer &t, const Geo &g, const fst_config &cf, const Phys &
 }
 
 int trainer_harvest2d(Trainer &t, const Geo &g, const fst_config &cf, const Phys &ph, const double *U,
-                      const double *rho, const double *p, const double *vel, unsigned stride, long long max_rows,
-                      long long *added) {
+                      double *rho, unsigned stride, long long max_rows, long long *added) {
   *added = 0;
   if (t.nd != 2 || g.ndim != 2) return FST_ERR_ARG;
   int rc = feat_bufs(t, g);
   if (rc) return rc;
   hipStream_t s = t.s;
-  rc = features2d(g, cf, ph, U, rho, p, vel, t.feat, s);
+  rc = features2d(g, cf, ph, U, rho, t.feat, s);
   if (rc) return rc;
   const G3 q = make_g3(g);
-  const dim3 ag((g.ngi + 63) / 64, (g.ngj + 3) / 4);
-  k_active2d<<<ag, dim3(64, 4), 0, s>>>(q, t.feat.M, t.feat.tau, stride, t.feat.list, t.feat.cnt, nullptr);
+  const dim3 ag((g.ngi + 63) / 64, (g.ngj + 4 * ACT2_ROWS - 1) / (4 * ACT2_ROWS));
+  k_active2d<<<ag, dim3(64, 4), 0, s>>>(q, t.feat.G1, t.feat.tau, stride, t.feat.list, t.feat.cnt, nullptr);
   return append_rows(t, g, cf, U, rho, max_rows, added);
 }
 

@@ -80,10 +80,11 @@ int learned_alloc(const Geo &g, LearnedBufs &L);
 // fallback, ghost layer of C_hat; maxCh into L.sc.  Stream-ordered.
 int learned_fields(const Geo &g, const fst_config &cf, const Phys &ph, const double *P, bool pf32, LearnedBufs &L,
                    hipStream_t s);
-// 2D (cart2d, reference-sequence state): features of var, rho, p and vel
-// (ghosts filled, primitives current), then C_hat as above.
-int learned_fields2d(const Geo &g, const fst_config &cf, const Phys &ph, const double *var, const double *rho,
-                     const double *p, const double *vel, LearnedBufs &L, hipStream_t s);
+// 2D (cart2d): features of var (ghosts filled; the pass computes the
+// primitives itself and writes rho, which the network and the check read,
+// to `rho`), then C_hat as above.
+int learned_fields2d(const Geo &g, const fst_config &cf, const Phys &ph, const double *var, double *rho,
+                     LearnedBufs &L, hipStream_t s);
 int learned_set_weights(LearnedBufs &L, const float *host, hipStream_t s);
 void learned_free(LearnedBufs &L);
 
@@ -124,12 +125,11 @@ int trainer_get_weights(Trainer &t, float *host);
 // Rows from the state U of a plain (C-equation) solver with primitives P:
 // every active cell plus every stride-th interior cell; targets
 // var(nv+1) * hmin / rho.  3D: P = the fused primitive planes; 2D: the
-// reference-sequence rho, p, vel (P2 = {rho, p, vel}).
+// feature pass writes rho into `rho`.
 int trainer_harvest(Trainer &t, const Geo &g, const fst_config &cf, const Phys &ph, const double *U,
                     const double *P, unsigned stride, long long max_rows, long long *added);
 int trainer_harvest2d(Trainer &t, const Geo &g, const fst_config &cf, const Phys &ph, const double *U,
-                      const double *rho, const double *p, const double *vel, unsigned stride, long long max_rows,
-                      long long *added);
+                      double *rho, unsigned stride, long long max_rows, long long *added);
 int trainer_train(Trainer &t, int epochs, int batch, float lr, unsigned long long seed, float *loss_hist);
 int trainer_forward_backward(Trainer &t, const float *X, const float *T, int M, float *Y, float *grad, float *loss);
 int trainer_read_rows(Trainer &t, long long r0, long long n, float *X, float *T);

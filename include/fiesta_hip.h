@@ -297,7 +297,8 @@ int fst_copy_probe(fst_ctx *ctx, size_t bytes, int reps, double *ms_per_copy);
  * step use these fields.
  *
  * 2D (cart2d, src/ceq.hpp:80-293, src/cart2d.cpp:201-236): the same on 5x5
- * stencils (75 inputs) and 3x3 neighbourhoods; the network stands in for
+ * stencils (75 inputs); a cell is evaluated when its 3x3 neighbourhood has a
+ * contact source G1 (the G above, h / rho scaled) above tau; the network stands in for
  * updateCeq2D's relaxation of var(nv+1) (the contact indicator), the check
  * uses a = kap eps (dx^2 + dy^2) and the two-direction Laplacian, and the
  * tangent (-d rho/dy, d rho/dx) (4th order) takes the place of the relaxed

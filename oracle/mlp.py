@@ -134,8 +134,8 @@ def stencils(F, cells, dims):
 
 
 def active_cells(M, dims, tau):
-    """Interior cells whose 3x3x3 (2D: 3x3) neighbourhood has max |feature|
-    > tau."""
+    """Interior cells whose 3x3x3 (2D: 3x3) neighbourhood has M > tau (M =
+    max |feature| in 3D, the contact source G in 2D)."""
     g = dims.ng
     Mp = np.asarray(M, np.float32)
     if dims.ndim == 2:
@@ -228,8 +228,10 @@ def chat_field(w, F, M, G, rho, dims, hmin, kap, eps, d, tau=1e-3, tol=0.05, xpe
     """C_hat over the grid as the GPU inference computes it (bf16 network):
     C_hat = max(y, 0) rho / h on the active cells, the face ghost layer, the
     residual check with its fallback, the ghost layer again.  Returns
-    (chat, y, fallback mask, active cells, normalised residuals, threshold)."""
-    cells = active_cells(M, dims, tau)
+    (chat, y, fallback mask, active cells, normalised residuals, threshold).
+    The activity field is M (max |feature|) in 3D and the contact source G in
+    2D."""
+    cells = active_cells(M if dims.ndim == 3 else G, dims, tau)
     X = stencils(F, cells, dims)
     y, _ = forward(w, X, bf=True)
     rf = np.asarray(rho, np.float64).ravel()

@@ -206,7 +206,7 @@ def test_harvest2d_and_training_lowers_loss():
     assert np.all(np.isfinite(loss))
     _, _, after = tr.forward_backward(X, T)
     print("rows %d, dataset loss before %.4g after %.4g (target variance %.4g)" % (n, before, after, T.var()))
-    assert after < 0.3 * before and after < 0.5 * T.var(), (before, after, T.var())
+    assert after < 0.4 * before and after < 0.5 * T.var(), (before, after, T.var())
     tr.close()
 
 
