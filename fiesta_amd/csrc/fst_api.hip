@@ -106,7 +106,7 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
     if (bcs[q] < 0 || bcs[q] > 3) { *why = "unknown boundary type"; return FST_ERR_ARG; }
   if (c.precision != FST_FP64 && !fused_supported(c)) {
     *why = "the FP32 and fast FP64 forms run on the fused 3D stage kernels only (cartesian, WENO5, ns <= 3, "
-           "every array below 2^29 elements)";
+           "every array below 2^32 elements)";
     return FST_ERR_ARG;
   }
   d->ngi = c.nci + 2 * c.ng;
@@ -465,7 +465,7 @@ int fst_set_fused(fst_ctx *ctx, int fused) {
   if (fused && !fused_supported(ctx->cf) && !fused2d_supported(ctx->cf))
     return fail(ctx, FST_ERR_ARG,
                 "the fused path supports cartesian WENO5 configurations with ns <= 3 (2D: FP64) and every array "
-                "below 2^29 elements (3D)");
+                "below 2^32 elements (3D)");
   if (!fused && ctx->cf.learned_c && ctx->g.ndim == 3) return learned_only(ctx, "fst_set_fused(0)");
   if (!fused && ctx->cf.precision != FST_FP64) return fp32_only(ctx, "fst_set_fused(0)");
   ctx->fused = fused ? 1 : 0;
@@ -804,7 +804,7 @@ int fst_mlp_harvest(fst_mlp *m, fst_ctx *ctx, uint32_t stride, int64_t max_rows,
   if (c.grid != 0 || !c.ceq || c.learned_c || c.ns > 3 || !fused_supported(c))
     return mfail(m, FST_ERR_ARG,
                  "fst_mlp_harvest: needs a 3D cartesian context with the C-equation (ns <= 3, every array "
-                 "below 2^29 elements)");
+                 "below 2^32 elements)");
   const long long np = 6 + c.ns;
   if (m->t.Pcells != ctx->g.N || m->t.Pplanes != np) {
     if (m->t.P) (void)hipFree(m->t.P);

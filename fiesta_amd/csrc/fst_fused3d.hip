@@ -33,7 +33,8 @@
 // store them as floats except the pressure (prim(), fst_common.hpp).
 //
 // Addressing: element offsets are 32-bit (checked on the host: every array
-// stays below 2^29 elements) on uniform base pointers.
+// stays below 2^32 elements) on uniform base pointers, scaled to 64-bit byte
+// offsets (ld/st).
 //
 // Fast FP64 form (precision = FST_FP64_FAST): this file compiled a second time
 // as fst_fast64.hip with FST_FAST64_TU defined.  There the WENO5 fluxes
@@ -93,10 +94,10 @@ struct Grid32 {
 };
 
 __device__ __forceinline__ double ld(const double *__restrict__ base, uint32_t off) {
-  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + (off << 3));
+  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + ((uint64_t)off << 3));
 }
 __device__ __forceinline__ void st(double *__restrict__ base, uint32_t off, double v) {
-  *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + (off << 3)) = v;
+  *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + ((uint64_t)off << 3)) = v;
 }
 
 // State value at element offset off of U (doubles), or of the float
@@ -2032,9 +2033,9 @@ void launch_gen_adv(const Geo &g, const Phys &ph, const fst_config &cf, const do
 #ifndef FST_FAST64_TU
 bool gen_adv_supported(const fst_config &cf, const Geo &g) {
   // 32-bit element offsets: the largest array (nvt planes, or the 5 primitive
-  // planes) below 2^29 elements
+  // planes) below 2^32 elements
   return cf.ndim == 3 && cf.grid == 1 && cf.ns >= 1 && cf.ns <= 3 && cf.scheme == FST_WENO5 &&
-         (double)g.N * std::max(g.nvt, 5) < 536870912.0;
+         (double)g.N * std::max(g.nvt, 5) < 4294967296.0;
 }
 
 bool fused_supported(const fst_config &cf) {
@@ -2042,10 +2043,10 @@ bool fused_supported(const fst_config &cf) {
         (cf.precision == FST_FP64 || cf.precision == FST_FP32_BULK || cf.precision == FST_FP64_FAST) &&
         cf.scheme == FST_WENO5 && cf.ns >= 1 && cf.ns <= 3 && (!cf.learned_c || cf.ceq)))
     return false;
-  // 32-bit element offsets: the largest array (nvt planes) below 2^29 elements
+  // 32-bit element offsets: the largest array (nvt planes) below 2^32 elements
   const double N = (double)(cf.nci + 2 * cf.ng) * (cf.ncj + 2 * cf.ng) * (cf.nck + 2 * cf.ng);
   const int nvt = 4 + cf.ns + (cf.ceq && !cf.learned_c ? 5 : 0);
-  return N * std::max(nvt, 6 + cf.ns) < 536870912.0;
+  return N * std::max(nvt, 6 + cf.ns) < 4294967296.0;
 }
 
 #endif  // FST_FAST64_TU
@@ -2165,9 +2166,9 @@ int fused_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, Fus
 int fused_primitives(const Geo &g, const Phys &ph, const fst_config &cf, const double *U, double *P, hipStream_t s) {
   StageFn sf;
   PrimFn pf;
-  // 32-bit element offsets in k_prims3d: 6 + ns planes of N cells below 2^29
+  // 32-bit element offsets in k_prims3d: 6 + ns planes of N cells below 2^32
   if (!pick(cf.ns, false, false, &sf, &pf) || g.ndim != 3 || cf.grid != 0 ||
-      (double)g.N * (6 + cf.ns) >= 536870912.0)
+      (double)g.N * (6 + cf.ns) >= 4294967296.0)
     return FST_ERR_ARG;
   const FC k = make_fc(ph, cf);
   const Grid32 q = make_grid(g);

@@ -89,10 +89,12 @@ inline unsigned nblk(long long n, int bs = 256) { return (unsigned)((n + bs - 1)
 // Per cell: F (cells with one neighbour each way), dir and G1 (cells with two
 // neighbours each way), M = max |F|.  P planes: rho, p, u, v, w, nrg, Y_s.
 // Every value is computed in FP64 in a fixed order and rounded once to FP32.
-// Launch: block (64, 4) over (i, j); each thread marches its (i, j) column
-// through FEAT_KCH planes, keeping the z stencils (rho and nrg at k-2..k+2,
-// p and w at k-1..k+1) in registers, so each plane is read once for z.
+// Launch: block (64, FEAT_BY) over (i, j); each thread marches its (i, j)
+// column through FEAT_KCH planes, keeping the z stencils (rho and nrg at
+// k-2..k+2, p and w at k-1..k+1) in registers, so each plane is read once for
+// z; the block's 16 rows share their y halo (2 rows each side) in L2.
 constexpr int FEAT_KCH = 16;
+constexpr int FEAT_BY = 4;
 
 // Arithmetic of the feature pass: FP64 with IEEE divisions (FP64 primitive
 // planes; equal to the oracle's features3d bit for bit), or binary32 with
@@ -114,10 +116,11 @@ struct FeatAr<true> {
 };
 
 template <bool PF>
-__global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst fc, const double *__restrict__ P,
-                                                    float *__restrict__ F, float *__restrict__ M,
-                                                    float *__restrict__ Dir, float *__restrict__ G1, Scalars *sc) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+__global__ void __launch_bounds__(64 * FEAT_BY) k_features3d(const G3 g, const FeatConst fc,
+                                                             const double *__restrict__ P, float *__restrict__ F,
+                                                             float *__restrict__ M, float *__restrict__ Dir,
+                                                             float *__restrict__ G1, Scalars *sc) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * FEAT_BY + threadIdx.y;
   const int k0 = blockIdx.z * FEAT_KCH, k1 = min(k0 + FEAT_KCH, g.ngk);
   const bool col = i < g.ngi && j < g.ngj;
   const long long N = g.N;
@@ -229,12 +232,13 @@ __global__ void __launch_bounds__(256) k_features3d(const G3 g, const FeatConst 
     }
   }
   // block maximum, then one atomic per block that has a non-zero value
-  __shared__ float red[4];
+  __shared__ float red[FEAT_BY];
   g1max = wave_maxf(g1max);
   if (threadIdx.x == 0) red[threadIdx.y] = g1max;
   __syncthreads();
   if (threadIdx.x == 0 && threadIdx.y == 0) {
-    const float m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    float m = 0.f;
+    for (int q = 0; q < FEAT_BY; ++q) m = fmaxf(m, red[q]);
     if (m > 0.f) atomic_max_nonneg(&sc->maxC, (double)m);
   }
 }
@@ -908,9 +912,10 @@ int features(const Geo &g, const fst_config &cf, const Phys &ph, const double *P
              hipStream_t s) {
   const G3 q = make_g3(g);
   k_clear_learned<<<1, 64, 0, s>>>(L.sc, L.cnt);
-  const dim3 fg((g.ngi + 63) / 64, (g.ngj + 3) / 4, (g.ngk + FEAT_KCH - 1) / FEAT_KCH);
-  if (pf32) k_features3d<true><<<fg, dim3(64, 4), 0, s>>>(q, make_feat(cf, ph), P, L.F, L.M, L.dir, L.G1, L.sc);
-  else k_features3d<false><<<fg, dim3(64, 4), 0, s>>>(q, make_feat(cf, ph), P, L.F, L.M, L.dir, L.G1, L.sc);
+  const dim3 fg((g.ngi + 63) / 64, (g.ngj + FEAT_BY - 1) / FEAT_BY, (g.ngk + FEAT_KCH - 1) / FEAT_KCH);
+  const dim3 fb(64, FEAT_BY);
+  if (pf32) k_features3d<true><<<fg, fb, 0, s>>>(q, make_feat(cf, ph), P, L.F, L.M, L.dir, L.G1, L.sc);
+  else k_features3d<false><<<fg, fb, 0, s>>>(q, make_feat(cf, ph), P, L.F, L.M, L.dir, L.G1, L.sc);
   return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 

@@ -140,7 +140,7 @@ def test_unknown_precision_is_rejected(field, value):
 
 
 @pytest.mark.parametrize("precision", [1, 2])
-@pytest.mark.parametrize("case", ["2d", "ns4", "gen3d", "over_2_29"])
+@pytest.mark.parametrize("case", ["2d", "ns4", "gen3d", "over_cap"])
 def test_fp32_and_fast64_forms_need_the_fused_3d_path(case, precision):
     """The FP32 form exists only in the fused 3D stage kernels; elsewhere the
     FP64 sequence would run without notice, so the configuration is refused."""
@@ -152,10 +152,10 @@ def test_fp32_and_fast64_forms_need_the_fused_3d_path(case, precision):
     elif case == "gen3d":
         kw["grid"] = 1
     else:
-        kw.update(nci=400, ncj=400, nck=400)  # 406^3 cells x 11 planes > 2^29
+        kw.update(nci=740, ncj=740, nck=740)  # 746^3 cells x 11 planes > 2^32
     lib = runtime.lib()
     ok = make_config(**kw)
-    if case == "over_2_29":
+    if case == "over_cap":
         assert lib.fst_config_dims(C.byref(ok), C.byref(FstDims())) == 0  # FP64 runs on the sequence kernels
     cfg = make_config(precision=precision, **kw)
     assert lib.fst_config_dims(C.byref(cfg), C.byref(FstDims())) == 1

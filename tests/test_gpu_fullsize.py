@@ -151,3 +151,34 @@ def test_fullsize_fp32_form_close_to_fp64(rt):
     for s in range(ns):
         ya, yb = a[d.ndim + 1 + s][sl[1:]] / rho_a, b[d.ndim + 1 + s][sl[1:]] / rho_b
         assert np.sqrt(((yb - ya) ** 2).sum() / (ya ** 2).sum()) < 1e-4
+
+
+def test_fused_path_above_2_29_elements():
+    """A grid whose nvt arrays hold more than 2^29 elements (1024 x 58 x 1024
+    cells, 678,976,000 elements per array: beyond 32-bit byte offsets) runs
+    on the fused stage kernels and equals the kernel sequence bit for bit."""
+    from fiesta_amd.config import make_config
+    cfg = make_config(ndim=3, ns=1, nci=1024, ncj=58, nck=1024, dx=1e-3, dy=1e-3, dz=1e-3, gamma=(1.4,),
+                      M=(0.029,), mu=(1.8e-5,), visc=True, ceq=True, kap=10.0, eps=5.0, alpha=20.0, xper=True,
+                      zper=True, bc=("outflow",) * 6, dt=1e-8, R=8.314462)
+    d = Dims(cfg)
+    assert d.N * d.nvt > 2 ** 29
+    _say("fused above 2^29 elements: %d x %d" % (d.N, d.nvt))
+    var = np.zeros(d.shape(d.nvt))
+    k, j, i = np.meshgrid(np.arange(d.ngk), np.arange(d.ngj), np.arange(d.ngi), indexing="ij", sparse=True)
+    rho = 1.0 + 0.1 * np.sin(0.05 * i) * np.cos(0.07 * j) * np.sin(0.03 * k)
+    u = 10.0 * np.sin(0.02 * j) + 0 * i + 0 * k
+    var[0] = rho * u
+    var[3] = 1e5 / 0.4 + 0.5 * rho * u * u
+    var[4] = rho
+    var[5] = np.abs(np.sin(0.1 * i) + 0 * j + 0 * k)
+    outs = []
+    for fused in (True, False):
+        m = Module(cfg)
+        m.upload(FST_VAR, var)
+        m.pre_sim()
+        m.set_fused(fused)
+        m.step(1)
+        outs.append(m.download(FST_VAR)[d.interior()])
+        m.close()
+    assert_bitwise(outs[0], outs[1], "fused vs kernel sequence above 2^29 elements")
