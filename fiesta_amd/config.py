@@ -33,7 +33,7 @@ class FstConfig(C.Structure):
         ("rk_order", C.c_int32),
         ("noise", C.c_int32), ("n_nt", C.c_int32), ("n_mode", C.c_int32),
         ("n_dh", C.c_double), ("n_eta", C.c_double), ("n_coff", C.c_double),
-        ("reserved", C.c_int32 * 4),
+        ("gen_ext", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -42,12 +42,14 @@ def make_config(ndim=3, ns=1, ng=3, nci=8, ncj=8, nck=8, dx=1.0, dy=1.0, dz=1.0,
                 scheme="weno5", kap=1.23456, eps=1.0, alpha=1.0, g=9.81, rhoRef=0.0,
                 xper=False, yper=False, zper=False, bc=("outflow",) * 6, dt=1e-6,
                 precision=FST_FP64, learned_c=False, grid=0, rk=2, noise=False, n_dh=0.0, n_eta=0.0,
-                n_coff=0.0, n_nt=1, n_mode=1):
+                n_coff=0.0, n_nt=1, n_mode=1, gen_ext=False):
     """Build an FstConfig.  `bc` lists xmin, xmax, ymin, ymax, zmin, zmax.
     grid=1 selects the generalized-grid modules (gen2d/gen3d), whose cell
     sizes are 1 (src/input.cpp:344-350).  rk: 2 = the reference's RK2
     midpoint, 3 = SSP-RK3 (an added option).  noise, n_*: the noise filter
-    of postStep (src/input.cpp:270-277: dh, eta, coff, nt = 1, mode = 1)."""
+    of postStep (src/input.cpp:270-277: dh, eta, coff, nt = 1, mode = 1).
+    gen_ext: on generalized grids, the cartesian ghost fill and the buoyancy
+    term (an option the reference's gen modules do not have)."""
     c = FstConfig()
     c.ndim, c.grid, c.ns, c.ng = ndim, int(grid), ns, ng
     c.nci, c.ncj, c.nck = nci, ncj, (nck if ndim == 3 else 1)
@@ -72,6 +74,7 @@ def make_config(ndim=3, ns=1, ng=3, nci=8, ncj=8, nck=8, dx=1.0, dy=1.0, dz=1.0,
     c.rk_order = int(rk)
     c.noise, c.n_nt, c.n_mode = int(noise), int(n_nt), int(n_mode)
     c.n_dh, c.n_eta, c.n_coff = n_dh, n_eta, n_coff
+    c.gen_ext = int(gen_ext)
     return c
 
 

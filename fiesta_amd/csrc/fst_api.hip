@@ -77,6 +77,19 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
     *why = "generalized grids run the reference arithmetic (FP64, no learned C-indicator)";
     return FST_ERR_ARG;
   }
+  if (c.gen_ext && c.grid != 1) {
+    *why = "gen_ext applies to generalized grids (grid = 1)";
+    return FST_ERR_ARG;
+  }
+  if (c.gen_ext) {
+    const int bcs[6] = {c.bcL, c.bcR, c.bcB, c.bcT, c.bcH, c.bcF};
+    for (int q = 0; q < 6; ++q)
+      if (bcs[q] == 3) {
+        *why = "gen_ext: the hydrostatic ghost fill needs the physical cell size, which the index space of a "
+               "generalized grid does not give";
+        return FST_ERR_ARG;
+      }
+  }
   if (c.learned_c && !c.ceq) {
     *why = "the learned C-indicator replaces the C-equation's relaxation solve: needs ceq = 1";
     return FST_ERR_ARG;
@@ -179,6 +192,8 @@ void free_all(fst_ctx *ctx) {
 // a step) and uses the last ones otherwise.
 int compute_seq(fst_ctx *ctx, bool fields = false) {
   if (ctx->cf.grid == 1) {
+    // gen_ext: the ghost fill cart3d applies before compute (src/cart3d.cpp:221)
+    if (ctx->cf.gen_ext && ctx->g.ndim == 3) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
     launch_compute_gen(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->stream);
     return FST_OK;
   }
@@ -419,6 +434,8 @@ int fst_pre_sim(fst_ctx *ctx) {
   ctx->fb.valid = false;
   if (ctx->cf.grid == 1) {  // gen3d_func::preSim / gen2d_func::preSim: metrics only
     launch_metrics(ctx->g, ctx->b, ctx->stream);
+    // gen_ext in 2D: the ghost fill of cart2d's preSim (src/cart2d.cpp:137)
+    if (ctx->cf.gen_ext && ctx->g.ndim == 2) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
     return check_launch(ctx, "fst_pre_sim");
   }
   if (ctx->g.ndim == 2) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);  // src/cart2d.cpp:137
@@ -429,8 +446,10 @@ int fst_pre_sim(fst_ctx *ctx) {
 int fst_pre_step(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
   ctx->fb.valid = false;
-  // cart2d: src/cart2d.cpp:147-149; cart3d, gen2d, gen3d: nothing
-  if (ctx->g.ndim == 2 && ctx->cf.grid == 0) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
+  // cart2d: src/cart2d.cpp:147-149 (gen2d with gen_ext likewise); cart3d,
+  // gen2d, gen3d: nothing
+  if (ctx->g.ndim == 2 && (ctx->cf.grid == 0 || ctx->cf.gen_ext))
+    launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
   return check_launch(ctx, "fst_pre_step");
 }
 
@@ -483,7 +502,8 @@ int fst_step(fst_ctx *ctx, int nsteps) {
   } else {
     ctx->fb.valid = false;
     for (int t = 0; t < nsteps; ++t) {
-      if (ctx->g.ndim == 2 && ctx->cf.grid == 0) launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
+      if (ctx->g.ndim == 2 && (ctx->cf.grid == 0 || ctx->cf.gen_ext))
+        launch_apply_bcs(ctx->g, ctx->cf, ctx->b.var, ctx->stream);
       const int ns = rk_stages(ctx->cf);
       for (int stage = 1; stage <= ns; ++stage) {
         const int rc = compute_seq(ctx, stage == 1);

@@ -1303,10 +1303,37 @@ void launch_compute_ref(const Geo &g, const Phys &ph, const fst_config &cf, cons
   }
 }
 
+// Buoyancy of cart3d / cart2d on the generalized grids (gen_ext): the
+// source terms of computeBuoyancy3D (src/buoyancy.hpp:22-47) and
+// computeBuoyancy (:49-73) on the stored physical momenta, cell range.
+__global__ void k_buoy_gen(Geo g, Phys ph, Range r, const double *__restrict__ var, const double *__restrict__ rho,
+                           double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long c = g.id(i, j, k), N = g.N;
+  if (g.ndim == 3) {
+    double vv = var[c + N] / rho[c];
+    double rhop = rho[c] - ph.rhoRef;
+    double f = -ph.g * rhop;
+    dvar[c + N] += f;
+    dvar[c + 3 * N] += vv * f;
+  } else {
+    double vv = var[c + N] / var[c + 3 * N];
+    double rhop = rho[c] - ph.rhoRef;
+    double eps = 1e-6;
+    if (rhop >= eps || rhop <= -eps) {
+      double f = -ph.g * rhop;
+      dvar[c + N] += f;
+      dvar[c + 2 * N] += fabs(vv) * f;
+    }
+  }
+}
+
 // gen3d_func::compute (src/gen3d.cpp:341-369) / gen2d_func::compute
 // (src/gen2d.cpp:100-131): primitives, contravariant velocity, per-variable
 // fluxes and flux differences, metric pressure gradient.  No boundary
-// conditions and no viscous, buoyancy or C-equation terms.  The 3D flux
+// conditions and no viscous, buoyancy or C-equation terms (gen_ext adds the
+// ghost fill, in compute_seq / preStep, and the buoyancy).  The 3D flux
 // kernel's dz is cf.dx (src/gen3d.cpp:359); both are 1 on these grids.
 void launch_compute_gen(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s) {
   launch_eos_vel(g, ph, b, s);
@@ -1333,6 +1360,9 @@ void launch_compute_gen(const Geo &g, const Phys &ph, const fst_config &cf, cons
     }
     k_genpresgrad2d<<<grid_for(cr), 256, 0, s>>>(g, cr, b.p, b.metrics, b.dvar);
   }
+  // gen_ext: the cartesian modules' buoyancy after the pressure gradient, as
+  // they order it (src/cart3d.cpp:268-270, src/cart2d.cpp:186-188)
+  if (cf.gen_ext && cf.buoyancy) k_buoy_gen<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.var, b.rho, b.dvar);
 }
 
 // gen3d_func::preSim / gen2d_func::preSim: metrics of the interior cells, then

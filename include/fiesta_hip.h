@@ -33,7 +33,7 @@ extern "C" {
 #endif
 
 #define FST_MAX_SPECIES 8
-#define FST_ABI_VERSION 6
+#define FST_ABI_VERSION 7
 
 /* Boundary types, in the order of `enum class BCType` (src/bc.hpp:31). */
 enum fst_bc_type {
@@ -122,7 +122,13 @@ typedef struct fst_config {
   double n_dh;           /* noise.dh, detection threshold                     */
   double n_eta;          /* noise.eta, smoothing coefficient                  */
   double n_coff;         /* noise.coff, C_hat cut-off factor (ceq on)         */
-  int32_t reserved[4];
+  int32_t gen_ext;       /* generalized grids (grid = 1) only, an option the
+                            reference does not have: 1 = the ghost fill of the
+                            cartesian modules (index space, the bc.* types)
+                            before each compute (3D) / in preStep (2D), and the
+                            buoyancy term when buoyancy = 1; 0 = gen2d/gen3d
+                            as the reference runs them (neither)             */
+  int32_t reserved[3];
 } fst_config;
 
 typedef struct fst_ctx fst_ctx;

@@ -1393,6 +1393,9 @@ void features2d(Orc &o, float *F, float *M, float *Dir, float *G1) {
 // viscous, buoyancy or C-equation terms; calculateFluxesG receives cf.dx as
 // its dz argument (:359), and cf.dx = cf.dz = 1 on generalized grids.
 void compute_gen3d(Orc &o) {
+  // gen_ext (an option the reference does not have): cart3d's ghost fill
+  // before the compute and its buoyancy after the pressure gradient
+  if (o.c.gen_ext) apply_bcs(o);
   eos3d(o);
   genvel3d(o);
   for (int v = 0; v < o.nv; ++v) {
@@ -1400,6 +1403,7 @@ void compute_gen3d(Orc &o) {
     advect3d(o, v);
   }
   genpresgrad3d(o);
+  if (o.c.gen_ext && o.c.buoyancy) buoyancy3d(o);
 }
 
 // gen2d_func::compute, src/gen2d.cpp:100-131
@@ -1411,6 +1415,7 @@ void compute_gen2d(Orc &o) {
     advect2d(o, v);
   }
   genpresgrad2d(o);
+  if (o.c.gen_ext && o.c.buoyancy) buoyancy2d(o);  // gen_ext: cart2d's buoyancy
 }
 
 // cart2d_func::compute, src/cart2d.cpp:151-237
@@ -1482,7 +1487,8 @@ void rk_stage(Orc &o, int stage) {
 }
 
 void pre_step(Orc &o) {
-  if (o.ndim == 2 && !o.gen) apply_bcs(o);  // src/cart2d.cpp:147-149; gen2d/gen3d: nothing
+  // src/cart2d.cpp:147-149; gen2d/gen3d: nothing (gen2d with gen_ext as cart2d)
+  if (o.ndim == 2 && (!o.gen || o.c.gen_ext)) apply_bcs(o);
 }
 
 // secondary variables: calculateRhoPT + computeVelocity + copyExtraVars
@@ -1662,6 +1668,7 @@ void pre_sim(Orc &o) {
     if (o.ndim == 3) metrics3d(o);
     else metrics2d(o);
     metric_ghosts(o);
+    if (o.c.gen_ext && o.ndim == 2) apply_bcs(o);  // gen_ext: cart2d's preSim fill (src/cart2d.cpp:137)
     return;
   }
   if (o.ndim == 2) apply_bcs(o);  // src/cart2d.cpp:137

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_struct_size():
     lib = runtime.lib()
-    assert lib.fst_abi_version() == 6
+    assert lib.fst_abi_version() == 7
     # fst_config layout: the ctypes mirror and the C struct must agree in size
     c = FstConfig()
     lib.fst_config_init(C.byref(c))
@@ -187,3 +187,19 @@ def test_fast64_form_has_no_learned_variant():
     lib = runtime.lib()
     assert lib.fst_config_dims(C.byref(make_config(precision=2, **kw)), C.byref(FstDims())) == 0
     assert lib.fst_config_dims(C.byref(make_config(precision=2, learned_c=True, **kw)), C.byref(FstDims())) == 1
+
+
+def test_gen_ext_validation():
+    lib = runtime.lib()
+    ok = make_config(ndim=3, ns=1, nci=8, ncj=8, nck=8, grid=1, gen_ext=True, buoyancy=True,
+                     bc=("reflective", "outflow", "noslip", "outflow", "outflow", "outflow"))
+    assert lib.fst_config_dims(C.byref(ok), C.byref(FstDims())) == 0
+    # only on generalized grids
+    cart = make_config(ndim=3, ns=1, nci=8, ncj=8, nck=8, gen_ext=True)
+    assert lib.fst_config_dims(C.byref(cart), C.byref(FstDims())) == 1
+    # the hydrostatic fill needs a physical cell size
+    hyd = make_config(ndim=2, ns=1, nci=8, ncj=8, grid=1, gen_ext=True,
+                      bc=("outflow", "outflow", "hydrostatic", "outflow", "outflow", "outflow"))
+    assert lib.fst_config_dims(C.byref(hyd), C.byref(FstDims())) == 1
+    hyd.gen_ext = 0
+    assert lib.fst_config_dims(C.byref(hyd), C.byref(FstDims())) == 0

@@ -105,3 +105,72 @@ def test_gen_rejects_fused_and_learned():
     bad.ceq = 1
     with pytest.raises(FiestaError):
         Module(bad)
+
+
+GEN_EXT = {
+    "gen3d_terrain_ns2_rk2": dict(ndim=3, n=(20, 14, 11), ns=2, rk=2),
+    "gen3d_terrain_ns3_rk3": dict(ndim=3, n=(67, 9, 13), ns=3, rk=3),
+    "gen2d_terrain_ns2_rk3": dict(ndim=2, n=(24, 18, 1), ns=2, rk=3),
+}
+EXT_BC = ("reflective", "outflow", "noslip", "outflow", "reflective", "outflow")
+
+
+def _ext_cfg(c, **kw):
+    cfg = _cfg(c["ndim"], c["n"], c["ns"])
+    cfg.rk_order = c["rk"]
+    for name, v in zip(("bcL", "bcR", "bcB", "bcT", "bcH", "bcF"), EXT_BC):
+        setattr(cfg, name, {"outflow": 0, "reflective": 1, "noslip": 2}[v])
+    cfg.gen_ext = 1
+    cfg.buoyancy = 1
+    cfg.rhoRef = 0.6
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+@pytest.mark.parametrize("name", sorted(GEN_EXT))
+def test_gen_ext_terrain_against_oracle(name):
+    """gen_ext: the cartesian ghost fill and buoyancy on a terrain grid, over
+    several steps of the reference sequence, bit for bit on every cell."""
+    _say("case gen_ext steps", name)
+    cfg = _ext_cfg(GEN_EXT[name])
+    nodes = W.terrain_grid(cfg)
+    var = W.random_state_all(cfg, seed=31, umag=20.0)
+    o = oracle.Oracle(cfg)
+    m = Module(cfg)
+    for x in (o, m):
+        (x.set if x is o else x.upload)(FST_GRID, nodes)
+        (x.set if x is o else x.upload)(FST_VAR, var)
+        x.pre_sim()
+    o.compute()
+    m.compute()
+    assert_bitwise(m.download(FST_DVAR), o.get(FST_DVAR), name + " dvar")
+    o.step(4)
+    m.step(4)
+    assert_bitwise(m.download(FST_VAR), o.get(FST_VAR), name + " var after 4 steps")
+    m.close()
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_gen_ext_uniform_grid_equals_cartesian_module(ndim):
+    """On a uniform grid of spacing 2^-5 the generalized module with gen_ext
+    and the cartesian module give the same state bit for bit."""
+    _say("case gen_ext uniform vs cart", ndim)
+    h = 2.0 ** -5
+    c = dict(ndim=ndim, n=(26, 15, 9 if ndim == 3 else 1), ns=2, rk=3)
+    gen = _ext_cfg(c)
+    cart = _ext_cfg(c, grid=0, gen_ext=0, dx=h, dy=h, dz=h)
+    nodes = W.grid_nodes(gen, lambda i, j, k: (h * i, h * j, h * k if ndim == 3 else None))
+    var = W.random_state_all(gen, seed=6, umag=20.0)
+    a = Module(gen)
+    a.upload(FST_GRID, nodes)
+    b = Module(cart)
+    for m in (a, b):
+        m.upload(FST_VAR, var)
+        m.pre_sim()
+    b.set_fused(False)
+    a.step(5)
+    b.step(5)
+    assert_bitwise(a.download(FST_VAR), b.download(FST_VAR), "gen_ext vs cart after 5 steps")
+    a.close()
+    b.close()
