@@ -20,6 +20,10 @@ wrong node offset, a swapped mirror index or a misplaced term fails:
                                                are exact powers of two), which
                                                ties the generalized path to the
                                                cartesian one pinned elsewhere
+  gen_ext (an option of this library: the      uniform 2^-5 grid: gen + gen_ext steps as
+    cartesian ghost fill and buoyancy on       cart with the same walls and gravity, bit
+    generalized grids)                         for bit; fluid at rest on a sheared grid:
+                                               RHS = -g (rho - rho_ref) on y momentum only
 """
 import numpy as np
 import pytest
