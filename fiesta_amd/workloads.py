@@ -110,15 +110,23 @@ def grid_nodes(cfg, fn):
     return g
 
 
-def terrain_grid(cfg, L=1.0, H=1.0, amp=0.05, beta=2.0):
+def terrain_grid(cfg, L=1.0, H=1.0, amp=0.05, beta=2.0, vertical=2):
     """Terrain-following nodes: ground z_g = amp L sin(2 pi x/L) cos(2 pi y/L),
     vertical stretch z = z_g + (H - z_g)(1 + tanh(beta (xi - 1))/tanh beta),
     xi = k/nck (3D); in 2D the ground is y_g = amp L sin(2 pi x/L) under the
-    same stretch in y."""
+    same stretch in y.  3D with vertical=1: y is the vertical axis (the axis
+    gravity acts along, src/buoyancy.hpp:22-47), the ground
+    y_g = amp L sin(2 pi x/L) cos(2 pi z/L) and the stretch runs along j."""
     d = Dims(cfg)
 
     def fn(i, j, k):
         x = L * i / d.nci
+        if cfg.ndim == 3 and vertical == 1:
+            z = L * k / d.nck
+            yg = amp * L * np.sin(2 * np.pi * x / L) * np.cos(2 * np.pi * z / L)
+            eta = j / d.ncj
+            y = yg + (H - yg) * (1.0 + np.tanh(beta * (eta - 1.0)) / np.tanh(beta))
+            return x, y, z
         if cfg.ndim == 3:
             y = L * j / d.ncj
             zg = amp * L * np.sin(2 * np.pi * x / L) * np.cos(2 * np.pi * y / L)
@@ -138,6 +146,83 @@ def gen3d_terrain_config(n=(512, 512, 256), ns=3, dt=None):
     ns species (air, hot gas, SF6)."""
     return make_config(ndim=3, ns=ns, nci=n[0], ncj=n[1], nck=n[2], grid=1, R=RU,
                        **_mix(ns, [AIR, HOT, SF6]), dt=dt if dt is not None else 1e-7)
+
+
+def cell_centres(cfg, nodes):
+    """Interior cell centres (the mean of each cell's 4 (2D) or 8 (3D) nodes)
+    of a node array (3, nk, nj, ni): a tuple of ndim arrays (k, j, i)."""
+    out = []
+    for a in range(cfg.ndim):
+        g = nodes[a]
+        if cfg.ndim == 3:
+            c = (g[:-1, :-1, :-1] + g[:-1, :-1, 1:] + g[:-1, 1:, :-1] + g[:-1, 1:, 1:] +
+                 g[1:, :-1, :-1] + g[1:, :-1, 1:] + g[1:, 1:, :-1] + g[1:, 1:, 1:]) * 0.125
+        else:
+            c = (g[:, :-1, :-1] + g[:, :-1, 1:] + g[:, 1:, :-1] + g[:, 1:, 1:]) * 0.25
+        out.append(c)
+    return tuple(out)
+
+
+def terrain_expansion_config(n=(512, 256, 512), ns=3, rk=3, dt=None):
+    """Ideal expansion over terrain on a generalized grid (gen3d with the
+    gen_ext option): n = (nci, ncj, nck) cells, y vertical (terrain_grid with
+    vertical=1), air, hot gas and SF6, gravity 9.81 along -y, periodic in x
+    and z, a no-slip ground (ymin) and a no-slip lid (ymax), SSP-RK3.
+    Closed in y and periodic in x and z, the box keeps every species' mass:
+    the index-space no-slip fill mirrors every momentum component and the
+    metric ghosts mirror the metrics, so the contravariant face velocity at
+    both walls is zero.  (A reflective wall flips only the y momentum, which
+    leaves a small face velocity where the grid lines meet it at an angle.)"""
+    return make_config(ndim=3, ns=ns, nci=n[0], ncj=n[1], nck=n[2], grid=1, gen_ext=True, R=RU,
+                       **_mix(ns, [AIR, HOT, SF6]), buoyancy=True, g=9.81, rhoRef=0.0, xper=True, zper=True,
+                       bc=("outflow", "outflow", "noslip", "noslip", "outflow", "outflow"), rk=rk,
+                       dt=dt if dt is not None else 1e-7)
+
+
+def terrain_expansion_state(cfg, nodes, T0=300.0, p0=1.0e5, r0=0.05, pr=50.0, blob=(0.5, 0.2, 0.5),
+                            layer=0.08):
+    """At rest: an isothermal air column in hydrostatic balance,
+    p = p0 exp(-g y M_air / (R T0)); a Gaussian hot-gas blob (radius r0,
+    pressure ratio pr) at height blob[1] above the ground under (blob[0],
+    blob[2]); with 3 species, SF6 pooled within `layer` of the ground.
+    Returns (var, cell_centres)."""
+    x, y, z = cell_centres(cfg, nodes)
+    yg = 0.05 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * z)  # the ground under each cell (terrain_grid)
+    cx, hy, cz = blob
+    r2 = (x - cx) ** 2 + (y - (0.05 * np.sin(2 * np.pi * cx) * np.cos(2 * np.pi * cz) + hy)) ** 2 + (z - cz) ** 2
+    yh = np.exp(-r2 / r0 ** 2)
+    ysf = np.zeros_like(yh) if cfg.ns < 3 else 0.5 * (1.0 - np.tanh((y - yg - layer) / (0.25 * layer)))
+    ysf = ysf * (1.0 - yh)
+    yair = 1.0 - yh - ysf
+    ph = p0 * np.exp(-cfg.gAccel * y * cfg.M[0] / (cfg.R * T0))
+    p = ph * (1.0 + (pr - 1.0) * yh)
+    Ys = [yair, yh, ysf][: cfg.ns]
+    Rm = sum(Ys[s] * cfg.R / cfg.M[s] for s in range(cfg.ns))
+    rho = p / (Rm * T0)
+    rho_s = [rho * Ys[s] for s in range(cfg.ns)]
+    zero = np.zeros_like(rho)
+    var = fill_conserved(cfg, rho_s, [zero, zero, zero], p)
+    return var, (x, y, z)
+
+
+def gen_cfl_dt(cfg, var, nodes, cfl):
+    """dt = cfl * h_min / max(|u| + c) on a generalized grid, h_min the
+    shortest cell edge of the node array."""
+    h = np.inf
+    for a in range(3):
+        e = np.sqrt(sum(np.diff(nodes[q], axis=2 - a) ** 2 for q in range(3)))
+        h = min(h, float(e.min()))
+    d = Dims(cfg)
+    v = var[d.interior()]
+    rho_s = v[4: 4 + cfg.ns]
+    rho = rho_s.sum(axis=0)
+    Cp = sum(rho_s[s] / rho * (cfg.gamma[s] * cfg.R / cfg.M[s] / (cfg.gamma[s] - 1)) for s in range(cfg.ns))
+    Cv = sum(rho_s[s] / rho * (cfg.R / cfg.M[s] / (cfg.gamma[s] - 1)) for s in range(cfg.ns))
+    gam = Cp / Cv
+    m2 = v[0] ** 2 + v[1] ** 2 + v[2] ** 2
+    p = (gam - 1) * (v[3] - 0.5 / rho * m2)
+    c = np.sqrt(gam * p / rho) + np.sqrt(m2) / rho
+    return float(cfl * h / c.max())
 
 
 def sod_config(nci=1000, ncj=4, dt=2e-4, ceq=True):

@@ -174,3 +174,49 @@ def test_gen_ext_uniform_grid_equals_cartesian_module(ndim):
     assert_bitwise(a.download(FST_VAR), b.download(FST_VAR), "gen_ext vs cart after 5 steps")
     a.close()
     b.close()
+
+
+def test_terrain_expansion_against_oracle():
+    """The terrain expansion workload (terrain_expansion_config: gen3d +
+    gen_ext, 3 species, gravity, no-slip ground and lid, SSP-RK3) at
+    40 x 20 x 40 cells: 12 steps of the blast on the GPU equal the oracle's
+    bit for bit on every cell."""
+    _say("case terrain expansion 40x20x40 vs oracle, 12 steps")
+    cfg = W.terrain_expansion_config((40, 20, 40))
+    nodes = W.terrain_grid(cfg, vertical=1)
+    var, _ = W.terrain_expansion_state(cfg, nodes)
+    cfg.dt = W.gen_cfl_dt(cfg, var, nodes, 0.4)
+    o = oracle.Oracle(cfg)
+    o.set(FST_GRID, nodes)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    m = Module(cfg)
+    m.upload(FST_GRID, nodes)
+    m.upload(FST_VAR, var)
+    m.pre_sim()
+    o.step(12)
+    m.step(12)
+    assert_bitwise(m.download(FST_VAR), o.get(FST_VAR), "terrain expansion var after 12 steps")
+    m.close()
+
+
+def test_terrain_expansion_keeps_species_mass():
+    """The same workload at 128 x 64 x 128 cells, 60 steps on the GPU: every
+    species' interior total stays what it was to 1e-13 relative."""
+    _say("case terrain expansion 128x64x128 mass, 60 steps")
+    cfg = W.terrain_expansion_config((128, 64, 128))
+    nodes = W.terrain_grid(cfg, vertical=1)
+    var, _ = W.terrain_expansion_state(cfg, nodes)
+    cfg.dt = W.gen_cfl_dt(cfg, var, nodes, 0.4)
+    d = W.Dims(cfg)
+    sl = d.interior()
+    m = Module(cfg)
+    m.upload(FST_GRID, nodes)
+    m.upload(FST_VAR, var)
+    m.pre_sim()
+    m0 = m.download(FST_VAR)[sl][4:].sum(axis=(1, 2, 3))
+    m.step(60)
+    v = m.download(FST_VAR)
+    m.close()
+    assert np.isfinite(v).all()
+    np.testing.assert_allclose(v[sl][4:].sum(axis=(1, 2, 3)), m0, rtol=1e-13, atol=0)
