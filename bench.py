@@ -369,10 +369,11 @@ def extras_leg(args, var):
 def gen3d_leg(args):
     """The generalized-grid module (gen3d: EOS, contravariant velocity, WENO
     fluxes, metric pressure gradient; src/gen3d.cpp:341-369) on the 3D
-    terrain workload, 512 x 512 x 256 cells, 3 species, FP64: EOS and
-    contravariant velocity, the fused advection kernel (fluxes of every
-    variable in one launch), the metric pressure gradient, the RK update.
-    Device ms per step from hipEvents over 3 steps after one warm-up step."""
+    terrain workload, 512 x 512 x 256 cells, 3 species, FP64, on the fused
+    gen3d stage: the advection kernel (fluxes of every variable and the
+    metric pressure gradient in one launch), then one pass with the RK update
+    and the EOS / contravariant velocity of the new state.  Device ms per
+    step from hipEvents over 3 steps after one warm-up step."""
     from fiesta_amd import workloads as W
     from fiesta_amd.config import FST_GRID, FST_VAR, Dims
     from fiesta_amd.runtime import Module
@@ -394,7 +395,8 @@ def gen3d_leg(args):
     balg = 20 * d.nvt          # U read/write of the RK2 stages
     bmet = balg + 9 * 8        # plus one read of the 9 metric terms per stage
     stage_ms = ms_step / 2
-    return {"workload": "gen3d terrain %dx%dx%d, 3 species, FP64 (fused advection + metric pressure gradient + RK kernels)" % n,
+    return {"workload": "gen3d terrain %dx%dx%d, 3 species, FP64 (fused gen3d stage: advection + metric pressure "
+                        "gradient kernel, RK update + EOS kernel)" % n,
             "ms_per_stage": round(stage_ms, 4), "value": round(cs / (ms_step * 1e-3) / 1e9, 4),
             "unit": "Gcell-stage/s",
             "roofline": {"bound": "hbm", "algorithmic_bytes_per_cell_stage": balg,

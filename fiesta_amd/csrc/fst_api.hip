@@ -216,6 +216,11 @@ int compute_seq(fst_ctx *ctx, bool fields = false) {
   return FST_OK;
 }
 
+// gen3d on the fused kernels (fused_gen_steps): FP64, WENO5, ns <= 3
+bool gen_fused(const fst_config &c, const Geo &g) {
+  return c.grid == 1 && c.ndim == 3 && c.precision == FST_FP64 && gen_adv_supported(c, g);
+}
+
 int learned_only(fst_ctx *ctx, const char *what) {
   return fail(ctx, FST_ERR_ARG, std::string(what) + ": the learned C-indicator runs through fst_step only");
 }
@@ -374,7 +379,7 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
     (void)fst_destroy(ctx);
     return FST_ERR_NOMEM;
   }
-  ctx->fused = fused_supported(c) || fused2d_supported(c) ? 1 : 0;
+  ctx->fused = fused_supported(c) || fused2d_supported(c) || gen_fused(c, g) ? 1 : 0;
   ctx->b.prof = &ctx->prof;
   *out = ctx;
   return FST_OK;
@@ -481,10 +486,10 @@ int fst_post_step(fst_ctx *ctx) {
 
 int fst_set_fused(fst_ctx *ctx, int fused) {
   if (!ctx) return FST_ERR_ARG;
-  if (fused && !fused_supported(ctx->cf) && !fused2d_supported(ctx->cf))
+  if (fused && !fused_supported(ctx->cf) && !fused2d_supported(ctx->cf) && !gen_fused(ctx->cf, ctx->g))
     return fail(ctx, FST_ERR_ARG,
                 "the fused path supports cartesian WENO5 configurations with ns <= 3 (2D: FP64) and every array "
-                "below 2^32 elements (3D)");
+                "below 2^32 elements (3D), and gen3d with WENO5 and ns <= 3");
   if (!fused && ctx->cf.learned_c && ctx->g.ndim == 3) return learned_only(ctx, "fst_set_fused(0)");
   if (!fused && ctx->cf.precision != FST_FP64) return fp32_only(ctx, "fst_set_fused(0)");
   ctx->fused = fused ? 1 : 0;
@@ -495,9 +500,12 @@ int fst_step(fst_ctx *ctx, int nsteps) {
   if (!ctx || nsteps < 0) return FST_ERR_ARG;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (ctx->fused) {
-    int rc = ctx->g.ndim == 2
-                 ? fused2d_steps(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->fb, &ctx->lb, nsteps, ctx->stream, &ctx->err)
-                 : fused_steps(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->fb, &ctx->lb, nsteps, ctx->stream, &ctx->err);
+    int rc = ctx->g.ndim == 2 ? fused2d_steps(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->fb, &ctx->lb, nsteps,
+                                              ctx->stream, &ctx->err)
+             : ctx->cf.grid == 1 ? fused_gen_steps(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->fb, nsteps, ctx->stream,
+                                                   &ctx->err)
+                                 : fused_steps(ctx->g, ctx->ph, ctx->cf, ctx->b, ctx->fb, &ctx->lb, nsteps,
+                                               ctx->stream, &ctx->err);
     if (rc) return rc;
   } else {
     ctx->fb.valid = false;

@@ -30,8 +30,15 @@ int fused2d_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, F
 // contravariant velocity in the 5 planes of P, metric terms MT);
 // gen_adv_supported says whether the configuration fits.
 bool gen_adv_supported(const fst_config &cf, const Geo &g);
+// buoy: add the gen_ext buoyancy after the pressure gradient in the same
+// launch (else k_buoy_gen does, as the reference-sequence order has it).
 void launch_gen_adv(const Geo &g, const Phys &ph, const fst_config &cf, const double *U, const double *P,
-                    const double *MT, double *dvar, hipStream_t s);
+                    const double *MT, double *dvar, hipStream_t s, int buoy = 0);
+// nsteps time steps of gen3d on the fused kernels (gen_adv_supported
+// configurations): per stage one EOS / contravariant-velocity pass and one
+// advection + stage-update launch (fst_gen3d.hip).
+int fused_gen_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb, int nsteps,
+                    hipStream_t s, std::string *err);
 // nsteps time steps of the fused path (RK2 midpoint, or SSP-RK3 with
 // rk_order = 3); lb carries the learned C-indicator fields when cf.learned_c
 // is set (else may be null).

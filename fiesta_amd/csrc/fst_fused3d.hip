@@ -662,7 +662,9 @@ __device__ __forceinline__ void face_adv_lds(const FC &k, const double *qs, int 
 
 // PG = false (the generalized-grid module): the fluxes, their divergence and
 // applyGenPressureGradient3D (src/presgrad.hpp:72-147) on the metric terms MT
-// (9 planes, m(i,j,k,a,n) at MT + N (a + 3 n)); gen3d has no buoyancy.
+// (9 planes, m(i,j,k,a,n) at MT + N (a + 3 n)); gen3d has no buoyancy of its
+// own (buoy there: the gen_ext option's, after the pressure gradient as
+// k_buoy_gen orders it).
 template <int NS, bool F32, bool PG>
 __device__ __forceinline__ void adv_tile(const Grid32 &g, const FC &k, const double *__restrict__ U,
                                          const double *__restrict__ P, double *__restrict__ dvar, int buoy,
@@ -764,7 +766,7 @@ __device__ __forceinline__ void adv_tile(const Grid32 &g, const FC &k, const dou
           dv[n] = dv[n] - (dxi + det + dzt);
         }
       }
-      if (PG && buoy) {
+      if (buoy) {
         const double rho = pget<F32>(P, N, 0, c);
         const double vv = qs[1 * QPL + rc * QX + qc] / rho;
         const double rhop = rho - k.rhoRef;
@@ -790,11 +792,13 @@ __global__ void __launch_bounds__(NTHREADS, 1)
 
 // gen3d_func::compute's fluxes, their divergence and the metric pressure
 // gradient (src/gen3d.cpp:352-366) in one launch
+// (buoy: the gen_ext buoyancy in the same launch, for the fused gen3d steps)
 template <int NS>
 __global__ void __launch_bounds__(NTHREADS, 1)
     k_adv3d_gen(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
-                double *__restrict__ dvar, const double *__restrict__ MT, int kchunk, int nbx, int nby, int nbz) {
-  adv_tile<NS, false, false>(g, k, U, P, dvar, 0, kchunk, nbx, nby, nbz, MT);
+                double *__restrict__ dvar, const double *__restrict__ MT, int buoy, int kchunk, int nbx, int nby,
+                int nbz) {
+  adv_tile<NS, false, false>(g, k, U, P, dvar, buoy, kchunk, nbx, nby, nbz, MT);
 }
 
 // ---------------------------------------------------------------- kernel 1, FP32 forms, float plane
@@ -2017,16 +2021,166 @@ bool pick(int ns, bool f32, bool lc, StageFn *sf, PrimFn *pf) {
 // k_advect3d + k_genpresgrad3d, so the result is bit for bit the reference
 // sequence's.
 void launch_gen_adv(const Geo &g, const Phys &ph, const fst_config &cf, const double *U, const double *P,
-                    const double *MT, double *dvar, hipStream_t s) {
+                    const double *MT, double *dvar, hipStream_t s, int buoy) {
   const Grid32 q = make_grid(g);
   const FC k = make_fc(ph, cf);
   constexpr int kchunk = 8;
   const Launch L = shape(g, kchunk);
   switch (cf.ns) {
-    case 1: k_adv3d_gen<1><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, MT, kchunk, L.nbx, L.nby, L.nbz); break;
-    case 2: k_adv3d_gen<2><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, MT, kchunk, L.nbx, L.nby, L.nbz); break;
-    default: k_adv3d_gen<3><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, MT, kchunk, L.nbx, L.nby, L.nbz); break;
+    case 1: k_adv3d_gen<1><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, MT, buoy, kchunk, L.nbx, L.nby, L.nbz); break;
+    case 2: k_adv3d_gen<2><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, MT, buoy, kchunk, L.nbx, L.nby, L.nbz); break;
+    default: k_adv3d_gen<3><<<L.nb, NTHREADS, 0, s>>>(q, k, U, P, dvar, MT, buoy, kchunk, L.nbx, L.nby, L.nbz); break;
   }
+}
+
+namespace {
+
+// calculateRhoPT3D (src/secondary.hpp:109-151) and computeGenVelocity3D
+// (src/velocity.hpp:23-45) of cell c of U: rho and p as k_eos_vel3d computes
+// them and the contravariant velocity as k_genvel does (the cartesian
+// velocity k_eos_vel3d writes first is only an intermediate of the reference
+// sequence), into the 5 planes of P.  u: the cell's flow variables.
+template <int NS>
+__device__ __forceinline__ void eos_genvel(const FC &k, long long N, long long c, const double *u,
+                                           const double *__restrict__ MT, double *__restrict__ P) {
+  double rr = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) rr = rr + u[4 + s];
+  double Cp = 0, Cv = 0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double Y = u[4 + s] / rr;
+    Cp = Cp + Y * k.cpf[s];
+    Cv = Cv + Y * k.cvf[s];
+  }
+  const double gamma = Cp / Cv;
+  const double m0 = u[0], m1 = u[1], m2 = u[2];
+  P[c] = rr;
+  P[c + N] = (gamma - 1) * (u[3] - (0.5 / rr) * (m0 * m0 + m1 * m1 + m2 * m2));
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    P[c + N * (2 + d)] = MT[c + N * (d + 0)] * m0 / rr + MT[c + N * (d + 3)] * m1 / rr + MT[c + N * (d + 6)] * m2 / rr;
+}
+
+// eos_genvel over every cell (ghost_only = 0) or over the ghost shell only
+// (after the gen_ext ghost fill rewrote it).
+template <int NS>
+__global__ void k_eos_genvel3d(const Grid32 q, const FC k, const double *__restrict__ U,
+                               const double *__restrict__ MT, double *__restrict__ P, int ghost_only) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t cell = t;
+  if (ghost_only) {
+    if (!ghost_cell(q, t, &cell)) return;
+  } else if (t >= q.N) {
+    return;
+  }
+  const long long N = q.N, c = cell;
+  double u[4 + NS];
+#pragma unroll
+  for (int v = 0; v < 4 + NS; ++v) u[v] = U[c + N * v];
+  eos_genvel<NS>(k, N, c, u, MT, P);
+}
+
+// The Runge-Kutta stage update of a fused gen3d stage over every cell
+// (rkAdvance, src/rk.cpp:28-72: dvar is zero on the ghosts, where the gen
+// modules never write it), Uo = upd(U, Un, dvar), then eos_genvel of the new
+// state: P describes Uo for the next stage.
+template <int NS>
+__global__ void k_rk_eos_gen(const Grid32 q, const FC k, const double *__restrict__ U,
+                             const double *__restrict__ Un, const double *__restrict__ dvar, double *__restrict__ Uo,
+                             const double *__restrict__ MT, double *__restrict__ P, Upd up) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= q.N) return;
+  const int i = (int)(t % (uint32_t)q.ngi), j = (int)((t / (uint32_t)q.ngi) % (uint32_t)q.ngj), kk = (int)(t / q.s2);
+  const bool interior = i >= q.ng && i < q.ng + q.nci && j >= q.ng && j < q.ng + q.ncj && kk >= q.ng &&
+                        kk < q.ng + q.nck;
+  const long long N = q.N, c = t;
+  double u[4 + NS];
+#pragma unroll
+  for (int v = 0; v < 4 + NS; ++v) {
+    const long long e = c + N * v;
+    const double us = up.form != 2 ? U[e] : 0.0;
+    const double ub = up.form != 1 ? Un[e] : 0.0;
+    u[v] = upd_val(up, us, ub, interior ? dvar[e] : 0.0);
+    Uo[e] = u[v];
+  }
+  eos_genvel<NS>(k, N, c, u, MT, P);
+}
+
+template <int NS>
+void gen_stage(const Grid32 &q, const FC &k, const Launch &L, int kchunk, const double *Uin, const double *Un,
+               double *Uout, const double *MT, double *P, double *dvar, Upd up, int eos, int buoy, uint32_t ngh,
+               Prof *prof, hipStream_t s) {
+  if (eos == 2) k_eos_genvel3d<NS><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, P, 0);
+  else if (eos == 1) k_eos_genvel3d<NS><<<(ngh + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, P, 1);
+  if (prof) prof->mark(s);
+  k_adv3d_gen<NS><<<L.nb, NTHREADS, 0, s>>>(q, k, Uin, P, dvar, MT, buoy, kchunk, L.nbx, L.nby, L.nbz);
+  if (prof) prof->mark(s);
+  k_rk_eos_gen<NS><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, Un, dvar, Uout, MT, P, up);
+}
+
+}  // namespace
+
+// Fused gen3d steps.  Per Runge-Kutta stage: the gen_ext ghost fill (when
+// set) and the primitives of the cells it rewrote, k_adv3d_gen (fluxes,
+// metric pressure gradient and the gen_ext buoyancy in one launch), then
+// k_rk_eos_gen (the stage update of every cell and the primitives of the new
+// state, which the next stage reads).  Stage states as the cartesian fused
+// path keeps them: RK2 U* in fb.stage, U^{n+1} in place over U^n (read
+// pointwise); SSP-RK3 U1, U2 in fb.stage / fb.stage2.  Bit for bit the
+// reference sequence (compute + rkAdvance) on every cell.  fb.valid: b.prim
+// holds the primitives of var.
+int fused_gen_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b, FusedBufs &fb, int nsteps,
+                    hipStream_t s, std::string *err) {
+  if (!gen_adv_supported(cf, g) || !b.prim || !b.metrics) {
+    *err = "fused gen3d path: configuration not supported";
+    return FST_ERR_ARG;
+  }
+  const bool rk3 = rk_stages(cf) == 3;
+  const size_t bytes = (size_t)g.N * g.nvt * sizeof(double);
+  if (!fb.stage && hipMalloc((void **)&fb.stage, bytes) != hipSuccess) {
+    *err = "fused gen3d path: device allocation failed";
+    return FST_ERR_NOMEM;
+  }
+  if (rk3 && !fb.stage2 && hipMalloc((void **)&fb.stage2, bytes) != hipSuccess) {
+    *err = "fused gen3d path: device allocation failed";
+    return FST_ERR_NOMEM;
+  }
+  const Grid32 q = make_grid(g);
+  const FC k = make_fc(ph, cf);
+  constexpr int kchunk = 8;
+  const Launch L = shape(g, kchunk);
+  const int buoy = cf.gen_ext && cf.buoyancy;
+  const uint32_t ngh = 2u * q.ng * q.s2 + 2u * q.ng * (uint32_t)q.ngi * q.nck + 2u * q.ng * (uint32_t)q.ncj * q.nck;
+  Prof *prof = b.prof && b.prof->on ? b.prof : nullptr;
+  if (prof) {
+    prof->name = "gen3d stage: [ghost fill + ghost primitives] + k_adv3d_gen + k_rk_eos_gen";
+    prof->per = 4;  // begin | ghost fill + primitives | advection | update + primitives
+  }
+  auto stage_run = [&](double *Uin, double *Uout, Upd up) {
+    if (prof) prof->begin(s);
+    if (cf.gen_ext) launch_apply_bcs(g, cf, Uin, s);  // src/cart3d.cpp:221, as gen_ext applies it
+    // P describes Uin unless the fill rewrote its ghosts (or var changed)
+    const int eos = !fb.valid ? 2 : (cf.gen_ext ? 1 : 0);
+    switch (cf.ns) {
+      case 1: gen_stage<1>(q, k, L, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, ngh, prof, s); break;
+      case 2: gen_stage<2>(q, k, L, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, ngh, prof, s); break;
+      default: gen_stage<3>(q, k, L, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, ngh, prof, s); break;
+    }
+    fb.valid = true;
+    if (prof) prof->end(s);
+  };
+  for (int t = 0; t < nsteps; ++t) {
+    if (!rk3) {
+      stage_run(b.var, fb.stage, Upd{0.5f * cf.dt, 0.0, 0.0, 1});
+      stage_run(fb.stage, b.var, Upd{cf.dt, 0.0, 0.0, 2});
+    } else {
+      stage_run(b.var, fb.stage, Upd{cf.dt, 0.0, 0.0, 1});
+      stage_run(fb.stage, fb.stage2, Upd{cf.dt, RK3_A2, RK3_B2, 3});
+      stage_run(fb.stage2, b.var, Upd{cf.dt, RK3_A3, RK3_B3, 3});
+    }
+  }
+  return hipGetLastError() == hipSuccess ? FST_OK : FST_ERR_HIP;
 }
 
 #else

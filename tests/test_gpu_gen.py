@@ -95,10 +95,16 @@ def test_gen_steps_and_post_step(name):
 
 
 def test_gen_rejects_fused_and_learned():
-    cfg = _cfg(3, (8, 8, 8), 2)
-    m = Module(cfg)
-    with pytest.raises(FiestaError):
-        m.set_fused(True)
+    """The fused path covers gen3d with ns <= 3 only: gen2d and 4 species
+    are refused, as is the learned C-indicator on generalized grids."""
+    for cfg in (_cfg(2, (8, 8, 1), 2), _cfg(3, (8, 8, 8), 4)):
+        m = Module(cfg)
+        with pytest.raises(FiestaError):
+            m.set_fused(True)
+        m.close()
+    m = Module(_cfg(3, (8, 8, 8), 2))
+    m.set_fused(True)
+    m.set_fused(False)
     m.close()
     bad = _cfg(3, (8, 8, 8), 2)
     bad.learned_c = 1
@@ -220,3 +226,44 @@ def test_terrain_expansion_keeps_species_mass():
     m.close()
     assert np.isfinite(v).all()
     np.testing.assert_allclose(v[sl][4:].sum(axis=(1, 2, 3)), m0, rtol=1e-13, atol=0)
+
+
+FUSED_GEN = {
+    "terrain_ns2_rk2": (dict(ndim=3, n=(20, 14, 11), ns=2, rk=2), False),
+    "terrain_ns3_ragged_rk3": (dict(ndim=3, n=(67, 9, 13), ns=3, rk=3), False),
+    "terrain_ns1_ragged_rk2_ext": (dict(ndim=3, n=(70, 15, 10), ns=1, rk=2), True),
+    "terrain_ns3_rk3_ext": (dict(ndim=3, n=(20, 14, 11), ns=3, rk=3), True),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FUSED_GEN))
+def test_gen3d_fused_steps_equal_sequence_and_oracle(name):
+    """gen3d on the fused kernels (one EOS / contravariant-velocity pass and
+    one advection + stage-update launch per stage, fst_step's default) and on
+    the reference kernel sequence (fst_set_fused(0)): both equal the oracle
+    bit for bit on every cell, ghosts included, after 4 steps; with and
+    without gen_ext (ghost fill + buoyancy), RK2 and SSP-RK3."""
+    _say("case gen3d fused vs sequence vs oracle", name)
+    c, ext = FUSED_GEN[name]
+    if ext:
+        cfg = _ext_cfg(c)
+    else:
+        cfg = _cfg(c["ndim"], c["n"], c["ns"])
+        cfg.rk_order = c["rk"]
+    nodes = W.terrain_grid(cfg)
+    var = W.random_state_all(cfg, seed=17, umag=20.0)
+    o = oracle.Oracle(cfg)
+    o.set(FST_GRID, nodes)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.step(4)
+    want = o.get(FST_VAR)
+    for fused in (True, False):
+        m = Module(cfg)
+        m.upload(FST_GRID, nodes)
+        m.upload(FST_VAR, var)
+        m.pre_sim()
+        m.set_fused(fused)
+        m.step(4)
+        assert_bitwise(m.download(FST_VAR), want, "%s %s var after 4 steps" % (name, "fused" if fused else "sequence"))
+        m.close()
