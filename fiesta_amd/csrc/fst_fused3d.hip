@@ -2107,14 +2107,23 @@ __global__ void k_rk_eos_gen(const Grid32 q, const FC k, const double *__restric
   eos_genvel<NS>(k, N, c, u, MT, P);
 }
 
+// visc: the gen_ext viscous terms (launch_viscgen3d on the stage input, whose
+// rho is P's plane 0) after the advection kernel.
 template <int NS>
 void gen_stage(const Grid32 &q, const FC &k, const Launch &L, int kchunk, const double *Uin, const double *Un,
-               double *Uout, const double *MT, double *P, double *dvar, Upd up, int eos, int buoy, uint32_t ngh,
-               Prof *prof, hipStream_t s) {
+               double *Uout, const double *MT, double *P, double *dvar, Upd up, int eos, int buoy, const Geo &g,
+               const Phys &ph, const Bufs &b, bool visc, uint32_t ngh, Prof *prof, hipStream_t s) {
   if (eos == 2) k_eos_genvel3d<NS><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, P, 0);
   else if (eos == 1) k_eos_genvel3d<NS><<<(ngh + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, P, 1);
   if (prof) prof->mark(s);
   k_adv3d_gen<NS><<<L.nb, NTHREADS, 0, s>>>(q, k, Uin, P, dvar, MT, buoy, kchunk, L.nbx, L.nby, L.nbz);
+  if (visc) {
+    Bufs vb = b;
+    vb.var = const_cast<double *>(Uin);
+    vb.rho = P;
+    vb.dvar = dvar;
+    launch_viscgen3d(g, ph, vb, s);
+  }
   if (prof) prof->mark(s);
   k_rk_eos_gen<NS><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, Un, dvar, Uout, MT, P, up);
 }
@@ -2123,8 +2132,8 @@ void gen_stage(const Grid32 &q, const FC &k, const Launch &L, int kchunk, const 
 
 // Fused gen3d steps.  Per Runge-Kutta stage: the gen_ext ghost fill (when
 // set) and the primitives of the cells it rewrote, k_adv3d_gen (fluxes,
-// metric pressure gradient and the gen_ext buoyancy in one launch), then
-// k_rk_eos_gen (the stage update of every cell and the primitives of the new
+// metric pressure gradient and the gen_ext buoyancy in one launch), the
+// gen_ext viscous terms (when set), then k_rk_eos_gen (the stage update of every cell and the primitives of the new
 // state, which the next stage reads).  Stage states as the cartesian fused
 // path keeps them: RK2 U* in fb.stage, U^{n+1} in place over U^n (read
 // pointwise); SSP-RK3 U1, U2 in fb.stage / fb.stage2.  Bit for bit the
@@ -2151,10 +2160,11 @@ int fused_gen_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b,
   constexpr int kchunk = 8;
   const Launch L = shape(g, kchunk);
   const int buoy = cf.gen_ext && cf.buoyancy;
+  const bool visc = cf.gen_ext && cf.visc && b.stress;
   const uint32_t ngh = 2u * q.ng * q.s2 + 2u * q.ng * (uint32_t)q.ngi * q.nck + 2u * q.ng * (uint32_t)q.ncj * q.nck;
   Prof *prof = b.prof && b.prof->on ? b.prof : nullptr;
   if (prof) {
-    prof->name = "gen3d stage: [ghost fill + ghost primitives] + k_adv3d_gen + k_rk_eos_gen";
+    prof->name = "gen3d stage: [ghost fill + ghost primitives] + k_adv3d_gen [+ viscous] + k_rk_eos_gen";
     prof->per = 4;  // begin | ghost fill + primitives | advection | update + primitives
   }
   auto stage_run = [&](double *Uin, double *Uout, Upd up) {
@@ -2163,9 +2173,18 @@ int fused_gen_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b,
     // P describes Uin unless the fill rewrote its ghosts (or var changed)
     const int eos = !fb.valid ? 2 : (cf.gen_ext ? 1 : 0);
     switch (cf.ns) {
-      case 1: gen_stage<1>(q, k, L, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, ngh, prof, s); break;
-      case 2: gen_stage<2>(q, k, L, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, ngh, prof, s); break;
-      default: gen_stage<3>(q, k, L, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, ngh, prof, s); break;
+      case 1:
+        gen_stage<1>(q, k, L, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, g, ph, b, visc,
+                     ngh, prof, s);
+        break;
+      case 2:
+        gen_stage<2>(q, k, L, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, g, ph, b, visc,
+                     ngh, prof, s);
+        break;
+      default:
+        gen_stage<3>(q, k, L, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, g, ph, b, visc,
+                     ngh, prof, s);
+        break;
     }
     fb.valid = true;
     if (prof) prof->end(s);

@@ -163,6 +163,9 @@ inline double key_to_double(unsigned long long k) {
 void launch_secondary(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
 // generalized grids (gen2d/gen3d modules)
 void launch_compute_gen(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);
+// gen_ext viscous terms of gen3d in metric form (face fluxes through b.stress,
+// then their differences into b.dvar), per direction x, y, z.
+void launch_viscgen3d(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
 void launch_metrics(const Geo &g, const Bufs &b, hipStream_t s);
 
 }  // namespace fst

@@ -1329,11 +1329,133 @@ __global__ void k_buoy_gen(Geo g, Phys ph, Range r, const double *__restrict__ v
   }
 }
 
+// The cartesian velocity var(w) / rho of every cell into u0, u1, u2 (the
+// input of the gen_ext viscous terms; the same quotients they would form).
+__global__ void k_cartvel3d(long long n, long long N, const double *__restrict__ var, const double *__restrict__ rho,
+                            double *__restrict__ u0, double *__restrict__ u1, double *__restrict__ u2) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double r = rho[c];
+  u0[c] = var[c] / r;
+  u1[c] = var[c + N] / r;
+  u2[c] = var[c + 2 * N] / r;
+}
+
+// gen_ext viscous terms on a generalized 3D grid (not in the reference: the
+// cartesian split of src/viscosity.hpp:416-663 in index space, with the chain
+// rule through the metric terms m(a,n) = d xi_a / d x_n as
+// applyGenPressureGradient3D applies it, src/presgrad.hpp:72-147).  On the
+// face between c and c + e_dir: index-space derivatives g[w][a] of the
+// cartesian velocity u_w = var(w) / rho (normal: u(c+e) - u(c); tangential: the 4-point
+// difference / 4), face metrics mf(a,n) = (m(c) + m(c+e)) / 2,
+// du_w/dx_n = (mf(0,n) g[w][0] + mf(1,n) g[w][1]) + mf(2,n) g[w][2], the
+// stress of calculateStressTensor, and the contravariant fluxes through the
+// face: momentum F_w = sum_n mf(dir,n) tau_wn and energy
+// H_w = sum_n mf(dir,n) (uf_w tau_wn), uf the face velocity (out[0..5]).
+// On a uniform grid of power-of-two spacing each value is the cartesian
+// one scaled by a power of two, so the module equals cart3d bit for bit.
+__device__ __forceinline__ void viscgen3d_face(const Geo &g, const Phys &ph, int dir, int i, int j, int k,
+                                               const double *__restrict__ var, const double *__restrict__ rho,
+                                               const double *__restrict__ u0, const double *__restrict__ u1,
+                                               const double *__restrict__ u2, const double *__restrict__ mt,
+                                               double *out) {
+  const long long N = g.N;
+  const int ip = dir == 0, jp = dir == 1, kp = dir == 2;
+  const double *const uw[3] = {u0, u1, u2};
+  auto U = [&](int a, int b, int cc, int w) { return uw[w][g.id(a, b, cc)]; };
+  const long long c = g.id(i, j, k), cn = g.id(i + ip, j + jp, k + kp);
+  double mu = 0.0;
+  for (int s = 0; s < g.ns; ++s)
+    mu += ((var[cn + N * (4 + s)] / rho[cn] + var[c + N * (4 + s)] / rho[c]) / 2.0) * ph.mus[s];
+  double gr[3][3], mf[3][3], du[3][3], uf[3];
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (a == dir) {
+        gr[w][a] = U(i + ip, j + jp, k + kp, w) - U(i, j, k, w);
+      } else {
+        const int ta = a == 0, tb = a == 1, tc = a == 2;
+        gr[w][a] = ((U(i + ta, j + tb, k + tc, w) + U(i + ip + ta, j + jp + tb, k + kp + tc, w)) -
+                    (U(i - ta, j - tb, k - tc, w) + U(i + ip - ta, j + jp - tb, k + kp - tc, w))) /
+                   4.0;
+      }
+    }
+    uf[w] = (U(i + ip, j + jp, k + kp, w) + U(i, j, k, w)) / 2.0;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int n = 0; n < 3; ++n) mf[a][n] = (mt[c + N * (a + 3 * n)] + mt[cn + N * (a + 3 * n)]) / 2.0;
+#pragma unroll
+  for (int w = 0; w < 3; ++w)
+#pragma unroll
+    for (int n = 0; n < 3; ++n) du[w][n] = (mf[0][n] * gr[w][0] + mf[1][n] * gr[w][1]) + mf[2][n] * gr[w][2];
+  const double dudx = du[0][0], dudy = du[0][1], dudz = du[0][2];
+  const double dvdx = du[1][0], dvdy = du[1][1], dvdz = du[1][2];
+  const double dwdx = du[2][0], dwdy = du[2][1], dwdz = du[2][2];
+  double t[3][3];
+  t[0][0] = (2.0 / 3.0) * mu * (2.0 * dudx - dvdy - dwdz);
+  t[1][1] = (2.0 / 3.0) * mu * (2.0 * dvdy - dudx - dwdz);
+  t[2][2] = (2.0 / 3.0) * mu * (2.0 * dwdz - dudx - dvdy);
+  t[0][1] = t[1][0] = mu * (dudy + dvdx);
+  t[0][2] = t[2][0] = mu * (dudz + dwdx);
+  t[1][2] = t[2][1] = mu * (dvdz + dwdy);
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    out[w] = (mf[dir][0] * t[w][0] + mf[dir][1] * t[w][1]) + mf[dir][2] * t[w][2];
+    out[3 + w] = (mf[dir][0] * (uf[w] * t[w][0]) + mf[dir][1] * (uf[w] * t[w][1])) + mf[dir][2] * (uf[w] * t[w][2]);
+  }
+}
+
+// The face values of direction dir over the face range into st(0..5) ...
+__global__ void k_viscgen3d_face(Geo g, Phys ph, Range r, int dir, const double *__restrict__ var,
+                                 const double *__restrict__ rho, const double *__restrict__ u0,
+                                 const double *__restrict__ u1, const double *__restrict__ u2,
+                                 const double *__restrict__ mt, double *__restrict__ st) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  double out[6];
+  viscgen3d_face(g, ph, dir, i, j, k, var, rho, u0, u1, u2, mt, out);
+  const long long N = g.N, c = g.id(i, j, k);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) st[c + N * q] = out[q];
+}
+
+// ... and their differences over the cell range: dvar(w) += F_w(c) - F_w(c-e),
+// dvar(3) += sum_w (H_w(c) - H_w(c-e)) (applyViscousTerm's accumulation
+// order, src/viscosity.hpp:460-498).
+__global__ void k_viscgen3d_apply(Geo g, Range r, int dir, const double *__restrict__ st,
+                                  double *__restrict__ dvar) {
+  int i, j, k;
+  if (!decode(r, i, j, k)) return;
+  const long long N = g.N;
+  const long long c = g.id(i, j, k), cm = g.id(i - (dir == 0), j - (dir == 1), k - (dir == 2));
+  auto S = [&](long long q, int s) { return st[q + N * s]; };
+  dvar[c] += S(c, 0) - S(cm, 0);
+  dvar[c + N] += S(c, 1) - S(cm, 1);
+  dvar[c + 2 * N] += S(c, 2) - S(cm, 2);
+  dvar[c + 3 * N] += ((S(c, 3) - S(cm, 3)) + (S(c, 4) - S(cm, 4))) + (S(c, 5) - S(cm, 5));
+}
+
+// Directions x, y, z in that order (src/cart3d.cpp:265-270); fx, fy, fz (the
+// per-variable flux planes, free once the fluxes are differenced) hold the
+// cartesian velocity, b.stress the face values.
+void launch_viscgen3d(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s) {
+  const Range fr = face_range(g), cr = cell_range(g);
+  k_cartvel3d<<<(unsigned)((g.N + 255) / 256), 256, 0, s>>>(g.N, g.N, b.var, b.rho, b.fx, b.fy, b.fz);
+  for (int d = 0; d < 3; ++d) {
+    k_viscgen3d_face<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, d, b.var, b.rho, b.fx, b.fy, b.fz, b.metrics, b.stress);
+    k_viscgen3d_apply<<<grid_for(cr), 256, 0, s>>>(g, cr, d, b.stress, b.dvar);
+  }
+}
+
 // gen3d_func::compute (src/gen3d.cpp:341-369) / gen2d_func::compute
 // (src/gen2d.cpp:100-131): primitives, contravariant velocity, per-variable
 // fluxes and flux differences, metric pressure gradient.  No boundary
 // conditions and no viscous, buoyancy or C-equation terms (gen_ext adds the
-// ghost fill, in compute_seq / preStep, and the buoyancy).  The 3D flux
+// ghost fill, in compute_seq / preStep, the buoyancy and, in 3D, the viscous
+// terms in metric form).  The 3D flux
 // kernel's dz is cf.dx (src/gen3d.cpp:359); both are 1 on these grids.
 void launch_compute_gen(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s) {
   launch_eos_vel(g, ph, b, s);
@@ -1363,6 +1485,9 @@ void launch_compute_gen(const Geo &g, const Phys &ph, const fst_config &cf, cons
   // gen_ext: the cartesian modules' buoyancy after the pressure gradient, as
   // they order it (src/cart3d.cpp:268-270, src/cart2d.cpp:186-188)
   if (cf.gen_ext && cf.buoyancy) k_buoy_gen<<<grid_for(cr), 256, 0, s>>>(g, ph, cr, b.var, b.rho, b.dvar);
+  // gen_ext: the viscous terms in metric form, after the buoyancy as cart3d
+  // orders them (src/cart3d.cpp:265-270)
+  if (cf.gen_ext && cf.visc && g.ndim == 3) launch_viscgen3d(g, ph, b, s);
 }
 
 // gen3d_func::preSim / gen2d_func::preSim: metrics of the interior cells, then

@@ -163,18 +163,20 @@ def cell_centres(cfg, nodes):
     return tuple(out)
 
 
-def terrain_expansion_config(n=(512, 256, 512), ns=3, rk=3, dt=None):
+def terrain_expansion_config(n=(512, 256, 512), ns=3, rk=3, dt=None, visc=True):
     """Ideal expansion over terrain on a generalized grid (gen3d with the
     gen_ext option): n = (nci, ncj, nck) cells, y vertical (terrain_grid with
-    vertical=1), air, hot gas and SF6, gravity 9.81 along -y, periodic in x
-    and z, a no-slip ground (ymin) and a no-slip lid (ymax), SSP-RK3.
+    vertical=1), air, hot gas and SF6, gravity 9.81 along -y, viscous (the
+    gen_ext viscous terms in metric form, mu = 1.8e-5), periodic in x and z,
+    a no-slip ground (ymin) and a no-slip lid (ymax), SSP-RK3.
     Closed in y and periodic in x and z, the box keeps every species' mass:
     the index-space no-slip fill mirrors every momentum component and the
     metric ghosts mirror the metrics, so the contravariant face velocity at
     both walls is zero.  (A reflective wall flips only the y momentum, which
     leaves a small face velocity where the grid lines meet it at an angle.)"""
     return make_config(ndim=3, ns=ns, nci=n[0], ncj=n[1], nck=n[2], grid=1, gen_ext=True, R=RU,
-                       **_mix(ns, [AIR, HOT, SF6]), buoyancy=True, g=9.81, rhoRef=0.0, xper=True, zper=True,
+                       **_mix(ns, [AIR, HOT, SF6]), visc=visc, buoyancy=True, g=9.81, rhoRef=0.0, xper=True,
+                       zper=True,
                        bc=("outflow", "outflow", "noslip", "noslip", "outflow", "outflow"), rk=rk,
                        dt=dt if dt is not None else 1e-7)
 

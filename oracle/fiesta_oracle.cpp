@@ -1389,12 +1389,97 @@ void features2d(Orc &o, float *F, float *M, float *Dir, float *G1) {
     }
 }
 
+// gen_ext viscous terms on a generalized grid (the reference's gen3d has none;
+// the cartesian split of src/viscosity.hpp:416-663 in index space, with the
+// chain rule through the metric terms m(a,n) = d xi_a / d x_n as
+// applyGenPressureGradient3D applies it, src/presgrad.hpp:72-147).  On the
+// face between c and c + e_D (stored at c, over the cartesian stress range):
+//   g[w][a]  index-space derivatives of the cartesian velocity u_w = var(w)/rho:
+//            a = D: u(c+e) - u(c); a != D: the 4-point difference / 4
+//   mf(a,n)  (m(c,a,n) + m(c+e,a,n)) / 2
+//   du_w/dx_n = (mf(0,n) g[w][0] + mf(1,n) g[w][1]) + mf(2,n) g[w][2]
+//   tau      from du/dx as calculateStressTensor (mass-fraction mu)
+//   F_w      = (mf(D,0) tau_w0 + mf(D,1) tau_w1) + mf(D,2) tau_w2     (momentum)
+//   H_w      = (mf(D,0) (uf_w tau_w0) + mf(D,1) (uf_w tau_w1)) + mf(D,2) (uf_w tau_w2)
+//            with uf the face velocity (u(c+e) + u(c)) / 2           (energy)
+// then dvar(w) += F_w(c) - F_w(c-e), dvar(3) += sum_w (H_w(c) - H_w(c-e)).
+// On a uniform grid of power-of-two spacing (mf = diag(1/d)) every step is
+// the cartesian operation scaled by a power of two, so the result equals the
+// cartesian module's bit for bit.  F and H go through o.stress (6 planes).
+void viscgen3d(Orc &o, int dir) {
+  const int ng = o.ng, ns = o.ns;
+  const size_t N = o.N;
+  const int ip = dir == 0, jp = dir == 1, kp = dir == 2;
+  auto U = [&](int a, int b, int cc, int w) {
+    const size_t q = o.id(a, b, cc);
+    return o.var[q + N * w] / o.rho[q];
+  };
+#pragma omp parallel for
+  for (int k = ng - 1; k < o.ngk - ng; ++k)
+    for (int j = ng - 1; j < o.ngj - ng; ++j)
+      for (int i = ng - 1; i < o.ngi - ng; ++i) {
+        double mu = 0.0;
+        for (int s = 0; s < ns; ++s) {
+          const size_t q1 = o.id(i + ip, j + jp, k + kp), q0 = o.id(i, j, k);
+          mu += ((o.var[q1 + N * (4 + s)] / o.rho[q1] + o.var[q0 + N * (4 + s)] / o.rho[q0]) / 2.0) *
+                o.cd[6 + 3 * s + 2];
+        }
+        double g[3][3], mf[3][3], du[3][3], uf[3];
+        for (int w = 0; w < 3; ++w) {
+          for (int a = 0; a < 3; ++a) {
+            if (a == dir) {
+              g[w][a] = U(i + ip, j + jp, k + kp, w) - U(i, j, k, w);
+            } else {
+              const int ta = a == 0, tb = a == 1, tc = a == 2;
+              g[w][a] = ((U(i + ta, j + tb, k + tc, w) + U(i + ip + ta, j + jp + tb, k + kp + tc, w)) -
+                         (U(i - ta, j - tb, k - tc, w) + U(i + ip - ta, j + jp - tb, k + kp - tc, w))) /
+                        4.0;
+            }
+          }
+          uf[w] = (U(i + ip, j + jp, k + kp, w) + U(i, j, k, w)) / 2.0;
+        }
+        for (int a = 0; a < 3; ++a)
+          for (int n = 0; n < 3; ++n) mf[a][n] = (MT(o, i, j, k, a, n) + MT(o, i + ip, j + jp, k + kp, a, n)) / 2.0;
+        for (int w = 0; w < 3; ++w)
+          for (int n = 0; n < 3; ++n) du[w][n] = (mf[0][n] * g[w][0] + mf[1][n] * g[w][1]) + mf[2][n] * g[w][2];
+        const double dudx = du[0][0], dudy = du[0][1], dudz = du[0][2];
+        const double dvdx = du[1][0], dvdy = du[1][1], dvdz = du[1][2];
+        const double dwdx = du[2][0], dwdy = du[2][1], dwdz = du[2][2];
+        double t[3][3];
+        t[0][0] = (2.0 / 3.0) * mu * (2.0 * dudx - dvdy - dwdz);
+        t[1][1] = (2.0 / 3.0) * mu * (2.0 * dvdy - dudx - dwdz);
+        t[2][2] = (2.0 / 3.0) * mu * (2.0 * dwdz - dudx - dvdy);
+        t[0][1] = t[1][0] = mu * (dudy + dvdx);
+        t[0][2] = t[2][0] = mu * (dudz + dwdx);
+        t[1][2] = t[2][1] = mu * (dvdz + dwdy);
+        const size_t c = o.id(i, j, k);
+        for (int w = 0; w < 3; ++w) {
+          o.stress[c + N * w] = (mf[dir][0] * t[w][0] + mf[dir][1] * t[w][1]) + mf[dir][2] * t[w][2];
+          o.stress[c + N * (3 + w)] =
+              (mf[dir][0] * (uf[w] * t[w][0]) + mf[dir][1] * (uf[w] * t[w][1])) + mf[dir][2] * (uf[w] * t[w][2]);
+        }
+      }
+#pragma omp parallel for
+  for (int k = ng; k < o.ngk - ng; ++k)
+    for (int j = ng; j < o.ngj - ng; ++j)
+      for (int i = ng; i < o.ngi - ng; ++i) {
+        const size_t c = o.id(i, j, k), cm = o.id(i - ip, j - jp, k - kp);
+        auto S = [&](size_t q, int s) { return o.stress[q + N * s]; };
+        o.dvar[c + N * 0] += S(c, 0) - S(cm, 0);
+        o.dvar[c + N * 1] += S(c, 1) - S(cm, 1);
+        o.dvar[c + N * 2] += S(c, 2) - S(cm, 2);
+        o.dvar[c + N * 3] += ((S(c, 3) - S(cm, 3)) + (S(c, 4) - S(cm, 4))) + (S(c, 5) - S(cm, 5));
+      }
+}
+
 // gen3d_func::compute, src/gen3d.cpp:341-369: no boundary conditions, no
 // viscous, buoyancy or C-equation terms; calculateFluxesG receives cf.dx as
 // its dz argument (:359), and cf.dx = cf.dz = 1 on generalized grids.
 void compute_gen3d(Orc &o) {
   // gen_ext (an option the reference does not have): cart3d's ghost fill
-  // before the compute and its buoyancy after the pressure gradient
+  // before the compute, its buoyancy after the pressure gradient and its
+  // viscous terms after that (src/cart3d.cpp:221, 268-270, 265-270), the
+  // latter in metric form (viscgen3d)
   if (o.c.gen_ext) apply_bcs(o);
   eos3d(o);
   genvel3d(o);
@@ -1404,6 +1489,8 @@ void compute_gen3d(Orc &o) {
   }
   genpresgrad3d(o);
   if (o.c.gen_ext && o.c.buoyancy) buoyancy3d(o);
+  if (o.c.gen_ext && o.c.visc)
+    for (int d = 0; d < 3; ++d) viscgen3d(o, d);
 }
 
 // gen2d_func::compute, src/gen2d.cpp:100-131

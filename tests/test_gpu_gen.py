@@ -184,7 +184,7 @@ def test_gen_ext_uniform_grid_equals_cartesian_module(ndim):
 
 def test_terrain_expansion_against_oracle():
     """The terrain expansion workload (terrain_expansion_config: gen3d +
-    gen_ext, 3 species, gravity, no-slip ground and lid, SSP-RK3) at
+    gen_ext, 3 species, gravity, viscous, no-slip ground and lid, SSP-RK3) at
     40 x 20 x 40 cells: 12 steps of the blast on the GPU equal the oracle's
     bit for bit on every cell."""
     _say("case terrain expansion 40x20x40 vs oracle, 12 steps")
@@ -233,6 +233,9 @@ FUSED_GEN = {
     "terrain_ns3_ragged_rk3": (dict(ndim=3, n=(67, 9, 13), ns=3, rk=3), False),
     "terrain_ns1_ragged_rk2_ext": (dict(ndim=3, n=(70, 15, 10), ns=1, rk=2), True),
     "terrain_ns3_rk3_ext": (dict(ndim=3, n=(20, 14, 11), ns=3, rk=3), True),
+    # gen_ext with the viscous terms in metric form
+    "terrain_ns2_rk2_ext_visc": (dict(ndim=3, n=(20, 14, 11), ns=2, rk=2, visc=True), True),
+    "terrain_ns3_ragged_rk3_ext_visc": (dict(ndim=3, n=(67, 9, 13), ns=3, rk=3, visc=True), True),
 }
 
 
@@ -242,11 +245,12 @@ def test_gen3d_fused_steps_equal_sequence_and_oracle(name):
     one advection + stage-update launch per stage, fst_step's default) and on
     the reference kernel sequence (fst_set_fused(0)): both equal the oracle
     bit for bit on every cell, ghosts included, after 4 steps; with and
-    without gen_ext (ghost fill + buoyancy), RK2 and SSP-RK3."""
+    without gen_ext (ghost fill + buoyancy, and the viscous terms in metric
+    form), RK2 and SSP-RK3."""
     _say("case gen3d fused vs sequence vs oracle", name)
     c, ext = FUSED_GEN[name]
     if ext:
-        cfg = _ext_cfg(c)
+        cfg = _ext_cfg(c, visc=int(c.get("visc", False)))
     else:
         cfg = _cfg(c["ndim"], c["n"], c["ns"])
         cfg.rk_order = c["rk"]

@@ -24,6 +24,12 @@ wrong node offset, a swapped mirror index or a misplaced term fails:
     cartesian ghost fill and buoyancy on       cart with the same walls and gravity, bit
     generalized grids)                         for bit; fluid at rest on a sheared grid:
                                                RHS = -g (rho - rho_ref) on y momentum only
+  gen_ext viscous terms in metric form         uniform 2^-5 grid: gen + gen_ext + visc steps
+    (viscgen3d)                                as cart + visc, bit for bit; affine (sheared)
+                                               grid: a quadratic velocity field gives the
+                                               momentum RHS mu (lap u + grad div u / 3), a
+                                               linear one zero momentum RHS and the energy
+                                               RHS tau : grad u
 """
 import numpy as np
 import pytest
@@ -300,3 +306,115 @@ def test_terrain_grid_with_vertical_y():
     np.testing.assert_allclose(x[0, 0, :], np.arange(17) / 16, atol=1e-15)
     np.testing.assert_allclose(z[:, 0, 0], np.arange(11) / 10, atol=1e-15)
     assert (np.diff(y, axis=1) > 0).all()
+
+
+@pytest.mark.parametrize("rk", [2, 3])
+def test_gen_ext_viscous_equals_cartesian_on_uniform_power_of_two_grid(rk):
+    """The gen_ext viscous terms (metric form) on a uniform grid of spacing
+    2^-5: the generalized module with gen_ext and visc follows the cartesian
+    module with visc bit for bit over several steps (every metric product is
+    a power-of-two scaling of the cartesian operation), and differs from the
+    same generalized run without the viscous terms."""
+    h = 2.0 ** -5
+    n = (10, 9, 7)
+    bc = ("reflective", "outflow", "noslip", "outflow", "reflective", "outflow")
+    kw = dict(ndim=3, ns=2, nci=n[0], ncj=n[1], nck=n[2], R=RU, gamma=(1.4, 1.67), M=(0.02896, 0.004),
+              mu=(1.8e-2, 2.6e-2), dt=2e-7, buoyancy=True, g=9.81, rhoRef=0.5, bc=bc, zper=True, rk=rk)
+    cart = make_config(dx=h, dy=h, dz=h, visc=True, **kw)
+    gen = make_config(grid=1, gen_ext=True, visc=True, **kw)
+    var = W.random_state(cart, seed=29)
+    oc = oracle.Oracle(cart)
+    oc.set(FST_VAR, var)
+    oc.pre_sim()
+    nodes = W.grid_nodes(gen, lambda i, j, k: (h * i, h * j, h * k))
+    og, _ = _metrics(gen, nodes)
+    og.set(FST_VAR, var)
+    og.pre_sim()
+    oc.step(3)
+    og.step(3)
+    a, b = oc.get(FST_VAR), og.get(FST_VAR)
+    assert np.array_equal(a, b), int((a != b).sum())
+    inv = make_config(grid=1, gen_ext=True, visc=False, **kw)
+    oi, _ = _metrics(inv, nodes)
+    oi.set(FST_VAR, var)
+    oi.pre_sim()
+    oi.step(3)
+    assert not np.array_equal(oi.get(FST_VAR), a)
+
+
+_A3 = np.array([[0.03, 0.004, -0.002], [0.001, 0.025, 0.003], [-0.002, 0.002, 0.035]])
+
+
+def _viscous_rhs_affine(ufun, mu=2.0e-2, rho0=1.3, p0=1e5):
+    """dvar of the gen_ext viscous terms alone (the RHS with visc minus the RHS
+    without) for the velocity field ufun(x, y, z) -> (u, v, w) on the affine
+    grid x = _A3 xi, single species, uniform density and pressure.  Returns
+    (momentum and energy RHS on the cells two away from every boundary, the
+    cell centres there)."""
+    outs = []
+    for visc in (True, False):
+        cfg = make_config(ndim=3, ns=1, nci=9, ncj=8, nck=7, grid=1, gen_ext=True, visc=visc, R=RU,
+                          gamma=(1.4,), M=(0.02896,), mu=(mu,), dt=1e-7, bc=("outflow",) * 6)
+        d = Dims(cfg)
+        nodes = W.grid_nodes(cfg, lambda i, j, k: (_A3[0, 0] * i + _A3[0, 1] * j + _A3[0, 2] * k,
+                                                   _A3[1, 0] * i + _A3[1, 1] * j + _A3[1, 2] * k,
+                                                   _A3[2, 0] * i + _A3[2, 1] * j + _A3[2, 2] * k))
+        x, y, z = W.cell_centres(cfg, nodes)
+        sh = x.shape
+        var = W.fill_conserved(cfg, [np.full(sh, rho0)], list(ufun(x, y, z)), np.full(sh, p0))
+        o, _ = _metrics(cfg, nodes)
+        o.set(FST_VAR, var)
+        o.compute()
+        outs.append(o.get(FST_DVAR)[d.interior()])
+    inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+    return (outs[0] - outs[1])[inner][:4], tuple(c[1:-1, 1:-1, 1:-1] for c in (x, y, z))
+
+
+def test_gen_ext_viscous_quadratic_velocity_affine_grid():
+    """A quadratic velocity field on a sheared, stretched affine grid: the
+    index-space face derivatives, the face metrics and the face-flux
+    differences are exact for it, so the viscous momentum RHS is the
+    analytic mu (lap u + grad(div u) / 3), every entry of grad u and of its
+    Hessian being nonzero (a transposed metric, a wrong tangential divisor or
+    a lost 1/3 fails)."""
+    mu = 2.0e-2
+    B = np.array([[0.3, -0.7, 0.2], [0.5, 0.1, -0.4], [-0.2, 0.6, 0.35]])
+    C = [np.array([[1.1, 0.3, -0.4], [0.3, -0.8, 0.5], [-0.4, 0.5, 0.6]]),
+         np.array([[-0.5, 0.7, 0.2], [0.7, 0.9, -0.3], [0.2, -0.3, -1.2]]),
+         np.array([[0.4, -0.6, 0.8], [-0.6, 0.2, 0.1], [0.8, 0.1, 0.7]])]
+
+    def ufun(x, y, z):
+        X = (x, y, z)
+        out = []
+        for w in range(3):
+            u = 2.0 + sum(B[w, n] * X[n] for n in range(3))
+            u = u + 0.5 * sum(C[w][n, m] * X[n] * X[m] for n in range(3) for m in range(3))
+            out.append(u)
+        return out
+
+    dv, _ = _viscous_rhs_affine(ufun, mu=mu)
+    lap = [np.trace(C[w]) for w in range(3)]
+    graddiv = [sum(C[n][n, w] for n in range(3)) for w in range(3)]
+    for w in range(3):
+        want = mu * (lap[w] + graddiv[w] / 3.0)
+        np.testing.assert_allclose(dv[w], want, rtol=0, atol=1e-9 * abs(want) + 1e-10)
+
+
+def test_gen_ext_viscous_linear_velocity_affine_grid_energy():
+    """A linear velocity field on the affine grid: uniform stress, so the
+    momentum RHS vanishes, and the energy RHS is the work tau : grad u (the
+    face velocity is exact for linear fields)."""
+    mu = 2.0e-2
+    B = np.array([[0.3, -0.7, 0.2], [0.5, 0.1, -0.4], [-0.2, 0.6, 0.35]])
+
+    def ufun(x, y, z):
+        X = (x, y, z)
+        return [1.0 + sum(B[w, n] * X[n] for n in range(3)) for w in range(3)]
+
+    dv, _ = _viscous_rhs_affine(ufun, mu=mu)
+    div = np.trace(B)
+    tau = mu * (B + B.T) - (2.0 / 3.0) * mu * div * np.eye(3)
+    work = float((tau * B).sum())
+    for w in range(3):
+        np.testing.assert_allclose(dv[w], 0.0, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(dv[3], work, rtol=1e-8, atol=0)
