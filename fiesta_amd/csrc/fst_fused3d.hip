@@ -60,6 +60,11 @@
 namespace fst {
 
 namespace {
+#ifdef FST_FAST64_TU
+// the fast FP64 form's kernels carry their own names (fst::(anonymous
+// namespace)::fast64::k_...), apart from the reference-arithmetic ones
+namespace fast64 {
+#endif
 
 // First statement of a function body whose arithmetic the fast FP64 form
 // contracts (a no-op in the reference-arithmetic units).
@@ -2010,6 +2015,10 @@ bool pick(int ns, bool f32, bool lc, StageFn *sf, PrimFn *pf) {
 
 #endif  // FST_GEN3D_TU
 
+#ifdef FST_FAST64_TU
+}  // namespace fast64
+using namespace fast64;
+#endif
 }  // namespace
 
 #ifdef FST_GEN3D_TU

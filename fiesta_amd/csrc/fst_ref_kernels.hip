@@ -1329,6 +1329,8 @@ __global__ void k_buoy_gen(Geo g, Phys ph, Range r, const double *__restrict__ v
   }
 }
 
+namespace {
+
 // The cartesian velocity var(w) / rho of every cell into u0, u1, u2 (the
 // input of the gen_ext viscous terms; the same quotients they would form).
 __global__ void k_cartvel3d(long long n, long long N, const double *__restrict__ var, const double *__restrict__ rho,
@@ -1437,6 +1439,8 @@ __global__ void k_viscgen3d_apply(Geo g, Range r, int dir, const double *__restr
   dvar[c + 2 * N] += S(c, 2) - S(cm, 2);
   dvar[c + 3 * N] += ((S(c, 3) - S(cm, 3)) + (S(c, 4) - S(cm, 4))) + (S(c, 5) - S(cm, 5));
 }
+
+}  // namespace
 
 // Directions x, y, z in that order (src/cart3d.cpp:265-270); fx, fy, fz (the
 // per-variable flux planes, free once the fluxes are differenced) hold the
