@@ -638,6 +638,24 @@ def measured_traffic(size, form):
                                                                                     d.get("command", ""))}
 
 
+def issue_floor(size, kernels):
+    """VALU issue time per stage of the given stage kernels, from the committed
+    rocprofv3 SQ counter summary: SQ_INSTS_VALU per dispatch / (1024 SIMDs)
+    x 4 cycles per wave64 instruction at 2.4 GHz, the time the kernels would
+    take if they did nothing but issue VALU instructions (None when the
+    summary or a kernel is missing)."""
+    path = os.path.join(ROOT, "profiles", "sq_counters_rt3d_%d.json" % size)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {})
+    if not all(n in k for n in kernels):
+        return None
+    per = d["cycles_per_wave_instruction"] / (d["simds"] * d["clock_ghz"] * 1e9) * 1e3
+    return {n: k[n]["SQ_INSTS_VALU"] * per for n in kernels}
+
+
 def form_errors(cfg, got, want, cfg64):
     """Relative L2 / Linf of density, pressure and mass fractions (interior)
     of a form's final state against the FP64 path's final state (the FP64
@@ -727,6 +745,19 @@ def main():
         if tr:
             roof["traffic"] = tr["bytes_per_stage"]
             roof["traffic_source"] = tr["source"]
+        # the FP64 stage kernels' VALU issue time: a floor under the stage
+        # time that no memory optimisation removes (bitwise arithmetic)
+        ns = r["cfg"].ns
+        fl = issue_floor(args.size, ["k_adv3d_lds<%d, false, true>" % ns,
+                                     "k_vcr3d_ring<%d, false, false, false, false>" % ns, "k_ceq3d<false, false>"])
+        if fl and r["fused"]:
+            tot = sum(fl.values())
+            roof["valu_issue_floor"] = {
+                "ms_per_stage": round(tot, 4), "kernels_ms": {n: round(v, 4) for n, v in fl.items()},
+                "stage_over_floor": round(avg_ms / tot, 3),
+                "hbm_frac_at_floor": round(per_launch / (tot * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "source": "profiles/sq_counters_rt3d_%d.json (SQ_INSTS_VALU x 4 cycles / 1024 SIMDs at 2.4 GHz)"
+                          % args.size}
     else:
         ach = d.ncells * balg * r["value"] * 1e9 / d.ncells / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
