@@ -202,7 +202,10 @@ def test_harvest2d_and_training_lowers_loss():
     print("targets: max %.4g min %.4g, %d negative" % (T.max(), T.min(), int((T < 0).sum())))
     assert T.max() > 0
     _, _, before = tr.forward_backward(X, T)
-    loss = tr.train(20, batch=256, lr=1e-3, seed=3)
+    # 40 epochs: after 20 the loss had fallen to about 0.4 of its start
+    # (0.38-0.42 over runs: the harvested rows are appended in the order the
+    # device's atomics give, so the batches differ from run to run)
+    loss = tr.train(40, batch=256, lr=1e-3, seed=3)
     assert np.all(np.isfinite(loss))
     _, _, after = tr.forward_backward(X, T)
     print("rows %d, dataset loss before %.4g after %.4g (target variance %.4g)" % (n, before, after, T.var()))
