@@ -345,7 +345,8 @@ int fst_create(const fst_config *cfg, fst_ctx **out) {
   ok = ok && alloc(&ctx->b.varx, g.nvx) && alloc(&ctx->b.T, 1);
   ok = ok && alloc(&ctx->b.fx, 1) && alloc(&ctx->b.fy, 1);
   ok = ok && alloc(&ctx->b.fz, g.ndim == 3 ? 1 : 0);
-  ok = ok && alloc(&ctx->b.stress, c.visc ? (g.ndim == 3 ? 6 : 8) : 0);
+  // 3D: 6 stress planes; the gen_ext viscous terms keep the mass fractions there
+  ok = ok && alloc(&ctx->b.stress, c.visc ? (g.ndim == 3 ? std::max(6, c.grid == 1 ? c.ns : 0) : 8) : 0);
   const bool ceq_vars = c.ceq && !c.learned_c;
   ok = ok && alloc(&ctx->b.gradRho, ceq_vars ? (g.ndim == 3 ? 5 : 4) : 0);
   // mFlux: the reference sequence's C-equation faces (the 2D learned form too)

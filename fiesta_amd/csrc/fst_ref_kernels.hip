@@ -1331,16 +1331,19 @@ __global__ void k_buoy_gen(Geo g, Phys ph, Range r, const double *__restrict__ v
 
 namespace {
 
-// The cartesian velocity var(w) / rho of every cell into u0, u1, u2 (the
-// input of the gen_ext viscous terms; the same quotients they would form).
-__global__ void k_cartvel3d(long long n, long long N, const double *__restrict__ var, const double *__restrict__ rho,
-                            double *__restrict__ u0, double *__restrict__ u1, double *__restrict__ u2) {
+// The cartesian velocity var(w) / rho and the mass fractions var(4+s) / rho of
+// every cell into u0, u1, u2 and Y (ns planes): the inputs of the gen_ext
+// viscous terms, the same quotients they would form.
+__global__ void k_cartvel3d(long long N, int ns, const double *__restrict__ var, const double *__restrict__ rho,
+                            double *__restrict__ u0, double *__restrict__ u1, double *__restrict__ u2,
+                            double *__restrict__ Y) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+  if (c >= N) return;
   const double r = rho[c];
   u0[c] = var[c] / r;
   u1[c] = var[c + N] / r;
   u2[c] = var[c + 2 * N] / r;
+  for (int q = 0; q < ns; ++q) Y[c + N * q] = var[c + N * (4 + q)] / r;
 }
 
 // gen_ext viscous terms on a generalized 3D grid (not in the reference: the
@@ -1356,34 +1359,33 @@ __global__ void k_cartvel3d(long long n, long long N, const double *__restrict__
 // H_w = sum_n mf(dir,n) (uf_w tau_wn), uf the face velocity (out[0..5]).
 // On a uniform grid of power-of-two spacing each value is the cartesian
 // one scaled by a power of two, so the module equals cart3d bit for bit.
-__device__ __forceinline__ void viscgen3d_face(const Geo &g, const Phys &ph, int dir, int i, int j, int k,
-                                               const double *__restrict__ var, const double *__restrict__ rho,
+// Cell offsets: the face lies between cell c and c + e_DIR; sy, sz the j and
+// k strides.
+template <int DIR>
+__device__ __forceinline__ void viscgen3d_face(long long N, long long sy, long long sz, int ns, const Phys &ph,
+                                               long long c, const double *__restrict__ Y,
                                                const double *__restrict__ u0, const double *__restrict__ u1,
                                                const double *__restrict__ u2, const double *__restrict__ mt,
                                                double *out) {
-  const long long N = g.N;
-  const int ip = dir == 0, jp = dir == 1, kp = dir == 2;
+  const long long st[3] = {1, sy, sz};
+  const long long e = st[DIR], cn = c + e;
   const double *const uw[3] = {u0, u1, u2};
-  auto U = [&](int a, int b, int cc, int w) { return uw[w][g.id(a, b, cc)]; };
-  const long long c = g.id(i, j, k), cn = g.id(i + ip, j + jp, k + kp);
   double mu = 0.0;
-  for (int s = 0; s < g.ns; ++s)
-    mu += ((var[cn + N * (4 + s)] / rho[cn] + var[c + N * (4 + s)] / rho[c]) / 2.0) * ph.mus[s];
+  for (int s = 0; s < ns; ++s) mu += ((Y[cn + N * s] + Y[c + N * s]) / 2.0) * ph.mus[s];
   double gr[3][3], mf[3][3], du[3][3], uf[3];
 #pragma unroll
   for (int w = 0; w < 3; ++w) {
+    const double *u = uw[w];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      if (a == dir) {
-        gr[w][a] = U(i + ip, j + jp, k + kp, w) - U(i, j, k, w);
+      if (a == DIR) {
+        gr[w][a] = u[cn] - u[c];
       } else {
-        const int ta = a == 0, tb = a == 1, tc = a == 2;
-        gr[w][a] = ((U(i + ta, j + tb, k + tc, w) + U(i + ip + ta, j + jp + tb, k + kp + tc, w)) -
-                    (U(i - ta, j - tb, k - tc, w) + U(i + ip - ta, j + jp - tb, k + kp - tc, w))) /
-                   4.0;
+        const long long t = st[a];
+        gr[w][a] = ((u[c + t] + u[cn + t]) - (u[c - t] + u[cn - t])) / 4.0;
       }
     }
-    uf[w] = (U(i + ip, j + jp, k + kp, w) + U(i, j, k, w)) / 2.0;
+    uf[w] = (u[cn] + u[c]) / 2.0;
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -1405,53 +1407,92 @@ __device__ __forceinline__ void viscgen3d_face(const Geo &g, const Phys &ph, int
   t[1][2] = t[2][1] = mu * (dvdz + dwdy);
 #pragma unroll
   for (int w = 0; w < 3; ++w) {
-    out[w] = (mf[dir][0] * t[w][0] + mf[dir][1] * t[w][1]) + mf[dir][2] * t[w][2];
-    out[3 + w] = (mf[dir][0] * (uf[w] * t[w][0]) + mf[dir][1] * (uf[w] * t[w][1])) + mf[dir][2] * (uf[w] * t[w][2]);
+    out[w] = (mf[DIR][0] * t[w][0] + mf[DIR][1] * t[w][1]) + mf[DIR][2] * t[w][2];
+    out[3 + w] = (mf[DIR][0] * (uf[w] * t[w][0]) + mf[DIR][1] * (uf[w] * t[w][1])) + mf[DIR][2] * (uf[w] * t[w][2]);
   }
 }
 
-// The face values of direction dir over the face range into st(0..5) ...
-__global__ void k_viscgen3d_face(Geo g, Phys ph, Range r, int dir, const double *__restrict__ var,
-                                 const double *__restrict__ rho, const double *__restrict__ u0,
-                                 const double *__restrict__ u1, const double *__restrict__ u2,
-                                 const double *__restrict__ mt, double *__restrict__ st) {
-  int i, j, k;
-  if (!decode(r, i, j, k)) return;
-  double out[6];
-  viscgen3d_face(g, ph, dir, i, j, k, var, rho, u0, u1, u2, mt, out);
-  const long long N = g.N, c = g.id(i, j, k);
+// One launch for the three directions.  A block is VW + 1 waves over a tile
+// of 63 (i) x VW (j) cells marching k: lane l holds column i0 - 1 + l (lane 0
+// the tile's left neighbour), wave w < VW row j0 + w, wave VW the row above
+// the tile.  Each face is computed once: the x face right of every lane's
+// column (its left face comes from the lane below through a cross-lane
+// move), the y face below every wave's row (through LDS), the z face above
+// each cell (the one below is the previous plane's, in registers).  Then
+// each cell adds x, y, z in that order, as the per-direction kernels of the
+// reference sequence do (applyViscousTerm, src/viscosity.hpp:460-498), so
+// the values are those of the face-by-face evaluation.
+constexpr int VW = 7, VTX = 63, VKC = 16;
+__global__ void __launch_bounds__(64 * (VW + 1)) k_viscgen3d_march(Geo g, Phys ph, const double *__restrict__ Y,
+                                                                     const double *__restrict__ u0,
+                                                                     const double *__restrict__ u1,
+                                                                     const double *__restrict__ u2,
+                                                                     const double *__restrict__ mt,
+                                                                     double *__restrict__ dvar, int nbx, int nby) {
+  __shared__ double ysh[VW + 1][6][64];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
+  const int i0 = g.ng + bx * VTX, j0 = g.ng + by * VW;
+  const int kb = g.ng + bz * VKC, ke = min(kb + VKC, g.ng + g.nck);
+  const int i = i0 - 1 + l, j = j0 + w;
+  const bool colok = i < g.ng + g.nci;                 // column inside the interior (or the left neighbour)
+  const bool own = l > 0 && colok && w < VW && j < g.ng + g.ncj;  // a cell this block updates
+  const bool rowok = w < VW && j < g.ng + g.ncj;
+  const bool yface = l > 0 && colok && j - 1 < g.ng + g.ncj;      // y face below row j0 + w
+  const long long N = g.N, sy = g.ngi, sz = (long long)g.ngi * g.ngj;
+  double Zp[6] = {0, 0, 0, 0, 0, 0};
+  if (own) viscgen3d_face<2>(N, sy, sz, g.ns, ph, g.id(i, j, kb - 1), Y, u0, u1, u2, mt, Zp);
+  for (int k = kb; k < ke; ++k) {
+    const long long c = g.id(i, j, k);
+    double X[6] = {0, 0, 0, 0, 0, 0}, Yf[6], Z[6];
+    if (rowok && colok) viscgen3d_face<0>(N, sy, sz, g.ns, ph, c, Y, u0, u1, u2, mt, X);
+    if (yface) {
+      viscgen3d_face<1>(N, sy, sz, g.ns, ph, c - sy, Y, u0, u1, u2, mt, Yf);
 #pragma unroll
-  for (int q = 0; q < 6; ++q) st[c + N * q] = out[q];
-}
-
-// ... and their differences over the cell range: dvar(w) += F_w(c) - F_w(c-e),
-// dvar(3) += sum_w (H_w(c) - H_w(c-e)) (applyViscousTerm's accumulation
-// order, src/viscosity.hpp:460-498).
-__global__ void k_viscgen3d_apply(Geo g, Range r, int dir, const double *__restrict__ st,
-                                  double *__restrict__ dvar) {
-  int i, j, k;
-  if (!decode(r, i, j, k)) return;
-  const long long N = g.N;
-  const long long c = g.id(i, j, k), cm = g.id(i - (dir == 0), j - (dir == 1), k - (dir == 2));
-  auto S = [&](long long q, int s) { return st[q + N * s]; };
-  dvar[c] += S(c, 0) - S(cm, 0);
-  dvar[c + N] += S(c, 1) - S(cm, 1);
-  dvar[c + 2 * N] += S(c, 2) - S(cm, 2);
-  dvar[c + 3 * N] += ((S(c, 3) - S(cm, 3)) + (S(c, 4) - S(cm, 4))) + (S(c, 5) - S(cm, 5));
+      for (int q = 0; q < 6; ++q) ysh[w][q][l] = Yf[q];
+    }
+    if (own) viscgen3d_face<2>(N, sy, sz, g.ns, ph, c, Y, u0, u1, u2, mt, Z);
+    double Xm[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Xm[q] = __shfl_up(X[q], 1, 64);
+    __syncthreads();
+    if (own) {
+      double d0 = dvar[c], d1 = dvar[c + N], d2 = dvar[c + 2 * N], d3 = dvar[c + 3 * N];
+      d0 += X[0] - Xm[0];
+      d1 += X[1] - Xm[1];
+      d2 += X[2] - Xm[2];
+      d3 += ((X[3] - Xm[3]) + (X[4] - Xm[4])) + (X[5] - Xm[5]);
+      auto YS = [&](int r, int q) { return ysh[r][q][l]; };
+      d0 += YS(w + 1, 0) - YS(w, 0);
+      d1 += YS(w + 1, 1) - YS(w, 1);
+      d2 += YS(w + 1, 2) - YS(w, 2);
+      d3 += ((YS(w + 1, 3) - YS(w, 3)) + (YS(w + 1, 4) - YS(w, 4))) + (YS(w + 1, 5) - YS(w, 5));
+      d0 += Z[0] - Zp[0];
+      d1 += Z[1] - Zp[1];
+      d2 += Z[2] - Zp[2];
+      d3 += ((Z[3] - Zp[3]) + (Z[4] - Zp[4])) + (Z[5] - Zp[5]);
+      dvar[c] = d0;
+      dvar[c + N] = d1;
+      dvar[c + 2 * N] = d2;
+      dvar[c + 3 * N] = d3;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) Zp[q] = Z[q];
+    }
+    __syncthreads();
+  }
 }
 
 }  // namespace
 
 // Directions x, y, z in that order (src/cart3d.cpp:265-270); fx, fy, fz (the
 // per-variable flux planes, free once the fluxes are differenced) hold the
-// cartesian velocity, b.stress the face values.
+// cartesian velocity, the stress planes (max(6, ns) on generalized grids) the
+// mass fractions.
 void launch_viscgen3d(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s) {
-  const Range fr = face_range(g), cr = cell_range(g);
-  k_cartvel3d<<<(unsigned)((g.N + 255) / 256), 256, 0, s>>>(g.N, g.N, b.var, b.rho, b.fx, b.fy, b.fz);
-  for (int d = 0; d < 3; ++d) {
-    k_viscgen3d_face<<<grid_for(fr), 256, 0, s>>>(g, ph, fr, d, b.var, b.rho, b.fx, b.fy, b.fz, b.metrics, b.stress);
-    k_viscgen3d_apply<<<grid_for(cr), 256, 0, s>>>(g, cr, d, b.stress, b.dvar);
-  }
+  k_cartvel3d<<<(unsigned)((g.N + 255) / 256), 256, 0, s>>>(g.N, g.ns, b.var, b.rho, b.fx, b.fy, b.fz, b.stress);
+  const int nbx = (g.nci + VTX - 1) / VTX, nby = (g.ncj + VW - 1) / VW, nbz = (g.nck + VKC - 1) / VKC;
+  k_viscgen3d_march<<<nbx * nby * nbz, 64 * (VW + 1), 0, s>>>(g, ph, b.stress, b.fx, b.fy, b.fz, b.metrics,
+                                                              b.dvar, nbx, nby);
 }
 
 // gen3d_func::compute (src/gen3d.cpp:341-369) / gen2d_func::compute
