@@ -21,13 +21,32 @@ more than its tolerance:
       mu (lap u + grad(div u)/3) and the energy term is the continuous
       d_j(u_i tau_ij) plus the truncation terms that the two-cell face
       average of u and the face differencing add (derived below).
+  3D fluxes over each direction's spacing (src/flux.hpp:266-309,
+      src/cart3d.cpp:243-248): a passive species density linear in x, y and
+      z, advected at a uniform velocity with dx, dy, dz distinct, gives
+      dvar = -u . grad(rho Y) exactly (WENO5 and the face velocity are exact
+      for linear data), so a flux divided by another direction's spacing
+      fails.
+  C-equation contact switch (src/ceq3d.hpp:138-141, Q9): the switch reads
+      nrg = E/rho - 0.5 rho |u|^2 (rho times the kinetic energy per mass),
+      not E/rho - 0.5 |u|^2.  A density ramp under a fast uniform stream with
+      p = p0 rho^1.2 has E/rho - |u|^2/2 rising with rho but nrg falling, so
+      the reference's switch marks a contact (gradRho[1] = |grad rho|) where
+      the textbook energy would not.  The 2D switch (src/ceq.hpp:80-151)
+      likewise.
+  2D C-equation sources and relaxation (src/ceq.hpp:80-195): the shock
+      source needs div u < 0 strictly (a density ramp at rest gives 0); the
+      direction source is the tangent (-d rho/dy, d rho/dx); the relaxation's
+      Laplacian sums both directions, each divided by d (not d^2), so
+      C = c0 + bx x^2 + by y^2 on a uniform state gives
+      dvar(C) = -maxS C/(eps |dx|) + kappa maxS |dx| (2 bx dx + 2 by dy).
 """
 import numpy as np
 import pytest
 
 import oracle
 from fiesta_amd import workloads as W
-from fiesta_amd.config import FST_DVAR, FST_VAR, Dims, make_config
+from fiesta_amd.config import FST_DVAR, FST_GRADRHO, FST_VAR, Dims, make_config
 
 
 def _centres(d, h, axis):
@@ -70,6 +89,37 @@ def test_presgrad3d_z_term_is_divided_by_dy():
         assert np.abs(dv[v][core]).max() <= 1e-9 * abs(want)
     for v in (3, 4):
         assert np.abs(dv[v][core]).max() == 0.0
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_pressure_gradient_every_direction_over_its_spacing(ndim):
+    """At rest, p linear in every direction with distinct spacings: the
+    momentum RHS is (-a_x, -a_y) in 2D and (-a_x, -a_y, -a_z dz/dy) in 3D
+    (the reference's dz := dy), each term over its own direction's spacing
+    (src/presgrad.hpp:22-71)."""
+    dx, dy, dz = 1.0e-2, 2.0e-2, 0.5e-2
+    ax, ay, az = 2.0e6, -1.5e6, 1.0e6
+    kw = dict(ns=1, nci=9, ncj=9, dx=dx, dy=dy, gamma=(1.4,), M=(0.02896,), mu=(0.0,), bc=("outflow",) * 6)
+    cfg = make_config(ndim=3, nck=9, dz=dz, **kw) if ndim == 3 else make_config(ndim=2, **kw)
+    d = Dims(cfg)
+    one = np.ones((d.ngk, d.ngj, d.ngi))
+    p = 1.0e5 + ax * _centres(d, dx, 0) * one + ay * _centres(d, dy, 1) * one
+    if ndim == 3:
+        p = p + az * _centres(d, dz, 2) * one
+    var = W.fill_conserved(cfg, [1.2 * one], [0 * one] * ndim, p, ghosts=True)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.compute()
+    dv = o.get(FST_DVAR)
+    g = d.ng
+    # cells two away from every boundary: the 3D compute refills the ghosts
+    # from the (outflow) boundary conditions first, so stencils reaching them
+    # read no linear data
+    core = (slice(g + 2, g + d.nck - 2), slice(g + 2, g + d.ncj - 2), slice(g + 2, g + d.nci - 2)) \
+        if ndim == 3 else (0, slice(g + 2, g + d.ncj - 2), slice(g + 2, g + d.nci - 2))
+    want = [-ax, -ay] + ([-az * dz / dy] if ndim == 3 else [])
+    for v, w in enumerate(want):
+        np.testing.assert_allclose(dv[v][core], w, rtol=1e-9)
 
 
 # ------------------------------------------------------------------ 2D QUICK and centered4
@@ -283,3 +333,125 @@ def test_viscous3d_quadratic_velocity_closed_form(seed):
     np.testing.assert_allclose(visc[3][core], energy, rtol=0, atol=1e-9 * escale)
     # the species equations carry no viscous term
     assert np.abs(visc[4:6][(slice(None),) + core]).max() == 0.0
+
+
+def test_3d_fluxes_use_each_directions_spacing():
+    """Uniform velocity (u, v, w) and pressure, a single species whose density
+    is linear in x, y and z, dx, dy, dz distinct: the species RHS is
+    -(u a_x + v a_y + w a_z) on every cell whose WENO5 stencils stay off the
+    mirrored ghost layers."""
+    dx, dy, dz = 1.0e-2, 1.3e-2, 0.7e-2
+    ax, ay, az = 3.0, -2.0, 5.0
+    U, V, Wv = 30.0, -20.0, 25.0
+    cfg = make_config(ndim=3, ns=1, nci=10, ncj=10, nck=10, dx=dx, dy=dy, dz=dz, gamma=(1.4,), M=(0.02896,),
+                      mu=(0.0,), dt=1e-7)
+    d = Dims(cfg)
+    sh = (d.nck, d.ncj, d.nci)
+    x = _centres(d, dx, 0)[:, :, d.ng:d.ng + d.nci] * np.ones(sh)
+    y = _centres(d, dy, 1)[:, d.ng:d.ng + d.ncj, :] * np.ones(sh)
+    z = _centres(d, dz, 2)[d.ng:d.ng + d.nck, :, :] * np.ones(sh)
+    rho = 1.0 + ax * x + ay * y + az * z
+    one = np.ones(sh)
+    var = W.fill_conserved(cfg, [rho], [U * one, V * one, Wv * one], 1e5 * one)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.compute()
+    dv = o.get(FST_DVAR)[4]
+    g = d.ng
+    core = (slice(g + 3, g + d.nck - 3), slice(g + 3, g + d.ncj - 3), slice(g + 3, g + d.nci - 3))
+    # the binary32-rounded WENO5 coefficients (their sums differ from 1 by up
+    # to 9e-8) leave a few 1e-9 of relative error on linear data; another
+    # direction's spacing would change the result by tens of percent
+    np.testing.assert_allclose(dv[core], -(U * ax + V * ay + Wv * az), rtol=1e-7)
+
+
+def test_ceq_contact_switch_uses_rho_weighted_kinetic_energy():
+    """rho = 1 + a x under a uniform stream of 400 m/s along y (div u = 0),
+    p = 1e5 rho^1.2: E/rho - |u|^2/2 = 2.5e5 rho^0.2 rises with rho, while the
+    reference's nrg = E/rho - 0.5 rho |u|^2 falls (its slope is
+    5e4 rho^-0.8 - 8e4 < 0), so the contact source gradRho[1] is |grad rho|
+    and the shock source gradRho[0] is 0."""
+    a = 20.0
+    cfg = make_config(ndim=3, ns=1, nci=12, ncj=4, nck=4, dx=1e-2, dy=1e-2, dz=1e-2, gamma=(1.4,), M=(0.02896,),
+                      mu=(0.0,), ceq=True, kap=1.0, eps=1.0, alpha=1.0, yper=True, zper=True)
+    d = Dims(cfg)
+    sh = (d.nck, d.ncj, d.nci)
+    x = ((np.arange(d.nci) + 0.5) * cfg.dx)[None, None, :] * np.ones(sh)
+    rho = 1.0 + a * x
+    p = 1e5 * rho ** 1.2
+    var = W.fill_conserved(cfg, [rho], [0 * x, 400.0 + 0 * x, 0 * x], p, [0 * x] * 5)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.compute()
+    G = o.get(FST_GRADRHO)
+    g = d.ng
+    core = (slice(g, g + d.nck), slice(g, g + d.ncj), slice(g + 3, g + d.nci - 3))
+    np.testing.assert_allclose(G[1][core], a, rtol=1e-9)
+    assert np.all(G[0][core] == 0)
+
+
+def _ceq2d(fields, cvars=None, nci=12, ncj=12, dx=1.0e-2, dy=1.2e-2):
+    """gradRho and the C-variable RHS of one 2D compute() (cart2d, one
+    species, C-equation on) for fields(x, y) -> (rho, u, v, p) over every cell;
+    cvars(x, y) -> 5 arrays of C variables (zeros when None).  Returns
+    (gradRho, dvar of the C variables, maxS, cfg, core slice)."""
+    cfg = make_config(ndim=2, ns=1, nci=nci, ncj=ncj, dx=dx, dy=dy, gamma=(1.4,), M=(0.02896,), mu=(0.0,),
+                      ceq=True, kap=2.0, eps=3.0, alpha=1.0, bc=("outflow",) * 6, dt=1e-7)
+    d = Dims(cfg)
+    x = _centres(d, dx, 0) * np.ones((1, d.ngj, d.ngi))
+    y = _centres(d, dy, 1) * np.ones((1, d.ngj, d.ngi))
+    rho, u, v, p = fields(x, y)
+    C = cvars(x, y) if cvars else [0 * x] * 5
+    var = W.fill_conserved(cfg, [rho], [u, v], p, C, ghosts=True)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.compute()  # no ghost fill: the fields extend into the ghost layers
+    maxS = o.maxima()[0]
+    g = d.ng
+    core = (0, slice(g, g + d.ncj), slice(g, g + d.nci))
+    return o.get(FST_GRADRHO), o.get(FST_DVAR)[d.nv:d.nv + 4], maxS, cfg, core
+
+
+def test_ceq2d_contact_switch_uses_rho_weighted_kinetic_energy():
+    """As the 3D pin, in 2D: rho = 1 + 20 x under a 400 m/s stream along y,
+    p = 1e5 rho^1.2 -> contact source |grad rho|, shock source 0."""
+    G, _, _, _, core = _ceq2d(lambda x, y: (1.0 + 20.0 * x, 0 * x, 400.0 + 0 * x, 1e5 * (1.0 + 20.0 * x) ** 1.2))
+    np.testing.assert_allclose(G[1][core], 20.0, rtol=1e-9)
+    assert np.all(G[0][core] == 0)
+
+
+def test_ceq2d_shock_source_needs_strict_compression():
+    """rho = 1 + 20 x, p = 1e5 rho^1.4 (both energies rise with rho: no contact):
+    at rest div u = 0 exactly and the shock source is 0; under the
+    compression u = -100 x it is |grad rho|."""
+    G, _, _, _, core = _ceq2d(lambda x, y: (1.0 + 20.0 * x, 0 * x, 0 * x, 1e5 * (1.0 + 20.0 * x) ** 1.4))
+    assert np.all(G[0][core] == 0) and np.all(G[1][core] == 0)
+    G, _, _, _, core = _ceq2d(lambda x, y: (1.0 + 20.0 * x, -100.0 * x, 0 * x, 1e5 * (1.0 + 20.0 * x) ** 1.4))
+    np.testing.assert_allclose(G[0][core], 20.0, rtol=1e-9)
+
+
+def test_ceq2d_direction_source_is_the_tangent():
+    """rho = 1 + 3 x - 5 y at rest: the direction sources are (-d rho/dy,
+    d rho/dx) = (5, 3)."""
+    G, _, _, _, core = _ceq2d(lambda x, y: (1.0 + 3.0 * x - 5.0 * y, 0 * x, 0 * x, 1e5 + 0 * x))
+    np.testing.assert_allclose(G[2][core], 5.0, rtol=1e-9)
+    np.testing.assert_allclose(G[3][core], 3.0, rtol=1e-9)
+
+
+def test_ceq2d_relaxation_laplacian_sums_both_directions_over_d():
+    """Uniform state (no sources), C = c0 + bx x^2 + by y^2 in every slot:
+    dvar(C) = -maxS C / (eps |dx|) + kappa maxS |dx| (2 bx dx + 2 by dy)."""
+    c0, bx, by = 0.4, 300.0, -170.0
+    G, dv, maxS, cfg, core = _ceq2d(lambda x, y: (1.2 + 0 * x, 0 * x, 0 * x, 1e5 + 0 * x),
+                                    cvars=lambda x, y: [c0 + bx * x * x + by * y * y] * 5)
+    assert np.all(G[:, core[1], core[2]] == 0)
+    d = Dims(cfg)
+    x = _centres(d, cfg.dx, 0)[0, :, d.ng:d.ng + d.nci]
+    y = _centres(d, cfg.dy, 1)[0, d.ng:d.ng + d.ncj, :]
+    C = c0 + bx * x * x + by * y * y
+    dxmag = np.hypot(cfg.dx, cfg.dy)
+    want = -maxS * C / (cfg.eps * dxmag) + cfg.kap * maxS * dxmag * (2 * bx * cfg.dx + 2 * by * cfg.dy)
+    for n in range(4):
+        np.testing.assert_allclose(dv[n][core], want, rtol=1e-9, atol=1e-9 * np.abs(want).max())
