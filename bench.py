@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--extras", action="store_true",
                     help="also time postStep's noise filter, the device-side output pack (fst_pack), the gen3d module and the 2D expansion on the "
                          "fp64 workload")
+    ap.add_argument("--gen3d", action="store_true",
+                    help="the gen3d module on the terrain grid (FP64 and FP32 forms); also part of --extras")
     ap.add_argument("--cart2d", action="store_true",
                     help="the 2D module on the expansion (fused stage and kernel sequence); also part of --extras")
     ap.add_argument("--multiblast", action="store_true",
@@ -369,54 +371,79 @@ def extras_leg(args, var):
 def gen3d_leg(args):
     """The generalized-grid module (gen3d: EOS, contravariant velocity, WENO
     fluxes, metric pressure gradient; src/gen3d.cpp:341-369) on the 3D
-    terrain workload, 512 x 512 x 256 cells, 3 species, FP64, on the fused
-    gen3d stage: the advection kernel (fluxes of every variable and the
-    metric pressure gradient in one launch), then one pass with the RK update
-    and the EOS / contravariant velocity of the new state.  Device ms per
-    step from hipEvents over 3 steps after one warm-up step."""
+    terrain workload, 512 x 512 x 256 cells, 3 species, on the fused gen3d
+    stage: the advection kernel (fluxes of every variable and the metric
+    pressure gradient in one launch), then one pass with the RK update and
+    the EOS / contravariant velocity of the new state.  FP64 (the reference
+    arithmetic) and the FP32 form (binary32 fluxes, float primitive planes;
+    FP64 pressure gradient, EOS and RK accumulation).  Device ms per step from
+    hipEvents over 3 steps after one warm-up step; the FP32 form's error
+    against FP64 after 20 steps from the same state."""
     from fiesta_amd import workloads as W
-    from fiesta_amd.config import FST_GRID, FST_VAR, Dims
+    from fiesta_amd.config import FST_FP32_BULK, FST_FP64, FST_GRID, FST_VAR, Dims
     from fiesta_amd.runtime import Module
 
     n = tuple(args.gen3d_size)
     cfg = W.gen3d_terrain_config(n=n, ns=3, dt=1e-7)
     d = Dims(cfg)
-    m = Module(cfg)
-    m.upload(FST_GRID, W.terrain_grid(cfg))
-    m.upload(FST_VAR, W.random_state_all(cfg, seed=5, umag=20.0))
-    m.pre_sim()
-    m.step(1)
-    m.sync()
-    steps = 3
-    m.step(steps)
-    ms_step = m.last_step_ms() / steps
-    m.close()
+    nodes = W.terrain_grid(cfg)
+    var = W.random_state_all(cfg, seed=5, umag=20.0)
+    acc = 20
+    ms, states = {}, {}
+    for form, prec in (("fp64", FST_FP64), ("fp32", FST_FP32_BULK)):
+        cfg.precision = prec
+        m = Module(cfg)
+        m.upload(FST_GRID, nodes)
+        m.upload(FST_VAR, var)
+        m.pre_sim()
+        m.step(1)
+        m.sync()
+        steps = 3
+        m.step(steps)
+        ms[form] = m.last_step_ms() / steps
+        m.step(acc - 1 - steps)
+        states[form] = m.download(FST_VAR)
+        m.close()
+    cfg.precision = FST_FP64
     cs = d.ncells * 2
     balg = 20 * d.nvt          # U read/write of the RK2 stages
     bmet = balg + 9 * 8        # plus one read of the 9 metric terms per stage
-    stage_ms = ms_step / 2
-    return {"workload": "gen3d terrain %dx%dx%d, 3 species, FP64 (fused gen3d stage: advection + metric pressure "
-                        "gradient kernel, RK update + EOS kernel)" % n,
-            "ms_per_stage": round(stage_ms, 4), "value": round(cs / (ms_step * 1e-3) / 1e9, 4),
-            "unit": "Gcell-stage/s",
-            "roofline": {"bound": "hbm", "algorithmic_bytes_per_cell_stage": balg,
-                         "with_metrics_bytes_per_cell_stage": bmet,
-                         "achieved_GBps": round(d.ncells * balg / (stage_ms * 1e-3) / 1e9, 1),
-                         "frac": round(d.ncells * balg / (stage_ms * 1e-3) / 8.0e12, 4)}}
+
+    def line(ms_step):
+        stage_ms = ms_step / 2
+        return {"ms_per_stage": round(stage_ms, 4), "value": round(cs / (ms_step * 1e-3) / 1e9, 4),
+                "unit": "Gcell-stage/s",
+                "roofline": {"bound": "hbm", "algorithmic_bytes_per_cell_stage": balg,
+                             "with_metrics_bytes_per_cell_stage": bmet,
+                             "achieved_GBps": round(d.ncells * balg / (stage_ms * 1e-3) / 1e9, 1),
+                             "frac": round(d.ncells * balg / (stage_ms * 1e-3) / 8.0e12, 4)}}
+
+    out = {"workload": "gen3d terrain %dx%dx%d, 3 species, RK2, FP64 (fused gen3d stage: advection + metric "
+                       "pressure gradient kernel, RK update + EOS kernel)" % n}
+    out.update(line(ms["fp64"]))
+    f32 = line(ms["fp32"])
+    f32["speedup_vs_fp64"] = round(ms["fp64"] / ms["fp32"], 3)
+    f32["error_vs_fp64_%d_steps" % acc] = form_errors(cfg, states["fp32"], states["fp64"], cfg)
+    out["fp32"] = f32
+    return out
 
 
 def terrain_leg(args):
     """3D ideal expansion over terrain: gen3d (src/gen3d.cpp:341-369) with the
-    gen_ext option (the cartesian ghost fill and buoyancy in index space,
-    which the reference's gen3d lacks), 512 x 256 x 512 cells with y
-    vertical, ground y_g = 0.05 sin(2 pi x) cos(2 pi z) and a tanh
-    (beta = 2) vertical stretch, air + hot gas + SF6, a 50:1 hot-gas blob
-    over a pooled SF6 layer, gravity, no-slip ground and lid, periodic x and
-    z, SSP-RK3 at CFL 0.4, FP64.  Device ms per step from hipEvents over the
-    steps after one warm-up step; then the state is checked: finite, and
-    every species' total the same to rounding (the closed box keeps mass)."""
+    gen_ext option (the cartesian ghost fill and buoyancy in index space and
+    the viscous terms in metric form, which the reference's gen3d lacks),
+    512 x 256 x 512 cells with y vertical, ground y_g = 0.05 sin(2 pi x)
+    cos(2 pi z) and a tanh (beta = 2) vertical stretch, air + hot gas + SF6,
+    a 50:1 hot-gas blob over a pooled SF6 layer, gravity, no-slip ground and
+    lid, periodic x and z, SSP-RK3 at CFL 0.4.  FP64 and the FP32 form (the
+    gen_ext viscous terms stay FP64 in it) from the same state: device ms per
+    step from hipEvents over the steps after one warm-up step; then each
+    final state is checked (finite, every species' total the same to
+    rounding: the closed box keeps mass) and the FP32 form's density,
+    pressure and mass fractions are compared with the FP64 path's (equal to
+    the oracle bit for bit) after the same args.terrain_steps steps."""
     from fiesta_amd import workloads as W
-    from fiesta_amd.config import FST_GRID, FST_VAR, Dims
+    from fiesta_amd.config import FST_FP32_BULK, FST_FP64, FST_GRID, FST_VAR, Dims
     from fiesta_amd.runtime import Module
 
     n = tuple(args.terrain_size)
@@ -427,41 +454,53 @@ def terrain_leg(args):
     d = Dims(cfg)
     sl = d.interior()
     mass0 = var[sl][4:].sum(axis=(1, 2, 3))
-    m = Module(cfg)
-    m.upload(FST_GRID, nodes)
-    m.upload(FST_VAR, var)
-    del nodes, var
-    m.pre_sim()
-    m.step(1)
-    m.sync()
     steps = max(1, args.terrain_steps - 1)
-    log("terrain: %d timed steps" % steps)
-    m.step(steps)
-    ms_step = m.last_step_ms() / steps
-    v = m.download(FST_VAR)
-    m.close()
-    vi = v[sl]
-    rho = vi[4:].sum(axis=0)
-    mass1 = vi[4:].sum(axis=(1, 2, 3))
-    umax = float((np.sqrt(vi[0] ** 2 + vi[1] ** 2 + vi[2] ** 2) / rho).max())
-    del v, vi, rho
     nst = 3
     cs = d.ncells * nst
     balg = 8 * 8 * d.nvt / 3.0     # SSP-RK3: U reads/writes of the three stages, per stage
     bmet = balg + 9 * 8            # plus one read of the 9 metric terms per stage
-    stage_ms = ms_step / nst
-    return {"workload": "3D ideal expansion over terrain %dx%dx%d (y vertical), 3 species, gravity, no-slip ground "
-                        "and lid, periodic x/z, SSP-RK3, FP64 (gen3d + gen_ext)" % n,
-            "dt": cfg.dt, "steps": steps + 1, "ms_per_stage": round(stage_ms, 4),
-            "value": round(cs / (ms_step * 1e-3) / 1e9, 4), "unit": "Gcell-stage/s",
-            "roofline": {"bound": "hbm", "algorithmic_bytes_per_cell_stage": round(balg, 1),
-                         "with_metrics_bytes_per_cell_stage": round(bmet, 1),
-                         "achieved_GBps": round(d.ncells * balg / (stage_ms * 1e-3) / 1e9, 1),
-                         "frac": round(d.ncells * balg / (stage_ms * 1e-3) / 8.0e12, 4)},
-            "state_after": {"finite": bool(np.isfinite(mass1).all()), "max_speed": round(umax, 2),
-                            "species_mass_rel_change": [float("%.3g" % x) for x in (mass1 - mass0) / mass0]},
-            "parity": "bitwise equal to the oracle on the same workload at 40x20x40 over 12 steps "
-                      "(tests/test_gpu_gen.py::test_terrain_expansion_against_oracle)"}
+    res, states = {}, {}
+    for form, prec in (("fp64", FST_FP64), ("fp32", FST_FP32_BULK)):
+        cfg.precision = prec
+        m = Module(cfg)
+        m.upload(FST_GRID, nodes)
+        m.upload(FST_VAR, var)
+        m.pre_sim()
+        m.step(1)
+        m.sync()
+        log("terrain: %s, %d timed steps" % (form, steps))
+        m.step(steps)
+        ms_step = m.last_step_ms() / steps
+        v = m.download(FST_VAR)
+        m.close()
+        vi = v[sl]
+        rho = vi[4:].sum(axis=0)
+        mass1 = vi[4:].sum(axis=(1, 2, 3))
+        umax = float((np.sqrt(vi[0] ** 2 + vi[1] ** 2 + vi[2] ** 2) / rho).max())
+        del vi, rho
+        states[form] = v
+        stage_ms = ms_step / nst
+        res[form] = {"ms_per_stage": round(stage_ms, 4), "value": round(cs / (ms_step * 1e-3) / 1e9, 4),
+                     "unit": "Gcell-stage/s",
+                     "roofline": {"bound": "hbm", "algorithmic_bytes_per_cell_stage": round(balg, 1),
+                                  "with_metrics_bytes_per_cell_stage": round(bmet, 1),
+                                  "achieved_GBps": round(d.ncells * balg / (stage_ms * 1e-3) / 1e9, 1),
+                                  "frac": round(d.ncells * balg / (stage_ms * 1e-3) / 8.0e12, 4)},
+                     "state_after": {"finite": bool(np.isfinite(mass1).all()), "max_speed": round(umax, 2),
+                                     "species_mass_rel_change": [float("%.3g" % x) for x in (mass1 - mass0) / mass0]}}
+    del nodes, var
+    cfg.precision = FST_FP64
+    out = {"workload": "3D ideal expansion over terrain %dx%dx%d (y vertical), 3 species, gravity, viscous, "
+                       "no-slip ground and lid, periodic x/z, SSP-RK3, FP64 (gen3d + gen_ext)" % n,
+           "dt": cfg.dt, "steps": steps + 1}
+    out.update(res["fp64"])
+    out["parity"] = ("bitwise equal to the oracle on the same workload at 40x20x40 over 12 steps "
+                     "(tests/test_gpu_gen.py::test_terrain_expansion_against_oracle)")
+    f32 = res["fp32"]
+    f32["speedup_vs_fp64"] = round(res["fp64"]["ms_per_stage"] / f32["ms_per_stage"], 3)
+    f32["error_vs_fp64_%d_steps" % (steps + 1)] = form_errors(cfg, states["fp32"], states["fp64"], cfg)
+    out["fp32"] = f32
+    return out
 
 
 def expansion2d_leg(args):
@@ -825,8 +864,9 @@ def main():
     if args.extras:
         log("extras: noise filter and output pack")
         out["extras"] = extras_leg(args, var)
+    if args.extras or args.gen3d:
         log("extras: generalized-grid module (gen3d)")
-        out["extras"]["gen3d"] = gen3d_leg(args)
+        out.setdefault("extras", {})["gen3d"] = gen3d_leg(args)
     if args.extras or args.terrain:
         log("3D ideal expansion over terrain (gen3d + gen_ext)")
         out.setdefault("extras", {})["terrain_expansion"] = terrain_leg(args)

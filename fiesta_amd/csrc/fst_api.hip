@@ -73,9 +73,20 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
     *why = "the learned C-indicator runs in the FP64 (0) or FP32 bulk (1) forms";
     return FST_ERR_ARG;
   }
-  if (c.grid == 1 && (c.learned_c || c.precision != FST_FP64)) {
-    *why = "generalized grids run the reference arithmetic (FP64, no learned C-indicator)";
+  if (c.grid == 1 && (c.learned_c || c.precision == FST_FP64_FAST)) {
+    *why = "generalized grids run in FP64 (0) or, on the fused gen3d stage, FP32 bulk (1); no learned C-indicator";
     return FST_ERR_ARG;
+  }
+  if (c.grid == 1 && c.precision == FST_FP32_BULK) {
+    // the fused gen3d stage's conditions (gen_adv_supported): 3D, WENO5,
+    // ns <= 3, every array (nvt or the 5 primitive planes) below 2^32 elements
+    const double N = (double)(c.nci + 2 * c.ng) * (c.ncj + 2 * c.ng) * (c.nck + 2 * c.ng);
+    if (!(c.ndim == 3 && c.scheme == FST_WENO5 && c.ns >= 1 && c.ns <= 3 &&
+          N * std::max(4 + c.ns + (c.ceq ? 5 : 0), 5) < 4294967296.0)) {
+      *why = "the FP32 form on generalized grids runs on the fused gen3d stage only (3D, WENO5, ns <= 3, every "
+             "array below 2^32 elements)";
+      return FST_ERR_ARG;
+    }
   }
   if (c.gen_ext && c.grid != 1) {
     *why = "gen_ext applies to generalized grids (grid = 1)";
@@ -117,7 +128,7 @@ int derive(const fst_config &c, fst_dims *d, std::string *why) {
   const int bcs[6] = {c.bcL, c.bcR, c.bcB, c.bcT, c.bcH, c.bcF};
   for (int q = 0; q < 6; ++q)
     if (bcs[q] < 0 || bcs[q] > 3) { *why = "unknown boundary type"; return FST_ERR_ARG; }
-  if (c.precision != FST_FP64 && !fused_supported(c)) {
+  if (c.precision != FST_FP64 && c.grid == 0 && !fused_supported(c)) {
     *why = "the FP32 and fast FP64 forms run on the fused 3D stage kernels only (cartesian, WENO5, ns <= 3, "
            "every array below 2^32 elements)";
     return FST_ERR_ARG;
@@ -216,9 +227,10 @@ int compute_seq(fst_ctx *ctx, bool fields = false) {
   return FST_OK;
 }
 
-// gen3d on the fused kernels (fused_gen_steps): FP64, WENO5, ns <= 3
+// gen3d on the fused kernels (fused_gen_steps): FP64 or FP32 bulk, WENO5, ns <= 3
 bool gen_fused(const fst_config &c, const Geo &g) {
-  return c.grid == 1 && c.ndim == 3 && c.precision == FST_FP64 && gen_adv_supported(c, g);
+  return c.grid == 1 && c.ndim == 3 && (c.precision == FST_FP64 || c.precision == FST_FP32_BULK) &&
+         gen_adv_supported(c, g);
 }
 
 int learned_only(fst_ctx *ctx, const char *what) {
@@ -477,6 +489,9 @@ int fst_rk_stage(fst_ctx *ctx, int stage) {
 
 int fst_post_step(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
+  // generalized grids: rho, p and vel alias the fused gen3d stage's primitive
+  // planes, and vel here is the cartesian velocity, not the contravariant one
+  if (ctx->cf.grid == 1) ctx->fb.valid = false;
   launch_secondary(ctx->g, ctx->ph, ctx->b, ctx->stream);
   if (noise_on(ctx->cf)) {
     ctx->fb.valid = false;  // var changes

@@ -920,11 +920,15 @@ __device__ __forceinline__ void face_s(const float *qs, float rd, int r, int q, 
   }
 }
 
-// FI: U is the float stage state of the learned FP32 form.
-template <int NS, bool FI>
-__global__ void __launch_bounds__(NTS, 1)
-    k_adv3d_s(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
-              double *__restrict__ dvar, int buoy, int kchunk, int nbx, int nby, int nbz) {
+// FI: U is the float stage state of the learned FP32 form.  GEN: the
+// generalized-grid module (P's float planes hold the contravariant velocity,
+// MT the 9 metric planes): applyGenPressureGradient3D's metric pressure
+// gradient (src/presgrad.hpp:72-147) in FP64 in place of the cartesian one,
+// and the partial RHS stored as FP64 (the gen_ext viscous terms add to it).
+template <int NS, bool FI, bool GEN>
+__device__ __forceinline__ void adv_s_tile(const Grid32 &g, const FC &k, const double *__restrict__ U,
+                                           const double *__restrict__ P, double *__restrict__ dvar, int buoy,
+                                           int kchunk, int nbx, int nby, int nbz, const double *__restrict__ MT) {
   using PS = PlaneS<NS>;
   constexpr int NV = 4 + NS;
   __shared__ float qs[PS::NQ * QPLS];
@@ -1003,19 +1007,46 @@ __global__ void __launch_bounds__(NTS, 1)
     }
     __syncthreads();
     if (t.cellok) {
+      const double *pp = ps + rc * QX + qc;
+      const uint32_t sz = g.s2;
+      double gp[3];  // pressure gradient terms, subtracted from dv[0..2]
+      if constexpr (!GEN) {
+        gp[0] = cdiv(pp[-2] - 8.0 * pp[-1] + 8.0 * pp[1] - pp[2], k.pg[0], k.rpg[0]);
+        gp[1] = cdiv(pp[-2 * QX] - 8.0 * pp[-QX] + 8.0 * pp[QX] - pp[2 * QX], k.pg[1], k.rpg[1]);
+        gp[2] = cdiv(ld(P, c - 2 * sz) - 8.0 * ld(P, c - sz) + 8.0 * ld(P, c + sz) - ld(P, c + 2 * sz), k.pg[2],
+                     k.rpg[2]);  // FP32 forms: P starts with the FP64 pressure plane
+      } else {
+        // the FP64 kernel's metric pressure gradient on the same FP64 p,
+        // before the flux divergence holds its registers, one n at a time
+        // (unrolled, the 36 metric loads were hoisted and spilled)
+        const uint32_t sy = g.s1;
+#pragma unroll 1
+        for (int n = 0; n < 3; ++n) {
+          const double *m0 = MT + (size_t)N * (size_t)(0 + 3 * n);
+          const double *m1 = MT + (size_t)N * (size_t)(1 + 3 * n);
+          const double *m2 = MT + (size_t)N * (size_t)(2 + 3 * n);
+          const double dxi = cdiv(pp[-2] * ld(m0, c - 2) - 8.0 * pp[-1] * ld(m0, c - 1) + 8.0 * pp[1] * ld(m0, c + 1) -
+                                      pp[2] * ld(m0, c + 2),
+                                  12.0, k.r12);
+          const double det = cdiv(pp[-2 * QX] * ld(m1, c - 2 * sy) - 8.0 * pp[-QX] * ld(m1, c - sy) +
+                                      8.0 * pp[QX] * ld(m1, c + sy) - pp[2 * QX] * ld(m1, c + 2 * sy),
+                                  12.0, k.r12);
+          const double dzt = cdiv(ld(P, c - 2 * sz) * ld(m2, c - 2 * sz) - 8.0 * ld(P, c - sz) * ld(m2, c - sz) +
+                                      8.0 * ld(P, c + sz) * ld(m2, c + sz) - ld(P, c + 2 * sz) * ld(m2, c + 2 * sz),
+                                  12.0, k.r12);
+          const double t3 = dxi + det + dzt;
+          if (n == 0) gp[0] = t3;
+          else if (n == 1) gp[1] = t3;
+          else gp[2] = t3;
+        }
+      }
       double dv[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v)
         dv[v] = (double)(-((xsh[w][v][l + 1] - xsh[w][v][l]) + (ysh[w + 1][v][l] - ysh[w][v][l]) + (Z[v] - Zp[v])));
-      const double *pp = ps + rc * QX + qc;
-      const uint32_t sz = g.s2;
-      const double dxp = cdiv(pp[-2] - 8.0 * pp[-1] + 8.0 * pp[1] - pp[2], k.pg[0], k.rpg[0]);
-      const double dyp = cdiv(pp[-2 * QX] - 8.0 * pp[-QX] + 8.0 * pp[QX] - pp[2 * QX], k.pg[1], k.rpg[1]);
-      const double dzp = cdiv(ld(P, c - 2 * sz) - 8.0 * ld(P, c - sz) + 8.0 * ld(P, c + sz) - ld(P, c + 2 * sz),
-                              k.pg[2], k.rpg[2]);  // FP32 forms: P starts with the FP64 pressure plane
-      dv[0] = dv[0] - dxp;
-      dv[1] = dv[1] - dyp;
-      dv[2] = dv[2] - dzp;
+      dv[0] = dv[0] - gp[0];
+      dv[1] = dv[1] - gp[1];
+      dv[2] = dv[2] - gp[2];
       if (buoy) {
         const double rho = pget<true>(P, N, 0, c);
         const double vv = uld<FI>(U, c + N) / rho;
@@ -1025,12 +1056,29 @@ __global__ void __launch_bounds__(NTS, 1)
         dv[3] += vv * f;
       }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) st_dv<true>(dvar, c + N * v, dv[v]);
+      for (int v = 0; v < NV; ++v) st_dv<!GEN>(dvar, c + N * v, dv[v]);
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) Zp[q] = Z[q];
     __syncthreads();
   }
+}
+
+template <int NS, bool FI>
+__global__ void __launch_bounds__(NTS, 1)
+    k_adv3d_s(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
+              double *__restrict__ dvar, int buoy, int kchunk, int nbx, int nby, int nbz) {
+  adv_s_tile<NS, FI, false>(g, k, U, P, dvar, buoy, kchunk, nbx, nby, nbz, nullptr);
+}
+
+// gen3d's fluxes in binary32 (FP32 form of the fused gen3d stage), their
+// divergence, the FP64 metric pressure gradient and the gen_ext buoyancy
+template <int NS>
+__global__ void __launch_bounds__(NTS, 1)
+    k_adv3d_sgen(const Grid32 g, const FC k, const double *__restrict__ U, const double *__restrict__ P,
+                 double *__restrict__ dvar, const double *__restrict__ MT, int buoy, int kchunk, int nbx, int nby,
+                 int nbz) {
+  adv_s_tile<NS, false, true>(g, k, U, P, dvar, buoy, kchunk, nbx, nby, nbz, MT);
 }
 
 // ---------------------------------------------------------------- kernel 2
@@ -2048,8 +2096,10 @@ namespace {
 // (src/velocity.hpp:23-45) of cell c of U: rho and p as k_eos_vel3d computes
 // them and the contravariant velocity as k_genvel does (the cartesian
 // velocity k_eos_vel3d writes first is only an intermediate of the reference
-// sequence), into the 5 planes of P.  u: the cell's flow variables.
-template <int NS>
+// sequence), into the 5 planes of P.  u: the cell's flow variables.  F32
+// (FP32 form): the layout of the cartesian FP32 forms, p as doubles at P, then
+// rho and the contravariant velocity rounded to floats.
+template <int NS, bool F32>
 __device__ __forceinline__ void eos_genvel(const FC &k, long long N, long long c, const double *u,
                                            const double *__restrict__ MT, double *__restrict__ P) {
   double rr = 0.0;
@@ -2064,16 +2114,28 @@ __device__ __forceinline__ void eos_genvel(const FC &k, long long N, long long c
   }
   const double gamma = Cp / Cv;
   const double m0 = u[0], m1 = u[1], m2 = u[2];
-  P[c] = rr;
-  P[c + N] = (gamma - 1) * (u[3] - (0.5 / rr) * (m0 * m0 + m1 * m1 + m2 * m2));
+  const double p = (gamma - 1) * (u[3] - (0.5 / rr) * (m0 * m0 + m1 * m1 + m2 * m2));
+  double U3[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d)
-    P[c + N * (2 + d)] = MT[c + N * (d + 0)] * m0 / rr + MT[c + N * (d + 3)] * m1 / rr + MT[c + N * (d + 6)] * m2 / rr;
+    U3[d] = MT[c + N * (d + 0)] * m0 / rr + MT[c + N * (d + 3)] * m1 / rr + MT[c + N * (d + 6)] * m2 / rr;
+  if constexpr (F32) {
+    float *Pf = reinterpret_cast<float *>(P + N);
+    P[c] = p;
+    Pf[c] = (float)rr;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) Pf[c + N * (1 + d)] = (float)U3[d];
+  } else {
+    P[c] = rr;
+    P[c + N] = p;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) P[c + N * (2 + d)] = U3[d];
+  }
 }
 
 // eos_genvel over every cell (ghost_only = 0) or over the ghost shell only
 // (after the gen_ext ghost fill rewrote it).
-template <int NS>
+template <int NS, bool F32>
 __global__ void k_eos_genvel3d(const Grid32 q, const FC k, const double *__restrict__ U,
                                const double *__restrict__ MT, double *__restrict__ P, int ghost_only) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2087,14 +2149,14 @@ __global__ void k_eos_genvel3d(const Grid32 q, const FC k, const double *__restr
   double u[4 + NS];
 #pragma unroll
   for (int v = 0; v < 4 + NS; ++v) u[v] = U[c + N * v];
-  eos_genvel<NS>(k, N, c, u, MT, P);
+  eos_genvel<NS, F32>(k, N, c, u, MT, P);
 }
 
 // The Runge-Kutta stage update of a fused gen3d stage over every cell
 // (rkAdvance, src/rk.cpp:28-72: dvar is zero on the ghosts, where the gen
 // modules never write it), Uo = upd(U, Un, dvar), then eos_genvel of the new
 // state: P describes Uo for the next stage.
-template <int NS>
+template <int NS, bool F32>
 __global__ void k_rk_eos_gen(const Grid32 q, const FC k, const double *__restrict__ U,
                              const double *__restrict__ Un, const double *__restrict__ dvar, double *__restrict__ Uo,
                              const double *__restrict__ MT, double *__restrict__ P, Upd up) {
@@ -2113,28 +2175,38 @@ __global__ void k_rk_eos_gen(const Grid32 q, const FC k, const double *__restric
     u[v] = upd_val(up, us, ub, interior ? dvar[e] : 0.0);
     Uo[e] = u[v];
   }
-  eos_genvel<NS>(k, N, c, u, MT, P);
+  eos_genvel<NS, F32>(k, N, c, u, MT, P);
 }
 
-// visc: the gen_ext viscous terms (launch_viscgen3d on the stage input, whose
-// rho is P's plane 0) after the advection kernel.
-template <int NS>
-void gen_stage(const Grid32 &q, const FC &k, const Launch &L, int kchunk, const double *Uin, const double *Un,
-               double *Uout, const double *MT, double *P, double *dvar, Upd up, int eos, int buoy, const Geo &g,
-               const Phys &ph, const Bufs &b, bool visc, uint32_t ngh, Prof *prof, hipStream_t s) {
-  if (eos == 2) k_eos_genvel3d<NS><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, P, 0);
-  else if (eos == 1) k_eos_genvel3d<NS><<<(ngh + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, P, 1);
+// visc: the gen_ext viscous terms (launch_viscgen3d on the stage input) after
+// the advection kernel; they read rho from P's plane 0 in the FP64 form and
+// sum it from the species densities (the EOS's sum) in the FP32 form, whose
+// rho plane is rounded.  F32: binary32 fluxes (k_adv3d_sgen, 11-row tiles)
+// and float primitive planes; the viscous terms, the pressure gradient, the
+// EOS and the RK accumulation stay FP64.
+template <int NS, bool F32>
+void gen_stage(const Grid32 &q, const FC &k, int kchunk, const double *Uin, const double *Un, double *Uout,
+               const double *MT, double *P, double *dvar, Upd up, int eos, int buoy, const Geo &g, const Phys &ph,
+               const Bufs &b, bool visc, uint32_t ngh, Prof *prof, hipStream_t s) {
+  if (eos == 2) k_eos_genvel3d<NS, F32><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, P, 0);
+  else if (eos == 1) k_eos_genvel3d<NS, F32><<<(ngh + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, P, 1);
   if (prof) prof->mark(s);
-  k_adv3d_gen<NS><<<L.nb, NTHREADS, 0, s>>>(q, k, Uin, P, dvar, MT, buoy, kchunk, L.nbx, L.nby, L.nbz);
+  const Launch L = shape(g, kchunk);
+  if constexpr (F32) {
+    const int nbys = (g.ncj + TYS - 1) / TYS, nbs = L.nbx * nbys * L.nbz;
+    k_adv3d_sgen<NS><<<nbs, NTS, 0, s>>>(q, k, Uin, P, dvar, MT, buoy, kchunk, L.nbx, nbys, L.nbz);
+  } else {
+    k_adv3d_gen<NS><<<L.nb, NTHREADS, 0, s>>>(q, k, Uin, P, dvar, MT, buoy, kchunk, L.nbx, L.nby, L.nbz);
+  }
   if (visc) {
     Bufs vb = b;
     vb.var = const_cast<double *>(Uin);
-    vb.rho = P;
+    vb.rho = F32 ? nullptr : P;
     vb.dvar = dvar;
     launch_viscgen3d(g, ph, vb, s);
   }
   if (prof) prof->mark(s);
-  k_rk_eos_gen<NS><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, Un, dvar, Uout, MT, P, up);
+  k_rk_eos_gen<NS, F32><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, Un, dvar, Uout, MT, P, up);
 }
 
 }  // namespace
@@ -2166,8 +2238,11 @@ int fused_gen_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b,
   }
   const Grid32 q = make_grid(g);
   const FC k = make_fc(ph, cf);
-  constexpr int kchunk = 8;
-  const Launch L = shape(g, kchunk);
+  const bool f32 = cf.precision == FST_FP32_BULK;
+  // planes per block: 8 (FP64), 32 (FP32 form; 11.73, 11.44, 11.38 ms per
+  // stage at 8, 16, 32 on the 512 x 512 x 256 terrain grid, 3 species)
+  int kchunk = f32 ? 32 : 8;
+  if (const char *ev = getenv("FST_KCHUNK")) kchunk = std::max(1, atoi(ev));  // tuning override
   const int buoy = cf.gen_ext && cf.buoyancy;
   const bool visc = cf.gen_ext && cf.visc && b.stress;
   const uint32_t ngh = 2u * q.ng * q.s2 + 2u * q.ng * (uint32_t)q.ngi * q.nck + 2u * q.ng * (uint32_t)q.ncj * q.nck;
@@ -2181,20 +2256,18 @@ int fused_gen_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b,
     if (cf.gen_ext) launch_apply_bcs(g, cf, Uin, s);  // src/cart3d.cpp:221, as gen_ext applies it
     // P describes Uin unless the fill rewrote its ghosts (or var changed)
     const int eos = !fb.valid ? 2 : (cf.gen_ext ? 1 : 0);
-    switch (cf.ns) {
-      case 1:
-        gen_stage<1>(q, k, L, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, g, ph, b, visc,
-                     ngh, prof, s);
-        break;
-      case 2:
-        gen_stage<2>(q, k, L, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, g, ph, b, visc,
-                     ngh, prof, s);
-        break;
-      default:
-        gen_stage<3>(q, k, L, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, g, ph, b, visc,
-                     ngh, prof, s);
-        break;
+#define FST_GEN_STAGE(NS_, F_)                                                                                  \
+  gen_stage<NS_, F_>(q, k, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, g, ph, b, visc, ngh, \
+                     prof, s)
+    switch (cf.ns * 2 + (f32 ? 1 : 0)) {
+      case 2: FST_GEN_STAGE(1, false); break;
+      case 3: FST_GEN_STAGE(1, true); break;
+      case 4: FST_GEN_STAGE(2, false); break;
+      case 5: FST_GEN_STAGE(2, true); break;
+      case 6: FST_GEN_STAGE(3, false); break;
+      default: FST_GEN_STAGE(3, true); break;
     }
+#undef FST_GEN_STAGE
     fb.valid = true;
     if (prof) prof->end(s);
   };

@@ -1333,13 +1333,20 @@ namespace {
 
 // The cartesian velocity var(w) / rho and the mass fractions var(4+s) / rho of
 // every cell into u0, u1, u2 and Y (ns planes): the inputs of the gen_ext
-// viscous terms, the same quotients they would form.
+// viscous terms, the same quotients they would form.  rho == nullptr: rho is
+// summed from the species densities as calculateRhoPT3D sums it
+// (src/secondary.hpp:126-130), so the same value.
 __global__ void k_cartvel3d(long long N, int ns, const double *__restrict__ var, const double *__restrict__ rho,
                             double *__restrict__ u0, double *__restrict__ u1, double *__restrict__ u2,
                             double *__restrict__ Y) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
-  const double r = rho[c];
+  double r = 0.0;
+  if (rho) {
+    r = rho[c];
+  } else {
+    for (int q = 0; q < ns; ++q) r = r + var[c + N * (4 + q)];
+  }
   u0[c] = var[c] / r;
   u1[c] = var[c + N] / r;
   u2[c] = var[c + 2 * N] / r;

@@ -142,8 +142,11 @@ def test_unknown_precision_is_rejected(field, value):
 @pytest.mark.parametrize("precision", [1, 2])
 @pytest.mark.parametrize("case", ["2d", "ns4", "gen3d", "over_cap"])
 def test_fp32_and_fast64_forms_need_the_fused_3d_path(case, precision):
-    """The FP32 form exists only in the fused 3D stage kernels; elsewhere the
-    FP64 sequence would run without notice, so the configuration is refused."""
+    """The FP32 form exists only in the fused 3D stage kernels (cartesian and
+    gen3d), the fast FP64 form only in the cartesian ones; elsewhere the FP64
+    sequence would run without notice, so the configuration is refused."""
+    if case == "gen3d" and precision == 1:
+        pytest.skip("the fused gen3d stage has an FP32 form (test_fp32_form_on_generalized_grids)")
     kw = dict(ndim=3, ns=2, nci=8, ncj=8, nck=8, gamma=(1.4, 1.67), M=(0.029, 0.004), mu=(1e-5, 1e-5), ceq=True)
     if case == "2d":
         kw["ndim"] = 2
@@ -180,6 +183,24 @@ def test_static_step_refuses_before_touching_a_device():
     assert lib.fst_step_static(C.byref(ok), None, 1, None) == 1  # null state
     assert lib.fst_step_static(C.byref(ok), dummy, -1, None) == 1
     assert lib.fst_step_static(None, dummy, 1, None) == 1
+
+
+def test_fp32_form_on_generalized_grids():
+    """precision 1 on grid 1: accepted where the fused gen3d stage runs (3D,
+    WENO5, ns <= 3, arrays below 2^32 elements, with or without gen_ext),
+    refused elsewhere; no learned C-indicator and no fast FP64 form there."""
+    lib = runtime.lib()
+    kw = dict(ndim=3, ns=3, nci=8, ncj=8, nck=8, grid=1, gamma=(1.4, 1.67, 1.09), M=(0.029, 0.004, 0.146),
+              mu=(1e-5,) * 3, precision=1)
+    assert lib.fst_config_dims(C.byref(make_config(**kw)), C.byref(FstDims())) == 0
+    ext = dict(kw, gen_ext=True, buoyancy=True, visc=True, bc=("outflow",) * 6)
+    assert lib.fst_config_dims(C.byref(make_config(**ext)), C.byref(FstDims())) == 0
+    for bad in (dict(kw, ndim=2),
+                dict(kw, ns=4, gamma=(1.4,) * 4, M=(0.029,) * 4, mu=(1e-5,) * 4),
+                dict(kw, nci=1700, ncj=1700, nck=400),  # 1706 x 1706 x 406 cells x 7 planes > 2^32
+                dict(kw, precision=2),
+                dict(kw, learned_c=True, ceq=True)):
+        assert lib.fst_config_dims(C.byref(make_config(**bad)), C.byref(FstDims())) == 1
 
 
 def test_fast64_form_has_no_learned_variant():
