@@ -435,8 +435,8 @@ def terrain_leg(args):
     512 x 256 x 512 cells with y vertical, ground y_g = 0.05 sin(2 pi x)
     cos(2 pi z) and a tanh (beta = 2) vertical stretch, air + hot gas + SF6,
     a 50:1 hot-gas blob over a pooled SF6 layer, gravity, no-slip ground and
-    lid, periodic x and z, SSP-RK3 at CFL 0.4.  FP64 and the FP32 form (the
-    gen_ext viscous terms stay FP64 in it) from the same state: device ms per
+    lid, periodic x and z, SSP-RK3 at CFL 0.4.  FP64 and the FP32 form
+    (binary32 fluxes and viscous faces) from the same state: device ms per
     step from hipEvents over the steps after one warm-up step; then each
     final state is checked (finite, every species' total the same to
     rounding: the closed box keeps mass) and the FP32 form's density,

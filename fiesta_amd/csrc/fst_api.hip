@@ -426,6 +426,7 @@ int fst_upload(fst_ctx *ctx, int array, const double *host, size_t count) {
   HIPCHK(ctx, hipMemcpyAsync(p, host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   if (array == FST_VAR) ctx->fb.valid = false;
+  ctx->fb.mt32_ok = false;  // the float metrics follow the next preSim or upload
   return FST_OK;
 }
 
@@ -450,6 +451,7 @@ int fst_apply_bcs(fst_ctx *ctx) {
 int fst_pre_sim(fst_ctx *ctx) {
   if (!ctx) return FST_ERR_ARG;
   ctx->fb.valid = false;
+  ctx->fb.mt32_ok = false;
   if (ctx->cf.grid == 1) {  // gen3d_func::preSim / gen2d_func::preSim: metrics only
     launch_metrics(ctx->g, ctx->b, ctx->stream);
     // gen_ext in 2D: the ghost fill of cart2d's preSim (src/cart2d.cpp:137)

@@ -12,6 +12,8 @@ struct FusedBufs {
   double *stage2 = nullptr;  // SSP-RK3: U2 (stage-2 output), nvt planes
   double *prim = nullptr;    // two sets of primitive planes rho, p, u, v, w, nrg, Y_s
   Scalars *sc = nullptr;     // two sets of per-stage maxima
+  float *mt32 = nullptr;     // gen3d FP32 form with viscous terms: the 9 metric planes as floats
+  bool mt32_ok = false;      // mt32 holds the current metrics
   int kchunk = 0;            // k planes per block (0: default)
   bool valid = false;        // prim[cur] / sc[cur] describe the current var (interior)
   int cur = 0;               // which set describes var (an SSP-RK3 step flips it)

@@ -2181,13 +2181,14 @@ __global__ void k_rk_eos_gen(const Grid32 q, const FC k, const double *__restric
 // visc: the gen_ext viscous terms (launch_viscgen3d on the stage input) after
 // the advection kernel; they read rho from P's plane 0 in the FP64 form and
 // sum it from the species densities (the EOS's sum) in the FP32 form, whose
-// rho plane is rounded.  F32: binary32 fluxes (k_adv3d_sgen, 11-row tiles)
-// and float primitive planes; the viscous terms, the pressure gradient, the
-// EOS and the RK accumulation stay FP64.
+// rho plane is rounded.  F32: binary32 fluxes (k_adv3d_sgen, 11-row tiles),
+// float primitive planes and binary32 viscous faces on float velocities,
+// mass fractions and metrics (mt32); the pressure gradient, the EOS and the
+// RK accumulation (dvar included) stay FP64.
 template <int NS, bool F32>
 void gen_stage(const Grid32 &q, const FC &k, int kchunk, const double *Uin, const double *Un, double *Uout,
                const double *MT, double *P, double *dvar, Upd up, int eos, int buoy, const Geo &g, const Phys &ph,
-               const Bufs &b, bool visc, uint32_t ngh, Prof *prof, hipStream_t s) {
+               const Bufs &b, bool visc, const float *mt32, uint32_t ngh, Prof *prof, hipStream_t s) {
   if (eos == 2) k_eos_genvel3d<NS, F32><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, P, 0);
   else if (eos == 1) k_eos_genvel3d<NS, F32><<<(ngh + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, P, 1);
   if (prof) prof->mark(s);
@@ -2203,7 +2204,7 @@ void gen_stage(const Grid32 &q, const FC &k, int kchunk, const double *Uin, cons
     vb.var = const_cast<double *>(Uin);
     vb.rho = F32 ? nullptr : P;
     vb.dvar = dvar;
-    launch_viscgen3d(g, ph, vb, s);
+    launch_viscgen3d(g, ph, vb, s, F32 ? mt32 : nullptr);
   }
   if (prof) prof->mark(s);
   k_rk_eos_gen<NS, F32><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, Un, dvar, Uout, MT, P, up);
@@ -2245,6 +2246,14 @@ int fused_gen_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b,
   if (const char *ev = getenv("FST_KCHUNK")) kchunk = std::max(1, atoi(ev));  // tuning override
   const int buoy = cf.gen_ext && cf.buoyancy;
   const bool visc = cf.gen_ext && cf.visc && b.stress;
+  if (f32 && visc && !fb.mt32_ok) {  // float metrics of the FP32 viscous faces (cleared by preSim and uploads)
+    if (!fb.mt32 && hipMalloc((void **)&fb.mt32, (size_t)g.N * 9 * sizeof(float)) != hipSuccess) {
+      *err = "fused gen3d path: device allocation failed";
+      return FST_ERR_NOMEM;
+    }
+    launch_metrics_f32(g, b.metrics, fb.mt32, s);
+    fb.mt32_ok = true;
+  }
   const uint32_t ngh = 2u * q.ng * q.s2 + 2u * q.ng * (uint32_t)q.ngi * q.nck + 2u * q.ng * (uint32_t)q.ncj * q.nck;
   Prof *prof = b.prof && b.prof->on ? b.prof : nullptr;
   if (prof) {
@@ -2257,8 +2266,8 @@ int fused_gen_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b,
     // P describes Uin unless the fill rewrote its ghosts (or var changed)
     const int eos = !fb.valid ? 2 : (cf.gen_ext ? 1 : 0);
 #define FST_GEN_STAGE(NS_, F_)                                                                                  \
-  gen_stage<NS_, F_>(q, k, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, g, ph, b, visc, ngh, \
-                     prof, s)
+  gen_stage<NS_, F_>(q, k, kchunk, Uin, b.var, Uout, b.metrics, b.prim, b.dvar, up, eos, buoy, g, ph, b, visc,   \
+                     fb.mt32, ngh, prof, s)
     switch (cf.ns * 2 + (f32 ? 1 : 0)) {
       case 2: FST_GEN_STAGE(1, false); break;
       case 3: FST_GEN_STAGE(1, true); break;
@@ -2436,8 +2445,11 @@ void fused_free(FusedBufs &fb) {
   if (fb.stage2) (void)hipFree(fb.stage2);
   if (fb.prim) (void)hipFree(fb.prim);
   if (fb.sc) (void)hipFree(fb.sc);
+  if (fb.mt32) (void)hipFree(fb.mt32);
   fb.stage = fb.stage2 = fb.prim = nullptr;
   fb.sc = nullptr;
+  fb.mt32 = nullptr;
+  fb.mt32_ok = false;
 }
 #endif  // FST_FAST64_TU
 

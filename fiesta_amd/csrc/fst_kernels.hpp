@@ -163,9 +163,11 @@ inline double key_to_double(unsigned long long k) {
 void launch_secondary(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
 // generalized grids (gen2d/gen3d modules)
 void launch_compute_gen(const Geo &g, const Phys &ph, const fst_config &cf, const Bufs &b, hipStream_t s);
-// gen_ext viscous terms of gen3d in metric form (face fluxes through b.stress,
-// then their differences into b.dvar), per direction x, y, z.
-void launch_viscgen3d(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s);
+// gen_ext viscous terms of gen3d in metric form (the faces of the three
+// directions and their differences into b.dvar, x, y, z); mt32: the FP32
+// form, with the metric planes as floats (launch_metrics_f32).
+void launch_viscgen3d(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s, const float *mt32 = nullptr);
+void launch_metrics_f32(const Geo &g, const double *mt, float *mt32, hipStream_t s);
 void launch_metrics(const Geo &g, const Bufs &b, hipStream_t s);
 
 }  // namespace fst

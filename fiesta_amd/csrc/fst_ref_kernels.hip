@@ -1335,10 +1335,11 @@ namespace {
 // every cell into u0, u1, u2 and Y (ns planes): the inputs of the gen_ext
 // viscous terms, the same quotients they would form.  rho == nullptr: rho is
 // summed from the species densities as calculateRhoPT3D sums it
-// (src/secondary.hpp:126-130), so the same value.
+// (src/secondary.hpp:126-130), so the same value.  T = float (FP32 form):
+// the FP64 quotients rounded once.
+template <typename T>
 __global__ void k_cartvel3d(long long N, int ns, const double *__restrict__ var, const double *__restrict__ rho,
-                            double *__restrict__ u0, double *__restrict__ u1, double *__restrict__ u2,
-                            double *__restrict__ Y) {
+                            T *__restrict__ u0, T *__restrict__ u1, T *__restrict__ u2, T *__restrict__ Y) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
   double r = 0.0;
@@ -1347,10 +1348,16 @@ __global__ void k_cartvel3d(long long N, int ns, const double *__restrict__ var,
   } else {
     for (int q = 0; q < ns; ++q) r = r + var[c + N * (4 + q)];
   }
-  u0[c] = var[c] / r;
-  u1[c] = var[c + N] / r;
-  u2[c] = var[c + 2 * N] / r;
-  for (int q = 0; q < ns; ++q) Y[c + N * q] = var[c + N * (4 + q)] / r;
+  u0[c] = (T)(var[c] / r);
+  u1[c] = (T)(var[c + N] / r);
+  u2[c] = (T)(var[c + 2 * N] / r);
+  for (int q = 0; q < ns; ++q) Y[c + N * q] = (T)(var[c + N * (4 + q)] / r);
+}
+
+// the 9 metric planes rounded to floats (the FP32 form's viscous terms)
+__global__ void k_metrics_f32(long long n, const double *__restrict__ mt, float *__restrict__ m32) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) m32[c] = (float)mt[c];
 }
 
 // gen_ext viscous terms on a generalized 3D grid (not in the reference: the
@@ -1367,48 +1374,48 @@ __global__ void k_cartvel3d(long long N, int ns, const double *__restrict__ var,
 // On a uniform grid of power-of-two spacing each value is the cartesian
 // one scaled by a power of two, so the module equals cart3d bit for bit.
 // Cell offsets: the face lies between cell c and c + e_DIR; sy, sz the j and
-// k strides.
-template <int DIR>
+// k strides.  T = float: the FP32 form (float inputs, binary32 arithmetic).
+template <int DIR, typename T>
 __device__ __forceinline__ void viscgen3d_face(long long N, long long sy, long long sz, int ns, const Phys &ph,
-                                               long long c, const double *__restrict__ Y,
-                                               const double *__restrict__ u0, const double *__restrict__ u1,
-                                               const double *__restrict__ u2, const double *__restrict__ mt,
-                                               double *out) {
+                                               long long c, const T *__restrict__ Y, const T *__restrict__ u0,
+                                               const T *__restrict__ u1, const T *__restrict__ u2,
+                                               const T *__restrict__ mt, T *out) {
   const long long st[3] = {1, sy, sz};
   const long long e = st[DIR], cn = c + e;
-  const double *const uw[3] = {u0, u1, u2};
-  double mu = 0.0;
-  for (int s = 0; s < ns; ++s) mu += ((Y[cn + N * s] + Y[c + N * s]) / 2.0) * ph.mus[s];
-  double gr[3][3], mf[3][3], du[3][3], uf[3];
+  const T *const uw[3] = {u0, u1, u2};
+  T mu = 0;
+  for (int s = 0; s < ns; ++s) mu += ((Y[cn + N * s] + Y[c + N * s]) / (T)2) * (T)ph.mus[s];
+  T gr[3][3], mf[3][3], du[3][3], uf[3];
 #pragma unroll
   for (int w = 0; w < 3; ++w) {
-    const double *u = uw[w];
+    const T *u = uw[w];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if (a == DIR) {
         gr[w][a] = u[cn] - u[c];
       } else {
         const long long t = st[a];
-        gr[w][a] = ((u[c + t] + u[cn + t]) - (u[c - t] + u[cn - t])) / 4.0;
+        gr[w][a] = ((u[c + t] + u[cn + t]) - (u[c - t] + u[cn - t])) / (T)4;
       }
     }
-    uf[w] = (u[cn] + u[c]) / 2.0;
+    uf[w] = (u[cn] + u[c]) / (T)2;
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int n = 0; n < 3; ++n) mf[a][n] = (mt[c + N * (a + 3 * n)] + mt[cn + N * (a + 3 * n)]) / 2.0;
+    for (int n = 0; n < 3; ++n) mf[a][n] = (mt[c + N * (a + 3 * n)] + mt[cn + N * (a + 3 * n)]) / (T)2;
 #pragma unroll
   for (int w = 0; w < 3; ++w)
 #pragma unroll
     for (int n = 0; n < 3; ++n) du[w][n] = (mf[0][n] * gr[w][0] + mf[1][n] * gr[w][1]) + mf[2][n] * gr[w][2];
-  const double dudx = du[0][0], dudy = du[0][1], dudz = du[0][2];
-  const double dvdx = du[1][0], dvdy = du[1][1], dvdz = du[1][2];
-  const double dwdx = du[2][0], dwdy = du[2][1], dwdz = du[2][2];
-  double t[3][3];
-  t[0][0] = (2.0 / 3.0) * mu * (2.0 * dudx - dvdy - dwdz);
-  t[1][1] = (2.0 / 3.0) * mu * (2.0 * dvdy - dudx - dwdz);
-  t[2][2] = (2.0 / 3.0) * mu * (2.0 * dwdz - dudx - dvdy);
+  const T dudx = du[0][0], dudy = du[0][1], dudz = du[0][2];
+  const T dvdx = du[1][0], dvdy = du[1][1], dvdz = du[1][2];
+  const T dwdx = du[2][0], dwdy = du[2][1], dwdz = du[2][2];
+  const T c23 = (T)(2.0 / 3.0);
+  T t[3][3];
+  t[0][0] = c23 * mu * ((T)2 * dudx - dvdy - dwdz);
+  t[1][1] = c23 * mu * ((T)2 * dvdy - dudx - dwdz);
+  t[2][2] = c23 * mu * ((T)2 * dwdz - dudx - dvdy);
   t[0][1] = t[1][0] = mu * (dudy + dvdx);
   t[0][2] = t[2][0] = mu * (dudz + dwdx);
   t[1][2] = t[2][1] = mu * (dvdz + dwdy);
@@ -1429,14 +1436,17 @@ __device__ __forceinline__ void viscgen3d_face(long long N, long long sy, long l
 // each cell adds x, y, z in that order, as the per-direction kernels of the
 // reference sequence do (applyViscousTerm, src/viscosity.hpp:460-498), so
 // the values are those of the face-by-face evaluation.
+// T = float: the FP32 form; the face differences are formed in binary32 and
+// added to the FP64 dvar.
 constexpr int VW = 7, VTX = 63, VKC = 16;
-__global__ void __launch_bounds__(64 * (VW + 1)) k_viscgen3d_march(Geo g, Phys ph, const double *__restrict__ Y,
-                                                                     const double *__restrict__ u0,
-                                                                     const double *__restrict__ u1,
-                                                                     const double *__restrict__ u2,
-                                                                     const double *__restrict__ mt,
+template <typename T>
+__global__ void __launch_bounds__(64 * (VW + 1)) k_viscgen3d_march(Geo g, Phys ph, const T *__restrict__ Y,
+                                                                     const T *__restrict__ u0,
+                                                                     const T *__restrict__ u1,
+                                                                     const T *__restrict__ u2,
+                                                                     const T *__restrict__ mt,
                                                                      double *__restrict__ dvar, int nbx, int nby) {
-  __shared__ double ysh[VW + 1][6][64];
+  __shared__ T ysh[VW + 1][6][64];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
   const int i0 = g.ng + bx * VTX, j0 = g.ng + by * VW;
@@ -1447,19 +1457,19 @@ __global__ void __launch_bounds__(64 * (VW + 1)) k_viscgen3d_march(Geo g, Phys p
   const bool rowok = w < VW && j < g.ng + g.ncj;
   const bool yface = l > 0 && colok && j - 1 < g.ng + g.ncj;      // y face below row j0 + w
   const long long N = g.N, sy = g.ngi, sz = (long long)g.ngi * g.ngj;
-  double Zp[6] = {0, 0, 0, 0, 0, 0};
-  if (own) viscgen3d_face<2>(N, sy, sz, g.ns, ph, g.id(i, j, kb - 1), Y, u0, u1, u2, mt, Zp);
+  T Zp[6] = {0, 0, 0, 0, 0, 0};
+  if (own) viscgen3d_face<2, T>(N, sy, sz, g.ns, ph, g.id(i, j, kb - 1), Y, u0, u1, u2, mt, Zp);
   for (int k = kb; k < ke; ++k) {
     const long long c = g.id(i, j, k);
-    double X[6] = {0, 0, 0, 0, 0, 0}, Yf[6], Z[6];
-    if (rowok && colok) viscgen3d_face<0>(N, sy, sz, g.ns, ph, c, Y, u0, u1, u2, mt, X);
+    T X[6] = {0, 0, 0, 0, 0, 0}, Yf[6], Z[6];
+    if (rowok && colok) viscgen3d_face<0, T>(N, sy, sz, g.ns, ph, c, Y, u0, u1, u2, mt, X);
     if (yface) {
-      viscgen3d_face<1>(N, sy, sz, g.ns, ph, c - sy, Y, u0, u1, u2, mt, Yf);
+      viscgen3d_face<1, T>(N, sy, sz, g.ns, ph, c - sy, Y, u0, u1, u2, mt, Yf);
 #pragma unroll
       for (int q = 0; q < 6; ++q) ysh[w][q][l] = Yf[q];
     }
-    if (own) viscgen3d_face<2>(N, sy, sz, g.ns, ph, c, Y, u0, u1, u2, mt, Z);
-    double Xm[6];
+    if (own) viscgen3d_face<2, T>(N, sy, sz, g.ns, ph, c, Y, u0, u1, u2, mt, Z);
+    T Xm[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) Xm[q] = __shfl_up(X[q], 1, 64);
     __syncthreads();
@@ -1494,12 +1504,26 @@ __global__ void __launch_bounds__(64 * (VW + 1)) k_viscgen3d_march(Geo g, Phys p
 // Directions x, y, z in that order (src/cart3d.cpp:265-270); fx, fy, fz (the
 // per-variable flux planes, free once the fluxes are differenced) hold the
 // cartesian velocity, the stress planes (max(6, ns) on generalized grids) the
-// mass fractions.
-void launch_viscgen3d(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s) {
-  k_cartvel3d<<<(unsigned)((g.N + 255) / 256), 256, 0, s>>>(g.N, g.ns, b.var, b.rho, b.fx, b.fy, b.fz, b.stress);
+// mass fractions.  mt32 (FP32 form): the metric planes as floats; the
+// velocity and mass fractions are then stored as floats in the same planes.
+void launch_viscgen3d(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s, const float *mt32) {
+  const unsigned nb = (unsigned)((g.N + 255) / 256);
   const int nbx = (g.nci + VTX - 1) / VTX, nby = (g.ncj + VW - 1) / VW, nbz = (g.nck + VKC - 1) / VKC;
-  k_viscgen3d_march<<<nbx * nby * nbz, 64 * (VW + 1), 0, s>>>(g, ph, b.stress, b.fx, b.fy, b.fz, b.metrics,
-                                                              b.dvar, nbx, nby);
+  if (mt32) {
+    auto F = [](double *p) { return reinterpret_cast<float *>(p); };
+    k_cartvel3d<float><<<nb, 256, 0, s>>>(g.N, g.ns, b.var, b.rho, F(b.fx), F(b.fy), F(b.fz), F(b.stress));
+    k_viscgen3d_march<float><<<nbx * nby * nbz, 64 * (VW + 1), 0, s>>>(g, ph, F(b.stress), F(b.fx), F(b.fy),
+                                                                       F(b.fz), mt32, b.dvar, nbx, nby);
+    return;
+  }
+  k_cartvel3d<double><<<nb, 256, 0, s>>>(g.N, g.ns, b.var, b.rho, b.fx, b.fy, b.fz, b.stress);
+  k_viscgen3d_march<double><<<nbx * nby * nbz, 64 * (VW + 1), 0, s>>>(g, ph, b.stress, b.fx, b.fy, b.fz,
+                                                                      b.metrics, b.dvar, nbx, nby);
+}
+
+void launch_metrics_f32(const Geo &g, const double *mt, float *mt32, hipStream_t s) {
+  const long long n = 9LL * g.N;
+  k_metrics_f32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, mt, mt32);
 }
 
 // gen3d_func::compute (src/gen3d.cpp:341-369) / gen2d_func::compute
