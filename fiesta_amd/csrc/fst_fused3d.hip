@@ -923,8 +923,8 @@ __device__ __forceinline__ void face_s(const float *qs, float rd, int r, int q, 
 // FI: U is the float stage state of the learned FP32 form.  GEN: the
 // generalized-grid module (P's float planes hold the contravariant velocity,
 // MT the 9 metric planes): applyGenPressureGradient3D's metric pressure
-// gradient (src/presgrad.hpp:72-147) in FP64 in place of the cartesian one,
-// and the partial RHS stored as FP64 (the gen_ext viscous terms add to it).
+// gradient (src/presgrad.hpp:72-147) in FP64 in place of the cartesian one.
+// The partial RHS is stored as FP32 in both.
 template <int NS, bool FI, bool GEN>
 __device__ __forceinline__ void adv_s_tile(const Grid32 &g, const FC &k, const double *__restrict__ U,
                                            const double *__restrict__ P, double *__restrict__ dvar, int buoy,
@@ -1056,7 +1056,7 @@ __device__ __forceinline__ void adv_s_tile(const Grid32 &g, const FC &k, const d
         dv[3] += vv * f;
       }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) st_dv<!GEN>(dvar, c + N * v, dv[v]);
+      for (int v = 0; v < NV; ++v) st_dv<true>(dvar, c + N * v, dv[v]);
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) Zp[q] = Z[q];
@@ -2099,9 +2099,9 @@ namespace {
 // sequence), into the 5 planes of P.  u: the cell's flow variables.  F32
 // (FP32 form): the layout of the cartesian FP32 forms, p as doubles at P, then
 // rho and the contravariant velocity rounded to floats.
-template <int NS, bool F32>
+template <int NS, bool F32, typename MTT>
 __device__ __forceinline__ void eos_genvel(const FC &k, long long N, long long c, const double *u,
-                                           const double *__restrict__ MT, double *__restrict__ P) {
+                                           const MTT *__restrict__ MT, double *__restrict__ P) {
   double rr = 0.0;
 #pragma unroll
   for (int s = 0; s < NS; ++s) rr = rr + u[4 + s];
@@ -2118,7 +2118,8 @@ __device__ __forceinline__ void eos_genvel(const FC &k, long long N, long long c
   double U3[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d)
-    U3[d] = MT[c + N * (d + 0)] * m0 / rr + MT[c + N * (d + 3)] * m1 / rr + MT[c + N * (d + 6)] * m2 / rr;
+    U3[d] = (double)MT[c + N * (d + 0)] * m0 / rr + (double)MT[c + N * (d + 3)] * m1 / rr +
+            (double)MT[c + N * (d + 6)] * m2 / rr;
   if constexpr (F32) {
     float *Pf = reinterpret_cast<float *>(P + N);
     P[c] = p;
@@ -2134,10 +2135,11 @@ __device__ __forceinline__ void eos_genvel(const FC &k, long long N, long long c
 }
 
 // eos_genvel over every cell (ghost_only = 0) or over the ghost shell only
-// (after the gen_ext ghost fill rewrote it).
+// (after the gen_ext ghost fill rewrote it).  F32: float metrics (MT32).
 template <int NS, bool F32>
 __global__ void k_eos_genvel3d(const Grid32 q, const FC k, const double *__restrict__ U,
-                               const double *__restrict__ MT, double *__restrict__ P, int ghost_only) {
+                               const double *__restrict__ MT, const float *__restrict__ MT32,
+                               double *__restrict__ P, int ghost_only) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t cell = t;
   if (ghost_only) {
@@ -2149,17 +2151,20 @@ __global__ void k_eos_genvel3d(const Grid32 q, const FC k, const double *__restr
   double u[4 + NS];
 #pragma unroll
   for (int v = 0; v < 4 + NS; ++v) u[v] = U[c + N * v];
-  eos_genvel<NS, F32>(k, N, c, u, MT, P);
+  if constexpr (F32) eos_genvel<NS, F32>(k, N, c, u, MT32, P);
+  else eos_genvel<NS, F32>(k, N, c, u, MT, P);
 }
 
 // The Runge-Kutta stage update of a fused gen3d stage over every cell
 // (rkAdvance, src/rk.cpp:28-72: dvar is zero on the ghosts, where the gen
 // modules never write it), Uo = upd(U, Un, dvar), then eos_genvel of the new
 // state: P describes Uo for the next stage.
+// F32: the partial RHS as floats, float metrics (MT32).
 template <int NS, bool F32>
 __global__ void k_rk_eos_gen(const Grid32 q, const FC k, const double *__restrict__ U,
                              const double *__restrict__ Un, const double *__restrict__ dvar, double *__restrict__ Uo,
-                             const double *__restrict__ MT, double *__restrict__ P, Upd up) {
+                             const double *__restrict__ MT, const float *__restrict__ MT32, double *__restrict__ P,
+                             Upd up) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= q.N) return;
   const int i = (int)(t % (uint32_t)q.ngi), j = (int)((t / (uint32_t)q.ngi) % (uint32_t)q.ngj), kk = (int)(t / q.s2);
@@ -2172,10 +2177,11 @@ __global__ void k_rk_eos_gen(const Grid32 q, const FC k, const double *__restric
     const long long e = c + N * v;
     const double us = up.form != 2 ? U[e] : 0.0;
     const double ub = up.form != 1 ? Un[e] : 0.0;
-    u[v] = upd_val(up, us, ub, interior ? dvar[e] : 0.0);
+    u[v] = upd_val(up, us, ub, interior ? ld_dv<F32>(dvar, (uint32_t)e) : 0.0);
     Uo[e] = u[v];
   }
-  eos_genvel<NS, F32>(k, N, c, u, MT, P);
+  if constexpr (F32) eos_genvel<NS, F32>(k, N, c, u, MT32, P);
+  else eos_genvel<NS, F32>(k, N, c, u, MT, P);
 }
 
 // visc: the gen_ext viscous terms (launch_viscgen3d on the stage input) after
@@ -2183,14 +2189,15 @@ __global__ void k_rk_eos_gen(const Grid32 q, const FC k, const double *__restric
 // sum it from the species densities (the EOS's sum) in the FP32 form, whose
 // rho plane is rounded.  F32: binary32 fluxes (k_adv3d_sgen, 11-row tiles),
 // float primitive planes and binary32 viscous faces on float velocities,
-// mass fractions and metrics (mt32); the pressure gradient, the EOS and the
-// RK accumulation (dvar included) stay FP64.
+// mass fractions and metrics (mt32, which also gives the contravariant
+// velocity), the partial RHS as floats; the pressure gradient, the EOS and
+// the RK accumulation stay FP64.
 template <int NS, bool F32>
 void gen_stage(const Grid32 &q, const FC &k, int kchunk, const double *Uin, const double *Un, double *Uout,
                const double *MT, double *P, double *dvar, Upd up, int eos, int buoy, const Geo &g, const Phys &ph,
                const Bufs &b, bool visc, const float *mt32, uint32_t ngh, Prof *prof, hipStream_t s) {
-  if (eos == 2) k_eos_genvel3d<NS, F32><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, P, 0);
-  else if (eos == 1) k_eos_genvel3d<NS, F32><<<(ngh + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, P, 1);
+  if (eos == 2) k_eos_genvel3d<NS, F32><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, mt32, P, 0);
+  else if (eos == 1) k_eos_genvel3d<NS, F32><<<(ngh + 255) / 256, 256, 0, s>>>(q, k, Uin, MT, mt32, P, 1);
   if (prof) prof->mark(s);
   const Launch L = shape(g, kchunk);
   if constexpr (F32) {
@@ -2207,7 +2214,7 @@ void gen_stage(const Grid32 &q, const FC &k, int kchunk, const double *Uin, cons
     launch_viscgen3d(g, ph, vb, s, F32 ? mt32 : nullptr);
   }
   if (prof) prof->mark(s);
-  k_rk_eos_gen<NS, F32><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, Un, dvar, Uout, MT, P, up);
+  k_rk_eos_gen<NS, F32><<<(q.N + 255) / 256, 256, 0, s>>>(q, k, Uin, Un, dvar, Uout, MT, mt32, P, up);
 }
 
 }  // namespace
@@ -2246,7 +2253,7 @@ int fused_gen_steps(const Geo &g, const Phys &ph, const fst_config &cf, Bufs &b,
   if (const char *ev = getenv("FST_KCHUNK")) kchunk = std::max(1, atoi(ev));  // tuning override
   const int buoy = cf.gen_ext && cf.buoyancy;
   const bool visc = cf.gen_ext && cf.visc && b.stress;
-  if (f32 && visc && !fb.mt32_ok) {  // float metrics of the FP32 viscous faces (cleared by preSim and uploads)
+  if (f32 && !fb.mt32_ok) {  // float metrics of the FP32 form (cleared by preSim and uploads)
     if (!fb.mt32 && hipMalloc((void **)&fb.mt32, (size_t)g.N * 9 * sizeof(float)) != hipSuccess) {
       *err = "fused gen3d path: device allocation failed";
       return FST_ERR_NOMEM;

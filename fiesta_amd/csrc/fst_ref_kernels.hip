@@ -1437,7 +1437,7 @@ __device__ __forceinline__ void viscgen3d_face(long long N, long long sy, long l
 // reference sequence do (applyViscousTerm, src/viscosity.hpp:460-498), so
 // the values are those of the face-by-face evaluation.
 // T = float: the FP32 form; the face differences are formed in binary32 and
-// added to the FP64 dvar.
+// added in FP64 to the float partial RHS, which is stored back as a float.
 constexpr int VW = 7, VTX = 63, VKC = 16;
 template <typename T>
 __global__ void __launch_bounds__(64 * (VW + 1)) k_viscgen3d_march(Geo g, Phys ph, const T *__restrict__ Y,
@@ -1445,7 +1445,7 @@ __global__ void __launch_bounds__(64 * (VW + 1)) k_viscgen3d_march(Geo g, Phys p
                                                                      const T *__restrict__ u1,
                                                                      const T *__restrict__ u2,
                                                                      const T *__restrict__ mt,
-                                                                     double *__restrict__ dvar, int nbx, int nby) {
+                                                                     T *__restrict__ dvar, int nbx, int nby) {
   __shared__ T ysh[VW + 1][6][64];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
@@ -1474,7 +1474,8 @@ __global__ void __launch_bounds__(64 * (VW + 1)) k_viscgen3d_march(Geo g, Phys p
     for (int q = 0; q < 6; ++q) Xm[q] = __shfl_up(X[q], 1, 64);
     __syncthreads();
     if (own) {
-      double d0 = dvar[c], d1 = dvar[c + N], d2 = dvar[c + 2 * N], d3 = dvar[c + 3 * N];
+      double d0 = (double)dvar[c], d1 = (double)dvar[c + N], d2 = (double)dvar[c + 2 * N],
+             d3 = (double)dvar[c + 3 * N];
       d0 += X[0] - Xm[0];
       d1 += X[1] - Xm[1];
       d2 += X[2] - Xm[2];
@@ -1488,10 +1489,10 @@ __global__ void __launch_bounds__(64 * (VW + 1)) k_viscgen3d_march(Geo g, Phys p
       d1 += Z[1] - Zp[1];
       d2 += Z[2] - Zp[2];
       d3 += ((Z[3] - Zp[3]) + (Z[4] - Zp[4])) + (Z[5] - Zp[5]);
-      dvar[c] = d0;
-      dvar[c + N] = d1;
-      dvar[c + 2 * N] = d2;
-      dvar[c + 3 * N] = d3;
+      dvar[c] = (T)d0;
+      dvar[c + N] = (T)d1;
+      dvar[c + 2 * N] = (T)d2;
+      dvar[c + 3 * N] = (T)d3;
 #pragma unroll
       for (int q = 0; q < 6; ++q) Zp[q] = Z[q];
     }
@@ -1505,7 +1506,8 @@ __global__ void __launch_bounds__(64 * (VW + 1)) k_viscgen3d_march(Geo g, Phys p
 // per-variable flux planes, free once the fluxes are differenced) hold the
 // cartesian velocity, the stress planes (max(6, ns) on generalized grids) the
 // mass fractions.  mt32 (FP32 form): the metric planes as floats; the
-// velocity and mass fractions are then stored as floats in the same planes.
+// velocity and mass fractions are then stored as floats in the same planes,
+// and b.dvar holds the partial RHS as floats.
 void launch_viscgen3d(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s, const float *mt32) {
   const unsigned nb = (unsigned)((g.N + 255) / 256);
   const int nbx = (g.nci + VTX - 1) / VTX, nby = (g.ncj + VW - 1) / VW, nbz = (g.nck + VKC - 1) / VKC;
@@ -1513,7 +1515,7 @@ void launch_viscgen3d(const Geo &g, const Phys &ph, const Bufs &b, hipStream_t s
     auto F = [](double *p) { return reinterpret_cast<float *>(p); };
     k_cartvel3d<float><<<nb, 256, 0, s>>>(g.N, g.ns, b.var, b.rho, F(b.fx), F(b.fy), F(b.fz), F(b.stress));
     k_viscgen3d_march<float><<<nbx * nby * nbz, 64 * (VW + 1), 0, s>>>(g, ph, F(b.stress), F(b.fx), F(b.fy),
-                                                                       F(b.fz), mt32, b.dvar, nbx, nby);
+                                                                       F(b.fz), mt32, F(b.dvar), nbx, nby);
     return;
   }
   k_cartvel3d<double><<<nb, 256, 0, s>>>(g.N, g.ns, b.var, b.rho, b.fx, b.fy, b.fz, b.stress);
