@@ -1277,10 +1277,12 @@ __device__ __forceinline__ typename Ar<F32>::R field_at(uint32_t N, int nv, cons
 
 // Loads of one plane p of the ring, field-major: thread t takes the ring
 // cells t, t + NTHREADS, ... of every field (compile-time field choice).
+// Without the C-equation (ceq = 0) var has no C planes: the ring's C fields
+// are zeros then (the faces read them only with ceq set).
 template <int NS, bool F32, bool LC>
 __device__ __forceinline__ void ring_fetch(const Grid32 &g, const double *__restrict__ U,
                                            const double *__restrict__ P, const float *__restrict__ CH,
-                                           const float *__restrict__ DR, int i0, int j0, int p,
+                                           const float *__restrict__ DR, int i0, int j0, int p, int ceq,
                                            typename Ar<F32>::R *pre) {
   using RF = RingF<NS, F32, LC>;
   using R = typename RF::R;
@@ -1293,7 +1295,8 @@ __device__ __forceinline__ void ring_fetch(const Grid32 &g, const double *__rest
     const uint32_t c = ok ? g.id(i, j, p) : 0u;
 #pragma unroll
     for (int f = 0; f < RF::NF; ++f)
-      pre[f * RF::NPC + m] = ok ? field_at<NS, F32, LC>(g.N, g.nv, U, P, CH, DR, f, c) : (R)0;
+      pre[f * RF::NPC + m] =
+          ok && (f < RF::FC || LC || ceq) ? field_at<NS, F32, LC>(g.N, g.nv, U, P, CH, DR, f, c) : (R)0;
   }
 }
 
@@ -1443,7 +1446,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
   R pre[RF::NPRE];
   // planes kb-1, kb, kb+1 of the ring
   for (int p = t.kb - 1; p <= t.kb + 1; ++p) {
-    ring_fetch<NS, F32, LC>(g, U, P, CH, DR, t.i0, t.j0, p, pre);
+    ring_fetch<NS, F32, LC>(g, U, P, CH, DR, t.i0, t.j0, p, ceq, pre);
     ring_store<NS, F32, LC>(ring, p & 3, pre);
   }
   __syncthreads();
@@ -1466,7 +1469,7 @@ __global__ void __launch_bounds__(NTHREADS, 1)
     uint32_t cxb = g.id(t.i0 - 1, t.j0 + l, kk);
     asm volatile("" : "+v"(c), "+v"(cxb));
     const bool fetch = kk + 2 <= t.ke;  // plane kk+2 is read by iteration kk+1
-    if (fetch) ring_fetch<NS, F32, LC>(g, U, P, CH, DR, t.i0, t.j0, kk + 2, pre);
+    if (fetch) ring_fetch<NS, F32, LC>(g, U, P, CH, DR, t.i0, t.j0, kk + 2, ceq, pre);
     // FP32 forms (registers to spare): the cell's partial RHS and stage base
     // are loaded before the face phase so that their latency overlaps it (in
     // the FP64 form this measured slower)

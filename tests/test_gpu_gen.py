@@ -348,3 +348,47 @@ def test_gen3d_fp32_form_runs_through_fst_step_only():
         m.set_fused(False)
     m.set_fused(True)
     m.close()
+
+
+@pytest.mark.parametrize("rk", [2, 3])
+def test_gen3d_fp32_equals_cartesian_fp32_on_uniform_power_of_two_grid(rk):
+    """The FP32 form of the fused gen3d stage with gen_ext on a uniform grid of
+    spacing 2^-5 follows the cartesian FP32 form with the same walls and
+    gravity bit for bit over 3 steps, ghosts included: the metrics are exact
+    powers of two (in FP64 and as floats), so every contravariant velocity,
+    binary32 face velocity and flux is the cartesian one scaled by a power of
+    two, and the metric pressure gradient equals the cartesian one as in FP64.
+    This ties the generalized FP32 kernels (k_adv3d_sgen, k_rk_eos_gen<true>)
+    to the cartesian FP32 kernels, which are held to the FP64 oracle."""
+    _say("case gen3d fp32 vs cart3d fp32 on a uniform 2^-5 grid, rk", rk)
+    h = 2.0 ** -5
+    n = (70, 15, 10)
+    bc = ("reflective", "outflow", "noslip", "outflow", "reflective", "outflow")
+    kw = dict(ndim=3, ns=2, nci=n[0], ncj=n[1], nck=n[2], R=RU, gamma=(1.4, 1.67), M=(0.02896, 0.004),
+              mu=(0.0, 0.0), dt=2e-7, buoyancy=True, g=9.81, rhoRef=0.5, bc=bc, zper=True, rk=rk,
+              precision=FST_FP32_BULK)
+    cart = make_config(dx=h, dy=h, dz=h, **kw)
+    gen = make_config(grid=1, gen_ext=True, **kw)
+    var = W.random_state(cart, seed=23)
+    nodes = W.grid_nodes(gen, lambda i, j, k: (h * i, h * j, h * k))
+    out = []
+    for cfg, grid in ((cart, None), (gen, nodes)):
+        m = Module(cfg)
+        if grid is not None:
+            m.upload(FST_GRID, grid)
+        m.upload(FST_VAR, var)
+        m.pre_sim()
+        m.step(3)
+        out.append(m.download(FST_VAR))
+        m.close()
+    assert np.isfinite(out[0]).all()
+    assert_bitwise(out[1], out[0], "gen3d fp32 vs cart3d fp32 after 3 steps")
+    # and it is the FP32 arithmetic, not the FP64 one
+    gen.precision = 0
+    m = Module(gen)
+    m.upload(FST_GRID, nodes)
+    m.upload(FST_VAR, var)
+    m.pre_sim()
+    m.step(3)
+    assert not np.array_equal(m.download(FST_VAR), out[0])
+    m.close()

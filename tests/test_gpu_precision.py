@@ -38,7 +38,10 @@ def rel_errors(a, b):
 
 @pytest.mark.parametrize("name,kw", [("rt_like", dict()), ("walls_ns3", dict(ns=3, per=(False, False, False),
                                      bc=("outflow", "reflective", "hydrostatic", "noslip", "reflective", "outflow"))),
-                                     ("rt_like_ssp_rk3", dict(rk=3))])
+                                     ("rt_like_ssp_rk3", dict(rk=3)),
+                                     # no C-equation: var holds no C planes, which the ring must not read
+                                     ("rt_like_no_ceq", dict(ceq=False)),
+                                     ("rt_like_no_ceq_no_visc", dict(ceq=False, visc=False))])
 def test_fp32_form_error_against_fp64_oracle(name, kw):
     kw = dict(kw)
     rk = kw.pop("rk", 2)
