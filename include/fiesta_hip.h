@@ -49,7 +49,8 @@ enum fst_scheme { FST_WENO5 = 1, FST_CENTERED4 = 2, FST_QUICK = 3 };
 /* Arithmetic forms of the path. */
 enum fst_precision {
   FST_FP64 = 0,       /* reference arithmetic, bitwise parity target */
-  FST_FP32_BULK = 1,  /* FP32 stencils/fluxes, FP64 EOS and RK accumulation */
+  FST_FP32_BULK = 1,  /* FP32 stencils/fluxes, FP64 EOS and RK accumulation
+                         (fused 3D paths: cartesian and gen3d; fst_step only) */
   FST_FP64_FAST = 2   /* FP64 throughout with FMA contraction, reciprocal-based
                          division and one-division WENO5 weights: within
                          relative L2 1e-12 of the reference arithmetic, not
