@@ -55,12 +55,12 @@ def _static(cfg, var, nsteps):
     return out
 
 
-@pytest.mark.parametrize("rk,precision", [(2, 0), (3, 0), (2, 1)])
-def test_static_step_equals_context_step_on_two_calls(rk, precision):
+@pytest.mark.parametrize("rk,precision,ceq", [(2, 0, True), (3, 0, True), (2, 1, True), (2, 1, False)])
+def test_static_step_equals_context_step_on_two_calls(rk, precision, ceq):
     for n, (nci, ncj, nck) in enumerate([(24, 13, 10), (12, 9, 7)]):
-        print("case: fst_step_static rk%d precision %d call %d grid %dx%dx%d" % (rk, precision, n, nci, ncj, nck),
-              flush=True)
-        cfg = cfg3d(nci=nci, ncj=ncj, nck=nck)
+        print("case: fst_step_static rk%d precision %d ceq %d call %d grid %dx%dx%d"
+              % (rk, precision, ceq, n, nci, ncj, nck), flush=True)
+        cfg = cfg3d(nci=nci, ncj=ncj, nck=nck, ceq=ceq)
         cfg.rk_order = rk
         cfg.precision = precision
         var = W.random_state(cfg, 11 + n)
