@@ -207,14 +207,18 @@ def test_terrain_expansion_against_oracle():
     m.close()
 
 
-def test_terrain_expansion_keeps_species_mass():
+@pytest.mark.parametrize("precision", [0, 1])
+def test_terrain_expansion_keeps_species_mass(precision):
     """The same workload at 128 x 64 x 128 cells, 60 steps on the GPU: every
-    species' interior total stays what it was to 1e-13 relative."""
-    _say("case terrain expansion 128x64x128 mass, 60 steps")
+    species' interior total stays what it was to 1e-13 relative in FP64, and
+    to 1e-11 in the FP32 form (its flux differences are rounded to binary32,
+    so they telescope only to that rounding)."""
+    _say("case terrain expansion 128x64x128 mass, 60 steps, precision", precision)
     cfg = W.terrain_expansion_config((128, 64, 128))
     nodes = W.terrain_grid(cfg, vertical=1)
     var, _ = W.terrain_expansion_state(cfg, nodes)
     cfg.dt = W.gen_cfl_dt(cfg, var, nodes, 0.4)
+    cfg.precision = precision
     d = W.Dims(cfg)
     sl = d.interior()
     m = Module(cfg)
@@ -226,7 +230,7 @@ def test_terrain_expansion_keeps_species_mass():
     v = m.download(FST_VAR)
     m.close()
     assert np.isfinite(v).all()
-    np.testing.assert_allclose(v[sl][4:].sum(axis=(1, 2, 3)), m0, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(v[sl][4:].sum(axis=(1, 2, 3)), m0, rtol=1e-13 if precision == 0 else 1e-11, atol=0)
 
 
 FUSED_GEN = {
