@@ -7,7 +7,8 @@ FP64 bit for bit, the fast FP64 form within 1e-12 relative L2 on every
 conserved variable, the FP32 form within 1e-4 relative L2 on density,
 pressure and mass fractions (the requirement).  gen3d on a terrain grid:
 species 1-3, plain / gen_ext / gen_ext with viscous terms, FP64 bit for bit
-and FP32 within 1e-4.  Combinations that no other test runs are where a
+and FP32 within 1e-4.  The fused cart2d stage (FP64, bit for bit) on the
+cartesian combinations.  Combinations that no other test runs are where a
 kernel can read a plane the configuration does not have (the FP32 form
 without the C-equation once did).
 """
@@ -122,3 +123,26 @@ def test_gen3d_forms(ns, mode):
     assert np.isfinite(got).all()
     for a, b, label in zip(_fields(cfg, got), _fields(cfg, want), ("rho", "p", "Y")):
         assert _rel_l2(a, b) < 1e-4, (label, _rel_l2(a, b))
+
+
+@pytest.mark.parametrize("ns,visc,ceq,buoy", CART)
+def test_cart2d_fused(ns, visc, ceq, buoy):
+    """The fused cart2d stage (FP64) on the same combinations, 37 x 13 cells,
+    x periodic and walls in y, 2 RK2 steps, bit for bit against the oracle."""
+    from helpers import cfg2d
+    cfg = cfg2d(nci=37, ncj=13, ns=ns, visc=visc, ceq=ceq, buoyancy=buoy, per=(True, False),
+                bc=("outflow", "outflow", "reflective", "noslip", "outflow", "outflow"))
+    _say("case cart2d ns %d visc %d ceq %d buoyancy %d" % (ns, visc, ceq, buoy))
+    var = W.random_state(cfg, 7 + ns)
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    o.step(2)
+    want = o.get(FST_VAR)
+    m = Module(cfg)
+    m.upload(FST_VAR, var)
+    m.pre_sim()
+    m.step(2)
+    got = m.download(FST_VAR)
+    m.close()
+    assert_bitwise(got, want, "cart2d fused after 2 steps")
