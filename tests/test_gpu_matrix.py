@@ -8,7 +8,10 @@ conserved variable, the FP32 form within 1e-4 relative L2 on density,
 pressure and mass fractions (the requirement).  gen3d on a terrain grid:
 species 1-3, plain / gen_ext / gen_ext with viscous terms, FP64 bit for bit
 and FP32 within 1e-4.  The fused cart2d stage (FP64, bit for bit) on the
-cartesian combinations.  Combinations that no other test runs are where a
+cartesian combinations.  The learned C-indicator form on species 1-3 with
+and without viscous terms and gravity (FP64 bit for bit against the
+oracle's learned steps, FP32 close to FP64).  Combinations that no other
+test runs are where a
 kernel can read a plane the configuration does not have (the FP32 form
 without the C-equation once did).
 """
@@ -146,3 +149,52 @@ def test_cart2d_fused(ns, visc, ceq, buoy):
     got = m.download(FST_VAR)
     m.close()
     assert_bitwise(got, want, "cart2d fused after 2 steps")
+
+
+LEARNED = list(itertools.product([1, 2, 3], [False, True], [False, True]))
+
+
+@pytest.mark.parametrize("ns,visc,buoy", LEARNED)
+def test_learned_forms(ns, visc, buoy):
+    """The learned C-indicator form (untrained network) on species 1-3, with
+    and without viscous terms and gravity: FP64 learned steps equal the
+    oracle's learned steps given the GPU's C_hat and direction fields (2 RK2
+    steps, interior), and the FP32 learned form stays within 1e-5 relative L2
+    of the FP64 learned form in density."""
+    from fiesta_amd.runtime import Trainer
+    cfg = cfg3d(nci=21, ncj=11, nck=9, ns=ns, visc=visc, ceq=True, buoyancy=buoy)
+    cfg.learned_c = 1
+    _say("case learned ns %d visc %d buoyancy %d" % (ns, visc, buoy))
+    var = W.random_state(cfg, 13 + ns)
+    d = Dims(cfg)
+    sl = d.interior()
+    tr = Trainer(seed=11)
+    w = tr.weights()
+    tr.close()
+    m = Module(cfg)
+    m.upload(FST_VAR, var)
+    m.set_mlp_weights(w)
+    m.pre_sim()
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    for s in range(2):
+        m.step(1)
+        chat, dr, _, _ = m.learned_fields()
+        _, _, mch = m.learned_stats()
+        o.set_learned(chat, dr, mch)
+        o.step(1)
+        got, want = m.download(FST_VAR), o.get(FST_VAR)
+        assert np.array_equal(got[sl], want[sl]), ("step", s, int((got[sl] != want[sl]).sum()))
+    m.close()
+    cfg.precision = FST_FP32_BULK
+    m = Module(cfg)
+    m.upload(FST_VAR, var)
+    m.set_mlp_weights(w)
+    m.pre_sim()
+    m.step(2)
+    g32 = m.download(FST_VAR)
+    m.close()
+    assert np.isfinite(g32).all()
+    r32, r64 = g32[sl][4:4 + ns].sum(axis=0), got[sl][4:4 + ns].sum(axis=0)
+    assert _rel_l2(r32, r64) < 1e-5, _rel_l2(r32, r64)
