@@ -11,9 +11,8 @@ and FP32 within 1e-4.  The fused cart2d stage (FP64, bit for bit) on the
 cartesian combinations.  The learned C-indicator form on species 1-3 with
 and without viscous terms and gravity (FP64 bit for bit against the
 oracle's learned steps, FP32 close to FP64).  Combinations that no other
-test runs are where a
-kernel can read a plane the configuration does not have (the FP32 form
-without the C-equation once did).
+test runs are where a kernel can read a plane the configuration does not
+have (the FP32 form without the C-equation once did).
 """
 import itertools
 import sys
