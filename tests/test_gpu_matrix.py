@@ -10,7 +10,8 @@ species 1-3, plain / gen_ext / gen_ext with viscous terms, FP64 bit for bit
 and FP32 within 1e-4.  The fused cart2d stage (FP64, bit for bit) on the
 cartesian combinations.  The learned C-indicator form on species 1-3 with
 and without viscous terms and gravity (FP64 bit for bit against the
-oracle's learned steps, FP32 close to FP64).  Combinations that no other
+oracle's learned steps, FP32 close to FP64), in 3D and on the fused cart2d
+stage.  Combinations that no other
 test runs are where a kernel can read a plane the configuration does not
 have (the FP32 form without the C-equation once did).
 """
@@ -197,3 +198,38 @@ def test_learned_forms(ns, visc, buoy):
     assert np.isfinite(g32).all()
     r32, r64 = g32[sl][4:4 + ns].sum(axis=0), got[sl][4:4 + ns].sum(axis=0)
     assert _rel_l2(r32, r64) < 1e-5, _rel_l2(r32, r64)
+
+
+@pytest.mark.parametrize("ns,visc,buoy", LEARNED)
+def test_learned2d_forms(ns, visc, buoy):
+    """The 2D learned form (fused cart2d stage, untrained network) on species
+    1-3 with and without viscous terms and gravity: 2 RK2 steps equal the
+    oracle's learned steps given the GPU's fields (interior)."""
+    from fiesta_amd.runtime import Trainer
+    from helpers import cfg2d
+    cfg = cfg2d(nci=37, ncj=13, ns=ns, visc=visc, ceq=True, buoyancy=buoy, per=(True, False),
+                bc=("outflow", "outflow", "reflective", "noslip", "outflow", "outflow"))
+    cfg.learned_c = 1
+    _say("case learned 2D ns %d visc %d buoyancy %d" % (ns, visc, buoy))
+    var = W.random_state(cfg, 17 + ns)
+    d = Dims(cfg)
+    sl = d.interior()
+    tr = Trainer(seed=11, ndim=2)
+    w = tr.weights()
+    tr.close()
+    m = Module(cfg)
+    m.upload(FST_VAR, var)
+    m.set_mlp_weights(w)
+    m.pre_sim()
+    o = oracle.Oracle(cfg)
+    o.set(FST_VAR, var)
+    o.pre_sim()
+    for s in range(2):
+        m.step(1)
+        chat, dr, _, _ = m.learned_fields()
+        _, _, mch = m.learned_stats()
+        o.set_learned(chat, dr, mch)
+        o.step(1)
+        got, want = m.download(FST_VAR), o.get(FST_VAR)
+        assert np.array_equal(got[sl], want[sl]), ("step", s, int((got[sl] != want[sl]).sum()))
+    m.close()
