@@ -1277,8 +1277,11 @@ __device__ __forceinline__ typename Ar<F32>::R field_at(uint32_t N, int nv, cons
 
 // Loads of one plane p of the ring, field-major: thread t takes the ring
 // cells t, t + NTHREADS, ... of every field (compile-time field choice).
-// Without the C-equation (ceq = 0) var has no C planes: the ring's C fields
-// are zeros then (the faces read them only with ceq set).
+// Without the C-equation (ceq = 0) var has no C planes: the ring's C slots
+// then take var's first planes instead, which exist (nv >= 5) and are never
+// read (the faces read the C fields only with ceq set).  A select of the
+// plane base keeps the loads unconditional: skipping them under a runtime
+// test made the FP32 kernel 35 % slower.
 template <int NS, bool F32, bool LC>
 __device__ __forceinline__ void ring_fetch(const Grid32 &g, const double *__restrict__ U,
                                            const double *__restrict__ P, const float *__restrict__ CH,
@@ -1286,6 +1289,7 @@ __device__ __forceinline__ void ring_fetch(const Grid32 &g, const double *__rest
                                            typename Ar<F32>::R *pre) {
   using RF = RingF<NS, F32, LC>;
   using R = typename RF::R;
+  const int nvc = ceq ? g.nv : 0;  // first C plane
 #pragma unroll
   for (int m = 0; m < RF::NPC; ++m) {
     const int e = threadIdx.x + m * NTHREADS;
@@ -1295,8 +1299,7 @@ __device__ __forceinline__ void ring_fetch(const Grid32 &g, const double *__rest
     const uint32_t c = ok ? g.id(i, j, p) : 0u;
 #pragma unroll
     for (int f = 0; f < RF::NF; ++f)
-      pre[f * RF::NPC + m] =
-          ok && (f < RF::FC || LC || ceq) ? field_at<NS, F32, LC>(g.N, g.nv, U, P, CH, DR, f, c) : (R)0;
+      pre[f * RF::NPC + m] = ok ? field_at<NS, F32, LC>(g.N, nvc, U, P, CH, DR, f, c) : (R)0;
   }
 }
 
