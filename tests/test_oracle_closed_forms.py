@@ -560,7 +560,7 @@ def test_ceq_artificial_viscosity_is_an_anisotropic_laplacian(ndim):
         assert abs(want) > 1.0
 
 
-# ------------------------------------------------------------------ 2D quirks (Q6, Q7)
+# ------------------------------------------------------------------ 2D quirks
 
 def _visc2d_delta(fields, mu_species=1.0e-3):
     """RHS of one 2D compute() with the viscous terms minus without them.
@@ -590,7 +590,7 @@ def _visc2d_delta(fields, mu_species=1.0e-3):
 def test_2d_viscous_uses_the_fixed_viscosity_and_conductivity():
     """calculateStressTensor2dv / calculateHeatFlux2dv / applyViscousTerm2dv
     (src/viscosity.hpp:25-199): mu = 2.928e-5 and k = 0.026 whatever the
-    species' mu (Q7).  u = a y^2: the x-momentum gains mu * 2a (the second
+    species' mu.  u = a y^2: the x-momentum gains mu * 2a (the second
     difference is exact); T = T0 + tau x^2 at rest: the energy gains k * 2 tau."""
     a, mu = 1.0e3, 2.928e-5
     rho0, p0 = 1.2, 1.0e5
@@ -606,7 +606,7 @@ def test_2d_viscous_uses_the_fixed_viscosity_and_conductivity():
 
 def test_2d_viscous_dudy_index_quirk():
     """The x-face dudy of calculateStressTensor2dv reads vel(i, j-1, 1) (the v
-    component) where u belongs (Q7, src/viscosity.hpp:68).  With u = 0 and
+    component) where u belongs (src/viscosity.hpp:68).  With u = 0 and
     v = V0 + s x, the correct stress has zero divergence; the reference's
     gives the y-momentum -mu s / (4 dy) (a uniform-in-j shift of the x-face
     shear stress by -v/(4 dy))."""
@@ -617,7 +617,7 @@ def test_2d_viscous_dudy_index_quirk():
 
 
 def test_2d_wave_speed_quirk():
-    """maxWaveSpeed2D (src/ceq.hpp:43-78, Q6): species s reads gamma and R/M
+    """maxWaveSpeed2D (src/ceq.hpp:43-78): species s reads gamma and R/M
     from cd(6+2s), cd(7+2s) of the (gamma, R/M, mu) x ns table, and adds
     |m| (momentum, not velocity).  On a uniform state the C relaxation
     (updateCeq2D, src/ceq.hpp:153-195) gives dvar(C_n) = -maxS C0 / (eps |dx|)

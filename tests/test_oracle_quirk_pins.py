@@ -27,7 +27,7 @@ more than its tolerance:
       dvar = -u . grad(rho Y) exactly (WENO5 and the face velocity are exact
       for linear data), so a flux divided by another direction's spacing
       fails.
-  C-equation contact switch (src/ceq3d.hpp:138-141, Q9): the switch reads
+  C-equation contact switch (src/ceq3d.hpp:138-141): the switch reads
       nrg = E/rho - 0.5 rho |u|^2 (rho times the kinetic energy per mass),
       not E/rho - 0.5 |u|^2.  A density ramp under a fast uniform stream with
       p = p0 rho^1.2 has E/rho - |u|^2/2 rising with rho but nrg falling, so
