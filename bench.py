@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs (baseline and configuration 1)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs (baseline and the 1000 x 4 Sod tube)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--path", choices=["fused", "reference"], default="fused")
     ap.add_argument("--copy-probe", action="store_true", help="also time a plain copy kernel")
@@ -208,7 +208,7 @@ def _blast_rays(cfg, var, blobs, rmax=0.3):
 
 
 def multiblast_leg(args, weights, train_info):
-    """Configuration 4: the 2D multi-blast (n^2 cells, 8 blobs, 3 species,
+    """The 2D multi-blast (n^2 cells, 8 blobs, 3 species,
     viscous, SSP-RK3) with the plain 2D C-equation, the learned C-indicator
     (FP64 cart2d sequence) and no C-equation; the learned and C-equation-free
     runs judged by the plain run on density (field and ray profiles) and
@@ -579,7 +579,7 @@ def cpu_leg(args, cfg_size):
 
 
 def sod_leg():
-    """Configuration 1 in full: Sod shock tube, 1000 x 4 cells, WENO5 +
+    """The Sod shock tube in full, 1000 x 4 cells, WENO5 +
     C-equation, RK2, dt 2e-4, 1000 steps, on the GPU (the cart2d module's
     kernels) and on the CPU with the oracle; the final states compared bit
     for bit and against the exact Riemann solution at t = 0.2."""
@@ -859,7 +859,7 @@ def main():
             if f in fm:
                 fm[f]["error_vs_fp64_%d_steps" % args.accuracy_steps] = e
     if not args.no_cpu:
-        log("configuration 1 (Sod, CPU and GPU)")
+        log("Sod shock tube 1000 x 4 (CPU and GPU)")
         out["sod_c1"] = sod_leg()
     if args.extras:
         log("extras: noise filter and output pack")
@@ -874,7 +874,7 @@ def main():
         log("2D module (expansion)")
         out.setdefault("extras", {})["cart2d_expansion"] = expansion2d_leg(args)
     if args.multiblast or args.extras:
-        log("configuration 4: 2D learned C-indicator on the multi-blast")
+        log("2D multi-blast 2048^2: learned C-indicator against the plain C-equation")
         w2, info2 = train_learned2d()
         log("trained 2D: %s" % info2)
         out.setdefault("extras", {})["multiblast_c4"] = multiblast_leg(args, w2, info2)

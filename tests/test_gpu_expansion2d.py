@@ -1,4 +1,4 @@
-"""Configuration 2: the 2D ideal expansion (two species, Gaussian 50 atm blob
+"""The 2D ideal expansion (two species, Gaussian 50 atm blob
 in 1 atm air, viscous, C-equation, outflow walls; the cart2d module,
 src/cart2d.cpp:136-237).
 

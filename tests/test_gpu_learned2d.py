@@ -245,7 +245,7 @@ def _train_small():
 
 
 def test_learned2d_gain_on_multiblast():
-    """Configuration 4 in small: trained on 2D Sod and expansion runs, the
+    """The 2D multi-blast in small: trained on 2D Sod and expansion runs, the
     learned run on an unseen multi-blast (4 blobs, 3 species, 256^2) is closer
     to the plain C-equation run than a run without the C-equation terms, and
     keeps the blast fronts where the plain run has them.  CFL 0.25: the plain
